@@ -1,0 +1,194 @@
+"""Benchmark: W-projection grid + degrid throughput on one MI355X.
+
+A step is one grid pass and one degrid pass (IVisGridder::grid and ::degrid, each a full
+TableVisGridder::generic: uvw prep, tile binning, kernel) over the whole W-projection array
+workload (36 antennas in a 6 km disc, 300 integrations over 8 h at dec -45 deg, 64 channels
+700-1000 MHz, 4 polarisations = 12.1M (row, chan) samples x 4 pols; 4096^2 grid, 129 w planes,
+oversample 4, support 64), with every input already resident in HBM.
+
+Metric: CF-tap updates/s (one complex multiply-accumulate of a (visibility, pol) with a CF tap =
+8 FP32 FLOP, TableVisGridder.cc:264-267,804), whole job.  One JSON line is printed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, g_grid, target_s: float = 12.0):
+    """Oracle's single-thread FP32 restatement of TableVisGridder::generic on a bounded sample
+    of the same workload (same CF table, downloaded from the device)."""
+    from oracle import oracle as O
+    os_ = wl.params["oversample"]
+    nz = wl.params["nwplanes"]
+    planes = [g_grid.cf_plane(k) for k in range(nz * os_ * os_)]
+    prm = O.WProjectParams(wmax=wl.params["wmax"], nwplanes=nz, oversample=os_,
+                           maxsupport=wl.params["maxsupport"], limitsupport=wl.params["limitsupport"])
+    ws = prm.sampling()
+
+    def run(nrow_s):
+        rows = np.linspace(0, wl.nrow - 1, nrow_s).astype(int)
+        uvw = wl.uvw[rows]
+        vis = np.transpose(wl.vis[:, :, rows], (2, 1, 0))
+        noise = np.transpose(wl.noise[:, :, rows], (2, 1, 0))
+        cm = np.zeros((wl.nchan, wl.npol, nrow_s), dtype=np.int32)
+        for c, f in enumerate(wl.freq):
+            cm[c, :, :] = np.array([ws.plane(w / O.C_LIGHT * f) for w in uvw[:, 2]], dtype=np.int32)[None, :]
+        ch = O.Chunk(uvw, wl.freq, vis, noise=noise)
+        og = O.OracleGridder("WProject", "f32", wproject=prm)
+        og.cf = O.CFCache(planes, os_)
+        og.cmap_override = cm
+        og.initialise_grid((wl.nx, wl.ny, wl.npol, 1), (wl.cell, wl.cell))
+        t0 = time.perf_counter()
+        og.grid(ch)
+        t1 = time.perf_counter()
+        od = O.OracleGridder("WProject", "f32", wproject=prm)
+        od.cf = og.cf
+        od.cmap_override = cm
+        od._setup((wl.nx, wl.ny, wl.npol, 1), (wl.cell, wl.cell))
+        od.mode = 3
+        od.freq_axis = None
+        od.model_empty = False
+        od.grid_ = og.grid_
+        t2 = time.perf_counter()
+        od.degrid(ch)
+        t3 = time.perf_counter()
+        taps = og.counters.taps_gridded + od.counters.taps_degridded
+        return taps, (t1 - t0) + (t3 - t2)
+
+    taps, dt = run(4)
+    n = max(4, int(4 * target_s / max(dt, 1e-3)))
+    n = min(n, wl.nrow)
+    taps, dt = run(n)
+    return {"value": taps / dt, "unit": "CF-tap updates/s", "cores": 1, "kind": "port",
+            "sample": f"{n} rows x {wl.nchan} chans x {wl.npol} pols of the same workload, grid + degrid, "
+                      f"oracle FP32 restatement (gcc -O2, 1 thread), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nint", type=int, default=300, help="integrations (workload rows / 630)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md"}))
+        sys.exit(2)
+
+    import torch
+    import yandasoft_amd as ya
+    from yandasoft_amd import workloads as W
+
+    wl = W.wproject_array(nint=args.nint)
+    log(f"workload rows={wl.nrow} chans={wl.nchan} pols={wl.npol} grid={wl.nx} wmax={wl.params['wmax']:.1f}")
+    dev = torch.device("cuda:0")
+    t_uvw = torch.from_numpy(np.ascontiguousarray(wl.uvw)).to(dev)
+    t_freq = torch.from_numpy(np.ascontiguousarray(wl.freq)).to(dev)
+    t_vis = torch.from_numpy(np.ascontiguousarray(wl.vis).view(np.float32)).to(dev)
+    t_noise = torch.from_numpy(np.ascontiguousarray(wl.noise).view(np.float32)).to(dev)
+    t_out = torch.zeros_like(t_vis)
+    torch.cuda.synchronize()
+    chunk = ya.Chunk(uvw=t_uvw, freq=t_freq, vis=t_vis, vis_out=t_out, noise=t_noise, device=True)
+    chunk.npol = wl.npol
+
+    ctx = ya.Context(0)
+    img = ya.ImageDesc(wl.nx, wl.ny, wl.npol, 1, wl.cell, wl.cell)
+    gg = ya.Gridder(ctx, "GPUWProject", wl.params)
+    dg = ya.Gridder(ctx, "GPUWProject", wl.params)
+    rng = np.random.default_rng(3)
+    model = np.zeros((1, wl.npol, wl.ny, wl.nx), np.float32)
+    for _ in range(50):
+        model[0, :, rng.integers(500, 3500), rng.integers(500, 3500)] = rng.normal(size=wl.npol)
+    t_model = torch.from_numpy(model).to(dev)
+    torch.cuda.synchronize()
+    dg.initialise_degrid(img, t_model)
+
+    def step():
+        gg.initialise_grid(img)
+        gg.grid(chunk)
+        dg.degrid(chunk)
+
+    t0 = time.perf_counter()
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i} done ({time.perf_counter() - t0:.1f} s)")
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    s0g, s0d = gg.stats(), dg.stats()
+    gg.set_profiling(True)
+    dg.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"step {i} done")
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s1g, s1d = gg.stats(), dg.stats()
+    taps_g = s1g.taps_gridded - s0g.taps_gridded
+    taps_d = s1d.taps_degridded - s0d.taps_degridded
+    taps = taps_g + taps_d
+    value = taps / elapsed
+    ms_step = 1e3 * elapsed / args.steps
+    # dominant kernel: average launch duration from HIP events on the gridder's stream
+    gk_ms = s1g.ms_grid_kernel / max(1.0, s1g.n_grid_launches)
+    dk_ms = s1d.ms_degrid_kernel / max(1.0, s1d.n_degrid_launches)
+    per_launch_g = taps_g / args.steps
+    per_launch_d = taps_d / args.steps
+    if gk_ms >= dk_ms:
+        kname, kms, kt = "k_grid<4>", gk_ms, per_launch_g
+    else:
+        kname, kms, kt = "k_degrid<4>", dk_ms, per_launch_d
+    achieved = 8.0 * kt / (kms * 1e-3) / 1e12
+    result = {
+        "metric": "CF-tap updates/s (W-proj grid+degrid)",
+        "value": value,
+        "unit": "CF-tap updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "wproject_array: 36 ant 6 km disc, 300 x 630 rows, 64 ch 700-1000 MHz, 4 pol, "
+                               "4096^2, 129 w-planes, os 4, support 64",
+                   "rows": wl.nrow, "chans": wl.nchan, "pols": wl.npol, "grid": wl.nx,
+                   "wplanes": wl.params["nwplanes"], "oversample": wl.params["oversample"],
+                   "support": s1g.support, "parallelism": "1 GPU"},
+        "fp32_pct_of_peak": 100.0 * 8.0 * value / (FP32_PEAK_TFLOPS * 1e12),
+        "phases_ms_per_step": {"grid_kernel": gk_ms, "degrid_kernel": dk_ms,
+                               "prep": (s1g.ms_prep + s1d.ms_prep) / args.steps,
+                               "binning": (s1g.ms_bin + s1d.ms_bin) / args.steps},
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(wl, gg)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,143 @@
+/*
+ * ygrid.h -- C ABI of the MI355X-native convolutional gridder (yandasoft_amd/libygrid.so).
+ *
+ * This boundary replaces yandasoft's gridding path behind askap::synthesis::IVisGridder
+ * (askap/gridding/IVisGridder.h:64-140) as implemented by TableVisGridder and its
+ * Box/SphFunc/WProject/AWProject subclasses.  A plugin adapter deriving from
+ * TableVisGridder forwards each virtual call to the function cited beside it.
+ *
+ * Conventions
+ *  - every function returns YGRID_OK (0) or a non-zero status; the message of the last
+ *    failure on the calling thread is returned by ygrid_last_error().  No C++ exception
+ *    crosses this boundary (the adapter turns a non-zero status into ASKAPTHROW).
+ *  - arrays follow casacore's column-major layouts so that an adapter can pass the
+ *    accessor's storage without reordering:
+ *      visibility / noise / flag cubes (nRow, nChan, nPol): row + nRow*(chan + nChan*pol)
+ *      images (nx, ny, nPol, nChan): x + nx*(y + ny*(pol + nPol*chan))
+ *    complex values are interleaved float pairs (casacore::Complex).
+ *  - chunk arrays may live in host memory or in device (HBM) memory (ygrid_chunk.mem).
+ *  - gridders are not thread-safe; clones are independent (IVisGridder.h:75).
+ */
+#ifndef YGRID_H
+#define YGRID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YGRID_VERSION 1
+
+enum ygrid_status {
+  YGRID_OK = 0,
+  YGRID_ERR_ARG = 1,              /* invalid argument */
+  YGRID_ERR_STATE = 2,            /* call out of order (e.g. grid before initialise) */
+  YGRID_ERR_DEVICE = 3,           /* HIP runtime / hipFFT failure */
+  YGRID_ERR_DATA = 4,             /* data check failed (zero noise weight, w out of range, ...) */
+  YGRID_ERR_UNKNOWN_GRIDDER = 5   /* factory name not registered */
+};
+
+enum ygrid_mem { YGRID_MEM_HOST = 0, YGRID_MEM_DEVICE = 1 };
+
+typedef struct ygrid_ctx ygrid_ctx;
+typedef struct ygrid_gridder ygrid_gridder;
+
+/* Image geometry, the part of scimath::Axes + shape the gridder uses
+ * (TableVisGridder::initialiseCellSize, TableVisGridder.cc:1038-1049;
+ *  FrequencyMapper::setupImage, FrequencyMapper.cc:46-53). */
+typedef struct {
+  int nx, ny;           /* unpadded image size */
+  int npol, nchan;      /* image polarisations and channels */
+  double cellx, celly;  /* direction-axis increments (radians; sign is ignored) */
+  int has_freq_axis;    /* FREQUENCY axis present: map accessor channels to image channels */
+  double freq_start, freq_end;
+} ygrid_image_desc;
+
+/* One accessor chunk (IConstDataAccessor / IDataAccessor getters used by
+ * TableVisGridder::generic, TableVisGridder.cc:487-567). */
+typedef struct {
+  int nrow, nchan, npol;
+  int mem;                   /* enum ygrid_mem, applies to every pointer below */
+  const double *uvw;         /* rotatedUVW: nrow x 3 (metres), row-major (u,v,w) */
+  const double *delay;       /* uvwRotationDelay: nrow (metres) or NULL (= 0) */
+  const double *freq;        /* frequency: nchan (Hz) */
+  const float *vis;          /* visibility cube (complex) */
+  float *vis_out;            /* degrid: rwVisibility cube, model is ADDED in place */
+  const float *noise;        /* noise cube (complex) or NULL (= 1) */
+  const uint8_t *flag;       /* flag cube or NULL */
+  const int32_t *feed1;      /* feed1: nrow or NULL (= 0) */
+  const double *pointing1;   /* pointingDir1 offsets from the image centre: nrow x 2 (rad) or NULL */
+} ygrid_chunk;
+
+/* Counters reported by TableVisGridder's destructor (TableVisGridder.cc:182-317). */
+typedef struct {
+  double samples_gridded, taps_gridded;
+  double samples_degridded, taps_degridded;
+  double vectors_flagged, vectors_wflagged;
+  double time_grid_s, time_degrid_s, time_cf_s;
+  int support;        /* itsSupport: largest CF support */
+  int oversample;     /* itsOverSample */
+  int n_cf;           /* number of CF planes (incl. oversampling) */
+  /* device time (HIP events on the gridder's stream) when profiling is enabled */
+  double ms_prep, ms_bin, ms_grid_kernel, ms_degrid_kernel;
+  double n_grid_launches, n_degrid_launches;
+} ygrid_stats;
+
+/* ---- library / context ---- */
+const char *ygrid_last_error(void);
+int ygrid_version(void);
+/* Per-process device context; gridders created from it share its stream. */
+int ygrid_ctx_create(int device, ygrid_ctx **out);
+int ygrid_ctx_destroy(ygrid_ctx *ctx);
+/* Use an existing hipStream_t (may be NULL for the default stream). */
+int ygrid_ctx_set_stream(ygrid_ctx *ctx, void *hip_stream);
+int ygrid_ctx_synchronize(ygrid_ctx *ctx);
+
+/* ---- factory: VisGridderFactory::make / createGridder (VisGridderFactory.cc:112-286) ----
+ * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUAWProject".
+ * parset: the gridder.<name>. subset as "key=value" lines (';' also separates), plus the
+ * global keys padding, alldatapsf, oversampleweight, cleargrids. */
+int ygrid_gridder_create(ygrid_ctx *ctx, const char *name, const char *parset, ygrid_gridder **out);
+/* IVisGridder::clone (IVisGridder.h:75; TableVisGridder.cc:126-168) */
+int ygrid_gridder_clone(const ygrid_gridder *g, ygrid_gridder **out);
+int ygrid_gridder_destroy(ygrid_gridder *g);
+
+/* IVisGridder::initialiseGrid (IVisGridder.h:81-84; TableVisGridder.cc:997-1030) */
+int ygrid_initialise_grid(ygrid_gridder *g, const ygrid_image_desc *img, int dopsf, int dopcf);
+/* IVisGridder::grid (IVisGridder.h:88; TableVisGridder.cc:908-913 -> generic(acc,false)) */
+int ygrid_grid(ygrid_gridder *g, const ygrid_chunk *acc);
+/* IVisGridder::finaliseGrid (IVisGridder.h:92; TableVisGridder.cc:1109-1236).
+ * out: nx*ny*npol*nchan floats (unpadded image), host or device per mem. */
+int ygrid_finalise_grid(ygrid_gridder *g, float *out, int mem);
+/* IVisGridder::finaliseWeights (IVisGridder.h:97; TableVisGridder.cc:1255-1284) */
+int ygrid_finalise_weights(ygrid_gridder *g, float *out, int mem);
+/* IVisGridder::initialiseDegrid (IVisGridder.h:102-103; TableVisGridder.cc:1286-1319).
+ * model: nx*ny*npol*nchan floats (unpadded image), host or device per mem. */
+int ygrid_initialise_degrid(ygrid_gridder *g, const ygrid_image_desc *img, const float *model, int mem);
+/* IVisGridder::degrid (IVisGridder.h:115; TableVisGridder.cc:898-902 -> generic(acc,true)) */
+int ygrid_degrid(ygrid_gridder *g, const ygrid_chunk *acc);
+/* IVisGridder::finaliseDegrid (IVisGridder.h:118; TableVisGridder.cc:1364-1372) */
+int ygrid_finalise_degrid(ygrid_gridder *g);
+/* IVisGridder::isModelEmpty (IVisGridder.h:139; TableVisGridder.cc:1467-1470) */
+int ygrid_is_model_empty(const ygrid_gridder *g, int *out);
+/* VisWeightsMultiFrequency via initVisWeights/customiseForContext (IVisGridder.h:107-111;
+ * VisWeightsMultiFrequency.cc:59-75): order < 0 disables visibility weighting. */
+int ygrid_set_mfs_weight(ygrid_gridder *g, double ref_freq, int order);
+
+/* Gridder statistics (TableVisGridder.cc:182-317). */
+int ygrid_get_stats(const ygrid_gridder *g, ygrid_stats *out);
+/* Record HIP events around the prep, binning and grid/degrid kernels of every pass
+ * (the reference's own timers, TableVisGridder.cc:465-500,879-883). */
+int ygrid_set_profiling(ygrid_gridder *g, int enable);
+/* Copy CF plane k (k = fracu + os*(fracv + os*plane), TableVisGridder.cc:720) to host:
+ * (2S+1)^2 complex values, ix fastest.  *support receives S; out may be NULL to query S. */
+int ygrid_get_cf(const ygrid_gridder *g, int k, float *out, int *support);
+/* Copy the padded grid (complex, casacore layout) to host; *n receives the float count. */
+int ygrid_get_grid(const ygrid_gridder *g, float *out, size_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YGRID_H */

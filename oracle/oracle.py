@@ -1,0 +1,695 @@
+"""CPU oracle of yandasoft's convolutional gridding path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+may import this module, and only as the checker (or as the timed CPU baseline).  The
+product (``yandasoft_amd``) never imports it.
+
+The hot loops live in ``ygrid_oracle.c`` (built into ``liboracle.so`` by ``oracle/Makefile``);
+this module restates the FFT-based parts with numpy (pocketfft, FP64) and wraps the C code.
+
+Restated reference functions (file:line in alxndrwllmsn/yandasoft):
+
+* ``fft_centred``            -- scimath::fft / fft2d as used with centred arrays
+                                 (TableVisGridder.cc:1215,1312; SphFuncVisGridder.cc:219-234).
+                                 base-scimath's FFTWrapper is an absent dependency: the
+                                 normalisation (forward unnormalised, inverse 1/N) is pinned by
+                                 the point-source closed form of
+                                 tests/measurementequation/ImageFFTEquationTest.h:124-175; the
+                                 exponent sign (forward = exp(-i)) is parity unpinned.
+* ``Spheroidal``             -- scimath::SpheroidalFunction(c, alpha) (SphFuncVisGridder.h:101-106).
+* ``box_cf``                 -- BoxVisGridder::initConvolutionFunction (BoxVisGridder.cc:62-72).
+* ``sphfunc_cf``             -- SphFuncVisGridder::initConvolutionFunction (SphFuncVisGridder.cc:90-169).
+* ``sph_grid_correction``    -- SphFuncVisGridder::correctConvolution (SphFuncVisGridder.cc:171-266)
+                                 with interpolateEdgeValues (SphFuncVisGridder.tcc:37-51).
+* ``LinearWSampling`` & co.  -- WDependentGridderBase::getWPlane/getWTerm
+                                 (WDependentGridderBase.cc:53-67,118-167), PowerWSampling.cc:60-84,
+                                 GaussianWSampling.cc:70-167 + GaussianWSampling.h:103-104.
+* ``wproject_cf``            -- WProjectVisGridder::generate/setup/populateThisPlane/calcSupport/
+                                 extractSupport/populateItsConvFunc/normalise
+                                 (WProjectVisGridder.cc:234-284,435-522,562-703).
+* ``disk_pattern`` / ``awproject_cf`` -- DiskIllumination::getPattern (DiskIllumination.cc:85-177)
+                                 and AWProjectVisGridder::initConvolutionFunction (AWProjectVisGridder.cc:259-611).
+* ``frequency_map``          -- FrequencyMapper::setupMapping (FrequencyMapper.cc:63-97).
+* ``OracleGridder``          -- TableVisGridder::initialiseGrid/grid/finaliseGrid/finaliseWeights/
+                                 initialiseDegrid/degrid (TableVisGridder.cc:451-884,997-1319).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+C_LIGHT = 299792458.0
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    """Load (building if needed) the oracle's C library."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            import subprocess
+            subprocess.check_call(["make", "-C", _HERE, "-s"])
+        L = ctypes.CDLL(path)
+        L.orc_nint.restype = ctypes.c_int
+        L.orc_nint.argtypes = [ctypes.c_double]
+        L.orc_sph_value.restype = ctypes.c_double
+        L.orc_schwab_grdsf.restype = ctypes.c_double
+        L.orc_schwab_grdsf.argtypes = [ctypes.c_double]
+        L.orc_support_search.restype = ctypes.c_int
+        L.orc_symmetrical_support.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def nint(x: float) -> int:
+    """askap::nint -- round half away from zero."""
+    return int(lib().orc_nint(float(x)))
+
+
+def cdiv(a: int, b: int) -> int:
+    """C/C++ integer division (truncates toward zero)."""
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b >= 0) else -q
+
+
+# --------------------------------------------------------------------------------------
+# FFT conventions
+# --------------------------------------------------------------------------------------
+def fft_centred(a: np.ndarray, forward: bool, axes=(-2, -1)) -> np.ndarray:
+    """Centred FFT: the origin is element n//2 on each axis.  Forward is unnormalised
+    exp(-i); inverse is exp(+i) scaled by 1/n."""
+    a = np.fft.ifftshift(a, axes=axes)
+    a = np.fft.fftn(a, axes=axes) if forward else np.fft.ifftn(a, axes=axes)
+    return np.fft.fftshift(a, axes=axes)
+
+
+# --------------------------------------------------------------------------------------
+# Prolate spheroidal function
+# --------------------------------------------------------------------------------------
+class Spheroidal:
+    """psi_alpha(c, nu), normalised to 1 at nu=0, zero for |nu|>=1."""
+
+    def __init__(self, c: float, alpha: float, nterms: int = 24):
+        self.m = int(alpha)
+        self.nterms = nterms
+        self.coeffs = np.zeros(nterms)
+        rc = lib().orc_sph_coeffs(ctypes.c_double(c), self.m, nterms, _p(self.coeffs))
+        assert rc == 0
+
+    def __call__(self, nu):
+        nu = np.atleast_1d(np.asarray(nu, dtype=np.float64))
+        out = np.zeros_like(nu)
+        lib().orc_sph_values(_p(self.coeffs), self.nterms, self.m, _p(np.ascontiguousarray(nu)), nu.size, _p(out))
+        return out
+
+
+def schwab_grdsf(nu: float) -> float:
+    return float(lib().orc_schwab_grdsf(float(nu)))
+
+
+# --------------------------------------------------------------------------------------
+# Convolution functions.  A CF cache is a list of complex (2S+1, 2S+1) arrays indexed
+# [iy, ix] (ix fastest, as casacore's column-major Matrix(ix, iy)), plane index
+# fracu + os*(fracv + os*plane).
+# --------------------------------------------------------------------------------------
+@dataclass
+class CFCache:
+    planes: list
+    oversample: int
+    offsets: list | None = None  # per plane before oversampling: (offU, offV)
+
+    @property
+    def support(self) -> int:
+        return max((p.shape[0] - 1) // 2 for p in self.planes)
+
+    def flat(self, dtype=np.complex64):
+        off = np.zeros(len(self.planes), dtype=np.int64)
+        supp = np.zeros(len(self.planes), dtype=np.int32)
+        total = 0
+        planes = [p if p is not None else np.zeros((1, 1)) for p in self.planes]
+        for k, p in enumerate(planes):
+            off[k] = total
+            supp[k] = (p.shape[0] - 1) // 2
+            total += p.size
+        data = np.zeros(total, dtype=dtype)
+        for k, p in enumerate(planes):
+            data[off[k]:off[k] + p.size] = p.ravel()
+        return data, off, supp
+
+
+def box_cf() -> CFCache:
+    c = np.zeros((3, 3), dtype=np.complex128)
+    c[1, 1] = 1.0
+    return CFCache([c], 1)
+
+
+def sphfunc_cf(alpha=1.0, support=3, oversample=128, pcf=False) -> CFCache:
+    os_ = oversample
+    S = support
+    planes = [None] * (os_ * os_)
+    if pcf:
+        for fracu in range(os_):
+            for fracv in range(os_):
+                c = np.zeros((3, 3), dtype=np.complex128)
+                ix = int(-float(fracu) / float(os_))
+                iy = int(-float(fracv) / float(os_))
+                c[iy + 1, ix + 1] = 1.0 + 3.0j
+                planes[fracu + os_ * fracv] = c
+    else:
+        sph = Spheroidal(math.pi * S, alpha)
+        ix = np.arange(2 * S + 1)
+        for fracv in range(os_):
+            nuy = np.abs(os_ * (ix - S) + fracv).astype(np.float64) / float(S * os_)
+            by = sph(nuy) * np.power(1.0 - nuy * nuy, alpha)
+            for fracu in range(os_):
+                nux = np.abs(os_ * (ix - S) + fracu).astype(np.float64) / float(S * os_)
+                bx = sph(nux) * np.power(1.0 - nux * nux, alpha)
+                planes[fracu + os_ * fracv] = np.outer(by, bx).astype(np.complex128)
+    for k, p in enumerate(planes):
+        norm = np.sum(np.abs(p))
+        planes[k] = p / norm
+    return CFCache(planes, os_)
+
+
+def interpolate_edge(v: np.ndarray) -> np.ndarray:
+    v = v.copy()
+    v[0] = 3.0 * (v[1] - v[2]) + v[3]
+    if v.size % 2 == 1:
+        v[-1] = v[0]
+    return v
+
+
+def sph_grid_correction(n: int, alpha: float = 1.0, support: int = 3) -> np.ndarray:
+    """1-D grid-correction factor ccf (SphFuncVisGridder::correctConvolution); the
+    grdsf used here is the c = pi*support spheroidal of the gridder."""
+    sph = Spheroidal(math.pi * support, alpha)
+    half = n // 2
+    nu = np.abs(np.arange(n) - half).astype(np.float64) / float(half)
+    buf = sph(nu).astype(np.complex128)
+    buf = interpolate_edge(buf)
+    max_before = buf[half]
+    B = fft_centred(buf, True, axes=(0,))
+    filt = 3
+    B[: half - filt] = 0.0
+    B[half + filt + 1:] = 0.0
+    b = fft_centred(B, False, axes=(0,))
+    b *= max_before / b[half]
+    val = b.real
+    return np.where(np.abs(val) > 1e-10, 1.0 / np.where(val == 0, 1, val), 0.0)
+
+
+def sph_grid_correction_xy(nx: int, ny: int, alpha: float = 1.0, support: int = 3):
+    """Both axes of correctConvolution: the y factor is scaled by the x axis' normalisation
+    maxBefore/bufx(nx/2), 1/nx of the x inverse transform included (SphFuncVisGridder.cc:216-238)."""
+    sph = Spheroidal(math.pi * support, alpha)
+
+    def filt(n):
+        half = n // 2
+        nu = np.abs(np.arange(n) - half).astype(np.float64) / float(half)
+        buf = interpolate_edge(sph(nu).astype(np.complex128))
+        B = fft_centred(buf, True, axes=(0,))
+        B[: half - 3] = 0.0
+        B[half + 4:] = 0.0
+        return buf[half], fft_centred(B, False, axes=(0,))
+
+    mb, bx = filt(nx)
+    _, by = filt(ny)
+    norm = mb / bx[nx // 2]
+    out = []
+    for b in (bx * norm, by * norm):
+        v = b.real
+        out.append(np.where(np.abs(v) > 1e-10, 1.0 / np.where(v == 0, 1, v), 0.0))
+    return out[0], out[1]
+
+
+# --------------------------------------------------------------------------------------
+# W sampling
+# --------------------------------------------------------------------------------------
+class WSampling:
+    """WDependentGridderBase plane<->w mapping (linear, powerlaw or gaussian)."""
+
+    def __init__(self, wmax: float, nwplanes: int, kind: str = "linear", exponent: float = 1.0,
+                 nwplanes50: float | None = None, wmaxclip: bool = False):
+        assert wmax > 0 and nwplanes > 0 and nwplanes % 2 == 1
+        self.wmax = wmax
+        self.nw = nwplanes
+        self.wscale = wmax / float((nwplanes - 1) // 2) if nwplanes > 1 else 1.0
+        self.kind = kind
+        self.exp = exponent
+        self.wmaxclip = wmaxclip
+        if kind == "gaussian":
+            self._gauss_setup(nwplanes50 / nwplanes)
+
+    # gaussian helpers (GaussianWSampling.cc:70-167)
+    def _gauss_target(self, ts):
+        return math.sqrt(-ts * math.log(0.5 + 0.5 * math.exp(-1.0 / ts)))
+
+    def _gauss_setup(self, w50):
+        lo = w50 * w50 / math.log(2.0) / 2.0
+        hi = 0.2 / (1 / math.sqrt(2) - w50)
+        if hi < lo:
+            lo, hi = hi, 5.0
+        ts = lo
+        for _ in range(100):
+            ts = (lo + hi) / 2.0
+            if self._gauss_target(ts) < w50:
+                lo = ts
+            else:
+                hi = ts
+            if hi - lo < 1e-5:
+                break
+        self.ts = ts
+        self.amp = 1.0 / (1.0 - math.exp(-1.0 / ts))
+
+    def map(self, plane):
+        if self.kind == "powerlaw":
+            return math.copysign(abs(plane) ** self.exp, plane) if plane != 0 else 0.0
+        if self.kind == "gaussian":
+            a = self.amp * (1 - math.exp(-plane * plane / self.ts))
+            return -a if plane < 0 else (a if plane > 0 else 0.0)
+        return plane
+
+    def index(self, wterm):
+        if self.kind == "powerlaw":
+            return math.copysign(abs(wterm) ** (1.0 / self.exp), wterm) if wterm != 0 else 0.0
+        if self.kind == "gaussian":
+            if wterm in (-1.0, 1.0):
+                return wterm
+            e = 1.0 - abs(wterm) / self.amp
+            if e <= 0:
+                raise ValueError("w-term outside the invertible range")
+            ps = -math.log(e) * self.ts
+            return -math.sqrt(ps) if wterm < 0 else (math.sqrt(ps) if wterm > 0 else 0.0)
+        return wterm
+
+    def plane(self, w: float) -> int:
+        if self.kind != "linear" and self.nw > 1:
+            half = (self.nw - 1) // 2
+            r = half + nint(self.index(w / self.wmax) * half)
+        else:
+            r = (self.nw - 1) // 2 + nint(w / self.wscale) if self.nw > 1 else 0
+        if -1 < r < self.nw:
+            return r
+        if not self.wmaxclip:
+            raise ValueError(f"W scaling error: w={w} outside the w-plane range")
+        return -1
+
+    def wterm(self, plane: int) -> float:
+        half = (self.nw - 1) // 2
+        if self.kind != "linear" and self.nw > 1:
+            return self.map(float(plane - half) / float(half)) * self.wmax
+        return float(plane - half) * self.wscale
+
+
+# --------------------------------------------------------------------------------------
+# Support search (C)
+# --------------------------------------------------------------------------------------
+@dataclass
+class Support:
+    peak: tuple
+    blc: tuple
+    trc: tuple
+    peak_val: float
+
+    def support(self):
+        return max(self.trc[0] - self.blc[0], self.trc[1] - self.blc[1])
+
+    def symmetrical(self, shape):
+        c0, c1 = shape[0] // 2, shape[1] // 2
+        xm = max(abs(self.trc[0] - c0), abs(c0 - self.blc[0]))
+        ym = max(abs(self.trc[1] - c1), abs(c1 - self.blc[1]))
+        return 2 * max(xm, ym)
+
+
+def support_search(buf_xy: np.ndarray, cutoff: float, value: float = -1.0) -> Support:
+    """buf_xy is indexed [ix, iy] (casacore order): passed as column-major."""
+    a = np.asfortranarray(buf_xy.astype(np.complex128))
+    flat = np.ascontiguousarray(a.T).view(np.float64)  # [iy][ix] row-major = column-major (ix,iy)
+    out = np.zeros(6, dtype=np.int32)
+    pk = np.zeros(1)
+    rc = lib().orc_support_search(_p(flat), buf_xy.shape[0], buf_xy.shape[1], ctypes.c_double(cutoff),
+                                  ctypes.c_double(value), _p(out), _p(pk))
+    if rc != 0:
+        raise ValueError(f"support search failed ({rc})")
+    return Support((int(out[0]), int(out[1])), (int(out[2]), int(out[3])), (int(out[4]), int(out[5])), float(pk[0]))
+
+
+# --------------------------------------------------------------------------------------
+# WProject
+# --------------------------------------------------------------------------------------
+@dataclass
+class WProjectParams:
+    wmax: float = 35000.0
+    nwplanes: int = 65
+    cutoff: float = 1e-3
+    oversample: int = 8
+    maxsupport: int = 256
+    limitsupport: int = 0
+    alpha: float = 1.0
+    variablesupport: bool = False
+    offsetsupport: bool = False
+    cutoff_absolute: bool = False
+    wsampling: str = "linear"
+    exponent: float = 1.0
+    nwplanes50: float | None = None
+    wmaxclip: bool = False
+
+    def sampling(self) -> WSampling:
+        return WSampling(self.wmax, self.nwplanes, self.wsampling, self.exponent, self.nwplanes50, self.wmaxclip)
+
+
+def _float32_taper(q: int, sph: Spheroidal) -> np.ndarray:
+    """WProjectVisGridder::setup ccfx: float arithmetic, then edge interpolation."""
+    ix = np.arange(q)
+    nu = (np.abs((ix - q // 2).astype(np.float32)) / np.float32(q // 2)).astype(np.float32)
+    v = (sph(nu.astype(np.float64)).astype(np.float32) / np.float32(q)).astype(np.float32)
+    return interpolate_edge(v).astype(np.float32)
+
+
+def _extract_support(plane_xy, prm, os_):
+    cut = prm.cutoff
+    ss = support_search(plane_xy, cut, 1.0 if prm.cutoff_absolute else -1.0)
+    offu = offv = 0
+    if prm.offsetsupport:
+        size = ss.support()
+        offu = cdiv(ss.peak[0] - plane_xy.shape[0] // 2, os_)
+        offv = cdiv(ss.peak[1] - plane_xy.shape[1] // 2, os_)
+    else:
+        size = ss.symmetrical(plane_xy.shape)
+        if size <= 0:
+            raise ValueError("Unable to determine support of convolution function")
+    size //= 2 * os_
+    if size < 3:
+        size = 3
+    return size, offu, offv
+
+
+def wproject_cf(prm: WProjectParams, nx_grid: int, ny_grid: int, uvcell: tuple, fp32_buffer: bool = True,
+                only=None):
+    """Returns (CFCache, support) for the WProject gridder on a (nx_grid, ny_grid) grid.
+    ``only``: optional set of w planes to generate (plane 0 is always searched for the common
+    support); the other planes are left as None."""
+    os_ = prm.oversample
+    nx = ny = prm.maxsupport
+    qnx, qny = nx // os_, ny // os_
+    cellx = 1.0 / (float(nx_grid) * uvcell[0])
+    celly = 1.0 / (float(ny_grid) * uvcell[1])
+    ccellx = float(nx_grid) * cellx / float(qnx)
+    ccelly = float(ny_grid) * celly / float(qny)
+    sph = Spheroidal(math.pi * 3, prm.alpha)  # WDependentGridderBase -> SphFuncVisGridder(alpha), support 3
+    ccfx = _float32_taper(qnx, sph)
+    ccfy = _float32_taper(qny, sph)
+    ws = prm.sampling()
+    iy = np.arange(qny)
+    ix = np.arange(qnx)
+    y2 = ((iy - qny // 2).astype(np.float64) * ccelly) ** 2
+    x2 = ((ix - qnx // 2).astype(np.float64) * ccellx) ** 2
+    r2 = y2[:, None] + x2[None, :]
+    inside = r2 < 1.0
+    wt = (ccfy[:, None] * ccfx[None, :]).astype(np.float32)
+    planes = [None] * (prm.nwplanes * os_ * os_)
+    offsets = [(0, 0)] * prm.nwplanes
+    common = 0
+    for iw in range(prm.nwplanes):
+        if only is not None and iw not in only and not (iw == 0 or prm.variablesupport):
+            continue
+        w = 2.0 * math.pi * ws.wterm(iw)
+        buf = np.zeros((ny, nx), dtype=np.complex128)
+        phase = w * (1.0 - np.sqrt(np.where(inside, 1.0 - r2, 1.0)))
+        val = np.where(inside, wt * np.cos(phase) - 1j * wt * np.sin(phase), 0)
+        if fp32_buffer:
+            val = val.astype(np.complex64)
+        buf[ny // 2 - qny // 2: ny // 2 - qny // 2 + qny, nx // 2 - qnx // 2: nx // 2 - qnx // 2 + qnx] = val
+        buf = fft_centred(buf, True)
+        if fp32_buffer:
+            buf = buf.astype(np.complex64).astype(np.complex128)
+        plane_xy = buf.T  # [ix, iy]
+        offu = offv = 0
+        size = common
+        if prm.variablesupport or common == 0:
+            size, offu, offv = _extract_support(plane_xy, prm, os_)
+            if not ((size + 1) * os_ < nx // 2):
+                raise ValueError("Overflowing convolution function")
+            if prm.limitsupport > 0 and size > prm.limitsupport:
+                size = prm.limitsupport
+            if common == 0:
+                common = size
+        if prm.offsetsupport:
+            offsets[iw] = (offu, offv)
+        S = size if prm.variablesupport else common
+        kxmin = (-S + offu) * os_ + nx // 2
+        kxmax = (S + offu) * os_ + os_ - 1 + nx // 2
+        kymin = (-S + offv) * os_ + ny // 2
+        kymax = (S + offv) * os_ + os_ - 1 + ny // 2
+        if kxmin < 0 or kymin < 0 or kxmax >= nx or kymax >= ny:
+            raise ValueError("Convolution function overflowing")
+        t = np.arange(-S, S + 1)
+        for fracu in range(os_):
+            kx = (t + offu) * os_ + fracu + nx // 2
+            for fracv in range(os_):
+                ky = (t + offv) * os_ + fracv + ny // 2
+                planes[fracu + os_ * (fracv + os_ * iw)] = buf[np.ix_(ky, kx)].copy()
+    for k, p in enumerate(planes):
+        if p is None:
+            continue
+        norm = np.sum(p.real)
+        if norm > 0:
+            planes[k] = p / norm
+    return CFCache(planes, os_, offsets if prm.offsetsupport else None), common
+
+
+# --------------------------------------------------------------------------------------
+# Frequency mapping
+# --------------------------------------------------------------------------------------
+def frequency_map(freqs, nchan_img: int, fstart: float | None = None, fend: float | None = None):
+    """FrequencyMapper: per accessor channel, the image channel or -1."""
+    freqs = np.asarray(freqs, dtype=np.float64)
+    if fstart is None or nchan_img == 1:
+        return np.zeros(freqs.size, dtype=np.int32)
+    inc = (fend - fstart) / float(nchan_img - 1)
+    out = np.zeros(freqs.size, dtype=np.int32)
+    for c, f in enumerate(freqs):
+        d = (f - fstart) / inc
+        d += -0.5 if d < 0 else 0.5
+        ic = int(d)
+        out[c] = ic if 0 <= ic < nchan_img else -1
+    return out
+
+
+# --------------------------------------------------------------------------------------
+# Gridder driver (TableVisGridder)
+# --------------------------------------------------------------------------------------
+class _Params(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("npol_img", ctypes.c_int), ("nchan_img", ctypes.c_int),
+                ("uvcell_u", ctypes.c_double), ("uvcell_v", ctypes.c_double), ("oversample", ctypes.c_int),
+                ("mode", ctypes.c_int), ("track_per_os", ctypes.c_int), ("nsw_rows", ctypes.c_int),
+                ("use_all_psf", ctypes.c_int), ("stale_offgrid", ctypes.c_int)]
+
+
+class _Counters(ctypes.Structure):
+    _fields_ = [("samples_gridded", ctypes.c_double), ("taps_gridded", ctypes.c_double),
+                ("samples_degridded", ctypes.c_double), ("taps_degridded", ctypes.c_double),
+                ("vectors_flagged", ctypes.c_double), ("vectors_wflagged", ctypes.c_double),
+                ("error", ctypes.c_int)]
+
+
+@dataclass
+class Chunk:
+    """One accessor chunk.  vis/noise/flag are (nrow, nchan, npol) arrays (numpy C order);
+    they are handed to the C loop in casacore column-major order."""
+    uvw: np.ndarray                  # (nrow, 3) metres (rotated uvw)
+    freq: np.ndarray                 # (nchan,) Hz
+    vis: np.ndarray                  # (nrow, nchan, npol) complex64
+    noise: np.ndarray | None = None  # (nrow, nchan, npol) complex64
+    flag: np.ndarray | None = None   # (nrow, nchan, npol) bool
+    delay: np.ndarray | None = None  # (nrow,)
+    feed1: np.ndarray | None = None  # (nrow,) int
+
+    @property
+    def nrow(self):
+        return self.uvw.shape[0]
+
+
+def _cube_f(a):
+    """(nrow, nchan, npol) -> flat casacore Cube(nRow, nChan, nPol) memory order."""
+    return np.ascontiguousarray(np.transpose(a, (2, 1, 0)))
+
+
+class OracleGridder:
+    """TableVisGridder restated; kind in {'Box', 'SphFunc', 'WProject'}.
+
+    Images/grids are numpy arrays indexed [chan, pol, y, x]."""
+
+    def __init__(self, kind: str, precision: str = "f32", sph_alpha=1.0, sph_support=3, sph_oversample=128,
+                 wproject: WProjectParams | None = None, padding: float = 1.0, use_all_psf: bool = True,
+                 track_per_os: bool = False, visweight=None):
+        self.kind = kind
+        self.precision = precision
+        self.sph = (sph_alpha, sph_support, sph_oversample)
+        self.wp = wproject
+        self.padding = padding
+        self.use_all_psf = use_all_psf
+        self.track_per_os = track_per_os
+        self.visweight = visweight  # callable freq -> weight, or None
+        self.cf = None
+        self.support = None
+        self.only_planes = None  # WProject: generate only these w planes (tests of large CF sets)
+        self.counters = _Counters()
+        self.model_empty = False
+
+    # ---- geometry ----
+    def _setup(self, shape, cell):
+        nx, ny, npol, nchan = shape
+        self.shape0 = shape
+        self.nx = int(nx * self.padding)
+        self.ny = int(ny * self.padding)
+        self.npol, self.nchan_img = npol, nchan
+        self.uvcell = (1.0 / (cell[0] * self.nx), 1.0 / (cell[1] * self.ny))
+
+    def _init_cf(self, pcf=False):
+        if self.cf is not None:
+            return
+        if self.kind == "Box":
+            self.cf = box_cf()
+        elif self.kind == "SphFunc":
+            a, s, o = self.sph
+            self.cf = sphfunc_cf(a, s, o, pcf=pcf)
+        elif self.kind == "WProject":
+            self.cf, self.support = wproject_cf(self.wp, self.nx, self.ny, self.uvcell, only=self.only_planes)
+        else:
+            raise ValueError(self.kind)
+
+    def n_cf_planes(self):
+        return self.wp.nwplanes if self.kind == "WProject" else 1
+
+    def correct(self, img):
+        """correctConvolution on a padded [.., y, x] image."""
+        if self.kind == "Box":
+            return img
+        alpha = self.sph[0] if self.kind == "SphFunc" else self.wp.alpha
+        support = self.sph[1] if self.kind == "SphFunc" else 3
+        cx, cy = sph_grid_correction_xy(self.nx, self.ny, alpha, support)
+        return img * (cy[:, None] * cx[None, :])
+
+    def initialise_grid(self, shape, cell, dopsf=False, dopcf=False, freq_axis=None):
+        self._setup(shape, cell)
+        self.mode = 1 if dopsf else (2 if dopcf else 0)
+        cdt = np.complex128 if self.precision == "f64" else np.complex64
+        self.grid_ = np.zeros((self.nchan_img, self.npol, self.ny, self.nx), dtype=cdt)
+        os_ = 1 if self.kind == "Box" else (self.sph[2] if self.kind == "SphFunc" else self.wp.oversample)
+        nsw = self.n_cf_planes() * (os_ * os_ if self.track_per_os else 1)
+        self.sumw = np.zeros((self.nchan_img, self.npol, nsw), dtype=np.float64)
+        self.freq_axis = freq_axis
+        self.counters = _Counters()
+
+    def initialise_degrid(self, shape, cell, model, freq_axis=None):
+        self._setup(shape, cell)
+        self.mode = 3
+        self.freq_axis = freq_axis
+        cdt = np.complex128 if self.precision == "f64" else np.complex64
+        self.counters = _Counters()
+        if np.max(np.abs(model)) > 0:
+            self.model_empty = False
+            pad = np.zeros((self.nchan_img, self.npol, self.ny, self.nx), dtype=np.float64)
+            ox, oy = (self.nx - shape[0]) // 2, (self.ny - shape[1]) // 2
+            pad[:, :, oy:oy + shape[1], ox:ox + shape[0]] = model
+            pad = self.correct(pad)
+            self.grid_ = fft_centred(pad.astype(np.complex128), True).astype(cdt)
+        else:
+            self.model_empty = True
+            self.grid_ = np.zeros((self.nchan_img, self.npol, self.ny, self.nx), dtype=cdt)
+
+    def _cmap(self, chunk: Chunk):
+        nrow, nchan, npol = chunk.vis.shape
+        if getattr(self, "cmap_override", None) is not None:
+            return self.cmap_override
+        if self.kind != "WProject":
+            return None
+        ws = self.wp.sampling()
+        cm = np.zeros((nchan, npol, nrow), dtype=np.int32)  # casacore (row, pol, chan)
+        for i in range(nrow):
+            w = chunk.uvw[i, 2] / C_LIGHT
+            for c in range(nchan):
+                cm[c, :, i] = ws.plane(w * chunk.freq[c])
+        return cm
+
+    def _run(self, chunk: Chunk, mode: int):
+        L = lib()
+        nrow, nchan, npol = chunk.vis.shape
+        self._init_cf(pcf=(mode == 2))
+        os_ = self.cf.oversample
+        f64 = self.precision == "f64"
+        cf_data, cf_off, cf_supp = self.cf.flat(np.complex128 if f64 else np.complex64)
+        ofs_u = ofs_v = None
+        if self.cf.offsets is not None:
+            ofs_u = np.array([o[0] for o in self.cf.offsets], dtype=np.int32)
+            ofs_v = np.array([o[1] for o in self.cf.offsets], dtype=np.int32)
+        p = _Params(self.nx, self.ny, self.npol, self.nchan_img, self.uvcell[0], self.uvcell[1], os_, mode,
+                    int(self.track_per_os), self.sumw.shape[2] if mode != 3 else 1, int(self.use_all_psf), 1)
+        vis = _cube_f(chunk.vis.astype(np.complex64))
+        noise = _cube_f(chunk.noise.astype(np.complex64)) if chunk.noise is not None else None
+        flag = _cube_f(chunk.flag.astype(np.uint8)) if chunk.flag is not None else None
+        cmap = self._cmap(chunk)
+        if self.freq_axis is not None:
+            chanmap = frequency_map(chunk.freq, self.nchan_img, *self.freq_axis)
+        else:
+            chanmap = np.zeros(nchan, dtype=np.int32)
+        vw = None
+        if self.visweight is not None:
+            vw = np.array([self.visweight(f) for f in chunk.freq], dtype=np.float32)
+        uvw = np.ascontiguousarray(chunk.uvw, dtype=np.float64)
+        freq = np.ascontiguousarray(chunk.freq, dtype=np.float64)
+        delay = np.ascontiguousarray(chunk.delay, dtype=np.float64) if chunk.delay is not None else None
+        feed1 = np.ascontiguousarray(chunk.feed1, dtype=np.int32) if chunk.feed1 is not None else None
+        fn = L.orc_generic_f64 if f64 else L.orc_generic_f32
+        if mode == 3:
+            vout = np.zeros((npol, nchan, nrow), dtype=np.complex128 if f64 else np.complex64)
+            sumw = np.zeros(1)
+        else:
+            vout = None
+            sumw = np.ascontiguousarray(self.sumw)
+        cnt = _Counters()
+        fn(ctypes.byref(p), nrow, nchan, npol, _p(uvw), _p(delay), _p(freq), _p(vis), _p(noise), _p(flag), _p(feed1),
+           _p(cmap), _p(chanmap), _p(vw), _p(cf_data), _p(cf_off), _p(cf_supp), len(self.cf.planes), _p(ofs_u),
+           _p(ofs_v), _p(self.grid_), _p(sumw), _p(vout), ctypes.byref(cnt))
+        if cnt.error:
+            raise ValueError(f"oracle generic error {cnt.error}")
+        for name, _ in _Counters._fields_[:-1]:
+            setattr(self.counters, name, getattr(self.counters, name) + getattr(cnt, name))
+        if mode != 3:
+            self.sumw = sumw
+        return vout
+
+    def grid(self, chunk: Chunk):
+        self._run(chunk, self.mode)
+
+    def degrid(self, chunk: Chunk):
+        """Returns the degridded contribution (nrow, nchan, npol) to be added to the vis."""
+        nrow, nchan, npol = chunk.vis.shape
+        if self.model_empty:
+            return np.zeros((nrow, nchan, npol), dtype=np.complex64)
+        vout = self._run(chunk, 3)
+        return np.transpose(vout, (2, 1, 0))
+
+    def finalise_grid(self):
+        g = self.grid_.astype(np.complex128)
+        img = fft_centred(g, False).real
+        img = self.correct(img) * float(self.nx) * float(self.ny)
+        nx0, ny0 = self.shape0[0], self.shape0[1]
+        ox, oy = (self.nx - nx0) // 2, (self.ny - ny0) // 2
+        return img[:, :, oy:oy + ny0, ox:ox + nx0]
+
+    def finalise_weights(self):
+        nx0, ny0 = self.shape0[0], self.shape0[1]
+        s = self.sumw.sum(axis=2)  # (chan, pol)
+        return np.broadcast_to(s[:, :, None, None], (self.nchan_img, self.npol, ny0, nx0)).copy()

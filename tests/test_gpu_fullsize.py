@@ -1,0 +1,152 @@
+"""Benchmark-size workloads on the GPU.
+
+* the SphFunc disc workload at full size (160k visibilities, 2048^2, S=3, os=128) against the
+  oracle (grid, image, weights, degridded visibilities), bar 1e-5 of peak;
+* the W-projection array geometry (4096^2, 129 w planes, os=4, S=64): a subset of rows against
+  the oracle (whose CF planes are generated only for the w planes that subset uses), and all
+  rows through size-independent properties: the adjoint identity of grid and degrid
+  (sum_s D_s(M) conj(W_s) = sum_p G(W)_p conj(M_p)) and linearity.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import announce
+from yandasoft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _pchunk(uvw, freq, vis, noise, vis_out=None):
+    import yandasoft_amd as ya
+    return ya.Chunk(uvw=np.ascontiguousarray(uvw), freq=np.ascontiguousarray(freq),
+                    vis=None if vis is None else np.ascontiguousarray(vis),
+                    vis_out=vis_out, noise=np.ascontiguousarray(noise))
+
+
+def test_sphfunc_disc_full_size(gpu_ctx):
+    announce("sphfunc disc 160k vis 2048^2 S=3 os=128")
+    import yandasoft_amd as ya
+    wl = W.sphfunc_disc()
+    img = ya.ImageDesc(wl.nx, wl.ny, 1, 1, wl.cell, wl.cell)
+    g = ya.Gridder(gpu_ctx, wl.gridder, wl.params)
+    g.initialise_grid(img)
+    g.grid(_pchunk(wl.uvw, wl.freq, wl.vis, wl.noise))
+    ggrid = g.grid_array().reshape(1, 1, wl.ny, wl.nx)
+    gimg = g.finalise_grid()
+    o = O.OracleGridder("SphFunc", "f32", sph_oversample=128)
+    o.initialise_grid((wl.nx, wl.ny, 1, 1), (wl.cell, wl.cell))
+    ch = O.Chunk(wl.uvw, wl.freq, np.transpose(wl.vis, (2, 1, 0)), noise=np.transpose(wl.noise, (2, 1, 0)))
+    o.grid(ch)
+    assert rel(ggrid, o.grid_) < TOL
+    assert rel(gimg, o.finalise_grid()) < TOL
+    assert rel(g.finalise_weights(), o.finalise_weights()) < 1e-6
+    # degrid the dirty image back
+    model = o.finalise_grid().astype(np.float32)
+    d = ya.Gridder(gpu_ctx, wl.gridder, wl.params)
+    d.initialise_degrid(img, model)
+    out = np.zeros_like(wl.vis)
+    d.degrid(_pchunk(wl.uvw, wl.freq, wl.vis, wl.noise, vis_out=out))
+    od = O.OracleGridder("SphFunc", "f32", sph_oversample=128)
+    od.initialise_degrid((wl.nx, wl.ny, 1, 1), (wl.cell, wl.cell), model)
+    ov = od.degrid(ch)
+    assert rel(np.transpose(out, (2, 1, 0)), ov) < TOL
+
+
+@pytest.fixture(scope="module")
+def wproj():
+    return W.wproject_array()
+
+
+def test_wproject_array_subset_vs_oracle(gpu_ctx, wproj):
+    announce("wproject array geometry 4096^2 S=64: row subset vs oracle")
+    import yandasoft_amd as ya
+    wl = wproj
+    rows = np.arange(0, wl.nrow, 379)[:300]
+    chans = np.array([0, 31, 63])
+    uvw = wl.uvw[rows]
+    freq = wl.freq[chans]
+    vis = np.ascontiguousarray(wl.vis[:, chans][:, :, rows])
+    noise = np.ones_like(vis)
+    prm = O.WProjectParams(wmax=wl.params["wmax"], nwplanes=wl.params["nwplanes"], cutoff=wl.params["cutoff"],
+                           oversample=wl.params["oversample"], maxsupport=wl.params["maxsupport"],
+                           limitsupport=wl.params["limitsupport"])
+    ws = prm.sampling()
+    used = {ws.plane(uvw[i, 2] / O.C_LIGHT * f) for i in range(len(rows)) for f in freq}
+    # keep the rows whose planes are few so that the oracle's FFT count stays small
+    keep_planes = sorted(used)[:: max(1, len(used) // 6)][:6]
+    sel = [i for i in range(len(rows)) if all(ws.plane(uvw[i, 2] / O.C_LIGHT * f) in keep_planes for f in freq)]
+    assert len(sel) >= 10
+    uvw, vis, noise = uvw[sel], np.ascontiguousarray(vis[:, :, sel]), np.ascontiguousarray(noise[:, :, sel])
+    img = ya.ImageDesc(wl.nx, wl.ny, 4, 1, wl.cell, wl.cell)
+    g = ya.Gridder(gpu_ctx, "GPUWProject", wl.params)
+    g.initialise_grid(img)
+    g.grid(_pchunk(uvw, freq, vis, noise))
+    o = O.OracleGridder("WProject", "f32", wproject=prm)
+    o.only_planes = set(keep_planes)
+    o.initialise_grid((wl.nx, wl.ny, 4, 1), (wl.cell, wl.cell))
+    ch = O.Chunk(uvw, freq, np.transpose(vis, (2, 1, 0)), noise=np.transpose(noise, (2, 1, 0)))
+    o.grid(ch)
+    assert o.support == 64 and g.stats().support == 64
+    for z in keep_planes:
+        for frac in (0, 5, 15):
+            k = frac + 16 * z
+            assert rel(g.cf_plane(k), o.cf.planes[k]) < TOL, k
+    ggrid = g.grid_array().reshape(1, 4, wl.ny, wl.nx)
+    assert rel(ggrid, o.grid_) < TOL
+    # degrid from a model with a few sources
+    rng = np.random.default_rng(5)
+    model = np.zeros((1, 4, wl.ny, wl.nx), np.float32)
+    for _ in range(10):
+        model[0, :, rng.integers(1000, 3000), rng.integers(1000, 3000)] = rng.normal(size=4)
+    d = ya.Gridder(gpu_ctx, "GPUWProject", wl.params)
+    d.initialise_degrid(img, model)
+    out = np.zeros_like(vis)
+    d.degrid(_pchunk(uvw, freq, vis, noise, vis_out=out))
+    od = O.OracleGridder("WProject", "f32", wproject=prm)
+    od.only_planes = set(keep_planes)
+    od.initialise_degrid((wl.nx, wl.ny, 4, 1), (wl.cell, wl.cell), model.astype(np.float64))
+    ov = od.degrid(ch)
+    assert rel(np.transpose(out, (2, 1, 0)), ov) < TOL
+
+
+def test_wproject_array_all_rows_adjoint_and_linearity(gpu_ctx, wproj):
+    announce("wproject array all rows: adjoint identity + linearity")
+    import yandasoft_amd as ya
+    wl = wproj
+    # a quarter of the channels keeps host memory modest; geometry and CFs are the full ones
+    chans = np.arange(0, wl.nchan, 4)
+    freq = wl.freq[chans]
+    V = np.ascontiguousarray(wl.vis[:, chans])
+    noise = np.ones_like(V)
+    img = ya.ImageDesc(wl.nx, wl.ny, 4, 1, wl.cell, wl.cell)
+    g = ya.Gridder(gpu_ctx, "GPUWProject", wl.params)
+    g.initialise_grid(img)
+    g.grid(_pchunk(wl.uvw, freq, V, noise))
+    G = g.grid_array().astype(np.complex128)
+    s = g.stats()
+    assert s.taps_gridded == s.samples_gridded * (2 * 64 + 1) ** 2
+    rng = np.random.default_rng(9)
+    model = np.zeros((1, 4, wl.ny, wl.nx), np.float32)
+    for _ in range(30):
+        model[0, :, rng.integers(500, 3500), rng.integers(500, 3500)] = rng.normal(size=4)
+    d = ya.Gridder(gpu_ctx, "GPUWProject", wl.params)
+    d.initialise_degrid(img, model)
+    M = d.grid_array().astype(np.complex128)
+    out = np.zeros_like(V)
+    d.degrid(_pchunk(wl.uvw, freq, V, noise, vis_out=out))
+    lhs = np.sum(out.astype(np.complex128) * np.conj(V.astype(np.complex128)))
+    rhs = np.sum(G * np.conj(M))
+    assert abs(lhs - rhs) <= 1e-5 * max(abs(lhs), abs(rhs))
+    # linearity: grid(2V) - 2 grid(V) = 0
+    g2 = ya.Gridder(gpu_ctx, "GPUWProject", wl.params)
+    g2.initialise_grid(img)
+    g2.grid(_pchunk(wl.uvw, freq, np.ascontiguousarray(2 * V), noise))
+    assert rel(g2.grid_array(), 2 * G) < TOL

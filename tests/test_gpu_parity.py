@@ -1,0 +1,290 @@
+"""HIP gridder (through the C ABI) against the CPU oracle on identical seeded inputs.
+
+Bar: max|gpu - oracle| / max|oracle| <= 1e-5 (FP32) for the grid, the dirty image, the sum of
+weights and the degridded visibilities; CF planes to 1e-5 of their peak.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import announce
+
+TOL = 1e-5
+ARC = math.pi / 180 / 3600
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    b = np.asarray(b)
+    return float(np.max(np.abs(np.asarray(a) - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def make_data(nrow, nchan, npol, N, cell, f0=1.4e9, df=5e6, frac=0.45, wmax_m=0.0, seed=7, flags=False,
+              delay=False, noise_var=False):
+    rng = np.random.default_rng(seed)
+    freq = f0 + df * np.arange(nchan)
+    lam = O.C_LIGHT / freq.max()
+    uvmax = frac * (N / 2) * (1 / (cell * N)) * lam
+    r = uvmax * np.sqrt(rng.random(nrow))
+    th = 2 * np.pi * rng.random(nrow)
+    w = (rng.random(nrow) * 2 - 1) * wmax_m
+    uvw = np.ascontiguousarray(np.stack([r * np.cos(th), r * np.sin(th), w], 1))
+    vis = (rng.normal(size=(npol, nchan, nrow)) + 1j * rng.normal(size=(npol, nchan, nrow))).astype(np.complex64)
+    noise = np.ones_like(vis)
+    if noise_var:
+        noise = (0.5 + rng.random((npol, nchan, nrow))).astype(np.float32).astype(np.complex64)
+    flag = None
+    if flags:
+        flag = (rng.random((npol, nchan, nrow)) < 0.05).astype(np.uint8)
+    dl = rng.normal(size=nrow) * 0.3 if delay else None
+    return dict(uvw=uvw, freq=freq, vis=vis, noise=noise, flag=flag, delay=dl)
+
+
+def ochunk(d):
+    t = lambda a: None if a is None else np.transpose(a, (2, 1, 0))
+    return O.Chunk(d["uvw"], d["freq"], t(d["vis"]), noise=t(d["noise"]),
+                   flag=None if d["flag"] is None else t(d["flag"]).astype(bool), delay=d["delay"])
+
+
+def pchunk(d, vis_out=None):
+    import yandasoft_amd as ya
+    return ya.Chunk(uvw=d["uvw"], freq=d["freq"], vis=d["vis"], vis_out=vis_out, noise=d["noise"], flag=d["flag"],
+                    delay=d["delay"])
+
+
+def grid_both(ctx, name, params, ogrid_kwargs, d, N, cell, npol, nchan_img=1, dopsf=False, dopcf=False,
+              freq_axis=None, mfs=None):
+    import yandasoft_amd as ya
+    g = ya.Gridder(ctx, name, params)
+    if mfs:
+        g.set_mfs_weight(*mfs)
+    img = ya.ImageDesc(N, N, npol, nchan_img, cell, cell, freq_axis)
+    g.initialise_grid(img, dopsf=dopsf, dopcf=dopcf)
+    g.grid(pchunk(d))
+    ggrid = g.grid_array().reshape(nchan_img, npol, N, N)
+    gimg = g.finalise_grid()
+    gw = g.finalise_weights()
+    o = O.OracleGridder(precision="f32", **ogrid_kwargs)
+    if mfs:
+        o.visweight = lambda f, r=mfs[0], k=mfs[1]: ((f - r) / r) ** k
+    o.initialise_grid((N, N, npol, nchan_img), (cell, cell), dopsf=dopsf, dopcf=dopcf, freq_axis=freq_axis)
+    o.grid(ochunk(d))
+    oimg = o.finalise_grid()
+    ow = o.finalise_weights()
+    return g, o, ggrid, o.grid_, gimg, oimg, gw, ow
+
+
+def degrid_both(ctx, name, params, ogrid_kwargs, d, N, cell, npol, model, nchan_img=1, freq_axis=None):
+    import yandasoft_amd as ya
+    g = ya.Gridder(ctx, name, params)
+    img = ya.ImageDesc(N, N, npol, nchan_img, cell, cell, freq_axis)
+    g.initialise_degrid(img, model.astype(np.float32))
+    out = np.zeros_like(d["vis"])
+    g.degrid(pchunk(d, vis_out=out))
+    o = O.OracleGridder(precision="f32", **ogrid_kwargs)
+    o.initialise_degrid((N, N, npol, nchan_img), (cell, cell), model, freq_axis=freq_axis)
+    ov = o.degrid(ochunk(d))
+    return np.transpose(out, (2, 1, 0)), ov, g, o
+
+
+def random_model(N, npol, nchan=1, seed=11, npts=20):
+    rng = np.random.default_rng(seed)
+    m = np.zeros((nchan, npol, N, N))
+    for _ in range(npts):
+        m[:, :, rng.integers(N // 4, 3 * N // 4), rng.integers(N // 4, 3 * N // 4)] += rng.normal(size=(nchan, npol))
+    return m
+
+
+# ---------------------------------------------------------------------------------------
+def test_sphfunc_cf_planes_match_oracle(gpu_ctx):
+    announce("sphfunc cf planes S=3 os=128")
+    import yandasoft_amd as ya
+    N, cell = 256, 10 * ARC
+    d = make_data(50, 1, 1, N, cell)
+    g = ya.Gridder(gpu_ctx, "GPUSphFunc", {"support": 3, "oversample": 128})
+    g.initialise_grid(ya.ImageDesc(N, N, 1, 1, cell, cell))
+    g.grid(pchunk(d))
+    ocf = O.sphfunc_cf(1.0, 3, 128)
+    for k in (0, 1, 127, 128, 5000, 16383):
+        assert rel(g.cf_plane(k), ocf.planes[k]) < TOL, k
+
+
+@pytest.mark.parametrize("npol,nchan,flags,delay,noise_var", [(1, 1, False, False, False), (4, 3, True, True, True)])
+def test_sphfunc_grid_matches_oracle(gpu_ctx, npol, nchan, flags, delay, noise_var):
+    announce(f"sphfunc grid npol={npol} nchan={nchan} flags={flags}")
+    N, cell = 256, 10 * ARC
+    d = make_data(3000, nchan, npol, N, cell, flags=flags, delay=delay, noise_var=noise_var)
+    g, o, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUSphFunc", {"oversample": 128},
+                                            dict(kind="SphFunc"), d, N, cell, npol)
+    assert rel(gg, og) < TOL
+    assert rel(gi, oi) < TOL
+    assert rel(gw, ow) < 1e-6
+    s = g.stats()
+    assert s.samples_gridded == o.counters.samples_gridded
+    assert s.taps_gridded == o.counters.taps_gridded
+    assert s.vectors_flagged == o.counters.vectors_flagged
+
+
+@pytest.mark.parametrize("npol", [1, 4])
+def test_sphfunc_degrid_matches_oracle(gpu_ctx, npol):
+    announce(f"sphfunc degrid npol={npol}")
+    N, cell = 256, 10 * ARC
+    d = make_data(3000, 2, npol, N, cell, flags=True, delay=True)
+    gv, ov, g, o = degrid_both(gpu_ctx, "GPUSphFunc", {"oversample": 128}, dict(kind="SphFunc"), d, N, cell, npol,
+                               random_model(N, npol))
+    assert rel(gv, ov) < TOL
+    assert g.stats().taps_degridded == o.counters.taps_degridded
+
+
+def test_box_grid_and_degrid(gpu_ctx):
+    announce("box grid/degrid")
+    N, cell = 128, 20 * ARC
+    d = make_data(2000, 2, 2, N, cell)
+    _, _, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUBox", {}, dict(kind="Box"), d, N, cell, 2)
+    assert rel(gg, og) < TOL and rel(gi, oi) < TOL and rel(gw, ow) < 1e-6
+    gv, ov, _, _ = degrid_both(gpu_ctx, "GPUBox", {}, dict(kind="Box"), d, N, cell, 2, random_model(N, 2))
+    assert rel(gv, ov) < TOL
+
+
+def test_psf_and_pcf_modes(gpu_ctx):
+    announce("sphfunc psf/pcf")
+    N, cell = 128, 20 * ARC
+    d = make_data(1500, 1, 1, N, cell, noise_var=True)
+    _, _, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUSphFunc", {"oversample": 32, "alldatapsf": True},
+                                            dict(kind="SphFunc", sph_oversample=32), d, N, cell, 1, dopsf=True)
+    assert rel(gg, og) < TOL and rel(gi, oi) < TOL and rel(gw, ow) < 1e-6
+    _, _, gg, og, gi, oi, _, _ = grid_both(gpu_ctx, "GPUSphFunc", {"oversample": 32},
+                                          dict(kind="SphFunc", sph_oversample=32), d, N, cell, 1, dopcf=True)
+    assert rel(gg, og) < TOL
+
+
+def test_mfs_weights_and_frequency_axis(gpu_ctx):
+    announce("mfs weights + spectral image")
+    N, cell = 128, 20 * ARC
+    d = make_data(1500, 6, 1, N, cell, f0=1.3e9, df=2e7)
+    fa = (1.3e9, 1.4e9)
+    _, _, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUSphFunc", {"oversample": 16},
+                                            dict(kind="SphFunc", sph_oversample=16), d, N, cell, 1, nchan_img=3,
+                                            freq_axis=fa, mfs=(1.35e9, 1))
+    assert rel(gg, og) < TOL and rel(gi, oi) < TOL and rel(gw, ow) < 1e-6
+
+
+WP = dict(wmax=3000.0, nwplanes=9, oversample=4, maxsupport=256, cutoff=1e-2)
+
+
+def _wp_oracle_params(**kw):
+    p = dict(WP)
+    p.update(kw)
+    return O.WProjectParams(**{k: v for k, v in p.items()})
+
+
+@pytest.mark.parametrize("extra", [{}, {"variablesupport": True}, {"variablesupport": True, "offsetsupport": True},
+                                   {"wsampling": "powerlaw", "wsampling.exponent": 0.5}])
+def test_wproject_cf_grid_degrid_match_oracle(gpu_ctx, extra):
+    announce(f"wproject {extra}")
+    N, cell = 256, 30 * ARC
+    d = make_data(1500, 2, 4, N, cell, f0=1.0e9, frac=0.35, wmax_m=2500.0 * O.C_LIGHT / 1.005e9)
+    params = dict(WP)
+    params.update(extra)
+    okw = dict(WP)
+    okw.update({k.replace("wsampling.exponent", "exponent"): v for k, v in extra.items()})
+    prm = O.WProjectParams(**okw)
+    g, o, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUWProject", params, dict(kind="WProject", wproject=prm), d,
+                                            N, cell, 4)
+    for k in (0, 7, 16 * 4 + 5, 16 * 8 + 15):
+        ref = o.cf.planes[k]
+        got = g.cf_plane(k)
+        assert got.shape == ref.shape, k
+        assert rel(got, ref) < TOL, k
+    assert rel(gg, og) < TOL
+    assert rel(gi, oi) < TOL
+    assert rel(gw, ow) < 1e-6
+    gv, ov, _, _ = degrid_both(gpu_ctx, "GPUWProject", params, dict(kind="WProject", wproject=prm), d, N, cell, 4,
+                               random_model(N, 4))
+    assert rel(gv, ov) < TOL
+
+
+def test_wproject_wmax_handling(gpu_ctx):
+    announce("wproject wmaxclip / w out of range")
+    import yandasoft_amd as ya
+    N, cell = 128, 30 * ARC
+    d = make_data(500, 1, 1, N, cell, f0=1.0e9, frac=0.3, wmax_m=5000.0 * O.C_LIGHT / 1.0e9)
+    p = dict(WP, wmax=2000.0)
+    g = ya.Gridder(gpu_ctx, "GPUWProject", p)
+    g.initialise_grid(ya.ImageDesc(N, N, 1, 1, cell, cell))
+    with pytest.raises(ya.YgridError):
+        g.grid(pchunk(d))
+    p["wmaxclip"] = True
+    _, o, gg, og, gi, oi, _, _ = grid_both(gpu_ctx, "GPUWProject", p,
+                                          dict(kind="WProject", wproject=_wp_oracle_params(wmax=2000.0, wmaxclip=True)),
+                                          d, N, cell, 1)
+    assert rel(gg, og) < TOL
+    assert o.counters.vectors_wflagged > 0
+
+
+def test_edge_cases(gpu_ctx):
+    announce("edge cases: empty chunk, all flagged, zero noise, off-grid, empty model")
+    import yandasoft_amd as ya
+    N, cell = 128, 20 * ARC
+    g = ya.Gridder(gpu_ctx, "GPUSphFunc", {"oversample": 8})
+    g.initialise_grid(ya.ImageDesc(N, N, 1, 1, cell, cell))
+    d = make_data(0, 1, 1, N, cell)
+    g.grid(pchunk(d))
+    d = make_data(100, 1, 1, N, cell)
+    d["flag"] = np.ones((1, 1, 100), np.uint8)
+    g.grid(pchunk(d))
+    assert np.all(g.grid_array() == 0)
+    assert g.stats().vectors_flagged == 100
+    d = make_data(100, 1, 1, N, cell)
+    d["noise"][0, 0, 5] = 0
+    with pytest.raises(ya.YgridError):
+        g.grid(pchunk(d))
+    # off-grid samples are dropped by grid and leave visibilities untouched in degrid
+    d = make_data(50, 1, 1, N, cell, frac=2.0)
+    g2 = ya.Gridder(gpu_ctx, "GPUSphFunc", {"oversample": 8})
+    g2.initialise_degrid(ya.ImageDesc(N, N, 1, 1, cell, cell), random_model(N, 1).astype(np.float32))
+    out = np.full_like(d["vis"], 7 + 7j)
+    g2.degrid(pchunk(d, vis_out=out))
+    o = O.OracleGridder("SphFunc", "f32", sph_oversample=8)
+    o.initialise_degrid((N, N, 1, 1), (cell, cell), random_model(N, 1))
+    ochk = ochunk(d)
+    ov = o.degrid(ochk)
+    lam = O.C_LIGHT / d["freq"][0]
+    r = np.hypot(d["uvw"][:, 0], d["uvw"][:, 1]) / lam * cell * N
+    ongrid = (np.abs(d["uvw"][:, 0]) / lam * cell * N < N / 2 - 5) & (np.abs(d["uvw"][:, 1]) / lam * cell * N < N / 2 - 5)
+    offgrid = (np.abs(d["uvw"][:, 0]) / lam * cell * N > N / 2 + 5) | (np.abs(d["uvw"][:, 1]) / lam * cell * N > N / 2 + 5)
+    assert np.all(out[0, 0, offgrid] == 7 + 7j)
+    got = np.transpose(out, (2, 1, 0))[ongrid] - (7 + 7j)
+    assert rel(got, ov[ongrid]) < TOL
+    # an empty model makes degridding a no-op
+    g3 = ya.Gridder(gpu_ctx, "GPUSphFunc", {"oversample": 8})
+    g3.initialise_degrid(ya.ImageDesc(N, N, 1, 1, cell, cell), np.zeros((1, 1, N, N), np.float32))
+    assert g3.is_model_empty()
+    out = np.ones_like(d["vis"])
+    g3.degrid(pchunk(d, vis_out=out))
+    assert np.all(out == 1)
+
+
+def test_clone_is_independent(gpu_ctx):
+    announce("clone")
+    import yandasoft_amd as ya
+    N, cell = 128, 20 * ARC
+    d = make_data(800, 1, 1, N, cell)
+    g = ya.Gridder(gpu_ctx, "GPUSphFunc", {"oversample": 16})
+    g.initialise_grid(ya.ImageDesc(N, N, 1, 1, cell, cell))
+    g.grid(pchunk(d))
+    c = g.clone()
+    c.initialise_grid(ya.ImageDesc(N, N, 1, 1, cell, cell))
+    c.grid(pchunk(d))
+    c.grid(pchunk(d))
+    assert rel(c.grid_array(), 2 * g.grid_array()) < TOL
+
+
+def test_unknown_gridder_is_rejected(gpu_ctx):
+    import yandasoft_amd as ya
+    with pytest.raises(ya.YgridError) as e:
+        ya.Gridder(gpu_ctx, "NoSuchGridder", {})
+    assert e.value.code == 5

@@ -1,0 +1,328 @@
+"""Pins of the CPU oracle against what the reference's tests and the mathematics fix.
+
+Each check is chosen so that a plausible slip in the oracle (a dropped term, a wrong sign or
+index, a transposed operand) fails it.  No GPU needed.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sps
+
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------------------------------
+# prolate spheroidal function (scimath::SpheroidalFunction, absent dependency)
+# ---------------------------------------------------------------------------------------
+def test_spheroidal_matches_schwab_rational_approximation():
+    # SphFuncVisGridder.cc:268-333: the m=6, alpha=1 approximation "agreed to about the 7th
+    # significant digit" with Schwab's sphfn; c = pi*support with support 3.
+    sph = O.Spheroidal(math.pi * 3, 1.0)
+    nu = np.linspace(0.0, 0.999, 200)
+    ours = sph(nu)
+    schwab = np.array([O.schwab_grdsf(x) for x in nu])
+    assert np.max(np.abs(ours - schwab)) < 1e-6
+
+
+@pytest.mark.parametrize("support,alpha", [(3, 1), (4, 1), (5, 2), (7, 0), (3, 2)])
+def test_spheroidal_matches_scipy_pro_ang1(support, alpha):
+    # psi_alpha(c, nu) = S_{alpha,alpha}(c, nu) (1-nu^2)^(-alpha/2) / S(0)
+    # (apps/gridderExperiments.cc:232-233)
+    c = math.pi * support
+    sph = O.Spheroidal(c, alpha)
+    nu = np.linspace(0.0, 0.98, 50)
+    ref = np.array([sps.pro_ang1(alpha, alpha, c, x)[0] for x in nu])
+    ref0 = sps.pro_ang1(alpha, alpha, c, 0.0)[0]
+    ref = ref / ref0 * np.power(1 - nu * nu, -alpha / 2.0)
+    assert np.max(np.abs(sph(nu) - ref)) < 1e-9
+
+
+def test_spheroidal_zero_outside_unit_interval():
+    sph = O.Spheroidal(math.pi * 3, 1.0)
+    assert np.all(sph(np.array([1.0, 1.5, -1.0])) == 0.0)
+    assert abs(sph(np.array([0.0]))[0] - 1.0) < 1e-14
+
+
+# ---------------------------------------------------------------------------------------
+# SupportSearcher (tests/gridding/SupportSearcherTest.h:52-115, restated)
+# ---------------------------------------------------------------------------------------
+def _gauss_buffer():
+    xo, yo = -3, 7
+    rows = np.arange(50)
+    cols = np.arange(50)
+    x = (rows[:, None] - 25 - xo) / 5.0
+    y = (cols[None, :] - 25 - yo) / 5.0
+    return (5.0 * np.exp(-(x * x + y * y) / 2)).astype(np.complex64), xo, yo
+
+
+def test_support_searcher_peak_find():
+    buf, xo, yo = _gauss_buffer()
+    ss = O.support_search(buf, 1e-2)
+    assert ss.peak == (25 + xo, 25 + yo)
+    assert abs(ss.peak_val - 5.0) < 1e-6
+
+
+@pytest.mark.parametrize("absolute", [False, True])
+def test_support_searcher_known_answers(absolute):
+    buf, xo, yo = _gauss_buffer()
+    cutoff = 1e-2
+    if absolute:
+        ss = O.support_search(buf, cutoff * 5.0, 1.0)
+    else:
+        ss = O.support_search(buf, cutoff)
+    hw = 5.0 * math.sqrt(-2.0 * math.log(cutoff))
+    assert abs(ss.support() - 2 * hw) < 1.0
+    assert abs(ss.symmetrical(buf.shape) - 2 * (hw + max(xo, yo))) < 1.0
+    assert abs(ss.blc[0] - (25 + xo - hw)) < 1.0
+    assert abs(ss.blc[1] - (25 + yo - hw)) < 1.0
+    assert abs(ss.trc[0] - (25 + xo + hw)) < 1.0
+    assert abs(ss.trc[1] - (25 + yo + hw)) < 1.0
+
+
+# ---------------------------------------------------------------------------------------
+# W sampling (tests/gridding/NonLinearWSamplingTest.h:50-90, restated) and getWPlane
+# ---------------------------------------------------------------------------------------
+def _round_trip(ws):
+    mid = 0.0
+    for i in range(101):
+        plane = 2.0 * (i - 50) / 100.0
+        m = ws.map(plane)
+        if -0.5 <= m <= 0.5:
+            mid += 1.0 / 101.0
+        assert abs(ws.index(m) - plane) < 1e-6
+    assert abs(ws.index(0.0)) < 1e-6
+    assert abs(ws.index(-1.0) + 1.0) < 1e-6
+    assert abs(ws.index(1.0) - 1.0) < 1e-6
+    return mid
+
+
+def test_power_law_sampling_round_trip():
+    for i in range(5):
+        exponent = 2.0 * (i - 5) / 10.0
+        _round_trip(O.WSampling(1.0, 3, "powerlaw", exponent=exponent))
+
+
+def test_gaussian_sampling_round_trip():
+    for i in range(3, 11):
+        w50 = i / 11.0 / math.sqrt(2)
+        ws = O.WSampling(1.0, 101, "gaussian", nwplanes50=w50 * 101)
+        mid = _round_trip(ws)
+        assert abs(w50 - mid) < 0.02
+
+
+def test_linear_wplane_mapping():
+    ws = O.WSampling(1000.0, 9)  # wscale = 250
+    assert ws.plane(0.0) == 4
+    assert ws.plane(1000.0) == 8 and ws.plane(-1000.0) == 0
+    assert ws.plane(124.9) == 4 and ws.plane(125.1) == 5 and ws.plane(-125.1) == 3
+    assert ws.wterm(0) == -1000.0 and ws.wterm(8) == 1000.0
+    with pytest.raises(ValueError):
+        ws.plane(1200.0)
+    assert O.WSampling(1000.0, 9, wmaxclip=True).plane(1200.0) == -1
+
+
+# ---------------------------------------------------------------------------------------
+# FrequencyMapper (tests/gridding/FrequencyMapperTest.h, restated)
+# ---------------------------------------------------------------------------------------
+def test_frequency_mapper():
+    freqs = 1.308e9 + 1.6e7 * (np.arange(10) - 1.0)
+    m = O.frequency_map(freqs, 8, 1.308e9, 1.42e9)
+    assert m[0] == -1 and m[-1] == -1
+    assert all(m[i] + 1 == i for i in range(1, 9))
+    freqs = 1.436e9 - 1.6e7 * np.arange(10) + np.where(np.arange(10) % 2 == 0, 1.0, -1.0) * 1e6
+    m = O.frequency_map(freqs, 8, 1.308e9, 1.42e9)
+    assert m[0] == -1 and m[-1] == -1
+    assert all(m[i] + 1 == 10 - i - 1 for i in range(1, 9))
+    freqs = 1.304e9 + 8e6 * (np.arange(20) - 2.0)
+    m = O.frequency_map(freqs, 8, 1.308e9, 1.42e9)
+    assert all((m[i] == -1) if (i < 2 or i + 2 >= 20) else (m[i] + 1 == i // 2) for i in range(20))
+    assert np.all(O.frequency_map(freqs, 1) == 0)
+
+
+# ---------------------------------------------------------------------------------------
+# convolution functions
+# ---------------------------------------------------------------------------------------
+def test_box_cf_closed_form():
+    cf = O.box_cf()
+    assert cf.oversample == 1 and len(cf.planes) == 1
+    expect = np.zeros((3, 3))
+    expect[1, 1] = 1
+    assert np.array_equal(cf.planes[0], expect)
+
+
+def test_sphfunc_cf_separable_and_normalised():
+    cf = O.sphfunc_cf(1.0, 3, 8)
+    sph = O.Spheroidal(3 * math.pi, 1.0)
+    for fracu, fracv in [(0, 0), (3, 5), (7, 1)]:
+        p = cf.planes[fracu + 8 * fracv]
+        assert abs(np.sum(np.abs(p)) - 1.0) < 1e-12
+        # separable closed form with nu = |os*(i-S)+frac|/(S*os)
+        i = np.arange(7)
+        nux = np.abs(8 * (i - 3) + fracu) / 24.0
+        nuy = np.abs(8 * (i - 3) + fracv) / 24.0
+        fx = sph(nux) * (1 - nux ** 2)
+        fy = sph(nuy) * (1 - nuy ** 2)
+        ref = np.outer(fy, fx)
+        ref /= ref.sum()
+        assert np.max(np.abs(p - ref)) < 1e-14
+    # fracu shifts the kernel along u (columns), not v: a transposed operand would fail
+    p = cf.planes[3]
+    assert np.allclose(p, p[::-1, :])
+    assert not np.allclose(p, p[:, ::-1])
+
+
+def test_grid_correction_closed_form():
+    n = 256
+    c = O.sph_grid_correction(n)
+    assert abs(c[n // 2] - 1.0) < 1e-12
+    assert np.all(np.diff(c[n // 2:200]) > 0)  # correction grows away from the centre
+    assert np.allclose(c[n // 2 + 1:], c[1:n // 2][::-1], rtol=1e-12)  # symmetric about n/2
+    # direct evaluation of the 7 low-frequency terms about the centre (no FFT shifts)
+    sph = O.Spheroidal(3 * math.pi, 1.0)
+    b = sph(np.abs(np.arange(n) - n // 2) / (n // 2))
+    b[0] = 3 * (b[1] - b[2]) + b[3]
+    x = np.arange(n) - n // 2
+    F = [np.sum(b * np.exp(-2j * np.pi * k * x / n)) for k in range(-3, 4)]
+    f = sum(F[k + 3] * np.exp(2j * np.pi * k * x / n) for k in range(-3, 4)).real
+    f *= b[n // 2] / f[n // 2]
+    assert np.allclose(c, 1.0 / f, rtol=1e-10)
+
+
+def test_wproject_cf_w0_plane_is_taper_transform_and_normalised():
+    prm = O.WProjectParams(wmax=2000.0, nwplanes=5, cutoff=1e-3, oversample=4, maxsupport=128)
+    N, cell = 256, math.radians(20 / 3600)
+    uvcell = (1 / (cell * N), 1 / (cell * N))
+    cf, S = O.wproject_cf(prm, N, N, uvcell, fp32_buffer=False)
+    os_ = 4
+    for z in range(5):
+        for frac in range(16):
+            p = cf.planes[frac + 16 * z]
+            assert abs(np.sum(p.real) - 1.0) < 1e-12
+    # brute-force DFT of the taper x phase screen, with coordinates relative to the centre:
+    # C(ix, iy) ~ sum_xy t(x) t(y) exp(-i w (1 - sqrt(1 - r^2))) exp(-2 pi i (kx x + ky y)/n),
+    # kx = ix*os + fracu (columns), ky = iy*os + fracv (rows), normalised by sum Re.
+    n = prm.maxsupport
+    q = n // os_
+    sph = O.Spheroidal(3 * math.pi, 1.0)
+    x = np.arange(q) - q // 2
+    t = (sph(np.abs(x) / (q // 2)) / q).astype(np.float32)
+    t[0] = 3 * (t[1] - t[2]) + t[3]
+    ccell = 1.0 / (q * uvcell[0])
+    r2 = (x[:, None] * ccell) ** 2 + (x[None, :] * ccell) ** 2
+    ws = prm.sampling()
+    for z, fu, fv in [(2, 0, 0), (0, 1, 3), (4, 3, 2)]:
+        w = 2 * math.pi * ws.wterm(z)
+        scr = np.where(r2 < 1, np.outer(t, t) * np.exp(-1j * w * (1 - np.sqrt(np.clip(1 - r2, 0, None)))), 0)
+        k = np.arange(-S, S + 1) * os_
+        ex = np.exp(-2j * np.pi * np.outer(k + fu, x) / n)  # (taps, x)
+        ey = np.exp(-2j * np.pi * np.outer(k + fv, x) / n)
+        ref = ey @ scr @ ex.T  # [iy, ix]
+        ref /= np.sum(ref.real)
+        assert np.max(np.abs(cf.planes[fu + os_ * (fv + os_ * z)] - ref)) < 1e-10
+
+
+# ---------------------------------------------------------------------------------------
+# the gridding loop: closed forms, brute force, adjoint identity
+# ---------------------------------------------------------------------------------------
+def _disc(nrow, N, cell, f, rng, frac=0.4):
+    lam = O.C_LIGHT / f
+    uvmax = frac * (N / 2) * (1 / (cell * N)) * lam
+    r = uvmax * np.sqrt(rng.random(nrow))
+    th = 2 * np.pi * rng.random(nrow)
+    return np.stack([r * np.cos(th), r * np.sin(th), np.zeros(nrow)], 1)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_point_source_degrids_to_flux(prec):
+    # ImageFFTEquationTest.h:124-175: a model pixel of flux F at the image centre predicts F.
+    rng = np.random.default_rng(1)
+    N, cell, f = 128, math.radians(10 / 3600), np.array([1.4e9])
+    uvw = _disc(300, N, cell, f[0], rng)
+    vis = np.zeros((300, 1, 1), np.complex64)
+    model = np.zeros((1, 1, N, N))
+    model[0, 0, N // 2, N // 2] = 2.5
+    g = O.OracleGridder("SphFunc", prec, sph_oversample=16)
+    g.initialise_degrid((N, N, 1, 1), (cell, cell), model)
+    v = g.degrid(O.Chunk(uvw, f, vis))
+    assert np.max(np.abs(v - 2.5)) < 1e-5
+
+
+def test_unit_visibilities_image_peak_is_sum_of_weights():
+    rng = np.random.default_rng(2)
+    N, cell, f = 128, math.radians(10 / 3600), np.array([1.4e9])
+    uvw = _disc(400, N, cell, f[0], rng)
+    vis = np.ones((400, 1, 1), np.complex64)
+    noise = np.full_like(vis, 0.5)  # weight 1/0.25 = 4
+    g = O.OracleGridder("SphFunc", "f64", sph_oversample=16)
+    g.initialise_grid((N, N, 1, 1), (cell, cell))
+    g.grid(O.Chunk(uvw, f, vis, noise=noise))
+    img = g.finalise_grid()
+    w = g.finalise_weights()
+    assert abs(w[0, 0, 0, 0] - 1600.0) < 1e-9
+    assert abs(img[0, 0, N // 2, N // 2] - 1600.0) < 1e-6
+    # the unit-visibility image is the PSF: its centre is the maximum of the inner quarter
+    inner = img[0, 0, N // 4:3 * N // 4, N // 4:3 * N // 4]
+    assert np.unravel_index(np.argmax(inner), inner.shape) == (N // 4, N // 4)
+
+
+def test_grid_brute_force_single_sample():
+    # one visibility, weight 1: grid(iu+du, iv+dv) = conj(V) * C(du, dv) on the chosen plane
+    N, cell, f = 64, math.radians(30 / 3600), np.array([1.0e9])
+    lam = O.C_LIGHT / f[0]
+    du = 1 / (cell * N)
+    os_ = 8
+    u = (5 + 3.0 / os_ + 0.01) * du * lam  # fractional offset avoids exact ties
+    v = (-7 - 2.0 / os_ - 0.01) * du * lam
+    uvw = np.array([[u, v, 0.0]])
+    vis = np.array([[[0.3 - 0.8j]]], np.complex64)
+    g = O.OracleGridder("SphFunc", "f64", sph_oversample=os_)
+    g.initialise_grid((N, N, 1, 1), (cell, cell))
+    g.grid(O.Chunk(uvw, f, vis))
+    uS, vS = u / lam / du, v / lam / du
+    iu, iv = O.nint(uS), O.nint(vS)
+    fu, fv = O.nint(os_ * (iu - uS)), O.nint(os_ * (iv - vS))
+    if fu < 0:
+        iu, fu = iu + 1, fu + os_
+    if fv < 0:
+        iv, fv = iv + 1, fv + os_
+    plane = g.cf.planes[fu + os_ * fv]
+    expect = np.zeros((N, N), complex)
+    expect[iv + N // 2 - 3:iv + N // 2 + 4, iu + N // 2 - 3:iu + N // 2 + 4] = np.conj(vis[0, 0, 0]) * plane
+    assert np.max(np.abs(g.grid_[0, 0] - expect)) < 1e-12
+
+
+def test_adjoint_identity_of_grid_and_degrid():
+    # sum_s D_s(M) conj(W_s) = sum_p G(W)[p] conj(M_p)   (GridKernel.cc:79-80 vs :121-122)
+    rng = np.random.default_rng(3)
+    N, cell, f = 96, math.radians(15 / 3600), np.array([1.2e9, 1.3e9])
+    nrow = 200
+    uvw = _disc(nrow, N, cell, f[1], rng, 0.5)
+    W = (rng.normal(size=(nrow, 2, 1)) + 1j * rng.normal(size=(nrow, 2, 1))).astype(np.complex64)
+    g = O.OracleGridder("SphFunc", "f64", sph_oversample=8)
+    g.initialise_grid((N, N, 1, 1), (cell, cell))
+    g.grid(O.Chunk(uvw, f, W))
+    G = g.grid_[0, 0].copy()
+    M = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    d = O.OracleGridder("SphFunc", "f64", sph_oversample=8)
+    d.initialise_degrid((N, N, 1, 1), (cell, cell), np.ones((1, 1, N, N)))
+    d.grid_[0, 0] = M
+    D = d.degrid(O.Chunk(uvw, f, np.zeros_like(W)))
+    lhs = np.sum(D * np.conj(W))
+    rhs = np.sum(G * np.conj(M))
+    assert abs(lhs - rhs) < 1e-9 * abs(rhs)
+
+
+def test_degrid_offgrid_sample_reference_adds_stale_value():
+    # TableVisGridder.cc:763-861: an off-grid sample leaves itsImagePolFrameVis unchanged and
+    # the stale value of the previous on-grid sample is still added to the visibility.
+    N, cell, f = 64, math.radians(30 / 3600), np.array([1.0e9])
+    lam = O.C_LIGHT / f[0]
+    du = 1 / (cell * N)
+    uvw = np.array([[3.1 * du * lam, 2.2 * du * lam, 0], [40.0 * du * lam, 0.0, 0]])
+    model = np.zeros((1, 1, N, N))
+    model[0, 0, N // 2, N // 2] = 1.0
+    d = O.OracleGridder("SphFunc", "f64", sph_oversample=4)
+    d.initialise_degrid((N, N, 1, 1), (cell, cell), model)
+    v = d.degrid(O.Chunk(uvw, f, np.zeros((2, 1, 1), np.complex64)))
+    assert abs(v[0, 0, 0] - 1.0) < 1e-9 and abs(v[1, 0, 0] - v[0, 0, 0]) < 1e-12

@@ -1,0 +1,268 @@
+"""ctypes binding of libygrid's C ABI (include/ygrid.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+YGRID_MEM_HOST = 0
+YGRID_MEM_DEVICE = 1
+
+EXPORTED = [
+    "ygrid_last_error", "ygrid_version", "ygrid_ctx_create", "ygrid_ctx_destroy", "ygrid_ctx_set_stream",
+    "ygrid_ctx_synchronize", "ygrid_gridder_create", "ygrid_gridder_clone", "ygrid_gridder_destroy",
+    "ygrid_initialise_grid", "ygrid_grid", "ygrid_finalise_grid", "ygrid_finalise_weights",
+    "ygrid_initialise_degrid", "ygrid_degrid", "ygrid_finalise_degrid", "ygrid_is_model_empty",
+    "ygrid_set_mfs_weight", "ygrid_get_stats", "ygrid_set_profiling", "ygrid_get_cf", "ygrid_get_grid",
+]
+
+
+class YgridError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ygrid error {code}: {msg}")
+        self.code = code
+
+
+class _ImageDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("npol", ctypes.c_int), ("nchan", ctypes.c_int),
+                ("cellx", ctypes.c_double), ("celly", ctypes.c_double), ("has_freq_axis", ctypes.c_int),
+                ("freq_start", ctypes.c_double), ("freq_end", ctypes.c_double)]
+
+
+class _Chunk(ctypes.Structure):
+    _fields_ = [("nrow", ctypes.c_int), ("nchan", ctypes.c_int), ("npol", ctypes.c_int), ("mem", ctypes.c_int),
+                ("uvw", ctypes.c_void_p), ("delay", ctypes.c_void_p), ("freq", ctypes.c_void_p),
+                ("vis", ctypes.c_void_p), ("vis_out", ctypes.c_void_p), ("noise", ctypes.c_void_p),
+                ("flag", ctypes.c_void_p), ("feed1", ctypes.c_void_p), ("pointing1", ctypes.c_void_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("samples_gridded", ctypes.c_double), ("taps_gridded", ctypes.c_double),
+                ("samples_degridded", ctypes.c_double), ("taps_degridded", ctypes.c_double),
+                ("vectors_flagged", ctypes.c_double), ("vectors_wflagged", ctypes.c_double),
+                ("time_grid_s", ctypes.c_double), ("time_degrid_s", ctypes.c_double), ("time_cf_s", ctypes.c_double),
+                ("support", ctypes.c_int), ("oversample", ctypes.c_int), ("n_cf", ctypes.c_int),
+                ("ms_prep", ctypes.c_double), ("ms_bin", ctypes.c_double), ("ms_grid_kernel", ctypes.c_double),
+                ("ms_degrid_kernel", ctypes.c_double), ("n_grid_launches", ctypes.c_double),
+                ("n_degrid_launches", ctypes.c_double)]
+
+
+def library_path() -> str:
+    return os.environ.get("YGRID_LIBRARY", os.path.join(_HERE, "libygrid.so"))
+
+
+_LIB = None
+
+
+def lib():
+    """Load libygrid.so; raises if it has not been built (no fallback exists)."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `python -m yandasoft_amd.build`")
+        L = ctypes.CDLL(path)
+        L.ygrid_last_error.restype = ctypes.c_char_p
+        for name in EXPORTED:
+            getattr(L, name)  # every declared entry point must be exported
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise YgridError(rc, lib().ygrid_last_error().decode())
+
+
+@dataclass
+class ImageDesc:
+    nx: int
+    ny: int
+    npol: int = 1
+    nchan: int = 1
+    cellx: float = 0.0
+    celly: float = 0.0
+    freq_axis: tuple | None = None  # (start, end) Hz
+
+    def c(self) -> _ImageDesc:
+        fa = self.freq_axis
+        return _ImageDesc(self.nx, self.ny, self.npol, self.nchan, self.cellx, self.celly, 1 if fa else 0,
+                          fa[0] if fa else 0.0, fa[1] if fa else 0.0)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())  # torch tensor
+
+
+@dataclass
+class Chunk:
+    """Accessor chunk.  Arrays are in casacore memory order: uvw (nrow, 3) float64;
+    cubes (npol, nchan, nrow) complex64 in numpy C order (= casacore Cube(nRow,nChan,nPol)).
+    Either all numpy (host) or all torch cuda tensors (device)."""
+    uvw: object
+    freq: object
+    vis: object = None
+    vis_out: object = None
+    noise: object = None
+    flag: object = None
+    delay: object = None
+    feed1: object = None
+    nrow: int = 0
+    nchan: int = 0
+    npol: int = 0
+    device: bool = False
+
+    def __post_init__(self):
+        self.nrow = int(self.uvw.shape[0])
+        self.nchan = int(self.freq.shape[0])
+        cube = self.vis if self.vis is not None else (self.vis_out if self.vis_out is not None else self.noise)
+        self.npol = int(cube.shape[0]) if cube is not None else self.npol
+        if not self.device:
+            for name in ("uvw", "freq", "vis", "vis_out", "noise", "flag", "delay", "feed1"):
+                a = getattr(self, name)
+                if a is not None:
+                    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], f"{name} must be C-contiguous"
+
+    def c(self) -> _Chunk:
+        return _Chunk(self.nrow, self.nchan, self.npol, YGRID_MEM_DEVICE if self.device else YGRID_MEM_HOST,
+                      _ptr(self.uvw), _ptr(self.delay), _ptr(self.freq), _ptr(self.vis), _ptr(self.vis_out),
+                      _ptr(self.noise), _ptr(self.flag), _ptr(self.feed1), None)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = ctypes.c_void_p()
+        _check(lib().ygrid_ctx_create(device, ctypes.byref(self.h)))
+
+    def set_stream(self, stream_ptr: int):
+        _check(lib().ygrid_ctx_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
+    def synchronize(self):
+        _check(lib().ygrid_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            _check(lib().ygrid_ctx_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def parset_text(params: dict | None) -> bytes:
+    if not params:
+        return b""
+    return "\n".join(f"{k}={str(v).lower() if isinstance(v, bool) else v}" for k, v in params.items()).encode()
+
+
+class Gridder:
+    """IVisGridder mirror over the C ABI."""
+
+    def __init__(self, ctx: Context, name: str, params: dict | None = None, _handle=None):
+        self.ctx = ctx
+        if _handle is not None:
+            self.h = _handle
+        else:
+            self.h = ctypes.c_void_p()
+            _check(lib().ygrid_gridder_create(ctx.h, name.encode(), parset_text(params), ctypes.byref(self.h)))
+        self.img = None
+
+    def clone(self) -> "Gridder":
+        h = ctypes.c_void_p()
+        _check(lib().ygrid_gridder_clone(self.h, ctypes.byref(h)))
+        g = Gridder(self.ctx, "", _handle=h)
+        g.img = self.img
+        return g
+
+    def initialise_grid(self, img: ImageDesc, dopsf: bool = False, dopcf: bool = False):
+        self.img = img
+        d = img.c()
+        _check(lib().ygrid_initialise_grid(self.h, ctypes.byref(d), int(dopsf), int(dopcf)))
+
+    def grid(self, chunk: Chunk):
+        c = chunk.c()
+        _check(lib().ygrid_grid(self.h, ctypes.byref(c)))
+
+    def finalise_grid(self, out=None):
+        """Returns the image as numpy (nchan, npol, ny, nx) unless a device tensor is given."""
+        img = self.img
+        if out is None:
+            out = np.zeros((img.nchan, img.npol, img.ny, img.nx), dtype=np.float32)
+            _check(lib().ygrid_finalise_grid(self.h, ctypes.c_void_p(out.ctypes.data), YGRID_MEM_HOST))
+        else:
+            _check(lib().ygrid_finalise_grid(self.h, ctypes.c_void_p(_ptr(out)), YGRID_MEM_DEVICE))
+        return out
+
+    def finalise_weights(self):
+        img = self.img
+        out = np.zeros((img.nchan, img.npol, img.ny, img.nx), dtype=np.float32)
+        _check(lib().ygrid_finalise_weights(self.h, ctypes.c_void_p(out.ctypes.data), YGRID_MEM_HOST))
+        return out
+
+    def initialise_degrid(self, img: ImageDesc, model):
+        self.img = img
+        d = img.c()
+        if isinstance(model, np.ndarray):
+            m = np.ascontiguousarray(model, dtype=np.float32)
+            _check(lib().ygrid_initialise_degrid(self.h, ctypes.byref(d), ctypes.c_void_p(m.ctypes.data), YGRID_MEM_HOST))
+        else:
+            _check(lib().ygrid_initialise_degrid(self.h, ctypes.byref(d), ctypes.c_void_p(_ptr(model)), YGRID_MEM_DEVICE))
+
+    def degrid(self, chunk: Chunk):
+        c = chunk.c()
+        _check(lib().ygrid_degrid(self.h, ctypes.byref(c)))
+
+    def finalise_degrid(self):
+        _check(lib().ygrid_finalise_degrid(self.h))
+
+    def is_model_empty(self) -> bool:
+        v = ctypes.c_int()
+        _check(lib().ygrid_is_model_empty(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def set_mfs_weight(self, ref_freq: float, order: int):
+        _check(lib().ygrid_set_mfs_weight(self.h, ctypes.c_double(ref_freq), order))
+
+    def set_profiling(self, on: bool = True):
+        _check(lib().ygrid_set_profiling(self.h, int(on)))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(lib().ygrid_get_stats(self.h, ctypes.byref(s)))
+        return s
+
+    def cf_plane(self, k: int) -> np.ndarray:
+        S = ctypes.c_int()
+        _check(lib().ygrid_get_cf(self.h, k, None, ctypes.byref(S)))
+        W = 2 * S.value + 1
+        out = np.zeros((W, W), dtype=np.complex64)
+        _check(lib().ygrid_get_cf(self.h, k, ctypes.c_void_p(out.ctypes.data), ctypes.byref(S)))
+        return out
+
+    def grid_array(self) -> np.ndarray:
+        n = ctypes.c_size_t()
+        _check(lib().ygrid_get_grid(self.h, None, ctypes.byref(n)))
+        out = np.zeros(n.value // 2, dtype=np.complex64)
+        _check(lib().ygrid_get_grid(self.h, ctypes.c_void_p(out.ctypes.data), ctypes.byref(n)))
+        return out
+
+    def close(self):
+        if self.h:
+            _check(lib().ygrid_gridder_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,130 @@
+// C++ host side of libygrid: a casacore-free mirror of askap::synthesis::TableVisGridder
+// (askap/gridding/TableVisGridder.h) and its Box / SphFunc / WProject / AWProject subclasses,
+// driving the HIP kernels.  The C ABI in ygrid_api.cpp wraps these classes.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ygrid.h"
+#include "ygrid_internal.h"
+
+namespace ygrid {
+
+using ParamMap = std::map<std::string, std::string>;
+ParamMap parse_parset(const char* text);
+
+struct Params {
+  const ParamMap& m;
+  bool has(const std::string& k) const { return m.count(k) != 0; }
+  std::string str(const std::string& k, const std::string& d) const;
+  int i32(const std::string& k, int d) const;
+  double f64(const std::string& k, double d) const;
+  bool flag(const std::string& k, bool d) const;
+};
+
+// Device-resident CF cache (TableVisGridder::itsConvFunc, TableVisGridder.h:356), shared by
+// clones once generated (WProjectVisGridder's shared cache, WProjectVisGridder.cc:51-52).
+struct CFTable {
+  int oversample = 1;
+  int nz = 0;            // planes before oversampling
+  int support = 0;       // largest support
+  std::vector<int32_t> off, supp;      // per CF plane (nz*os*os)
+  std::vector<int32_t> ofs_u, ofs_v;   // per z plane (empty: no offsets)
+  DevBuf table, d_off, d_supp, d_ofs_u, d_ofs_v;
+  size_t n_elems = 0;
+  void upload(const Ctx& c);
+};
+
+// Device copies of one accessor chunk (staged from host when needed).
+struct ChunkDev {
+  int nrow = 0, nchan = 0, npol = 0;
+  const double* uvw = nullptr;
+  const double* delay = nullptr;
+  const double* freq = nullptr;
+  const float2* vis = nullptr;
+  float2* vis_out = nullptr;
+  const float2* noise = nullptr;
+  const uint8_t* flag = nullptr;
+  const int32_t* feed1 = nullptr;
+  std::vector<double> freq_host;
+  int feed0 = 0;
+};
+
+class Gridder {
+ public:
+  explicit Gridder(std::shared_ptr<Ctx> ctx) : ctx_(std::move(ctx)) {}
+  virtual ~Gridder();
+  virtual std::unique_ptr<Gridder> clone() const = 0;
+  virtual std::string name() const = 0;
+
+  void configure_common(const Params& p);
+  void initialiseGrid(const ygrid_image_desc& img, bool dopsf, bool dopcf);
+  void grid(const ygrid_chunk& acc);
+  void finaliseGrid(float* out, int mem);
+  void finaliseWeights(float* out, int mem);
+  void initialiseDegrid(const ygrid_image_desc& img, const float* model, int mem);
+  void degrid(const ygrid_chunk& acc);
+  void finaliseDegrid();
+  bool isModelEmpty() const { return model_empty_; }
+  void setMFSWeight(double ref, int order) { vw_ref_ = ref; vw_order_ = order; }
+  void setProfiling(bool on);
+  void stats(ygrid_stats* out) const;
+  void getCF(int k, float* out, int* support) const;
+  void getGrid(float* out, size_t* n) const;
+
+ protected:
+  Gridder(const Gridder& o);
+  // TableVisGridder::initConvolutionFunction: build cf_ if not built yet.
+  virtual void initConvolutionFunction(const ChunkDev& acc) = 0;
+  // TableVisGridder::cIndex as a device-side descriptor.
+  virtual CIndexDesc cindex(const ChunkDev& acc) const { (void)acc; return CIndexDesc{}; }
+  // number of CF planes before oversampling (rows of the sum of weights)
+  virtual int nZPlanes() const { return 1; }
+  // itsOverSample as configured (the CF cache may not exist yet)
+  virtual int oversampling() const { return 1; }
+  // 1-D factors of correctConvolution (empty: no correction, e.g. Box)
+  virtual void gridCorrection(std::vector<double>& cx, std::vector<double>& cy) const;
+  virtual bool doesCorrection() const { return true; }
+
+  void initialiseCommon(const ygrid_image_desc& img);
+  void generic(const ygrid_chunk& acc, bool forward);
+  void stage(const ygrid_chunk& acc, ChunkDev& d, bool forward);
+  void uploadCorrection();
+
+  std::shared_ptr<Ctx> ctx_;
+  std::shared_ptr<CFTable> cf_;
+  int shape_[4] = {0, 0, 1, 1};  // padded nx, ny, npol, nchan (itsShape)
+  int img_[4] = {0, 0, 1, 1};    // unpadded image
+  double uvcell_[2] = {0, 0};    // itsUVCellSize
+  float padding_ = 1.f;
+  bool dopsf_ = false, dopcf_ = false;
+  bool model_empty_ = false;
+  bool track_per_os_ = false, use_all_psf_ = false, clear_grid_ = false;
+  bool first_psf_ = true;
+  int psf_feed_ = -1;
+  bool has_freq_axis_ = false;
+  double fstart_ = 0, fend_ = 0;
+  double vw_ref_ = 0;
+  int vw_order_ = -1;
+  int nsw_rows_ = 1;
+  int alpha_int_ = 1;
+  bool initialised_ = false, degrid_mode_ = false;
+
+  DevBuf grid_, sumw_, scratch_, ccfx_, ccfy_;
+  DevBuf recs_, acc_, counters_, err_, chanmap_, visweight_, staging_;
+  Binning bin_;
+  PlanPtr plan_;
+  double st_samples_g_ = 0, st_taps_g_ = 0, st_samples_d_ = 0, st_taps_d_ = 0, st_flag_ = 0, st_wflag_ = 0;
+  double t_grid_ = 0, t_degrid_ = 0, t_cf_ = 0;
+  // profiling events: 0 start, 1 after prep, 2 after binning, 3 after the main kernel
+  bool profiling_ = false;
+  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  double ms_prep_ = 0, ms_bin_ = 0, ms_gk_ = 0, ms_dk_ = 0, n_gk_ = 0, n_dk_ = 0;
+};
+
+std::unique_ptr<Gridder> make_gridder(std::shared_ptr<Ctx> ctx, const std::string& name, const ParamMap& parset);
+
+}  // namespace ygrid

@@ -1,0 +1,166 @@
+// Image <-> grid transforms on the device.
+//
+// The reference transforms with a centred FFT (the origin sits at element n/2 on each axis;
+// forward exp(-i) unnormalised, inverse normalised by 1/n: scimath::fft2d as used at
+// TableVisGridder.cc:1215,1312).  For even n a centred transform is the plain transform
+// between two checkerboard multiplications by (-1)^(x+y), with an extra (-1)^(nx/2+ny/2) on
+// the output; the checkerboards are fused into the kernels that read or write the buffers.
+// finaliseGrid's 1/(nx*ny) of the inverse transform and its later *(nx*ny)
+// (TableVisGridder.cc:1225-1226) cancel; both are dropped.
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include "ygrid_internal.h"
+
+namespace ygrid {
+
+struct FFTPlan {
+  hipfftHandle h = 0;
+  int nx = 0, ny = 0, batch = 0;
+};
+
+void destroy_plan(FFTPlan* p) {
+  if (p) {
+    if (p->h) (void)hipfftDestroy(p->h);
+    delete p;
+  }
+}
+
+namespace {
+
+__device__ __forceinline__ float cb_sign(int x, int y) { return ((x + y) & 1) ? -1.f : 1.f; }
+
+__global__ void k_checkerboard(float2* data, int nx, int ny, float gsign) {
+  const size_t plane = (size_t)nx * ny;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  float2* p = data + blockIdx.z * plane + (size_t)y * nx + x;
+  const float s = cb_sign(x, y) * gsign;
+  float2 v = *p;
+  v.x *= s;
+  v.y *= s;
+  *p = v;
+}
+
+// out[plane][y][x] = sign * Re(fftd[plane][y+oy][x+ox]) * ccfx[x+ox] * ccfy[y+oy]
+__global__ void k_finalise_image(const float2* __restrict__ fftd, int nxp, int nyp, const double* __restrict__ ccfx,
+                                 const double* __restrict__ ccfy, int nx, int ny, int ox, int oy, float gsign,
+                                 float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  const int X = x + ox, Y = y + oy;
+  const float2 v = fftd[(size_t)blockIdx.z * nxp * nyp + (size_t)Y * nxp + X];
+  const double c = (ccfx ? ccfx[X] : 1.0) * (ccfy ? ccfy[Y] : 1.0);
+  out[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] = (float)((double)(cb_sign(X, Y) * gsign * v.x) * c);
+}
+
+// grid[plane][Y][X] = checkerboard * ccf * padded(model)
+__global__ void k_model_to_grid(const float* __restrict__ model, int nx, int ny, int ox, int oy,
+                                const double* __restrict__ ccfx, const double* __restrict__ ccfy, int nxp, int nyp,
+                                float2* __restrict__ grid) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y;
+  if (X >= nxp) return;
+  const int x = X - ox, y = Y - oy;
+  float val = 0.f;
+  if (x >= 0 && x < nx && y >= 0 && y < ny) {
+    const double c = (ccfx ? ccfx[X] : 1.0) * (ccfy ? ccfy[Y] : 1.0);
+    val = (float)((double)model[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] * c) * cb_sign(X, Y);
+  }
+  grid[(size_t)blockIdx.z * nxp * nyp + (size_t)Y * nxp + X] = make_float2(val, 0.f);
+}
+
+}  // namespace
+
+static float global_sign(int nx, int ny) { return (((nx / 2) + (ny / 2)) & 1) ? -1.f : 1.f; }
+
+void launch_checkerboard(const Ctx& c, float2* data, int nx, int ny, int batch) {
+  hipLaunchKernelGGL(k_checkerboard, dim3((nx + 255) / 256, ny, batch), dim3(256), 0, c.stream, data, nx, ny, 1.f);
+  YG_HIP(hipGetLastError());
+}
+
+static void ensure_plan(const Ctx& c, int nx, int ny, int batch, PlanPtr& cache) {
+  if (cache && cache->nx == nx && cache->ny == ny && cache->batch == batch) {
+    YG_CHECK(hipfftSetStream(cache->h, c.stream) == HIPFFT_SUCCESS, 3, "hipfftSetStream failed");
+    return;
+  }
+  if (cache) {
+    cache.reset();
+  }
+  auto* p = new FFTPlan();
+  int n[2] = {ny, nx};
+  hipfftResult r = hipfftPlanMany(&p->h, 2, n, nullptr, 1, nx * ny, nullptr, 1, nx * ny, HIPFFT_C2C, batch);
+  if (r != HIPFFT_SUCCESS) {
+    delete p;
+    throw Error(3, "hipfftPlanMany failed (" + std::to_string((int)r) + ")");
+  }
+  p->nx = nx;
+  p->ny = ny;
+  p->batch = batch;
+  cache.reset(p);
+  YG_CHECK(hipfftSetStream(p->h, c.stream) == HIPFFT_SUCCESS, 3, "hipfftSetStream failed");
+}
+
+// In-place centred transform of `batch` planes (input and output both centred).
+void fft_centred_c2c(const Ctx& c, float2* data, int nx, int ny, int batch, bool forward,
+                     PlanPtr& cache) {
+  YG_CHECK(nx % 2 == 0 && ny % 2 == 0, 1, "centred FFT needs even sizes");
+  ensure_plan(c, nx, ny, batch, cache);
+  hipLaunchKernelGGL(k_checkerboard, dim3((nx + 255) / 256, ny, batch), dim3(256), 0, c.stream, data, nx, ny, 1.f);
+  hipfftResult r = hipfftExecC2C(cache->h, reinterpret_cast<hipfftComplex*>(data), reinterpret_cast<hipfftComplex*>(data),
+                                 forward ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
+  YG_CHECK(r == HIPFFT_SUCCESS, 3, "hipfftExecC2C failed");
+  hipLaunchKernelGGL(k_checkerboard, dim3((nx + 255) / 256, ny, batch), dim3(256), 0, c.stream, data, nx, ny,
+                     global_sign(nx, ny));
+  YG_HIP(hipGetLastError());
+}
+
+// fftd holds the un-centred result of a backward transform of a checkerboarded grid.
+void launch_finalise_image(const Ctx& c, const float2* fftd, int nxp, int nyp, int nplanes, const double* ccfx,
+                           const double* ccfy, int nx, int ny, float* out) {
+  const int ox = (nxp - nx) / 2, oy = (nyp - ny) / 2;
+  hipLaunchKernelGGL(k_finalise_image, dim3((nx + 255) / 256, ny, nplanes), dim3(256), 0, c.stream, fftd, nxp, nyp,
+                     ccfx, ccfy, nx, ny, ox, oy, global_sign(nxp, nyp), out);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_model_to_grid(const Ctx& c, const float* model, int nx, int ny, int nplanes, const double* ccfx,
+                          const double* ccfy, int nxp, int nyp, float2* grid) {
+  const int ox = (nxp - nx) / 2, oy = (nyp - ny) / 2;
+  hipLaunchKernelGGL(k_model_to_grid, dim3((nxp + 255) / 256, nyp, nplanes), dim3(256), 0, c.stream, model, nx, ny, ox,
+                     oy, ccfx, ccfy, nxp, nyp, grid);
+  YG_HIP(hipGetLastError());
+}
+
+// Plain (un-centred) transform used after the callers have applied the input checkerboard.
+void fft_plain_c2c(const Ctx& c, float2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache) {
+  ensure_plan(c, nx, ny, batch, cache);
+  hipfftResult r = hipfftExecC2C(cache->h, reinterpret_cast<hipfftComplex*>(data), reinterpret_cast<hipfftComplex*>(data),
+                                 forward ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
+  YG_CHECK(r == HIPFFT_SUCCESS, 3, "hipfftExecC2C failed");
+}
+
+void launch_checkerboard_sign(const Ctx& c, float2* data, int nx, int ny, int batch) {
+  hipLaunchKernelGGL(k_checkerboard, dim3((nx + 255) / 256, ny, batch), dim3(256), 0, c.stream, data, nx, ny,
+                     global_sign(nx, ny));
+  YG_HIP(hipGetLastError());
+}
+
+}  // namespace ygrid
+
+namespace ygrid {
+namespace {
+__global__ void k_any_nonzero(const float* __restrict__ d, size_t n, int* flag) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    if (d[i] != 0.f) { *flag = 1; return; }
+}
+}  // namespace
+// isModelEmpty test of TableVisGridder::initialiseDegrid (TableVisGridder.cc:1304)
+void launch_any_nonzero(const Ctx& c, const float* data, size_t n, int* flag) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_any_nonzero, dim3(1024), dim3(256), 0, c.stream, data, n, flag);
+  YG_HIP(hipGetLastError());
+}
+}  // namespace ygrid

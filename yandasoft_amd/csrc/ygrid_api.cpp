@@ -1,0 +1,220 @@
+// extern "C" boundary of libygrid (declared in include/ygrid.h).  Every entry point catches
+// C++ exceptions and turns them into a status code plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "gridders.h"
+
+struct ygrid_ctx {
+  std::shared_ptr<ygrid::Ctx> c;
+};
+struct ygrid_gridder {
+  std::unique_ptr<ygrid::Gridder> g;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return YGRID_OK;
+  } catch (const ygrid::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return YGRID_ERR_ARG;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return YGRID_ERR_ARG;
+  }
+}
+
+#define NEED(p) YG_CHECK((p) != nullptr, YGRID_ERR_ARG, #p " is null")
+}  // namespace
+
+extern "C" {
+
+const char* ygrid_last_error(void) { return g_last_error.c_str(); }
+int ygrid_version(void) { return YGRID_VERSION; }
+
+int ygrid_ctx_create(int device, ygrid_ctx** out) {
+  return guard([&] {
+    NEED(out);
+    int n = 0;
+    YG_HIP(hipGetDeviceCount(&n));
+    YG_CHECK(device >= 0 && device < n, YGRID_ERR_ARG, "device index out of range");
+    YG_HIP(hipSetDevice(device));
+    auto c = std::make_shared<ygrid::Ctx>();
+    c->device = device;
+    hipDeviceProp_t prop;
+    YG_HIP(hipGetDeviceProperties(&prop, device));
+    c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    YG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+    *out = new ygrid_ctx{c};
+  });
+}
+
+int ygrid_ctx_destroy(ygrid_ctx* ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    if (ctx->c.use_count() == 1 && ctx->c->own_stream && ctx->c->stream) {
+      (void)hipStreamSynchronize(ctx->c->stream);
+      (void)hipStreamDestroy(ctx->c->stream);
+      ctx->c->stream = nullptr;
+    }
+    delete ctx;
+  });
+}
+
+int ygrid_ctx_set_stream(ygrid_ctx* ctx, void* stream) {
+  return guard([&] {
+    NEED(ctx);
+    if (ctx->c->own_stream && ctx->c->stream) {
+      YG_HIP(hipStreamSynchronize(ctx->c->stream));
+      YG_HIP(hipStreamDestroy(ctx->c->stream));
+    }
+    ctx->c->stream = static_cast<hipStream_t>(stream);
+    ctx->c->own_stream = false;
+  });
+}
+
+int ygrid_ctx_synchronize(ygrid_ctx* ctx) {
+  return guard([&] {
+    NEED(ctx);
+    YG_HIP(hipStreamSynchronize(ctx->c->stream));
+  });
+}
+
+int ygrid_gridder_create(ygrid_ctx* ctx, const char* name, const char* parset, ygrid_gridder** out) {
+  return guard([&] {
+    NEED(ctx);
+    NEED(name);
+    NEED(out);
+    YG_HIP(hipSetDevice(ctx->c->device));
+    auto g = ygrid::make_gridder(ctx->c, name, ygrid::parse_parset(parset));
+    *out = new ygrid_gridder{std::move(g)};
+  });
+}
+
+int ygrid_gridder_clone(const ygrid_gridder* g, ygrid_gridder** out) {
+  return guard([&] {
+    NEED(g);
+    NEED(out);
+    *out = new ygrid_gridder{g->g->clone()};
+  });
+}
+
+int ygrid_gridder_destroy(ygrid_gridder* g) {
+  return guard([&] { delete g; });
+}
+
+int ygrid_initialise_grid(ygrid_gridder* g, const ygrid_image_desc* img, int dopsf, int dopcf) {
+  return guard([&] {
+    NEED(g);
+    NEED(img);
+    g->g->initialiseGrid(*img, dopsf != 0, dopcf != 0);
+  });
+}
+
+int ygrid_grid(ygrid_gridder* g, const ygrid_chunk* acc) {
+  return guard([&] {
+    NEED(g);
+    NEED(acc);
+    g->g->grid(*acc);
+  });
+}
+
+int ygrid_finalise_grid(ygrid_gridder* g, float* out, int mem) {
+  return guard([&] {
+    NEED(g);
+    NEED(out);
+    g->g->finaliseGrid(out, mem);
+  });
+}
+
+int ygrid_finalise_weights(ygrid_gridder* g, float* out, int mem) {
+  return guard([&] {
+    NEED(g);
+    NEED(out);
+    g->g->finaliseWeights(out, mem);
+  });
+}
+
+int ygrid_initialise_degrid(ygrid_gridder* g, const ygrid_image_desc* img, const float* model, int mem) {
+  return guard([&] {
+    NEED(g);
+    NEED(img);
+    NEED(model);
+    g->g->initialiseDegrid(*img, model, mem);
+  });
+}
+
+int ygrid_degrid(ygrid_gridder* g, const ygrid_chunk* acc) {
+  return guard([&] {
+    NEED(g);
+    NEED(acc);
+    g->g->degrid(*acc);
+  });
+}
+
+int ygrid_finalise_degrid(ygrid_gridder* g) {
+  return guard([&] {
+    NEED(g);
+    g->g->finaliseDegrid();
+  });
+}
+
+int ygrid_is_model_empty(const ygrid_gridder* g, int* out) {
+  return guard([&] {
+    NEED(g);
+    NEED(out);
+    *out = g->g->isModelEmpty() ? 1 : 0;
+  });
+}
+
+int ygrid_set_mfs_weight(ygrid_gridder* g, double ref_freq, int order) {
+  return guard([&] {
+    NEED(g);
+    YG_CHECK(order < 0 || ref_freq > 0, YGRID_ERR_ARG, "reference frequency must be positive");
+    g->g->setMFSWeight(ref_freq, order);
+  });
+}
+
+int ygrid_get_stats(const ygrid_gridder* g, ygrid_stats* out) {
+  return guard([&] {
+    NEED(g);
+    NEED(out);
+    std::memset(out, 0, sizeof(*out));
+    g->g->stats(out);
+  });
+}
+
+int ygrid_set_profiling(ygrid_gridder* g, int enable) {
+  return guard([&] {
+    NEED(g);
+    g->g->setProfiling(enable != 0);
+  });
+}
+
+int ygrid_get_cf(const ygrid_gridder* g, int k, float* out, int* support) {
+  return guard([&] {
+    NEED(g);
+    g->g->getCF(k, out, support);
+  });
+}
+
+int ygrid_get_grid(const ygrid_gridder* g, float* out, size_t* n) {
+  return guard([&] {
+    NEED(g);
+    g->g->getGrid(out, n);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+// Internal declarations shared by the host code and the HIP kernels of libygrid.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ygrid {
+
+constexpr double kC = 299792458.0;  // casacore::C::c
+constexpr double kPi = 3.14159265358979323846;
+
+// Grid tile owned by one workgroup in the grid/degrid kernels.
+constexpr int kTile = 64;
+// Guard band of zeros around every CF plane in device memory so that the 16x4-lane
+// blocks of the grid/degrid kernels may read past the (2S+1)^2 footprint without a
+// per-lane bounds test: columns -kGuardU..2S+kGuardU-1, rows -kGuardV..2S+kGuardV-1.
+constexpr int kGuardU = 16;
+constexpr int kGuardV = 4;
+inline int cf_pitch(int support) { return 2 * support + 1 + 2 * kGuardU; }
+inline int cf_rows(int support) { return 2 * support + 1 + 2 * kGuardV; }
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define YG_CHECK(cond, code, msg)                                 \
+  do {                                                            \
+    if (!(cond)) throw ::ygrid::Error((code), std::string(msg));  \
+  } while (0)
+
+#define YG_HIP(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      throw ::ygrid::Error(3, std::string(#call) + " failed: " + hipGetErrorString(e_));          \
+  } while (0)
+
+// Owning device allocation.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { resize(bytes); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  void resize(size_t bytes) {
+    if (bytes <= n_) return;
+    release();
+    if (bytes == 0) return;
+    YG_HIP(hipMalloc(&p_, bytes));
+    n_ = bytes;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p_); }
+  void* get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// How the CF plane (before oversampling) of a sample is chosen: TableVisGridder::cIndex
+// overrides (TableVisGridder.cc:1410-1412; WProjectVisGridder.cc:114-147,286-292;
+// AWProjectVisGridder.cc:123-172,855-863).
+struct CIndexDesc {
+  int kind = 0;            // 0: constant 0; 1: w-plane; 2: w-plane + feed/chan (AW)
+  int nw = 1;              // number of w planes
+  double wscale = 1.0;     // linear sampling step (WDependentGridderBase.cc:60-65)
+  double wmax = 1.0;
+  int sampling = 0;        // 0 linear, 1 power law, 2 gaussian
+  double exponent = 1.0;   // power law
+  double g_ts = 1.0, g_amp = 1.0;  // gaussian: 2 sigma^2 and amplitude
+  int wmaxclip = 0;
+  int freqdep = 0;         // AW: channel-dependent CFs
+  int nchan_cf = 1;        // AW: channels in the CF cache
+  int maxfeeds = 1;        // AW
+  int field = 0;           // AW: current field
+};
+
+// What kind of pass the prep kernel prepares.
+enum PassMode : int { kModeGrid = 0, kModePSF = 1, kModePCF = 2, kModeDegrid = 3 };
+
+// Per (row, chan) record written by the prep kernel and read (wave-uniformly) by the
+// grid/degrid kernels.  iu0 == INT32_MIN marks a sample that is not gridded.
+struct alignas(16) SampleRec {
+  int32_t iu0, iv0;     // footprint origin: (iuOffset - S, ivOffset - S)
+  int32_t cfoff;        // element offset of the padded CF plane in the CF table
+  int32_t spack;        // support S (low 16 bits) | pitch << 16
+  float v[8];           // grid: per-pol complex value to spread; degrid: [0..1] phasor*weight
+  int32_t key;          // binning key: (ichan, z-group) plane index
+  int32_t conj;         // PCF: spread conj(CF)
+  int32_t pad0, pad1;
+};
+static_assert(sizeof(SampleRec) == 64, "SampleRec must be 64 bytes");
+
+struct PrepParams {
+  int nrow, nchan, npol;        // chunk shape (npol == image pols)
+  int nx, ny;                   // padded grid
+  int nchan_img;
+  double uvcell_u, uvcell_v;
+  int oversample;
+  int mode;                     // PassMode
+  int track_per_os;             // sum-of-weights per oversampled plane
+  int nsw_rows;
+  int psf_feed;                 // -1: all data
+  CIndexDesc ci;
+  int zdiv;                     // z-group divisor for binning keys
+  int nzg;                      // number of z groups
+};
+
+struct PrepOutputs {
+  SampleRec* recs;              // nrow*nchan
+  double* sumw;                 // nsw_rows * npol * nchan_img
+  unsigned long long* counters; // [0] samples, [1] taps, [2] flagged, [3] wflagged
+  int* error;                   // bit 1: zero noise weight; bit 2: w out of range
+};
+
+struct PrepInputs {
+  const double* uvw;
+  const double* delay;
+  const double* freq;
+  const float2* vis;
+  const float2* noise;
+  const uint8_t* flag;
+  const int32_t* feed1;
+  const int32_t* chanmap;      // nchan: image channel or -1
+  const float* visweight;      // nchan or null
+  const int32_t* cf_off;       // per CF plane (incl. oversampling)
+  const int32_t* cf_supp;
+  const int32_t* cf_ofs_u;     // per plane before oversampling, or null
+  const int32_t* cf_ofs_v;
+  int n_cf;
+};
+
+// Device work for one grid/degrid pass, produced by the binning step.
+struct Binning {
+  DevBuf counts, offsets, cursor, ioffsets, pairs, items, nitems, counter, temp;
+  size_t nkeys = 0, npairs_cap = 0;
+};
+
+// hipStream + device id of a context.
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int n_cu = 256;
+};
+
+// Kernel launchers (kernels_grid.hip)
+void launch_prep(const Ctx&, const PrepParams&, const PrepInputs&, const PrepOutputs&);
+// Bins valid records by (key, tile); returns nothing, work lands in b.
+void launch_binning(const Ctx&, const SampleRec* recs, int nsamples, int nx, int ny, int nkeys_z,
+                    Binning& b);
+void launch_grid(const Ctx&, const Binning& b, const SampleRec* recs, const float2* cf, float2* grid,
+                 int nx, int ny, int npol, int nzg);
+void launch_degrid(const Ctx&, const Binning& b, const SampleRec* recs, const float2* cf, const float2* grid,
+                   int nx, int ny, int npol, int nzg, float* acc);
+void launch_degrid_finish(const Ctx&, const SampleRec* recs, const float* acc, int nrow, int nchan, int npol,
+                          float2* vis_out);
+
+// FFT-domain helpers (kernels_fft.hip)
+struct FFTPlan;
+void destroy_plan(FFTPlan*);
+struct FFTPlanDeleter { void operator()(FFTPlan* p) const { destroy_plan(p); } };
+using PlanPtr = std::unique_ptr<FFTPlan, FFTPlanDeleter>;
+// In-place centred 2-D transform of `batch` planes of ny rows x nx columns.
+void fft_centred_c2c(const Ctx&, float2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache);
+// Plain transform (no centring) and the two checkerboard passes it is combined with.
+void fft_plain_c2c(const Ctx&, float2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache);
+void launch_checkerboard(const Ctx&, float2* data, int nx, int ny, int batch);
+void launch_checkerboard_sign(const Ctx&, float2* data, int nx, int ny, int batch);
+void launch_finalise_image(const Ctx&, const float2* fftd, int nxp, int nyp, int nplanes, const double* ccfx,
+                           const double* ccfy, int nx, int ny, float* out);
+void launch_model_to_grid(const Ctx&, const float* model, int nx, int ny, int nplanes, const double* ccfx,
+                          const double* ccfy, int nxp, int nyp, float2* grid);
+
+// CF generation kernels (kernels_cf.hip)
+// Fill planes [0,nb) of buf (nx*ny complex each) with taper x phase screen for w terms wt[]
+// (WProjectVisGridder::populateThisPlane, WProjectVisGridder.cc:632-660).
+void launch_wproj_screen(const Ctx&, float2* buf, int nx, int ny, int qnx, int qny, double ccellx, double ccelly,
+                         const float* ccfx, const float* ccfy, const double* two_pi_w, int nb);
+// Argmax of |c|^2 (first in memory order among equals) of one plane; result (flat index) in *out.
+void launch_argmax(const Ctx&, const float2* plane, int n, unsigned long long* out);
+// Cut out os^2 CF planes per FFT plane, normalise by the sum of real parts (mode 0: WProject),
+// or scale by `rescale` (mode 1: AWProject), and write them, zero-padded, into the table.
+void launch_cf_cutout(const Ctx&, const float2* buf, int nx, int ny, int nb, int os, const int* supp_of_plane,
+                      const int* offu_of_plane, const int* offv_of_plane, const int32_t* cf_off_first, int mode,
+                      double rescale, float2* table);
+// SphFunc CF: planes from 1-D factors bx[fracu][ix], by[fracv][iy], normalised by the sum
+// of |C| (SphFuncVisGridder.cc:137-167).
+void launch_sph_cf(const Ctx&, const double* bx, const double* by, int os, int support, const int32_t* cf_off,
+                   float2* table);
+
+// Host helpers (spheroidal.cpp)
+class Spheroidal {
+ public:
+  Spheroidal(double c, double alpha, int nterms = 24);
+  double operator()(double nu) const;
+ private:
+  int m_;
+  std::vector<double> d_;
+  double norm0_;
+  double series(double x) const;
+};
+// 1-D grid-correction factors of SphFuncVisGridder::correctConvolution (SphFuncVisGridder.cc:171-266).
+void sph_grid_correction_xy(int nx, int ny, double alpha, int support, std::vector<double>& cx,
+                            std::vector<double>& cy);
+
+}  // namespace ygrid

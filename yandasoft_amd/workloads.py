@@ -1,0 +1,108 @@
+"""Synthetic accessor chunks of the benchmark workloads (numpy, seeded).
+
+The 36-antenna array follows the fake accessor of askap/gridding/TestCFGenPerformance.cc:70-133
+(36 antennas, stub data); uvw use the textbook earth-rotation relation (as the simulator's,
+askap/simulation/Simulator.cc:932-963).
+
+* ``sphfunc_disc``   -- plain SphFunc gridding: single-pol visibilities with uv uniform in a
+                        disc, w = 0 (the tConvolve / tGridding style case).
+* ``wproject_array`` -- W-projection: 36 antennas uniform in a disc, earth rotation over the
+                        given hour-angle span at a declination, linear channels, 4 pols.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+C_LIGHT = 299792458.0
+ARCSEC = math.pi / 180.0 / 3600.0
+
+
+@dataclass
+class Workload:
+    name: str
+    uvw: np.ndarray          # (nrow, 3) float64 metres
+    freq: np.ndarray         # (nchan,) Hz
+    vis: np.ndarray          # (npol, nchan, nrow) complex64 (casacore cube order)
+    noise: np.ndarray        # same shape, complex64
+    nx: int
+    ny: int
+    cell: float              # radians
+    gridder: str
+    params: dict = field(default_factory=dict)
+
+    @property
+    def nrow(self):
+        return self.uvw.shape[0]
+
+    @property
+    def nchan(self):
+        return self.freq.shape[0]
+
+    @property
+    def npol(self):
+        return self.vis.shape[0]
+
+
+def _cn(rng, shape):
+    """Complex normal CN(0,1): real and imaginary parts N(0, 1/2)."""
+    s = math.sqrt(0.5)
+    return (rng.normal(0.0, s, shape) + 1j * rng.normal(0.0, s, shape)).astype(np.complex64)
+
+
+def sphfunc_disc(nvis: int = 160_000, radius_m: float = 2000.0, freq_hz: float = 1.4e9, n: int = 2048,
+                 cell_arcsec: float = 5.0, support: int = 3, oversample: int = 128, seed: int = 20261203) -> Workload:
+    rng = np.random.default_rng(seed)
+    r = radius_m * np.sqrt(rng.random(nvis))
+    th = 2.0 * math.pi * rng.random(nvis)
+    uvw = np.stack([r * np.cos(th), r * np.sin(th), np.zeros(nvis)], axis=1)
+    vis = _cn(rng, (1, 1, nvis))
+    noise = np.ones_like(vis)
+    return Workload("sphfunc_disc", uvw, np.array([freq_hz]), vis, noise, n, n, cell_arcsec * ARCSEC, "GPUSphFunc",
+                    {"support": support, "oversample": oversample, "alpha": 1.0})
+
+
+def array_layout(nant: int = 36, diameter_m: float = 6000.0, seed: int = 20261203) -> np.ndarray:
+    """East/north antenna positions uniform in a disc of the given diameter."""
+    rng = np.random.default_rng(seed)
+    r = 0.5 * diameter_m * np.sqrt(rng.random(nant))
+    th = 2.0 * math.pi * rng.random(nant)
+    return np.stack([r * np.cos(th), r * np.sin(th)], axis=1)
+
+
+def earth_rotation_uvw(enu: np.ndarray, nint: int, hours: float, dec_deg: float, lat_deg: float = -26.7) -> np.ndarray:
+    """uvw (metres) of every baseline (i<j) for nint integrations over the hour-angle span."""
+    lat = math.radians(lat_deg)
+    dec = math.radians(dec_deg)
+    e, nn = enu[:, 0], enu[:, 1]
+    # local (east, north, up=0) -> equatorial (X towards H=0, Y east, Z pole)
+    X = -math.sin(lat) * nn
+    Y = e
+    Z = math.cos(lat) * nn
+    i, j = np.triu_indices(enu.shape[0], k=1)
+    bx, by, bz = X[j] - X[i], Y[j] - Y[i], Z[j] - Z[i]
+    H = np.linspace(-hours / 2, hours / 2, nint) * (2 * math.pi / 24.0)
+    sH, cH = np.sin(H)[:, None], np.cos(H)[:, None]
+    u = sH * bx + cH * by
+    v = -math.sin(dec) * cH * bx + math.sin(dec) * sH * by + math.cos(dec) * bz
+    w = math.cos(dec) * cH * bx - math.cos(dec) * sH * by + math.sin(dec) * bz
+    return np.stack([u.ravel(), v.ravel(), w.ravel()], axis=1)
+
+
+def wproject_array(nint: int = 300, nchan: int = 64, npol: int = 4, fmin: float = 700e6, fmax: float = 1000e6,
+                   hours: float = 8.0, dec_deg: float = -45.0, n: int = 4096, cell_arcsec: float = 4.5,
+                   nwplanes: int = 129, oversample: int = 4, limitsupport: int = 64, maxsupport: int = 2048,
+                   cutoff: float = 1e-3, seed: int = 20261203, with_vis: bool = True) -> Workload:
+    enu = array_layout(seed=seed)
+    uvw = earth_rotation_uvw(enu, nint, hours, dec_deg)
+    freq = np.linspace(fmin, fmax, nchan) if nchan > 1 else np.array([fmax])
+    rng = np.random.default_rng(seed + 1)
+    nrow = uvw.shape[0]
+    vis = _cn(rng, (npol, nchan, nrow)) if with_vis else np.zeros((npol, nchan, nrow), np.complex64)
+    noise = np.ones((npol, nchan, nrow), dtype=np.complex64)
+    wmax = float(np.max(np.abs(uvw[:, 2]))) * fmax / C_LIGHT * 1.0001
+    params = {"wmax": wmax, "nwplanes": nwplanes, "oversample": oversample, "maxsupport": maxsupport,
+              "limitsupport": limitsupport, "cutoff": cutoff}
+    return Workload("wproject_array", uvw, freq, vis, noise, n, n, cell_arcsec * ARCSEC, "GPUWProject", params)
