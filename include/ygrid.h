@@ -53,6 +53,9 @@ typedef struct {
   double cellx, celly;  /* direction-axis increments (radians; sign is ignored) */
   int has_freq_axis;    /* FREQUENCY axis present: map accessor channels to image channels */
   double freq_start, freq_end;
+  /* image centre direction (TableVisGridder::getImageCentre, TableVisGridder.cc:919-934),
+   * longitude/latitude in radians; used by the A-projection beam offsets */
+  double centre_lon, centre_lat;
 } ygrid_image_desc;
 
 /* One accessor chunk (IConstDataAccessor / IDataAccessor getters used by
@@ -68,7 +71,7 @@ typedef struct {
   const float *noise;        /* noise cube (complex) or NULL (= 1) */
   const uint8_t *flag;       /* flag cube or NULL */
   const int32_t *feed1;      /* feed1: nrow or NULL (= 0) */
-  const double *pointing1;   /* pointingDir1 offsets from the image centre: nrow x 2 (rad) or NULL */
+  const double *pointing1;   /* pointingDir1: nrow x 2 (longitude, latitude; rad) or NULL (= image centre) */
 } ygrid_chunk;
 
 /* Counters reported by TableVisGridder's destructor (TableVisGridder.cc:182-317). */

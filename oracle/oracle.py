@@ -469,6 +469,136 @@ def wproject_cf(prm: WProjectParams, nx_grid: int, ny_grid: int, uvcell: tuple, 
 
 
 # --------------------------------------------------------------------------------------
+# AWProject
+# --------------------------------------------------------------------------------------
+@dataclass
+class AWProjectParams(WProjectParams):
+    """AProjectGridderBase::createAProjectGridder defaults (AProjectGridderBase.tcc:53-115)."""
+    wmax: float = 10000.0
+    nwplanes: int = 65
+    oversample: int = 8
+    maxsupport: int = 128
+    maxfeeds: int = 1
+    maxfields: int = 1
+    freqdep: bool = True
+    spheroidaltaper: bool = False
+    spheroidalweightscutoff: float = 0.0
+    diameter: float = 12.0
+    blockage: float = 0.0
+
+
+def direction_lm(beam, centre):
+    """l, m of a beam centre relative to the image centre (DiskIllumination.cc:111-116)."""
+    (bl, bb), (cl, cb) = beam, centre
+    l = math.sin(bl - cl) * math.cos(bb)
+    m = math.sin(bb) * math.cos(cb) - math.cos(bb) * math.sin(cb) * math.cos(bl - cl)
+    return l, m
+
+
+def disk_pattern(nu, nv, ucell, vcell, os_, freq, diameter, blockage, l=0.0, m=0.0):
+    """DiskIllumination::getPattern (DiskIllumination.cc:85-177): returns [iv, iu] complex."""
+    cellU, cellV = ucell / os_, vcell / os_
+    lS, mS = 2 * math.pi * cellU * l, 2 * math.pi * cellV * m
+    cell = abs(cellU * (C_LIGHT / freq))
+    rmax2 = (diameter / (2.0 * cell)) ** 2
+    rmin2 = (blockage / (2.0 * cell)) ** 2
+    if not (rmax2 > rmin2):
+        raise ValueError("Disk hole is supposed to be less than the disk size")
+    if not (nu * nu > rmax2 and nv * nv > rmax2):
+        raise ValueError("pattern buffer too small for the dish")
+    if rmin2 + 1 >= rmax2:
+        rmin2 -= 1
+    ou = np.arange(nu) - nu / 2.0
+    ov = np.arange(nv) - nv / 2.0
+    r2 = ov[:, None] ** 2 + ou[None, :] ** 2
+    inside = (r2 >= rmin2) & (r2 <= rmax2)
+    ph = lS * ou[None, :] + mS * ov[:, None]
+    pat = np.where(inside, np.cos(ph) - 1j * np.sin(ph), 0).astype(np.complex64)
+    s = float(np.count_nonzero(inside))
+    return pat * np.float32(np.float32(nu) * np.float32(nv) / np.float32(s))
+
+
+def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_of_feed, field=0, psf=False,
+                     state=None):
+    """AWProjectVisGridder::initConvolutionFunction for the feeds in ``lm_of_feed`` (dict feed ->
+    (l, m) beam offset) of one field.  ``state`` carries the CF cache and the common support
+    between calls.  Returns the state: {'planes': list, 'support': int}."""
+    os_ = prm.oversample
+    nx = ny = prm.maxsupport
+    qnx, qny = nx // os_, ny // os_
+    ccellx = 1.0 / (qnx * uvcell[0])
+    ccelly = 1.0 / (qny * uvcell[1])
+    nchan = len(freqs) if prm.freqdep else 1
+    nw = prm.nwplanes
+    if state is None:
+        state = {"planes": [None] * (os_ * os_ * nw * prm.maxfeeds * prm.maxfields * nchan), "support": 0,
+                 "nchan": nchan}
+    ws = prm.sampling()
+    if prm.spheroidaltaper:
+        sph = Spheroidal(math.pi * 3, prm.alpha)
+        tx = interpolate_edge(sph(np.abs(np.arange(qnx) - qnx // 2) / (qnx // 2)))
+        ty = interpolate_edge(sph(np.abs(np.arange(qny) - qny // 2) / (qny // 2)))
+    x = (np.arange(nx) - nx / 2.0) * ccellx
+    y = (np.arange(ny) - ny / 2.0) * ccelly
+    r2 = y[:, None] ** 2 + x[None, :] ** 2
+    qix = np.arange(nx) + qnx // 2 - nx // 2
+    qiy = np.arange(ny) + qny // 2 - ny // 2
+    region = r2 < 1.0
+    taper = np.ones((ny, nx))
+    if prm.spheroidaltaper:
+        inq = (qix >= 0) & (qix < qnx)
+        inqy = (qiy >= 0) & (qiy < qny)
+        region &= inqy[:, None] & inq[None, :]
+        taper = np.zeros((ny, nx))
+        taper[np.ix_(inqy, inq)] = np.outer(ty, tx)
+    for feed, (l, m) in lm_of_feed.items():
+        for chan in range(nchan):
+            pat = disk_pattern(nx, ny, uvcell[0], uvcell[1], os_, freqs[chan], prm.diameter, prm.blockage,
+                               0.0 if psf else l, 0.0 if psf else m)
+            A = fft_centred(pat.astype(np.complex128), False)
+            A = A.astype(np.complex64).astype(np.complex128)
+            wt = (A * np.conj(A)) * taper
+            for iw in range(nw):
+                w = 2.0 * math.pi * ws.wterm(iw)
+                phase = w * (1.0 - np.sqrt(np.where(region, 1.0 - r2, 1.0)))
+                buf = np.where(region, wt * (np.cos(phase) - 1j * np.sin(phase)), 0)
+                buf = buf.astype(np.complex64).astype(np.complex128)
+                buf = fft_centred(buf, True) / (nx * ny)
+                buf = buf.astype(np.complex64).astype(np.complex128)
+                z = iw + nw * (chan + nchan * (feed + prm.maxfeeds * field))
+                if state["support"] == 0:
+                    size, _, _ = _extract_support(buf.T, prm, os_)
+                    if not ((size + 1) * os_ < nx // 2):
+                        raise ValueError("Overflowing convolution function")
+                    if prm.limitsupport > 0 and size > prm.limitsupport:
+                        size = prm.limitsupport
+                    state["support"] = size
+                S = state["support"]
+                t = np.arange(-S, S + 1)
+                for fracu in range(os_):
+                    kx = t * os_ + fracu + nx // 2
+                    for fracv in range(os_):
+                        ky = t * os_ + fracv + ny // 2
+                        if kx.min() < 0 or ky.min() < 0 or kx.max() >= nx or ky.max() >= ny:
+                            raise ValueError("Convolution function overflowing")
+                        state["planes"][fracu + os_ * (fracv + os_ * z)] = float(os_ * os_) * buf[np.ix_(ky, kx)]
+    return state
+
+
+def fft_pad(a: np.ndarray, ny_out: int, nx_out: int) -> np.ndarray:
+    """Fourier interpolation of real [.., y, x] planes onto a larger grid (PaddingUtils::fftPad as
+    called at AWProjectVisGridder.cc:765; base-scimath is absent: parity unpinned)."""
+    ny_in, nx_in = a.shape[-2:]
+    if (ny_in, nx_in) == (ny_out, nx_out):
+        return a.copy()
+    F = fft_centred(a.astype(np.complex128), False)
+    out = np.zeros(a.shape[:-2] + (ny_out, nx_out), dtype=np.complex128)
+    oy, ox = (ny_out - ny_in) // 2, (nx_out - nx_in) // 2
+    out[..., oy:oy + ny_in, ox:ox + nx_in] = F
+    return fft_centred(out, True).real
+
+
+# --------------------------------------------------------------------------------------
 # Frequency mapping
 # --------------------------------------------------------------------------------------
 def frequency_map(freqs, nchan_img: int, fstart: float | None = None, fend: float | None = None):
@@ -544,6 +674,10 @@ class OracleGridder:
         self.cf = None
         self.support = None
         self.only_planes = None  # WProject: generate only these w planes (tests of large CF sets)
+        self.aw_lm = {}          # AWProject: feed -> (l, m) beam offset
+        self.aw_done = set()
+        self.aw_state = None
+        self.aw_nchan = None     # AWProject: channels of the CF cache (set before initialise_grid)
         self.counters = _Counters()
         self.model_empty = False
 
@@ -569,12 +703,31 @@ class OracleGridder:
         else:
             raise ValueError(self.kind)
 
+    def _init_aw(self, chunk: Chunk, psf: bool):
+        """AW: CFs for the feeds of this chunk that are not in the cache yet (one field)."""
+        feeds = chunk.feed1 if chunk.feed1 is not None else np.zeros(chunk.nrow, dtype=np.int32)
+        new = {}
+        for f in feeds:
+            f = int(f)
+            if f not in self.aw_done and f not in new:
+                new[f] = self.aw_lm.get(f, (0.0, 0.0))
+        if new or self.aw_state is None:
+            self.aw_state = awproject_planes(self.wp, self.nx, self.ny, self.uvcell, chunk.freq, new, 0, psf,
+                                             self.aw_state)
+            self.aw_done.update(new.keys())
+        self.support = self.aw_state["support"]
+        self.cf = CFCache(self.aw_state["planes"], self.wp.oversample)
+
     def n_cf_planes(self):
-        return self.wp.nwplanes if self.kind == "WProject" else 1
+        if self.kind == "WProject":
+            return self.wp.nwplanes
+        if self.kind == "AWProject":
+            return self.wp.nwplanes * self.wp.maxfeeds * self.wp.maxfields * (self.aw_nchan or 1)
+        return 1
 
     def correct(self, img):
         """correctConvolution on a padded [.., y, x] image."""
-        if self.kind == "Box":
+        if self.kind == "Box" or (self.kind == "AWProject" and not self.wp.spheroidaltaper):
             return img
         alpha = self.sph[0] if self.kind == "SphFunc" else self.wp.alpha
         support = self.sph[1] if self.kind == "SphFunc" else 3
@@ -613,20 +766,30 @@ class OracleGridder:
         nrow, nchan, npol = chunk.vis.shape
         if getattr(self, "cmap_override", None) is not None:
             return self.cmap_override
-        if self.kind != "WProject":
+        if self.kind not in ("WProject", "AWProject"):
             return None
         ws = self.wp.sampling()
         cm = np.zeros((nchan, npol, nrow), dtype=np.int32)  # casacore (row, pol, chan)
+        aw = self.kind == "AWProject"
+        nch_cf = self.aw_state["nchan"] if aw else 1
         for i in range(nrow):
             w = chunk.uvw[i, 2] / C_LIGHT
+            feed = int(chunk.feed1[i]) if (aw and chunk.feed1 is not None) else 0
             for c in range(nchan):
-                cm[c, :, i] = ws.plane(w * chunk.freq[c])
+                iw = ws.plane(w * chunk.freq[c])
+                if aw and iw >= 0:
+                    cc = c if self.wp.freqdep else 0
+                    iw += self.wp.nwplanes * (cc + nch_cf * (feed + self.wp.maxfeeds * 0))
+                cm[c, :, i] = iw
         return cm
 
     def _run(self, chunk: Chunk, mode: int):
         L = lib()
         nrow, nchan, npol = chunk.vis.shape
-        self._init_cf(pcf=(mode == 2))
+        if self.kind == "AWProject":
+            self._init_aw(chunk, psf=(mode in (1, 2)))
+        else:
+            self._init_cf(pcf=(mode == 2))
         os_ = self.cf.oversample
         f64 = self.precision == "f64"
         cf_data, cf_off, cf_supp = self.cf.flat(np.complex128 if f64 else np.complex64)
@@ -691,5 +854,42 @@ class OracleGridder:
 
     def finalise_weights(self):
         nx0, ny0 = self.shape0[0], self.shape0[1]
+        if self.kind == "AWProject":
+            return self._aw_weights()
         s = self.sumw.sum(axis=2)  # (chan, pol)
         return np.broadcast_to(s[:, :, None, None], (self.nchan_img, self.npol, ny0, nx0)).copy()
+
+    def _aw_weights(self):
+        """AWProjectVisGridder::finaliseWeights (AWProjectVisGridder.cc:618-770)."""
+        os_ = self.wp.oversample
+        cnx, cny = min(self.wp.maxsupport, self.nx), min(self.wp.maxsupport, self.ny)
+        cx0, cy0 = cnx // 2, cny // 2
+        if self.wp.spheroidaltaper:
+            sph = Spheroidal(math.pi * 3, self.wp.alpha)
+            fx = sph(np.abs(np.arange(cnx) - cx0) / cx0)
+            fy = sph(np.abs(np.arange(cny) - cy0) / cy0)
+            fx = interpolate_edge(np.where(fx > self.wp.spheroidalweightscutoff, fx, 0.0))
+            fy = interpolate_edge(np.where(fy > self.wp.spheroidalweightscutoff, fy, 0.0))
+        cout = np.zeros((self.nchan_img, self.npol, cny, cnx))
+        for iz in range(self.sumw.shape[2]):
+            wts = self.sumw[:, :, iz]
+            if not np.any(wts > 0):
+                continue
+            k = iz if self.track_per_os else os_ * os_ * iz
+            cf = self.cf.planes[k]
+            S = (cf.shape[0] - 1) // 2
+            pl = np.zeros((cny, cnx), dtype=np.complex128)
+            for iy in range(-S, S + 1):
+                for ix in range(-S, S + 1):
+                    xp, yp = ix + cx0, iy + cy0
+                    if 0 <= xp < cnx and 0 <= yp < cny:
+                        pl[yp, xp] = cf[iy + S, ix + S]
+            pl = fft_centred(pl, False) * (cnx * cny)
+            if self.wp.spheroidaltaper:
+                t = np.outer(fy, fx)
+                pl = np.where(t > 0, pl / np.where(t > 0, t, 1), 0)
+            cout += wts[:, :, None, None] * np.abs(pl)[None, None] ** 2
+        full = fft_pad(cout, self.ny, self.nx)
+        nx0, ny0 = self.shape0[0], self.shape0[1]
+        ox, oy = (self.nx - nx0) // 2, (self.ny - ny0) // 2
+        return full[:, :, oy:oy + ny0, ox:ox + nx0]

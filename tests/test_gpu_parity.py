@@ -207,6 +207,63 @@ def test_wproject_cf_grid_degrid_match_oracle(gpu_ctx, extra):
     assert rel(gv, ov) < TOL
 
 
+@pytest.mark.parametrize("taper", [False, True])
+def test_awproject_disk_matches_oracle(gpu_ctx, taper):
+    announce(f"awproject disk 12m, 2 feeds, 2 chans, spheroidaltaper={taper}")
+    import yandasoft_amd as ya
+    N, cell = 128, 60 * ARC
+    centre = (0.5, -0.7)
+    d = make_data(600, 2, 2, N, cell, f0=1.4e9, df=1e7, frac=0.3, wmax_m=400.0 * O.C_LIGHT / 1.41e9)
+    feed = (np.arange(600) % 2).astype(np.int32)
+    off = math.radians(0.5)
+    point = np.zeros((600, 2))
+    point[:, 0] = centre[0] + np.where(feed == 1, off, 0.0)
+    point[:, 1] = centre[1]
+    params = {"wmax": 500.0, "nwplanes": 3, "oversample": 4, "maxsupport": 128, "maxfeeds": 2, "cutoff": 0.01,
+              "diameter": "12m", "blockage": "1m", "spheroidaltaper": taper}
+    prm = O.AWProjectParams(wmax=500.0, nwplanes=3, oversample=4, maxsupport=128, maxfeeds=2, cutoff=0.01,
+                            diameter=12.0, blockage=1.0, spheroidaltaper=taper)
+    g = ya.Gridder(gpu_ctx, "GPUAWProject", params)
+    img = ya.ImageDesc(N, N, 2, 1, cell, cell, centre=centre)
+    g.initialise_grid(img)
+    ch = ya.Chunk(uvw=d["uvw"], freq=d["freq"], vis=d["vis"], noise=d["noise"], feed1=feed,
+                  pointing1=np.ascontiguousarray(point))
+    g.grid(ch)
+    o = O.OracleGridder("AWProject", "f32", wproject=prm)
+    o.aw_lm = {0: O.direction_lm(tuple(point[0]), centre), 1: O.direction_lm(tuple(point[1]), centre)}
+    o.aw_nchan = 2
+    o.initialise_grid((N, N, 2, 1), (cell, cell))
+    och = ochunk(d)
+    och.feed1 = feed
+    o.grid(och)
+    assert g.stats().support == o.support
+    os_ = 4
+    for feedn in (0, 1):
+        for chan in (0, 1):
+            for iw in (0, 2):
+                z = iw + 3 * (chan + 2 * feedn)
+                for frac in (0, 9):
+                    k = frac + os_ * os_ * z
+                    assert rel(g.cf_plane(k), o.cf.planes[k]) < TOL, (feedn, chan, iw, frac)
+    gg = g.grid_array().reshape(1, 2, N, N)
+    assert rel(gg, o.grid_) < TOL
+    assert rel(g.finalise_grid(), o.finalise_grid()) < TOL
+    assert rel(g.finalise_weights(), o.finalise_weights()) < TOL
+    # degrid
+    model = random_model(N, 2)
+    dg = ya.Gridder(gpu_ctx, "GPUAWProject", params)
+    dg.initialise_degrid(img, model.astype(np.float32))
+    out = np.zeros_like(d["vis"])
+    dg.degrid(ya.Chunk(uvw=d["uvw"], freq=d["freq"], vis=d["vis"], vis_out=out, noise=d["noise"], feed1=feed,
+                       pointing1=np.ascontiguousarray(point)))
+    od = O.OracleGridder("AWProject", "f32", wproject=prm)
+    od.aw_lm = o.aw_lm
+    od.aw_nchan = 2
+    od.initialise_degrid((N, N, 2, 1), (cell, cell), model)
+    ov = od.degrid(och)
+    assert rel(np.transpose(out, (2, 1, 0)), ov) < TOL
+
+
 def test_wproject_wmax_handling(gpu_ctx):
     announce("wproject wmaxclip / w out of range")
     import yandasoft_amd as ya

@@ -29,7 +29,8 @@ class YgridError(RuntimeError):
 class _ImageDesc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("npol", ctypes.c_int), ("nchan", ctypes.c_int),
                 ("cellx", ctypes.c_double), ("celly", ctypes.c_double), ("has_freq_axis", ctypes.c_int),
-                ("freq_start", ctypes.c_double), ("freq_end", ctypes.c_double)]
+                ("freq_start", ctypes.c_double), ("freq_end", ctypes.c_double),
+                ("centre_lon", ctypes.c_double), ("centre_lat", ctypes.c_double)]
 
 
 class _Chunk(ctypes.Structure):
@@ -86,11 +87,12 @@ class ImageDesc:
     cellx: float = 0.0
     celly: float = 0.0
     freq_axis: tuple | None = None  # (start, end) Hz
+    centre: tuple = (0.0, 0.0)      # image centre (lon, lat) rad
 
     def c(self) -> _ImageDesc:
         fa = self.freq_axis
         return _ImageDesc(self.nx, self.ny, self.npol, self.nchan, self.cellx, self.celly, 1 if fa else 0,
-                          fa[0] if fa else 0.0, fa[1] if fa else 0.0)
+                          fa[0] if fa else 0.0, fa[1] if fa else 0.0, self.centre[0], self.centre[1])
 
 
 def _ptr(a):
@@ -114,6 +116,7 @@ class Chunk:
     flag: object = None
     delay: object = None
     feed1: object = None
+    pointing1: object = None
     nrow: int = 0
     nchan: int = 0
     npol: int = 0
@@ -125,7 +128,7 @@ class Chunk:
         cube = self.vis if self.vis is not None else (self.vis_out if self.vis_out is not None else self.noise)
         self.npol = int(cube.shape[0]) if cube is not None else self.npol
         if not self.device:
-            for name in ("uvw", "freq", "vis", "vis_out", "noise", "flag", "delay", "feed1"):
+            for name in ("uvw", "freq", "vis", "vis_out", "noise", "flag", "delay", "feed1", "pointing1"):
                 a = getattr(self, name)
                 if a is not None:
                     assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], f"{name} must be C-contiguous"
@@ -133,7 +136,7 @@ class Chunk:
     def c(self) -> _Chunk:
         return _Chunk(self.nrow, self.nchan, self.npol, YGRID_MEM_DEVICE if self.device else YGRID_MEM_HOST,
                       _ptr(self.uvw), _ptr(self.delay), _ptr(self.freq), _ptr(self.vis), _ptr(self.vis_out),
-                      _ptr(self.noise), _ptr(self.flag), _ptr(self.feed1), None)
+                      _ptr(self.noise), _ptr(self.flag), _ptr(self.feed1), _ptr(self.pointing1))
 
 
 class Context:

@@ -1,8 +1,421 @@
-// AWProject gridder (filled in below the WProject path).
+// AWProject gridder: A-projection (disk illumination) x W-projection convolution functions
+// (AWProjectVisGridder.cc, AProjectGridderBase.cc/.tcc, DiskIllumination.cc), built on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "gridders.h"
 
 namespace ygrid {
-std::unique_ptr<Gridder> make_awproject(std::shared_ptr<Ctx>, const Params&) {
-  throw Error(5, "GPUAWProject is not available in this build");
+
+namespace {
+
+// SynthesisParamsHelper::convertQuantity for the units the A-projection keys use.
+double quantity(const std::string& s, const std::string& unit) {
+  size_t i = 0;
+  double v = 0;
+  try { v = std::stod(s, &i); } catch (...) { throw Error(1, "bad quantity " + s); }
+  std::string u = s.substr(i);
+  u.erase(0, u.find_first_not_of(" \t"));
+  if (u.empty() || u == unit) return v;
+  if (unit == "rad") {
+    if (u == "deg") return v * kPi / 180.;
+    if (u == "arcmin") return v * kPi / 180. / 60.;
+    if (u == "arcsec") return v * kPi / 180. / 3600.;
+  }
+  if (unit == "m") {
+    if (u == "cm") return v * 0.01;
+    if (u == "mm") return v * 1e-3;
+    if (u == "km") return v * 1e3;
+  }
+  throw Error(1, "cannot convert " + s + " to " + unit);
 }
+
+// MVDirection::separation (great-circle angle, numerically robust form)
+double separation(double l1, double b1, double l2, double b2) {
+  const double dl = l2 - l1;
+  const double a = std::cos(b2) * std::sin(dl);
+  const double b = std::cos(b1) * std::sin(b2) - std::sin(b1) * std::cos(b2) * std::cos(dl);
+  const double c = std::sin(b1) * std::sin(b2) + std::cos(b1) * std::cos(b2) * std::cos(dl);
+  return std::atan2(std::sqrt(a * a + b * b), c);
+}
+
+}  // namespace
+
+class AWProjectGridder : public Gridder {
+ public:
+  AWProjectGridder(std::shared_ptr<Ctx> c, const Params& p) : Gridder(std::move(c)) {
+    // AProjectGridderBase::createAProjectGridder (AProjectGridderBase.tcc:53-115)
+    pointing_tol_ = quantity(p.str("pointingtolerance", "0.0001rad"), "rad");
+    pa_tol_ = quantity(p.str("patolerance", "0.1rad"), "rad");
+    std::string ft = p.str("freqtolerance", "undefined");
+    std::transform(ft.begin(), ft.end(), ft.begin(), ::tolower);
+    freq_tol_ = -1.;
+    if (ft != "infinite") {
+      freq_tol_ = p.f64("freqtolerance", 1e-6);
+      YG_CHECK(freq_tol_ >= 0., 1, "Frequency tolerance parameter is supposed to be either a non-negative number or infinite");
+    }
+    wmax_ = p.f64("wmax", 10000.0);
+    nw_ = p.i32("nwplanes", 65);
+    cutoff_ = p.f64("cutoff", 1e-3);
+    os_ = p.i32("oversample", 8);
+    maxsupport_ = p.i32("maxsupport", 128);
+    limitsupport_ = p.i32("limitsupport", 0);
+    maxfeeds_ = p.i32("maxfeeds", 1);
+    maxfields_ = p.i32("maxfields", 1);
+    freqdep_ = p.flag("frequencydependent", true);
+    sph_taper_ = p.flag("spheroidaltaper", false);
+    sph_wcut_ = p.f64("spheroidalweightscutoff", 0.0);
+    alpha_ = p.f64("alpha", 1.0);
+    abs_cutoff_ = p.flag("cutoff.absolute", false);
+    YG_CHECK(!p.flag("variablesupport", false), 1, "variablesupport is not supported by GPUAWProject");
+    YG_CHECK(!p.flag("offsetsupport", false), 1, "offsetsupport is not supported by GPUAWProject");
+    YG_CHECK(wmax_ > 0.0, 1, "Baseline length must be greater than zero");
+    YG_CHECK(nw_ > 0 && nw_ % 2 == 1, 1, "Number of w planes must be a positive odd number");
+    YG_CHECK(os_ > 0 && maxsupport_ > 0 && maxsupport_ % 2 == 0, 1, "bad oversample / maxsupport");
+    YG_CHECK(maxfeeds_ > 0 && maxfields_ > 0, 1, "maxfeeds and maxfields must be positive");
+    if (!abs_cutoff_) YG_CHECK(cutoff_ > 0.0 && cutoff_ < 1.0, 1, "Cutoff must be in (0, 1)");
+    // makeIllumination (AProjectGridderBase.cc:290-303): the disk model
+    const std::string ill = p.str("illumination", "disk");
+    YG_CHECK(ill == "disk", 1, "only the disk illumination is available in GPUAWProject, not " + ill);
+    YG_CHECK(p.has("diameter") && p.has("blockage"), 1, "disk illumination needs diameter and blockage");
+    diameter_ = quantity(p.str("diameter", "12m"), "m");
+    blockage_ = quantity(p.str("blockage", "0m"), "m");
+    YG_CHECK(diameter_ > 0 && blockage_ >= 0 && diameter_ > blockage_, 1, "bad disk diameter/blockage");
+    const std::string smp = p.str("wsampling", "linear");
+    YG_CHECK(smp == "linear", 1, "GPUAWProject supports linear w sampling");
+    wmaxclip_ = p.flag("wmaxclip", false);
+    done_.assign((size_t)maxfeeds_ * maxfields_, 0);
+    pointings_.assign((size_t)maxfeeds_ * maxfields_ * 2, 0.0);
+  }
+  std::unique_ptr<Gridder> clone() const override {
+    auto g = std::unique_ptr<AWProjectGridder>(new AWProjectGridder(*this));
+    return g;
+  }
+  std::string name() const override { return "GPUAWProject"; }
+
+ protected:
+  int oversampling() const override { return os_; }
+  int nZPlanes() const override { return nw_ * maxfeeds_ * maxfields_ * std::max(1, nchan_cf_); }
+  bool needsRowMeta() const override { return true; }
+  bool doesCorrection() const override { return sph_taper_; }
+  void gridCorrection(std::vector<double>& cx, std::vector<double>& cy) const override {
+    if (sph_taper_) sph_grid_correction_xy(shape_[0], shape_[1], alpha_, 3, cx, cy);
+  }
+  CIndexDesc cindex(const ChunkDev&) const override {
+    CIndexDesc d;
+    d.kind = 2;
+    d.nw = nw_;
+    d.wscale = nw_ > 1 ? wmax_ / double((nw_ - 1) / 2) : 1.0;
+    d.wmax = wmax_;
+    d.wmaxclip = wmaxclip_ ? 1 : 0;
+    d.freqdep = freqdep_ ? 1 : 0;
+    d.nchan_cf = std::max(1, nchan_cf_);
+    d.maxfeeds = maxfeeds_;
+    d.field = field_;
+    return d;
+  }
+  void initConvolutionFunction(const ChunkDev& acc) override;
+  void finaliseWeights(float* out, int mem) override;
+
+ private:
+  double wterm(int plane) const {
+    const int half = (nw_ - 1) / 2;
+    return double(plane - half) * (nw_ > 1 ? wmax_ / double(half) : 1.0);
+  }
+  void indexField(const ChunkDev& acc);
+  void validateCache(const ChunkDev& acc);
+  void generateFeed(const ChunkDev& acc, int feed, double lon, double lat);
+  void ensureTable(int S);
+
+  double pointing_tol_ = 1e-4, pa_tol_ = 0.1, freq_tol_ = 1e-6;
+  double wmax_ = 10000, cutoff_ = 1e-3, alpha_ = 1.0, diameter_ = 12, blockage_ = 0, sph_wcut_ = 0;
+  int nw_ = 65, os_ = 8, maxsupport_ = 128, limitsupport_ = 0, maxfeeds_ = 1, maxfields_ = 1;
+  bool freqdep_ = true, sph_taper_ = false, abs_cutoff_ = false, wmaxclip_ = false;
+  int nchan_cf_ = 0, field_ = 0, last_field_ = -1;
+  std::vector<char> done_;
+  std::vector<double> pointings_;
+  std::vector<double> cached_freq_;
+};
+
+// AProjectGridderBase::indexField (AProjectGridderBase.cc:166-194): the field of the first row.
+void AWProjectGridder::indexField(const ChunkDev& acc) {
+  const int feed = acc.feed_h.empty() ? 0 : acc.feed_h[0];
+  YG_CHECK(feed >= 0 && feed < maxfeeds_, 1, "Too many feeds: increase maxfeeds");
+  const double lon = acc.point_h.empty() ? centre_[0] : acc.point_h[0];
+  const double lat = acc.point_h.empty() ? centre_[1] : acc.point_h[1];
+  for (int f = last_field_; f > -1; --f) {
+    const double* pt = &pointings_[((size_t)feed * maxfields_ + f) * 2];
+    if (separation(lon, lat, pt[0], pt[1]) < pointing_tol_) {
+      field_ = f;
+      return;
+    }
+  }
+  ++last_field_;
+  field_ = last_field_;
+  YG_CHECK(field_ < maxfields_, 1, "Too many fields: increase maxfields");
+  double* pt = &pointings_[((size_t)feed * maxfields_ + field_) * 2];
+  pt[0] = lon;
+  pt[1] = lat;
+}
+
+// AProjectGridderBase::validateCFCache (AProjectGridderBase.cc:202-253); the disk is symmetric so
+// the parallactic angle never invalidates the cache.
+void AWProjectGridder::validateCache(const ChunkDev& acc) {
+  if (freq_tol_ < 0.) return;
+  bool invalid = acc.freq_host.size() != cached_freq_.size();
+  for (size_t c = 0; !invalid && c < acc.freq_host.size(); ++c)
+    invalid = std::fabs(cached_freq_[c] - acc.freq_host[c]) / acc.freq_host[c] > freq_tol_;
+  if (invalid) {
+    std::fill(done_.begin(), done_.end(), 0);
+    cached_freq_ = acc.freq_host;
+  }
+}
+
+void AWProjectGridder::ensureTable(int S) {
+  if (cf_ && cf_.use_count() == 1 && cf_->support == S) return;
+  auto t = std::make_shared<CFTable>();
+  const int nz = nw_ * maxfeeds_ * maxfields_ * nchan_cf_;
+  t->nz = nz;
+  t->oversample = os_;
+  t->support = S;
+  const size_t blk = (size_t)cf_pitch(S) * cf_rows(S);
+  const size_t n = (size_t)nz * os_ * os_;
+  YG_CHECK(n * blk < (size_t)INT_MAX, 1, "convolution function table exceeds 2^31 elements");
+  t->off.resize(n);
+  t->supp.assign(n, S);
+  for (size_t k = 0; k < n; ++k) t->off[k] = (int32_t)(k * blk);
+  t->n_elems = n * blk;
+  t->table.resize(t->n_elems * sizeof(float2));
+  if (cf_ && cf_->support == S && cf_->n_elems == t->n_elems) {
+    // copy on write: a clone shares the cache until it has to add planes
+    YG_HIP(hipMemcpyAsync(t->table.get(), cf_->table.get(), t->n_elems * sizeof(float2), hipMemcpyDeviceToDevice,
+                          ctx_->stream));
+  } else {
+    YG_HIP(hipMemsetAsync(t->table.get(), 0, t->n_elems * sizeof(float2), ctx_->stream));
+  }
+  t->upload(*ctx_);
+  cf_ = t;
+}
+
+// The body of AWProjectVisGridder::initConvolutionFunction for one (feed, field)
+// (AWProjectVisGridder.cc:366-573): per channel, the disk pattern is transformed to the image
+// plane; per w plane, |A|^2 x taper x phase screen is transformed back, the support is found on
+// the first plane ever generated, and the os^2 cut-outs are scaled by os^2.
+void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, double lat) {
+  const Ctx& c = *ctx_;
+  const int nx = maxsupport_, ny = maxsupport_;
+  const int qnx = nx / os_, qny = ny / os_;
+  const double ccellx = 1.0 / (double(qnx) * uvcell_[0]);
+  const double ccelly = 1.0 / (double(qny) * uvcell_[1]);
+  const bool psf = dopsf_ || dopcf_;
+  double l = 0., m = 0.;
+  if (!psf) {  // DiskIllumination.cc:111-116
+    l = std::sin(lon - centre_[0]) * std::cos(lat);
+    m = std::sin(lat) * std::cos(centre_[1]) - std::cos(lat) * std::sin(centre_[1]) * std::cos(lon - centre_[0]);
+  }
+  DevBuf dtx, dty;
+  if (sph_taper_) {
+    const Spheroidal sph(kPi * 3, alpha_);
+    auto axis = [&](int q) {
+      std::vector<double> v(q);
+      for (int i = 0; i < q; ++i) v[i] = sph(std::fabs(double(i) - double(q / 2)) / double(q / 2));
+      v[0] = 3. * (v[1] - v[2]) + v[3];
+      if (q % 2 == 1) v[q - 1] = v[0];
+      return v;
+    };
+    const auto hx = axis(qnx), hy = axis(qny);
+    dtx.resize(hx.size() * 8);
+    dty.resize(hy.size() * 8);
+    YG_HIP(hipMemcpy(dtx.get(), hx.data(), hx.size() * 8, hipMemcpyHostToDevice));
+    YG_HIP(hipMemcpy(dty.get(), hy.data(), hy.size() * 8, hipMemcpyHostToDevice));
+  }
+  std::vector<double> tpw(nw_);
+  for (int iw = 0; iw < nw_; ++iw) tpw[iw] = 2.0f * kPi * wterm(iw);
+  DevBuf dtpw(nw_ * sizeof(double));
+  YG_HIP(hipMemcpy(dtpw.get(), tpw.data(), nw_ * sizeof(double), hipMemcpyHostToDevice));
+  const size_t plane_bytes = (size_t)nx * ny * sizeof(float2);
+  int nb = (int)std::max<size_t>(1, ((size_t)1 << 29) / plane_bytes);
+  nb = std::min(nb, nw_);
+  DevBuf pat(plane_bytes), buf(plane_bytes * nb), dprm(sizeof(DiskParams));
+  PlanPtr plan1, planb;
+  const int nchan = nchan_cf_;
+  for (int chan = 0; chan < nchan; ++chan) {
+    // DiskIllumination::getPattern (DiskIllumination.cc:85-177)
+    const double freq = acc.freq_host[chan];
+    const double cellU = uvcell_[0] / os_, cellV = uvcell_[1] / os_;
+    YG_CHECK(std::fabs(std::fabs(cellU / cellV) - 1.) < 1e-7, 1, "Rectangular cells are not supported");
+    const double cell = std::fabs(cellU * (kC / freq));
+    const double rmax2 = (diameter_ / (2.0 * cell)) * (diameter_ / (2.0 * cell));
+    double rmin2 = (blockage_ / (2.0 * cell)) * (blockage_ / (2.0 * cell));
+    YG_CHECK(rmax2 > rmin2, 1, "Disk hole is supposed to be less than the disk size");
+    YG_CHECK(double(nx) * nx > rmax2 && double(ny) * ny > rmax2, 1,
+             "The pattern buffer is too small for the dish: increase maxsupport");
+    if (rmin2 + 1 >= rmax2) --rmin2;
+    double count = 0;
+    for (int iu = 0; iu < nx; ++iu) {
+      const double ou = double(iu) - double(nx) / 2.;
+      for (int iv = 0; iv < ny; ++iv) {
+        const double ov = double(iv) - double(ny) / 2.;
+        const double r2 = ou * ou + ov * ov;
+        if (r2 >= rmin2 && r2 <= rmax2) count += 1.;
+      }
+    }
+    YG_CHECK(count > 0., 4, "Integral of the disk should be non-zero");
+    DiskParams dp{2. * kPi * cellU * l, 2. * kPi * cellV * m, rmin2, rmax2,
+                  (float)(float(nx) * float(ny) / float(count)), 0.f};
+    YG_HIP(hipMemcpyAsync(dprm.get(), &dp, sizeof(dp), hipMemcpyHostToDevice, c.stream));
+    launch_disk_pattern(c, pat.as<float2>(), nx, ny, dprm.as<DiskParams>(), 1);
+    // fft2d(pattern, false): backward, normalised by 1/(nx ny) -> folded into the screen kernel
+    fft_plain_c2c(c, pat.as<float2>(), nx, ny, 1, false, plan1);
+    for (int z0 = 0; z0 < nw_; z0 += nb) {
+      const int cnt = std::min(nb, nw_ - z0);
+      launch_aw_screen(c, pat.as<float2>(), nx, ny, qnx, qny, ccellx, ccelly, (float)(1.0 / (double(nx) * ny)),
+                       sph_taper_ ? dtx.as<double>() : nullptr, sph_taper_ ? dty.as<double>() : nullptr,
+                       dtpw.as<double>() + z0, cnt, buf.as<float2>());
+      fft_plain_c2c(c, buf.as<float2>(), nx, ny, cnt, true, planb);
+      launch_scale_sign(c, buf.as<float2>(), nx, ny, cnt, (float)(1.0 / (double(nx) * ny)));
+      if (!cf_ || cf_->support == 0) {
+        // support from the first plane generated (AWProjectVisGridder.cc:436-466)
+        const SupportResult r = search_support(c, buf.as<float2>(), nx, ny, cutoff_, abs_cutoff_, false, os_);
+        YG_CHECK((r.size + 1) * os_ < nx / 2, 4,
+                 "Overflowing convolution function - increase maxSupport or cutoff or decrease overSample");
+        int S = r.size;
+        if (limitsupport_ > 0 && S > limitsupport_) S = limitsupport_;
+        ensureTable(S);
+      } else {
+        ensureTable(cf_->support);
+      }
+      const int S = cf_->support;
+      const int kmin = -S * os_ + nx / 2, kmax = S * os_ + os_ - 1 + nx / 2;
+      YG_CHECK(kmin >= 0 && kmax < nx && kmax < ny, 4,
+               "Convolution function overflowing - increase maxsupport or cutoff or decrease oversample");
+      std::vector<int32_t> first(cnt);
+      std::vector<int> sup(cnt, S), zero(cnt, 0);
+      for (int b = 0; b < cnt; ++b) {
+        const int z = (z0 + b) + nw_ * (chan + nchan * (feed + maxfeeds_ * field_));
+        first[b] = cf_->off[(size_t)z * os_ * os_];
+      }
+      DevBuf dfirst(cnt * 4), dsup(cnt * 4), dzero(cnt * 4);
+      YG_HIP(hipMemcpy(dfirst.get(), first.data(), cnt * 4, hipMemcpyHostToDevice));
+      YG_HIP(hipMemcpy(dsup.get(), sup.data(), cnt * 4, hipMemcpyHostToDevice));
+      YG_HIP(hipMemcpy(dzero.get(), zero.data(), cnt * 4, hipMemcpyHostToDevice));
+      launch_cf_cutout(c, buf.as<float2>(), nx, ny, cnt, os_, dsup.as<int>(), dzero.as<int>(), dzero.as<int>(),
+                       dfirst.as<int32_t>(), 1, double(os_) * os_, cf_->table.as<float2>());
+      YG_HIP(hipStreamSynchronize(c.stream));
+    }
+  }
+}
+
+void AWProjectGridder::initConvolutionFunction(const ChunkDev& acc) {
+  YG_CHECK(!acc.freq_host.empty(), 1, "no channels");
+  if (nchan_cf_ == 0) nchan_cf_ = freqdep_ ? acc.nchan : 1;
+  YG_CHECK(!freqdep_ || acc.nchan == nchan_cf_, 1, "the number of channels changed between chunks");
+  indexField(acc);
+  validateCache(acc);
+  for (int row = 0; row < acc.nrow; ++row) {
+    const int feed = acc.feed_h.empty() ? 0 : acc.feed_h[row];
+    YG_CHECK(feed >= 0 && feed < maxfeeds_, 1, "Exceeded specified maximum number of feeds");
+    char& d = done_[(size_t)feed * maxfields_ + field_];
+    if (d) continue;
+    d = 1;
+    const double lon = acc.point_h.empty() ? centre_[0] : acc.point_h[2 * (size_t)row];
+    const double lat = acc.point_h.empty() ? centre_[1] : acc.point_h[2 * (size_t)row + 1];
+    generateFeed(acc, feed, lon, lat);
+  }
+}
+
+// AWProjectVisGridder::finaliseWeights (AWProjectVisGridder.cc:618-770): the weights image is
+// sum_z wt_z |iFFT(CF_z)|^2 on a min(maxsupport, nx) grid, Fourier-interpolated to the image.
+void AWProjectGridder::finaliseWeights(float* out, int mem) {
+  YG_CHECK(initialised_ && !degrid_mode_, 2, "finaliseWeights without an initialised grid");
+  const Ctx& c = *ctx_;
+  const int nx = shape_[0], ny = shape_[1], npol = shape_[2], nchan = shape_[3], ncp = npol * nchan;
+  const int cnx = std::min(maxsupport_, nx), cny = std::min(maxsupport_, ny);
+  std::vector<double> sw((size_t)nsw_rows_ * ncp);
+  YG_HIP(hipMemcpy(sw.data(), sumw_.get(), sw.size() * sizeof(double), hipMemcpyDeviceToHost));
+  DevBuf cout((size_t)cnx * cny * ncp * sizeof(double));
+  YG_HIP(hipMemsetAsync(cout.get(), 0, (size_t)cnx * cny * ncp * sizeof(double), c.stream));
+  DevBuf dtx, dty;
+  if (sph_taper_) {
+    const Spheroidal sph(kPi * 3, alpha_);
+    auto axis = [&](int n) {
+      std::vector<double> v(n);
+      for (int i = 0; i < n; ++i) {
+        const double val = sph(std::fabs(double(i) - double(n / 2)) / double(n / 2));
+        v[i] = val > sph_wcut_ ? val : 0.;
+      }
+      v[0] = 3. * (v[1] - v[2]) + v[3];
+      if (n % 2 == 1) v[n - 1] = v[0];
+      return v;
+    };
+    const auto hx = axis(cnx), hy = axis(cny);
+    dtx.resize(hx.size() * 8);
+    dty.resize(hy.size() * 8);
+    YG_HIP(hipMemcpy(dtx.get(), hx.data(), hx.size() * 8, hipMemcpyHostToDevice));
+    YG_HIP(hipMemcpy(dty.get(), hy.data(), hy.size() * 8, hipMemcpyHostToDevice));
+  }
+  // rows with data
+  std::vector<int> rows;
+  for (int z = 0; z < nsw_rows_; ++z) {
+    bool has = false;
+    for (int cp = 0; cp < ncp; ++cp) has |= sw[z + (size_t)nsw_rows_ * cp] > 0.0;
+    if (has) rows.push_back(z);
+  }
+  if (cf_ && !rows.empty()) {
+    const int nb = std::max(1, std::min((int)rows.size(), (int)(((size_t)1 << 28) / ((size_t)cnx * cny * 8))));
+    DevBuf planes((size_t)nb * cnx * cny * sizeof(float2)), dw((size_t)nb * ncp * sizeof(double));
+    DevBuf doff(nb * 4), dsup(nb * 4), dzero(nb * 4);
+    PlanPtr plan;
+    for (size_t r0 = 0; r0 < rows.size(); r0 += nb) {
+      const int cnt = (int)std::min<size_t>(nb, rows.size() - r0);
+      std::vector<int32_t> off(cnt);
+      std::vector<int> sup(cnt), zero(cnt, 0);
+      std::vector<double> w((size_t)cnt * ncp);
+      for (int b = 0; b < cnt; ++b) {
+        const int z = rows[r0 + b];
+        const int k = track_per_os_ ? z : os_ * os_ * z;  // cfIndexFromSumOfWeightsRow
+        off[b] = cf_->off[k];
+        sup[b] = cf_->supp[k];
+        for (int cp = 0; cp < ncp; ++cp) w[(size_t)b * ncp + cp] = sw[z + (size_t)nsw_rows_ * cp];
+      }
+      YG_HIP(hipMemcpy(doff.get(), off.data(), cnt * 4, hipMemcpyHostToDevice));
+      YG_HIP(hipMemcpy(dsup.get(), sup.data(), cnt * 4, hipMemcpyHostToDevice));
+      YG_HIP(hipMemcpy(dzero.get(), zero.data(), cnt * 4, hipMemcpyHostToDevice));
+      YG_HIP(hipMemcpy(dw.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice));
+      launch_place_cf(c, cf_->table.as<float2>(), doff.as<int32_t>(), dsup.as<int>(), dzero.as<int>(), dzero.as<int>(),
+                      cnx, cny, cnt, planes.as<float2>());
+      // fft2d(thisPlane, false) * (cnx*cny): the unnormalised backward transform
+      fft_plain_c2c(c, planes.as<float2>(), cnx, cny, cnt, false, plan);
+      launch_aw_weights_acc(c, planes.as<float2>(), cnt, cnx, cny, dw.as<double>(), ncp,
+                            sph_taper_ ? dtx.as<double>() : nullptr, sph_taper_ ? dty.as<double>() : nullptr,
+                            cout.as<double>());
+    }
+  }
+  // PaddingUtils::fftPad onto the padded image, then the unpadded centre
+  // (the centred backward transform is unnormalised: its 1/(cnx cny) is applied on conversion)
+  DevBuf small((size_t)cnx * cny * ncp * sizeof(float2)), big((size_t)nx * ny * ncp * sizeof(float2));
+  launch_d2c(c, cout.as<double>(), (size_t)cnx * cny * ncp, 1.0 / (double(cnx) * cny), small.as<float2>());
+  PlanPtr p1, p2;
+  fft_centred_c2c(c, small.as<float2>(), cnx, cny, ncp, false, p1);
+  launch_pad_centre(c, small.as<float2>(), cnx, cny, nx, ny, ncp, big.as<float2>());
+  fft_centred_c2c(c, big.as<float2>(), nx, ny, ncp, true, p2);
+  const size_t nout = (size_t)img_[0] * img_[1] * ncp;
+  DevBuf tmp;
+  float* dout = out;
+  if (mem != YGRID_MEM_DEVICE) {
+    tmp.resize(nout * sizeof(float));
+    dout = tmp.as<float>();
+  }
+  launch_extract_real(c, big.as<float2>(), nx, ny, img_[0], img_[1], ncp, dout);
+  if (mem != YGRID_MEM_DEVICE) YG_HIP(hipMemcpyAsync(out, dout, nout * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+  YG_HIP(hipStreamSynchronize(c.stream));
+}
+
+std::unique_ptr<Gridder> make_awproject(std::shared_ptr<Ctx> ctx, const Params& p) {
+  return std::unique_ptr<Gridder>(new AWProjectGridder(std::move(ctx), p));
+}
+
 }  // namespace ygrid

@@ -152,6 +152,8 @@ void Gridder::initialiseCommon(const ygrid_image_desc& img) {
   has_freq_axis_ = img.has_freq_axis != 0;
   fstart_ = img.freq_start;
   fend_ = img.freq_end;
+  centre_[0] = img.centre_lon;
+  centre_[1] = img.centre_lat;
   initialised_ = true;
 }
 
@@ -207,6 +209,16 @@ void Gridder::stage(const ygrid_chunk& a, ChunkDev& d, bool forward) {
   const size_t nr = a.nrow, nc = a.nchan, np = a.npol;
   const size_t ncube = nr * nc * np;
   d.freq_host.resize(nc);
+  if (needsRowMeta()) {
+    d.feed_h.assign(nr, 0);
+    d.point_h.clear();
+    const hipMemcpyKind k = a.mem == YGRID_MEM_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+    if (a.feed1 && nr) YG_HIP(hipMemcpy(d.feed_h.data(), a.feed1, nr * sizeof(int32_t), k));
+    if (a.pointing1 && nr) {
+      d.point_h.resize(nr * 2);
+      YG_HIP(hipMemcpy(d.point_h.data(), a.pointing1, nr * 2 * sizeof(double), k));
+    }
+  }
   if (a.mem == YGRID_MEM_DEVICE) {
     d.uvw = a.uvw;
     d.delay = a.delay;
@@ -300,6 +312,18 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   initConvolutionFunction(d);
   t_cf_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - tcf).count();
   YG_CHECK(cf_ && cf_->n_elems > 0, 2, "convolution function not initialised");
+  if (!forward) {
+    // the sum-of-weights cube follows the CF cache (resizeSumOfWeights when the cache is
+    // first built, AWProjectVisGridder.cc:274-276)
+    const int rows = nZPlanes() * (track_per_os_ ? cf_->oversample * cf_->oversample : 1);
+    if (rows != nsw_rows_) {
+      nsw_rows_ = rows;
+      const size_t nsw = (size_t)nsw_rows_ * shape_[2] * shape_[3];
+      sumw_.release();
+      sumw_.resize(nsw * sizeof(double));
+      YG_HIP(hipMemsetAsync(sumw_.get(), 0, nsw * sizeof(double), ctx_->stream));
+    }
+  }
 
   const int nsamp = d.nrow * d.nchan;
   recs_.resize((size_t)nsamp * sizeof(SampleRec));
@@ -742,10 +766,7 @@ class WProjectGridder : public Gridder {
 
 // SupportSearcher on the row/column through the peak (SupportSearcher.tcc:161-257) and
 // WProjectVisGridder::extractSupport (WProjectVisGridder.cc:198-263).
-struct SupportResult {
-  int size, offu, offv;
-};
-static SupportResult search_support(const Ctx& c, const float2* plane, int nx, int ny, double cutoff, bool absolute,
+SupportResult search_support(const Ctx& c, const float2* plane, int nx, int ny, double cutoff, bool absolute,
                                     bool offset, int os) {
   DevBuf am(sizeof(unsigned long long)), cross((size_t)(nx + ny) * sizeof(float2));
   launch_argmax(c, plane, nx * ny, am.as<unsigned long long>());

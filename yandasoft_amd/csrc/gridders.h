@@ -51,7 +51,17 @@ struct ChunkDev {
   const int32_t* feed1 = nullptr;
   std::vector<double> freq_host;
   int feed0 = 0;
+  // host copies of per-row metadata for gridders whose CF cache depends on it (AWProject)
+  std::vector<int32_t> feed_h;
+  std::vector<double> point_h;  // nrow x 2, empty: every row points at the image centre
 };
+
+struct SupportResult {
+  int size, offu, offv;
+};
+// SupportSearcher + WProjectVisGridder::extractSupport on one device plane (nx x ny).
+SupportResult search_support(const Ctx& c, const float2* plane, int nx, int ny, double cutoff, bool absolute,
+                             bool offset, int os);
 
 class Gridder {
  public:
@@ -64,7 +74,7 @@ class Gridder {
   void initialiseGrid(const ygrid_image_desc& img, bool dopsf, bool dopcf);
   void grid(const ygrid_chunk& acc);
   void finaliseGrid(float* out, int mem);
-  void finaliseWeights(float* out, int mem);
+  virtual void finaliseWeights(float* out, int mem);
   void initialiseDegrid(const ygrid_image_desc& img, const float* model, int mem);
   void degrid(const ygrid_chunk& acc);
   void finaliseDegrid();
@@ -88,6 +98,8 @@ class Gridder {
   // 1-D factors of correctConvolution (empty: no correction, e.g. Box)
   virtual void gridCorrection(std::vector<double>& cx, std::vector<double>& cy) const;
   virtual bool doesCorrection() const { return true; }
+  // whether initConvolutionFunction needs host copies of feed1/pointing1
+  virtual bool needsRowMeta() const { return false; }
 
   void initialiseCommon(const ygrid_image_desc& img);
   void generic(const ygrid_chunk& acc, bool forward);
@@ -107,6 +119,7 @@ class Gridder {
   int psf_feed_ = -1;
   bool has_freq_axis_ = false;
   double fstart_ = 0, fend_ = 0;
+  double centre_[2] = {0, 0};  // image centre (lon, lat)
   double vw_ref_ = 0;
   int vw_order_ = -1;
   int nsw_rows_ = 1;

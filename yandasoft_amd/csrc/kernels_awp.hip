@@ -1,0 +1,192 @@
+// AWProject convolution functions and weights image on the device
+// (AWProjectVisGridder.cc:259-611,618-770; DiskIllumination.cc:85-177).
+#include <hip/hip_runtime.h>
+
+#include "ygrid_internal.h"
+
+namespace ygrid {
+
+namespace {
+
+__device__ __forceinline__ float cb_sign(int x, int y) { return ((x + y) & 1) ? -1.f : 1.f; }
+
+// Disk aperture in the uv plane (iU fastest), pre-multiplied by the input checkerboard of a
+// centred transform.  One plane per channel of the batch.
+__global__ void k_disk_pattern(float2* __restrict__ buf, int nu, int nv, const DiskParams* __restrict__ prm) {
+  const int iU = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iV = blockIdx.y;
+  if (iU >= nu) return;
+  const DiskParams p = prm[blockIdx.z];
+  const double ou = double(iU) - double(nu) / 2., ov = double(iV) - double(nv) / 2.;
+  const double r2 = ou * ou + ov * ov;
+  float2 v = make_float2(0.f, 0.f);
+  if (r2 >= p.rmin2 && r2 <= p.rmax2) {
+    const double ph = p.lscaled * ou + p.mscaled * ov;
+    const float s = p.scale * cb_sign(iU, iV);
+    v = make_float2((float)cos(ph) * s, (float)(-sin(ph)) * s);
+  }
+  buf[(size_t)blockIdx.z * nu * nv + (size_t)iV * nu + iU] = v;
+}
+
+// |A|^2 x taper x w-screen (AWProjectVisGridder.cc:397-418) for the w terms of the batch; A is
+// the raw backward transform of the disk (its centring sign drops out of |A|^2, its 1/(nu nv)
+// normalisation is `ascale`).  Written pre-multiplied by the forward transform's checkerboard.
+__global__ void k_aw_screen(const float2* __restrict__ A, int nx, int ny, int qnx, int qny, double ccellx,
+                            double ccelly, float ascale, const double* __restrict__ tx, const double* __restrict__ ty,
+                            const double* __restrict__ two_pi_w, float2* __restrict__ out) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y;
+  if (ix >= nx) return;
+  const double y = (double(iy) - double(ny) / 2) * ccelly;
+  const double x = (double(ix) - double(nx) / 2) * ccellx;
+  const double r2 = x * x + y * y;
+  const int qix = ix + qnx / 2 - nx / 2, qiy = iy + qny / 2 - ny / 2;
+  float2 o = make_float2(0.f, 0.f);
+  const bool taper = tx != nullptr;
+  if (r2 < 1.0 && (!taper || (qix >= 0 && qix < qnx && qiy >= 0 && qiy < qny))) {
+    const double phase = two_pi_w[blockIdx.z] * (1.0 - sqrt(1.0 - r2));
+    const float2 a = A[(size_t)iy * nx + ix];
+    const float ar = a.x * ascale, ai = a.y * ascale;
+    const float t = taper ? (float)(tx[qix] * ty[qiy]) : 1.f;
+    const float wt = (ar * ar + ai * ai) * t;
+    const float s = cb_sign(ix, iy);
+    o = make_float2((float)(wt * cos(phase)) * s, (float)(wt * -sin(phase)) * s);
+  }
+  out[(size_t)blockIdx.z * nx * ny + (size_t)iy * nx + ix] = o;
+}
+
+__global__ void k_scale_sign(float2* data, int nx, int ny, float gsign, float scale) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  float2* p = data + (size_t)blockIdx.z * nx * ny + (size_t)y * nx + x;
+  const float s = cb_sign(x, y) * gsign * scale;
+  float2 v = *p;
+  v.x *= s;
+  v.y *= s;
+  *p = v;
+}
+
+// Weights image: place the frac-0 CF plane of row z at the centre of a cnx x cny plane
+// (AWProjectVisGridder.cc:683-697), pre-multiplied by the centred backward transform's checkerboard.
+__global__ void k_place_cf(const float2* __restrict__ table, const int32_t* __restrict__ off, const int* __restrict__ supp,
+                           const int* __restrict__ ou, const int* __restrict__ ov, int cnx, int cny,
+                           float2* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, b = blockIdx.z;
+  if (x >= cnx) return;
+  const int S = supp[b];
+  const int ix = x - cnx / 2 - ou[b], iy = y - cny / 2 - ov[b];
+  float2 v = make_float2(0.f, 0.f);
+  if (ix >= -S && ix <= S && iy >= -S && iy <= S) {
+    const int P = 2 * S + 1 + 2 * kGuardU;
+    v = table[off[b] + (size_t)(iy + S + kGuardV) * P + ix + S + kGuardU];
+    const float s = cb_sign(x, y);
+    v.x *= s;
+    v.y *= s;
+  }
+  out[(size_t)b * cnx * cny + (size_t)y * cnx + x] = v;
+}
+
+// cOut[cp] += sum_b wt[b][cp] |val_b / taper|^2 where val_b is the raw transform of plane b
+// (centring sign drops out of |.|^2).
+__global__ void k_aw_weights_acc(const float2* __restrict__ planes, int nb, int cnx, int cny, const double* __restrict__ wt,
+                                 int ncp, const double* __restrict__ tx, const double* __restrict__ ty,
+                                 double* __restrict__ cout) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, cp = blockIdx.z;
+  if (x >= cnx) return;
+  double t = 1.0;
+  if (tx) t = tx[x] * ty[y];
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const double w = wt[(size_t)b * ncp + cp];
+    if (w == 0.0) continue;
+    const float2 v = planes[(size_t)b * cnx * cny + (size_t)y * cnx + x];
+    double vr = v.x, vi = v.y;
+    if (tx) {
+      if (t > 0) { vr /= t; vi /= t; } else { vr = vi = 0.0; }
+    }
+    acc += w * (vr * vr + vi * vi);
+  }
+  cout[(size_t)cp * cnx * cny + (size_t)y * cnx + x] += acc;
+}
+
+// Zero-padding of centred spectra: out (nx x ny) <- in (cnx x cny) at the centre.
+__global__ void k_pad_centre(const float2* __restrict__ in, int cnx, int cny, int nx, int ny, float2* __restrict__ out) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y;
+  if (X >= nx) return;
+  const int x = X - (nx - cnx) / 2, y = Y - (ny - cny) / 2;
+  float2 v = make_float2(0.f, 0.f);
+  if (x >= 0 && x < cnx && y >= 0 && y < cny) v = in[(size_t)blockIdx.z * cnx * cny + (size_t)y * cnx + x];
+  out[(size_t)blockIdx.z * nx * ny + (size_t)Y * nx + X] = v;
+}
+
+__global__ void k_d2c(const double* __restrict__ in, size_t n, double scale, float2* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2((float)(in[i] * scale), 0.f);
+}
+
+// out[plane][y][x] = Re(in[plane][y+oy][x+ox])
+__global__ void k_extract_real(const float2* __restrict__ in, int nxp, int nyp, int nx, int ny, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  const int ox = (nxp - nx) / 2, oy = (nyp - ny) / 2;
+  out[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] = in[(size_t)blockIdx.z * nxp * nyp + (size_t)(y + oy) * nxp + x + ox].x;
+}
+
+}  // namespace
+
+static float gsign(int nx, int ny) { return (((nx / 2) + (ny / 2)) & 1) ? -1.f : 1.f; }
+
+void launch_disk_pattern(const Ctx& c, float2* buf, int nu, int nv, const DiskParams* prm, int nb) {
+  hipLaunchKernelGGL(k_disk_pattern, dim3((nu + 255) / 256, nv, nb), dim3(256), 0, c.stream, buf, nu, nv, prm);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_aw_screen(const Ctx& c, const float2* A, int nx, int ny, int qnx, int qny, double ccellx, double ccelly,
+                      float ascale, const double* tx, const double* ty, const double* two_pi_w, int nb, float2* out) {
+  hipLaunchKernelGGL(k_aw_screen, dim3((nx + 255) / 256, ny, nb), dim3(256), 0, c.stream, A, nx, ny, qnx, qny, ccellx,
+                     ccelly, ascale, tx, ty, two_pi_w, out);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_scale_sign(const Ctx& c, float2* data, int nx, int ny, int nb, float scale) {
+  hipLaunchKernelGGL(k_scale_sign, dim3((nx + 255) / 256, ny, nb), dim3(256), 0, c.stream, data, nx, ny, gsign(nx, ny),
+                     scale);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_place_cf(const Ctx& c, const float2* table, const int32_t* off, const int* supp, const int* ou, const int* ov,
+                     int cnx, int cny, int nb, float2* out) {
+  hipLaunchKernelGGL(k_place_cf, dim3((cnx + 255) / 256, cny, nb), dim3(256), 0, c.stream, table, off, supp, ou, ov, cnx,
+                     cny, out);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_aw_weights_acc(const Ctx& c, const float2* planes, int nb, int cnx, int cny, const double* wt, int ncp,
+                           const double* tx, const double* ty, double* cout) {
+  hipLaunchKernelGGL(k_aw_weights_acc, dim3((cnx + 255) / 256, cny, ncp), dim3(256), 0, c.stream, planes, nb, cnx, cny,
+                     wt, ncp, tx, ty, cout);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_pad_centre(const Ctx& c, const float2* in, int cnx, int cny, int nx, int ny, int nb, float2* out) {
+  hipLaunchKernelGGL(k_pad_centre, dim3((nx + 255) / 256, ny, nb), dim3(256), 0, c.stream, in, cnx, cny, nx, ny, out);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_d2c(const Ctx& c, const double* in, size_t n, double scale, float2* out) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_d2c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, in, n, scale, out);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_extract_real(const Ctx& c, const float2* in, int nxp, int nyp, int nx, int ny, int nb, float* out) {
+  hipLaunchKernelGGL(k_extract_real, dim3((nx + 255) / 256, ny, nb), dim3(256), 0, c.stream, in, nxp, nyp, nx, ny, out);
+  YG_HIP(hipGetLastError());
+}
+
+}  // namespace ygrid

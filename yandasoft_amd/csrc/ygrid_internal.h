@@ -207,6 +207,26 @@ void launch_cf_cutout(const Ctx&, const float2* buf, int nx, int ny, int nb, int
 void launch_sph_cf(const Ctx&, const double* bx, const double* by, int os, int support, const int32_t* cf_off,
                    float2* table);
 
+// AWProject kernels (kernels_awp.hip)
+struct DiskParams {
+  double lscaled, mscaled;  // 2 pi cell l, 2 pi cell m
+  double rmin2, rmax2;      // annulus in pattern cells^2
+  float scale;              // nU nV / number of cells inside
+  float pad;
+};
+void launch_disk_pattern(const Ctx&, float2* buf, int nu, int nv, const DiskParams* prm, int nb);
+void launch_aw_screen(const Ctx&, const float2* A, int nx, int ny, int qnx, int qny, double ccellx, double ccelly,
+                      float ascale, const double* tx, const double* ty, const double* two_pi_w, int nb, float2* out);
+// multiply by the output checkerboard of a centred transform (incl. its global sign) and a scale
+void launch_scale_sign(const Ctx&, float2* data, int nx, int ny, int nb, float scale);
+void launch_place_cf(const Ctx&, const float2* table, const int32_t* off, const int* supp, const int* ou,
+                     const int* ov, int cnx, int cny, int nb, float2* out);
+void launch_aw_weights_acc(const Ctx&, const float2* planes, int nb, int cnx, int cny, const double* wt, int ncp,
+                           const double* tx, const double* ty, double* cout);
+void launch_pad_centre(const Ctx&, const float2* in, int cnx, int cny, int nx, int ny, int nb, float2* out);
+void launch_d2c(const Ctx&, const double* in, size_t n, double scale, float2* out);
+void launch_extract_real(const Ctx&, const float2* in, int nxp, int nyp, int nx, int ny, int nb, float* out);
+
 // Host helpers (spheroidal.cpp)
 class Spheroidal {
  public:
