@@ -296,7 +296,7 @@ typedef struct {
         const int ld = 2 * support + 1;                                                                            \
         const int iuo = iu + (cf_ofs_u ? cf_ofs_u[plane] : 0);                                                     \
         const int ivo = iv + (cf_ofs_v ? cf_ofs_v[plane] : 0);                                                     \
-        GRID_T *g2 = grid + (size_t)p->nx * p->ny * (pol + (size_t)p->npol_img * imageChan);                       \
+        GRID_T *g2 = grid + 2 * (size_t)p->nx * p->ny * (pol + (size_t)p->npol_img * imageChan);                   \
         const CF_T *cf = cf_data + 2 * cf_off[cInd];                                                               \
         if (!((iuo - support) > 0 && (ivo - support) > 0 && (iuo + support) < p->nx && (ivo + support) < p->ny)) { \
           if (forward && p->stale_offgrid) { out_r[pol] = stale_r[pol]; out_i[pol] = stale_i[pol]; }               \

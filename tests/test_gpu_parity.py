@@ -1,7 +1,13 @@
 """HIP gridder (through the C ABI) against the CPU oracle on identical seeded inputs.
 
-Bar: max|gpu - oracle| / max|oracle| <= 1e-5 (FP32) for the grid, the dirty image, the sum of
-weights and the degridded visibilities; CF planes to 1e-5 of their peak.
+Bar: max|gpu - oracle| / max|oracle| <= 1e-5 (FP32) for the grid, the sum of weights and the
+degridded visibilities; CF planes to 1e-5 of their peak.
+
+Dirty images: the grid correction multiplies the image edges by 1/grdsf (up to ~1e2), which
+lifts the rounding of any FP32 grid accumulation there to ~1e-4 of the peak (the oracle's own
+FP32 restatement differs from its FP64 form by that much).  Images are therefore compared with
+the FP64 oracle: <= 1e-5 of the peak where the correction gain is <= 10, and over the whole
+image no worse than twice the FP32 restatement's own deviation from FP64.
 """
 import math
 
@@ -20,6 +26,20 @@ pytestmark = pytest.mark.gpu
 def rel(a, b):
     b = np.asarray(b)
     return float(np.max(np.abs(np.asarray(a) - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def image_ok(gi, oi32, oi64, gain):
+    """gain: (ny, nx) grid-correction factor of the oracle gridder."""
+    peak = max(float(np.max(np.abs(oi64))), 1e-300)
+    err = np.abs(np.asarray(gi, np.float64) - oi64)
+    inner = float(np.max(err[..., gain <= 10.0])) / peak
+    full = float(np.max(err)) / peak
+    ref = float(np.max(np.abs(oi32 - oi64))) / peak
+    return inner <= TOL and full <= max(TOL, 2.0 * ref), (inner, full, ref)
+
+
+def gain_of(o):
+    return o.correct(np.ones((1, 1, o.ny, o.nx)))[0, 0]
 
 
 def make_data(nrow, nchan, npol, N, cell, f0=1.4e9, df=5e6, frac=0.45, wmax_m=0.0, seed=7, flags=False,
@@ -56,7 +76,10 @@ def pchunk(d, vis_out=None):
 
 
 def grid_both(ctx, name, params, ogrid_kwargs, d, N, cell, npol, nchan_img=1, dopsf=False, dopcf=False,
-              freq_axis=None, mfs=None):
+              freq_axis=None, mfs=None, extra_oracle=None):
+    """Grids d with the GPU gridder and the oracle (FP32 and FP64); checks the dirty image
+    (image_ok) and returns (gpu gridder, f32 oracle, gpu grid, oracle grid, gpu image,
+    f32 oracle image, gpu weights, oracle weights)."""
     import yandasoft_amd as ya
     g = ya.Gridder(ctx, name, params)
     if mfs:
@@ -67,14 +90,20 @@ def grid_both(ctx, name, params, ogrid_kwargs, d, N, cell, npol, nchan_img=1, do
     ggrid = g.grid_array().reshape(nchan_img, npol, N, N)
     gimg = g.finalise_grid()
     gw = g.finalise_weights()
-    o = O.OracleGridder(precision="f32", **ogrid_kwargs)
-    if mfs:
-        o.visweight = lambda f, r=mfs[0], k=mfs[1]: ((f - r) / r) ** k
-    o.initialise_grid((N, N, npol, nchan_img), (cell, cell), dopsf=dopsf, dopcf=dopcf, freq_axis=freq_axis)
-    o.grid(ochunk(d))
-    oimg = o.finalise_grid()
+    imgs = {}
+    for prec in ("f64", "f32"):
+        o = O.OracleGridder(precision=prec, **ogrid_kwargs)
+        for k, v in (extra_oracle or {}).items():
+            setattr(o, k, v)
+        if mfs:
+            o.visweight = lambda f, r=mfs[0], k=mfs[1]: ((f - r) / r) ** k
+        o.initialise_grid((N, N, npol, nchan_img), (cell, cell), dopsf=dopsf, dopcf=dopcf, freq_axis=freq_axis)
+        o.grid(ochunk(d))
+        imgs[prec] = o.finalise_grid()
     ow = o.finalise_weights()
-    return g, o, ggrid, o.grid_, gimg, oimg, gw, ow
+    ok, info = image_ok(gimg, imgs["f32"], imgs["f64"], gain_of(o))
+    assert ok, f"image parity (inner, full, f32-oracle) = {info}"
+    return g, o, ggrid, o.grid_, gimg, imgs["f32"], gw, ow
 
 
 def degrid_both(ctx, name, params, ogrid_kwargs, d, N, cell, npol, model, nchan_img=1, freq_axis=None):
@@ -120,7 +149,7 @@ def test_sphfunc_grid_matches_oracle(gpu_ctx, npol, nchan, flags, delay, noise_v
     g, o, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUSphFunc", {"oversample": 128},
                                             dict(kind="SphFunc"), d, N, cell, npol)
     assert rel(gg, og) < TOL
-    assert rel(gi, oi) < TOL
+    # image checked inside grid_both
     assert rel(gw, ow) < 1e-6
     s = g.stats()
     assert s.samples_gridded == o.counters.samples_gridded
@@ -144,7 +173,7 @@ def test_box_grid_and_degrid(gpu_ctx):
     N, cell = 128, 20 * ARC
     d = make_data(2000, 2, 2, N, cell)
     _, _, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUBox", {}, dict(kind="Box"), d, N, cell, 2)
-    assert rel(gg, og) < TOL and rel(gi, oi) < TOL and rel(gw, ow) < 1e-6
+    assert rel(gg, og) < TOL and rel(gw, ow) < 1e-6
     gv, ov, _, _ = degrid_both(gpu_ctx, "GPUBox", {}, dict(kind="Box"), d, N, cell, 2, random_model(N, 2))
     assert rel(gv, ov) < TOL
 
@@ -155,7 +184,7 @@ def test_psf_and_pcf_modes(gpu_ctx):
     d = make_data(1500, 1, 1, N, cell, noise_var=True)
     _, _, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUSphFunc", {"oversample": 32, "alldatapsf": True},
                                             dict(kind="SphFunc", sph_oversample=32), d, N, cell, 1, dopsf=True)
-    assert rel(gg, og) < TOL and rel(gi, oi) < TOL and rel(gw, ow) < 1e-6
+    assert rel(gg, og) < TOL and rel(gw, ow) < 1e-6
     _, _, gg, og, gi, oi, _, _ = grid_both(gpu_ctx, "GPUSphFunc", {"oversample": 32},
                                           dict(kind="SphFunc", sph_oversample=32), d, N, cell, 1, dopcf=True)
     assert rel(gg, og) < TOL
@@ -169,7 +198,7 @@ def test_mfs_weights_and_frequency_axis(gpu_ctx):
     _, _, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUSphFunc", {"oversample": 16},
                                             dict(kind="SphFunc", sph_oversample=16), d, N, cell, 1, nchan_img=3,
                                             freq_axis=fa, mfs=(1.35e9, 1))
-    assert rel(gg, og) < TOL and rel(gi, oi) < TOL and rel(gw, ow) < 1e-6
+    assert rel(gg, og) < TOL and rel(gw, ow) < 1e-6
 
 
 WP = dict(wmax=3000.0, nwplanes=9, oversample=4, maxsupport=256, cutoff=1e-2)
@@ -200,7 +229,7 @@ def test_wproject_cf_grid_degrid_match_oracle(gpu_ctx, extra):
         assert got.shape == ref.shape, k
         assert rel(got, ref) < TOL, k
     assert rel(gg, og) < TOL
-    assert rel(gi, oi) < TOL
+    # image checked inside grid_both
     assert rel(gw, ow) < 1e-6
     gv, ov, _, _ = degrid_both(gpu_ctx, "GPUWProject", params, dict(kind="WProject", wproject=prm), d, N, cell, 4,
                                random_model(N, 4))
@@ -247,7 +276,12 @@ def test_awproject_disk_matches_oracle(gpu_ctx, taper):
                     assert rel(g.cf_plane(k), o.cf.planes[k]) < TOL, (feedn, chan, iw, frac)
     gg = g.grid_array().reshape(1, 2, N, N)
     assert rel(gg, o.grid_) < TOL
-    assert rel(g.finalise_grid(), o.finalise_grid()) < TOL
+    o64 = O.OracleGridder("AWProject", "f64", wproject=prm)
+    o64.aw_lm, o64.aw_nchan = o.aw_lm, 2
+    o64.initialise_grid((N, N, 2, 1), (cell, cell))
+    o64.grid(och)
+    ok, info = image_ok(g.finalise_grid(), o.finalise_grid(), o64.finalise_grid(), gain_of(o))
+    assert ok, info
     assert rel(g.finalise_weights(), o.finalise_weights()) < TOL
     # degrid
     model = random_model(N, 2)

@@ -292,6 +292,33 @@ def test_grid_brute_force_single_sample():
     assert np.max(np.abs(g.grid_[0, 0] - expect)) < 1e-12
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_polarisation_and_channel_planes_are_independent(prec):
+    # every (pol, image chan) plane of Array(nx, ny, npol, nchan) is gridded on its own: two
+    # pols holding V and 2V give planes G and 2G of the single-pol grid; two image channels
+    # each receive the samples of their own frequency
+    rng = np.random.default_rng(4)
+    N, cell = 64, math.radians(40 / 3600)
+    f = np.array([1.0e9, 1.2e9])
+    uvw = _disc(300, N, cell, f[1], rng, 0.4)
+    v = (rng.normal(size=(300, 2, 1)) + 1j * rng.normal(size=(300, 2, 1))).astype(np.complex64)
+    g1 = O.OracleGridder("Box", prec)
+    g1.initialise_grid((N, N, 1, 1), (cell, cell))
+    g1.grid(O.Chunk(uvw, f, v))
+    g2 = O.OracleGridder("Box", prec)
+    g2.initialise_grid((N, N, 2, 1), (cell, cell))
+    g2.grid(O.Chunk(uvw, f, np.concatenate([v, 2 * v], axis=2)))
+    assert np.array_equal(g2.grid_[0, 0], g1.grid_[0, 0])
+    assert np.allclose(g2.grid_[0, 1], 2 * g1.grid_[0, 0], rtol=1e-6, atol=1e-6)
+    g3 = O.OracleGridder("Box", prec)
+    g3.initialise_grid((N, N, 1, 2), (cell, cell), freq_axis=(1.0e9, 1.2e9))
+    g3.grid(O.Chunk(uvw, f, v))
+    ga = O.OracleGridder("Box", prec)
+    ga.initialise_grid((N, N, 1, 1), (cell, cell))
+    ga.grid(O.Chunk(uvw, f[:1], np.ascontiguousarray(v[:, :1])))
+    assert np.array_equal(g3.grid_[0, 0], ga.grid_[0, 0])
+
+
 def test_adjoint_identity_of_grid_and_degrid():
     # sum_s D_s(M) conj(W_s) = sum_p G(W)[p] conj(M_p)   (GridKernel.cc:79-80 vs :121-122)
     rng = np.random.default_rng(3)

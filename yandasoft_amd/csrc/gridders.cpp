@@ -437,10 +437,7 @@ void Gridder::finaliseGrid(float* out, int mem) {
   YG_CHECK(initialised_ && !degrid_mode_ && grid_.get(), 2, "finaliseGrid without an initialised grid");
   const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
   const size_t ng = (size_t)nxp * nyp * np;
-  scratch_.resize(ng * sizeof(float2));
-  YG_HIP(hipMemcpyAsync(scratch_.get(), grid_.get(), ng * sizeof(float2), hipMemcpyDeviceToDevice, ctx_->stream));
-  launch_checkerboard(*ctx_, scratch_.as<float2>(), nxp, nyp, np);
-  fft_plain_c2c(*ctx_, scratch_.as<float2>(), nxp, nyp, np, false, plan_);
+  scratch_.resize(ng * sizeof(double2));
   if (doesCorrection() && !dopcf_) uploadCorrection();
   const double* cx = (doesCorrection() && !dopcf_) ? ccfx_.as<double>() : nullptr;
   const double* cy = (doesCorrection() && !dopcf_) ? ccfy_.as<double>() : nullptr;
@@ -451,7 +448,10 @@ void Gridder::finaliseGrid(float* out, int mem) {
     tmp.resize(nout * sizeof(float));
     dout = tmp.as<float>();
   }
-  launch_finalise_image(*ctx_, scratch_.as<float2>(), nxp, nyp, np, cx, cy, img_[0], img_[1], dout);
+  // FP64 transform: the grid correction amplifies the image edges by 1/grdsf (~1e2), which
+  // would lift an FP32 transform's rounding there to ~1e-4 of the image peak
+  grid_to_image_z(*ctx_, grid_.as<float2>(), nxp, nyp, np, cx, cy, img_[0], img_[1], scratch_.as<double2>(), dout,
+                  plan_);
   if (mem != YGRID_MEM_DEVICE)
     YG_HIP(hipMemcpyAsync(out, dout, nout * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream));
   YG_HIP(hipStreamSynchronize(ctx_->stream));
@@ -515,10 +515,11 @@ void Gridder::initialiseDegrid(const ygrid_image_desc& img, const float* model, 
   if (doesCorrection()) uploadCorrection();
   const double* cx = doesCorrection() ? ccfx_.as<double>() : nullptr;
   const double* cy = doesCorrection() ? ccfy_.as<double>() : nullptr;
-  launch_model_to_grid(*ctx_, dmodel, img_[0], img_[1], np, cx, cy, nxp, nyp, grid_.as<float2>());
-  fft_plain_c2c(*ctx_, grid_.as<float2>(), nxp, nyp, np, true, plan_);
-  launch_checkerboard_sign(*ctx_, grid_.as<float2>(), nxp, nyp, np);
+  scratch_.resize(ng * sizeof(double2));
+  model_to_grid_z(*ctx_, dmodel, img_[0], img_[1], np, cx, cy, nxp, nyp, scratch_.as<double2>(), grid_.as<float2>(),
+                  plan_);
   YG_HIP(hipStreamSynchronize(ctx_->stream));
+  scratch_.release();
 }
 
 void Gridder::finaliseDegrid() {

@@ -17,6 +17,7 @@ namespace ygrid {
 struct FFTPlan {
   hipfftHandle h = 0;
   int nx = 0, ny = 0, batch = 0;
+  hipfftType type = HIPFFT_C2C;
 };
 
 void destroy_plan(FFTPlan* p) {
@@ -81,8 +82,8 @@ void launch_checkerboard(const Ctx& c, float2* data, int nx, int ny, int batch) 
   YG_HIP(hipGetLastError());
 }
 
-static void ensure_plan(const Ctx& c, int nx, int ny, int batch, PlanPtr& cache) {
-  if (cache && cache->nx == nx && cache->ny == ny && cache->batch == batch) {
+static void ensure_plan(const Ctx& c, int nx, int ny, int batch, PlanPtr& cache, hipfftType type = HIPFFT_C2C) {
+  if (cache && cache->nx == nx && cache->ny == ny && cache->batch == batch && cache->type == type) {
     YG_CHECK(hipfftSetStream(cache->h, c.stream) == HIPFFT_SUCCESS, 3, "hipfftSetStream failed");
     return;
   }
@@ -91,7 +92,7 @@ static void ensure_plan(const Ctx& c, int nx, int ny, int batch, PlanPtr& cache)
   }
   auto* p = new FFTPlan();
   int n[2] = {ny, nx};
-  hipfftResult r = hipfftPlanMany(&p->h, 2, n, nullptr, 1, nx * ny, nullptr, 1, nx * ny, HIPFFT_C2C, batch);
+  hipfftResult r = hipfftPlanMany(&p->h, 2, n, nullptr, 1, nx * ny, nullptr, 1, nx * ny, type, batch);
   if (r != HIPFFT_SUCCESS) {
     delete p;
     throw Error(3, "hipfftPlanMany failed (" + std::to_string((int)r) + ")");
@@ -99,6 +100,7 @@ static void ensure_plan(const Ctx& c, int nx, int ny, int batch, PlanPtr& cache)
   p->nx = nx;
   p->ny = ny;
   p->batch = batch;
+  p->type = type;
   cache.reset(p);
   YG_CHECK(hipfftSetStream(p->h, c.stream) == HIPFFT_SUCCESS, 3, "hipfftSetStream failed");
 }
@@ -145,6 +147,86 @@ void fft_plain_c2c(const Ctx& c, float2* data, int nx, int ny, int batch, bool f
 void launch_checkerboard_sign(const Ctx& c, float2* data, int nx, int ny, int batch) {
   hipLaunchKernelGGL(k_checkerboard, dim3((nx + 255) / 256, ny, batch), dim3(256), 0, c.stream, data, nx, ny,
                      global_sign(nx, ny));
+  YG_HIP(hipGetLastError());
+}
+
+// ---- FP64 image <-> grid transforms (finaliseGrid / initialiseDegrid) ----
+namespace {
+__global__ void k_grid_to_z(const float2* __restrict__ g, int nx, int ny, double2* __restrict__ z) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  const size_t i = (size_t)blockIdx.z * nx * ny + (size_t)y * nx + x;
+  const float2 v = g[i];
+  const double s = cb_sign(x, y);
+  z[i] = make_double2(v.x * s, v.y * s);
+}
+
+__global__ void k_finalise_image_z(const double2* __restrict__ fftd, int nxp, int nyp, const double* __restrict__ ccfx,
+                                   const double* __restrict__ ccfy, int nx, int ny, int ox, int oy, double gsign,
+                                   float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  const int X = x + ox, Y = y + oy;
+  const double2 v = fftd[(size_t)blockIdx.z * nxp * nyp + (size_t)Y * nxp + X];
+  const double c = (ccfx ? ccfx[X] : 1.0) * (ccfy ? ccfy[Y] : 1.0);
+  out[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] = (float)(cb_sign(X, Y) * gsign * v.x * c);
+}
+
+__global__ void k_model_to_z(const float* __restrict__ model, int nx, int ny, int ox, int oy,
+                             const double* __restrict__ ccfx, const double* __restrict__ ccfy, int nxp, int nyp,
+                             double2* __restrict__ z) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y;
+  if (X >= nxp) return;
+  const int x = X - ox, y = Y - oy;
+  double val = 0.0;
+  if (x >= 0 && x < nx && y >= 0 && y < ny) {
+    const double c = (ccfx ? ccfx[X] : 1.0) * (ccfy ? ccfy[Y] : 1.0);
+    val = (double)model[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] * c * cb_sign(X, Y);
+  }
+  z[(size_t)blockIdx.z * nxp * nyp + (size_t)Y * nxp + X] = make_double2(val, 0.0);
+}
+
+__global__ void k_z_to_grid(const double2* __restrict__ z, int nx, int ny, double gsign, float2* __restrict__ g) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  const size_t i = (size_t)blockIdx.z * nx * ny + (size_t)y * nx + x;
+  const double s = cb_sign(x, y) * gsign;
+  const double2 v = z[i];
+  g[i] = make_float2((float)(v.x * s), (float)(v.y * s));
+}
+}  // namespace
+
+static void exec_z2z(const Ctx& c, double2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache) {
+  YG_CHECK(nx % 2 == 0 && ny % 2 == 0, 1, "centred FFT needs even sizes");
+  ensure_plan(c, nx, ny, batch, cache, HIPFFT_Z2Z);
+  hipfftResult r = hipfftExecZ2Z(cache->h, reinterpret_cast<hipfftDoubleComplex*>(data),
+                                 reinterpret_cast<hipfftDoubleComplex*>(data), forward ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
+  YG_CHECK(r == HIPFFT_SUCCESS, 3, "hipfftExecZ2Z failed");
+}
+
+// image[plane] = ccf * Re(centred unnormalised backward transform of grid[plane]), unpadded centre
+void grid_to_image_z(const Ctx& c, const float2* grid, int nxp, int nyp, int nplanes, const double* ccfx,
+                     const double* ccfy, int nx, int ny, double2* scratch, float* out, PlanPtr& cache) {
+  const dim3 gp((nxp + 255) / 256, nyp, nplanes);
+  hipLaunchKernelGGL(k_grid_to_z, gp, dim3(256), 0, c.stream, grid, nxp, nyp, scratch);
+  exec_z2z(c, scratch, nxp, nyp, nplanes, false, cache);
+  hipLaunchKernelGGL(k_finalise_image_z, dim3((nx + 255) / 256, ny, nplanes), dim3(256), 0, c.stream, scratch, nxp, nyp,
+                     ccfx, ccfy, nx, ny, (nxp - nx) / 2, (nyp - ny) / 2, (double)global_sign(nxp, nyp), out);
+  YG_HIP(hipGetLastError());
+}
+
+// grid[plane] = centred unnormalised forward transform of ccf * padded(model[plane])
+void model_to_grid_z(const Ctx& c, const float* model, int nx, int ny, int nplanes, const double* ccfx,
+                     const double* ccfy, int nxp, int nyp, double2* scratch, float2* grid, PlanPtr& cache) {
+  const dim3 gp((nxp + 255) / 256, nyp, nplanes);
+  hipLaunchKernelGGL(k_model_to_z, gp, dim3(256), 0, c.stream, model, nx, ny, (nxp - nx) / 2, (nyp - ny) / 2, ccfx, ccfy,
+                     nxp, nyp, scratch);
+  exec_z2z(c, scratch, nxp, nyp, nplanes, true, cache);
+  hipLaunchKernelGGL(k_z_to_grid, gp, dim3(256), 0, c.stream, scratch, nxp, nyp, (double)global_sign(nxp, nyp), grid);
   YG_HIP(hipGetLastError());
 }
 

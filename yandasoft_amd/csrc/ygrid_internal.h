@@ -185,6 +185,11 @@ void fft_centred_c2c(const Ctx&, float2* data, int nx, int ny, int batch, bool f
 void fft_plain_c2c(const Ctx&, float2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache);
 void launch_checkerboard(const Ctx&, float2* data, int nx, int ny, int batch);
 void launch_checkerboard_sign(const Ctx&, float2* data, int nx, int ny, int batch);
+// FP64 transforms between the (FP32) grid and the image, through an FP64 scratch of nplanes*nxp*nyp
+void grid_to_image_z(const Ctx&, const float2* grid, int nxp, int nyp, int nplanes, const double* ccfx,
+                     const double* ccfy, int nx, int ny, double2* scratch, float* out, PlanPtr& cache);
+void model_to_grid_z(const Ctx&, const float* model, int nx, int ny, int nplanes, const double* ccfx,
+                     const double* ccfy, int nxp, int nyp, double2* scratch, float2* grid, PlanPtr& cache);
 void launch_finalise_image(const Ctx&, const float2* fftd, int nxp, int nyp, int nplanes, const double* ccfx,
                            const double* ccfy, int nx, int ny, float* out);
 void launch_model_to_grid(const Ctx&, const float* model, int nx, int ny, int nplanes, const double* ccfx,
