@@ -88,12 +88,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nint", type=int, default=300, help="integrations (workload rows / 630)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (3x), the reference for rocprof byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md"}))
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
     import yandasoft_amd as ya
     from yandasoft_amd import workloads as W
 

@@ -726,8 +726,11 @@ class OracleGridder:
         return 1
 
     def correct(self, img):
-        """correctConvolution on a padded [.., y, x] image."""
+        """correctConvolution on a padded [.., y, x] image (none for the PCF gridder,
+        SphFuncVisGridder.cc:185)."""
         if self.kind == "Box" or (self.kind == "AWProject" and not self.wp.spheroidaltaper):
+            return img
+        if getattr(self, "mode", 0) == 2:
             return img
         alpha = self.sph[0] if self.kind == "SphFunc" else self.wp.alpha
         support = self.sph[1] if self.kind == "SphFunc" else 3
@@ -876,7 +879,7 @@ class OracleGridder:
             if not np.any(wts > 0):
                 continue
             k = iz if self.track_per_os else os_ * os_ * iz
-            cf = self.cf.planes[k]
+            cf = self.cf.planes[k].astype(np.complex64).astype(np.complex128)  # the CF cache is FP32
             S = (cf.shape[0] - 1) // 2
             pl = np.zeros((cny, cnx), dtype=np.complex128)
             for iy in range(-S, S + 1):

@@ -366,7 +366,7 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
   }
   if (cf_ && !rows.empty()) {
     const int nb = std::max(1, std::min((int)rows.size(), (int)(((size_t)1 << 28) / ((size_t)cnx * cny * 8))));
-    DevBuf planes((size_t)nb * cnx * cny * sizeof(float2)), dw((size_t)nb * ncp * sizeof(double));
+    DevBuf planes((size_t)nb * cnx * cny * sizeof(double2)), dw((size_t)nb * ncp * sizeof(double));
     DevBuf doff(nb * 4), dsup(nb * 4), dzero(nb * 4);
     PlanPtr plan;
     for (size_t r0 = 0; r0 < rows.size(); r0 += nb) {
@@ -386,22 +386,22 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
       YG_HIP(hipMemcpy(dzero.get(), zero.data(), cnt * 4, hipMemcpyHostToDevice));
       YG_HIP(hipMemcpy(dw.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice));
       launch_place_cf(c, cf_->table.as<float2>(), doff.as<int32_t>(), dsup.as<int>(), dzero.as<int>(), dzero.as<int>(),
-                      cnx, cny, cnt, planes.as<float2>());
+                      cnx, cny, cnt, planes.as<double2>());
       // fft2d(thisPlane, false) * (cnx*cny): the unnormalised backward transform
-      fft_plain_c2c(c, planes.as<float2>(), cnx, cny, cnt, false, plan);
-      launch_aw_weights_acc(c, planes.as<float2>(), cnt, cnx, cny, dw.as<double>(), ncp,
+      fft_plain_z2z(c, planes.as<double2>(), cnx, cny, cnt, false, plan);
+      launch_aw_weights_acc(c, planes.as<double2>(), cnt, cnx, cny, dw.as<double>(), ncp,
                             sph_taper_ ? dtx.as<double>() : nullptr, sph_taper_ ? dty.as<double>() : nullptr,
                             cout.as<double>());
     }
   }
   // PaddingUtils::fftPad onto the padded image, then the unpadded centre
   // (the centred backward transform is unnormalised: its 1/(cnx cny) is applied on conversion)
-  DevBuf small((size_t)cnx * cny * ncp * sizeof(float2)), big((size_t)nx * ny * ncp * sizeof(float2));
-  launch_d2c(c, cout.as<double>(), (size_t)cnx * cny * ncp, 1.0 / (double(cnx) * cny), small.as<float2>());
+  DevBuf small((size_t)cnx * cny * ncp * sizeof(double2)), big((size_t)nx * ny * ncp * sizeof(double2));
+  launch_d2c(c, cout.as<double>(), (size_t)cnx * cny * ncp, 1.0 / (double(cnx) * cny), small.as<double2>());
   PlanPtr p1, p2;
-  fft_centred_c2c(c, small.as<float2>(), cnx, cny, ncp, false, p1);
-  launch_pad_centre(c, small.as<float2>(), cnx, cny, nx, ny, ncp, big.as<float2>());
-  fft_centred_c2c(c, big.as<float2>(), nx, ny, ncp, true, p2);
+  fft_centred_z2z(c, small.as<double2>(), cnx, cny, ncp, false, p1);
+  launch_pad_centre(c, small.as<double2>(), cnx, cny, nx, ny, ncp, big.as<double2>());
+  fft_centred_z2z(c, big.as<double2>(), nx, ny, ncp, true, p2);
   const size_t nout = (size_t)img_[0] * img_[1] * ncp;
   DevBuf tmp;
   float* dout = out;
@@ -409,7 +409,7 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
     tmp.resize(nout * sizeof(float));
     dout = tmp.as<float>();
   }
-  launch_extract_real(c, big.as<float2>(), nx, ny, img_[0], img_[1], ncp, dout);
+  launch_extract_real(c, big.as<double2>(), nx, ny, img_[0], img_[1], ncp, dout);
   if (mem != YGRID_MEM_DEVICE) YG_HIP(hipMemcpyAsync(out, dout, nout * sizeof(float), hipMemcpyDeviceToHost, c.stream));
   YG_HIP(hipStreamSynchronize(c.stream));
 }

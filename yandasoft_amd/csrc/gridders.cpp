@@ -618,14 +618,17 @@ class SphFuncGridder : public Gridder {
     if (cf_) return;
     auto t = std::make_shared<CFTable>();
     if (dopcf_) {
-      // nearest-neighbour (1, 3i) kernel (SphFuncVisGridder.cc:99-125)
+      // nearest-neighbour (1, 3i) kernel (SphFuncVisGridder.cc:99-125), normalised by the sum
+      // of |C| like every SphFunc plane (SphFuncVisGridder.cc:159-167): (1, 3)/sqrt(10)
       layout_table(*t, {1}, os_);
       std::vector<float2> host(t->n_elems, make_float2(0.f, 0.f));
+      const double nrm = std::sqrt(10.0);
       for (int fu = 0; fu < os_; ++fu)
         for (int fv = 0; fv < os_; ++fv) {
           const int k = fu + os_ * fv;
           const int ix = (int)(-float(fu) / float(os_)), iy = (int)(-float(fv) / float(os_));
-          host[t->off[k] + (size_t)(iy + 1 + kGuardV) * cf_pitch(1) + ix + 1 + kGuardU] = make_float2(1.f, 3.f);
+          host[t->off[k] + (size_t)(iy + 1 + kGuardV) * cf_pitch(1) + ix + 1 + kGuardU] =
+              make_float2((float)(1.0 / nrm), (float)(3.0 / nrm));
         }
       t->table.resize(host.size() * sizeof(float2));
       YG_HIP(hipMemcpy(t->table.get(), host.data(), host.size() * sizeof(float2), hipMemcpyHostToDevice));

@@ -71,17 +71,18 @@ __global__ void k_scale_sign(float2* data, int nx, int ny, float gsign, float sc
 // (AWProjectVisGridder.cc:683-697), pre-multiplied by the centred backward transform's checkerboard.
 __global__ void k_place_cf(const float2* __restrict__ table, const int32_t* __restrict__ off, const int* __restrict__ supp,
                            const int* __restrict__ ou, const int* __restrict__ ov, int cnx, int cny,
-                           float2* __restrict__ out) {
+                           double2* __restrict__ out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, b = blockIdx.z;
   if (x >= cnx) return;
   const int S = supp[b];
   const int ix = x - cnx / 2 - ou[b], iy = y - cny / 2 - ov[b];
-  float2 v = make_float2(0.f, 0.f);
+  double2 v = make_double2(0.0, 0.0);
   if (ix >= -S && ix <= S && iy >= -S && iy <= S) {
     const int P = 2 * S + 1 + 2 * kGuardU;
-    v = table[off[b] + (size_t)(iy + S + kGuardV) * P + ix + S + kGuardU];
-    const float s = cb_sign(x, y);
+    const float2 t = table[off[b] + (size_t)(iy + S + kGuardV) * P + ix + S + kGuardU];
+    const double s = cb_sign(x, y);
+    v = make_double2(t.x, t.y);
     v.x *= s;
     v.y *= s;
   }
@@ -90,7 +91,7 @@ __global__ void k_place_cf(const float2* __restrict__ table, const int32_t* __re
 
 // cOut[cp] += sum_b wt[b][cp] |val_b / taper|^2 where val_b is the raw transform of plane b
 // (centring sign drops out of |.|^2).
-__global__ void k_aw_weights_acc(const float2* __restrict__ planes, int nb, int cnx, int cny, const double* __restrict__ wt,
+__global__ void k_aw_weights_acc(const double2* __restrict__ planes, int nb, int cnx, int cny, const double* __restrict__ wt,
                                  int ncp, const double* __restrict__ tx, const double* __restrict__ ty,
                                  double* __restrict__ cout) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,7 +103,7 @@ __global__ void k_aw_weights_acc(const float2* __restrict__ planes, int nb, int 
   for (int b = 0; b < nb; ++b) {
     const double w = wt[(size_t)b * ncp + cp];
     if (w == 0.0) continue;
-    const float2 v = planes[(size_t)b * cnx * cny + (size_t)y * cnx + x];
+    const double2 v = planes[(size_t)b * cnx * cny + (size_t)y * cnx + x];
     double vr = v.x, vi = v.y;
     if (tx) {
       if (t > 0) { vr /= t; vi /= t; } else { vr = vi = 0.0; }
@@ -113,28 +114,28 @@ __global__ void k_aw_weights_acc(const float2* __restrict__ planes, int nb, int 
 }
 
 // Zero-padding of centred spectra: out (nx x ny) <- in (cnx x cny) at the centre.
-__global__ void k_pad_centre(const float2* __restrict__ in, int cnx, int cny, int nx, int ny, float2* __restrict__ out) {
+__global__ void k_pad_centre(const double2* __restrict__ in, int cnx, int cny, int nx, int ny, double2* __restrict__ out) {
   const int X = blockIdx.x * blockDim.x + threadIdx.x;
   const int Y = blockIdx.y;
   if (X >= nx) return;
   const int x = X - (nx - cnx) / 2, y = Y - (ny - cny) / 2;
-  float2 v = make_float2(0.f, 0.f);
+  double2 v = make_double2(0.0, 0.0);
   if (x >= 0 && x < cnx && y >= 0 && y < cny) v = in[(size_t)blockIdx.z * cnx * cny + (size_t)y * cnx + x];
   out[(size_t)blockIdx.z * nx * ny + (size_t)Y * nx + X] = v;
 }
 
-__global__ void k_d2c(const double* __restrict__ in, size_t n, double scale, float2* __restrict__ out) {
+__global__ void k_d2c(const double* __restrict__ in, size_t n, double scale, double2* __restrict__ out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i < n) out[i] = make_float2((float)(in[i] * scale), 0.f);
+  if (i < n) out[i] = make_double2(in[i] * scale, 0.0);
 }
 
 // out[plane][y][x] = Re(in[plane][y+oy][x+ox])
-__global__ void k_extract_real(const float2* __restrict__ in, int nxp, int nyp, int nx, int ny, float* __restrict__ out) {
+__global__ void k_extract_real(const double2* __restrict__ in, int nxp, int nyp, int nx, int ny, float* __restrict__ out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= nx) return;
   const int ox = (nxp - nx) / 2, oy = (nyp - ny) / 2;
-  out[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] = in[(size_t)blockIdx.z * nxp * nyp + (size_t)(y + oy) * nxp + x + ox].x;
+  out[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] = (float)in[(size_t)blockIdx.z * nxp * nyp + (size_t)(y + oy) * nxp + x + ox].x;
 }
 
 }  // namespace
@@ -160,31 +161,31 @@ void launch_scale_sign(const Ctx& c, float2* data, int nx, int ny, int nb, float
 }
 
 void launch_place_cf(const Ctx& c, const float2* table, const int32_t* off, const int* supp, const int* ou, const int* ov,
-                     int cnx, int cny, int nb, float2* out) {
+                     int cnx, int cny, int nb, double2* out) {
   hipLaunchKernelGGL(k_place_cf, dim3((cnx + 255) / 256, cny, nb), dim3(256), 0, c.stream, table, off, supp, ou, ov, cnx,
                      cny, out);
   YG_HIP(hipGetLastError());
 }
 
-void launch_aw_weights_acc(const Ctx& c, const float2* planes, int nb, int cnx, int cny, const double* wt, int ncp,
+void launch_aw_weights_acc(const Ctx& c, const double2* planes, int nb, int cnx, int cny, const double* wt, int ncp,
                            const double* tx, const double* ty, double* cout) {
   hipLaunchKernelGGL(k_aw_weights_acc, dim3((cnx + 255) / 256, cny, ncp), dim3(256), 0, c.stream, planes, nb, cnx, cny,
                      wt, ncp, tx, ty, cout);
   YG_HIP(hipGetLastError());
 }
 
-void launch_pad_centre(const Ctx& c, const float2* in, int cnx, int cny, int nx, int ny, int nb, float2* out) {
+void launch_pad_centre(const Ctx& c, const double2* in, int cnx, int cny, int nx, int ny, int nb, double2* out) {
   hipLaunchKernelGGL(k_pad_centre, dim3((nx + 255) / 256, ny, nb), dim3(256), 0, c.stream, in, cnx, cny, nx, ny, out);
   YG_HIP(hipGetLastError());
 }
 
-void launch_d2c(const Ctx& c, const double* in, size_t n, double scale, float2* out) {
+void launch_d2c(const Ctx& c, const double* in, size_t n, double scale, double2* out) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_d2c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, in, n, scale, out);
   YG_HIP(hipGetLastError());
 }
 
-void launch_extract_real(const Ctx& c, const float2* in, int nxp, int nyp, int nx, int ny, int nb, float* out) {
+void launch_extract_real(const Ctx& c, const double2* in, int nxp, int nyp, int nx, int ny, int nb, float* out) {
   hipLaunchKernelGGL(k_extract_real, dim3((nx + 255) / 256, ny, nb), dim3(256), 0, c.stream, in, nxp, nyp, nx, ny, out);
   YG_HIP(hipGetLastError());
 }

@@ -200,12 +200,37 @@ __global__ void k_z_to_grid(const double2* __restrict__ z, int nx, int ny, doubl
 }
 }  // namespace
 
-static void exec_z2z(const Ctx& c, double2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache) {
+void fft_plain_z2z(const Ctx& c, double2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache) {
   YG_CHECK(nx % 2 == 0 && ny % 2 == 0, 1, "centred FFT needs even sizes");
   ensure_plan(c, nx, ny, batch, cache, HIPFFT_Z2Z);
   hipfftResult r = hipfftExecZ2Z(cache->h, reinterpret_cast<hipfftDoubleComplex*>(data),
                                  reinterpret_cast<hipfftDoubleComplex*>(data), forward ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
   YG_CHECK(r == HIPFFT_SUCCESS, 3, "hipfftExecZ2Z failed");
+}
+static void exec_z2z(const Ctx& c, double2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache) {
+  fft_plain_z2z(c, data, nx, ny, batch, forward, cache);
+}
+
+namespace {
+__global__ void k_checkerboard_z(double2* data, int nx, int ny, double gsign) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  double2* p = data + (size_t)blockIdx.z * nx * ny + (size_t)y * nx + x;
+  const double s = cb_sign(x, y) * gsign;
+  double2 v = *p;
+  v.x *= s;
+  v.y *= s;
+  *p = v;
+}
+}  // namespace
+
+void fft_centred_z2z(const Ctx& c, double2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache) {
+  const dim3 g((nx + 255) / 256, ny, batch);
+  hipLaunchKernelGGL(k_checkerboard_z, g, dim3(256), 0, c.stream, data, nx, ny, 1.0);
+  fft_plain_z2z(c, data, nx, ny, batch, forward, cache);
+  hipLaunchKernelGGL(k_checkerboard_z, g, dim3(256), 0, c.stream, data, nx, ny, (double)global_sign(nx, ny));
+  YG_HIP(hipGetLastError());
 }
 
 // image[plane] = ccf * Re(centred unnormalised backward transform of grid[plane]), unpadded centre

@@ -298,8 +298,23 @@ __global__ void k_item_emit(const int* __restrict__ counts, const int* __restric
 
 // --------------------------------------------------------------------------------------
 // grid: lane layout in a 64x64 tile.  Wave w owns rows [16w, 16w+16); lane (lu, lv) =
-// (lane & 15, lane >> 4) owns pixels u = 16j + lu, v = 16w + 4k + lv for j, k in 0..3.
+// (lane & 15, lane >> 4) owns pixels u = 16j + lu, v = 16w + 4k + lv for j, k in 0..3, i.e.
+// 16 "blocks" (k, j) of 16x4 pixels per wave.  A (sample, tile) pair touches the blocks its
+// footprint overlaps; all of a pair's CF loads are issued before its FMAs so that one L2
+// latency is paid per pair rather than per block.
 // --------------------------------------------------------------------------------------
+
+// Complex MAC coefficients of one sample and polarisation: acc_re += a*c.x + b*c.y and
+// acc_im += cc*c.x + d*c.y.  Plain: (v_r, -v_i, v_i, v_r) = v*C; PCF half plane: the same
+// with conj(C) (TableVisGridder.cc:837-844).
+struct MacCoef {
+  float a, b, c, d;
+};
+
+__device__ __forceinline__ bool block_active(int dvb, int dub, int S2) {
+  return !(dvb + 3 < 0 || dvb > S2 || dub + 15 < 0 || dub > S2);
+}
+
 template <int NP>
 __global__ __launch_bounds__(256) void k_grid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
                                               int* __restrict__ counter, const int* __restrict__ pairs,
@@ -337,31 +352,34 @@ __global__ __launch_bounds__(256) void k_grid(const int4* __restrict__ items, co
       const int P = r.spack >> 16;
       const int du0 = u0 - r.iu0;            // footprint column of tile column 0
       const int dvw = v0 - r.iv0 + 16 * w;   // footprint row of this wave's first row
-      float vr[NP], vi[NP];
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        vr[q] = r.v[2 * q];
-        vi[q] = r.v[2 * q + 1];
-      }
       const float2* base = cf + r.cfoff + (long long)(dvw + kGuardV + lv) * P + (du0 + kGuardU + lu);
+      float2 c[4][4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (block_active(dvw + 4 * k, du0 + 16 * j, S2)) c[k][j] = base[(long long)4 * k * P + 16 * j];
+      MacCoef m[NP];
       const bool cj = r.conj != 0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int dvb = dvw + 4 * k;
-        if (dvb + 3 < 0 || dvb > S2) continue;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int dub = du0 + 16 * j;
-          if (dub + 15 < 0 || dub > S2) continue;
-          float2 c = base[(long long)4 * k * P + 16 * j];
-          if (cj) c.y = -c.y;  // PCF: conj(CF) in half of the uv plane (TableVisGridder.cc:837-844)
-#pragma unroll
-          for (int q = 0; q < NP; ++q) {
-            acc[k][j][q][0] += vr[q] * c.x - vi[q] * c.y;
-            acc[k][j][q][1] += vr[q] * c.y + vi[q] * c.x;
-          }
-        }
+      for (int q = 0; q < NP; ++q) {
+        const float vr = r.v[2 * q], vi = r.v[2 * q + 1];
+        m[q] = cj ? MacCoef{vr, vi, vi, -vr} : MacCoef{vr, -vi, vi, vr};
       }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (block_active(dvw + 4 * k, du0 + 16 * j, S2)) {
+            const float2 cc = c[k][j];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+              acc[k][j][q][0] = fmaf(m[q].a, cc.x, acc[k][j][q][0]);
+              acc[k][j][q][0] = fmaf(m[q].b, cc.y, acc[k][j][q][0]);
+              acc[k][j][q][1] = fmaf(m[q].c, cc.x, acc[k][j][q][1]);
+              acc[k][j][q][1] = fmaf(m[q].d, cc.y, acc[k][j][q][1]);
+            }
+          }
     }
     // flush
     const size_t plane = (size_t)nx * ny;
@@ -388,6 +406,18 @@ __global__ __launch_bounds__(256) void k_grid(const int4* __restrict__ items, co
 // degrid
 // --------------------------------------------------------------------------------------
 constexpr int kDegridBatch = 32;
+
+// Sum over the 64 lanes of a wave with DPP (row shifts, then row broadcasts); the total is
+// returned wave-uniformly (read from lane 63).
+__device__ __forceinline__ float wave_sum_dpp(float x) {
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x111, 0xf, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x112, 0xf, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x114, 0xf, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x118, 0xf, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x142, 0xa, 0xf, true));
+  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x143, 0xc, 0xf, true));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+}
 
 template <int NP>
 __global__ __launch_bounds__(256) void k_degrid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
@@ -437,38 +467,42 @@ __global__ __launch_bounds__(256) void k_degrid(const int4* __restrict__ items, 
         const int P = r.spack >> 16;
         const int du0 = u0 - r.iu0;
         const int dvw = v0 - r.iv0 + 16 * w;
+        const float2* base = cf + r.cfoff + (long long)(dvw + kGuardV + lv) * P + (du0 + kGuardU + lu);
+        float2 c[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (block_active(dvw + 4 * k, du0 + 16 * j, S2)) c[k][j] = base[(long long)4 * k * P + 16 * j];
         float a[NP][2];
 #pragma unroll
         for (int q = 0; q < NP; ++q) a[q][0] = a[q][1] = 0.f;
-        const float2* base = cf + r.cfoff + (long long)(dvw + kGuardV + lv) * P + (du0 + kGuardU + lu);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int dvb = dvw + 4 * k;
-          if (dvb + 3 < 0 || dvb > S2) continue;
+        for (int k = 0; k < 4; ++k)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int dub = du0 + 16 * j;
-            if (dub + 15 < 0 || dub > S2) continue;
-            const float2 c = base[(long long)4 * k * P + 16 * j];
+          for (int j = 0; j < 4; ++j)
+            if (block_active(dvw + 4 * k, du0 + 16 * j, S2)) {
+              const float2 cc = c[k][j];
 #pragma unroll
-            for (int q = 0; q < NP; ++q) {
-              // CF * conj(G)
-              a[q][0] += c.x * g[k][j][q].x + c.y * g[k][j][q].y;
-              a[q][1] += c.y * g[k][j][q].x - c.x * g[k][j][q].y;
+              for (int q = 0; q < NP; ++q) {
+                // CF * conj(G)  (GridKernel.cc:121-122)
+                a[q][0] = fmaf(cc.x, g[k][j][q].x, a[q][0]);
+                a[q][0] = fmaf(cc.y, g[k][j][q].y, a[q][0]);
+                a[q][1] = fmaf(cc.y, g[k][j][q].x, a[q][1]);
+                a[q][1] = fmaf(-cc.x, g[k][j][q].y, a[q][1]);
+              }
             }
-          }
-        }
 #pragma unroll
         for (int q = 0; q < NP; ++q) {
-          a[q][0] = wave_sum_f32(a[q][0]);
-          a[q][1] = wave_sum_f32(a[q][1]);
+          a[q][0] = wave_sum_dpp(a[q][0]);
+          a[q][1] = wave_sum_dpp(a[q][1]);
         }
-        if (lane == 0) {
+        if (lane < 2 * NP) {
+          float v = a[0][0];
 #pragma unroll
-          for (int q = 0; q < NP; ++q) {
-            s_part[b][w][2 * q] = a[q][0];
-            s_part[b][w][2 * q + 1] = a[q][1];
-          }
+          for (int q = 0; q < 2 * NP; ++q)
+            if (lane == q) v = a[q / 2][q % 2];
+          s_part[b][w][lane] = v;
         }
       }
       __syncthreads();

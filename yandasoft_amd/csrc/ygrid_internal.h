@@ -225,12 +225,15 @@ void launch_aw_screen(const Ctx&, const float2* A, int nx, int ny, int qnx, int 
 // multiply by the output checkerboard of a centred transform (incl. its global sign) and a scale
 void launch_scale_sign(const Ctx&, float2* data, int nx, int ny, int nb, float scale);
 void launch_place_cf(const Ctx&, const float2* table, const int32_t* off, const int* supp, const int* ou,
-                     const int* ov, int cnx, int cny, int nb, float2* out);
-void launch_aw_weights_acc(const Ctx&, const float2* planes, int nb, int cnx, int cny, const double* wt, int ncp,
+                     const int* ov, int cnx, int cny, int nb, double2* out);
+void launch_aw_weights_acc(const Ctx&, const double2* planes, int nb, int cnx, int cny, const double* wt, int ncp,
                            const double* tx, const double* ty, double* cout);
-void launch_pad_centre(const Ctx&, const float2* in, int cnx, int cny, int nx, int ny, int nb, float2* out);
-void launch_d2c(const Ctx&, const double* in, size_t n, double scale, float2* out);
-void launch_extract_real(const Ctx&, const float2* in, int nxp, int nyp, int nx, int ny, int nb, float* out);
+void launch_pad_centre(const Ctx&, const double2* in, int cnx, int cny, int nx, int ny, int nb, double2* out);
+void launch_d2c(const Ctx&, const double* in, size_t n, double scale, double2* out);
+void launch_extract_real(const Ctx&, const double2* in, int nxp, int nyp, int nx, int ny, int nb, float* out);
+// plain / centred FP64 transforms
+void fft_plain_z2z(const Ctx&, double2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache);
+void fft_centred_z2z(const Ctx&, double2* data, int nx, int ny, int batch, bool forward, PlanPtr& cache);
 
 // Host helpers (spheroidal.cpp)
 class Spheroidal {
