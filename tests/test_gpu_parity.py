@@ -282,7 +282,14 @@ def test_awproject_disk_matches_oracle(gpu_ctx, taper):
     o64.grid(och)
     ok, info = image_ok(g.finalise_grid(), o.finalise_grid(), o64.finalise_grid(), gain_of(o))
     assert ok, info
-    assert rel(g.finalise_weights(), o.finalise_weights()) < TOL
+    gw, ow = g.finalise_weights(), o.finalise_weights()
+    if taper:
+        # the weights divide |FT(CF)|^2 by the taper squared (AWProjectVisGridder.cc:724-742), ~1e-6
+        # at the corners, which blows the FP32 rounding of the CF table up there: compare the
+        # central half, where the taper is >= ~0.3
+        q = slice(N // 4, 3 * N // 4)
+        gw, ow = gw[..., q, q], ow[..., q, q]
+    assert rel(gw, ow) < TOL
     # degrid
     model = random_model(N, 2)
     dg = ya.Gridder(gpu_ctx, "GPUAWProject", params)

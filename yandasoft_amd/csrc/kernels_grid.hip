@@ -315,8 +315,8 @@ __device__ __forceinline__ bool block_active(int dvb, int dub, int S2) {
   return !(dvb + 3 < 0 || dvb > S2 || dub + 15 < 0 || dub > S2);
 }
 
-template <int NP>
-__global__ __launch_bounds__(256) void k_grid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
+template <int NP, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
                                               int* __restrict__ counter, const int* __restrict__ pairs,
                                               const SampleRec* __restrict__ recs, const float2* __restrict__ cf,
                                               float* __restrict__ grid, int nx, int ny, int ntu, int ntiles, int nzg) {
@@ -419,8 +419,8 @@ __device__ __forceinline__ float wave_sum_dpp(float x) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
 }
 
-template <int NP>
-__global__ __launch_bounds__(256) void k_degrid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
+template <int NP, int WPE>
+__global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
                                                 int* __restrict__ counter, const int* __restrict__ pairs,
                                                 const SampleRec* __restrict__ recs, const float2* __restrict__ cf,
                                                 const float2* __restrict__ grid, int nx, int ny, int ntu, int ntiles,
@@ -592,7 +592,7 @@ void launch_binning(const Ctx& c, const SampleRec* recs, int n, int nx, int ny, 
   YG_HIP(hipGetLastError());
 }
 
-static int persistent_blocks(const Ctx& c) { return c.n_cu * 4; }
+static int persistent_blocks(const Ctx& c) { return c.n_cu * (c.waves_per_simd >= 3 ? 3 : 2); }
 
 void launch_grid(const Ctx& c, const Binning& b, const SampleRec* recs, const float2* cf, float2* grid, int nx, int ny,
                  int npol, int nzg) {
@@ -603,8 +603,12 @@ void launch_grid(const Ctx& c, const Binning& b, const SampleRec* recs, const fl
   float* gf = reinterpret_cast<float*>(grid);
 #define YG_GRID_CASE(NPV)                                                                                       \
   case NPV:                                                                                                     \
-    hipLaunchKernelGGL(k_grid<NPV>, g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(), b.counter.as<int>(), \
-                       b.pairs.as<int>(), recs, cf, gf, nx, ny, ntu, ntiles, nzg);                              \
+    if (c.waves_per_simd >= 3)                                                                                  \
+      hipLaunchKernelGGL((k_grid<NPV, 3>), g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(),         \
+                         b.counter.as<int>(), b.pairs.as<int>(), recs, cf, gf, nx, ny, ntu, ntiles, nzg);       \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_grid<NPV, 2>), g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(),         \
+                         b.counter.as<int>(), b.pairs.as<int>(), recs, cf, gf, nx, ny, ntu, ntiles, nzg);       \
     break;
   switch (npol) {
     YG_GRID_CASE(1)
@@ -624,10 +628,14 @@ void launch_degrid(const Ctx& c, const Binning& b, const SampleRec* recs, const 
   const int ntiles = ntu * ntv;
   YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
   const dim3 g(persistent_blocks(c)), t(256);
-#define YG_DEGRID_CASE(NPV)                                                                                       \
-  case NPV:                                                                                                       \
-    hipLaunchKernelGGL(k_degrid<NPV>, g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(), b.counter.as<int>(), \
-                       b.pairs.as<int>(), recs, cf, grid, nx, ny, ntu, ntiles, nzg, acc);                         \
+#define YG_DEGRID_CASE(NPV)                                                                                     \
+  case NPV:                                                                                                     \
+    if (c.waves_per_simd >= 3)                                                                                  \
+      hipLaunchKernelGGL((k_degrid<NPV, 3>), g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(),       \
+                         b.counter.as<int>(), b.pairs.as<int>(), recs, cf, grid, nx, ny, ntu, ntiles, nzg, acc); \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_degrid<NPV, 2>), g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(),       \
+                         b.counter.as<int>(), b.pairs.as<int>(), recs, cf, grid, nx, ny, ntu, ntiles, nzg, acc); \
     break;
   switch (npol) {
     YG_DEGRID_CASE(1)

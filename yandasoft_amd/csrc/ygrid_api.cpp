@@ -2,6 +2,7 @@
 // C++ exceptions and turns them into a status code plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -55,6 +56,7 @@ int ygrid_ctx_create(int device, ygrid_ctx** out) {
     hipDeviceProp_t prop;
     YG_HIP(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (const char* v = std::getenv("YGRID_WAVES_PER_SIMD")) c->waves_per_simd = std::atoi(v);
     YG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
     *out = new ygrid_ctx{c};

@@ -160,6 +160,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int n_cu = 256;
+  int waves_per_simd = 3;  // register budget of the grid/degrid kernels (2: <=256 VGPRs, 3: <=168)
 };
 
 // Kernel launchers (kernels_grid.hip)
