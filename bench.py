@@ -27,6 +27,23 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
+PROFILE = "profiles/k_grid_k_degrid_wproject_array.md"
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) recorded by
+    the rocprofv3 --pmc runs of this benchmark summarised in PROFILE; None if not recorded."""
+    try:
+        with open(os.path.join(ROOT, PROFILE)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -164,10 +181,11 @@ def main():
     dk_ms = s1d.ms_degrid_kernel / max(1.0, s1d.n_degrid_launches)
     per_launch_g = taps_g / args.steps
     per_launch_d = taps_d / args.steps
+    wps = 3 if int(os.environ.get("YGRID_WAVES_PER_SIMD", "3")) >= 3 else 2
     if gk_ms >= dk_ms:
-        kname, kms, kt = "k_grid<4>", gk_ms, per_launch_g
+        kname, kms, kt = f"k_grid<4,{wps}>", gk_ms, per_launch_g
     else:
-        kname, kms, kt = "k_degrid<4>", dk_ms, per_launch_d
+        kname, kms, kt = f"k_degrid<4,{wps}>", dk_ms, per_launch_d
     achieved = 8.0 * kt / (kms * 1e-3) / 1e12
     result = {
         "metric": "CF-tap updates/s (W-proj grid+degrid)",
@@ -192,7 +210,8 @@ def main():
                                "prep": (s1g.ms_prep + s1d.ms_prep) / args.steps,
                                "binning": (s1g.ms_bin + s1d.ms_bin) / args.steps},
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": profiled_traffic(kname),
+                     "traffic_source": PROFILE},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
