@@ -326,7 +326,9 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   }
 
   const int nsamp = d.nrow * d.nchan;
-  recs_.resize((size_t)nsamp * sizeof(SampleRec));
+  smp_hdr_.resize((size_t)nsamp * sizeof(SampleHdr));
+  smp_key_.resize((size_t)nsamp * sizeof(int32_t));
+  smp_coef_.resize((size_t)nsamp * kCoef * sizeof(float));
   counters_.resize(8 * sizeof(unsigned long long));
   err_.resize(sizeof(int) * 2);
   YG_HIP(hipMemsetAsync(counters_.get(), 0, 8 * sizeof(unsigned long long), ctx_->stream));
@@ -371,26 +373,29 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   in.cf_ofs_v = cf_->ofs_v.empty() ? nullptr : cf_->d_ofs_v.as<int32_t>();
   in.n_cf = (int)cf_->off.size();
   PrepOutputs out{};
-  out.recs = recs_.as<SampleRec>();
+  const Samples smp = samples();
+  out.smp = smp;
   out.sumw = forward ? nullptr : sumw_.as<double>();
   out.counters = counters_.as<unsigned long long>();
   out.error = err_.as<int>();
   if (profiling_) YG_HIP(hipEventRecord(ev_[0], ctx_->stream));
   launch_prep(*ctx_, pp, in, out);
   if (profiling_) YG_HIP(hipEventRecord(ev_[1], ctx_->stream));
-  launch_binning(*ctx_, recs_.as<SampleRec>(), nsamp, shape_[0], shape_[1], pp.nzg * shape_[3], bin_);
+  launch_binning(*ctx_, smp, nsamp, shape_[0], shape_[1], pp.nzg * shape_[3], bin_);
   const float2* cft = cf_->table.as<float2>();
+  const size_t cfb = cf_->table.size();
+  const int smax = cf_->supp.empty() ? 0 : *std::max_element(cf_->supp.begin(), cf_->supp.end());
   if (forward) {
     acc_.resize((size_t)nsamp * 8 * sizeof(float));
     YG_HIP(hipMemsetAsync(acc_.get(), 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_degrid(*ctx_, bin_, recs_.as<SampleRec>(), cft, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg,
+    launch_degrid(*ctx_, bin_, smp, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg,
                   acc_.as<float>());
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
-    launch_degrid_finish(*ctx_, recs_.as<SampleRec>(), acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);
+    launch_degrid_finish(*ctx_, smp, acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);
   } else {
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_grid(*ctx_, bin_, recs_.as<SampleRec>(), cft, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg);
+    launch_grid(*ctx_, bin_, smp, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
   }
   unsigned long long cnt[4] = {0, 0, 0, 0};

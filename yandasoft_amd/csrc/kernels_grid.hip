@@ -2,7 +2,7 @@
 //
 // Pass structure (one call of TableVisGridder::generic, TableVisGridder.cc:451-884):
 //   k_prep        one thread per (row, chan): uvw -> (iu, fracu, iv, fracv) in FP64, CF plane,
-//                 on-grid test, flags, noise weight, sum of weights; writes a SampleRec.
+//                 on-grid test, flags, noise weight, sum of weights; writes the sample header, key and coefficients.
 //   k_bin_*       counting sort of (sample, grid tile) pairs by (image chan, CF plane group, tile);
 //                 the key order is plane-major so that concurrently running workgroups read the
 //                 same few CF planes, which then stay resident in each XCD's L2.
@@ -82,16 +82,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
   const int chan = inb ? s / p.nrow : 0;
   const bool grid_mode = p.mode != kModeDegrid;
 
-  SampleRec rec;
+  SampleHdr rec;
   rec.iu0 = INT_MIN;
   rec.iv0 = 0;
   rec.cfoff = 0;
   rec.spack = 0;
-  rec.key = 0;
-  rec.conj = 0;
-  rec.pad0 = rec.pad1 = 0;
+  int key = 0;
+  bool conj = false;
+  float coef[kCoef];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) rec.v[q] = 0.f;
+  for (int q = 0; q < kCoef; ++q) coef[q] = 0.f;
 
   bool valid = inb;
   bool flagged = false, wflagged = false;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
             rec.iv0 = ivo - S;
             rec.cfoff = in.cf_off[cInd];
             rec.spack = S | ((2 * S + 1 + 2 * kGuardU) << 16);
-            rec.key = ichan * p.nzg + plane / p.zdiv;
+            key = ichan * p.nzg + plane / p.zdiv;
             swrow = p.track_per_os ? cInd : plane;
             const float vw = in.visweight ? in.visweight[chan] : 1.0f;
             if (grid_mode) {
@@ -169,34 +169,40 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
                 if ((in.feed1 ? in.feed1[row] : 0) != p.psf_feed) valid = false;
               }
               if (p.mode == kModePCF) {
-                rec.conj = (ivo < p.ny / 2 && iuo >= p.nx / 2) || (ivo <= p.ny / 2 && iuo < p.nx / 2);
+                conj = (ivo < p.ny / 2 && iuo >= p.nx / 2) || (ivo <= p.ny / 2 && iuo < p.nx / 2);
               }
-              for (int pol = 0; pol < p.npol && valid; ++pol) {
+#pragma unroll
+              for (int pol = 0; pol < 4; ++pol) {
+                if (pol >= p.npol || !valid) break;
                 const size_t ci = row + (size_t)p.nrow * (chan + (size_t)p.nchan * pol);
                 const float nre = in.noise ? in.noise[ci].x : 1.0f;
                 const float visNoise = nre * nre;
                 const float wt = (visNoise > 0.f) ? (float)(1. / (double)visNoise) : 0.f;
                 if (!(wt > 0.f)) { err |= 1; valid = false; break; }
                 wts[pol] = wt;
+                float rr = 0.f, ri = 0.f;
                 if (p.mode == kModeGrid) {
                   const float2 v = in.vis[ci];
                   // rVis = phasor * conj(vis) * wt (* visweight)  (TableVisGridder.cc:797-800)
-                  float rr = (ph_r * v.x + ph_i * v.y) * wt;
-                  float ri = (ph_i * v.x - ph_r * v.y) * wt;
+                  rr = (ph_r * v.x + ph_i * v.y) * wt;
+                  ri = (ph_i * v.x - ph_r * v.y) * wt;
                   rr *= vw;
                   ri *= vw;
-                  rec.v[2 * pol] = rr;
-                  rec.v[2 * pol + 1] = ri;
                 } else if (p.mode == kModePSF) {
-                  rec.v[2 * pol] = wt * vw;
+                  rr = wt * vw;
                 } else {
-                  rec.v[2 * pol] = wt;
+                  rr = wt;
                 }
+                // MAC coefficients of rVis*C, or rVis*conj(C) (TableVisGridder.cc:837-844)
+                coef[4 * pol + 0] = rr;
+                coef[4 * pol + 1] = conj ? ri : -ri;
+                coef[4 * pol + 2] = ri;
+                coef[4 * pol + 3] = conj ? -rr : rr;
               }
               if (!valid) rec.iu0 = INT_MIN;
             } else {
-              rec.v[0] = ph_r * vw;
-              rec.v[1] = ph_i * vw;
+              coef[0] = ph_r * vw;
+              coef[1] = ph_i * vw;
             }
             if (valid) taps = (unsigned long long)(2 * S + 1) * (2 * S + 1) * p.npol;
           }
@@ -204,7 +210,13 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
       }
     }
   }
-  if (inb) out.recs[s] = rec;
+  if (inb) {
+    out.smp.hdr[s] = rec;
+    out.smp.key[s] = key;
+    float4* c4 = reinterpret_cast<float4*>(out.smp.coef + (size_t)s * kCoef);
+#pragma unroll
+    for (int q = 0; q < kCoef / 4; ++q) c4[q] = make_float4(coef[4 * q], coef[4 * q + 1], coef[4 * q + 2], coef[4 * q + 3]);
+  }
 
   // counters (wave aggregated)
   const unsigned long long nvalid = __popcll(__ballot(valid));
@@ -245,29 +257,29 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
 struct TileRange {
   int tu0, tu1, tv0, tv1;
 };
-__device__ __forceinline__ TileRange tiles_of(const SampleRec& r) {
+__device__ __forceinline__ TileRange tiles_of(const SampleHdr& r) {
   const int W = 2 * (r.spack & 0xffff) + 1;
   return {r.iu0 / kTile, (r.iu0 + W - 1) / kTile, r.iv0 / kTile, (r.iv0 + W - 1) / kTile};
 }
 
-__global__ void k_bin_count(const SampleRec* __restrict__ recs, int n, int ntu, int ntiles, int* counts) {
+__global__ void k_bin_count(Samples smp, int n, int ntu, int ntiles, int* counts) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const SampleRec r = recs[s];
+  const SampleHdr r = smp.hdr[s];
   if (r.iu0 == INT_MIN) return;
   const TileRange t = tiles_of(r);
-  const long long base = (long long)r.key * ntiles;
+  const long long base = (long long)smp.key[s] * ntiles;
   for (int tv = t.tv0; tv <= t.tv1; ++tv)
     for (int tu = t.tu0; tu <= t.tu1; ++tu) atomicAdd(&counts[base + tv * ntu + tu], 1);
 }
 
-__global__ void k_bin_fill(const SampleRec* __restrict__ recs, int n, int ntu, int ntiles, int* cursor, int* pairs) {
+__global__ void k_bin_fill(Samples smp, int n, int ntu, int ntiles, int* cursor, int* pairs) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const SampleRec r = recs[s];
+  const SampleHdr r = smp.hdr[s];
   if (r.iu0 == INT_MIN) return;
   const TileRange t = tiles_of(r);
-  const long long base = (long long)r.key * ntiles;
+  const long long base = (long long)smp.key[s] * ntiles;
   for (int tv = t.tv0; tv <= t.tv1; ++tv)
     for (int tu = t.tu0; tu <= t.tu1; ++tu) {
       const int pos = atomicAdd(&cursor[base + tv * ntu + tu], 1);
@@ -300,29 +312,217 @@ __global__ void k_item_emit(const int* __restrict__ counts, const int* __restric
 // grid: lane layout in a 64x64 tile.  Wave w owns rows [16w, 16w+16); lane (lu, lv) =
 // (lane & 15, lane >> 4) owns pixels u = 16j + lu, v = 16w + 4k + lv for j, k in 0..3, i.e.
 // 16 "blocks" (k, j) of 16x4 pixels per wave.  A (sample, tile) pair touches the blocks its
-// footprint overlaps; all of a pair's CF loads are issued before its FMAs so that one L2
-// latency is paid per pair rather than per block.
+// footprint overlaps.  Everything that depends on the pair only (footprint geometry, which
+// blocks are active, CF row addresses, MAC coefficients) is wave-uniform and lives in SGPRs;
+// per lane there is one 32-bit CF offset per pair.  All of a pair's CF loads are issued before
+// its FMAs; the pipelined form (PIPE) issues the loads of pair i+1 before the FMAs of pair i
+// into a second register set, so that a pair's load latency hides behind the previous pair's
+// arithmetic.
 // --------------------------------------------------------------------------------------
 
-// Complex MAC coefficients of one sample and polarisation: acc_re += a*c.x + b*c.y and
-// acc_im += cc*c.x + d*c.y.  Plain: (v_r, -v_i, v_i, v_r) = v*C; PCF half plane: the same
-// with conj(C) (TableVisGridder.cc:837-844).
-struct MacCoef {
-  float a, b, c, d;
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt unconstrained): every vector memory load issued so
+// far has returned.  Placed before the loads of the next pair are issued, it tells the
+// compiler's wait-count analysis that the current pair's CF registers are ready.
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+// Buffer resource word 3 for raw 32-bit-element buffer accesses on gfx9-family parts
+// (DATA_FORMAT = 32); bounds are checked against num_records in bytes.
+constexpr int kBufferWord3 = 0x00020000;
+
+// Wave-uniform data of one (sample, tile) pair: the header and NV coefficient floats.
+template <int NV>
+struct PairU {
+  int iu0, iv0, cfoff, spack;
+  float v[NV > 0 ? NV : 1];
 };
 
-__device__ __forceinline__ bool block_active(int dvb, int dub, int S2) {
-  return !(dvb + 3 < 0 || dvb > S2 || dub + 15 < 0 || dub > S2);
+// Read-only views of the sample arrays for the tile kernels (restrict-qualified so that the
+// compiler may read them through the scalar cache).
+struct SamplesIn {
+  const SampleHdr* __restrict__ hdr;
+  const float* __restrict__ coef;
+};
+
+template <int NV>
+__device__ __forceinline__ PairU<NV> fetch_pair(const int* __restrict__ pairs, const SamplesIn& smp, int pi) {
+  const int s = pairs[pi];
+  const SampleHdr h = smp.hdr[s];
+  PairU<NV> p;
+  p.iu0 = h.iu0;
+  p.iv0 = h.iv0;
+  p.cfoff = h.cfoff;
+  p.spack = h.spack;
+  const float* c = smp.coef + (size_t)s * kCoef;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) p.v[q] = c[q];
+  return p;
 }
 
-template <int NP, int WPE>
+// Footprint geometry of a pair seen from one wave.  (du0, dvw) = footprint coordinates of the
+// wave's first pixel; block (k, j) covers footprint rows dvw+4k .. +3 and columns du0+16j .. +15
+// and is active when it overlaps [0, 2S] x [0, 2S]: bit 4k+j of mask.  soff0 = byte offset in
+// the CF table of the wave's first pixel (lane part excluded), rowstep = bytes per 4 CF rows.
+// Active blocks stay inside the padded CF plane: the guard bands cover the 15 columns / 3 rows
+// a block may overhang the footprint.
+struct PairGeom {
+  int P;
+  uint32_t mask, soff0, rowstep;
+};
+
+// bits lo..hi (clamped to 0..3) of a 4-bit set
+__device__ __forceinline__ uint32_t bit_range4(int lo, int hi) {
+  lo = max(lo, 0);
+  hi = min(hi, 3);
+  return lo > hi ? 0u : ((2u << hi) - (1u << lo));
+}
+
+// pad: bytes by which the buffer resource starts before the CF table.  Every offset of an
+// active block, and soff0 of a wave with an active block (dvw >= -15, du0 >= -63), is then
+// non-negative, so no buffer offset field ever holds a wrapped negative value.
+template <int NV>
+__device__ __forceinline__ PairGeom pair_geom(const PairU<NV>& p, int u0, int v0w, uint32_t pad) {
+  PairGeom g;
+  const int S2 = 2 * (p.spack & 0xffff);
+  g.P = p.spack >> 16;
+  const int du0 = u0 - p.iu0;
+  const int dvw = v0w - p.iv0;
+  // block row k active: dvw+4k+3 >= 0 and dvw+4k <= S2  (>> 2 is floor division by 4)
+  const uint32_t rows = bit_range4(-((dvw + 3) >> 2), (S2 - dvw) >> 2);
+  const uint32_t cols = bit_range4(-((du0 + 15) >> 4), (S2 - du0) >> 4);
+  // spread the row bits to positions 4k and combine with the column bits
+  const uint32_t rows16 = (rows & 1u) | ((rows & 2u) << 3) | ((rows & 4u) << 6) | ((rows & 8u) << 9);
+  g.mask = rows16 * cols;
+  g.soff0 = pad + (uint32_t)(p.cfoff + (dvw + kGuardV) * g.P + (du0 + kGuardU)) * (uint32_t)sizeof(float2);
+  g.rowstep = (uint32_t)(4 * g.P) * (uint32_t)sizeof(float2);
+  return g;
+}
+
+// The CF taps of the active blocks: a buffer load per block with the row in the scalar offset,
+// the column block in the immediate and the lane's position (voff = (lv*P + lu) * 8) in the
+// vector offset.
+__device__ __forceinline__ void load_cf(float2 (&c)[4][4], __amdgpu_buffer_rsrc_t cfr, const PairGeom& g,
+                                        uint32_t voff) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (g.mask & (1u << (4 * k + j))) {
+        // two dword loads, merged into one dwordx2 by the compiler (the vector-valued buffer
+        // load builtins of this toolchain load only the first dword)
+        const uint32_t so = g.soff0 + k * g.rowstep;
+        const uint32_t re = __builtin_amdgcn_raw_buffer_load_b32(cfr, voff + 128u * j, so, 0);
+        const uint32_t im = __builtin_amdgcn_raw_buffer_load_b32(cfr, voff + 128u * j + 4u, so, 0);
+        c[k][j] = make_float2(__builtin_bit_cast(float, re), __builtin_bit_cast(float, im));
+      }
+}
+
+// acc += v * C (or v * conj(C) in the PCF half plane, TableVisGridder.cc:837-844) for the
+// active blocks with the sample's MAC coefficients (a, b, c, d) per polarisation, which are
+// wave-uniform scalar FMA operands: acc_re += a*C.x + b*C.y, acc_im += c*C.x + d*C.y.
+template <int NP>
+__device__ __forceinline__ void grid_pair(float (&acc)[4][4][NP][2], const float2 (&c)[4][4], const PairGeom& g,
+                                          const PairU<4 * NP>& p) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (g.mask & (1u << (4 * k + j))) {
+        const float2 x = c[k][j];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+          acc[k][j][q][0] = fmaf(p.v[4 * q + 0], x.x, acc[k][j][q][0]);
+          acc[k][j][q][0] = fmaf(p.v[4 * q + 1], x.y, acc[k][j][q][0]);
+          acc[k][j][q][1] = fmaf(p.v[4 * q + 2], x.x, acc[k][j][q][1]);
+          acc[k][j][q][1] = fmaf(p.v[4 * q + 3], x.y, acc[k][j][q][1]);
+        }
+      }
+}
+
+// Runs consume(c, geom, pair, pi) for the pairs [pb, pe) of a work item, with the CF taps of the
+// pair in c.  PIPE: two register sets; the loads (and the scalar record fetch two pairs ahead)
+// are issued before the previous pair is consumed.  consume must not issue vector memory loads
+// whose results it waits for after returning (the wait before the next loads covers them).
+// The CF table as a buffer resource that starts pad bytes before the table.
+struct CFView {
+  __amdgpu_buffer_rsrc_t r;
+  uint32_t pad;
+};
+
+// Kernel argument describing the CF table: its address and size, and the pad (bytes, at least
+// 16 rows of the widest plane) by which the buffer resource starts before it.
+struct CFRef {
+  const float2* ptr;
+  uint32_t bytes, pad;
+};
+
+__device__ __forceinline__ CFView make_cf_view(const CFRef& t) {
+  CFView v;
+  v.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(reinterpret_cast<const char*>(t.ptr)) - t.pad, (short)0,
+                                          (int)(t.bytes + t.pad), kBufferWord3);
+  v.pad = t.pad;
+  return v;
+}
+
+template <int NV, bool PIPE, class F>
+__device__ __forceinline__ void for_each_pair(const int* __restrict__ pairs, const SamplesIn& smp, const CFView& cfv,
+                                              int pb, int pe, int u0, int v0w, int lu, int lv, F&& consume) {
+  const auto voff = [&](const PairGeom& g) { return (uint32_t)(lv * g.P + lu) * (uint32_t)sizeof(float2); };
+  if constexpr (!PIPE) {
+    for (int pi = pb; pi < pe; ++pi) {
+      // Nothing is in flight here (the previous pair consumed its taps); saying so keeps the
+      // wait-count analysis, which cannot correlate the load and use conditions of a block,
+      // from waiting for all loads before each load that reuses a register.
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      const PairU<NV> p = fetch_pair<NV>(pairs, smp, pi);
+      const PairGeom g = pair_geom(p, u0, v0w, cfv.pad);
+      float2 c[4][4];
+      load_cf(c, cfv.r, g, voff(g));
+      consume(c, g, p, pi);
+    }
+  } else {
+    if (pb >= pe) return;
+    int pi = pb;
+    float2 cA[4][4], cB[4][4];
+    PairU<NV> pA = fetch_pair<NV>(pairs, smp, pi);
+    PairGeom gA = pair_geom(pA, u0, v0w, cfv.pad);
+    load_cf(cA, cfv.r, gA, voff(gA));
+    PairU<NV> pB = pA;
+    if (pi + 1 < pe) pB = fetch_pair<NV>(pairs, smp, pi + 1);
+    for (;;) {
+      // pair pi: taps in flight into cA; pB = record of pair pi+1
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      const PairGeom gB = pair_geom(pB, u0, v0w, cfv.pad);
+      if (pi + 1 < pe) load_cf(cB, cfv.r, gB, voff(gB));
+      PairU<NV> pN = pB;
+      if (pi + 2 < pe) pN = fetch_pair<NV>(pairs, smp, pi + 2);
+      consume(cA, gA, pA, pi);
+      if (++pi >= pe) break;
+      // pair pi: taps in flight into cB; pN = record of pair pi+1
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      gA = pair_geom(pN, u0, v0w, cfv.pad);
+      if (pi + 1 < pe) load_cf(cA, cfv.r, gA, voff(gA));
+      PairU<NV> pM = pN;
+      if (pi + 2 < pe) pM = fetch_pair<NV>(pairs, smp, pi + 2);
+      consume(cB, gB, pB, pi);
+      if (++pi >= pe) break;
+      pA = pN;
+      pB = pM;
+    }
+  }
+}
+
+template <int NP, int WPE, bool PIPE>
 __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
                                               int* __restrict__ counter, const int* __restrict__ pairs,
-                                              const SampleRec* __restrict__ recs, const float2* __restrict__ cf,
-                                              float* __restrict__ grid, int nx, int ny, int ntu, int ntiles, int nzg) {
+                                              const SampleHdr* __restrict__ hdr, const float* __restrict__ coef,
+                                              CFRef cft, float* __restrict__ grid, int nx, int ny, int ntu, int ntiles,
+                                              int nzg) {
   __shared__ int s_item;
+  const SamplesIn smp{hdr, coef};
+  const CFView cfv = make_cf_view(cft);
   const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lu = lane & 15, lv = lane >> 4;
   const int nitems = *nitems_p;
   for (;;) {
@@ -336,6 +536,7 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     const int kz = it.x / ntiles;
     const int ichan = kz / nzg;
     const int u0 = (tile % ntu) * kTile, v0 = (tile / ntu) * kTile;
+    const int v0w = v0 + 16 * w;
 
     float acc[4][4][NP][2];
 #pragma unroll
@@ -345,42 +546,10 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
 #pragma unroll
         for (int q = 0; q < NP; ++q) acc[k][j][q][0] = acc[k][j][q][1] = 0.f;
 
-    for (int pi = it.y; pi < it.z; ++pi) {
-      const int s = pairs[pi];
-      const SampleRec& r = recs[s];
-      const int S2 = 2 * (r.spack & 0xffff);
-      const int P = r.spack >> 16;
-      const int du0 = u0 - r.iu0;            // footprint column of tile column 0
-      const int dvw = v0 - r.iv0 + 16 * w;   // footprint row of this wave's first row
-      const float2* base = cf + r.cfoff + (long long)(dvw + kGuardV + lv) * P + (du0 + kGuardU + lu);
-      float2 c[4][4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (block_active(dvw + 4 * k, du0 + 16 * j, S2)) c[k][j] = base[(long long)4 * k * P + 16 * j];
-      MacCoef m[NP];
-      const bool cj = r.conj != 0;
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        const float vr = r.v[2 * q], vi = r.v[2 * q + 1];
-        m[q] = cj ? MacCoef{vr, vi, vi, -vr} : MacCoef{vr, -vi, vi, vr};
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (block_active(dvw + 4 * k, du0 + 16 * j, S2)) {
-            const float2 cc = c[k][j];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) {
-              acc[k][j][q][0] = fmaf(m[q].a, cc.x, acc[k][j][q][0]);
-              acc[k][j][q][0] = fmaf(m[q].b, cc.y, acc[k][j][q][0]);
-              acc[k][j][q][1] = fmaf(m[q].c, cc.x, acc[k][j][q][1]);
-              acc[k][j][q][1] = fmaf(m[q].d, cc.y, acc[k][j][q][1]);
-            }
-          }
-    }
+    for_each_pair<4 * NP, PIPE>(pairs, smp, cfv, it.y, it.z, u0, v0w, lu, lv,
+                                [&](const float2(&c)[4][4], const PairGeom& g, const PairU<4 * NP>& p, int) {
+                                  grid_pair<NP>(acc, c, g, p);
+                                });
     // flush
     const size_t plane = (size_t)nx * ny;
 #pragma unroll
@@ -407,29 +576,83 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
 // --------------------------------------------------------------------------------------
 constexpr int kDegridBatch = 32;
 
-// Sum over the 64 lanes of a wave with DPP (row shifts, then row broadcasts); the total is
-// returned wave-uniformly (read from lane 63).
-__device__ __forceinline__ float wave_sum_dpp(float x) {
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x111, 0xf, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x112, 0xf, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x114, 0xf, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x118, 0xf, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x142, 0xa, 0xf, true));
-  x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x143, 0xc, 0xf, true));
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, true));
 }
 
-template <int NP, int WPE>
+// Reduce-scatter of 8 values over the 64 lanes of a wave: on return, lane l holds the sum over
+// all lanes of x[bitrev3(l & 7)] (bitrev3 reverses the three low bits).  Exchanges: lane l^1 and
+// l^2 by DPP quad permutes (halving the values each time), l^4 by row shifts, l^8 by a row
+// rotate, l^16 and l^32 by the gfx950 permlane swaps.
+__device__ __forceinline__ float wave_reduce_scatter8(const float (&x)[8], int lane) {
+  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4;
+  float y[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float send = b0 ? x[i] : x[i + 4];
+    const float keep = b0 ? x[i + 4] : x[i];
+    y[i] = keep + dpp<0xB1>(send);  // quad_perm [1,0,3,2]: lane l^1
+  }
+  float z[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const float send = b1 ? y[i] : y[i + 2];
+    const float keep = b1 ? y[i + 2] : y[i];
+    z[i] = keep + dpp<0x4E>(send);  // quad_perm [2,3,0,1]: lane l^2
+  }
+  const float send = b2 ? z[0] : z[1];
+  const float keep = b2 ? z[1] : z[0];
+  const float from_up = dpp<0x104>(send);  // row_shl:4: lane l+4
+  const float from_dn = dpp<0x114>(send);  // row_shr:4: lane l-4
+  float r = keep + (b2 ? from_dn : from_up);
+  r += dpp<0x128>(r);  // row_ror:8: lane l^8 within the row of 16
+  const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(r), __float_as_uint(r), false, false);
+  r = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);  // rows 2m and 2m+1 exchanged: lane l^16
+  const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(r), __float_as_uint(r), false, false);
+  r = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);  // halves exchanged: lane l^32
+  return r;
+}
+
+// sum over the wave's active blocks of C * conj(G) per polarisation (GridKernel.cc:121-122);
+// x[2q] / x[2q+1] = real / imaginary part for polarisation q, x[2NP..7] = 0
+template <int NP>
+__device__ __forceinline__ void degrid_pair(float (&x)[8], const float2 (&c)[4][4], const float2 (&gt)[4][4][NP],
+                                            const PairGeom& g) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x[i] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (g.mask & (1u << (4 * k + j))) {
+        const float2 cc = c[k][j];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+          x[2 * q] = fmaf(cc.x, gt[k][j][q].x, x[2 * q]);
+          x[2 * q] = fmaf(cc.y, gt[k][j][q].y, x[2 * q]);
+          x[2 * q + 1] = fmaf(cc.y, gt[k][j][q].x, x[2 * q + 1]);
+          x[2 * q + 1] = fmaf(-cc.x, gt[k][j][q].y, x[2 * q + 1]);
+        }
+      }
+}
+
+template <int NP, int WPE, bool PIPE>
 __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
                                                 int* __restrict__ counter, const int* __restrict__ pairs,
-                                                const SampleRec* __restrict__ recs, const float2* __restrict__ cf,
-                                                const float2* __restrict__ grid, int nx, int ny, int ntu, int ntiles,
-                                                int nzg, float* __restrict__ acc_out) {
+                                                const SampleHdr* __restrict__ hdr, const float* __restrict__ coef,
+                                                CFRef cft, const float2* __restrict__ grid, int nx, int ny, int ntu,
+                                                int ntiles, int nzg, float* __restrict__ acc_out) {
   __shared__ int s_item;
-  __shared__ float s_part[kDegridBatch][4][2 * NP];
+  const CFView cfv = make_cf_view(cft);
+  __shared__ float s_part[kDegridBatch][4][8];
+  __shared__ int s_sid[kDegridBatch];
+  const SamplesIn smp{hdr, coef};
   const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lu = lane & 15, lv = lane >> 4;
+  const int slot = ((lane & 1) << 2) | (lane & 2) | ((lane >> 2) & 1);  // value held after the reduce-scatter
   const int nitems = *nitems_p;
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
@@ -444,7 +667,7 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     const int u0 = (tile % ntu) * kTile, v0 = (tile / ntu) * kTile;
     const size_t plane = (size_t)nx * ny;
 
-    float2 g[4][4][NP];
+    float2 gt[4][4][NP];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int v = v0 + 16 * w + 4 * k + lv;
@@ -453,77 +676,45 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
         const int u = u0 + 16 * j + lu;
 #pragma unroll
         for (int q = 0; q < NP; ++q)
-          g[k][j][q] = (u < nx && v < ny) ? grid[(size_t)(ichan * NP + q) * plane + (size_t)v * nx + u]
-                                          : make_float2(0.f, 0.f);
+          gt[k][j][q] = (u < nx && v < ny) ? grid[(size_t)(ichan * NP + q) * plane + (size_t)v * nx + u]
+                                           : make_float2(0.f, 0.f);
       }
     }
 
-    for (int pb = it.y; pb < it.z; pb += kDegridBatch) {
-      const int nb = min(kDegridBatch, it.z - pb);
-      for (int b = 0; b < nb; ++b) {
-        const int s = pairs[pb + b];
-        const SampleRec& r = recs[s];
-        const int S2 = 2 * (r.spack & 0xffff);
-        const int P = r.spack >> 16;
-        const int du0 = u0 - r.iu0;
-        const int dvw = v0 - r.iv0 + 16 * w;
-        const float2* base = cf + r.cfoff + (long long)(dvw + kGuardV + lv) * P + (du0 + kGuardU + lu);
-        float2 c[4][4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (block_active(dvw + 4 * k, du0 + 16 * j, S2)) c[k][j] = base[(long long)4 * k * P + 16 * j];
-        float a[NP][2];
-#pragma unroll
-        for (int q = 0; q < NP; ++q) a[q][0] = a[q][1] = 0.f;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (block_active(dvw + 4 * k, du0 + 16 * j, S2)) {
-              const float2 cc = c[k][j];
-#pragma unroll
-              for (int q = 0; q < NP; ++q) {
-                // CF * conj(G)  (GridKernel.cc:121-122)
-                a[q][0] = fmaf(cc.x, g[k][j][q].x, a[q][0]);
-                a[q][0] = fmaf(cc.y, g[k][j][q].y, a[q][0]);
-                a[q][1] = fmaf(cc.y, g[k][j][q].x, a[q][1]);
-                a[q][1] = fmaf(-cc.x, g[k][j][q].y, a[q][1]);
-              }
-            }
-#pragma unroll
-        for (int q = 0; q < NP; ++q) {
-          a[q][0] = wave_sum_dpp(a[q][0]);
-          a[q][1] = wave_sum_dpp(a[q][1]);
-        }
-        if (lane < 2 * NP) {
-          float v = a[0][0];
-#pragma unroll
-          for (int q = 0; q < 2 * NP; ++q)
-            if (lane == q) v = a[q / 2][q % 2];
-          s_part[b][w][lane] = v;
-        }
-      }
-      __syncthreads();
-      for (int t = tid; t < nb * 2 * NP; t += blockDim.x) {
-        const int b = t / (2 * NP), q = t % (2 * NP);
-        const float sum = s_part[b][0][q] + s_part[b][1][q] + s_part[b][2][q] + s_part[b][3][q];
-        if (sum != 0.f) atomicAdd(&acc_out[(size_t)pairs[pb + b] * 8 + q], sum);
-      }
-      __syncthreads();
-    }
+    // per pair: wave dot product, reduce-scatter, one LDS slot per (pair, wave, value); every
+    // kDegridBatch pairs the four waves' partial sums meet and go to the per-sample accumulators
+    const int pb = it.y, pe = it.z;
+    for_each_pair<0, PIPE>(pairs, smp, cfv, pb, pe, u0, v0 + 16 * w, lu, lv,
+                           [&](const float2(&c)[4][4], const PairGeom& g, const PairU<0>&, int pi) {
+                             float x[8];
+                             degrid_pair<NP>(x, c, gt, g);
+                             const float r = wave_reduce_scatter8(x, lane);
+                             const int b = (pi - pb) % kDegridBatch;
+                             if (lane < 8) s_part[b][w][slot] = r;
+                             if (tid == 0) s_sid[b] = pairs[pi];
+                             if (b == kDegridBatch - 1 || pi == pe - 1) {
+                               __syncthreads();
+                               const int nb = b + 1;
+                               for (int t = tid; t < nb * 2 * NP; t += blockDim.x) {
+                                 const int bb = t / (2 * NP), q = t % (2 * NP);
+                                 const float sum =
+                                     s_part[bb][0][q] + s_part[bb][1][q] + s_part[bb][2][q] + s_part[bb][3][q];
+                                 if (sum != 0.f) atomicAdd(&acc_out[(size_t)s_sid[bb] * 8 + q], sum);
+                               }
+                               __syncthreads();
+                             }
+                           });
   }
 }
 
-__global__ void k_degrid_finish(const SampleRec* __restrict__ recs, const float* __restrict__ acc, int nrow, int nchan,
-                                int npol, float2* __restrict__ vis_out) {
+__global__ void k_degrid_finish(Samples smp, const float* __restrict__ acc, int nrow, int nchan, int npol,
+                                float2* __restrict__ vis_out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nrow * nchan) return;
-  const SampleRec r = recs[s];
-  if (r.iu0 == INT_MIN) return;
+  if (smp.hdr[s].iu0 == INT_MIN) return;
   const int row = s % nrow, chan = s / nrow;
-  const float fr = r.v[0], fi = r.v[1];
+  const float2 ph = *reinterpret_cast<const float2*>(smp.coef + (size_t)s * kCoef);
+  const float fr = ph.x, fi = ph.y;
   for (int pol = 0; pol < npol; ++pol) {
     const float cr = acc[(size_t)s * 8 + 2 * pol], ci = acc[(size_t)s * 8 + 2 * pol + 1];
     float2* o = vis_out + row + (size_t)nrow * (chan + (size_t)nchan * pol);
@@ -546,7 +737,7 @@ void launch_prep(const Ctx& c, const PrepParams& p, const PrepInputs& in, const 
   YG_HIP(hipGetLastError());
 }
 
-void launch_binning(const Ctx& c, const SampleRec* recs, int n, int nx, int ny, int nkz, Binning& b) {
+void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, int nkz, Binning& b) {
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const size_t ntiles = (size_t)ntu * ntv;
   const size_t nkeys = ntiles * (size_t)nkz;
@@ -594,62 +785,77 @@ void launch_binning(const Ctx& c, const SampleRec* recs, int n, int nx, int ny, 
 
 static int persistent_blocks(const Ctx& c) { return c.n_cu * (c.waves_per_simd >= 3 ? 3 : 2); }
 
-void launch_grid(const Ctx& c, const Binning& b, const SampleRec* recs, const float2* cf, float2* grid, int nx, int ny,
-                 int npol, int nzg) {
+// kernel variant by polarisation count, register budget (waves per SIMD) and pipelining
+template <int NP, int WPE, bool PIPE>
+struct TileKernels {
+  static void grid(hipStream_t s, dim3 g, const Binning& b, const Samples& smp, CFRef cft, float* gf, int nx, int ny,
+                   int ntu, int ntiles, int nzg) {
+    hipLaunchKernelGGL((k_grid<NP, WPE, PIPE>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
+                       b.counter.as<int>(), b.pairs.as<int>(), smp.hdr, smp.coef, cft, gf, nx, ny, ntu, ntiles, nzg);
+  }
+  static void degrid(hipStream_t s, dim3 g, const Binning& b, const Samples& smp, CFRef cft, const float2* grid,
+                     int nx, int ny, int ntu, int ntiles, int nzg, float* acc) {
+    hipLaunchKernelGGL((k_degrid<NP, WPE, PIPE>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
+                       b.counter.as<int>(), b.pairs.as<int>(), smp.hdr, smp.coef, cft, grid, nx, ny, ntu, ntiles, nzg,
+                       acc);
+  }
+};
+
+// The tile kernels address the CF table through a buffer resource with 32-bit byte offsets
+// that starts 16 rows of the widest plane before the table.
+static CFRef cf_ref(const float2* cf, size_t cf_bytes, int max_support) {
+  const size_t pad = (size_t)(16 * (2 * (size_t)max_support + 1 + 2 * kGuardU) + 64) * sizeof(float2);
+  YG_CHECK(cf_bytes > 0 && cf_bytes + pad <= 0xFFFFFFF0ull, 1, "CF table must be smaller than 4 GiB");
+  return CFRef{cf, (uint32_t)cf_bytes, (uint32_t)pad};
+}
+
+template <template <int, int, bool> class K, int NP, class F>
+static void with_variant(const Ctx& c, F&& f) {
+  if (c.waves_per_simd >= 3) {
+    if (c.pipeline) f(K<NP, 3, true>{});
+    else f(K<NP, 3, false>{});
+  } else {
+    if (c.pipeline) f(K<NP, 2, true>{});
+    else f(K<NP, 2, false>{});
+  }
+}
+
+template <class F>
+static void with_tile_kernels(const Ctx& c, int npol, F&& f) {
+  switch (npol) {
+    case 1: with_variant<TileKernels, 1>(c, f); break;
+    case 2: with_variant<TileKernels, 2>(c, f); break;
+    case 3: with_variant<TileKernels, 3>(c, f); break;
+    case 4: with_variant<TileKernels, 4>(c, f); break;
+    default: throw Error(1, "unsupported number of polarisations (1..4)");
+  }
+}
+
+void launch_grid(const Ctx& c, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes,
+                 int max_support, float2* grid, int nx, int ny, int npol, int nzg) {
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const int ntiles = ntu * ntv;
+  const CFRef cft = cf_ref(cf, cf_bytes, max_support);
   YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
-  const dim3 g(persistent_blocks(c)), t(256);
+  const dim3 g(persistent_blocks(c));
   float* gf = reinterpret_cast<float*>(grid);
-#define YG_GRID_CASE(NPV)                                                                                       \
-  case NPV:                                                                                                     \
-    if (c.waves_per_simd >= 3)                                                                                  \
-      hipLaunchKernelGGL((k_grid<NPV, 3>), g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(),         \
-                         b.counter.as<int>(), b.pairs.as<int>(), recs, cf, gf, nx, ny, ntu, ntiles, nzg);       \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_grid<NPV, 2>), g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(),         \
-                         b.counter.as<int>(), b.pairs.as<int>(), recs, cf, gf, nx, ny, ntu, ntiles, nzg);       \
-    break;
-  switch (npol) {
-    YG_GRID_CASE(1)
-    YG_GRID_CASE(2)
-    YG_GRID_CASE(3)
-    YG_GRID_CASE(4)
-    default:
-      throw Error(1, "unsupported number of polarisations (1..4)");
-  }
-#undef YG_GRID_CASE
+  with_tile_kernels(c, npol, [&](auto k) { k.grid(c.stream, g, b, smp, cft, gf, nx, ny, ntu, ntiles, nzg); });
   YG_HIP(hipGetLastError());
 }
 
-void launch_degrid(const Ctx& c, const Binning& b, const SampleRec* recs, const float2* cf, const float2* grid, int nx,
-                   int ny, int npol, int nzg, float* acc) {
+void launch_degrid(const Ctx& c, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes,
+                   int max_support, const float2* grid, int nx, int ny, int npol, int nzg, float* acc) {
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const int ntiles = ntu * ntv;
+  const CFRef cft = cf_ref(cf, cf_bytes, max_support);
   YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
-  const dim3 g(persistent_blocks(c)), t(256);
-#define YG_DEGRID_CASE(NPV)                                                                                     \
-  case NPV:                                                                                                     \
-    if (c.waves_per_simd >= 3)                                                                                  \
-      hipLaunchKernelGGL((k_degrid<NPV, 3>), g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(),       \
-                         b.counter.as<int>(), b.pairs.as<int>(), recs, cf, grid, nx, ny, ntu, ntiles, nzg, acc); \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_degrid<NPV, 2>), g, t, 0, c.stream, b.items.as<int4>(), b.nitems.as<int>(),       \
-                         b.counter.as<int>(), b.pairs.as<int>(), recs, cf, grid, nx, ny, ntu, ntiles, nzg, acc); \
-    break;
-  switch (npol) {
-    YG_DEGRID_CASE(1)
-    YG_DEGRID_CASE(2)
-    YG_DEGRID_CASE(3)
-    YG_DEGRID_CASE(4)
-    default:
-      throw Error(1, "unsupported number of polarisations (1..4)");
-  }
-#undef YG_DEGRID_CASE
+  const dim3 g(persistent_blocks(c));
+  with_tile_kernels(c, npol,
+                    [&](auto k) { k.degrid(c.stream, g, b, smp, cft, grid, nx, ny, ntu, ntiles, nzg, acc); });
   YG_HIP(hipGetLastError());
 }
 
-void launch_degrid_finish(const Ctx& c, const SampleRec* recs, const float* acc, int nrow, int nchan, int npol,
+void launch_degrid_finish(const Ctx& c, const Samples& recs, const float* acc, int nrow, int nchan, int npol,
                           float2* vis_out) {
   const int n = nrow * nchan;
   if (n == 0) return;

@@ -57,6 +57,7 @@ int ygrid_ctx_create(int device, ygrid_ctx** out) {
     YG_HIP(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (const char* v = std::getenv("YGRID_WAVES_PER_SIMD")) c->waves_per_simd = std::atoi(v);
+    if (const char* v = std::getenv("YGRID_PIPELINE")) c->pipeline = std::atoi(v);
     YG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
     *out = new ygrid_ctx{c};

@@ -96,18 +96,25 @@ struct CIndexDesc {
 // What kind of pass the prep kernel prepares.
 enum PassMode : int { kModeGrid = 0, kModePSF = 1, kModePCF = 2, kModeDegrid = 3 };
 
-// Per (row, chan) record written by the prep kernel and read (wave-uniformly) by the
-// grid/degrid kernels.  iu0 == INT32_MIN marks a sample that is not gridded.
-struct alignas(16) SampleRec {
+// Per (row, chan) sample written by the prep kernel, as three arrays indexed by
+// s = row + nrow*chan.  The grid/degrid kernels read a sample wave-uniformly (scalar loads).
+// hdr.iu0 == INT32_MIN marks a sample that is not gridded.
+struct alignas(16) SampleHdr {
   int32_t iu0, iv0;     // footprint origin: (iuOffset - S, ivOffset - S)
   int32_t cfoff;        // element offset of the padded CF plane in the CF table
   int32_t spack;        // support S (low 16 bits) | pitch << 16
-  float v[8];           // grid: per-pol complex value to spread; degrid: [0..1] phasor*weight
-  int32_t key;          // binning key: (ichan, z-group) plane index
-  int32_t conj;         // PCF: spread conj(CF)
-  int32_t pad0, pad1;
 };
-static_assert(sizeof(SampleRec) == 64, "SampleRec must be 64 bytes");
+static_assert(sizeof(SampleHdr) == 16, "SampleHdr must be 16 bytes");
+constexpr int kCoef = 16;  // floats per sample in Samples::coef
+struct Samples {
+  SampleHdr* hdr;
+  int32_t* key;         // binning key: (ichan, z-group) plane index
+  // grid: per polarisation q the MAC coefficients (a, b, c, d) at [4q..4q+3] with
+  //   grid_re += a*C.re + b*C.im, grid_im += c*C.re + d*C.im
+  // (the value spread is v*C, or v*conj(C) in the PCF half plane); degrid: [0..1] = the
+  // phasor times the visibility weight.
+  float* coef;
+};
 
 struct PrepParams {
   int nrow, nchan, npol;        // chunk shape (npol == image pols)
@@ -125,7 +132,7 @@ struct PrepParams {
 };
 
 struct PrepOutputs {
-  SampleRec* recs;              // nrow*nchan
+  Samples smp;                  // nrow*nchan
   double* sumw;                 // nsw_rows * npol * nchan_img
   unsigned long long* counters; // [0] samples, [1] taps, [2] flagged, [3] wflagged
   int* error;                   // bit 1: zero noise weight; bit 2: w out of range
@@ -161,18 +168,20 @@ struct Ctx {
   bool own_stream = false;
   int n_cu = 256;
   int waves_per_simd = 3;  // register budget of the grid/degrid kernels (2: <=256 VGPRs, 3: <=168)
+  int pipeline = 0;        // grid/degrid: issue pair i+1's CF loads before pair i's arithmetic
 };
 
 // Kernel launchers (kernels_grid.hip)
 void launch_prep(const Ctx&, const PrepParams&, const PrepInputs&, const PrepOutputs&);
 // Bins valid records by (key, tile); returns nothing, work lands in b.
-void launch_binning(const Ctx&, const SampleRec* recs, int nsamples, int nx, int ny, int nkeys_z,
-                    Binning& b);
-void launch_grid(const Ctx&, const Binning& b, const SampleRec* recs, const float2* cf, float2* grid,
-                 int nx, int ny, int npol, int nzg);
-void launch_degrid(const Ctx&, const Binning& b, const SampleRec* recs, const float2* cf, const float2* grid,
-                   int nx, int ny, int npol, int nzg, float* acc);
-void launch_degrid_finish(const Ctx&, const SampleRec* recs, const float* acc, int nrow, int nchan, int npol,
+void launch_binning(const Ctx&, const Samples& smp, int nsamples, int nx, int ny, int nkeys_z, Binning& b);
+// cf_bytes: size of the CF table (the kernels address it with 32-bit byte offsets);
+// max_support: largest support of its planes
+void launch_grid(const Ctx&, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes, int max_support,
+                 float2* grid, int nx, int ny, int npol, int nzg);
+void launch_degrid(const Ctx&, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes,
+                   int max_support, const float2* grid, int nx, int ny, int npol, int nzg, float* acc);
+void launch_degrid_finish(const Ctx&, const Samples& smp, const float* acc, int nrow, int nchan, int npol,
                           float2* vis_out);
 
 // FFT-domain helpers (kernels_fft.hip)
