@@ -381,15 +381,15 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   if (profiling_) YG_HIP(hipEventRecord(ev_[0], ctx_->stream));
   launch_prep(*ctx_, pp, in, out);
   if (profiling_) YG_HIP(hipEventRecord(ev_[1], ctx_->stream));
-  launch_binning(*ctx_, smp, nsamp, shape_[0], shape_[1], pp.nzg * shape_[3], bin_);
   const float2* cft = cf_->table.as<float2>();
   const size_t cfb = cf_->table.size();
   const int smax = cf_->supp.empty() ? 0 : *std::max_element(cf_->supp.begin(), cf_->supp.end());
+  launch_binning(*ctx_, smp, nsamp, shape_[0], shape_[1], pp.nzg * shape_[3], smax, bin_);
   if (forward) {
     acc_.resize((size_t)nsamp * 8 * sizeof(float));
     YG_HIP(hipMemsetAsync(acc_.get(), 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_degrid(*ctx_, bin_, smp, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg,
+    launch_degrid(*ctx_, bin_, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg,
                   acc_.as<float>());
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
     launch_degrid_finish(*ctx_, smp, acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);

@@ -273,7 +273,46 @@ __global__ void k_bin_count(Samples smp, int n, int ntu, int ntiles, int* counts
     for (int tu = t.tu0; tu <= t.tu1; ++tu) atomicAdd(&counts[base + tv * ntu + tu], 1);
 }
 
-__global__ void k_bin_fill(Samples smp, int n, int ntu, int ntiles, int* cursor, int* pairs) {
+// bits lo..hi (clamped to 0..n-1) of an n-bit set
+template <int N>
+__device__ __forceinline__ uint32_t bit_range(int lo, int hi) {
+  lo = max(lo, 0);
+  hi = min(hi, N - 1);
+  return lo > hi ? 0u : ((2u << hi) - (1u << lo));
+}
+
+// Descriptor of one (sample, tile) pair, read wave-uniformly by the tile kernels.  The tile is
+// worked by four waves, wave w on rows [16w, 16w+16), each in 8 blocks (k, j) of 4 rows x 32
+// columns (block rows 16w+4k .. +3, columns 32j .. +31 of the tile).
+//   x: sample index
+//   y: byte offset (in a CF buffer that starts pad bytes before the table) of the CF tap under
+//      the tile's pixel (0, 0), modulo 2^32; wave w adds 16*w rows
+//   z: bits 8w+2k+j: block (k, j) of wave w overlaps the footprint
+//   w: CF row pitch P (bits 0..15), and the signed bytes A (16..23), B (24..31) with: the two
+//      taps of lane column pair x = 32j + 2lu (tile columns x, x+1) touch the footprint when
+//      A <= x <= B (A = -1 - du0, B = 2S - du0, clamped to what x can take)
+__device__ __forceinline__ int4 pair_descriptor(const SampleHdr& r, int s, int u0, int v0, uint32_t pad) {
+  const int S2 = 2 * (r.spack & 0xffff), P = r.spack >> 16;
+  const int du0 = u0 - r.iu0, dv0 = v0 - r.iv0;
+  // block column j: du0+32j+31 >= 0 and du0+32j <= 2S; block row k of wave w, with
+  // dvw = dv0+16w: dvw+4k+3 >= 0 and dvw+4k <= 2S  (>> is floor division by a power of two)
+  const uint32_t cols = bit_range<2>(-((du0 + 31) >> 5), (S2 - du0) >> 5);
+  uint32_t masks = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int dvw = dv0 + 16 * w;
+    const uint32_t rows = bit_range<4>(-((dvw + 3) >> 2), (S2 - dvw) >> 2);
+    const uint32_t rows8 = (rows & 1u) | ((rows & 2u) << 1) | ((rows & 4u) << 2) | ((rows & 8u) << 3);
+    masks |= (rows8 * cols) << (8 * w);
+  }
+  const uint32_t soff = pad + (uint32_t)(r.cfoff + (dv0 + kGuardV) * P + (du0 + kGuardU)) * (uint32_t)sizeof(float2);
+  const int A = min(max(-1 - du0, -2), 64);
+  const int B = min(max(S2 - du0, -1), 100);
+  const uint32_t pw = (uint32_t)(P & 0xffff) | ((uint32_t)(A & 0xff) << 16) | ((uint32_t)(B & 0xff) << 24);
+  return make_int4(s, (int)soff, (int)masks, (int)pw);
+}
+
+__global__ void k_bin_fill(Samples smp, int n, int ntu, int ntiles, uint32_t pad, int* cursor, int4* descs) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const SampleHdr r = smp.hdr[s];
@@ -283,7 +322,7 @@ __global__ void k_bin_fill(Samples smp, int n, int ntu, int ntiles, int* cursor,
   for (int tv = t.tv0; tv <= t.tv1; ++tv)
     for (int tu = t.tu0; tu <= t.tu1; ++tu) {
       const int pos = atomicAdd(&cursor[base + tv * ntu + tu], 1);
-      pairs[pos] = s;
+      descs[pos] = pair_descriptor(r, s, tu * kTile, tv * kTile, pad);
     }
 }
 
@@ -310,15 +349,20 @@ __global__ void k_item_emit(const int* __restrict__ counts, const int* __restric
 
 // --------------------------------------------------------------------------------------
 // grid: lane layout in a 64x64 tile.  Wave w owns rows [16w, 16w+16); lane (lu, lv) =
-// (lane & 15, lane >> 4) owns pixels u = 16j + lu, v = 16w + 4k + lv for j, k in 0..3, i.e.
-// 16 "blocks" (k, j) of 16x4 pixels per wave.  A (sample, tile) pair touches the blocks its
-// footprint overlaps.  Everything that depends on the pair only (footprint geometry, which
-// blocks are active, CF row addresses, MAC coefficients) is wave-uniform and lives in SGPRs;
-// per lane there is one 32-bit CF offset per pair.  All of a pair's CF loads are issued before
-// its FMAs; the pipelined form (PIPE) issues the loads of pair i+1 before the FMAs of pair i
-// into a second register set, so that a pair's load latency hides behind the previous pair's
-// arithmetic.
+// (lane & 15, lane >> 4) owns pixels u = 32j + 2lu + e, v = 16w + 4k + lv for k in 0..3,
+// j, e in 0..1, i.e. 8 "blocks" (k, j) of 32x4 pixels per wave, two adjacent pixels per lane
+// and block, so that a block's CF taps are one 16-byte load per lane (the vector memory path
+// moves four lanes per clock, whatever their width).  A (sample, tile) pair touches the blocks
+// its footprint overlaps.  Everything that depends on the pair only (footprint geometry, which
+// blocks are active, CF row offsets, MAC coefficients) is wave-uniform and lives in SGPRs; per
+// lane there is one CF byte offset per column block, set out of the buffer's range for lanes
+// whose taps lie beyond the footprint's guard band (such loads return zeros and touch no
+// memory).  All of a pair's CF loads are issued before its FMAs; the pipelined form (PIPE)
+// issues the loads of pair i+1 before the FMAs of pair i into a second register set.
 // --------------------------------------------------------------------------------------
+
+// CF taps of one pair for one lane: [k][j][e]
+typedef float2 Taps[4][2][2];
 
 // s_waitcnt vmcnt(0) (expcnt and lgkmcnt unconstrained): every vector memory load issued so
 // far has returned.  Placed before the loads of the next pair are issued, it tells the
@@ -329,90 +373,80 @@ constexpr int kWaitVmcnt0 = 0x0F70;
 // (DATA_FORMAT = 32); bounds are checked against num_records in bytes.
 constexpr int kBufferWord3 = 0x00020000;
 
-// Wave-uniform data of one (sample, tile) pair: the header and NV coefficient floats.
+// Wave-uniform data of one (sample, tile) pair: its descriptor (pair_descriptor) and NV
+// coefficient floats of the sample.
 template <int NV>
 struct PairU {
-  int iu0, iv0, cfoff, spack;
+  int4 d;
   float v[NV > 0 ? NV : 1];
 };
 
-// Read-only views of the sample arrays for the tile kernels (restrict-qualified so that the
-// compiler may read them through the scalar cache).
-struct SamplesIn {
-  const SampleHdr* __restrict__ hdr;
-  const float* __restrict__ coef;
-};
-
 template <int NV>
-__device__ __forceinline__ PairU<NV> fetch_pair(const int* __restrict__ pairs, const SamplesIn& smp, int pi) {
-  const int s = pairs[pi];
-  const SampleHdr h = smp.hdr[s];
+__device__ __forceinline__ PairU<NV> fetch_pair(const int4* __restrict__ descs, const float* __restrict__ coef,
+                                                int pi) {
   PairU<NV> p;
-  p.iu0 = h.iu0;
-  p.iv0 = h.iv0;
-  p.cfoff = h.cfoff;
-  p.spack = h.spack;
-  const float* c = smp.coef + (size_t)s * kCoef;
+  p.d = descs[pi];
+  const float* c = coef + (size_t)p.d.x * kCoef;
 #pragma unroll
   for (int q = 0; q < NV; ++q) p.v[q] = c[q];
   return p;
 }
 
-// Footprint geometry of a pair seen from one wave.  (du0, dvw) = footprint coordinates of the
-// wave's first pixel; block (k, j) covers footprint rows dvw+4k .. +3 and columns du0+16j .. +15
-// and is active when it overlaps [0, 2S] x [0, 2S]: bit 4k+j of mask.  soff0 = byte offset in
-// the CF table of the wave's first pixel (lane part excluded), rowstep = bytes per 4 CF rows.
-// Active blocks stay inside the padded CF plane: the guard bands cover the 15 columns / 3 rows
-// a block may overhang the footprint.
+// Geometry of a pair seen from wave w: bit 2k+j of mask = block (k, j) active, soff0 = byte
+// offset of the CF tap under the wave's first pixel (lane part excluded), rowstep = bytes per 4
+// CF rows, and the lane column-pair range [A, B] (see pair_descriptor).  The rows of an active
+// block stay inside the padded CF plane (the guard band covers the 3 rows a block may overhang)
+// and every offset of an active block is non-negative (the buffer starts pad bytes early).
 struct PairGeom {
-  int P;
+  int P, A, B;
   uint32_t mask, soff0, rowstep;
 };
 
-// bits lo..hi (clamped to 0..3) of a 4-bit set
-__device__ __forceinline__ uint32_t bit_range4(int lo, int hi) {
-  lo = max(lo, 0);
-  hi = min(hi, 3);
-  return lo > hi ? 0u : ((2u << hi) - (1u << lo));
-}
-
-// pad: bytes by which the buffer resource starts before the CF table.  Every offset of an
-// active block, and soff0 of a wave with an active block (dvw >= -15, du0 >= -63), is then
-// non-negative, so no buffer offset field ever holds a wrapped negative value.
 template <int NV>
-__device__ __forceinline__ PairGeom pair_geom(const PairU<NV>& p, int u0, int v0w, uint32_t pad) {
+__device__ __forceinline__ PairGeom pair_geom(const PairU<NV>& p, int w) {
   PairGeom g;
-  const int S2 = 2 * (p.spack & 0xffff);
-  g.P = p.spack >> 16;
-  const int du0 = u0 - p.iu0;
-  const int dvw = v0w - p.iv0;
-  // block row k active: dvw+4k+3 >= 0 and dvw+4k <= S2  (>> 2 is floor division by 4)
-  const uint32_t rows = bit_range4(-((dvw + 3) >> 2), (S2 - dvw) >> 2);
-  const uint32_t cols = bit_range4(-((du0 + 15) >> 4), (S2 - du0) >> 4);
-  // spread the row bits to positions 4k and combine with the column bits
-  const uint32_t rows16 = (rows & 1u) | ((rows & 2u) << 3) | ((rows & 4u) << 6) | ((rows & 8u) << 9);
-  g.mask = rows16 * cols;
-  g.soff0 = pad + (uint32_t)(p.cfoff + (dvw + kGuardV) * g.P + (du0 + kGuardU)) * (uint32_t)sizeof(float2);
-  g.rowstep = (uint32_t)(4 * g.P) * (uint32_t)sizeof(float2);
+  const uint32_t pw = (uint32_t)p.d.w;
+  g.P = (int)(pw & 0xffffu);
+  g.A = (int)(pw << 8) >> 24;
+  g.B = (int)pw >> 24;
+  g.mask = ((uint32_t)p.d.z >> (8 * w)) & 0xffu;
+  g.soff0 = (uint32_t)p.d.y + (uint32_t)w * ((uint32_t)g.P << 7);  // 16 rows of 8-byte taps
+  g.rowstep = (uint32_t)g.P << 5;                                   // 4 rows
   return g;
 }
 
-// The CF taps of the active blocks: a buffer load per block with the row in the scalar offset,
-// the column block in the immediate and the lane's position (voff = (lv*P + lu) * 8) in the
-// vector offset.
-__device__ __forceinline__ void load_cf(float2 (&c)[4][4], __amdgpu_buffer_rsrc_t cfr, const PairGeom& g,
-                                        uint32_t voff) {
+// Per lane and column block j, the byte offset of the lane's two taps relative to soff0 (tile
+// columns x, x+1 with x = 32j + 2lu).  A lane with no tap in the footprint gets an offset beyond
+// the buffer's num_records (which stays below 2 GiB) so that its load returns zeros; the other
+// lanes read inside the padded row (the partner of an edge tap lies in the guard band).
+constexpr uint32_t kOutOfRange = 0x80000000u;
+__device__ __forceinline__ void lane_offsets(uint32_t (&vo)[2], const PairGeom& g, int lu, int lv) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int x = 32 * j + 2 * lu;
+    const bool in = (x >= g.A) && (x <= g.B);
+    vo[j] = in ? (uint32_t)(lv * g.P + x) * (uint32_t)sizeof(float2) : kOutOfRange;
+  }
+}
+
+// The CF taps of the active blocks: one 16-byte buffer load per lane and block, with the row in
+// the scalar offset and the lane's position in the vector offset.
+__device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, const PairGeom& g, int lu, int lv) {
+  uint32_t vo[2];
+  lane_offsets(vo, g, lu, lv);
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (g.mask & (1u << (4 * k + j))) {
-        // two dword loads, merged into one dwordx2 by the compiler (the vector-valued buffer
+    for (int j = 0; j < 2; ++j)
+      if (g.mask & (1u << (2 * k + j))) {
+        // four dword loads, merged into one dwordx4 by the compiler (the vector-valued buffer
         // load builtins of this toolchain load only the first dword)
         const uint32_t so = g.soff0 + k * g.rowstep;
-        const uint32_t re = __builtin_amdgcn_raw_buffer_load_b32(cfr, voff + 128u * j, so, 0);
-        const uint32_t im = __builtin_amdgcn_raw_buffer_load_b32(cfr, voff + 128u * j + 4u, so, 0);
-        c[k][j] = make_float2(__builtin_bit_cast(float, re), __builtin_bit_cast(float, im));
+        uint32_t d[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo[j] + 4u * i, so, 0);
+        c[k][j][0] = make_float2(__builtin_bit_cast(float, d[0]), __builtin_bit_cast(float, d[1]));
+        c[k][j][1] = make_float2(__builtin_bit_cast(float, d[2]), __builtin_bit_cast(float, d[3]));
       }
 }
 
@@ -420,33 +454,26 @@ __device__ __forceinline__ void load_cf(float2 (&c)[4][4], __amdgpu_buffer_rsrc_
 // active blocks with the sample's MAC coefficients (a, b, c, d) per polarisation, which are
 // wave-uniform scalar FMA operands: acc_re += a*C.x + b*C.y, acc_im += c*C.x + d*C.y.
 template <int NP>
-__device__ __forceinline__ void grid_pair(float (&acc)[4][4][NP][2], const float2 (&c)[4][4], const PairGeom& g,
+__device__ __forceinline__ void grid_pair(float (&acc)[4][2][2][NP][2], const Taps& c, const PairGeom& g,
                                           const PairU<4 * NP>& p) {
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (g.mask & (1u << (4 * k + j))) {
-        const float2 x = c[k][j];
+    for (int j = 0; j < 2; ++j)
+      if (g.mask & (1u << (2 * k + j))) {
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
-          acc[k][j][q][0] = fmaf(p.v[4 * q + 0], x.x, acc[k][j][q][0]);
-          acc[k][j][q][0] = fmaf(p.v[4 * q + 1], x.y, acc[k][j][q][0]);
-          acc[k][j][q][1] = fmaf(p.v[4 * q + 2], x.x, acc[k][j][q][1]);
-          acc[k][j][q][1] = fmaf(p.v[4 * q + 3], x.y, acc[k][j][q][1]);
+        for (int e = 0; e < 2; ++e) {
+          const float2 x = c[k][j][e];
+#pragma unroll
+          for (int q = 0; q < NP; ++q) {
+            acc[k][j][e][q][0] = fmaf(p.v[4 * q + 0], x.x, acc[k][j][e][q][0]);
+            acc[k][j][e][q][0] = fmaf(p.v[4 * q + 1], x.y, acc[k][j][e][q][0]);
+            acc[k][j][e][q][1] = fmaf(p.v[4 * q + 2], x.x, acc[k][j][e][q][1]);
+            acc[k][j][e][q][1] = fmaf(p.v[4 * q + 3], x.y, acc[k][j][e][q][1]);
+          }
         }
       }
 }
-
-// Runs consume(c, geom, pair, pi) for the pairs [pb, pe) of a work item, with the CF taps of the
-// pair in c.  PIPE: two register sets; the loads (and the scalar record fetch two pairs ahead)
-// are issued before the previous pair is consumed.  consume must not issue vector memory loads
-// whose results it waits for after returning (the wait before the next loads covers them).
-// The CF table as a buffer resource that starts pad bytes before the table.
-struct CFView {
-  __amdgpu_buffer_rsrc_t r;
-  uint32_t pad;
-};
 
 // Kernel argument describing the CF table: its address and size, and the pad (bytes, at least
 // 16 rows of the widest plane) by which the buffer resource starts before it.
@@ -455,54 +482,58 @@ struct CFRef {
   uint32_t bytes, pad;
 };
 
-__device__ __forceinline__ CFView make_cf_view(const CFRef& t) {
-  CFView v;
-  v.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(reinterpret_cast<const char*>(t.ptr)) - t.pad, (short)0,
-                                          (int)(t.bytes + t.pad), kBufferWord3);
-  v.pad = t.pad;
-  return v;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(reinterpret_cast<const char*>(t.ptr)) - t.pad, (short)0,
+                                           (int)(t.bytes + t.pad), kBufferWord3);
 }
 
+// Runs consume(c, geom, pair, pi) for the pairs [pb, pe) of a work item in wave w, with the CF
+// taps of the pair in c.  The descriptor (and coefficients) of the next pair are fetched before
+// the current pair is worked.  PIPE: two register sets; the CF loads of pair i+1 are issued
+// before pair i is consumed.  consume must not issue vector memory loads whose results it waits
+// for after returning (the wait before the next loads covers them).
 template <int NV, bool PIPE, class F>
-__device__ __forceinline__ void for_each_pair(const int* __restrict__ pairs, const SamplesIn& smp, const CFView& cfv,
-                                              int pb, int pe, int u0, int v0w, int lu, int lv, F&& consume) {
-  const auto voff = [&](const PairGeom& g) { return (uint32_t)(lv * g.P + lu) * (uint32_t)sizeof(float2); };
+__device__ __forceinline__ void for_each_pair(const int4* __restrict__ descs, const float* __restrict__ coef,
+                                              __amdgpu_buffer_rsrc_t cfr, int pb, int pe, int w, int lu, int lv,
+                                              F&& consume) {
+  if (pb >= pe) return;
   if constexpr (!PIPE) {
+    PairU<NV> pn = fetch_pair<NV>(descs, coef, pb);
     for (int pi = pb; pi < pe; ++pi) {
       // Nothing is in flight here (the previous pair consumed its taps); saying so keeps the
       // wait-count analysis, which cannot correlate the load and use conditions of a block,
       // from waiting for all loads before each load that reuses a register.
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      const PairU<NV> p = fetch_pair<NV>(pairs, smp, pi);
-      const PairGeom g = pair_geom(p, u0, v0w, cfv.pad);
-      float2 c[4][4];
-      load_cf(c, cfv.r, g, voff(g));
+      const PairU<NV> p = pn;
+      if (pi + 1 < pe) pn = fetch_pair<NV>(descs, coef, pi + 1);
+      const PairGeom g = pair_geom(p, w);
+      Taps c;
+      load_cf(c, cfr, g, lu, lv);
       consume(c, g, p, pi);
     }
   } else {
-    if (pb >= pe) return;
     int pi = pb;
-    float2 cA[4][4], cB[4][4];
-    PairU<NV> pA = fetch_pair<NV>(pairs, smp, pi);
-    PairGeom gA = pair_geom(pA, u0, v0w, cfv.pad);
-    load_cf(cA, cfv.r, gA, voff(gA));
+    Taps cA, cB;
+    PairU<NV> pA = fetch_pair<NV>(descs, coef, pi);
+    PairGeom gA = pair_geom(pA, w);
+    load_cf(cA, cfr, gA, lu, lv);
     PairU<NV> pB = pA;
-    if (pi + 1 < pe) pB = fetch_pair<NV>(pairs, smp, pi + 1);
+    if (pi + 1 < pe) pB = fetch_pair<NV>(descs, coef, pi + 1);
     for (;;) {
       // pair pi: taps in flight into cA; pB = record of pair pi+1
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      const PairGeom gB = pair_geom(pB, u0, v0w, cfv.pad);
-      if (pi + 1 < pe) load_cf(cB, cfv.r, gB, voff(gB));
+      const PairGeom gB = pair_geom(pB, w);
+      if (pi + 1 < pe) load_cf(cB, cfr, gB, lu, lv);
       PairU<NV> pN = pB;
-      if (pi + 2 < pe) pN = fetch_pair<NV>(pairs, smp, pi + 2);
+      if (pi + 2 < pe) pN = fetch_pair<NV>(descs, coef, pi + 2);
       consume(cA, gA, pA, pi);
       if (++pi >= pe) break;
       // pair pi: taps in flight into cB; pN = record of pair pi+1
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      gA = pair_geom(pN, u0, v0w, cfv.pad);
-      if (pi + 1 < pe) load_cf(cA, cfv.r, gA, voff(gA));
+      gA = pair_geom(pN, w);
+      if (pi + 1 < pe) load_cf(cA, cfr, gA, lu, lv);
       PairU<NV> pM = pN;
-      if (pi + 2 < pe) pM = fetch_pair<NV>(pairs, smp, pi + 2);
+      if (pi + 2 < pe) pM = fetch_pair<NV>(descs, coef, pi + 2);
       consume(cB, gB, pB, pi);
       if (++pi >= pe) break;
       pA = pN;
@@ -513,13 +544,11 @@ __device__ __forceinline__ void for_each_pair(const int* __restrict__ pairs, con
 
 template <int NP, int WPE, bool PIPE>
 __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
-                                              int* __restrict__ counter, const int* __restrict__ pairs,
-                                              const SampleHdr* __restrict__ hdr, const float* __restrict__ coef,
-                                              CFRef cft, float* __restrict__ grid, int nx, int ny, int ntu, int ntiles,
-                                              int nzg) {
+                                              int* __restrict__ counter, const int4* __restrict__ descs,
+                                              const float* __restrict__ coef, CFRef cft, float* __restrict__ grid,
+                                              int nx, int ny, int ntu, int ntiles, int nzg) {
   __shared__ int s_item;
-  const SamplesIn smp{hdr, coef};
-  const CFView cfv = make_cf_view(cft);
+  const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(cft);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -536,18 +565,19 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     const int kz = it.x / ntiles;
     const int ichan = kz / nzg;
     const int u0 = (tile % ntu) * kTile, v0 = (tile / ntu) * kTile;
-    const int v0w = v0 + 16 * w;
 
-    float acc[4][4][NP][2];
+    float acc[4][2][2][NP][2];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int q = 0; q < NP; ++q) acc[k][j][q][0] = acc[k][j][q][1] = 0.f;
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+          for (int q = 0; q < NP; ++q) acc[k][j][e][q][0] = acc[k][j][e][q][1] = 0.f;
 
-    for_each_pair<4 * NP, PIPE>(pairs, smp, cfv, it.y, it.z, u0, v0w, lu, lv,
-                                [&](const float2(&c)[4][4], const PairGeom& g, const PairU<4 * NP>& p, int) {
+    for_each_pair<4 * NP, PIPE>(descs, coef, cfr, it.y, it.z, w, lu, lv,
+                                [&](const Taps& c, const PairGeom& g, const PairU<4 * NP>& p, int) {
                                   grid_pair<NP>(acc, c, g, p);
                                 });
     // flush
@@ -556,14 +586,16 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     for (int k = 0; k < 4; ++k) {
       const int v = v0 + 16 * w + 4 * k + lv;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int u = u0 + 16 * j + lu;
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+        const int u = u0 + 32 * j + 2 * lu + e;
         if (u < nx && v < ny) {
 #pragma unroll
           for (int q = 0; q < NP; ++q) {
             float* g = grid + 2 * ((size_t)(ichan * NP + q) * plane + (size_t)v * nx + u);
-            if (acc[k][j][q][0] != 0.f) atomicAdd(g, acc[k][j][q][0]);
-            if (acc[k][j][q][1] != 0.f) atomicAdd(g + 1, acc[k][j][q][1]);
+            if (acc[k][j][e][q][0] != 0.f) atomicAdd(g, acc[k][j][e][q][0]);
+            if (acc[k][j][e][q][1] != 0.f) atomicAdd(g + 1, acc[k][j][e][q][1]);
           }
         }
       }
@@ -617,37 +649,38 @@ __device__ __forceinline__ float wave_reduce_scatter8(const float (&x)[8], int l
 // sum over the wave's active blocks of C * conj(G) per polarisation (GridKernel.cc:121-122);
 // x[2q] / x[2q+1] = real / imaginary part for polarisation q, x[2NP..7] = 0
 template <int NP>
-__device__ __forceinline__ void degrid_pair(float (&x)[8], const float2 (&c)[4][4], const float2 (&gt)[4][4][NP],
+__device__ __forceinline__ void degrid_pair(float (&x)[8], const Taps& c, const float2 (&gt)[4][2][2][NP],
                                             const PairGeom& g) {
 #pragma unroll
   for (int i = 0; i < 8; ++i) x[i] = 0.f;
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (g.mask & (1u << (4 * k + j))) {
-        const float2 cc = c[k][j];
+    for (int j = 0; j < 2; ++j)
+      if (g.mask & (1u << (2 * k + j))) {
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
-          x[2 * q] = fmaf(cc.x, gt[k][j][q].x, x[2 * q]);
-          x[2 * q] = fmaf(cc.y, gt[k][j][q].y, x[2 * q]);
-          x[2 * q + 1] = fmaf(cc.y, gt[k][j][q].x, x[2 * q + 1]);
-          x[2 * q + 1] = fmaf(-cc.x, gt[k][j][q].y, x[2 * q + 1]);
+        for (int e = 0; e < 2; ++e) {
+          const float2 cc = c[k][j][e];
+#pragma unroll
+          for (int q = 0; q < NP; ++q) {
+            x[2 * q] = fmaf(cc.x, gt[k][j][e][q].x, x[2 * q]);
+            x[2 * q] = fmaf(cc.y, gt[k][j][e][q].y, x[2 * q]);
+            x[2 * q + 1] = fmaf(cc.y, gt[k][j][e][q].x, x[2 * q + 1]);
+            x[2 * q + 1] = fmaf(-cc.x, gt[k][j][e][q].y, x[2 * q + 1]);
+          }
         }
       }
 }
 
 template <int NP, int WPE, bool PIPE>
 __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
-                                                int* __restrict__ counter, const int* __restrict__ pairs,
-                                                const SampleHdr* __restrict__ hdr, const float* __restrict__ coef,
+                                                int* __restrict__ counter, const int4* __restrict__ descs,
                                                 CFRef cft, const float2* __restrict__ grid, int nx, int ny, int ntu,
                                                 int ntiles, int nzg, float* __restrict__ acc_out) {
   __shared__ int s_item;
-  const CFView cfv = make_cf_view(cft);
+  const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(cft);
   __shared__ float s_part[kDegridBatch][4][8];
   __shared__ int s_sid[kDegridBatch];
-  const SamplesIn smp{hdr, coef};
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -667,31 +700,33 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     const int u0 = (tile % ntu) * kTile, v0 = (tile / ntu) * kTile;
     const size_t plane = (size_t)nx * ny;
 
-    float2 gt[4][4][NP];
+    float2 gt[4][2][2][NP];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int v = v0 + 16 * w + 4 * k + lv;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int u = u0 + 16 * j + lu;
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int q = 0; q < NP; ++q)
-          gt[k][j][q] = (u < nx && v < ny) ? grid[(size_t)(ichan * NP + q) * plane + (size_t)v * nx + u]
-                                           : make_float2(0.f, 0.f);
-      }
+        for (int e = 0; e < 2; ++e) {
+          const int u = u0 + 32 * j + 2 * lu + e;
+#pragma unroll
+          for (int q = 0; q < NP; ++q)
+            gt[k][j][e][q] = (u < nx && v < ny) ? grid[(size_t)(ichan * NP + q) * plane + (size_t)v * nx + u]
+                                                : make_float2(0.f, 0.f);
+        }
     }
 
     // per pair: wave dot product, reduce-scatter, one LDS slot per (pair, wave, value); every
     // kDegridBatch pairs the four waves' partial sums meet and go to the per-sample accumulators
     const int pb = it.y, pe = it.z;
-    for_each_pair<0, PIPE>(pairs, smp, cfv, pb, pe, u0, v0 + 16 * w, lu, lv,
-                           [&](const float2(&c)[4][4], const PairGeom& g, const PairU<0>&, int pi) {
+    for_each_pair<0, PIPE>(descs, nullptr, cfr, pb, pe, w, lu, lv,
+                           [&](const Taps& c, const PairGeom& g, const PairU<0>& p, int pi) {
                              float x[8];
                              degrid_pair<NP>(x, c, gt, g);
                              const float r = wave_reduce_scatter8(x, lane);
                              const int b = (pi - pb) % kDegridBatch;
                              if (lane < 8) s_part[b][w][slot] = r;
-                             if (tid == 0) s_sid[b] = pairs[pi];
+                             if (tid == 0) s_sid[b] = p.d.x;
                              if (b == kDegridBatch - 1 || pi == pe - 1) {
                                __syncthreads();
                                const int nb = b + 1;
@@ -737,7 +772,14 @@ void launch_prep(const Ctx& c, const PrepParams& p, const PrepInputs& in, const 
   YG_HIP(hipGetLastError());
 }
 
-void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, int nkz, Binning& b) {
+// Bytes by which the tile kernels' CF buffer resource starts before the table: 16 rows of the
+// widest plane and a little (a wave with an active block starts at most 11 rows and 47 columns
+// before its plane).
+static uint32_t cf_pad_bytes(int max_support) {
+  return (uint32_t)((16 * (2 * (size_t)max_support + 1 + 2 * kGuardU) + 64) * sizeof(float2));
+}
+
+void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, int nkz, int max_support, Binning& b) {
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const size_t ntiles = (size_t)ntu * ntv;
   const size_t nkeys = ntiles * (size_t)nkz;
@@ -761,11 +803,11 @@ void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, in
   YG_HIP(hipMemcpyAsync(&npairs, offsets + nkeys, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   YG_HIP(hipStreamSynchronize(c.stream));
   YG_CHECK(npairs >= 0, 3, "pair count overflow");
-  b.pairs.resize((size_t)(npairs > 0 ? npairs : 1) * sizeof(int));
+  b.pairs.resize((size_t)(npairs > 0 ? npairs : 1) * sizeof(int4));
   YG_HIP(hipMemcpyAsync(b.cursor.get(), offsets, (nkeys + 1) * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, n, ntu, (int)ntiles,
-                       b.cursor.as<int>(), b.pairs.as<int>());
+                       cf_pad_bytes(max_support), b.cursor.as<int>(), b.pairs.as<int4>());
   // work items: reuse cursor as the per-key item count and its scan
   int* icount = b.cursor.as<int>();
   hipLaunchKernelGGL(k_item_count, dim3((nkeys + 1 + 255) / 256), dim3(256), 0, c.stream, counts, (int)nkeys, icount);
@@ -783,7 +825,7 @@ void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, in
   YG_HIP(hipGetLastError());
 }
 
-static int persistent_blocks(const Ctx& c) { return c.n_cu * (c.waves_per_simd >= 3 ? 3 : 2); }
+static int persistent_blocks(const Ctx& c, const TileVariant& v) { return c.n_cu * (v.waves >= 3 ? 3 : 2); }
 
 // kernel variant by polarisation count, register budget (waves per SIMD) and pipelining
 template <int NP, int WPE, bool PIPE>
@@ -791,42 +833,43 @@ struct TileKernels {
   static void grid(hipStream_t s, dim3 g, const Binning& b, const Samples& smp, CFRef cft, float* gf, int nx, int ny,
                    int ntu, int ntiles, int nzg) {
     hipLaunchKernelGGL((k_grid<NP, WPE, PIPE>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
-                       b.counter.as<int>(), b.pairs.as<int>(), smp.hdr, smp.coef, cft, gf, nx, ny, ntu, ntiles, nzg);
+                       b.counter.as<int>(), b.pairs.as<int4>(), smp.coef, cft, gf, nx, ny, ntu, ntiles, nzg);
   }
-  static void degrid(hipStream_t s, dim3 g, const Binning& b, const Samples& smp, CFRef cft, const float2* grid,
-                     int nx, int ny, int ntu, int ntiles, int nzg, float* acc) {
+  static void degrid(hipStream_t s, dim3 g, const Binning& b, CFRef cft, const float2* grid, int nx, int ny, int ntu,
+                     int ntiles, int nzg, float* acc) {
     hipLaunchKernelGGL((k_degrid<NP, WPE, PIPE>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
-                       b.counter.as<int>(), b.pairs.as<int>(), smp.hdr, smp.coef, cft, grid, nx, ny, ntu, ntiles, nzg,
-                       acc);
+                       b.counter.as<int>(), b.pairs.as<int4>(), cft, grid, nx, ny, ntu, ntiles, nzg, acc);
   }
 };
 
 // The tile kernels address the CF table through a buffer resource with 32-bit byte offsets
-// that starts 16 rows of the widest plane before the table.
+// that starts cf_pad_bytes before the table.
 static CFRef cf_ref(const float2* cf, size_t cf_bytes, int max_support) {
-  const size_t pad = (size_t)(16 * (2 * (size_t)max_support + 1 + 2 * kGuardU) + 64) * sizeof(float2);
-  YG_CHECK(cf_bytes > 0 && cf_bytes + pad <= 0xFFFFFFF0ull, 1, "CF table must be smaller than 4 GiB");
+  const size_t pad = cf_pad_bytes(max_support);
+  // below 2 GiB, so that an out-of-range lane offset (2^31) plus any in-range scalar offset
+  // neither wraps nor lands inside the buffer
+  YG_CHECK(cf_bytes > 0 && cf_bytes + pad < kOutOfRange, 1, "CF table must be smaller than 2 GiB");
   return CFRef{cf, (uint32_t)cf_bytes, (uint32_t)pad};
 }
 
 template <template <int, int, bool> class K, int NP, class F>
-static void with_variant(const Ctx& c, F&& f) {
-  if (c.waves_per_simd >= 3) {
-    if (c.pipeline) f(K<NP, 3, true>{});
+static void with_variant(const TileVariant& v, F&& f) {
+  if (v.waves >= 3) {
+    if (v.pipeline) f(K<NP, 3, true>{});
     else f(K<NP, 3, false>{});
   } else {
-    if (c.pipeline) f(K<NP, 2, true>{});
+    if (v.pipeline) f(K<NP, 2, true>{});
     else f(K<NP, 2, false>{});
   }
 }
 
 template <class F>
-static void with_tile_kernels(const Ctx& c, int npol, F&& f) {
+static void with_tile_kernels(const TileVariant& v, int npol, F&& f) {
   switch (npol) {
-    case 1: with_variant<TileKernels, 1>(c, f); break;
-    case 2: with_variant<TileKernels, 2>(c, f); break;
-    case 3: with_variant<TileKernels, 3>(c, f); break;
-    case 4: with_variant<TileKernels, 4>(c, f); break;
+    case 1: with_variant<TileKernels, 1>(v, f); break;
+    case 2: with_variant<TileKernels, 2>(v, f); break;
+    case 3: with_variant<TileKernels, 3>(v, f); break;
+    case 4: with_variant<TileKernels, 4>(v, f); break;
     default: throw Error(1, "unsupported number of polarisations (1..4)");
   }
 }
@@ -837,21 +880,22 @@ void launch_grid(const Ctx& c, const Binning& b, const Samples& smp, const float
   const int ntiles = ntu * ntv;
   const CFRef cft = cf_ref(cf, cf_bytes, max_support);
   YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
-  const dim3 g(persistent_blocks(c));
+  const dim3 g(persistent_blocks(c, c.grid_variant));
   float* gf = reinterpret_cast<float*>(grid);
-  with_tile_kernels(c, npol, [&](auto k) { k.grid(c.stream, g, b, smp, cft, gf, nx, ny, ntu, ntiles, nzg); });
+  with_tile_kernels(c.grid_variant, npol,
+                    [&](auto k) { k.grid(c.stream, g, b, smp, cft, gf, nx, ny, ntu, ntiles, nzg); });
   YG_HIP(hipGetLastError());
 }
 
-void launch_degrid(const Ctx& c, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes,
-                   int max_support, const float2* grid, int nx, int ny, int npol, int nzg, float* acc) {
+void launch_degrid(const Ctx& c, const Binning& b, const float2* cf, size_t cf_bytes, int max_support,
+                   const float2* grid, int nx, int ny, int npol, int nzg, float* acc) {
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const int ntiles = ntu * ntv;
   const CFRef cft = cf_ref(cf, cf_bytes, max_support);
   YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
-  const dim3 g(persistent_blocks(c));
-  with_tile_kernels(c, npol,
-                    [&](auto k) { k.degrid(c.stream, g, b, smp, cft, grid, nx, ny, ntu, ntiles, nzg, acc); });
+  const dim3 g(persistent_blocks(c, c.degrid_variant));
+  with_tile_kernels(c.degrid_variant, npol,
+                    [&](auto k) { k.degrid(c.stream, g, b, cft, grid, nx, ny, ntu, ntiles, nzg, acc); });
   YG_HIP(hipGetLastError());
 }
 

@@ -2,6 +2,7 @@
 // C++ exceptions and turns them into a status code plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -56,8 +57,15 @@ int ygrid_ctx_create(int device, ygrid_ctx** out) {
     hipDeviceProp_t prop;
     YG_HIP(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    if (const char* v = std::getenv("YGRID_WAVES_PER_SIMD")) c->waves_per_simd = std::atoi(v);
-    if (const char* v = std::getenv("YGRID_PIPELINE")) c->pipeline = std::atoi(v);
+    // tile-kernel variants "waves,pipeline" (e.g. "3,0"): performance tuning only
+    const auto variant = [](const char* env, ygrid::TileVariant& v) {
+      if (const char* s = std::getenv(env)) {
+        int w = 0, p = 0;
+        if (std::sscanf(s, "%d,%d", &w, &p) == 2 && (w == 2 || w == 3)) v = ygrid::TileVariant{w, p != 0};
+      }
+    };
+    variant("YGRID_GRID_VARIANT", c->grid_variant);
+    variant("YGRID_DEGRID_VARIANT", c->degrid_variant);
     YG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
     *out = new ygrid_ctx{c};

@@ -162,25 +162,35 @@ struct Binning {
 };
 
 // hipStream + device id of a context.
+// Register budget and pipelining of a tile kernel: waves = waves per SIMD the kernel is built
+// for (2: <= 256 VGPRs, 3: <= 168), pipeline = CF loads of pair i+1 issued before pair i's
+// arithmetic (a second register set).
+struct TileVariant {
+  int waves;
+  int pipeline;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int n_cu = 256;
-  int waves_per_simd = 3;  // register budget of the grid/degrid kernels (2: <=256 VGPRs, 3: <=168)
-  int pipeline = 0;        // grid/degrid: issue pair i+1's CF loads before pair i's arithmetic
+  TileVariant grid_variant{3, 0};
+  TileVariant degrid_variant{2, 1};
 };
 
 // Kernel launchers (kernels_grid.hip)
 void launch_prep(const Ctx&, const PrepParams&, const PrepInputs&, const PrepOutputs&);
 // Bins valid records by (key, tile); returns nothing, work lands in b.
-void launch_binning(const Ctx&, const Samples& smp, int nsamples, int nx, int ny, int nkeys_z, Binning& b);
+// max_support: largest support of the CF table the pairs will be worked with
+void launch_binning(const Ctx&, const Samples& smp, int nsamples, int nx, int ny, int nkeys_z, int max_support,
+                    Binning& b);
 // cf_bytes: size of the CF table (the kernels address it with 32-bit byte offsets);
 // max_support: largest support of its planes
 void launch_grid(const Ctx&, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes, int max_support,
                  float2* grid, int nx, int ny, int npol, int nzg);
-void launch_degrid(const Ctx&, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes,
-                   int max_support, const float2* grid, int nx, int ny, int npol, int nzg, float* acc);
+void launch_degrid(const Ctx&, const Binning& b, const float2* cf, size_t cf_bytes, int max_support,
+                   const float2* grid, int nx, int ny, int npol, int nzg, float* acc);
 void launch_degrid_finish(const Ctx&, const Samples& smp, const float* acc, int nrow, int nchan, int npol,
                           float2* vis_out);
 
