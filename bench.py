@@ -149,6 +149,9 @@ def main():
     dg.initialise_degrid(img, t_model)
 
     def step():
+        # every step bins the samples once: the gridding pass bins, the degridding pass of the
+        # same chunk reuses that binning (as in a major cycle); nothing carries over between steps
+        ctx.clear_cache()
         gg.initialise_grid(img)
         gg.grid(chunk)
         dg.degrid(chunk)

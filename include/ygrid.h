@@ -86,6 +86,7 @@ typedef struct {
   /* device time (HIP events on the gridder's stream) when profiling is enabled */
   double ms_prep, ms_bin, ms_grid_kernel, ms_degrid_kernel;
   double n_grid_launches, n_degrid_launches;
+  double n_bin_reused; /* passes that reused the context's cached sample binning */
 } ygrid_stats;
 
 /* ---- library / context ---- */
@@ -97,6 +98,11 @@ int ygrid_ctx_destroy(ygrid_ctx *ctx);
 /* Use an existing hipStream_t (may be NULL for the default stream). */
 int ygrid_ctx_set_stream(ygrid_ctx *ctx, void *hip_stream);
 int ygrid_ctx_synchronize(ygrid_ctx *ctx);
+/* Drops the context's cached sample binning (the next grid/degrid pass bins again).  A pass
+ * reuses the binning of the previous pass on the context when its samples map to the same
+ * pixels, CF planes and supports, as the degridding and gridding of one chunk do; no
+ * counterpart in the reference (TableVisGridder recomputes the mapping on every pass). */
+int ygrid_ctx_clear_cache(ygrid_ctx *ctx);
 
 /* ---- factory: VisGridderFactory::make / createGridder (VisGridderFactory.cc:112-286) ----
  * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUAWProject".

@@ -386,3 +386,41 @@ def test_unknown_gridder_is_rejected(gpu_ctx):
     with pytest.raises(ya.YgridError) as e:
         ya.Gridder(gpu_ctx, "NoSuchGridder", {})
     assert e.value.code == 5
+
+
+def test_binning_cache_reuse_and_invalidation(gpu_ctx):
+    announce("binning cache: degrid after grid of one chunk reuses it; changed samples bin again")
+    import yandasoft_amd as ya
+    N, cell, npol = 128, 20 * ARC, 2
+    img = ya.ImageDesc(N, N, npol, 1, cell, cell)
+    d = make_data(3000, 3, npol, N, cell, seed=11)
+    rng = np.random.default_rng(5)
+    model = np.zeros((1, npol, N, N), np.float32)
+    model[0, :, rng.integers(20, 108, 30), rng.integers(20, 108, 30)] = rng.normal(size=(30, npol))
+
+    gpu_ctx.clear_cache()
+    g = ya.Gridder(gpu_ctx, "GPUSphFunc", {"oversample": 8})
+    g.initialise_grid(img)
+    g.grid(pchunk(d))
+    dg = ya.Gridder(gpu_ctx, "GPUSphFunc", {"oversample": 8})
+    dg.initialise_degrid(img, model)
+    out1 = np.zeros_like(d["vis"])
+    dg.degrid(pchunk(d, vis_out=out1))
+    assert dg.stats().n_bin_reused == 1 and g.stats().n_bin_reused == 0
+
+    gpu_ctx.clear_cache()
+    out2 = np.zeros_like(d["vis"])
+    dg.degrid(pchunk(d, vis_out=out2))
+    assert dg.stats().n_bin_reused == 1
+    assert rel(out1, out2) < 1e-6
+
+    # different samples (and, separately, only different flags) must not reuse the binning
+    for d2 in (make_data(3000, 3, npol, N, cell, seed=12), dict(d, flag=make_data(3000, 3, npol, N, cell, flags=True)["flag"])):
+        g.initialise_grid(img)
+        g.grid(pchunk(d2))
+        assert g.stats().n_bin_reused == 0
+        gpu_ctx.clear_cache()
+        ref = ya.Gridder(gpu_ctx, "GPUSphFunc", {"oversample": 8})
+        ref.initialise_grid(img)
+        ref.grid(pchunk(d2))
+        assert rel(g.grid_array(), ref.grid_array()) < 1e-6

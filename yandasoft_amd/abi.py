@@ -13,7 +13,7 @@ YGRID_MEM_DEVICE = 1
 
 EXPORTED = [
     "ygrid_last_error", "ygrid_version", "ygrid_ctx_create", "ygrid_ctx_destroy", "ygrid_ctx_set_stream",
-    "ygrid_ctx_synchronize", "ygrid_gridder_create", "ygrid_gridder_clone", "ygrid_gridder_destroy",
+    "ygrid_ctx_synchronize", "ygrid_ctx_clear_cache", "ygrid_gridder_create", "ygrid_gridder_clone", "ygrid_gridder_destroy",
     "ygrid_initialise_grid", "ygrid_grid", "ygrid_finalise_grid", "ygrid_finalise_weights",
     "ygrid_initialise_degrid", "ygrid_degrid", "ygrid_finalise_degrid", "ygrid_is_model_empty",
     "ygrid_set_mfs_weight", "ygrid_get_stats", "ygrid_set_profiling", "ygrid_get_cf", "ygrid_get_grid",
@@ -48,7 +48,7 @@ class Stats(ctypes.Structure):
                 ("support", ctypes.c_int), ("oversample", ctypes.c_int), ("n_cf", ctypes.c_int),
                 ("ms_prep", ctypes.c_double), ("ms_bin", ctypes.c_double), ("ms_grid_kernel", ctypes.c_double),
                 ("ms_degrid_kernel", ctypes.c_double), ("n_grid_launches", ctypes.c_double),
-                ("n_degrid_launches", ctypes.c_double)]
+                ("n_degrid_launches", ctypes.c_double), ("n_bin_reused", ctypes.c_double)]
 
 
 def library_path() -> str:
@@ -149,6 +149,10 @@ class Context:
 
     def synchronize(self):
         _check(lib().ygrid_ctx_synchronize(self.h))
+
+    def clear_cache(self):
+        """Drop the cached sample binning; the next pass bins again."""
+        _check(lib().ygrid_ctx_clear_cache(self.h))
 
     def close(self):
         if self.h:

@@ -384,18 +384,19 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   const float2* cft = cf_->table.as<float2>();
   const size_t cfb = cf_->table.size();
   const int smax = cf_->supp.empty() ? 0 : *std::max_element(cf_->supp.begin(), cf_->supp.end());
-  launch_binning(*ctx_, smp, nsamp, shape_[0], shape_[1], pp.nzg * shape_[3], smax, bin_);
+  if (bin_samples(*ctx_, smp, nsamp, shape_[0], shape_[1], pp.nzg * shape_[3], smax)) ++n_bin_reused_;
+  const Binning& bins = ctx_->bins.bin;
   if (forward) {
     acc_.resize((size_t)nsamp * 8 * sizeof(float));
     YG_HIP(hipMemsetAsync(acc_.get(), 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_degrid(*ctx_, bin_, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg,
+    launch_degrid(*ctx_, bins, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg,
                   acc_.as<float>());
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
     launch_degrid_finish(*ctx_, smp, acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);
   } else {
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_grid(*ctx_, bin_, smp, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg);
+    launch_grid(*ctx_, bins, smp, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
   }
   unsigned long long cnt[4] = {0, 0, 0, 0};
@@ -550,6 +551,7 @@ void Gridder::stats(ygrid_stats* o) const {
   o->ms_degrid_kernel = ms_dk_;
   o->n_grid_launches = n_gk_;
   o->n_degrid_launches = n_dk_;
+  o->n_bin_reused = n_bin_reused_;
 }
 
 void Gridder::getCF(int k, float* out, int* support) const {

@@ -129,7 +129,7 @@ class Gridder {
   DevBuf grid_, sumw_, scratch_, ccfx_, ccfy_;
   DevBuf smp_hdr_, smp_key_, smp_coef_, acc_, counters_, err_, chanmap_, visweight_, staging_;
   Samples samples() const { return Samples{smp_hdr_.as<SampleHdr>(), smp_key_.as<int32_t>(), smp_coef_.as<float>()}; }
-  Binning bin_;
+  double n_bin_reused_ = 0;  // passes that reused the context's cached binning
   PlanPtr plan_;
   double st_samples_g_ = 0, st_taps_g_ = 0, st_samples_d_ = 0, st_taps_d_ = 0, st_flag_ = 0, st_wflag_ = 0;
   double t_grid_ = 0, t_degrid_ = 0, t_cf_ = 0;

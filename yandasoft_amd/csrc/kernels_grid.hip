@@ -825,6 +825,50 @@ void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, in
   YG_HIP(hipGetLastError());
 }
 
+namespace {
+// flag = 1 when sample s differs between (ha, ka) and (hb, kb) in the fields binning uses
+__global__ void k_samples_differ(const SampleHdr* __restrict__ ha, const int32_t* __restrict__ ka,
+                                 const SampleHdr* __restrict__ hb, const int32_t* __restrict__ kb, int n, int* flag) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const SampleHdr a = ha[s], b = hb[s];
+  const bool valid = a.iu0 != INT_MIN;
+  const bool same = a.iu0 == b.iu0 &&
+                    (!valid || (a.iv0 == b.iv0 && a.cfoff == b.cfoff && a.spack == b.spack && ka[s] == kb[s]));
+  if (!same) *flag = 1;
+}
+}  // namespace
+
+bool bin_samples(Ctx& c, const Samples& smp, int n, int nx, int ny, int nkz, int max_support) {
+  BinCache& bc = c.bins;
+  if (bc.valid && bc.nsamp == n && bc.nx == nx && bc.ny == ny && bc.nkz == nkz && bc.smax == max_support) {
+    bc.flag.resize(sizeof(int));
+    YG_HIP(hipMemsetAsync(bc.flag.get(), 0, sizeof(int), c.stream));
+    if (n > 0)
+      hipLaunchKernelGGL(k_samples_differ, dim3((n + 255) / 256), dim3(256), 0, c.stream, smp.hdr, smp.key,
+                         bc.hdr.as<SampleHdr>(), bc.key.as<int32_t>(), n, bc.flag.as<int>());
+    int differ = 1;
+    YG_HIP(hipMemcpyAsync(&differ, bc.flag.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    YG_HIP(hipStreamSynchronize(c.stream));
+    if (!differ) return true;
+  }
+  bc.valid = false;
+  launch_binning(c, smp, n, nx, ny, nkz, max_support, bc.bin);
+  bc.hdr.resize((size_t)(n > 0 ? n : 1) * sizeof(SampleHdr));
+  bc.key.resize((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+  if (n > 0) {
+    YG_HIP(hipMemcpyAsync(bc.hdr.get(), smp.hdr, (size_t)n * sizeof(SampleHdr), hipMemcpyDeviceToDevice, c.stream));
+    YG_HIP(hipMemcpyAsync(bc.key.get(), smp.key, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToDevice, c.stream));
+  }
+  bc.nsamp = n;
+  bc.nx = nx;
+  bc.ny = ny;
+  bc.nkz = nkz;
+  bc.smax = max_support;
+  bc.valid = true;
+  return false;
+}
+
 static int persistent_blocks(const Ctx& c, const TileVariant& v) { return c.n_cu * (v.waves >= 3 ? 3 : 2); }
 
 // kernel variant by polarisation count, register budget (waves per SIMD) and pipelining

@@ -103,6 +103,13 @@ int ygrid_ctx_synchronize(ygrid_ctx* ctx) {
   });
 }
 
+int ygrid_ctx_clear_cache(ygrid_ctx* ctx) {
+  return guard([&] {
+    NEED(ctx);
+    ctx->c->bins.valid = false;
+  });
+}
+
 int ygrid_gridder_create(ygrid_ctx* ctx, const char* name, const char* parset, ygrid_gridder** out) {
   return guard([&] {
     NEED(ctx);

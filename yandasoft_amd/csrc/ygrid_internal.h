@@ -170,6 +170,16 @@ struct TileVariant {
   int pipeline;
 };
 
+// The binning of the last pass run on a context, with the sample headers and keys it was built
+// from.  A pass whose samples bin identically (same headers and keys, grid and support: e.g. the
+// degridding and gridding of one chunk in a major cycle) reuses it.
+struct BinCache {
+  DevBuf hdr, key, flag;
+  int nsamp = -1, nx = 0, ny = 0, nkz = 0, smax = 0;
+  bool valid = false;
+  Binning bin;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -177,6 +187,7 @@ struct Ctx {
   int n_cu = 256;
   TileVariant grid_variant{3, 0};
   TileVariant degrid_variant{2, 1};
+  BinCache bins;  // shared by the gridders of the context (their passes are ordered on stream)
 };
 
 // Kernel launchers (kernels_grid.hip)
@@ -185,6 +196,9 @@ void launch_prep(const Ctx&, const PrepParams&, const PrepInputs&, const PrepOut
 // max_support: largest support of the CF table the pairs will be worked with
 void launch_binning(const Ctx&, const Samples& smp, int nsamples, int nx, int ny, int nkeys_z, int max_support,
                     Binning& b);
+// Bins the samples into ctx.bins unless the cached binning was built from identical sample
+// headers and keys with the same parameters; returns true when the cache was reused.
+bool bin_samples(Ctx& c, const Samples& smp, int nsamples, int nx, int ny, int nkeys_z, int max_support);
 // cf_bytes: size of the CF table (the kernels address it with 32-bit byte offsets);
 // max_support: largest support of its planes
 void launch_grid(const Ctx&, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes, int max_support,
