@@ -193,11 +193,15 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
                 } else {
                   rr = wt;
                 }
-                // MAC coefficients of rVis*C, or rVis*conj(C) (TableVisGridder.cc:837-844)
-                coef[4 * pol + 0] = rr;
-                coef[4 * pol + 1] = conj ? ri : -ri;
-                coef[4 * pol + 2] = ri;
-                coef[4 * pol + 3] = conj ? -rr : rr;
+                // MAC coefficients of rVis*C, or rVis*conj(C) (TableVisGridder.cc:837-844):
+                // re += a*C.re + b*C.im, im += c*C.re + d*C.im, stored as the gridding kernel's
+                // MFMA B operands: coef[4*o + m], output o = 2*(pol&1) + (re 0 / im 1),
+                // operand m = 2*(pol/2) + (C.re 0 / C.im 1)
+                const int ore = 2 * (pol & 1), mre = 2 * (pol >> 1);
+                coef[4 * ore + mre] = rr;
+                coef[4 * ore + mre + 1] = conj ? ri : -ri;
+                coef[4 * (ore + 1) + mre] = ri;
+                coef[4 * (ore + 1) + mre + 1] = conj ? -rr : rr;
               }
               if (!valid) rec.iu0 = INT_MIN;
             } else {
@@ -326,24 +330,24 @@ __global__ void k_bin_fill(Samples smp, int n, int ntu, int ntiles, uint32_t pad
     }
 }
 
-__global__ void k_item_count(const int* __restrict__ counts, int nkeys, int* icount) {
+__global__ void k_item_count(const int* __restrict__ counts, int nkeys, int chunk, int* icount) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > nkeys) return;
-  icount[k] = (k < nkeys) ? (counts[k] + kChunk - 1) / kChunk : 0;
+  icount[k] = (k < nkeys) ? (counts[k] + chunk - 1) / chunk : 0;
 }
 
 __global__ void k_item_emit(const int* __restrict__ counts, const int* __restrict__ offsets,
-                            const int* __restrict__ ioffsets, int nkeys, int4* items) {
+                            const int* __restrict__ ioffsets, int nkeys, int chunk, int4* items) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nkeys) return;
   const int c = counts[k];
   if (c == 0) return;
   const int o = offsets[k];
   const int io = ioffsets[k];
-  const int nch = (c + kChunk - 1) / kChunk;
+  const int nch = (c + chunk - 1) / chunk;
   for (int j = 0; j < nch; ++j) {
-    const int b = o + j * kChunk;
-    items[io + j] = make_int4(k, b, min(b + kChunk, o + c), 0);
+    const int b = o + j * chunk;
+    items[io + j] = make_int4(k, b, min(b + chunk, o + c), 0);
   }
 }
 
@@ -361,8 +365,14 @@ __global__ void k_item_emit(const int* __restrict__ counts, const int* __restric
 // issues the loads of pair i+1 before the FMAs of pair i into a second register set.
 // --------------------------------------------------------------------------------------
 
-// CF taps of one pair for one lane: [k][j][e]
-typedef float2 Taps[4][2][2];
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Operands of one pair for one lane: the CF taps [k][j][e] and, for gridding, the four MFMA
+// B operands of the sample (Samples::coef in the lane-major layout written by k_prep).
+struct Taps {
+  float2 t[4][2][2];
+  f32x4 b;
+};
 
 // s_waitcnt vmcnt(0) (expcnt and lgkmcnt unconstrained): every vector memory load issued so
 // far has returned.  Placed before the loads of the next pair are issued, it tells the
@@ -430,8 +440,12 @@ __device__ __forceinline__ void lane_offsets(uint32_t (&vo)[2], const PairGeom& 
 }
 
 // The CF taps of the active blocks: one 16-byte buffer load per lane and block, with the row in
-// the scalar offset and the lane's position in the vector offset.
-__device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, const PairGeom& g, int lu, int lv) {
+// the scalar offset and the lane's position in the vector offset.  COEF: also the sample's MFMA
+// B operands (16 bytes at coef + 16*s + boff, boff = the lane's output column).
+template <bool COEF>
+__device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, const PairGeom& g, int lu, int lv,
+                                        const float* __restrict__ coef, int s, int boff) {
+  if constexpr (COEF) c.b = *reinterpret_cast<const f32x4*>(coef + (size_t)s * kCoef + boff);
   uint32_t vo[2];
   lane_offsets(vo, g, lu, lv);
 #pragma unroll
@@ -445,17 +459,19 @@ __device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, con
         uint32_t d[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) d[i] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo[j] + 4u * i, so, 0);
-        c[k][j][0] = make_float2(__builtin_bit_cast(float, d[0]), __builtin_bit_cast(float, d[1]));
-        c[k][j][1] = make_float2(__builtin_bit_cast(float, d[2]), __builtin_bit_cast(float, d[3]));
+        c.t[k][j][0] = make_float2(__builtin_bit_cast(float, d[0]), __builtin_bit_cast(float, d[1]));
+        c.t[k][j][1] = make_float2(__builtin_bit_cast(float, d[2]), __builtin_bit_cast(float, d[3]));
       }
 }
 
 // acc += v * C (or v * conj(C) in the PCF half plane, TableVisGridder.cc:837-844) for the
-// active blocks with the sample's MAC coefficients (a, b, c, d) per polarisation, which are
-// wave-uniform scalar FMA operands: acc_re += a*C.x + b*C.y, acc_im += c*C.x + d*C.y.
+// active blocks, on the matrix cores: per block, tap e and output set m (polarisations 2m, 2m+1)
+// two 4x4x1 MFMAs (16 independent 4x4 outer products per wave) add C.re (x) B[2m] and
+// C.im (x) B[2m+1].  MFMA block b = lane/4 holds the taps of its lanes 4b..4b+3 (4 rows) as A and
+// the sample's coefficients for outputs (pol 2m re, pol 2m im, pol 2m+1 re, pol 2m+1 im) as B;
+// the result for tap row i and output o lands in lane 4b+o, register i.
 template <int NP>
-__device__ __forceinline__ void grid_pair(float (&acc)[4][2][2][NP][2], const Taps& c, const PairGeom& g,
-                                          const PairU<4 * NP>& p) {
+__device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], const Taps& c, const PairGeom& g) {
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -463,13 +479,12 @@ __device__ __forceinline__ void grid_pair(float (&acc)[4][2][2][NP][2], const Ta
       if (g.mask & (1u << (2 * k + j))) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          const float2 x = c[k][j][e];
-#pragma unroll
-          for (int q = 0; q < NP; ++q) {
-            acc[k][j][e][q][0] = fmaf(p.v[4 * q + 0], x.x, acc[k][j][e][q][0]);
-            acc[k][j][e][q][0] = fmaf(p.v[4 * q + 1], x.y, acc[k][j][e][q][0]);
-            acc[k][j][e][q][1] = fmaf(p.v[4 * q + 2], x.x, acc[k][j][e][q][1]);
-            acc[k][j][e][q][1] = fmaf(p.v[4 * q + 3], x.y, acc[k][j][e][q][1]);
+          const float2 x = c.t[k][j][e];
+          acc[k][j][e][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(x.x, c.b[0], acc[k][j][e][0], 0, 0, 0);
+          acc[k][j][e][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(x.y, c.b[1], acc[k][j][e][0], 0, 0, 0);
+          if constexpr (NP > 2) {
+            acc[k][j][e][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(x.x, c.b[2], acc[k][j][e][1], 0, 0, 0);
+            acc[k][j][e][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(x.y, c.b[3], acc[k][j][e][1], 0, 0, 0);
           }
         }
       }
@@ -488,52 +503,52 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
 }
 
 // Runs consume(c, geom, pair, pi) for the pairs [pb, pe) of a work item in wave w, with the CF
-// taps of the pair in c.  The descriptor (and coefficients) of the next pair are fetched before
-// the current pair is worked.  PIPE: two register sets; the CF loads of pair i+1 are issued
-// before pair i is consumed.  consume must not issue vector memory loads whose results it waits
-// for after returning (the wait before the next loads covers them).
-template <int NV, bool PIPE, class F>
+// taps (and, COEF, the MFMA B operands) of the pair in c.  The descriptor of the next pair is
+// fetched before the current pair is worked.  PIPE: two register sets; the loads of pair i+1
+// are issued before pair i is consumed.  consume must not issue vector memory loads whose
+// results it waits for after returning (the wait before the next loads covers them).
+template <bool COEF, bool PIPE, class F>
 __device__ __forceinline__ void for_each_pair(const int4* __restrict__ descs, const float* __restrict__ coef,
                                               __amdgpu_buffer_rsrc_t cfr, int pb, int pe, int w, int lu, int lv,
-                                              F&& consume) {
+                                              int boff, F&& consume) {
   if (pb >= pe) return;
   if constexpr (!PIPE) {
-    PairU<NV> pn = fetch_pair<NV>(descs, coef, pb);
+    PairU<0> pn = fetch_pair<0>(descs, coef, pb);
     for (int pi = pb; pi < pe; ++pi) {
       // Nothing is in flight here (the previous pair consumed its taps); saying so keeps the
       // wait-count analysis, which cannot correlate the load and use conditions of a block,
       // from waiting for all loads before each load that reuses a register.
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      const PairU<NV> p = pn;
-      if (pi + 1 < pe) pn = fetch_pair<NV>(descs, coef, pi + 1);
+      const PairU<0> p = pn;
+      if (pi + 1 < pe) pn = fetch_pair<0>(descs, coef, pi + 1);
       const PairGeom g = pair_geom(p, w);
       Taps c;
-      load_cf(c, cfr, g, lu, lv);
+      load_cf<COEF>(c, cfr, g, lu, lv, coef, p.d.x, boff);
       consume(c, g, p, pi);
     }
   } else {
     int pi = pb;
     Taps cA, cB;
-    PairU<NV> pA = fetch_pair<NV>(descs, coef, pi);
+    PairU<0> pA = fetch_pair<0>(descs, coef, pi);
     PairGeom gA = pair_geom(pA, w);
-    load_cf(cA, cfr, gA, lu, lv);
-    PairU<NV> pB = pA;
-    if (pi + 1 < pe) pB = fetch_pair<NV>(descs, coef, pi + 1);
+    load_cf<COEF>(cA, cfr, gA, lu, lv, coef, pA.d.x, boff);
+    PairU<0> pB = pA;
+    if (pi + 1 < pe) pB = fetch_pair<0>(descs, coef, pi + 1);
     for (;;) {
       // pair pi: taps in flight into cA; pB = record of pair pi+1
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       const PairGeom gB = pair_geom(pB, w);
-      if (pi + 1 < pe) load_cf(cB, cfr, gB, lu, lv);
-      PairU<NV> pN = pB;
-      if (pi + 2 < pe) pN = fetch_pair<NV>(descs, coef, pi + 2);
+      if (pi + 1 < pe) load_cf<COEF>(cB, cfr, gB, lu, lv, coef, pB.d.x, boff);
+      PairU<0> pN = pB;
+      if (pi + 2 < pe) pN = fetch_pair<0>(descs, coef, pi + 2);
       consume(cA, gA, pA, pi);
       if (++pi >= pe) break;
       // pair pi: taps in flight into cB; pN = record of pair pi+1
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       gA = pair_geom(pN, w);
-      if (pi + 1 < pe) load_cf(cA, cfr, gA, lu, lv);
-      PairU<NV> pM = pN;
-      if (pi + 2 < pe) pM = fetch_pair<NV>(descs, coef, pi + 2);
+      if (pi + 1 < pe) load_cf<COEF>(cA, cfr, gA, lu, lv, coef, pN.d.x, boff);
+      PairU<0> pM = pN;
+      if (pi + 2 < pe) pM = fetch_pair<0>(descs, coef, pi + 2);
       consume(cB, gB, pB, pi);
       if (++pi >= pe) break;
       pA = pN;
@@ -552,7 +567,11 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int lu = lane & 15, lv = lane >> 4;
+  // MFMA layout: lane 4b+i loads the taps of row i, columns 2b, 2b+1 of a block (A operands) and
+  // holds, after the products, output o = lane & 3 of rows 0..3 at columns 2b+e
+  const int lu = lane >> 2, lv = lane & 3;
+  const int o = lane & 3;
+  constexpr int NS = (NP + 1) / 2;  // output sets of two polarisations
   const int nitems = *nitems_p;
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
@@ -566,7 +585,7 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     const int ichan = kz / nzg;
     const int u0 = (tile % ntu) * kTile, v0 = (tile / ntu) * kTile;
 
-    float acc[4][2][2][NP][2];
+    f32x4 acc[4][2][2][NS];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -574,31 +593,33 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
 #pragma unroll
         for (int e = 0; e < 2; ++e)
 #pragma unroll
-          for (int q = 0; q < NP; ++q) acc[k][j][e][q][0] = acc[k][j][e][q][1] = 0.f;
+          for (int m = 0; m < NS; ++m) acc[k][j][e][m] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for_each_pair<4 * NP, PIPE>(descs, coef, cfr, it.y, it.z, w, lu, lv,
-                                [&](const Taps& c, const PairGeom& g, const PairU<4 * NP>& p, int) {
-                                  grid_pair<NP>(acc, c, g, p);
-                                });
-    // flush
+    for_each_pair<true, PIPE>(descs, coef, cfr, it.y, it.z, w, lu, lv, 4 * o,
+                              [&](const Taps& c, const PairGeom& g, const PairU<0>&, int) {
+                                grid_pair<NP>(acc, c, g);
+                              });
+    // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of rows 4k+i, column 32j+2b+e
     const size_t plane = (size_t)nx * ny;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int v = v0 + 16 * w + 4 * k + lv;
+    for (int m = 0; m < NS; ++m) {
+      const int q = 2 * m + (o >> 1);
+      if (q >= NP) continue;
+      float* gq = grid + 2 * (size_t)(ichan * NP + q) * plane + (o & 1);
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 4; ++k)
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-        const int u = u0 + 32 * j + 2 * lu + e;
-        if (u < nx && v < ny) {
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-          for (int q = 0; q < NP; ++q) {
-            float* g = grid + 2 * ((size_t)(ichan * NP + q) * plane + (size_t)v * nx + u);
-            if (acc[k][j][e][q][0] != 0.f) atomicAdd(g, acc[k][j][e][q][0]);
-            if (acc[k][j][e][q][1] != 0.f) atomicAdd(g + 1, acc[k][j][e][q][1]);
+          for (int e = 0; e < 2; ++e) {
+            const int u = u0 + 32 * j + 2 * lu + e;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int v = v0 + 16 * w + 4 * k + i;
+              const float val = acc[k][j][e][m][i];
+              if (val != 0.f && u < nx && v < ny) atomicAdd(gq + 2 * ((size_t)v * nx + u), val);
+            }
           }
-        }
-      }
     }
   }
 }
@@ -660,7 +681,7 @@ __device__ __forceinline__ void degrid_pair(float (&x)[8], const Taps& c, const 
       if (g.mask & (1u << (2 * k + j))) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          const float2 cc = c[k][j][e];
+          const float2 cc = c.t[k][j][e];
 #pragma unroll
           for (int q = 0; q < NP; ++q) {
             x[2 * q] = fmaf(cc.x, gt[k][j][e][q].x, x[2 * q]);
@@ -719,7 +740,7 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     // per pair: wave dot product, reduce-scatter, one LDS slot per (pair, wave, value); every
     // kDegridBatch pairs the four waves' partial sums meet and go to the per-sample accumulators
     const int pb = it.y, pe = it.z;
-    for_each_pair<0, PIPE>(descs, nullptr, cfr, pb, pe, w, lu, lv,
+    for_each_pair<false, PIPE>(descs, nullptr, cfr, pb, pe, w, lu, lv, 0,
                            [&](const Taps& c, const PairGeom& g, const PairU<0>& p, int pi) {
                              float x[8];
                              degrid_pair<NP>(x, c, gt, g);
@@ -810,17 +831,19 @@ void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, in
                        cf_pad_bytes(max_support), b.cursor.as<int>(), b.pairs.as<int4>());
   // work items: reuse cursor as the per-key item count and its scan
   int* icount = b.cursor.as<int>();
-  hipLaunchKernelGGL(k_item_count, dim3((nkeys + 1 + 255) / 256), dim3(256), 0, c.stream, counts, (int)nkeys, icount);
+  const int chunk = c.item_pairs > 0 ? c.item_pairs : kChunk;
+  hipLaunchKernelGGL(k_item_count, dim3((nkeys + 1 + 255) / 256), dim3(256), 0, c.stream, counts, (int)nkeys, chunk,
+                     icount);
   DevBuf& ioffsets = b.ioffsets;
   ioffsets.resize((nkeys + 1) * sizeof(int));
   size_t tb2 = 0;
   YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, icount, ioffsets.as<int>(), (int)(nkeys + 1), c.stream));
   b.temp.resize(tb2);
   YG_HIP(hipcub::DeviceScan::ExclusiveSum(b.temp.get(), tb2, icount, ioffsets.as<int>(), (int)(nkeys + 1), c.stream));
-  const size_t max_items = (size_t)npairs / kChunk + nkeys + 1;
+  const size_t max_items = (size_t)npairs / chunk + nkeys + 1;
   b.items.resize(max_items * sizeof(int4));
   hipLaunchKernelGGL(k_item_emit, dim3((nkeys + 255) / 256), dim3(256), 0, c.stream, counts, offsets,
-                     ioffsets.as<int>(), (int)nkeys, b.items.as<int4>());
+                     ioffsets.as<int>(), (int)nkeys, chunk, b.items.as<int4>());
   YG_HIP(hipMemcpyAsync(b.nitems.get(), ioffsets.as<int>() + nkeys, sizeof(int), hipMemcpyDeviceToDevice, c.stream));
   YG_HIP(hipGetLastError());
 }

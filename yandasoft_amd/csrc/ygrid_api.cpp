@@ -2,6 +2,7 @@
 // C++ exceptions and turns them into a status code plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -66,6 +67,7 @@ int ygrid_ctx_create(int device, ygrid_ctx** out) {
     };
     variant("YGRID_GRID_VARIANT", c->grid_variant);
     variant("YGRID_DEGRID_VARIANT", c->degrid_variant);
+    if (const char* s = std::getenv("YGRID_ITEM_PAIRS")) c->item_pairs = std::max(16, std::atoi(s));
     YG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
     *out = new ygrid_ctx{c};

@@ -109,10 +109,11 @@ constexpr int kCoef = 16;  // floats per sample in Samples::coef
 struct Samples {
   SampleHdr* hdr;
   int32_t* key;         // binning key: (ichan, z-group) plane index
-  // grid: per polarisation q the MAC coefficients (a, b, c, d) at [4q..4q+3] with
+  // grid: the MAC coefficients (a, b, c, d) of each polarisation q, with
   //   grid_re += a*C.re + b*C.im, grid_im += c*C.re + d*C.im
-  // (the value spread is v*C, or v*conj(C) in the PCF half plane); degrid: [0..1] = the
-  // phasor times the visibility weight.
+  // (the value spread is v*C, or v*conj(C) in the PCF half plane), laid out as the gridding
+  // kernel's MFMA operands: [4*o + m] with output o = 2*(q&1) + (0 re, 1 im) and operand
+  // m = 2*(q/2) + (0 for C.re, 1 for C.im); degrid: [0..1] = the phasor times the weight.
   float* coef;
 };
 
@@ -187,6 +188,7 @@ struct Ctx {
   int n_cu = 256;
   TileVariant grid_variant{3, 0};
   TileVariant degrid_variant{2, 1};
+  int item_pairs = 0;  // pairs per work item of the tile kernels (0: the built-in default)
   BinCache bins;  // shared by the gridders of the context (their passes are ordered on stream)
 };
 
