@@ -395,7 +395,16 @@ template <int NV>
 __device__ __forceinline__ PairU<NV> fetch_pair(const int4* __restrict__ descs, const float* __restrict__ coef,
                                                 int pi) {
   PairU<NV> p;
-  p.d = descs[pi];
+  // read through the constant address space: the descriptors are not written while a tile
+  // kernel runs, so the load is always a scalar one (never a vector load that a vmcnt wait
+  // would then have to cover together with the CF loads in flight)
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) int* ConstWords;
+  const ConstWords dw = (ConstWords)(uintptr_t)(descs + pi);
+  p.d = make_int4(dw[0], dw[1], dw[2], dw[3]);
+#else
+  p.d = descs[pi];  // host pass of this device-only function
+#endif
   const float* c = coef + (size_t)p.d.x * kCoef;
 #pragma unroll
   for (int q = 0; q < NV; ++q) p.v[q] = c[q];
@@ -576,7 +585,7 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int item = s_item;
+    const int item = __builtin_amdgcn_readfirstlane(s_item);  // uniform: item data via scalar loads
     __syncthreads();
     if (item >= nitems) break;
     const int4 it = items[item];
@@ -711,7 +720,7 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int item = s_item;
+    const int item = __builtin_amdgcn_readfirstlane(s_item);  // uniform: item data via scalar loads
     __syncthreads();
     if (item >= nitems) break;
     const int4 it = items[item];

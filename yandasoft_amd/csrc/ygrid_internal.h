@@ -186,7 +186,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int n_cu = 256;
-  TileVariant grid_variant{3, 0};
+  TileVariant grid_variant{2, 1};
   TileVariant degrid_variant{2, 1};
   int item_pairs = 0;  // pairs per work item of the tile kernels (0: the built-in default)
   BinCache bins;  // shared by the gridders of the context (their passes are ordered on stream)
