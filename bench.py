@@ -37,7 +37,7 @@ def profiled_traffic(kernel):
         with open(os.path.join(ROOT, PROFILE)) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel.replace(" ", ""):
                     return float(parts[2])
     except OSError:
         pass
@@ -184,11 +184,14 @@ def main():
     dk_ms = s1d.ms_degrid_kernel / max(1.0, s1d.n_degrid_launches)
     per_launch_g = taps_g / args.steps
     per_launch_d = taps_d / args.steps
-    wps = 3 if int(os.environ.get("YGRID_WAVES_PER_SIMD", "3")) >= 3 else 2
+    # kernel names as the profiler prints them: k_grid<pols, waves per SIMD, pipelined>
+    tf = lambda b: "true" if b else "false"
+    gname = f"k_grid<{wl.npol}, {s1g.grid_waves}, {tf(s1g.grid_pipeline)}>"
+    dname = f"k_degrid<{wl.npol}, {s1d.degrid_waves}, {tf(s1d.degrid_pipeline)}>"
     if gk_ms >= dk_ms:
-        kname, kms, kt = f"k_grid<4,{wps}>", gk_ms, per_launch_g
+        kname, kms, kt = gname, gk_ms, per_launch_g
     else:
-        kname, kms, kt = f"k_degrid<4,{wps}>", dk_ms, per_launch_d
+        kname, kms, kt = dname, dk_ms, per_launch_d
     achieved = 8.0 * kt / (kms * 1e-3) / 1e12
     result = {
         "metric": "CF-tap updates/s (W-proj grid+degrid)",

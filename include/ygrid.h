@@ -87,6 +87,8 @@ typedef struct {
   double ms_prep, ms_bin, ms_grid_kernel, ms_degrid_kernel;
   double n_grid_launches, n_degrid_launches;
   double n_bin_reused; /* passes that reused the context's cached sample binning */
+  /* tile-kernel variants of the context: waves per SIMD the kernel is built for, pipelined */
+  int grid_waves, grid_pipeline, degrid_waves, degrid_pipeline;
 } ygrid_stats;
 
 /* ---- library / context ---- */

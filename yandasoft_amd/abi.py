@@ -48,7 +48,9 @@ class Stats(ctypes.Structure):
                 ("support", ctypes.c_int), ("oversample", ctypes.c_int), ("n_cf", ctypes.c_int),
                 ("ms_prep", ctypes.c_double), ("ms_bin", ctypes.c_double), ("ms_grid_kernel", ctypes.c_double),
                 ("ms_degrid_kernel", ctypes.c_double), ("n_grid_launches", ctypes.c_double),
-                ("n_degrid_launches", ctypes.c_double), ("n_bin_reused", ctypes.c_double)]
+                ("n_degrid_launches", ctypes.c_double), ("n_bin_reused", ctypes.c_double),
+                ("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int),
+                ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int)]
 
 
 def library_path() -> str:

@@ -552,6 +552,10 @@ void Gridder::stats(ygrid_stats* o) const {
   o->n_grid_launches = n_gk_;
   o->n_degrid_launches = n_dk_;
   o->n_bin_reused = n_bin_reused_;
+  o->grid_waves = ctx_->grid_variant.waves;
+  o->grid_pipeline = ctx_->grid_variant.pipeline;
+  o->degrid_waves = ctx_->degrid_variant.waves;
+  o->degrid_pipeline = ctx_->degrid_variant.pipeline;
 }
 
 void Gridder::getCF(int k, float* out, int* support) const {
