@@ -384,7 +384,7 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   const float2* cft = cf_->table.as<float2>();
   const size_t cfb = cf_->table.size();
   const int smax = cf_->supp.empty() ? 0 : *std::max_element(cf_->supp.begin(), cf_->supp.end());
-  if (bin_samples(*ctx_, smp, nsamp, shape_[0], shape_[1], pp.nzg * shape_[3], smax)) ++n_bin_reused_;
+  if (bin_samples(*ctx_, smp, d.nrow, d.nchan, shape_[0], shape_[1], pp.nzg * shape_[3], smax)) ++n_bin_reused_;
   const Binning& bins = ctx_->bins.bin;
   if (forward) {
     acc_.resize((size_t)nsamp * 8 * sizeof(float));

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <climits>
 
 #include "ygrid_internal.h"
@@ -73,14 +74,30 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
 // --------------------------------------------------------------------------------------
 // prep
 // --------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepOutputs out) {
+// Blocks loop over the samples in rounds of 256.  With lds_sumw the sum of weights of the block
+// is first accumulated in shared memory (nsw_rows*npol*nchan_img doubles) and added to the
+// global sums once at the end; otherwise per distinct row within each wave.
+__global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepOutputs out, int lds_sumw) {
 #pragma clang fp contract(off)
+  extern __shared__ double s_sumw[];
   const int n = p.nrow * p.nchan;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;  // s = row + nrow*chan
+  const bool grid_mode = p.mode != kModeDegrid;
+  const bool sumw_on = grid_mode && p.mode != kModePCF;
+  const bool use_lds = sumw_on && lds_sumw != 0;
+  const int nsw = p.nsw_rows * p.npol * p.nchan_img;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  if (use_lds) {
+    for (int e = tid; e < nsw; e += blockDim.x) s_sumw[e] = 0.0;
+    __syncthreads();
+  }
+  unsigned long long c_valid = 0, c_taps = 0, c_flag = 0, c_wflag = 0;
+  int err = 0;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += (long long)gridDim.x * blockDim.x) {
+  const int s = (int)base + tid;  // s = row + nrow*chan
   const bool inb = s < n;
   const int row = inb ? s % p.nrow : 0;
   const int chan = inb ? s / p.nrow : 0;
-  const bool grid_mode = p.mode != kModeDegrid;
 
   SampleHdr rec;
   rec.iu0 = INT_MIN;
@@ -95,7 +112,6 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
 
   bool valid = inb;
   bool flagged = false, wflagged = false;
-  int err = 0;
   int swrow = 0, ichan = 0;
   float wts[4] = {0.f, 0.f, 0.f, 0.f};
   unsigned long long taps = 0;
@@ -222,22 +238,18 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
     for (int q = 0; q < kCoef / 4; ++q) c4[q] = make_float4(coef[4 * q], coef[4 * q + 1], coef[4 * q + 2], coef[4 * q + 3]);
   }
 
-  // counters (wave aggregated)
-  const unsigned long long nvalid = __popcll(__ballot(valid));
-  const unsigned long long nflag = __popcll(__ballot(flagged && grid_mode));
-  const unsigned long long nwflag = __popcll(__ballot(wflagged));
-  const unsigned long long tsum = wave_sum_u64(taps);
-  const int lane = threadIdx.x & 63;
-  if (lane == 0) {
-    if (nvalid) atomicAdd(&out.counters[0], nvalid * (unsigned long long)p.npol);
-    if (tsum) atomicAdd(&out.counters[1], tsum);
-    if (nflag) atomicAdd(&out.counters[2], nflag);
-    if (nwflag) atomicAdd(&out.counters[3], nwflag * (grid_mode ? 2ull : 1ull));
-  }
-  if (err) atomicOr(out.error, err);
+  c_valid += valid ? 1 : 0;
+  c_taps += taps;
+  c_flag += (flagged && grid_mode) ? 1 : 0;
+  c_wflag += wflagged ? 1 : 0;
 
-  // sum of weights, aggregated per distinct (row of sumw, image chan) within the wave
-  if (grid_mode && p.mode != kModePCF) {
+  // sum of weights per (row of sumw, pol, image chan): TableVisGridder.cc:776-782
+  if (use_lds) {
+    if (valid)
+      for (int pol = 0; pol < p.npol; ++pol)
+        atomicAdd(&s_sumw[swrow + p.nsw_rows * (pol + p.npol * ichan)], (double)wts[pol]);
+  } else if (sumw_on) {
+    // aggregated per distinct (row of sumw, image chan) within the wave
     long long key = valid ? (long long)swrow + (long long)p.nsw_rows * (long long)p.npol * ichan : -1;
     bool done = !valid;
     while (true) {
@@ -253,6 +265,25 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
       if (match) done = true;
     }
   }
+  }  // rounds
+
+  // counters (wave aggregated)
+  const unsigned long long nvalid = wave_sum_u64(c_valid);
+  const unsigned long long nflag = wave_sum_u64(c_flag);
+  const unsigned long long nwflag = wave_sum_u64(c_wflag);
+  const unsigned long long tsum = wave_sum_u64(c_taps);
+  if (lane == 0) {
+    if (nvalid) atomicAdd(&out.counters[0], nvalid * (unsigned long long)p.npol);
+    if (tsum) atomicAdd(&out.counters[1], tsum);
+    if (nflag) atomicAdd(&out.counters[2], nflag);
+    if (nwflag) atomicAdd(&out.counters[3], nwflag * (grid_mode ? 2ull : 1ull));
+  }
+  if (err) atomicOr(out.error, err);
+  if (use_lds) {
+    __syncthreads();
+    for (int e = tid; e < nsw; e += blockDim.x)
+      if (s_sumw[e] != 0.0) atomicAdd(&out.sumw[e], s_sumw[e]);
+  }
 }
 
 // --------------------------------------------------------------------------------------
@@ -266,15 +297,79 @@ __device__ __forceinline__ TileRange tiles_of(const SampleHdr& r) {
   return {r.iu0 / kTile, (r.iu0 + W - 1) / kTile, r.iv0 / kTile, (r.iv0 + W - 1) / kTile};
 }
 
-__global__ void k_bin_count(Samples smp, int n, int ntu, int ntiles, int* counts) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const SampleHdr r = smp.hdr[s];
-  if (r.iu0 == INT_MIN) return;
-  const TileRange t = tiles_of(r);
-  const long long base = (long long)smp.key[s] * ntiles;
-  for (int tv = t.tv0; tv <= t.tv1; ++tv)
-    for (int tu = t.tu0; tu <= t.tu1; ++tu) atomicAdd(&counts[base + tv * ntu + tu], 1);
+// The binning kernels take the samples channel-fastest (thread t: row t / nchan, channel
+// t % nchan), so that neighbouring lanes are neighbouring channels of one row: their footprints
+// move by a few pixels per channel and mostly fall in the same (plane, tile) bins.  Lanes with
+// equal bins in consecutive lanes ("runs") then share one atomic.
+struct BinLane {
+  int s;        // sample index (row + nrow*chan), -1 when the lane has none
+  SampleHdr r;
+  TileRange t;
+  long long base;  // key * ntiles
+};
+
+__device__ __forceinline__ BinLane bin_lane(const Samples& smp, long long tglob, int nrow, int nchan, int ntiles) {
+  BinLane b;
+  b.s = -1;
+  b.r.iu0 = INT_MIN;
+  b.t = TileRange{0, -1, 0, -1};
+  b.base = 0;
+  const long long n = (long long)nrow * nchan;
+  if (tglob < n) {
+    const int row = (int)(tglob / nchan), chan = (int)(tglob % nchan);
+    const int s = row + nrow * chan;
+    const SampleHdr r = smp.hdr[s];
+    if (r.iu0 != INT_MIN) {
+      b.s = s;
+      b.r = r;
+      b.t = tiles_of(r);
+      b.base = (long long)smp.key[s] * ntiles;
+    }
+  }
+  return b;
+}
+
+// Runs of equal idx among the active lanes of a wave: head = lowest lane of the run, rank =
+// lane - head, len = run length (meaningful at the head).
+struct Run {
+  bool head;
+  int hlane, rank, len;
+};
+__device__ __forceinline__ Run wave_run(long long idx, bool active, int lane) {
+  const long long prev = __shfl_up(idx, 1, 64);
+  const int pa = __shfl_up(active ? 1 : 0, 1, 64);
+  Run r;
+  r.head = active && (lane == 0 || !pa || prev != idx);
+  const unsigned long long heads = __ballot(r.head);
+  const unsigned long long act = __ballot(active);
+  const unsigned long long upto = (2ull << lane) - 1;  // lanes 0..lane
+  const unsigned long long hb = heads & upto;
+  r.hlane = hb ? 63 - __clzll(hb) : lane;
+  r.rank = lane - r.hlane;
+  const unsigned long long after = (heads | ~act) & ~upto;
+  r.len = (after ? __ffsll((long long)after) - 1 : 64) - lane;
+  return r;
+}
+
+// widest footprint in tiles over the wave (loop bounds shared by all lanes)
+__device__ __forceinline__ int wave_max_int(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__global__ void k_bin_count(Samples smp, int nrow, int nchan, int ntu, int ntiles, int* counts) {
+  const long long tglob = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const BinLane b = bin_lane(smp, tglob, nrow, nchan, ntiles);
+  const int nv = wave_max_int(b.t.tv1 - b.t.tv0 + 1), nu = wave_max_int(b.t.tu1 - b.t.tu0 + 1);
+  for (int dv = 0; dv < nv; ++dv)
+    for (int du = 0; du < nu; ++du) {
+      const int tv = b.t.tv0 + dv, tu = b.t.tu0 + du;
+      const bool active = b.s >= 0 && tv <= b.t.tv1 && tu <= b.t.tu1;
+      const long long idx = active ? b.base + tv * ntu + tu : -1;
+      const Run run = wave_run(idx, active, lane);
+      if (run.head) atomicAdd(&counts[idx], run.len);
+    }
 }
 
 // bits lo..hi (clamped to 0..n-1) of an n-bit set
@@ -316,17 +411,21 @@ __device__ __forceinline__ int4 pair_descriptor(const SampleHdr& r, int s, int u
   return make_int4(s, (int)soff, (int)masks, (int)pw);
 }
 
-__global__ void k_bin_fill(Samples smp, int n, int ntu, int ntiles, uint32_t pad, int* cursor, int4* descs) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const SampleHdr r = smp.hdr[s];
-  if (r.iu0 == INT_MIN) return;
-  const TileRange t = tiles_of(r);
-  const long long base = (long long)smp.key[s] * ntiles;
-  for (int tv = t.tv0; tv <= t.tv1; ++tv)
-    for (int tu = t.tu0; tu <= t.tu1; ++tu) {
-      const int pos = atomicAdd(&cursor[base + tv * ntu + tu], 1);
-      descs[pos] = pair_descriptor(r, s, tu * kTile, tv * kTile, pad);
+__global__ void k_bin_fill(Samples smp, int nrow, int nchan, int ntu, int ntiles, uint32_t pad, int* cursor,
+                           int4* descs) {
+  const long long tglob = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const BinLane b = bin_lane(smp, tglob, nrow, nchan, ntiles);
+  const int nv = wave_max_int(b.t.tv1 - b.t.tv0 + 1), nu = wave_max_int(b.t.tu1 - b.t.tu0 + 1);
+  for (int dv = 0; dv < nv; ++dv)
+    for (int du = 0; du < nu; ++du) {
+      const int tv = b.t.tv0 + dv, tu = b.t.tu0 + du;
+      const bool active = b.s >= 0 && tv <= b.t.tv1 && tu <= b.t.tu1;
+      const long long idx = active ? b.base + tv * ntu + tu : -1;
+      const Run run = wave_run(idx, active, lane);
+      const int hpos = run.head ? atomicAdd(&cursor[idx], run.len) : 0;
+      const int pos = __shfl(hpos, run.hlane, 64) + run.rank;
+      if (active) descs[pos] = pair_descriptor(b.r, b.s, tu * kTile, tv * kTile, pad);
     }
 }
 
@@ -798,7 +897,17 @@ __global__ void k_degrid_finish(Samples smp, const float* __restrict__ acc, int 
 void launch_prep(const Ctx& c, const PrepParams& p, const PrepInputs& in, const PrepOutputs& out) {
   const int n = p.nrow * p.nchan;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, c.stream, p, in, out);
+  const int blocks = (n + 255) / 256;
+  // sums of weights in shared memory when they fit in 64 KB; then a few blocks per CU loop over
+  // the samples, so that each block adds its sums to the global ones once
+  const long long nsw = (long long)p.nsw_rows * p.npol * p.nchan_img;
+  const bool lds = p.mode != kModeDegrid && p.mode != kModePCF && nsw > 0 && nsw <= 8192;
+  if (lds) {
+    const int g = std::min(blocks, 4 * c.n_cu);
+    hipLaunchKernelGGL(k_prep, dim3(g), dim3(256), (size_t)nsw * sizeof(double), c.stream, p, in, out, 1);
+  } else {
+    hipLaunchKernelGGL(k_prep, dim3(blocks), dim3(256), 0, c.stream, p, in, out, 0);
+  }
   YG_HIP(hipGetLastError());
 }
 
@@ -809,7 +918,9 @@ static uint32_t cf_pad_bytes(int max_support) {
   return (uint32_t)((16 * (2 * (size_t)max_support + 1 + 2 * kGuardU) + 64) * sizeof(float2));
 }
 
-void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, int nkz, int max_support, Binning& b) {
+void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int nx, int ny, int nkz, int max_support,
+                    Binning& b) {
+  const int n = nrow * nchan;
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const size_t ntiles = (size_t)ntu * ntv;
   const size_t nkeys = ntiles * (size_t)nkz;
@@ -823,7 +934,8 @@ void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, in
   int* counts = b.counts.as<int>();
   int* offsets = b.offsets.as<int>();
   YG_HIP(hipMemsetAsync(counts, 0, (nkeys + 1) * sizeof(int), c.stream));
-  if (n > 0) hipLaunchKernelGGL(k_bin_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, n, ntu, (int)ntiles, counts);
+  if (n > 0) hipLaunchKernelGGL(k_bin_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, nrow, nchan, ntu,
+                                (int)ntiles, counts);
   size_t tbytes = 0;
   YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, counts, offsets, (int)(nkeys + 1), c.stream));
   b.temp.resize(tbytes);
@@ -836,7 +948,7 @@ void launch_binning(const Ctx& c, const Samples& recs, int n, int nx, int ny, in
   b.pairs.resize((size_t)(npairs > 0 ? npairs : 1) * sizeof(int4));
   YG_HIP(hipMemcpyAsync(b.cursor.get(), offsets, (nkeys + 1) * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
   if (n > 0)
-    hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, n, ntu, (int)ntiles,
+    hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, nrow, nchan, ntu, (int)ntiles,
                        cf_pad_bytes(max_support), b.cursor.as<int>(), b.pairs.as<int4>());
   // work items: reuse cursor as the per-key item count and its scan
   int* icount = b.cursor.as<int>();
@@ -871,9 +983,10 @@ __global__ void k_samples_differ(const SampleHdr* __restrict__ ha, const int32_t
 }
 }  // namespace
 
-bool bin_samples(Ctx& c, const Samples& smp, int n, int nx, int ny, int nkz, int max_support) {
+bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkz, int max_support) {
   BinCache& bc = c.bins;
-  if (bc.valid && bc.nsamp == n && bc.nx == nx && bc.ny == ny && bc.nkz == nkz && bc.smax == max_support) {
+  const int n = nrow * nchan;
+  if (bc.valid && bc.nrow == nrow && bc.nchan == nchan && bc.nx == nx && bc.ny == ny && bc.nkz == nkz && bc.smax == max_support) {
     bc.flag.resize(sizeof(int));
     YG_HIP(hipMemsetAsync(bc.flag.get(), 0, sizeof(int), c.stream));
     if (n > 0)
@@ -885,14 +998,15 @@ bool bin_samples(Ctx& c, const Samples& smp, int n, int nx, int ny, int nkz, int
     if (!differ) return true;
   }
   bc.valid = false;
-  launch_binning(c, smp, n, nx, ny, nkz, max_support, bc.bin);
+  launch_binning(c, smp, nrow, nchan, nx, ny, nkz, max_support, bc.bin);
   bc.hdr.resize((size_t)(n > 0 ? n : 1) * sizeof(SampleHdr));
   bc.key.resize((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
   if (n > 0) {
     YG_HIP(hipMemcpyAsync(bc.hdr.get(), smp.hdr, (size_t)n * sizeof(SampleHdr), hipMemcpyDeviceToDevice, c.stream));
     YG_HIP(hipMemcpyAsync(bc.key.get(), smp.key, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToDevice, c.stream));
   }
-  bc.nsamp = n;
+  bc.nrow = nrow;
+  bc.nchan = nchan;
   bc.nx = nx;
   bc.ny = ny;
   bc.nkz = nkz;

@@ -176,7 +176,7 @@ struct TileVariant {
 // degridding and gridding of one chunk in a major cycle) reuses it.
 struct BinCache {
   DevBuf hdr, key, flag;
-  int nsamp = -1, nx = 0, ny = 0, nkz = 0, smax = 0;
+  int nrow = -1, nchan = -1, nx = 0, ny = 0, nkz = 0, smax = 0;
   bool valid = false;
   Binning bin;
 };
@@ -194,13 +194,14 @@ struct Ctx {
 
 // Kernel launchers (kernels_grid.hip)
 void launch_prep(const Ctx&, const PrepParams&, const PrepInputs&, const PrepOutputs&);
-// Bins valid records by (key, tile); returns nothing, work lands in b.
+// Bins valid records by (key, tile); returns nothing, work lands in b.  The samples are the
+// nrow x nchan (row, channel) samples of one chunk, s = row + nrow*chan.
 // max_support: largest support of the CF table the pairs will be worked with
-void launch_binning(const Ctx&, const Samples& smp, int nsamples, int nx, int ny, int nkeys_z, int max_support,
+void launch_binning(const Ctx&, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkeys_z, int max_support,
                     Binning& b);
 // Bins the samples into ctx.bins unless the cached binning was built from identical sample
 // headers and keys with the same parameters; returns true when the cache was reused.
-bool bin_samples(Ctx& c, const Samples& smp, int nsamples, int nx, int ny, int nkeys_z, int max_support);
+bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkeys_z, int max_support);
 // cf_bytes: size of the CF table (the kernels address it with 32-bit byte offsets);
 // max_support: largest support of its planes
 void launch_grid(const Ctx&, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes, int max_support,
