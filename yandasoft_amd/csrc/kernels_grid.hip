@@ -482,31 +482,35 @@ constexpr int kWaitVmcnt0 = 0x0F70;
 // (DATA_FORMAT = 32); bounds are checked against num_records in bytes.
 constexpr int kBufferWord3 = 0x00020000;
 
-// Wave-uniform data of one (sample, tile) pair: its descriptor (pair_descriptor) and NV
-// coefficient floats of the sample.
-template <int NV>
+// Staging of a work item: before its pairs are worked, the workgroup copies the item's pair
+// descriptors (and, for gridding, the 16 MAC coefficients of each pair's sample) into LDS with
+// all its threads at once, so that the latency of those scattered reads is paid once per item
+// instead of once per pair on each wave's critical path.  A wave then reads a pair's descriptor
+// from LDS (every lane the same 16 bytes) and makes it wave-uniform.
+template <bool COEF>
+__device__ __forceinline__ void stage_item(const int4* __restrict__ descs, const float* __restrict__ coef, int pb,
+                                           int n, int4* s_desc, f32x4* s_coef, int tid, int nthreads) {
+  for (int t = tid; t < n; t += nthreads) {
+    const int4 d = descs[pb + t];
+    s_desc[t] = d;
+    if constexpr (COEF) {
+      const f32x4* src = reinterpret_cast<const f32x4*>(coef + (size_t)d.x * kCoef);
+#pragma unroll
+      for (int q = 0; q < kCoef / 4; ++q) s_coef[(kCoef / 4) * t + q] = src[q];
+    }
+  }
+  __syncthreads();
+}
+
 struct PairU {
   int4 d;
-  float v[NV > 0 ? NV : 1];
 };
 
-template <int NV>
-__device__ __forceinline__ PairU<NV> fetch_pair(const int4* __restrict__ descs, const float* __restrict__ coef,
-                                                int pi) {
-  PairU<NV> p;
-  // read through the constant address space: the descriptors are not written while a tile
-  // kernel runs, so the load is always a scalar one (never a vector load that a vmcnt wait
-  // would then have to cover together with the CF loads in flight)
-#if defined(__HIP_DEVICE_COMPILE__)
-  typedef const __attribute__((address_space(4))) int* ConstWords;
-  const ConstWords dw = (ConstWords)(uintptr_t)(descs + pi);
-  p.d = make_int4(dw[0], dw[1], dw[2], dw[3]);
-#else
-  p.d = descs[pi];  // host pass of this device-only function
-#endif
-  const float* c = coef + (size_t)p.d.x * kCoef;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) p.v[q] = c[q];
+__device__ __forceinline__ PairU staged_pair(const int4* s_desc, int i) {
+  const int4 d = s_desc[i];
+  PairU p;
+  p.d = make_int4(__builtin_amdgcn_readfirstlane(d.x), __builtin_amdgcn_readfirstlane(d.y),
+                  __builtin_amdgcn_readfirstlane(d.z), __builtin_amdgcn_readfirstlane(d.w));
   return p;
 }
 
@@ -520,8 +524,7 @@ struct PairGeom {
   uint32_t mask, soff0, rowstep;
 };
 
-template <int NV>
-__device__ __forceinline__ PairGeom pair_geom(const PairU<NV>& p, int w) {
+__device__ __forceinline__ PairGeom pair_geom(const PairU& p, int w) {
   PairGeom g;
   const uint32_t pw = (uint32_t)p.d.w;
   g.P = (int)(pw & 0xffffu);
@@ -549,11 +552,11 @@ __device__ __forceinline__ void lane_offsets(uint32_t (&vo)[2], const PairGeom& 
 
 // The CF taps of the active blocks: one 16-byte buffer load per lane and block, with the row in
 // the scalar offset and the lane's position in the vector offset.  COEF: also the sample's MFMA
-// B operands (16 bytes at coef + 16*s + boff, boff = the lane's output column).
+// B operands (the 16 bytes of the staged coefficients for the lane's output column o).
 template <bool COEF>
 __device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, const PairGeom& g, int lu, int lv,
-                                        const float* __restrict__ coef, int s, int boff) {
-  if constexpr (COEF) c.b = *reinterpret_cast<const f32x4*>(coef + (size_t)s * kCoef + boff);
+                                        const f32x4* s_coef_pair, int o) {
+  if constexpr (COEF) c.b = s_coef_pair[o];
   uint32_t vo[2];
   lane_offsets(vo, g, lu, lv);
 #pragma unroll
@@ -610,55 +613,55 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
                                            (int)(t.bytes + t.pad), kBufferWord3);
 }
 
-// Runs consume(c, geom, pair, pi) for the pairs [pb, pe) of a work item in wave w, with the CF
-// taps (and, COEF, the MFMA B operands) of the pair in c.  The descriptor of the next pair is
-// fetched before the current pair is worked.  PIPE: two register sets; the loads of pair i+1
-// are issued before pair i is consumed.  consume must not issue vector memory loads whose
-// results it waits for after returning (the wait before the next loads covers them).
+// Runs consume(c, geom, pair, i) for the n staged pairs of a work item in wave w, with the CF
+// taps (and, COEF, the MFMA B operands) of pair i in c.  The descriptor of the next pair is read
+// before the current pair is worked.  PIPE: two register sets; the loads of pair i+1 are issued
+// before pair i is consumed.  consume must not issue vector memory loads whose results it waits
+// for after returning (the wait before the next loads covers them).
 template <bool COEF, bool PIPE, class F>
-__device__ __forceinline__ void for_each_pair(const int4* __restrict__ descs, const float* __restrict__ coef,
-                                              __amdgpu_buffer_rsrc_t cfr, int pb, int pe, int w, int lu, int lv,
-                                              int boff, F&& consume) {
-  if (pb >= pe) return;
+__device__ __forceinline__ void for_each_pair(const int4* s_desc, const f32x4* s_coef, __amdgpu_buffer_rsrc_t cfr,
+                                              int n, int w, int lu, int lv, int o, F&& consume) {
+  if (n <= 0) return;
+  constexpr int CW = kCoef / 4;  // f32x4 per staged pair
   if constexpr (!PIPE) {
-    PairU<0> pn = fetch_pair<0>(descs, coef, pb);
-    for (int pi = pb; pi < pe; ++pi) {
+    PairU pn = staged_pair(s_desc, 0);
+    for (int i = 0; i < n; ++i) {
       // Nothing is in flight here (the previous pair consumed its taps); saying so keeps the
       // wait-count analysis, which cannot correlate the load and use conditions of a block,
       // from waiting for all loads before each load that reuses a register.
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      const PairU<0> p = pn;
-      if (pi + 1 < pe) pn = fetch_pair<0>(descs, coef, pi + 1);
+      const PairU p = pn;
+      if (i + 1 < n) pn = staged_pair(s_desc, i + 1);
       const PairGeom g = pair_geom(p, w);
       Taps c;
-      load_cf<COEF>(c, cfr, g, lu, lv, coef, p.d.x, boff);
-      consume(c, g, p, pi);
+      load_cf<COEF>(c, cfr, g, lu, lv, COEF ? s_coef + CW * i : s_coef, o);
+      consume(c, g, p, i);
     }
   } else {
-    int pi = pb;
+    int i = 0;
     Taps cA, cB;
-    PairU<0> pA = fetch_pair<0>(descs, coef, pi);
+    PairU pA = staged_pair(s_desc, 0);
     PairGeom gA = pair_geom(pA, w);
-    load_cf<COEF>(cA, cfr, gA, lu, lv, coef, pA.d.x, boff);
-    PairU<0> pB = pA;
-    if (pi + 1 < pe) pB = fetch_pair<0>(descs, coef, pi + 1);
+    load_cf<COEF>(cA, cfr, gA, lu, lv, s_coef, o);
+    PairU pB = pA;
+    if (n > 1) pB = staged_pair(s_desc, 1);
     for (;;) {
-      // pair pi: taps in flight into cA; pB = record of pair pi+1
+      // pair i: taps in flight into cA; pB = record of pair i+1
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       const PairGeom gB = pair_geom(pB, w);
-      if (pi + 1 < pe) load_cf<COEF>(cB, cfr, gB, lu, lv, coef, pB.d.x, boff);
-      PairU<0> pN = pB;
-      if (pi + 2 < pe) pN = fetch_pair<0>(descs, coef, pi + 2);
-      consume(cA, gA, pA, pi);
-      if (++pi >= pe) break;
-      // pair pi: taps in flight into cB; pN = record of pair pi+1
+      if (i + 1 < n) load_cf<COEF>(cB, cfr, gB, lu, lv, COEF ? s_coef + CW * (i + 1) : s_coef, o);
+      PairU pN = pB;
+      if (i + 2 < n) pN = staged_pair(s_desc, i + 2);
+      consume(cA, gA, pA, i);
+      if (++i >= n) break;
+      // pair i: taps in flight into cB; pN = record of pair i+1
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       gA = pair_geom(pN, w);
-      if (pi + 1 < pe) load_cf<COEF>(cA, cfr, gA, lu, lv, coef, pN.d.x, boff);
-      PairU<0> pM = pN;
-      if (pi + 2 < pe) pM = fetch_pair<0>(descs, coef, pi + 2);
-      consume(cB, gB, pB, pi);
-      if (++pi >= pe) break;
+      if (i + 1 < n) load_cf<COEF>(cA, cfr, gA, lu, lv, COEF ? s_coef + CW * (i + 1) : s_coef, o);
+      PairU pM = pN;
+      if (i + 2 < n) pM = staged_pair(s_desc, i + 2);
+      consume(cB, gB, pB, i);
+      if (++i >= n) break;
       pA = pN;
       pB = pM;
     }
@@ -671,6 +674,8 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
                                               const float* __restrict__ coef, CFRef cft, float* __restrict__ grid,
                                               int nx, int ny, int ntu, int ntiles, int nzg) {
   __shared__ int s_item;
+  __shared__ int4 s_desc[kChunk];
+  __shared__ f32x4 s_coef[kChunk * (kCoef / 4)];
   const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(cft);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -703,10 +708,10 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
 #pragma unroll
           for (int m = 0; m < NS; ++m) acc[k][j][e][m] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for_each_pair<true, PIPE>(descs, coef, cfr, it.y, it.z, w, lu, lv, 4 * o,
-                              [&](const Taps& c, const PairGeom& g, const PairU<0>&, int) {
-                                grid_pair<NP>(acc, c, g);
-                              });
+    const int n = it.z - it.y;
+    stage_item<true>(descs, coef, it.y, n, s_desc, s_coef, tid, blockDim.x);
+    for_each_pair<true, PIPE>(s_desc, s_coef, cfr, n, w, lu, lv, o,
+                              [&](const Taps& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
     // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of rows 4k+i, column 32j+2b+e
     const size_t plane = (size_t)nx * ny;
 #pragma unroll
@@ -807,6 +812,7 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
                                                 CFRef cft, const float2* __restrict__ grid, int nx, int ny, int ntu,
                                                 int ntiles, int nzg, float* __restrict__ acc_out) {
   __shared__ int s_item;
+  __shared__ int4 s_desc[kChunk];
   const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(cft);
   __shared__ float s_part[kDegridBatch][4][8];
   __shared__ int s_sid[kDegridBatch];
@@ -847,16 +853,17 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
 
     // per pair: wave dot product, reduce-scatter, one LDS slot per (pair, wave, value); every
     // kDegridBatch pairs the four waves' partial sums meet and go to the per-sample accumulators
-    const int pb = it.y, pe = it.z;
-    for_each_pair<false, PIPE>(descs, nullptr, cfr, pb, pe, w, lu, lv, 0,
-                           [&](const Taps& c, const PairGeom& g, const PairU<0>& p, int pi) {
+    const int n = it.z - it.y;
+    stage_item<false>(descs, nullptr, it.y, n, s_desc, nullptr, tid, blockDim.x);
+    for_each_pair<false, PIPE>(s_desc, nullptr, cfr, n, w, lu, lv, 0,
+                           [&](const Taps& c, const PairGeom& g, const PairU& p, int i) {
                              float x[8];
                              degrid_pair<NP>(x, c, gt, g);
                              const float r = wave_reduce_scatter8(x, lane);
-                             const int b = (pi - pb) % kDegridBatch;
+                             const int b = i % kDegridBatch;
                              if (lane < 8) s_part[b][w][slot] = r;
                              if (tid == 0) s_sid[b] = p.d.x;
-                             if (b == kDegridBatch - 1 || pi == pe - 1) {
+                             if (b == kDegridBatch - 1 || i == n - 1) {
                                __syncthreads();
                                const int nb = b + 1;
                                for (int t = tid; t < nb * 2 * NP; t += blockDim.x) {
@@ -952,7 +959,8 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
                        cf_pad_bytes(max_support), b.cursor.as<int>(), b.pairs.as<int4>());
   // work items: reuse cursor as the per-key item count and its scan
   int* icount = b.cursor.as<int>();
-  const int chunk = c.item_pairs > 0 ? c.item_pairs : kChunk;
+  // the tile kernels stage a whole item in LDS: at most kChunk pairs
+  const int chunk = c.item_pairs > 0 ? std::min(c.item_pairs, kChunk) : kChunk;
   hipLaunchKernelGGL(k_item_count, dim3((nkeys + 1 + 255) / 256), dim3(256), 0, c.stream, counts, (int)nkeys, chunk,
                      icount);
   DevBuf& ioffsets = b.ioffsets;
