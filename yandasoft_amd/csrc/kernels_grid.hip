@@ -27,6 +27,8 @@ namespace ygrid {
 namespace {
 
 constexpr int kChunk = 512;  // max pairs per work item
+// floats of one wave's flush area in k_grid: 4 rows x 64 columns x 4 outputs, one pad per 32
+constexpr int kFlushWave = 4 * 64 * 4 + 4 * 64 * 4 / 32;
 
 __device__ __forceinline__ int nint_d(double x) { return (x > 0.0) ? (int)(x + 0.5) : (int)(x - 0.5); }
 
@@ -578,9 +580,9 @@ __device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, con
 // acc += v * C (or v * conj(C) in the PCF half plane, TableVisGridder.cc:837-844) for the
 // active blocks, on the matrix cores: per block, tap e and output set m (polarisations 2m, 2m+1)
 // two 4x4x1 MFMAs (16 independent 4x4 outer products per wave) add C.re (x) B[2m] and
-// C.im (x) B[2m+1].  MFMA block b = lane/4 holds the taps of its lanes 4b..4b+3 (4 rows) as A and
-// the sample's coefficients for outputs (pol 2m re, pol 2m im, pol 2m+1 re, pol 2m+1 im) as B;
-// the result for tap row i and output o lands in lane 4b+o, register i.
+// C.im (x) B[2m+1].  MFMA block b = lane/4 holds the taps of its lanes 4b..4b+3 (4 pixels) as A
+// and the sample's coefficients for outputs (pol 2m re, pol 2m im, pol 2m+1 re, pol 2m+1 im) as B;
+// the result for the pixel of lane 4b+i and output o lands in lane 4b+o, register i.
 template <int NP>
 __device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], const Taps& c, const PairGeom& g) {
 #pragma unroll
@@ -676,14 +678,19 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
   __shared__ int s_item;
   __shared__ int4 s_desc[kChunk];
   __shared__ f32x4 s_coef[kChunk * (kCoef / 4)];
+  static_assert(4 * kFlushWave <= kChunk * kCoef, "flush areas must fit in the staging space");
   const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(cft);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // MFMA layout: lane 4b+i loads the taps of row i, columns 2b, 2b+1 of a block (A operands) and
-  // holds, after the products, output o = lane & 3 of rows 0..3 at columns 2b+e
-  const int lu = lane >> 2, lv = lane & 3;
+  // Lane (lu, lv) = (lane & 15, lane >> 4) loads the taps of block row lv, columns 2lu, 2lu+1 (so
+  // that the four lanes of a quad read 64 contiguous bytes: one L1 tag lookup).  The MFMA block
+  // b = lane/4 then holds the four column pairs 4(b&3)..4(b&3)+3 of row b>>2; after the products
+  // lane 4b+o holds output o = lane & 3 of those four column pairs (register i: column pair
+  // 4(b&3)+i).
+  const int lu = lane & 15, lv = lane >> 4;
   const int o = lane & 3;
+  const int ob = (lane >> 2) & 3;
   constexpr int NS = (NP + 1) / 2;  // output sets of two polarisations
   const int nitems = *nitems_p;
   for (;;) {
@@ -712,27 +719,43 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     stage_item<true>(descs, coef, it.y, n, s_desc, s_coef, tid, blockDim.x);
     for_each_pair<true, PIPE>(s_desc, s_coef, cfr, n, w, lu, lv, o,
                               [&](const Taps& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
-    // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of rows 4k+i, column 32j+2b+e
+    // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of row 4k+lv, column
+    // 32j+8(b&3)+2i+e.  Per (m, k) the wave writes its 4 rows x 64 columns x 4 outputs to its own
+    // LDS area (the coefficient staging space, free until the next item is staged) and reads them
+    // back transposed, so that each float atomic instruction covers one row's 16 consecutive
+    // pixels of two polarisations (two cache lines).  One float of padding per 32 spreads banks.
+    // The barrier keeps a wave from overwriting coefficients the other waves still read.
+    __syncthreads();
+    float* s_fl = reinterpret_cast<float*>(s_coef) + w * kFlushWave;
     const size_t plane = (size_t)nx * ny;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
-      const int q = 2 * m + (o >> 1);
-      if (q >= NP) continue;
-      float* gq = grid + 2 * (size_t)(ichan * NP + q) * plane + (o & 1);
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < 4; ++k) {
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-          for (int e = 0; e < 2; ++e) {
-            const int u = u0 + 32 * j + 2 * lu + e;
+          for (int e = 0; e < 2; ++e)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              const int v = v0 + 16 * w + 4 * k + i;
-              const float val = acc[k][j][e][m][i];
-              if (val != 0.f && u < nx && v < ny) atomicAdd(gq + 2 * ((size_t)v * nx + u), val);
+              const int idx = ((lv * kTile) + 32 * j + 8 * ob + 2 * i + e) * 4 + o;
+              s_fl[idx + (idx >> 5)] = acc[k][j][e][m][i];
             }
+        __builtin_amdgcn_wave_barrier();
+        const int q = 2 * m + (o >> 1);
+        if (q < NP) {
+          float* gq = grid + 2 * (size_t)(ichan * NP + q) * plane + (o & 1);
+#pragma unroll
+          for (int t = 0; t < 16; ++t) {
+            const int idx = t * 64 + lane;
+            const float val = s_fl[idx + (idx >> 5)];
+            const int v = v0 + 16 * w + 4 * k + (t >> 2);
+            const int u = u0 + 16 * (t & 3) + (lane >> 2);
+            if (val != 0.f && u < nx && v < ny) atomicAdd(gq + 2 * ((size_t)v * nx + u), val);
           }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
     }
   }
 }
