@@ -89,6 +89,9 @@ typedef struct {
   double n_bin_reused; /* passes that reused the context's cached sample binning */
   /* tile-kernel variants of the context: waves per SIMD the kernel is built for, pipelined */
   int grid_waves, grid_pipeline, degrid_waves, degrid_pipeline;
+  /* storage precision of the CF taps and of the gridding coefficients (parset keys cfprecision,
+   * visprecision): 0 FP32, 1 FP16, 2 BF16; accumulation is FP32 in every case */
+  int cf_precision, vis_precision;
 } ygrid_stats;
 
 /* ---- library / context ---- */

@@ -50,7 +50,8 @@ class Stats(ctypes.Structure):
                 ("ms_degrid_kernel", ctypes.c_double), ("n_grid_launches", ctypes.c_double),
                 ("n_degrid_launches", ctypes.c_double), ("n_bin_reused", ctypes.c_double),
                 ("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int),
-                ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int)]
+                ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int),
+                ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int)]
 
 
 def library_path() -> str:

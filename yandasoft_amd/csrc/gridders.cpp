@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -107,7 +108,8 @@ static void layout_table(CFTable& t, const std::vector<int>& supp_z, int os) {
 // ---------------------------------------------------------------------------------------
 Gridder::Gridder(const Gridder& o)
     : ctx_(o.ctx_), cf_(o.cf_), padding_(o.padding_), track_per_os_(o.track_per_os_), use_all_psf_(o.use_all_psf_),
-      clear_grid_(o.clear_grid_), vw_ref_(o.vw_ref_), vw_order_(o.vw_order_), alpha_int_(o.alpha_int_) {
+      clear_grid_(o.clear_grid_), vw_ref_(o.vw_ref_), vw_order_(o.vw_order_), alpha_int_(o.alpha_int_),
+      cf_prec_(o.cf_prec_), vis_prec_(o.vis_prec_) {
   std::memcpy(shape_, o.shape_, sizeof(shape_));
   std::memcpy(img_, o.img_, sizeof(img_));
   uvcell_[0] = o.uvcell_[0];
@@ -133,6 +135,30 @@ void Gridder::configure_common(const Params& p) {
   use_all_psf_ = p.flag("alldatapsf", false);
   track_per_os_ = p.flag("oversampleweight", false);
   clear_grid_ = p.flag("cleargrids", false);
+  // Storage precision of the CF taps and of the gridding coefficients (weighted visibilities);
+  // no counterpart in the reference, whose tables and accessors are FP32 throughout.
+  cf_prec_ = parse_precision(p.str("cfprecision", "fp32"), "cfprecision");
+  vis_prec_ = parse_precision(p.str("visprecision", "fp32"), "visprecision");
+}
+
+int parse_precision(const std::string& v, const char* key) {
+  std::string s;
+  for (char ch : v) s += (char)std::tolower((unsigned char)ch);
+  if (s == "fp32" || s == "float32" || s == "float") return kPrecF32;
+  if (s == "fp16" || s == "float16" || s == "half") return kPrecF16;
+  if (s == "bf16" || s == "bfloat16") return kPrecBF16;
+  throw Error(1, std::string(key) + " must be fp32, fp16 or bf16, not " + v);
+}
+
+CFView Gridder::cf_view() {
+  const int smax = cf_->supp.empty() ? 0 : *std::max_element(cf_->supp.begin(), cf_->supp.end());
+  if (cf_prec_ == kPrecF32) return CFView{cf_->table.get(), cf_->table.size(), smax, kPrecF32};
+  if (cf_->table16_prec != cf_prec_ || cf_->table16.size() != cf_->n_elems * sizeof(uint32_t)) {
+    cf_->table16.resize(cf_->n_elems * sizeof(uint32_t));
+    launch_cf_round(*ctx_, cf_->table.as<float2>(), cf_->table16.as<uint32_t>(), cf_->n_elems, cf_prec_);
+    cf_->table16_prec = cf_prec_;
+  }
+  return CFView{cf_->table16.get(), cf_->table16.size(), smax, cf_prec_};
 }
 
 void Gridder::initialiseCommon(const ygrid_image_desc& img) {
@@ -329,6 +355,7 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   smp_hdr_.resize((size_t)nsamp * sizeof(SampleHdr));
   smp_key_.resize((size_t)nsamp * sizeof(int32_t));
   smp_coef_.resize((size_t)nsamp * kCoef * sizeof(float));
+  if (vis_prec_ != kPrecF32) smp_coef16_.resize((size_t)nsamp * kCoef * sizeof(uint16_t));
   counters_.resize(8 * sizeof(unsigned long long));
   err_.resize(sizeof(int) * 2);
   YG_HIP(hipMemsetAsync(counters_.get(), 0, 8 * sizeof(unsigned long long), ctx_->stream));
@@ -356,6 +383,7 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   if (zdiv < 1) zdiv = 1;
   pp.zdiv = (int)zdiv;
   pp.nzg = (int)((nz + zdiv - 1) / zdiv);
+  pp.vis_prec = vis_prec_;
 
   PrepInputs in{};
   in.uvw = d.uvw;
@@ -381,22 +409,19 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   if (profiling_) YG_HIP(hipEventRecord(ev_[0], ctx_->stream));
   launch_prep(*ctx_, pp, in, out);
   if (profiling_) YG_HIP(hipEventRecord(ev_[1], ctx_->stream));
-  const float2* cft = cf_->table.as<float2>();
-  const size_t cfb = cf_->table.size();
-  const int smax = cf_->supp.empty() ? 0 : *std::max_element(cf_->supp.begin(), cf_->supp.end());
-  if (bin_samples(*ctx_, smp, d.nrow, d.nchan, shape_[0], shape_[1], pp.nzg * shape_[3], smax)) ++n_bin_reused_;
+  const CFView cfv = cf_view();
+  if (bin_samples(*ctx_, smp, d.nrow, d.nchan, shape_[0], shape_[1], pp.nzg * shape_[3], cfv)) ++n_bin_reused_;
   const Binning& bins = ctx_->bins.bin;
   if (forward) {
     acc_.resize((size_t)nsamp * 8 * sizeof(float));
     YG_HIP(hipMemsetAsync(acc_.get(), 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_degrid(*ctx_, bins, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg,
-                  acc_.as<float>());
+    launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg, acc_.as<float>());
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
     launch_degrid_finish(*ctx_, smp, acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);
   } else {
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_grid(*ctx_, bins, smp, cft, cfb, smax, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg);
+    launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
   }
   unsigned long long cnt[4] = {0, 0, 0, 0};
@@ -556,6 +581,8 @@ void Gridder::stats(ygrid_stats* o) const {
   o->grid_pipeline = ctx_->grid_variant.pipeline;
   o->degrid_waves = ctx_->degrid_variant.waves;
   o->degrid_pipeline = ctx_->degrid_variant.pipeline;
+  o->cf_precision = cf_prec_;
+  o->vis_precision = vis_prec_;
 }
 
 void Gridder::getCF(int k, float* out, int* support) const {

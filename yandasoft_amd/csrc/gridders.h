@@ -35,6 +35,9 @@ struct CFTable {
   std::vector<int32_t> ofs_u, ofs_v;   // per z plane (empty: no offsets)
   DevBuf table, d_off, d_supp, d_ofs_u, d_ofs_v;
   size_t n_elems = 0;
+  // the table rounded to 16-bit precision table16_prec (built on first use at that precision)
+  DevBuf table16;
+  int table16_prec = 0;
   void upload(const Ctx& c);
 };
 
@@ -128,7 +131,14 @@ class Gridder {
 
   DevBuf grid_, sumw_, scratch_, ccfx_, ccfy_;
   DevBuf smp_hdr_, smp_key_, smp_coef_, acc_, counters_, err_, chanmap_, visweight_, staging_;
-  Samples samples() const { return Samples{smp_hdr_.as<SampleHdr>(), smp_key_.as<int32_t>(), smp_coef_.as<float>()}; }
+  Samples samples() const {
+    return Samples{smp_hdr_.as<SampleHdr>(), smp_key_.as<int32_t>(), smp_coef_.as<float>(),
+                   vis_prec_ != kPrecF32 ? smp_coef16_.as<uint16_t>() : nullptr, vis_prec_};
+  }
+  // the CF table at the configured storage precision (building the 16-bit copy if needed)
+  CFView cf_view();
+  int cf_prec_ = kPrecF32, vis_prec_ = kPrecF32;  // storage precisions (cfprecision, visprecision)
+  DevBuf smp_coef16_;
   double n_bin_reused_ = 0;  // passes that reused the context's cached binning
   PlanPtr plan_;
   double st_samples_g_ = 0, st_taps_g_ = 0, st_samples_d_ = 0, st_taps_d_ = 0, st_flag_ = 0, st_wflag_ = 0;
@@ -140,5 +150,7 @@ class Gridder {
 };
 
 std::unique_ptr<Gridder> make_gridder(std::shared_ptr<Ctx> ctx, const std::string& name, const ParamMap& parset);
+// "fp32" | "fp16" | "bf16" (case-insensitive; also float32, float16, half, bfloat16) -> Prec
+int parse_precision(const std::string& v, const char* key);
 
 }  // namespace ygrid

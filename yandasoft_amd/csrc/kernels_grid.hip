@@ -60,6 +60,18 @@ __device__ int w_plane(const CIndexDesc& ci, double w) {
   return (r < ci.nw && r > -1) ? r : -1;
 }
 
+// FP32 -> FP16 or BF16 bits, round to nearest even (finite values; BF16 by the carry into the
+// upper half, FP16 by the hardware conversion), and back.
+__device__ __forceinline__ uint16_t round16(float x, int prec) {
+  if (prec == kPrecF16) return __builtin_bit_cast(uint16_t, (_Float16)x);
+  const uint32_t u = __builtin_bit_cast(uint32_t, x);
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float widen16(uint32_t h, int prec) {
+  if (prec == kPrecF16) return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+  return __builtin_bit_cast(float, h << 16);
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -235,9 +247,21 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
   if (inb) {
     out.smp.hdr[s] = rec;
     out.smp.key[s] = key;
-    float4* c4 = reinterpret_cast<float4*>(out.smp.coef + (size_t)s * kCoef);
+    if (p.vis_prec != kPrecF32 && p.mode != kModeDegrid) {
+      // gridding coefficients stored at 16-bit precision: 32 bytes per sample
+      uint4* c4 = reinterpret_cast<uint4*>(out.smp.coef16 + (size_t)s * kCoef);
+      uint32_t wds[kCoef / 2];
 #pragma unroll
-    for (int q = 0; q < kCoef / 4; ++q) c4[q] = make_float4(coef[4 * q], coef[4 * q + 1], coef[4 * q + 2], coef[4 * q + 3]);
+      for (int q = 0; q < kCoef / 2; ++q)
+        wds[q] = (uint32_t)round16(coef[2 * q], p.vis_prec) | ((uint32_t)round16(coef[2 * q + 1], p.vis_prec) << 16);
+#pragma unroll
+      for (int q = 0; q < kCoef / 8; ++q) c4[q] = make_uint4(wds[4 * q], wds[4 * q + 1], wds[4 * q + 2], wds[4 * q + 3]);
+    } else {
+      float4* c4 = reinterpret_cast<float4*>(out.smp.coef + (size_t)s * kCoef);
+#pragma unroll
+      for (int q = 0; q < kCoef / 4; ++q)
+        c4[q] = make_float4(coef[4 * q], coef[4 * q + 1], coef[4 * q + 2], coef[4 * q + 3]);
+    }
   }
 
   c_valid += valid ? 1 : 0;
@@ -386,13 +410,13 @@ __device__ __forceinline__ uint32_t bit_range(int lo, int hi) {
 // worked by four waves, wave w on rows [16w, 16w+16), each in 8 blocks (k, j) of 4 rows x 32
 // columns (block rows 16w+4k .. +3, columns 32j .. +31 of the tile).
 //   x: sample index
-//   y: byte offset (in a CF buffer that starts pad bytes before the table) of the CF tap under
-//      the tile's pixel (0, 0), modulo 2^32; wave w adds 16*w rows
+//   y: byte offset (in a CF buffer that starts pad bytes before the table, taps of esz bytes)
+//      of the CF tap under the tile's pixel (0, 0), modulo 2^32; wave w adds 16*w rows
 //   z: bits 8w+2k+j: block (k, j) of wave w overlaps the footprint
 //   w: CF row pitch P (bits 0..15), and the signed bytes A (16..23), B (24..31) with: the two
 //      taps of lane column pair x = 32j + 2lu (tile columns x, x+1) touch the footprint when
 //      A <= x <= B (A = -1 - du0, B = 2S - du0, clamped to what x can take)
-__device__ __forceinline__ int4 pair_descriptor(const SampleHdr& r, int s, int u0, int v0, uint32_t pad) {
+__device__ __forceinline__ int4 pair_descriptor(const SampleHdr& r, int s, int u0, int v0, uint32_t pad, uint32_t esz) {
   const int S2 = 2 * (r.spack & 0xffff), P = r.spack >> 16;
   const int du0 = u0 - r.iu0, dv0 = v0 - r.iv0;
   // block column j: du0+32j+31 >= 0 and du0+32j <= 2S; block row k of wave w, with
@@ -406,15 +430,15 @@ __device__ __forceinline__ int4 pair_descriptor(const SampleHdr& r, int s, int u
     const uint32_t rows8 = (rows & 1u) | ((rows & 2u) << 1) | ((rows & 4u) << 2) | ((rows & 8u) << 3);
     masks |= (rows8 * cols) << (8 * w);
   }
-  const uint32_t soff = pad + (uint32_t)(r.cfoff + (dv0 + kGuardV) * P + (du0 + kGuardU)) * (uint32_t)sizeof(float2);
+  const uint32_t soff = pad + (uint32_t)(r.cfoff + (dv0 + kGuardV) * P + (du0 + kGuardU)) * esz;
   const int A = min(max(-1 - du0, -2), 64);
   const int B = min(max(S2 - du0, -1), 100);
   const uint32_t pw = (uint32_t)(P & 0xffff) | ((uint32_t)(A & 0xff) << 16) | ((uint32_t)(B & 0xff) << 24);
   return make_int4(s, (int)soff, (int)masks, (int)pw);
 }
 
-__global__ void k_bin_fill(Samples smp, int nrow, int nchan, int ntu, int ntiles, uint32_t pad, int* cursor,
-                           int4* descs) {
+__global__ void k_bin_fill(Samples smp, int nrow, int nchan, int ntu, int ntiles, uint32_t pad, uint32_t esz,
+                           int* cursor, int4* descs) {
   const long long tglob = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const BinLane b = bin_lane(smp, tglob, nrow, nchan, ntiles);
@@ -427,7 +451,7 @@ __global__ void k_bin_fill(Samples smp, int nrow, int nchan, int ntu, int ntiles
       const Run run = wave_run(idx, active, lane);
       const int hpos = run.head ? atomicAdd(&cursor[idx], run.len) : 0;
       const int pos = __shfl(hpos, run.hlane, 64) + run.rank;
-      if (active) descs[pos] = pair_descriptor(b.r, b.s, tu * kTile, tv * kTile, pad);
+      if (active) descs[pos] = pair_descriptor(b.r, b.s, tu * kTile, tv * kTile, pad, esz);
     }
 }
 
@@ -490,15 +514,29 @@ constexpr int kBufferWord3 = 0x00020000;
 // instead of once per pair on each wave's critical path.  A wave then reads a pair's descriptor
 // from LDS (every lane the same 16 bytes) and makes it wave-uniform.
 template <bool COEF>
-__device__ __forceinline__ void stage_item(const int4* __restrict__ descs, const float* __restrict__ coef, int pb,
-                                           int n, int4* s_desc, f32x4* s_coef, int tid, int nthreads) {
+__device__ __forceinline__ void stage_item(const int4* __restrict__ descs, const Samples& smp, int pb, int n,
+                                           int4* s_desc, f32x4* s_coef, int tid, int nthreads) {
   for (int t = tid; t < n; t += nthreads) {
     const int4 d = descs[pb + t];
     s_desc[t] = d;
     if constexpr (COEF) {
-      const f32x4* src = reinterpret_cast<const f32x4*>(coef + (size_t)d.x * kCoef);
+      if (smp.coef16) {
+        const uint4* src = reinterpret_cast<const uint4*>(smp.coef16 + (size_t)d.x * kCoef);
 #pragma unroll
-      for (int q = 0; q < kCoef / 4; ++q) s_coef[(kCoef / 4) * t + q] = src[q];
+        for (int h = 0; h < kCoef / 8; ++h) {
+          const uint4 u = src[h];
+          const uint32_t wd[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+            s_coef[(kCoef / 4) * t + 2 * h + q] =
+                f32x4{widen16(wd[2 * q] & 0xffffu, smp.coef_prec), widen16(wd[2 * q] >> 16, smp.coef_prec),
+                      widen16(wd[2 * q + 1] & 0xffffu, smp.coef_prec), widen16(wd[2 * q + 1] >> 16, smp.coef_prec)};
+        }
+      } else {
+        const f32x4* src = reinterpret_cast<const f32x4*>(smp.coef + (size_t)d.x * kCoef);
+#pragma unroll
+        for (int q = 0; q < kCoef / 4; ++q) s_coef[(kCoef / 4) * t + q] = src[q];
+      }
     }
   }
   __syncthreads();
@@ -526,6 +564,7 @@ struct PairGeom {
   uint32_t mask, soff0, rowstep;
 };
 
+template <int SH>
 __device__ __forceinline__ PairGeom pair_geom(const PairU& p, int w) {
   PairGeom g;
   const uint32_t pw = (uint32_t)p.d.w;
@@ -533,8 +572,8 @@ __device__ __forceinline__ PairGeom pair_geom(const PairU& p, int w) {
   g.A = (int)(pw << 8) >> 24;
   g.B = (int)pw >> 24;
   g.mask = ((uint32_t)p.d.z >> (8 * w)) & 0xffu;
-  g.soff0 = (uint32_t)p.d.y + (uint32_t)w * ((uint32_t)g.P << 7);  // 16 rows of 8-byte taps
-  g.rowstep = (uint32_t)g.P << 5;                                   // 4 rows
+  g.soff0 = (uint32_t)p.d.y + (uint32_t)w * ((uint32_t)g.P << (4 + SH));  // 16 rows of 2^SH-byte taps
+  g.rowstep = (uint32_t)g.P << (2 + SH);                                   // 4 rows
   return g;
 }
 
@@ -543,37 +582,53 @@ __device__ __forceinline__ PairGeom pair_geom(const PairU& p, int w) {
 // the buffer's num_records (which stays below 2 GiB) so that its load returns zeros; the other
 // lanes read inside the padded row (the partner of an edge tap lies in the guard band).
 constexpr uint32_t kOutOfRange = 0x80000000u;
+template <int SH>
 __device__ __forceinline__ void lane_offsets(uint32_t (&vo)[2], const PairGeom& g, int lu, int lv) {
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int x = 32 * j + 2 * lu;
     const bool in = (x >= g.A) && (x <= g.B);
-    vo[j] = in ? (uint32_t)(lv * g.P + x) * (uint32_t)sizeof(float2) : kOutOfRange;
+    vo[j] = in ? (uint32_t)(lv * g.P + x) << SH : kOutOfRange;
   }
 }
 
 // The CF taps of the active blocks: one 16-byte buffer load per lane and block, with the row in
 // the scalar offset and the lane's position in the vector offset.  COEF: also the sample's MFMA
 // B operands (the 16 bytes of the staged coefficients for the lane's output column o).
-template <bool COEF>
+// CFP: storage precision of the table (FP32: 16 bytes per lane and block; FP16 / BF16: 8 bytes,
+// widened to FP32 in registers).
+template <int CFP>
+constexpr int cf_shift() {
+  return CFP == kPrecF32 ? 3 : 2;
+}
+
+template <bool COEF, int CFP>
 __device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, const PairGeom& g, int lu, int lv,
                                         const f32x4* s_coef_pair, int o) {
   if constexpr (COEF) c.b = s_coef_pair[o];
   uint32_t vo[2];
-  lane_offsets(vo, g, lu, lv);
+  lane_offsets<cf_shift<CFP>()>(vo, g, lu, lv);
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
       if (g.mask & (1u << (2 * k + j))) {
-        // four dword loads, merged into one dwordx4 by the compiler (the vector-valued buffer
-        // load builtins of this toolchain load only the first dword)
+        // dword loads, merged into one dwordx4 (dwordx2) by the compiler (the vector-valued
+        // buffer load builtins of this toolchain load only the first dword)
         const uint32_t so = g.soff0 + k * g.rowstep;
-        uint32_t d[4];
+        if constexpr (CFP == kPrecF32) {
+          uint32_t d[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) d[i] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo[j] + 4u * i, so, 0);
-        c.t[k][j][0] = make_float2(__builtin_bit_cast(float, d[0]), __builtin_bit_cast(float, d[1]));
-        c.t[k][j][1] = make_float2(__builtin_bit_cast(float, d[2]), __builtin_bit_cast(float, d[3]));
+          for (int i = 0; i < 4; ++i) d[i] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo[j] + 4u * i, so, 0);
+          c.t[k][j][0] = make_float2(__builtin_bit_cast(float, d[0]), __builtin_bit_cast(float, d[1]));
+          c.t[k][j][1] = make_float2(__builtin_bit_cast(float, d[2]), __builtin_bit_cast(float, d[3]));
+        } else {
+          uint32_t d[2];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) d[i] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo[j] + 4u * i, so, 0);
+#pragma unroll
+          for (int e = 0; e < 2; ++e) c.t[k][j][e] = make_float2(widen16(d[e] & 0xffffu, CFP), widen16(d[e] >> 16, CFP));
+        }
       }
 }
 
@@ -606,7 +661,7 @@ __device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], c
 // Kernel argument describing the CF table: its address and size, and the pad (bytes, at least
 // 16 rows of the widest plane) by which the buffer resource starts before it.
 struct CFRef {
-  const float2* ptr;
+  const void* ptr;
   uint32_t bytes, pad;
 };
 
@@ -620,9 +675,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
 // before the current pair is worked.  PIPE: two register sets; the loads of pair i+1 are issued
 // before pair i is consumed.  consume must not issue vector memory loads whose results it waits
 // for after returning (the wait before the next loads covers them).
-template <bool COEF, bool PIPE, class F>
+template <bool COEF, bool PIPE, int CFP, class F>
 __device__ __forceinline__ void for_each_pair(const int4* s_desc, const f32x4* s_coef, __amdgpu_buffer_rsrc_t cfr,
                                               int n, int w, int lu, int lv, int o, F&& consume) {
+  constexpr int SH = cf_shift<CFP>();
   if (n <= 0) return;
   constexpr int CW = kCoef / 4;  // f32x4 per staged pair
   if constexpr (!PIPE) {
@@ -634,32 +690,32 @@ __device__ __forceinline__ void for_each_pair(const int4* s_desc, const f32x4* s
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       const PairU p = pn;
       if (i + 1 < n) pn = staged_pair(s_desc, i + 1);
-      const PairGeom g = pair_geom(p, w);
+      const PairGeom g = pair_geom<SH>(p, w);
       Taps c;
-      load_cf<COEF>(c, cfr, g, lu, lv, COEF ? s_coef + CW * i : s_coef, o);
+      load_cf<COEF, CFP>(c, cfr, g, lu, lv, COEF ? s_coef + CW * i : s_coef, o);
       consume(c, g, p, i);
     }
   } else {
     int i = 0;
     Taps cA, cB;
     PairU pA = staged_pair(s_desc, 0);
-    PairGeom gA = pair_geom(pA, w);
-    load_cf<COEF>(cA, cfr, gA, lu, lv, s_coef, o);
+    PairGeom gA = pair_geom<SH>(pA, w);
+    load_cf<COEF, CFP>(cA, cfr, gA, lu, lv, s_coef, o);
     PairU pB = pA;
     if (n > 1) pB = staged_pair(s_desc, 1);
     for (;;) {
       // pair i: taps in flight into cA; pB = record of pair i+1
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      const PairGeom gB = pair_geom(pB, w);
-      if (i + 1 < n) load_cf<COEF>(cB, cfr, gB, lu, lv, COEF ? s_coef + CW * (i + 1) : s_coef, o);
+      const PairGeom gB = pair_geom<SH>(pB, w);
+      if (i + 1 < n) load_cf<COEF, CFP>(cB, cfr, gB, lu, lv, COEF ? s_coef + CW * (i + 1) : s_coef, o);
       PairU pN = pB;
       if (i + 2 < n) pN = staged_pair(s_desc, i + 2);
       consume(cA, gA, pA, i);
       if (++i >= n) break;
       // pair i: taps in flight into cB; pN = record of pair i+1
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      gA = pair_geom(pN, w);
-      if (i + 1 < n) load_cf<COEF>(cA, cfr, gA, lu, lv, COEF ? s_coef + CW * (i + 1) : s_coef, o);
+      gA = pair_geom<SH>(pN, w);
+      if (i + 1 < n) load_cf<COEF, CFP>(cA, cfr, gA, lu, lv, COEF ? s_coef + CW * (i + 1) : s_coef, o);
       PairU pM = pN;
       if (i + 2 < n) pM = staged_pair(s_desc, i + 2);
       consume(cB, gB, pB, i);
@@ -670,11 +726,11 @@ __device__ __forceinline__ void for_each_pair(const int4* s_desc, const f32x4* s
   }
 }
 
-template <int NP, int WPE, bool PIPE>
+template <int NP, int WPE, bool PIPE, int CFP>
 __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
-                                              int* __restrict__ counter, const int4* __restrict__ descs,
-                                              const float* __restrict__ coef, CFRef cft, float* __restrict__ grid,
-                                              int nx, int ny, int ntu, int ntiles, int nzg) {
+                                              int* __restrict__ counter, const int4* __restrict__ descs, Samples smp,
+                                              CFRef cft, float* __restrict__ grid, int nx, int ny, int ntu,
+                                              int ntiles, int nzg) {
   __shared__ int s_item;
   __shared__ int4 s_desc[kChunk];
   __shared__ f32x4 s_coef[kChunk * (kCoef / 4)];
@@ -716,8 +772,8 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
           for (int m = 0; m < NS; ++m) acc[k][j][e][m] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int n = it.z - it.y;
-    stage_item<true>(descs, coef, it.y, n, s_desc, s_coef, tid, blockDim.x);
-    for_each_pair<true, PIPE>(s_desc, s_coef, cfr, n, w, lu, lv, o,
+    stage_item<true>(descs, smp, it.y, n, s_desc, s_coef, tid, blockDim.x);
+    for_each_pair<true, PIPE, CFP>(s_desc, s_coef, cfr, n, w, lu, lv, o,
                               [&](const Taps& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
     // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of row 4k+lv, column
     // 32j+8(b&3)+2i+e.  Per (m, k) the wave writes its 4 rows x 64 columns x 4 outputs to its own
@@ -829,7 +885,7 @@ __device__ __forceinline__ void degrid_pair(float (&x)[8], const Taps& c, const 
       }
 }
 
-template <int NP, int WPE, bool PIPE>
+template <int NP, int WPE, bool PIPE, int CFP>
 __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
                                                 int* __restrict__ counter, const int4* __restrict__ descs,
                                                 CFRef cft, const float2* __restrict__ grid, int nx, int ny, int ntu,
@@ -877,8 +933,8 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     // per pair: wave dot product, reduce-scatter, one LDS slot per (pair, wave, value); every
     // kDegridBatch pairs the four waves' partial sums meet and go to the per-sample accumulators
     const int n = it.z - it.y;
-    stage_item<false>(descs, nullptr, it.y, n, s_desc, nullptr, tid, blockDim.x);
-    for_each_pair<false, PIPE>(s_desc, nullptr, cfr, n, w, lu, lv, 0,
+    stage_item<false>(descs, Samples{}, it.y, n, s_desc, nullptr, tid, blockDim.x);
+    for_each_pair<false, PIPE, CFP>(s_desc, nullptr, cfr, n, w, lu, lv, 0,
                            [&](const Taps& c, const PairGeom& g, const PairU& p, int i) {
                              float x[8];
                              degrid_pair<NP>(x, c, gt, g);
@@ -944,11 +1000,11 @@ void launch_prep(const Ctx& c, const PrepParams& p, const PrepInputs& in, const 
 // Bytes by which the tile kernels' CF buffer resource starts before the table: 16 rows of the
 // widest plane and a little (a wave with an active block starts at most 11 rows and 47 columns
 // before its plane).
-static uint32_t cf_pad_bytes(int max_support) {
-  return (uint32_t)((16 * (2 * (size_t)max_support + 1 + 2 * kGuardU) + 64) * sizeof(float2));
+static uint32_t cf_pad_bytes(int max_support, int prec) {
+  return (uint32_t)((16 * (2 * (size_t)max_support + 1 + 2 * kGuardU) + 64) * cf_elem_bytes(prec));
 }
 
-void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int nx, int ny, int nkz, int max_support,
+void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
                     Binning& b) {
   const int n = nrow * nchan;
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
@@ -979,7 +1035,8 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
   YG_HIP(hipMemcpyAsync(b.cursor.get(), offsets, (nkeys + 1) * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, nrow, nchan, ntu, (int)ntiles,
-                       cf_pad_bytes(max_support), b.cursor.as<int>(), b.pairs.as<int4>());
+                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), b.cursor.as<int>(),
+                       b.pairs.as<int4>());
   // work items: reuse cursor as the per-key item count and its scan
   int* icount = b.cursor.as<int>();
   // the tile kernels stage a whole item in LDS: at most kChunk pairs
@@ -1014,10 +1071,11 @@ __global__ void k_samples_differ(const SampleHdr* __restrict__ ha, const int32_t
 }
 }  // namespace
 
-bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkz, int max_support) {
+bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf) {
   BinCache& bc = c.bins;
   const int n = nrow * nchan;
-  if (bc.valid && bc.nrow == nrow && bc.nchan == nchan && bc.nx == nx && bc.ny == ny && bc.nkz == nkz && bc.smax == max_support) {
+  if (bc.valid && bc.nrow == nrow && bc.nchan == nchan && bc.nx == nx && bc.ny == ny && bc.nkz == nkz &&
+      bc.smax == cf.max_support && bc.cf_prec == cf.prec) {
     bc.flag.resize(sizeof(int));
     YG_HIP(hipMemsetAsync(bc.flag.get(), 0, sizeof(int), c.stream));
     if (n > 0)
@@ -1029,7 +1087,7 @@ bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny
     if (!differ) return true;
   }
   bc.valid = false;
-  launch_binning(c, smp, nrow, nchan, nx, ny, nkz, max_support, bc.bin);
+  launch_binning(c, smp, nrow, nchan, nx, ny, nkz, cf, bc.bin);
   bc.hdr.resize((size_t)(n > 0 ? n : 1) * sizeof(SampleHdr));
   bc.key.resize((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
   if (n > 0) {
@@ -1041,82 +1099,116 @@ bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny
   bc.nx = nx;
   bc.ny = ny;
   bc.nkz = nkz;
-  bc.smax = max_support;
+  bc.smax = cf.max_support;
+  bc.cf_prec = cf.prec;
   bc.valid = true;
   return false;
 }
 
 static int persistent_blocks(const Ctx& c, const TileVariant& v) { return c.n_cu * (v.waves >= 3 ? 3 : 2); }
 
-// kernel variant by polarisation count, register budget (waves per SIMD) and pipelining
-template <int NP, int WPE, bool PIPE>
+// kernel variant by polarisation count, register budget (waves per SIMD), pipelining and CF
+// storage precision
+template <int NP, int WPE, bool PIPE, int CFP>
 struct TileKernels {
   static void grid(hipStream_t s, dim3 g, const Binning& b, const Samples& smp, CFRef cft, float* gf, int nx, int ny,
                    int ntu, int ntiles, int nzg) {
-    hipLaunchKernelGGL((k_grid<NP, WPE, PIPE>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
-                       b.counter.as<int>(), b.pairs.as<int4>(), smp.coef, cft, gf, nx, ny, ntu, ntiles, nzg);
+    hipLaunchKernelGGL((k_grid<NP, WPE, PIPE, CFP>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
+                       b.counter.as<int>(), b.pairs.as<int4>(), smp, cft, gf, nx, ny, ntu, ntiles, nzg);
   }
   static void degrid(hipStream_t s, dim3 g, const Binning& b, CFRef cft, const float2* grid, int nx, int ny, int ntu,
                      int ntiles, int nzg, float* acc) {
-    hipLaunchKernelGGL((k_degrid<NP, WPE, PIPE>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
+    hipLaunchKernelGGL((k_degrid<NP, WPE, PIPE, CFP>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
                        b.counter.as<int>(), b.pairs.as<int4>(), cft, grid, nx, ny, ntu, ntiles, nzg, acc);
   }
 };
 
 // The tile kernels address the CF table through a buffer resource with 32-bit byte offsets
 // that starts cf_pad_bytes before the table.
-static CFRef cf_ref(const float2* cf, size_t cf_bytes, int max_support) {
-  const size_t pad = cf_pad_bytes(max_support);
+static CFRef cf_ref(const CFView& cf) {
+  const size_t pad = cf_pad_bytes(cf.max_support, cf.prec);
   // below 2 GiB, so that an out-of-range lane offset (2^31) plus any in-range scalar offset
   // neither wraps nor lands inside the buffer
-  YG_CHECK(cf_bytes > 0 && cf_bytes + pad < kOutOfRange, 1, "CF table must be smaller than 2 GiB");
-  return CFRef{cf, (uint32_t)cf_bytes, (uint32_t)pad};
+  YG_CHECK(cf.bytes > 0 && cf.bytes + pad < kOutOfRange, 1, "CF table must be smaller than 2 GiB");
+  return CFRef{cf.ptr, (uint32_t)cf.bytes, (uint32_t)pad};
 }
 
-template <template <int, int, bool> class K, int NP, class F>
+// FP32 tables: every variant; 16-bit tables: the two-wave pipelined form only
+template <int NP, int CFP, class F>
 static void with_variant(const TileVariant& v, F&& f) {
-  if (v.waves >= 3) {
-    if (v.pipeline) f(K<NP, 3, true>{});
-    else f(K<NP, 3, false>{});
+  if constexpr (CFP != kPrecF32) {
+    (void)v;
+    f(TileKernels<NP, 2, true, CFP>{});
+  } else if (v.waves >= 3) {
+    if (v.pipeline) f(TileKernels<NP, 3, true, CFP>{});
+    else f(TileKernels<NP, 3, false, CFP>{});
   } else {
-    if (v.pipeline) f(K<NP, 2, true>{});
-    else f(K<NP, 2, false>{});
+    if (v.pipeline) f(TileKernels<NP, 2, true, CFP>{});
+    else f(TileKernels<NP, 2, false, CFP>{});
+  }
+}
+
+template <int NP, class F>
+static void with_precision(const TileVariant& v, int prec, F&& f) {
+  switch (prec) {
+    case kPrecF32: with_variant<NP, kPrecF32>(v, f); break;
+    case kPrecF16: with_variant<NP, kPrecF16>(v, f); break;
+    case kPrecBF16: with_variant<NP, kPrecBF16>(v, f); break;
+    default: throw Error(1, "unknown CF precision");
   }
 }
 
 template <class F>
-static void with_tile_kernels(const TileVariant& v, int npol, F&& f) {
+static void with_tile_kernels(const TileVariant& v, int npol, int prec, F&& f) {
   switch (npol) {
-    case 1: with_variant<TileKernels, 1>(v, f); break;
-    case 2: with_variant<TileKernels, 2>(v, f); break;
-    case 3: with_variant<TileKernels, 3>(v, f); break;
-    case 4: with_variant<TileKernels, 4>(v, f); break;
+    case 1: with_precision<1>(v, prec, f); break;
+    case 2: with_precision<2>(v, prec, f); break;
+    case 3: with_precision<3>(v, prec, f); break;
+    case 4: with_precision<4>(v, prec, f); break;
     default: throw Error(1, "unsupported number of polarisations (1..4)");
   }
 }
 
-void launch_grid(const Ctx& c, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes,
-                 int max_support, float2* grid, int nx, int ny, int npol, int nzg) {
+void launch_grid(const Ctx& c, const Binning& b, const Samples& smp, const CFView& cf, float2* grid, int nx, int ny,
+                 int npol, int nzg) {
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const int ntiles = ntu * ntv;
-  const CFRef cft = cf_ref(cf, cf_bytes, max_support);
+  const CFRef cft = cf_ref(cf);
   YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
-  const dim3 g(persistent_blocks(c, c.grid_variant));
+  const TileVariant v = cf.prec == kPrecF32 ? c.grid_variant : TileVariant{2, 1};
+  const dim3 g(persistent_blocks(c, v));
   float* gf = reinterpret_cast<float*>(grid);
-  with_tile_kernels(c.grid_variant, npol,
+  with_tile_kernels(v, npol, cf.prec,
                     [&](auto k) { k.grid(c.stream, g, b, smp, cft, gf, nx, ny, ntu, ntiles, nzg); });
   YG_HIP(hipGetLastError());
 }
 
-void launch_degrid(const Ctx& c, const Binning& b, const float2* cf, size_t cf_bytes, int max_support,
-                   const float2* grid, int nx, int ny, int npol, int nzg, float* acc) {
+void launch_degrid(const Ctx& c, const Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol,
+                   int nzg, float* acc) {
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const int ntiles = ntu * ntv;
-  const CFRef cft = cf_ref(cf, cf_bytes, max_support);
+  const CFRef cft = cf_ref(cf);
   YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
-  const dim3 g(persistent_blocks(c, c.degrid_variant));
-  with_tile_kernels(c.degrid_variant, npol,
+  const TileVariant v = cf.prec == kPrecF32 ? c.degrid_variant : TileVariant{2, 1};
+  const dim3 g(persistent_blocks(c, v));
+  with_tile_kernels(v, npol, cf.prec,
                     [&](auto k) { k.degrid(c.stream, g, b, cft, grid, nx, ny, ntu, ntiles, nzg, acc); });
+  YG_HIP(hipGetLastError());
+}
+
+namespace {
+__global__ void k_cf_round(const float2* __restrict__ src, uint32_t* __restrict__ dst, size_t n, int prec) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 v = src[i];
+  dst[i] = (uint32_t)round16(v.x, prec) | ((uint32_t)round16(v.y, prec) << 16);
+}
+}  // namespace
+
+void launch_cf_round(const Ctx& c, const float2* src, uint32_t* dst, size_t n, int prec) {
+  YG_CHECK(prec == kPrecF16 || prec == kPrecBF16, 1, "CF rounding needs a 16-bit precision");
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_cf_round, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, src, dst, n, prec);
   YG_HIP(hipGetLastError());
 }
 

@@ -106,6 +106,12 @@ struct alignas(16) SampleHdr {
 };
 static_assert(sizeof(SampleHdr) == 16, "SampleHdr must be 16 bytes");
 constexpr int kCoef = 16;  // floats per sample in Samples::coef
+
+// Storage precision of CF taps and of the gridding coefficients (the weighted visibilities):
+// FP32, or FP16 / BF16 (rounded to nearest even when stored); arithmetic is FP32 throughout.
+enum Prec : int { kPrecF32 = 0, kPrecF16 = 1, kPrecBF16 = 2 };
+// bytes of one complex CF tap stored at precision p
+inline int cf_elem_bytes(int p) { return p == kPrecF32 ? 8 : 4; }
 struct Samples {
   SampleHdr* hdr;
   int32_t* key;         // binning key: (ichan, z-group) plane index
@@ -115,6 +121,10 @@ struct Samples {
   // kernel's MFMA operands: [4*o + m] with output o = 2*(q&1) + (0 re, 1 im) and operand
   // m = 2*(q/2) + (0 for C.re, 1 for C.im); degrid: [0..1] = the phasor times the weight.
   float* coef;
+  // the gridding coefficients stored at 16-bit precision coef_prec (kCoef per sample, same
+  // order) instead of coef, or null
+  uint16_t* coef16;
+  int coef_prec;
 };
 
 struct PrepParams {
@@ -130,6 +140,7 @@ struct PrepParams {
   CIndexDesc ci;
   int zdiv;                     // z-group divisor for binning keys
   int nzg;                      // number of z groups
+  int vis_prec;                 // Prec of the stored gridding coefficients
 };
 
 struct PrepOutputs {
@@ -176,7 +187,7 @@ struct TileVariant {
 // degridding and gridding of one chunk in a major cycle) reuses it.
 struct BinCache {
   DevBuf hdr, key, flag;
-  int nrow = -1, nchan = -1, nx = 0, ny = 0, nkz = 0, smax = 0;
+  int nrow = -1, nchan = -1, nx = 0, ny = 0, nkz = 0, smax = 0, cf_prec = 0;
   bool valid = false;
   Binning bin;
 };
@@ -194,20 +205,29 @@ struct Ctx {
 
 // Kernel launchers (kernels_grid.hip)
 void launch_prep(const Ctx&, const PrepParams&, const PrepInputs&, const PrepOutputs&);
+// The CF table as the tile kernels read it: taps stored at precision prec (cf_elem_bytes each),
+// bytes in all (the kernels address it with 32-bit byte offsets), max_support the largest
+// support of its planes.
+struct CFView {
+  const void* ptr;
+  size_t bytes;
+  int max_support;
+  int prec;
+};
 // Bins valid records by (key, tile); returns nothing, work lands in b.  The samples are the
-// nrow x nchan (row, channel) samples of one chunk, s = row + nrow*chan.
-// max_support: largest support of the CF table the pairs will be worked with
-void launch_binning(const Ctx&, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkeys_z, int max_support,
+// nrow x nchan (row, channel) samples of one chunk, s = row + nrow*chan.  The pair descriptors
+// address the CF table cf describes.
+void launch_binning(const Ctx&, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkeys_z, const CFView& cf,
                     Binning& b);
 // Bins the samples into ctx.bins unless the cached binning was built from identical sample
 // headers and keys with the same parameters; returns true when the cache was reused.
-bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkeys_z, int max_support);
-// cf_bytes: size of the CF table (the kernels address it with 32-bit byte offsets);
-// max_support: largest support of its planes
-void launch_grid(const Ctx&, const Binning& b, const Samples& smp, const float2* cf, size_t cf_bytes, int max_support,
-                 float2* grid, int nx, int ny, int npol, int nzg);
-void launch_degrid(const Ctx&, const Binning& b, const float2* cf, size_t cf_bytes, int max_support,
-                   const float2* grid, int nx, int ny, int npol, int nzg, float* acc);
+bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkeys_z, const CFView& cf);
+void launch_grid(const Ctx&, const Binning& b, const Samples& smp, const CFView& cf, float2* grid, int nx, int ny,
+                 int npol, int nzg);
+void launch_degrid(const Ctx&, const Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol,
+                   int nzg, float* acc);
+// Rounds the FP32 CF table (n complex taps) to prec (FP16 or BF16), packed (re, im) in 32 bits.
+void launch_cf_round(const Ctx&, const float2* src, uint32_t* dst, size_t n, int prec);
 void launch_degrid_finish(const Ctx&, const Samples& smp, const float* acc, int nrow, int nchan, int npol,
                           float2* vis_out);
 
