@@ -8,6 +8,11 @@ oversample 4, support 64), with every input already resident in HBM.
 
 Metric: CF-tap updates/s (one complex multiply-accumulate of a (visibility, pol) with a CF tap =
 8 FP32 FLOP, TableVisGridder.cc:264-267,804), whole job.  One JSON line is printed.
+
+``--workload awproject_feeds`` measures the A-projection mixed-precision case instead: 160,000
+rows x 32 channels around 1.4 GHz (uv uniform in a 2 km disc), 9 feeds on a 1 degree grid, a
+12 m uniform disk, 2048^2 grid; grid + degrid with the CF taps and gridding coefficients stored
+in FP32, FP16 and BF16 (FP32 accumulation), the value being the FP16 throughput.
 """
 from __future__ import annotations
 
@@ -107,6 +112,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (3x), the reference for rocprof byte counters")
+    ap.add_argument("--workload", default="wproject_array", choices=["wproject_array", "awproject_feeds"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md"}))
@@ -120,6 +126,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
+    if args.workload == "awproject_feeds":
+        bench_awproject(args)
         return
     import yandasoft_amd as ya
     from yandasoft_amd import workloads as W
@@ -222,6 +231,146 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(wl, gg)
+    print(json.dumps(result), flush=True)
+
+
+def aw_cpu_baseline(wl, target_s: float = 12.0):
+    """Oracle's single-thread FP32 restatement of the A-projection grid + degrid on a bounded row
+    sample of the same workload (its own CF generation is not timed)."""
+    from oracle import oracle as O
+    p = wl.params
+    prm = O.AWProjectParams(wmax=p["wmax"], nwplanes=p["nwplanes"], oversample=p["oversample"],
+                            maxsupport=p["maxsupport"], maxfeeds=p["maxfeeds"], cutoff=p["cutoff"], diameter=12.0,
+                            blockage=0.0)
+    lm = {f: O.direction_lm(tuple(wl.pointing1[f]), wl.centre) for f in range(p["maxfeeds"])}
+
+    def run(nrow_s):
+        rows = np.linspace(0, wl.nrow - 1, nrow_s).astype(int)
+        ch = O.Chunk(wl.uvw[rows], wl.freq, np.transpose(wl.vis[:, :, rows], (2, 1, 0)),
+                     noise=np.transpose(wl.noise[:, :, rows], (2, 1, 0)))
+        ch.feed1 = wl.feed1[rows]
+        og = O.OracleGridder("AWProject", "f32", wproject=prm)
+        og.aw_lm, og.aw_nchan = lm, wl.nchan
+        og.initialise_grid((wl.nx, wl.ny, 1, 1), (wl.cell, wl.cell))
+        og._init_aw(ch, False)  # CF generation outside the timed region
+        t0 = time.perf_counter()
+        og.grid(ch)
+        t1 = time.perf_counter()
+        od = O.OracleGridder("AWProject", "f32", wproject=prm)
+        od.aw_lm, od.aw_nchan = lm, wl.nchan
+        od.initialise_degrid((wl.nx, wl.ny, 1, 1), (wl.cell, wl.cell), np.ones((1, 1, wl.ny, wl.nx)))
+        od.cf, od.aw_state, od.aw_done = og.cf, og.aw_state, og.aw_done
+        t2 = time.perf_counter()
+        od.degrid(ch)
+        t3 = time.perf_counter()
+        return og.counters.taps_gridded + od.counters.taps_degridded, (t1 - t0) + (t3 - t2)
+
+    n = 200
+    taps, dt = run(n)
+    for _ in range(3):  # the first sizes are dominated by fixed costs: rescale until near the target
+        if dt >= 0.5 * target_s or n >= wl.nrow:
+            break
+        n = min(wl.nrow, max(n + 1, int(n * target_s / max(dt, 1e-3))))
+        taps, dt = run(n)
+    return {"value": taps / dt, "unit": "CF-tap updates/s", "cores": 1, "kind": "port",
+            "sample": f"{n} rows x {wl.nchan} chans of the same workload, grid + degrid, oracle FP32 "
+                      f"restatement (gcc -O2, 1 thread), {dt:.1f} s"}
+
+
+def bench_awproject(args):
+    import torch
+    import yandasoft_amd as ya
+    from yandasoft_amd import workloads as W
+
+    wl = W.awproject_feeds()
+    log(f"workload rows={wl.nrow} chans={wl.nchan} feeds={wl.params['maxfeeds']} grid={wl.nx}")
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    chunk = ya.Chunk(uvw=t(wl.uvw), freq=t(wl.freq), vis=t(wl.vis.view(np.float32)),
+                     vis_out=torch.zeros(wl.vis.size * 2, dtype=torch.float32, device=dev),
+                     noise=t(wl.noise.view(np.float32)), feed1=t(wl.feed1), pointing1=t(wl.pointing1), device=True)
+    chunk.npol = 1
+    img = ya.ImageDesc(wl.nx, wl.ny, 1, 1, wl.cell, wl.cell, centre=wl.centre)
+    rng = np.random.default_rng(3)
+    model = np.zeros((1, 1, wl.ny, wl.nx), np.float32)
+    for _ in range(30):
+        model[0, 0, rng.integers(600, 1400), rng.integers(600, 1400)] = rng.normal()
+    t_model = t(model)
+    ctx = ya.Context(0)
+    modes, images, vis_out = {}, {}, {}
+    for prec in ("fp32", "fp16", "bf16"):
+        params = dict(wl.params, cfprecision=prec, visprecision=prec)
+        gg = ya.Gridder(ctx, "GPUAWProject", params)
+        dg = ya.Gridder(ctx, "GPUAWProject", params)
+        t0 = time.perf_counter()
+        gg.initialise_grid(img)
+        gg.grid(chunk)  # first pass: CF generation for the 9 feeds x 32 channels, then gridding
+        ctx.synchronize()
+        first = time.perf_counter() - t0
+        cf_s = gg.stats().time_cf_s
+        dg.initialise_degrid(img, t_model)
+
+        def step():
+            ctx.clear_cache()
+            gg.initialise_grid(img)
+            gg.grid(chunk)
+            dg.degrid(chunk)
+
+        for _ in range(args.warmup):
+            step()
+        ctx.synchronize()
+        s0g, s0d = gg.stats(), dg.stats()
+        gg.set_profiling(True)
+        dg.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        s1g, s1d = gg.stats(), dg.stats()
+        taps_g = s1g.taps_gridded - s0g.taps_gridded
+        taps_d = s1d.taps_degridded - s0d.taps_degridded
+        gk = s1g.ms_grid_kernel / max(1.0, s1g.n_grid_launches)
+        dk = s1d.ms_degrid_kernel / max(1.0, s1d.n_degrid_launches)
+        modes[prec] = {"value": (taps_g + taps_d) / el, "ms_per_step": 1e3 * el / args.steps,
+                       "first_pass_s": first, "cf_generation_s": cf_s, "grid_kernel_ms": gk, "degrid_kernel_ms": dk,
+                       "taps_per_step": (taps_g + taps_d) / args.steps, "support": s1g.support,
+                       "_grid_taps": taps_g / args.steps}
+        chunk.vis_out.zero_()
+        dg.degrid(chunk)
+        vis_out[prec] = chunk.vis_out.cpu().numpy()
+        images[prec] = gg.finalise_grid()
+        log(f"{prec}: {modes[prec]['value'] / 1e12:.3f} Ttaps/s, {modes[prec]['ms_per_step']:.1f} ms/step")
+    ref_i, ref_v = images["fp32"], vis_out["fp32"]
+    for prec in ("fp16", "bf16"):
+        q = slice(wl.nx // 4, 3 * wl.nx // 4)
+        modes[prec]["deviation_vs_fp32_gpu"] = {
+            "image_central_half": float(np.max(np.abs(images[prec][..., q, q] - ref_i[..., q, q])) /
+                                        np.max(np.abs(ref_i))),
+            "degridded_vis": float(np.max(np.abs(vis_out[prec] - ref_v)) / np.max(np.abs(ref_v)))}
+    m = modes["fp16"]
+    kname, kms = ("k_grid<1, 2, true, 1>", m["grid_kernel_ms"])
+    achieved = 8.0 * m["_grid_taps"] / (kms * 1e-3) / 1e12
+    for v in modes.values():
+        v.pop("_grid_taps")
+    result = {
+        "metric": "CF-tap updates/s (AW-proj grid+degrid, FP16 CF and visibilities)",
+        "value": m["value"], "unit": "CF-tap updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16 storage, f32 arithmetic", "data": "synthetic",
+        "config": {"workload": "awproject_feeds: 160000 rows x 32 ch 1.384-1.416 GHz, uv uniform in a 2 km disc, "
+                               "9 feeds on a 1 deg grid, 12 m disk, 2048^2, os 8",
+                   "rows": wl.nrow, "chans": wl.nchan, "feeds": wl.params["maxfeeds"], "grid": wl.nx,
+                   "parallelism": "1 GPU"},
+        "modes": modes,
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = aw_cpu_baseline(wl)
     print(json.dumps(result), flush=True)
 
 

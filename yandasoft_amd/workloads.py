@@ -8,6 +8,9 @@ askap/simulation/Simulator.cc:932-963).
                         disc, w = 0 (the tConvolve / tGridding style case).
 * ``wproject_array`` -- W-projection: 36 antennas uniform in a disc, earth rotation over the
                         given hour-angle span at a declination, linear channels, 4 pols.
+* ``awproject_feeds`` -- A-projection: the ``sphfunc_disc`` uv generator at 32 channels around
+                        1.4 GHz, 9 feeds pointed on a 3x3 grid of 1 degree spacing, a 12 m
+                        uniform disk illumination.
 """
 from __future__ import annotations
 
@@ -32,6 +35,10 @@ class Workload:
     cell: float              # radians
     gridder: str
     params: dict = field(default_factory=dict)
+    feed1: np.ndarray | None = None       # (nrow,) int32 feed of each row
+    pointing1: np.ndarray | None = None   # (nrow, 2) pointing (lon, lat) rad of each row
+    centre: tuple = (0.0, 0.0)            # image centre (lon, lat) rad
+    pa: np.ndarray | None = None          # (nrow,) parallactic angle rad (no effect on a symmetric disk)
 
     @property
     def nrow(self):
@@ -106,3 +113,33 @@ def wproject_array(nint: int = 300, nchan: int = 64, npol: int = 4, fmin: float 
     params = {"wmax": wmax, "nwplanes": nwplanes, "oversample": oversample, "maxsupport": maxsupport,
               "limitsupport": limitsupport, "cutoff": cutoff}
     return Workload("wproject_array", uvw, freq, vis, noise, n, n, cell_arcsec * ARCSEC, "GPUWProject", params)
+
+
+def awproject_feeds(nvis: int = 160_000, nchan: int = 32, fcentre: float = 1.4e9, bandwidth: float = 32e6,
+                    radius_m: float = 2000.0, n: int = 2048, cell_arcsec: float = 5.0, nfeed_side: int = 3,
+                    spacing_deg: float = 1.0, nint: int = 160, pa_step_deg: float = 5.0, oversample: int = 8,
+                    maxsupport: int = 128, cutoff: float = 1e-3, diameter: str = "12m",
+                    centre: tuple = (0.5, -0.7), seed: int = 20261203) -> Workload:
+    """A-projection workload: rows of the sphfunc_disc generator (uv uniform in a disc, w = 0) at
+    nchan linear channels over fcentre +- bandwidth/2; row r belongs to feed r % 9 of a 3x3 grid
+    of pointings spacing_deg apart around the image centre, and to integration r // (nvis/nint)
+    whose parallactic angle steps by pa_step_deg (a uniform disk is rotationally symmetric, so the
+    angle changes no convolution function: AWProjectVisGridder.cc:312,373)."""
+    rng = np.random.default_rng(seed)
+    r = radius_m * np.sqrt(rng.random(nvis))
+    th = 2.0 * math.pi * rng.random(nvis)
+    uvw = np.stack([r * np.cos(th), r * np.sin(th), np.zeros(nvis)], axis=1)
+    freq = fcentre + np.linspace(-bandwidth / 2, bandwidth / 2, nchan)
+    vis = _cn(rng, (1, nchan, nvis))
+    noise = np.ones_like(vis)
+    nfeed = nfeed_side * nfeed_side
+    feed1 = (np.arange(nvis) % nfeed).astype(np.int32)
+    sp = math.radians(spacing_deg)
+    fx = (feed1 % nfeed_side - nfeed_side // 2) * sp
+    fy = (feed1 // nfeed_side - nfeed_side // 2) * sp
+    pointing = np.stack([centre[0] + fx / math.cos(centre[1]), centre[1] + fy], axis=1)
+    pa = np.radians(pa_step_deg) * (np.arange(nvis) // max(1, nvis // nint))
+    params = {"wmax": 1.0, "nwplanes": 1, "oversample": oversample, "maxsupport": maxsupport, "maxfeeds": nfeed,
+              "cutoff": cutoff, "diameter": diameter, "blockage": "0m", "frequencydependent": True}
+    return Workload("awproject_feeds", uvw, freq, vis, noise, n, n, cell_arcsec * ARCSEC, "GPUAWProject", params,
+                    feed1=feed1, pointing1=np.ascontiguousarray(pointing), centre=centre, pa=pa)
