@@ -151,6 +151,36 @@ int ygrid_get_cf(const ygrid_gridder *g, int k, float *out, int *support);
 /* Copy the padded grid (complex, casacore layout) to host; *n receives the float count. */
 int ygrid_get_grid(const ygrid_gridder *g, float *out, size_t *n);
 
+/* ---- batched passes: no counterpart in the reference, whose imaging equation runs one gridder
+ * after the other (ImageFFTEquation.cc:362-387).  For callers that grid or degrid many images per
+ * major cycle (beams x channels x polarisations of uneven sizes).
+ * A batch holds passes (gridder, chunk in device memory); ygrid_batch_run gives every gridder the
+ * result of ygrid_grid / ygrid_degrid on its chunk (by how the gridder was initialised), running
+ * per pass the prep and binning kernels and then ONE persistent tile-kernel launch per (grid or
+ * degrid, polarisations, CF precision) over the work items of all passes.  A gridder takes part
+ * in one pass per batch; chunks must keep their addresses and shapes while the batch is used
+ * (their contents may change between runs).  Not thread-safe; gridders of a batch share its
+ * context. */
+typedef struct ygrid_batch ygrid_batch;
+typedef struct {
+  int grid_waves, grid_pipeline, degrid_waves, degrid_pipeline; /* tile-kernel variant chosen */
+  int item_pairs;   /* pairs per work item chosen */
+  double ms;        /* best run time of the batch, milliseconds */
+  int trials;       /* configurations timed */
+} ygrid_tuning;
+int ygrid_batch_create(ygrid_ctx *ctx, ygrid_batch **out);
+int ygrid_batch_destroy(ygrid_batch *b);
+int ygrid_batch_add(ygrid_batch *b, ygrid_gridder *g, const ygrid_chunk *acc);
+/* enable: the next run captures the whole device sequence in a HIP graph, later runs replay it */
+int ygrid_batch_set_graph(ygrid_batch *b, int enable);
+int ygrid_batch_run(ygrid_batch *b);
+/* Times the tile-kernel variants and work-item sizes on this batch (min over `trials` runs each)
+ * and keeps the fastest on the context.  Each trial is a full run: grids and degridded
+ * visibilities accumulate it, so initialise them again before the runs that count. */
+int ygrid_batch_autotune(ygrid_batch *b, int trials, ygrid_tuning *best);
+/* wall time of the last run (ms) and number of graph captures so far */
+int ygrid_batch_info(const ygrid_batch *b, double *last_ms, int *captures);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@ library has not been built, loading fails loudly.
 from .abi import (  # noqa: F401
     YGRID_MEM_DEVICE,
     YGRID_MEM_HOST,
+    Batch,
     Chunk,
     Context,
     Gridder,

@@ -17,6 +17,8 @@ EXPORTED = [
     "ygrid_initialise_grid", "ygrid_grid", "ygrid_finalise_grid", "ygrid_finalise_weights",
     "ygrid_initialise_degrid", "ygrid_degrid", "ygrid_finalise_degrid", "ygrid_is_model_empty",
     "ygrid_set_mfs_weight", "ygrid_get_stats", "ygrid_set_profiling", "ygrid_get_cf", "ygrid_get_grid",
+    "ygrid_batch_create", "ygrid_batch_destroy", "ygrid_batch_add", "ygrid_batch_set_graph", "ygrid_batch_run",
+    "ygrid_batch_autotune", "ygrid_batch_info",
 ]
 
 
@@ -52,6 +54,12 @@ class Stats(ctypes.Structure):
                 ("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int),
                 ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int),
                 ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int), ("degrid_waves", ctypes.c_int),
+                ("degrid_pipeline", ctypes.c_int), ("item_pairs", ctypes.c_int), ("ms", ctypes.c_double),
+                ("trials", ctypes.c_int)]
 
 
 def library_path() -> str:
@@ -269,6 +277,50 @@ class Gridder:
     def close(self):
         if self.h:
             _check(lib().ygrid_gridder_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched passes over several gridders (ygrid_batch_*): chunks in device memory (torch
+    tensors); run() gives every gridder the result of grid() / degrid() on its chunk."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        self._keep = []
+        _check(lib().ygrid_batch_create(ctx.h, ctypes.byref(self.h)))
+
+    def add(self, gridder: Gridder, chunk: Chunk):
+        assert chunk.device, "batched passes need device chunks"
+        c = chunk.c()
+        self._keep.append((gridder, chunk))
+        _check(lib().ygrid_batch_add(self.h, gridder.h, ctypes.byref(c)))
+
+    def set_graph(self, on: bool = True):
+        _check(lib().ygrid_batch_set_graph(self.h, int(on)))
+
+    def run(self):
+        _check(lib().ygrid_batch_run(self.h))
+
+    def autotune(self, trials: int = 2) -> Tuning:
+        t = Tuning()
+        _check(lib().ygrid_batch_autotune(self.h, trials, ctypes.byref(t)))
+        return t
+
+    def info(self):
+        ms, caps = ctypes.c_double(), ctypes.c_int()
+        _check(lib().ygrid_batch_info(self.h, ctypes.byref(ms), ctypes.byref(caps)))
+        return ms.value, caps.value
+
+    def close(self):
+        if self.h:
+            _check(lib().ygrid_batch_destroy(self.h))
             self.h = ctypes.c_void_p()
 
     def __del__(self):

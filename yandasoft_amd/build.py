@@ -28,6 +28,7 @@ SOURCES = [
     "spheroidal.cpp",
     "gridders.cpp",
     "awproject.cpp",
+    "batch.cpp",
     "ygrid_api.cpp",
 ]
 

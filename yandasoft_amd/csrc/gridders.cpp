@@ -289,17 +289,19 @@ void Gridder::stage(const ygrid_chunk& a, ChunkDev& d, bool forward) {
 }
 
 // TableVisGridder::generic (TableVisGridder.cc:451-884)
-void Gridder::generic(const ygrid_chunk& a, bool forward) {
+// The part of TableVisGridder::generic before the prep kernel: stage the chunk, channel map,
+// MFS weights, CF generation, buffer sizes, and the prep kernel's arguments.  Returns false
+// when the pass has nothing to do (an empty model when degridding, no rows).
+bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
   YG_CHECK(initialised_, 2, "gridder not initialised");
   YG_CHECK(forward == degrid_mode_, 2,
            forward ? "degrid called on a gridder initialised for gridding" : "grid called on a degridder");
-  if (forward && model_empty_) return;
-  const auto t0 = std::chrono::steady_clock::now();
-  ChunkDev d;
+  if (forward && model_empty_) return false;
+  ChunkDev& d = ps.d;
   stage(a, d, forward);
   YG_CHECK(d.npol == shape_[2], 1,
            "only the identity polarisation conversion is supported: chunk npol must equal image npol");
-  if (d.nrow == 0) return;
+  if (d.nrow == 0) return false;
   const double r0 = d.freq_host[0] / kC;
   YG_CHECK(r0 > 0.1 && r0 < 30000, 4,
            "Check frequencies in the input file as the order of magnitude is likely to be wrong");
@@ -358,10 +360,9 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   if (vis_prec_ != kPrecF32) smp_coef16_.resize((size_t)nsamp * kCoef * sizeof(uint16_t));
   counters_.resize(8 * sizeof(unsigned long long));
   err_.resize(sizeof(int) * 2);
-  YG_HIP(hipMemsetAsync(counters_.get(), 0, 8 * sizeof(unsigned long long), ctx_->stream));
-  YG_HIP(hipMemsetAsync(err_.get(), 0, sizeof(int), ctx_->stream));
+  if (forward) acc_.resize((size_t)nsamp * 8 * sizeof(float));
 
-  PrepParams pp{};
+  PrepParams& pp = ps.pp;
   pp.nrow = d.nrow;
   pp.nchan = d.nchan;
   pp.npol = d.npol;
@@ -385,7 +386,7 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   pp.nzg = (int)((nz + zdiv - 1) / zdiv);
   pp.vis_prec = vis_prec_;
 
-  PrepInputs in{};
+  PrepInputs& in = ps.in;
   in.uvw = d.uvw;
   in.delay = d.delay;
   in.freq = d.freq;
@@ -400,28 +401,59 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   in.cf_ofs_u = cf_->ofs_u.empty() ? nullptr : cf_->d_ofs_u.as<int32_t>();
   in.cf_ofs_v = cf_->ofs_v.empty() ? nullptr : cf_->d_ofs_v.as<int32_t>();
   in.n_cf = (int)cf_->off.size();
-  PrepOutputs out{};
+  PrepOutputs& out = ps.out;
   const Samples smp = samples();
   out.smp = smp;
   out.sumw = forward ? nullptr : sumw_.as<double>();
   out.counters = counters_.as<unsigned long long>();
   out.error = err_.as<int>();
-  if (profiling_) YG_HIP(hipEventRecord(ev_[0], ctx_->stream));
-  launch_prep(*ctx_, pp, in, out);
-  if (profiling_) YG_HIP(hipEventRecord(ev_[1], ctx_->stream));
-  const CFView cfv = cf_view();
-  if (bin_samples(*ctx_, smp, d.nrow, d.nchan, shape_[0], shape_[1], pp.nzg * shape_[3], cfv)) ++n_bin_reused_;
-  const Binning& bins = ctx_->bins.bin;
+  ps.nsamp = nsamp;
+  ps.cfv = cf_view();
+  return true;
+}
+
+// Error flags and counters of a finished pass (TableVisGridder.cc:182-317 statistics).
+void Gridder::account_pass(bool forward, const unsigned long long cnt[4], int err, double seconds) {
+  YG_CHECK(!(err & 1), 4, "Weight is supposed to be a positive number (zero noise)");
+  YG_CHECK(!(err & 2), 4, "W scaling error: recommend allowing larger range of w");
+  YG_CHECK(!(err & 4), 4, "Index into convolution functions exceeds number of planes");
   if (forward) {
-    acc_.resize((size_t)nsamp * 8 * sizeof(float));
+    st_samples_d_ += (double)cnt[0];
+    st_taps_d_ += (double)cnt[1];
+    t_degrid_ += seconds;
+  } else {
+    st_samples_g_ += (double)cnt[0];
+    st_taps_g_ += (double)cnt[1];
+    st_flag_ += (double)cnt[2];
+    t_grid_ += seconds;
+  }
+  st_wflag_ += (double)cnt[3];
+}
+
+void Gridder::generic(const ygrid_chunk& a, bool forward) {
+  const auto t0 = std::chrono::steady_clock::now();
+  PassSetup ps;
+  if (!setup_pass(a, forward, ps)) return;
+  const ChunkDev& d = ps.d;
+  const int nsamp = ps.nsamp;
+  YG_HIP(hipMemsetAsync(counters_.get(), 0, 8 * sizeof(unsigned long long), ctx_->stream));
+  YG_HIP(hipMemsetAsync(err_.get(), 0, sizeof(int), ctx_->stream));
+  if (profiling_) YG_HIP(hipEventRecord(ev_[0], ctx_->stream));
+  launch_prep(*ctx_, ps.pp, ps.in, ps.out);
+  if (profiling_) YG_HIP(hipEventRecord(ev_[1], ctx_->stream));
+  const Samples smp = ps.out.smp;
+  const CFView& cfv = ps.cfv;
+  if (bin_samples(*ctx_, smp, d.nrow, d.nchan, shape_[0], shape_[1], ps.pp.nzg * shape_[3], cfv)) ++n_bin_reused_;
+  Binning& bins = ctx_->bins.bin;
+  if (forward) {
     YG_HIP(hipMemsetAsync(acc_.get(), 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg, acc_.as<float>());
+    launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, acc_.as<float>());
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
     launch_degrid_finish(*ctx_, smp, acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);
   } else {
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], pp.nzg);
+    launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
   }
   unsigned long long cnt[4] = {0, 0, 0, 0};
@@ -433,31 +465,16 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
                           ctx_->stream));
   }
   YG_HIP(hipStreamSynchronize(ctx_->stream));
-  YG_CHECK(!(err & 1), 4, "Weight is supposed to be a positive number (zero noise)");
-  YG_CHECK(!(err & 2), 4, "W scaling error: recommend allowing larger range of w");
-  YG_CHECK(!(err & 4), 4, "Index into convolution functions exceeds number of planes");
   if (profiling_) {
-    float a = 0, b = 0, k = 0;
-    YG_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
-    YG_HIP(hipEventElapsedTime(&b, ev_[1], ev_[2]));
-    YG_HIP(hipEventElapsedTime(&k, ev_[2], ev_[3]));
-    ms_prep_ += a;
-    ms_bin_ += b;
-    if (forward) { ms_dk_ += k; n_dk_ += 1; } else { ms_gk_ += k; n_gk_ += 1; }
+    float ta = 0, tb = 0, tk = 0;
+    YG_HIP(hipEventElapsedTime(&ta, ev_[0], ev_[1]));
+    YG_HIP(hipEventElapsedTime(&tb, ev_[1], ev_[2]));
+    YG_HIP(hipEventElapsedTime(&tk, ev_[2], ev_[3]));
+    ms_prep_ += ta;
+    ms_bin_ += tb;
+    if (forward) { ms_dk_ += tk; n_dk_ += 1; } else { ms_gk_ += tk; n_gk_ += 1; }
   }
-  if (forward) {
-    st_samples_d_ += (double)cnt[0];
-    st_taps_d_ += (double)cnt[1];
-    t_degrid_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  } else {
-    if (!dopcf_ || true) {
-      st_samples_g_ += (double)cnt[0];
-      st_taps_g_ += (double)cnt[1];
-    }
-    st_flag_ += (double)cnt[2];
-    t_grid_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  }
-  st_wflag_ += (double)cnt[3];
+  account_pass(forward, cnt, err, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
 }
 
 void Gridder::grid(const ygrid_chunk& a) { generic(a, false); }

@@ -106,6 +106,21 @@ class Gridder {
 
   void initialiseCommon(const ygrid_image_desc& img);
   void generic(const ygrid_chunk& acc, bool forward);
+  friend class Batch;
+  struct PassSetup {
+    ChunkDev d;
+    PrepParams pp{};
+    PrepInputs in{};
+    PrepOutputs out{};
+    CFView cfv{};
+    int nsamp = 0;
+  };
+  bool setup_pass(const ygrid_chunk& acc, bool forward, PassSetup& ps);
+  void account_pass(bool forward, const unsigned long long cnt[4], int err, double seconds);
+ public:
+  bool degrid_mode() const { return degrid_mode_; }
+  int npol() const { return shape_[2]; }
+ protected:
   void stage(const ygrid_chunk& acc, ChunkDev& d, bool forward);
   void uploadCorrection();
 

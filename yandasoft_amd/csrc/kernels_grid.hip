@@ -461,19 +461,36 @@ __global__ void k_item_count(const int* __restrict__ counts, int nkeys, int chun
   icount[k] = (k < nkeys) ? (counts[k] + chunk - 1) / chunk : 0;
 }
 
+// work items of one job (its keys' pair ranges in chunks of <= chunk pairs) written from
+// items[*base] on, tagged with the job index
 __global__ void k_item_emit(const int* __restrict__ counts, const int* __restrict__ offsets,
-                            const int* __restrict__ ioffsets, int nkeys, int chunk, int4* items) {
+                            const int* __restrict__ ioffsets, int nkeys, int chunk, const int* __restrict__ base,
+                            int job, int4* items) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nkeys) return;
   const int c = counts[k];
   if (c == 0) return;
   const int o = offsets[k];
-  const int io = ioffsets[k];
+  const int io = *base + ioffsets[k];
   const int nch = (c + chunk - 1) / chunk;
   for (int j = 0; j < nch; ++j) {
     const int b = o + j * chunk;
-    items[io + j] = make_int4(k, b, min(b + chunk, o + c), 0);
+    items[io + j] = make_int4(k, b, min(b + chunk, o + c), job);
   }
+}
+
+// *base_out = *base_in + (item count of the job: the last entry of its exclusive item scan)
+__global__ void k_item_base(const int* __restrict__ base_in, const int* __restrict__ iscan_end, int* base_out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *base_out = *base_in + *iscan_end;
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_copy_i32(const int* __restrict__ src, int* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 // --------------------------------------------------------------------------------------
@@ -658,12 +675,6 @@ __device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], c
       }
 }
 
-// Kernel argument describing the CF table: its address and size, and the pad (bytes, at least
-// 16 rows of the widest plane) by which the buffer resource starts before it.
-struct CFRef {
-  const void* ptr;
-  uint32_t bytes, pad;
-};
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(reinterpret_cast<const char*>(t.ptr)) - t.pad, (short)0,
@@ -728,14 +739,11 @@ __device__ __forceinline__ void for_each_pair(const int4* s_desc, const f32x4* s
 
 template <int NP, int WPE, bool PIPE, int CFP>
 __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
-                                              int* __restrict__ counter, const int4* __restrict__ descs, Samples smp,
-                                              CFRef cft, float* __restrict__ grid, int nx, int ny, int ntu,
-                                              int ntiles, int nzg) {
+                                              int* __restrict__ counter, const TileJob* __restrict__ jobs) {
   __shared__ int s_item;
   __shared__ int4 s_desc[kChunk];
   __shared__ f32x4 s_coef[kChunk * (kCoef / 4)];
   static_assert(4 * kFlushWave <= kChunk * kCoef, "flush areas must fit in the staging space");
-  const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(cft);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -756,10 +764,14 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     __syncthreads();
     if (item >= nitems) break;
     const int4 it = items[item];
-    const int tile = it.x % ntiles;
-    const int kz = it.x / ntiles;
-    const int ichan = kz / nzg;
-    const int u0 = (tile % ntu) * kTile, v0 = (tile / ntu) * kTile;
+    const TileJob& J = jobs[it.w];
+    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
+    const int nx = J.nx, ny = J.ny;
+    const int tile = it.x % J.ntiles;
+    const int kz = it.x / J.ntiles;
+    const int ichan = kz / J.nzg;
+    const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
+    float* __restrict__ grid = J.grid;
 
     f32x4 acc[4][2][2][NS];
 #pragma unroll
@@ -772,7 +784,7 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
           for (int m = 0; m < NS; ++m) acc[k][j][e][m] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int n = it.z - it.y;
-    stage_item<true>(descs, smp, it.y, n, s_desc, s_coef, tid, blockDim.x);
+    stage_item<true>(J.descs, J.smp, it.y, n, s_desc, s_coef, tid, blockDim.x);
     for_each_pair<true, PIPE, CFP>(s_desc, s_coef, cfr, n, w, lu, lv, o,
                               [&](const Taps& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
     // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of row 4k+lv, column
@@ -887,12 +899,9 @@ __device__ __forceinline__ void degrid_pair(float (&x)[8], const Taps& c, const 
 
 template <int NP, int WPE, bool PIPE, int CFP>
 __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ items, const int* __restrict__ nitems_p,
-                                                int* __restrict__ counter, const int4* __restrict__ descs,
-                                                CFRef cft, const float2* __restrict__ grid, int nx, int ny, int ntu,
-                                                int ntiles, int nzg, float* __restrict__ acc_out) {
+                                                int* __restrict__ counter, const TileJob* __restrict__ jobs) {
   __shared__ int s_item;
   __shared__ int4 s_desc[kChunk];
-  const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(cft);
   __shared__ float s_part[kDegridBatch][4][8];
   __shared__ int s_sid[kDegridBatch];
   const int tid = threadIdx.x;
@@ -908,11 +917,16 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     __syncthreads();
     if (item >= nitems) break;
     const int4 it = items[item];
-    const int tile = it.x % ntiles;
-    const int kz = it.x / ntiles;
-    const int ichan = kz / nzg;
-    const int u0 = (tile % ntu) * kTile, v0 = (tile / ntu) * kTile;
+    const TileJob& J = jobs[it.w];
+    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
+    const int nx = J.nx, ny = J.ny;
+    const int tile = it.x % J.ntiles;
+    const int kz = it.x / J.ntiles;
+    const int ichan = kz / J.nzg;
+    const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
     const size_t plane = (size_t)nx * ny;
+    const float2* __restrict__ grid = reinterpret_cast<const float2*>(J.grid);
+    float* __restrict__ acc_out = J.acc;
 
     float2 gt[4][2][2][NP];
 #pragma unroll
@@ -933,7 +947,7 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     // per pair: wave dot product, reduce-scatter, one LDS slot per (pair, wave, value); every
     // kDegridBatch pairs the four waves' partial sums meet and go to the per-sample accumulators
     const int n = it.z - it.y;
-    stage_item<false>(descs, Samples{}, it.y, n, s_desc, nullptr, tid, blockDim.x);
+    stage_item<false>(J.descs, Samples{}, it.y, n, s_desc, nullptr, tid, blockDim.x);
     for_each_pair<false, PIPE, CFP>(s_desc, nullptr, cfr, n, w, lu, lv, 0,
                            [&](const Taps& c, const PairGeom& g, const PairU& p, int i) {
                              float x[8];
@@ -1017,6 +1031,8 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
   b.cursor.resize((nkeys + 1) * sizeof(int));
   b.nitems.resize(sizeof(int) * 2);
   b.counter.resize(sizeof(int) * 2);
+  // nitems[1] = 0: the item base of this single-job binning
+  YG_HIP(hipMemsetAsync(b.nitems.get(), 0, 2 * sizeof(int), c.stream));
   int* counts = b.counts.as<int>();
   int* offsets = b.offsets.as<int>();
   YG_HIP(hipMemsetAsync(counts, 0, (nkeys + 1) * sizeof(int), c.stream));
@@ -1052,8 +1068,75 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
   const size_t max_items = (size_t)npairs / chunk + nkeys + 1;
   b.items.resize(max_items * sizeof(int4));
   hipLaunchKernelGGL(k_item_emit, dim3((nkeys + 255) / 256), dim3(256), 0, c.stream, counts, offsets,
-                     ioffsets.as<int>(), (int)nkeys, chunk, b.items.as<int4>());
+                     ioffsets.as<int>(), (int)nkeys, chunk, b.nitems.as<int>() + 1, 0, b.items.as<int4>());
   YG_HIP(hipMemcpyAsync(b.nitems.get(), ioffsets.as<int>() + nkeys, sizeof(int), hipMemcpyDeviceToDevice, c.stream));
+  YG_HIP(hipGetLastError());
+}
+
+void size_static_bins(const Ctx& c, StaticBins& sb, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
+                      int chunk) {
+  (void)c;
+  const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
+  const size_t nkeys = (size_t)ntu * ntv * (size_t)nkz;
+  YG_CHECK(nkeys < (size_t)INT_MAX, 1, "too many binning keys");
+  // a footprint of W pixels starting anywhere in a tile spans at most ceil((63 + W) / 64) tiles
+  const size_t W = 2 * (size_t)cf.max_support + 1;
+  const size_t span = (63 + W + kTile - 1) / kTile;
+  const size_t nsamp = (size_t)nrow * nchan;
+  const size_t cap = nsamp * span * span;
+  YG_CHECK(cap < (size_t)INT_MAX, 1, "too many (sample, tile) pairs for one pass");
+  sb.nkeys = nkeys;
+  sb.npairs_cap = cap;
+  sb.chunk = std::min(std::max(chunk, 16), kChunk);
+  sb.nitems_cap = cap / sb.chunk + std::min(nkeys, cap) + 1;
+  sb.nrow = nrow;
+  sb.nchan = nchan;
+  sb.nx = nx;
+  sb.ny = ny;
+  sb.nkz = nkz;
+  sb.counts.resize((nkeys + 1) * sizeof(int));
+  sb.offsets.resize((nkeys + 1) * sizeof(int));
+  sb.cursor.resize((nkeys + 1) * sizeof(int));
+  sb.icount.resize((nkeys + 1) * sizeof(int));
+  sb.ioffsets.resize((nkeys + 1) * sizeof(int));
+  sb.pairs.resize(std::max<size_t>(cap, 1) * sizeof(int4));
+  size_t t1 = 0, t2 = 0;
+  YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, sb.counts.as<int>(), sb.offsets.as<int>(), (int)(nkeys + 1),
+                                          c.stream));
+  YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, sb.icount.as<int>(), sb.ioffsets.as<int>(), (int)(nkeys + 1),
+                                          c.stream));
+  sb.temp_bytes = std::max(t1, t2);
+  sb.temp.resize(std::max<size_t>(sb.temp_bytes, 1));
+}
+
+void enqueue_static_binning(const Ctx& c, StaticBins& sb, const Samples& smp, const CFView& cf, int4* items_out,
+                            const int* base_in, int* base_out, int job) {
+  const int n = sb.nrow * sb.nchan;
+  const int ntu = (sb.nx + kTile - 1) / kTile, ntv = (sb.ny + kTile - 1) / kTile;
+  const int ntiles = ntu * ntv;
+  const int nk = (int)sb.nkeys;
+  int* counts = sb.counts.as<int>();
+  int* offsets = sb.offsets.as<int>();
+  launch_fill_u32(c, counts, (size_t)nk + 1, 0u);
+  if (n > 0)
+    hipLaunchKernelGGL(k_bin_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, smp, sb.nrow, sb.nchan, ntu, ntiles,
+                       counts);
+  size_t tb = sb.temp_bytes;
+  YG_HIP(hipcub::DeviceScan::ExclusiveSum(sb.temp.get(), tb, counts, offsets, nk + 1, c.stream));
+  hipLaunchKernelGGL(k_copy_i32, dim3(std::min((nk + 256) / 256, 4 * c.n_cu)), dim3(256), 0, c.stream, offsets,
+                     sb.cursor.as<int>(), (size_t)nk + 1);
+  if (n > 0)
+    hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, smp, sb.nrow, sb.nchan, ntu, ntiles,
+                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), sb.cursor.as<int>(),
+                       sb.pairs.as<int4>());
+  hipLaunchKernelGGL(k_item_count, dim3((nk + 1 + 255) / 256), dim3(256), 0, c.stream, counts, nk, sb.chunk,
+                     sb.icount.as<int>());
+  tb = sb.temp_bytes;
+  YG_HIP(hipcub::DeviceScan::ExclusiveSum(sb.temp.get(), tb, sb.icount.as<int>(), sb.ioffsets.as<int>(), nk + 1,
+                                          c.stream));
+  hipLaunchKernelGGL(k_item_emit, dim3((nk + 255) / 256), dim3(256), 0, c.stream, counts, offsets,
+                     sb.ioffsets.as<int>(), nk, sb.chunk, base_in, job, items_out);
+  hipLaunchKernelGGL(k_item_base, dim3(1), dim3(64), 0, c.stream, base_in, sb.ioffsets.as<int>() + nk, base_out);
   YG_HIP(hipGetLastError());
 }
 
@@ -1111,15 +1194,12 @@ static int persistent_blocks(const Ctx& c, const TileVariant& v) { return c.n_cu
 // storage precision
 template <int NP, int WPE, bool PIPE, int CFP>
 struct TileKernels {
-  static void grid(hipStream_t s, dim3 g, const Binning& b, const Samples& smp, CFRef cft, float* gf, int nx, int ny,
-                   int ntu, int ntiles, int nzg) {
-    hipLaunchKernelGGL((k_grid<NP, WPE, PIPE, CFP>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
-                       b.counter.as<int>(), b.pairs.as<int4>(), smp, cft, gf, nx, ny, ntu, ntiles, nzg);
-  }
-  static void degrid(hipStream_t s, dim3 g, const Binning& b, CFRef cft, const float2* grid, int nx, int ny, int ntu,
-                     int ntiles, int nzg, float* acc) {
-    hipLaunchKernelGGL((k_degrid<NP, WPE, PIPE, CFP>), g, dim3(256), 0, s, b.items.as<int4>(), b.nitems.as<int>(),
-                       b.counter.as<int>(), b.pairs.as<int4>(), cft, grid, nx, ny, ntu, ntiles, nzg, acc);
+  static void run(hipStream_t s, dim3 g, bool forward, const int4* items, const int* nitems, int* counter,
+                  const TileJob* jobs) {
+    if (forward)
+      hipLaunchKernelGGL((k_degrid<NP, WPE, PIPE, CFP>), g, dim3(256), 0, s, items, nitems, counter, jobs);
+    else
+      hipLaunchKernelGGL((k_grid<NP, WPE, PIPE, CFP>), g, dim3(256), 0, s, items, nitems, counter, jobs);
   }
 };
 
@@ -1169,30 +1249,64 @@ static void with_tile_kernels(const TileVariant& v, int npol, int prec, F&& f) {
   }
 }
 
-void launch_grid(const Ctx& c, const Binning& b, const Samples& smp, const CFView& cf, float2* grid, int nx, int ny,
-                 int npol, int nzg) {
-  const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
-  const int ntiles = ntu * ntv;
-  const CFRef cft = cf_ref(cf);
-  YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
-  const TileVariant v = cf.prec == kPrecF32 ? c.grid_variant : TileVariant{2, 1};
-  const dim3 g(persistent_blocks(c, v));
-  float* gf = reinterpret_cast<float*>(grid);
-  with_tile_kernels(v, npol, cf.prec,
-                    [&](auto k) { k.grid(c.stream, g, b, smp, cft, gf, nx, ny, ntu, ntiles, nzg); });
+TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, float* grid, float* acc, int nx, int ny,
+                      int npol, int nzg) {
+  TileJob j{};
+  j.descs = descs;
+  j.smp = smp;
+  j.cf = cf_ref(cf);
+  j.grid = grid;
+  j.acc = acc;
+  j.nx = nx;
+  j.ny = ny;
+  j.ntu = (nx + kTile - 1) / kTile;
+  j.ntiles = j.ntu * ((ny + kTile - 1) / kTile);
+  j.nzg = nzg;
+  j.npol = npol;
+  return j;
+}
+
+void launch_fill_u32(const Ctx& c, void* p, size_t n, uint32_t value) {
+  if (n == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)4 * c.n_cu);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, c.stream, (uint32_t*)p, n, value);
   YG_HIP(hipGetLastError());
 }
 
-void launch_degrid(const Ctx& c, const Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol,
-                   int nzg, float* acc) {
-  const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
-  const int ntiles = ntu * ntv;
-  const CFRef cft = cf_ref(cf);
-  YG_HIP(hipMemsetAsync(b.counter.get(), 0, sizeof(int), c.stream));
-  const TileVariant v = cf.prec == kPrecF32 ? c.degrid_variant : TileVariant{2, 1};
+void launch_tile_jobs(const Ctx& c, const TileVariant& v0, bool forward, int npol, int prec, const TileJob* jobs,
+                      const int4* items, const int* nitems, int* counter) {
+  const TileVariant v = prec == kPrecF32 ? v0 : TileVariant{2, 1};
+  launch_fill_u32(c, counter, 1, 0u);
   const dim3 g(persistent_blocks(c, v));
-  with_tile_kernels(v, npol, cf.prec,
-                    [&](auto k) { k.degrid(c.stream, g, b, cft, grid, nx, ny, ntu, ntiles, nzg, acc); });
+  with_tile_kernels(v, npol, prec, [&](auto k) { k.run(c.stream, g, forward, items, nitems, counter, jobs); });
+  YG_HIP(hipGetLastError());
+}
+
+void launch_grid(const Ctx& c, Binning& b, const Samples& smp, const CFView& cf, float2* grid, int nx, int ny,
+                 int npol, int nzg) {
+  b.job_host = make_tile_job(b.pairs.as<int4>(), smp, cf, reinterpret_cast<float*>(grid), nullptr, nx, ny, npol, nzg);
+  b.job.resize(sizeof(TileJob));
+  YG_HIP(hipMemcpyAsync(b.job.get(), &b.job_host, sizeof(TileJob), hipMemcpyHostToDevice, c.stream));
+  launch_tile_jobs(c, c.grid_variant, false, npol, cf.prec, b.job.as<TileJob>(), b.items.as<int4>(),
+                   b.nitems.as<int>(), b.counter.as<int>());
+}
+
+void launch_degrid(const Ctx& c, Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol, int nzg,
+                   float* acc) {
+  b.job_host = make_tile_job(b.pairs.as<int4>(), Samples{}, cf, const_cast<float*>(reinterpret_cast<const float*>(grid)),
+                             acc, nx, ny, npol, nzg);
+  b.job.resize(sizeof(TileJob));
+  YG_HIP(hipMemcpyAsync(b.job.get(), &b.job_host, sizeof(TileJob), hipMemcpyHostToDevice, c.stream));
+  launch_tile_jobs(c, c.degrid_variant, true, npol, cf.prec, b.job.as<TileJob>(), b.items.as<int4>(),
+                   b.nitems.as<int>(), b.counter.as<int>());
+}
+
+void launch_degrid_finish(const Ctx& c, const Samples& recs, const float* acc, int nrow, int nchan, int npol,
+                          float2* vis_out) {
+  const int n = nrow * nchan;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_degrid_finish, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, acc, nrow, nchan, npol,
+                     vis_out);
   YG_HIP(hipGetLastError());
 }
 
@@ -1209,15 +1323,6 @@ void launch_cf_round(const Ctx& c, const float2* src, uint32_t* dst, size_t n, i
   YG_CHECK(prec == kPrecF16 || prec == kPrecBF16, 1, "CF rounding needs a 16-bit precision");
   if (n == 0) return;
   hipLaunchKernelGGL(k_cf_round, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, src, dst, n, prec);
-  YG_HIP(hipGetLastError());
-}
-
-void launch_degrid_finish(const Ctx& c, const Samples& recs, const float* acc, int nrow, int nchan, int npol,
-                          float2* vis_out) {
-  const int n = nrow * nchan;
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_degrid_finish, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, acc, nrow, nchan, npol,
-                     vis_out);
   YG_HIP(hipGetLastError());
 }
 

@@ -9,6 +9,7 @@
 #include <memory>
 #include <string>
 
+#include "batch.h"
 #include "gridders.h"
 
 struct ygrid_ctx {
@@ -16,6 +17,9 @@ struct ygrid_ctx {
 };
 struct ygrid_gridder {
   std::unique_ptr<ygrid::Gridder> g;
+};
+struct ygrid_batch {
+  std::unique_ptr<ygrid::Batch> b;
 };
 
 namespace {
@@ -234,6 +238,66 @@ int ygrid_get_grid(const ygrid_gridder* g, float* out, size_t* n) {
   return guard([&] {
     NEED(g);
     g->g->getGrid(out, n);
+  });
+}
+
+int ygrid_batch_create(ygrid_ctx* ctx, ygrid_batch** out) {
+  return guard([&] {
+    NEED(ctx);
+    NEED(out);
+    YG_HIP(hipSetDevice(ctx->c->device));
+    *out = new ygrid_batch{std::make_unique<ygrid::Batch>(ctx->c)};
+  });
+}
+
+int ygrid_batch_destroy(ygrid_batch* b) {
+  return guard([&] { delete b; });
+}
+
+int ygrid_batch_add(ygrid_batch* b, ygrid_gridder* g, const ygrid_chunk* acc) {
+  return guard([&] {
+    NEED(b);
+    NEED(g);
+    NEED(acc);
+    b->b->add(g->g.get(), *acc);
+  });
+}
+
+int ygrid_batch_set_graph(ygrid_batch* b, int enable) {
+  return guard([&] {
+    NEED(b);
+    b->b->set_graph(enable != 0);
+  });
+}
+
+int ygrid_batch_run(ygrid_batch* b) {
+  return guard([&] {
+    NEED(b);
+    b->b->run();
+  });
+}
+
+int ygrid_batch_autotune(ygrid_batch* b, int trials, ygrid_tuning* best) {
+  return guard([&] {
+    NEED(b);
+    const ygrid::BatchTuning t = b->b->autotune(trials);
+    if (best) {
+      best->grid_waves = t.grid_waves;
+      best->grid_pipeline = t.grid_pipeline;
+      best->degrid_waves = t.degrid_waves;
+      best->degrid_pipeline = t.degrid_pipeline;
+      best->item_pairs = t.item_pairs;
+      best->ms = t.ms;
+      best->trials = t.trials;
+    }
+  });
+}
+
+int ygrid_batch_info(const ygrid_batch* b, double* last_ms, int* captures) {
+  return guard([&] {
+    NEED(b);
+    if (last_ms) *last_ms = b->b->last_ms();
+    if (captures) *captures = b->b->captures();
   });
 }
 

@@ -168,9 +168,31 @@ struct PrepInputs {
 };
 
 // Device work for one grid/degrid pass, produced by the binning step.
+// The CF buffer as a tile kernel addresses it: table address and size, and the pad (bytes, at
+// least 16 rows of the widest plane) by which the kernel's buffer resource starts before it.
+struct CFRef {
+  const void* ptr;
+  uint32_t bytes, pad;
+};
+
+// One grid of a tile-kernel launch: its pair descriptors, samples, CF buffer, grid (gridding:
+// accumulated; degridding: read) and, degridding, the per-sample accumulators.  Work items name
+// their job in int4::w.
+struct TileJob {
+  const int4* descs;
+  Samples smp;
+  CFRef cf;
+  float* grid;
+  float* acc;
+  int nx, ny, ntu, ntiles, nzg, npol;
+};
+
 struct Binning {
   DevBuf counts, offsets, cursor, ioffsets, pairs, items, nitems, counter, temp;
   size_t nkeys = 0, npairs_cap = 0;
+  // the one-job table of a single-grid launch, and its host copy (kept until the pass has synced)
+  DevBuf job;
+  TileJob job_host{};
 };
 
 // hipStream + device id of a context.
@@ -222,10 +244,34 @@ void launch_binning(const Ctx&, const Samples& smp, int nrow, int nchan, int nx,
 // Bins the samples into ctx.bins unless the cached binning was built from identical sample
 // headers and keys with the same parameters; returns true when the cache was reused.
 bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkeys_z, const CFView& cf);
-void launch_grid(const Ctx&, const Binning& b, const Samples& smp, const CFView& cf, float2* grid, int nx, int ny,
+void launch_grid(const Ctx&, Binning& b, const Samples& smp, const CFView& cf, float2* grid, int nx, int ny,
                  int npol, int nzg);
-void launch_degrid(const Ctx&, const Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol,
+void launch_degrid(const Ctx&, Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol,
                    int nzg, float* acc);
+// The job of one grid for the tile kernels (descs: its binning's pair descriptors).
+TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, float* grid, float* acc, int nx, int ny,
+                      int npol, int nzg);
+// One persistent tile-kernel launch over the work items of several jobs (all with npol
+// polarisations and CF precision prec): items[0 .. *nitems) in device memory, counter zeroed
+// by the launch's own fill kernel.  forward: degridding.
+void launch_tile_jobs(const Ctx&, const TileVariant& v, bool forward, int npol, int prec, const TileJob* jobs,
+                      const int4* items, const int* nitems, int* counter);
+
+// Sync-free binning of one pass into buffers sized by upper bounds (for batches and graph
+// capture): no host readback, no allocation, no memset nodes.  The pass's work items go to
+// items_out[*base_in ...) with job index `job`, and *base_out = *base_in + its item count.
+struct StaticBins {
+  DevBuf counts, offsets, cursor, icount, ioffsets, pairs, temp;
+  size_t nkeys = 0, npairs_cap = 0, nitems_cap = 0, temp_bytes = 0;
+  int nrow = 0, nchan = 0, nx = 0, ny = 0, nkz = 0, chunk = 0;
+};
+// Sizes sb for nrow x nchan samples of support <= cf.max_support (allocates).
+void size_static_bins(const Ctx&, StaticBins& sb, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
+                      int chunk);
+void enqueue_static_binning(const Ctx&, StaticBins& sb, const Samples& smp, const CFView& cf, int4* items_out,
+                            const int* base_in, int* base_out, int job);
+// fills n 32-bit words with value (a kernel: usable inside captured graphs)
+void launch_fill_u32(const Ctx&, void* p, size_t n, uint32_t value);
 // Rounds the FP32 CF table (n complex taps) to prec (FP16 or BF16), packed (re, im) in 32 bits.
 void launch_cf_round(const Ctx&, const float2* src, uint32_t* dst, size_t n, int prec);
 void launch_degrid_finish(const Ctx&, const Samples& smp, const float* acc, int nrow, int nchan, int npol,
