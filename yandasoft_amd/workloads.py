@@ -8,6 +8,9 @@ askap/simulation/Simulator.cc:932-963).
                         disc, w = 0 (the tConvolve / tGridding style case).
 * ``wproject_array`` -- W-projection: 36 antennas uniform in a disc, earth rotation over the
                         given hour-angle span at a declination, linear channels, 4 pols.
+* ``batched_beams``  -- many W-projection grids at once: 36 beams with uneven channel counts
+                        (8..64), visibility counts (log-uniform 1e4..2e6 per polarisation) and
+                        grid sizes (1024^2, 2048^2, 4096^2), rows of the 36-antenna array.
 * ``awproject_feeds`` -- A-projection: the ``sphfunc_disc`` uv generator at 32 channels around
                         1.4 GHz, 9 feeds pointed on a 3x3 grid of 1 degree spacing, a 12 m
                         uniform disk illumination.
@@ -143,3 +146,33 @@ def awproject_feeds(nvis: int = 160_000, nchan: int = 32, fcentre: float = 1.4e9
               "cutoff": cutoff, "diameter": diameter, "blockage": "0m", "frequencydependent": True}
     return Workload("awproject_feeds", uvw, freq, vis, noise, n, n, cell_arcsec * ARCSEC, "GPUAWProject", params,
                     feed1=feed1, pointing1=np.ascontiguousarray(pointing), centre=centre, pa=pa)
+
+
+def batched_beams(nbeam: int = 36, chan_range=(8, 64), nvis_range=(1e4, 2e6), sizes=(1024, 2048, 4096),
+                  npol: int = 4, fmin: float = 700e6, fmax: float = 1000e6, nwplanes: int = 129, oversample: int = 4,
+                  limitsupport: int = 64, maxsupport: int = 2048, cutoff: float = 1e-3, seed: int = 20261204):
+    """One W-projection Workload per beam.  Beam b has nchan_b channels drawn uniformly from
+    chan_range over fmin..fmax, nvis_b (row x channel samples per polarisation) log-uniform in
+    nvis_range, and a grid of sizes[k] pixels with the cell scaled so that the array's longest
+    baseline keeps the same fraction of the grid (4.5 arcsec at 4096).  Rows are integrations of
+    the 36-antenna array (earth rotation over 8 h at dec -45 deg) drawn at random; w-planes span
+    the |w| of each beam's rows."""
+    rng = np.random.default_rng(seed)
+    enu = array_layout(seed=20261203)
+    base = earth_rotation_uvw(enu, 300, 8.0, -45.0)
+    out = []
+    for b in range(nbeam):
+        nchan = int(rng.integers(chan_range[0], chan_range[1] + 1))
+        nvis = float(np.exp(rng.uniform(math.log(nvis_range[0]), math.log(nvis_range[1]))))
+        nrow = max(1, int(round(nvis / nchan)))
+        n = int(sizes[int(rng.integers(0, len(sizes)))])
+        rows = rng.integers(0, base.shape[0], nrow)
+        uvw = np.ascontiguousarray(base[rows])
+        freq = np.linspace(fmin, fmax, nchan)
+        vis = _cn(rng, (npol, nchan, nrow))
+        noise = np.ones((npol, nchan, nrow), dtype=np.complex64)
+        wmax = float(np.max(np.abs(uvw[:, 2]))) * fmax / C_LIGHT * 1.0001
+        params = {"wmax": wmax, "nwplanes": nwplanes, "oversample": oversample, "maxsupport": maxsupport,
+                  "limitsupport": limitsupport, "cutoff": cutoff}
+        out.append(Workload(f"beam{b}", uvw, freq, vis, noise, n, n, 4.5 * ARCSEC * 4096 / n, "GPUWProject", params))
+    return out
