@@ -92,6 +92,8 @@ typedef struct {
   /* storage precision of the CF taps and of the gridding coefficients (parset keys cfprecision,
    * visprecision): 0 FP32, 1 FP16, 2 BF16; accumulation is FP32 in every case */
   int cf_precision, vis_precision;
+  /* 1 when the gridder's CFs come from an attached learned network (ygrid_gridder_set_learned_cf) */
+  int learned_cf_active;
 } ygrid_stats;
 
 /* ---- library / context ---- */
@@ -180,6 +182,47 @@ int ygrid_batch_run(ygrid_batch *b);
 int ygrid_batch_autotune(ygrid_batch *b, int trials, ygrid_tuning *best);
 /* wall time of the last run (ms) and number of graph captures so far */
 int ygrid_batch_info(const ygrid_batch *b, double *last_ms, int *captures);
+
+/* ---- learned convolution functions: no counterpart in the reference (its CFs are FFT tables
+ * on a fixed set of w planes, WProjectVisGridder.cc:435-522).  For callers that want CFs at each
+ * sample's exact w instead of the nearest plane.
+ * A coordinate network (dense layers of `hidden` units with SiLU, `layers` of them, complex output)
+ * maps (w, fracu, fracv, dx, dy) to a CF tap.  It is trained on the GPU on a W-projection
+ * gridder's own CF table; every holdout_every-th w plane is left out of training and the table
+ * judges the network's error there.  A W-projection gridder with an attached network gives every
+ * sample its own CF plane from the network at the sample's w (FP16 weights, FP32 accumulation),
+ * as long as that held-out error is <= max_err; otherwise, and when detached, its table. */
+typedef struct ygrid_lcf ygrid_lcf;
+typedef struct {
+  int epochs;          /* passes over the training taps */
+  int batch;           /* examples per step */
+  float lr;            /* peak Adam learning rate (cosine decay to lr/20) */
+  unsigned long long seed;
+  int holdout_every;   /* every n-th w plane is held out */
+} ygrid_lcf_train_opts;
+typedef struct {
+  int epochs, steps, examples, cfs, holdout_cfs;
+  double first_loss, last_loss;                /* mean over the epoch, targets scaled by 1/peak */
+  double holdout_max_err, holdout_rms_err;     /* |network - table| / max|table| on held-out planes */
+  double train_s;
+  int support;                                 /* support of the network's planes */
+} ygrid_lcf_report;
+int ygrid_lcf_create(ygrid_ctx *ctx, int hidden, int layers, unsigned long long seed, ygrid_lcf **out);
+int ygrid_lcf_destroy(ygrid_lcf *l);
+int ygrid_lcf_num_params(const ygrid_lcf *l, int *n);
+/* parameters (host, FP32): per layer the weights (inputs x outputs, row-major) then the biases */
+int ygrid_lcf_get_params(const ygrid_lcf *l, float *out);
+int ygrid_lcf_set_params(ygrid_lcf *l, const float *in);
+/* one forward and backward pass over n examples x (n x 5) with targets t (n x 2), host arrays:
+ * loss = sum (y - t)^2 / (2n) and its gradient (num_params floats) */
+int ygrid_lcf_loss_grad(ygrid_lcf *l, const float *x, const float *t, int n, double *loss, float *grad);
+/* inference of n rows (host arrays); half: FP16 weights */
+int ygrid_lcf_forward(ygrid_lcf *l, const float *x, float *y, int n, int half);
+/* trains on the CF table of a W-projection gridder (which must have run a pass) */
+int ygrid_lcf_train_wproject(ygrid_lcf *l, const ygrid_gridder *wproject, const ygrid_lcf_train_opts *opts,
+                             ygrid_lcf_report *report);
+/* attach (lcf non-null) or detach a network; max_err bounds its held-out error */
+int ygrid_gridder_set_learned_cf(ygrid_gridder *g, ygrid_lcf *lcf, double max_err);
 
 #ifdef __cplusplus
 }

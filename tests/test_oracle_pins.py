@@ -353,3 +353,33 @@ def test_degrid_offgrid_sample_reference_adds_stale_value():
     d.initialise_degrid((N, N, 1, 1), (cell, cell), model)
     v = d.degrid(O.Chunk(uvw, f, np.zeros((2, 1, 1), np.complex64)))
     assert abs(v[0, 0, 0] - 1.0) < 1e-9 and abs(v[1, 0, 0] - v[0, 0, 0]) < 1e-12
+
+
+# ---------------------------------------------------------------------------------------
+# learned-CF network oracle: analytic gradient vs central differences, SiLU' closed form
+# ---------------------------------------------------------------------------------------
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp as M
+    rng = np.random.default_rng(3)
+    hidden, layers = 6, 2
+    n = sum(a * b + b for a, b in zip(M.dims(hidden, layers)[:-1], M.dims(hidden, layers)[1:]))
+    p = rng.normal(0, 0.5, n)
+    x = rng.uniform(-1, 1, (7, 5))
+    t = rng.normal(0, 1, (7, 2))
+    loss, g = M.loss_grad(p, x, t, hidden, layers)
+    assert loss == pytest.approx(np.sum((M.forward(p, x, hidden, layers) - t) ** 2) / 14, rel=1e-14)
+    h = 1e-6
+    fd = np.zeros(n)
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = h
+        fd[i] = (M.loss_grad(p + e, x, t, hidden, layers)[0] - M.loss_grad(p - e, x, t, hidden, layers)[0]) / (2 * h)
+    assert np.max(np.abs(fd - g)) <= 1e-7 * max(1.0, np.max(np.abs(g)))
+
+
+def test_mlp_oracle_silu_derivative():
+    from oracle import mlp as M
+    z = np.linspace(-8, 8, 101)
+    h = 1e-6
+    assert np.max(np.abs((M.silu(z + h) - M.silu(z - h)) / (2 * h) - M.silu_grad(z))) < 1e-8
+    assert M.silu(0.0) == 0.0 and M.silu_grad(0.0) == 0.5

@@ -14,6 +14,7 @@ from .abi import (  # noqa: F401
     Context,
     Gridder,
     ImageDesc,
+    LearnedCF,
     YgridError,
     lib,
     library_path,

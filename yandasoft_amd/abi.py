@@ -19,6 +19,8 @@ EXPORTED = [
     "ygrid_set_mfs_weight", "ygrid_get_stats", "ygrid_set_profiling", "ygrid_get_cf", "ygrid_get_grid",
     "ygrid_batch_create", "ygrid_batch_destroy", "ygrid_batch_add", "ygrid_batch_set_graph", "ygrid_batch_run",
     "ygrid_batch_autotune", "ygrid_batch_info",
+    "ygrid_lcf_create", "ygrid_lcf_destroy", "ygrid_lcf_num_params", "ygrid_lcf_get_params", "ygrid_lcf_set_params",
+    "ygrid_lcf_loss_grad", "ygrid_lcf_forward", "ygrid_lcf_train_wproject", "ygrid_gridder_set_learned_cf",
 ]
 
 
@@ -53,13 +55,25 @@ class Stats(ctypes.Structure):
                 ("n_degrid_launches", ctypes.c_double), ("n_bin_reused", ctypes.c_double),
                 ("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int),
                 ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int),
-                ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int)]
+                ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int), ("learned_cf_active", ctypes.c_int)]
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int), ("degrid_waves", ctypes.c_int),
                 ("degrid_pipeline", ctypes.c_int), ("item_pairs", ctypes.c_int), ("ms", ctypes.c_double),
                 ("trials", ctypes.c_int)]
+
+
+class LcfTrainOpts(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int), ("batch", ctypes.c_int), ("lr", ctypes.c_float),
+                ("seed", ctypes.c_ulonglong), ("holdout_every", ctypes.c_int)]
+
+
+class LcfReport(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int), ("steps", ctypes.c_int), ("examples", ctypes.c_int), ("cfs", ctypes.c_int),
+                ("holdout_cfs", ctypes.c_int), ("first_loss", ctypes.c_double), ("last_loss", ctypes.c_double),
+                ("holdout_max_err", ctypes.c_double), ("holdout_rms_err", ctypes.c_double),
+                ("train_s", ctypes.c_double), ("support", ctypes.c_int)]
 
 
 def library_path() -> str:
@@ -251,6 +265,12 @@ class Gridder:
     def set_mfs_weight(self, ref_freq: float, order: int):
         _check(lib().ygrid_set_mfs_weight(self.h, ctypes.c_double(ref_freq), order))
 
+    def set_learned_cf(self, lcf, max_err: float):
+        """Attach a trained LearnedCF (used while its held-out error <= max_err) or detach (None)."""
+        _check(lib().ygrid_gridder_set_learned_cf(self.h, lcf.h if lcf is not None else None,
+                                                  ctypes.c_double(max_err)))
+        self._lcf = lcf
+
     def set_profiling(self, on: bool = True):
         _check(lib().ygrid_set_profiling(self.h, int(on)))
 
@@ -321,6 +341,65 @@ class Batch:
     def close(self):
         if self.h:
             _check(lib().ygrid_batch_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LearnedCF:
+    """Coordinate network for CF taps (ygrid_lcf_*)."""
+
+    def __init__(self, ctx: Context, hidden: int = 128, layers: int = 4, seed: int = 1):
+        self.ctx = ctx
+        self.hidden, self.layers = hidden, layers
+        self.h = ctypes.c_void_p()
+        _check(lib().ygrid_lcf_create(ctx.h, hidden, layers, ctypes.c_ulonglong(seed), ctypes.byref(self.h)))
+
+    @property
+    def nparams(self) -> int:
+        n = ctypes.c_int()
+        _check(lib().ygrid_lcf_num_params(self.h, ctypes.byref(n)))
+        return n.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().ygrid_lcf_get_params(self.h, ctypes.c_void_p(out.ctypes.data)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        _check(lib().ygrid_lcf_set_params(self.h, ctypes.c_void_p(p.ctypes.data)))
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_double()
+        _check(lib().ygrid_lcf_loss_grad(self.h, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(t.ctypes.data),
+                                         x.shape[0], ctypes.byref(loss), ctypes.c_void_p(g.ctypes.data)))
+        return loss.value, g
+
+    def forward(self, x, half: bool = False) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], 2), np.float32)
+        _check(lib().ygrid_lcf_forward(self.h, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(y.ctypes.data),
+                                       x.shape[0], int(half)))
+        return y
+
+    def train_wproject(self, gridder: Gridder, epochs=20, batch=4096, lr=2e-3, seed=1, holdout_every=8) -> LcfReport:
+        o = LcfTrainOpts(epochs, batch, lr, seed, holdout_every)
+        r = LcfReport()
+        _check(lib().ygrid_lcf_train_wproject(self.h, gridder.h, ctypes.byref(o), ctypes.byref(r)))
+        return r
+
+    def close(self):
+        if self.h:
+            _check(lib().ygrid_lcf_destroy(self.h))
             self.h = ctypes.c_void_p()
 
     def __del__(self):

@@ -25,10 +25,12 @@ SOURCES = [
     "kernels_fft.hip",
     "kernels_cf.hip",
     "kernels_awp.hip",
+    "kernels_mlp.hip",
     "spheroidal.cpp",
     "gridders.cpp",
     "awproject.cpp",
     "batch.cpp",
+    "learned_cf.cpp",
     "ygrid_api.cpp",
 ]
 

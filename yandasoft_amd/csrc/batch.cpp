@@ -107,6 +107,7 @@ void Batch::enqueue() {
     launch_fill_u32(c, g.err_.get(), 1, 0u);
     if (p.forward) launch_fill_u32(c, g.acc_.get(), (size_t)p.ps.nsamp * 8, 0u);
     launch_prep(c, p.ps.pp, p.ps.in, p.ps.out);
+    g.afterPrep(p.ps);
     int* base = gr.base.as<int>();
     enqueue_static_binning(c, p.sb, p.ps.out.smp, p.ps.cfv, gr.items.as<int4>(), base + p.job, base + p.job + 1,
                            p.job);

@@ -409,6 +409,7 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
   out.error = err_.as<int>();
   ps.nsamp = nsamp;
   ps.cfv = cf_view();
+  customizePass(ps);
   return true;
 }
 
@@ -440,6 +441,7 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   YG_HIP(hipMemsetAsync(err_.get(), 0, sizeof(int), ctx_->stream));
   if (profiling_) YG_HIP(hipEventRecord(ev_[0], ctx_->stream));
   launch_prep(*ctx_, ps.pp, ps.in, ps.out);
+  afterPrep(ps);
   if (profiling_) YG_HIP(hipEventRecord(ev_[1], ctx_->stream));
   const Samples smp = ps.out.smp;
   const CFView& cfv = ps.cfv;
@@ -600,6 +602,7 @@ void Gridder::stats(ygrid_stats* o) const {
   o->degrid_pipeline = ctx_->degrid_variant.pipeline;
   o->cf_precision = cf_prec_;
   o->vis_precision = vis_prec_;
+  o->learned_cf_active = learnedActive() ? 1 : 0;
 }
 
 void Gridder::getCF(int k, float* out, int* support) const {
@@ -794,6 +797,19 @@ class WProjectGridder : public Gridder {
   }
   std::unique_ptr<Gridder> clone() const override { return std::unique_ptr<Gridder>(new WProjectGridder(*this)); }
   std::string name() const override { return "GPUWProject"; }
+  bool wTerms(std::vector<double>& w, double& wmax) const override {
+    w.resize(ws_.nw);
+    for (int z = 0; z < ws_.nw; ++z) w[z] = ws_.wterm(z);
+    wmax = ws_.wmax;
+    return true;
+  }
+  void setLearnedCF(std::shared_ptr<LearnedCF> lcf, double max_err) override {
+    lcf_ = std::move(lcf);
+    lcf_max_err_ = max_err;
+  }
+  bool learnedActive() const override {
+    return lcf_ && lcf_->trained && lcf_->holdout_max_err <= lcf_max_err_;
+  }
 
  protected:
   int oversampling() const override { return os_; }
@@ -816,7 +832,49 @@ class WProjectGridder : public Gridder {
   }
   void initConvolutionFunction(const ChunkDev&) override;
 
+  // Learned CFs: every sample its own padded plane of the network's support, at the gridder's
+  // CF storage precision, filled by the network after the prep kernel.
+  void customizePass(PassSetup& ps) override {
+    if (!learnedActive()) return;
+    const int S = lcf_->support;
+    const int P = 2 * S + 1 + 2 * kGuardU, R = 2 * S + 1 + 2 * kGuardV;
+    const long long plane = (long long)R * P;
+    const size_t bytes = (size_t)ps.nsamp * plane * cf_elem_bytes(cf_prec_);
+    YG_CHECK(bytes < ((size_t)1 << 31) - ((size_t)1 << 24), 1,
+             "learned CF planes of this chunk exceed 2 GiB: use fewer samples per chunk or cfprecision fp16");
+    if (lcf_buf_.table.size() < bytes || lcf_buf_.S != S || lcf_buf_.prec != cf_prec_) {
+      if (lcf_buf_.table.size() < bytes) lcf_buf_.table.release();
+      lcf_buf_.table.resize(bytes);
+      YG_HIP(hipMemsetAsync(lcf_buf_.table.get(), 0, lcf_buf_.table.size(), ctx_->stream));  // guard bands
+      lcf_buf_.S = S;
+      lcf_buf_.prec = cf_prec_;
+    }
+    lcf_buf_.in.resize((size_t)ps.nsamp * sizeof(float4));
+    lcf_->reserve_eval(S);
+    ps.pp.lcf_S = S;
+    ps.pp.lcf_plane_elems = plane;
+    ps.out.lcf_in = lcf_buf_.in.as<float4>();
+    ps.cfv = CFView{lcf_buf_.table.get(), lcf_buf_.table.size(), S, cf_prec_};
+  }
+  void afterPrep(PassSetup& ps) override {
+    if (!ps.out.lcf_in) return;
+    const int S = ps.pp.lcf_S;
+    lcf_->eval_samples(ps.out.lcf_in, ps.nsamp, lcf_buf_.table.get(), 2 * S + 1 + 2 * kGuardU, ps.pp.lcf_plane_elems,
+                       ps.cfv.prec);
+  }
+
  private:
+  // per-gridder buffers of the learned planes (clones start without them)
+  struct LcfBuffers {
+    DevBuf table, in;
+    int S = -1, prec = -1;
+    LcfBuffers() = default;
+    LcfBuffers(const LcfBuffers&) {}
+    LcfBuffers& operator=(const LcfBuffers&) { return *this; }
+  };
+  std::shared_ptr<LearnedCF> lcf_;
+  double lcf_max_err_ = 0.0;
+  LcfBuffers lcf_buf_;
   WSamplingDesc ws_;
   double cutoff_ = 1e-3, alpha_ = 1.0;
   int os_ = 8, maxsupport_ = 256, limitsupport_ = 0;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/ygrid.h"
+#include "learned_cf.h"
 #include "ygrid_internal.h"
 
 namespace ygrid {
@@ -87,6 +88,21 @@ class Gridder {
   void stats(ygrid_stats* out) const;
   void getCF(int k, float* out, int* support) const;
   void getGrid(float* out, size_t* n) const;
+  const CFTable* cfTable() const { return cf_.get(); }
+  // W-projection: the w (wavelengths) of every CF plane and wmax; false for other gridders
+  virtual bool wTerms(std::vector<double>& w, double& wmax) const {
+    (void)w;
+    (void)wmax;
+    return false;
+  }
+  // Learned CFs (W-projection only): the network gives every sample its CF at its exact w while
+  // its held-out error is <= max_err; the table otherwise (and with lcf null).
+  virtual void setLearnedCF(std::shared_ptr<LearnedCF> lcf, double max_err) {
+    (void)lcf;
+    (void)max_err;
+    throw Error(1, "learned CFs need a W-projection gridder");
+  }
+  virtual bool learnedActive() const { return false; }
 
  protected:
   Gridder(const Gridder& o);
@@ -117,6 +133,10 @@ class Gridder {
   };
   bool setup_pass(const ygrid_chunk& acc, bool forward, PassSetup& ps);
   void account_pass(bool forward, const unsigned long long cnt[4], int err, double seconds);
+  // hooks of a pass: after its setup (may replace the CF view and prep arguments), and after the
+  // prep kernel (device work only: also runs inside captured graphs)
+  virtual void customizePass(PassSetup& ps) { (void)ps; }
+  virtual void afterPrep(PassSetup& ps) { (void)ps; }
  public:
   bool degrid_mode() const { return degrid_mode_; }
   int npol() const { return shape_[2]; }

@@ -128,6 +128,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
   bool flagged = false, wflagged = false;
   int swrow = 0, ichan = 0;
   float wts[4] = {0.f, 0.f, 0.f, 0.f};
+  float4 lcf = make_float4(0.f, 0.f, 0.f, 0.f);
   unsigned long long taps = 0;
 
   if (valid) {
@@ -180,16 +181,18 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
         const int cInd = fracu + p.oversample * (fracv + p.oversample * plane);
         if (cInd >= in.n_cf) { err |= 4; valid = false; }
         else {
-          const int S = in.cf_supp[cInd];
-          const int iuo = iu + (in.cf_ofs_u ? in.cf_ofs_u[plane] : 0);
-          const int ivo = iv + (in.cf_ofs_v ? in.cf_ofs_v[plane] : 0);
+          const bool learned = p.lcf_plane_elems > 0;
+          const int S = learned ? p.lcf_S : in.cf_supp[cInd];
+          const int iuo = iu + (!learned && in.cf_ofs_u ? in.cf_ofs_u[plane] : 0);
+          const int ivo = iv + (!learned && in.cf_ofs_v ? in.cf_ofs_v[plane] : 0);
           // strict on-grid test (TableVisGridder.cc:763-764)
           if (!((iuo - S) > 0 && (ivo - S) > 0 && (iuo + S) < p.nx && (ivo + S) < p.ny)) {
             valid = false;
           } else {
             rec.iu0 = iuo - S;
             rec.iv0 = ivo - S;
-            rec.cfoff = in.cf_off[cInd];
+            rec.cfoff = learned ? (int32_t)((long long)s * p.lcf_plane_elems) : in.cf_off[cInd];
+            if (learned) lcf = make_float4((float)(in.uvw[3 * row + 2] / kC * f), (float)fracu, (float)fracv, 1.f);
             rec.spack = S | ((2 * S + 1 + 2 * kGuardU) << 16);
             key = ichan * p.nzg + plane / p.zdiv;
             swrow = p.track_per_os ? cInd : plane;
@@ -247,6 +250,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
   if (inb) {
     out.smp.hdr[s] = rec;
     out.smp.key[s] = key;
+    if (out.lcf_in) out.lcf_in[s] = valid ? lcf : make_float4(0.f, 0.f, 0.f, 0.f);
     if (p.vis_prec != kPrecF32 && p.mode != kModeDegrid) {
       // gridding coefficients stored at 16-bit precision: 32 bytes per sample
       uint4* c4 = reinterpret_cast<uint4*>(out.smp.coef16 + (size_t)s * kCoef);

@@ -21,6 +21,9 @@ struct ygrid_gridder {
 struct ygrid_batch {
   std::unique_ptr<ygrid::Batch> b;
 };
+struct ygrid_lcf {
+  std::shared_ptr<ygrid::LearnedCF> l;
+};
 
 namespace {
 thread_local std::string g_last_error;
@@ -290,6 +293,100 @@ int ygrid_batch_autotune(ygrid_batch* b, int trials, ygrid_tuning* best) {
       best->ms = t.ms;
       best->trials = t.trials;
     }
+  });
+}
+
+int ygrid_lcf_create(ygrid_ctx* ctx, int hidden, int layers, unsigned long long seed, ygrid_lcf** out) {
+  return guard([&] {
+    NEED(ctx);
+    NEED(out);
+    YG_HIP(hipSetDevice(ctx->c->device));
+    *out = new ygrid_lcf{std::make_shared<ygrid::LearnedCF>(ctx->c, hidden, layers, seed)};
+  });
+}
+
+int ygrid_lcf_destroy(ygrid_lcf* l) {
+  return guard([&] { delete l; });
+}
+
+int ygrid_lcf_num_params(const ygrid_lcf* l, int* n) {
+  return guard([&] {
+    NEED(l);
+    NEED(n);
+    *n = l->l->nparams();
+  });
+}
+
+int ygrid_lcf_get_params(const ygrid_lcf* l, float* out) {
+  return guard([&] {
+    NEED(l);
+    NEED(out);
+    l->l->get_params(out);
+  });
+}
+
+int ygrid_lcf_set_params(ygrid_lcf* l, const float* in) {
+  return guard([&] {
+    NEED(l);
+    NEED(in);
+    l->l->set_params(in);
+  });
+}
+
+int ygrid_lcf_loss_grad(ygrid_lcf* l, const float* x, const float* t, int n, double* loss, float* grad) {
+  return guard([&] {
+    NEED(l);
+    NEED(x);
+    NEED(t);
+    const double v = l->l->loss_and_grads_host(x, t, n, grad);
+    if (loss) *loss = v;
+  });
+}
+
+int ygrid_lcf_forward(ygrid_lcf* l, const float* x, float* y, int n, int half) {
+  return guard([&] {
+    NEED(l);
+    NEED(x);
+    NEED(y);
+    l->l->forward_host(x, y, n, half != 0);
+  });
+}
+
+int ygrid_lcf_train_wproject(ygrid_lcf* l, const ygrid_gridder* wp, const ygrid_lcf_train_opts* opts,
+                             ygrid_lcf_report* report) {
+  return guard([&] {
+    NEED(l);
+    NEED(wp);
+    ygrid::LcfTrainOptions o;
+    if (opts) {
+      o.epochs = opts->epochs;
+      o.batch = opts->batch;
+      o.lr = opts->lr;
+      o.seed = opts->seed;
+      o.holdout_every = opts->holdout_every;
+    }
+    YG_CHECK(o.epochs > 0 && o.batch > 0 && o.lr > 0.f, YGRID_ERR_ARG, "bad training options");
+    const ygrid::LcfReport r = l->l->train_wproject(*wp->g, o);
+    if (report) {
+      report->epochs = r.epochs;
+      report->steps = r.steps;
+      report->examples = r.examples;
+      report->cfs = r.cfs;
+      report->holdout_cfs = r.holdout_cfs;
+      report->first_loss = r.first_loss;
+      report->last_loss = r.last_loss;
+      report->holdout_max_err = r.holdout_max_err;
+      report->holdout_rms_err = r.holdout_rms_err;
+      report->train_s = r.train_s;
+      report->support = l->l->support;
+    }
+  });
+}
+
+int ygrid_gridder_set_learned_cf(ygrid_gridder* g, ygrid_lcf* lcf, double max_err) {
+  return guard([&] {
+    NEED(g);
+    g->g->setLearnedCF(lcf ? lcf->l : nullptr, max_err);
   });
 }
 

@@ -141,6 +141,10 @@ struct PrepParams {
   int zdiv;                     // z-group divisor for binning keys
   int nzg;                      // number of z groups
   int vis_prec;                 // Prec of the stored gridding coefficients
+  // learned CFs: every sample gets its own padded plane of support lcf_S at s * lcf_plane_elems
+  // (0: the CF table's planes)
+  int lcf_S;
+  long long lcf_plane_elems;
 };
 
 struct PrepOutputs {
@@ -148,6 +152,7 @@ struct PrepOutputs {
   double* sumw;                 // nsw_rows * npol * nchan_img
   unsigned long long* counters; // [0] samples, [1] taps, [2] flagged, [3] wflagged
   int* error;                   // bit 1: zero noise weight; bit 2: w out of range
+  float4* lcf_in;               // learned CFs: (w in wavelengths, fracu, fracv, gridded) per sample
 };
 
 struct PrepInputs {
