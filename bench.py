@@ -14,6 +14,12 @@ visibility counts (log-uniform 1e4..2e6 per polarisation) and grid sizes (1024^2
 beam gridded and degridded per step by one batch (ygrid_batch_*), tuned by its autotuner and
 replayed from a HIP graph.
 
+``--workload learned_wdisc`` measures W-projection with learned CFs: 160,000 single-pol
+visibilities (uv uniform in a 2 km disc at 1.4 GHz) with continuous w in [-300, 300] m, 2048^2
+grid; a coordinate network (4 x 128 SiLU) is trained on the GPU on a 129-plane W-projection table
+and then gives every sample its CF at its own w (FP16 weights and CF storage); the plain
+129-plane table gridder runs beside it.
+
 ``--workload awproject_feeds`` measures the A-projection mixed-precision case instead: 160,000
 rows x 32 channels around 1.4 GHz (uv uniform in a 2 km disc), 9 feeds on a 1 degree grid, a
 12 m uniform disk, 2048^2 grid; grid + degrid with the CF taps and gridding coefficients stored
@@ -118,7 +124,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (3x), the reference for rocprof byte counters")
     ap.add_argument("--workload", default="wproject_array",
-                    choices=["wproject_array", "awproject_feeds", "batched_beams"])
+                    choices=["wproject_array", "awproject_feeds", "batched_beams", "learned_wdisc"])
+    ap.add_argument("--train-epochs", type=int, default=200, help="learned_wdisc: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md"}))
@@ -138,6 +145,9 @@ def main():
         return
     if args.workload == "batched_beams":
         bench_batched(args)
+        return
+    if args.workload == "learned_wdisc":
+        bench_learned(args)
         return
     import yandasoft_amd as ya
     from yandasoft_amd import workloads as W
@@ -380,6 +390,87 @@ def bench_awproject(args):
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = aw_cpu_baseline(wl)
+    print(json.dumps(result), flush=True)
+
+
+def bench_learned(args):
+    import torch
+    import yandasoft_amd as ya
+    from yandasoft_amd import workloads as W
+
+    wl = W.learned_wdisc()
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    out = torch.zeros(wl.vis.size * 2, dtype=torch.float32, device=dev)
+    chunk = ya.Chunk(uvw=t(wl.uvw), freq=t(wl.freq), vis=t(wl.vis.view(np.float32)), vis_out=out,
+                     noise=t(wl.noise.view(np.float32)), device=True)
+    chunk.npol = 1
+    img = ya.ImageDesc(wl.nx, wl.ny, 1, 1, wl.cell, wl.cell)
+    ctx = ya.Context(0)
+    model = np.zeros((1, 1, wl.ny, wl.nx), np.float32)
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        model[0, 0, rng.integers(wl.nx // 4, 3 * wl.nx // 4), rng.integers(wl.nx // 4, 3 * wl.nx // 4)] = 1.0
+    t_model = t(model)
+    tg = ya.Gridder(ctx, "GPUWProject", wl.params)
+    tg.initialise_grid(img)
+    tg.grid(chunk)  # generates the 129-plane table
+    lcf = ya.LearnedCF(ctx, hidden=128, layers=4, seed=2)
+    rep = lcf.train_wproject(tg, epochs=args.train_epochs, batch=1024, lr=3e-3, seed=3, holdout_every=8)
+    log(f"trained {rep.steps} steps in {rep.train_s:.1f} s: support {rep.support}, held-out max err "
+        f"{rep.holdout_max_err:.2e} rms {rep.holdout_rms_err:.2e}")
+    modes = {}
+    grids = {}
+    for name, use_lcf, prec in (("table_fp32", False, "fp32"), ("learned_fp16", True, "fp16")):
+        params = dict(wl.params, cfprecision=prec, visprecision=prec)
+        gg = ya.Gridder(ctx, "GPUWProject", params)
+        dg = ya.Gridder(ctx, "GPUWProject", params)
+        if use_lcf:
+            gg.set_learned_cf(lcf, 1.0)
+            dg.set_learned_cf(lcf, 1.0)
+        dg.initialise_degrid(img, t_model)
+
+        def step():
+            ctx.clear_cache()
+            gg.initialise_grid(img)
+            gg.grid(chunk)
+            dg.degrid(chunk)
+
+        for _ in range(args.warmup):
+            step()
+        ctx.synchronize()
+        s0g, s0d = gg.stats(), dg.stats()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        s1g, s1d = gg.stats(), dg.stats()
+        taps = (s1g.taps_gridded - s0g.taps_gridded) + (s1d.taps_degridded - s0d.taps_degridded)
+        modes[name] = {"value": taps / el, "ms_per_step": 1e3 * el / args.steps, "support": s1g.support,
+                       "learned_cf_active": s1g.learned_cf_active}
+        grids[name] = gg.grid_array()
+        log(f"{name}: {taps / el / 1e9:.2f} Gtaps/s, {1e3 * el / args.steps:.1f} ms/step")
+    dev_ = float(np.max(np.abs(grids["learned_fp16"] - grids["table_fp32"])) / np.max(np.abs(grids["table_fp32"])))
+    m = modes["learned_fp16"]
+    result = {
+        "metric": "CF-tap updates/s (learned-CF W-proj grid+degrid, continuous w)",
+        "value": m["value"], "unit": "CF-tap updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16 weights and CF storage, f32 arithmetic", "data": "synthetic",
+        "config": {"workload": "learned_wdisc: 160000 vis, uv uniform in a 2 km disc, w uniform in [-300, 300] m, "
+                               "1.4 GHz, 2048^2, network 4 x 128 SiLU trained on a 129-plane W-proj table (os 4)",
+                   "rows": wl.nrow, "grid": wl.nx, "parallelism": "1 GPU"},
+        "training": {"epochs": rep.epochs, "steps": rep.steps, "examples": rep.examples, "cfs": rep.cfs,
+                     "holdout_cfs": rep.holdout_cfs, "seconds": rep.train_s, "first_loss": rep.first_loss,
+                     "last_loss": rep.last_loss, "holdout_max_err_vs_table": rep.holdout_max_err,
+                     "holdout_rms_err_vs_table": rep.holdout_rms_err, "support": rep.support},
+        "modes": modes,
+        "grid_deviation_learned_vs_129_plane_table": dev_,
+        "roofline": {"bound": "mfma", "kernel": "whole pass (network evaluation per sample tap + gridding)",
+                     "achieved": 8.0 * m["value"] / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": 8.0 * m["value"] / 1e12 / FP32_PEAK_TFLOPS, "traffic": None},
+    }
     print(json.dumps(result), flush=True)
 
 

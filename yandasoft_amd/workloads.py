@@ -11,6 +11,8 @@ askap/simulation/Simulator.cc:932-963).
 * ``batched_beams``  -- many W-projection grids at once: 36 beams with uneven channel counts
                         (8..64), visibility counts (log-uniform 1e4..2e6 per polarisation) and
                         grid sizes (1024^2, 2048^2, 4096^2), rows of the 36-antenna array.
+* ``learned_wdisc``  -- the ``sphfunc_disc`` rows with w drawn uniformly in [-wmax, wmax]
+                        (continuous w) for W-projection with learned CFs.
 * ``awproject_feeds`` -- A-projection: the ``sphfunc_disc`` uv generator at 32 channels around
                         1.4 GHz, 9 feeds pointed on a 3x3 grid of 1 degree spacing, a 12 m
                         uniform disk illumination.
@@ -176,3 +178,18 @@ def batched_beams(nbeam: int = 36, chan_range=(8, 64), nvis_range=(1e4, 2e6), si
                   "limitsupport": limitsupport, "cutoff": cutoff}
         out.append(Workload(f"beam{b}", uvw, freq, vis, noise, n, n, 4.5 * ARCSEC * 4096 / n, "GPUWProject", params))
     return out
+
+
+def learned_wdisc(nvis: int = 160_000, radius_m: float = 2000.0, w_m: float = 300.0, freq_hz: float = 1.4e9,
+                  n: int = 2048, cell_arcsec: float = 5.0, nwplanes: int = 129, oversample: int = 4,
+                  cutoff: float = 1e-2, maxsupport: int = 256, seed: int = 20261203) -> Workload:
+    """The sphfunc_disc rows (uv uniform in a disc, one channel) with w uniform in [-w_m, w_m]
+    metres instead of 0: every sample has its own w, between the planes of any w sampling."""
+    wl = sphfunc_disc(nvis=nvis, radius_m=radius_m, freq_hz=freq_hz, n=n, cell_arcsec=cell_arcsec, seed=seed)
+    rng = np.random.default_rng(seed + 7)
+    uvw = wl.uvw.copy()
+    uvw[:, 2] = rng.uniform(-w_m, w_m, wl.nrow)
+    wmax = w_m * freq_hz / C_LIGHT * 1.0001
+    params = {"wmax": wmax, "nwplanes": nwplanes, "oversample": oversample, "maxsupport": maxsupport,
+              "cutoff": cutoff}
+    return Workload("learned_wdisc", uvw, wl.freq, wl.vis, wl.noise, n, n, wl.cell, "GPUWProject", params)
