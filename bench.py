@@ -9,6 +9,10 @@ oversample 4, support 64), with every input already resident in HBM.
 Metric: CF-tap updates/s (one complex multiply-accumulate of a (visibility, pol) with a CF tap =
 8 FP32 FLOP, TableVisGridder.cc:264-267,804), whole job.  One JSON line is printed.
 
+``--workload sphfunc_disc`` measures the plain SphFunc case: 160,000 single-pol visibilities
+(uv uniform in a 2 km disc at 1.4 GHz, w = 0), 2048^2 grid, support 3, oversample 128, grid then
+degrid; HBM- and launch-bound by construction.
+
 ``--workload batched_beams`` measures 36 W-projection beams of uneven channel counts (8..64),
 visibility counts (log-uniform 1e4..2e6 per polarisation) and grid sizes (1024^2..4096^2): every
 beam gridded and degridded per step by one batch (ygrid_batch_*), tuned by its autotuner and
@@ -126,8 +130,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (3x), the reference for rocprof byte counters")
+    ap.add_argument("--cf-precision", default="fp32", choices=["fp32", "fp16", "bf16"],
+                    help="wproject_array: CF tap and coefficient storage (the metric is quoted for fp32)")
     ap.add_argument("--workload", default="wproject_array",
-                    choices=["wproject_array", "awproject_feeds", "batched_beams", "learned_wdisc"])
+                    choices=["wproject_array", "sphfunc_disc", "awproject_feeds", "batched_beams", "learned_wdisc"])
     ap.add_argument("--train-epochs", type=int, default=200, help="learned_wdisc: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -152,6 +158,9 @@ def main():
     if args.workload == "learned_wdisc":
         bench_learned(args)
         return
+    if args.workload == "sphfunc_disc":
+        bench_sphfunc(args)
+        return
     import yandasoft_amd as ya
     from yandasoft_amd import workloads as W
 
@@ -169,8 +178,11 @@ def main():
 
     ctx = ya.Context(0)
     img = ya.ImageDesc(wl.nx, wl.ny, wl.npol, 1, wl.cell, wl.cell)
-    gg = ya.Gridder(ctx, "GPUWProject", wl.params)
-    dg = ya.Gridder(ctx, "GPUWProject", wl.params)
+    prm = dict(wl.params)
+    if args.cf_precision != "fp32":
+        prm.update(cfprecision=args.cf_precision, visprecision=args.cf_precision)
+    gg = ya.Gridder(ctx, "GPUWProject", prm)
+    dg = ya.Gridder(ctx, "GPUWProject", prm)
     rng = np.random.default_rng(3)
     model = np.zeros((1, wl.npol, wl.ny, wl.nx), np.float32)
     for _ in range(50):
@@ -235,7 +247,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.cf_precision == "fp32" else f"{args.cf_precision} storage, f32 arithmetic",
         "data": "synthetic",
         "config": {"workload": "wproject_array: 36 ant 6 km disc, 300 x 630 rows, 64 ch 700-1000 MHz, 4 pol, "
                                "4096^2, 129 w-planes, os 4, support 64",
@@ -393,6 +405,65 @@ def bench_awproject(args):
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = aw_cpu_baseline(wl)
+    print(json.dumps(result), flush=True)
+
+
+def bench_sphfunc(args):
+    import torch
+    import yandasoft_amd as ya
+    from yandasoft_amd import workloads as W
+
+    wl = W.sphfunc_disc()
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    out = torch.zeros(wl.vis.size * 2, dtype=torch.float32, device=dev)
+    chunk = ya.Chunk(uvw=t(wl.uvw), freq=t(wl.freq), vis=t(wl.vis.view(np.float32)), vis_out=out,
+                     noise=t(wl.noise.view(np.float32)), device=True)
+    chunk.npol = 1
+    img = ya.ImageDesc(wl.nx, wl.ny, 1, 1, wl.cell, wl.cell)
+    ctx = ya.Context(0)
+    model = np.zeros((1, 1, wl.ny, wl.nx), np.float32)
+    model[0, 0, wl.ny // 2, wl.nx // 2] = 1.0
+    gg = ya.Gridder(ctx, "GPUSphFunc", wl.params)
+    dg = ya.Gridder(ctx, "GPUSphFunc", wl.params)
+    dg.initialise_degrid(img, t(model))
+
+    def step():
+        ctx.clear_cache()
+        gg.initialise_grid(img)
+        gg.grid(chunk)
+        dg.degrid(chunk)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    ctx.synchronize()
+    s0g, s0d = gg.stats(), dg.stats()
+    gg.set_profiling(True)
+    dg.set_profiling(True)
+    nsteps = max(args.steps, 20)  # a step is ~1 ms: time enough of them
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        step()
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    s1g, s1d = gg.stats(), dg.stats()
+    taps = (s1g.taps_gridded - s0g.taps_gridded) + (s1d.taps_degridded - s0d.taps_degridded)
+    gk = s1g.ms_grid_kernel / max(1.0, s1g.n_grid_launches)
+    # compulsory bytes of one gridding pass: uvw 24 B + vis 8 B + noise 8 B per sample, the CF
+    # table once (128^2 planes x 7 x 7 taps x 8 B), the grid written once (2048^2 x 8 B)
+    nbytes = wl.nrow * (24 + 8 + 8) + 128 * 128 * 49 * 8 + wl.nx * wl.ny * 8
+    achieved = nbytes / (gk * 1e-3) / 1e9
+    result = {
+        "metric": "CF-tap updates/s (SphFunc grid+degrid)", "value": taps / el, "unit": "CF-tap updates/s",
+        "n_gpus": 1, "steps": nsteps, "warmup": args.warmup, "ms_per_step": 1e3 * el / nsteps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "sphfunc_disc: 160000 vis, uv uniform in a 2 km disc, 1.4 GHz, w = 0, 2048^2, "
+                               "support 3, oversample 128", "rows": wl.nrow, "grid": wl.nx, "parallelism": "1 GPU"},
+        "phases_ms_per_pass": {"grid_kernel": gk, "prep": s1g.ms_prep / max(1.0, s1g.n_grid_launches),
+                               "binning": s1g.ms_bin / max(1.0, s1g.n_grid_launches)},
+        "roofline": {"bound": "hbm", "kernel": "k_grid<1, 2, true, 0>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
     print(json.dumps(result), flush=True)
 
 

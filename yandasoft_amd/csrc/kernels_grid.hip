@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "ygrid_internal.h"
 
@@ -515,9 +516,19 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Operands of one pair for one lane: the CF taps [k][j][e] and, for gridding, the four MFMA
 // B operands of the sample (Samples::coef in the lane-major layout written by k_prep).
+template <int CFP>
 struct Taps {
-  float2 t[4][2][2];
+  // FP32 tables: the taps as loaded; 16-bit tables: the packed (re, im) words, widened where the
+  // taps are used (widening at the load would make the compiler wait for each load at once)
+  typename std::conditional<CFP == kPrecF32, float2, uint32_t>::type t[4][2][2];
   f32x4 b;
+  __device__ __forceinline__ float2 tap(int k, int j, int e) const {
+    if constexpr (CFP == kPrecF32) {
+      return t[k][j][e];
+    } else {
+      return make_float2(widen16(t[k][j][e] & 0xffffu, CFP), widen16(t[k][j][e] >> 16, CFP));
+    }
+  }
 };
 
 // s_waitcnt vmcnt(0) (expcnt and lgkmcnt unconstrained): every vector memory load issued so
@@ -624,7 +635,7 @@ constexpr int cf_shift() {
 }
 
 template <bool COEF, int CFP>
-__device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, const PairGeom& g, int lu, int lv,
+__device__ __forceinline__ void load_cf(Taps<CFP>& c, __amdgpu_buffer_rsrc_t cfr, const PairGeom& g, int lu, int lv,
                                         const f32x4* s_coef_pair, int o) {
   if constexpr (COEF) c.b = s_coef_pair[o];
   uint32_t vo[2];
@@ -644,11 +655,8 @@ __device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, con
           c.t[k][j][0] = make_float2(__builtin_bit_cast(float, d[0]), __builtin_bit_cast(float, d[1]));
           c.t[k][j][1] = make_float2(__builtin_bit_cast(float, d[2]), __builtin_bit_cast(float, d[3]));
         } else {
-          uint32_t d[2];
 #pragma unroll
-          for (int i = 0; i < 2; ++i) d[i] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo[j] + 4u * i, so, 0);
-#pragma unroll
-          for (int e = 0; e < 2; ++e) c.t[k][j][e] = make_float2(widen16(d[e] & 0xffffu, CFP), widen16(d[e] >> 16, CFP));
+          for (int e = 0; e < 2; ++e) c.t[k][j][e] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo[j] + 4u * e, so, 0);
         }
       }
 }
@@ -659,8 +667,8 @@ __device__ __forceinline__ void load_cf(Taps& c, __amdgpu_buffer_rsrc_t cfr, con
 // C.im (x) B[2m+1].  MFMA block b = lane/4 holds the taps of its lanes 4b..4b+3 (4 pixels) as A
 // and the sample's coefficients for outputs (pol 2m re, pol 2m im, pol 2m+1 re, pol 2m+1 im) as B;
 // the result for the pixel of lane 4b+i and output o lands in lane 4b+o, register i.
-template <int NP>
-__device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], const Taps& c, const PairGeom& g) {
+template <int NP, class T>
+__device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], const T& c, const PairGeom& g) {
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -668,7 +676,7 @@ __device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], c
       if (g.mask & (1u << (2 * k + j))) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          const float2 x = c.t[k][j][e];
+          const float2 x = c.tap(k, j, e);
           acc[k][j][e][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(x.x, c.b[0], acc[k][j][e][0], 0, 0, 0);
           acc[k][j][e][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(x.y, c.b[1], acc[k][j][e][0], 0, 0, 0);
           if constexpr (NP > 2) {
@@ -706,13 +714,13 @@ __device__ __forceinline__ void for_each_pair(const int4* s_desc, const f32x4* s
       const PairU p = pn;
       if (i + 1 < n) pn = staged_pair(s_desc, i + 1);
       const PairGeom g = pair_geom<SH>(p, w);
-      Taps c;
+      Taps<CFP> c;
       load_cf<COEF, CFP>(c, cfr, g, lu, lv, COEF ? s_coef + CW * i : s_coef, o);
       consume(c, g, p, i);
     }
   } else {
     int i = 0;
-    Taps cA, cB;
+    Taps<CFP> cA, cB;
     PairU pA = staged_pair(s_desc, 0);
     PairGeom gA = pair_geom<SH>(pA, w);
     load_cf<COEF, CFP>(cA, cfr, gA, lu, lv, s_coef, o);
@@ -790,7 +798,7 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     const int n = it.z - it.y;
     stage_item<true>(J.descs, J.smp, it.y, n, s_desc, s_coef, tid, blockDim.x);
     for_each_pair<true, PIPE, CFP>(s_desc, s_coef, cfr, n, w, lu, lv, o,
-                              [&](const Taps& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
+                              [&](const auto& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
     // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of row 4k+lv, column
     // 32j+8(b&3)+2i+e.  Per (m, k) the wave writes its 4 rows x 64 columns x 4 outputs to its own
     // LDS area (the coefficient staging space, free until the next item is staged) and reads them
@@ -877,8 +885,8 @@ __device__ __forceinline__ float wave_reduce_scatter8(const float (&x)[8], int l
 
 // sum over the wave's active blocks of C * conj(G) per polarisation (GridKernel.cc:121-122);
 // x[2q] / x[2q+1] = real / imaginary part for polarisation q, x[2NP..7] = 0
-template <int NP>
-__device__ __forceinline__ void degrid_pair(float (&x)[8], const Taps& c, const float2 (&gt)[4][2][2][NP],
+template <int NP, class T>
+__device__ __forceinline__ void degrid_pair(float (&x)[8], const T& c, const float2 (&gt)[4][2][2][NP],
                                             const PairGeom& g) {
 #pragma unroll
   for (int i = 0; i < 8; ++i) x[i] = 0.f;
@@ -889,7 +897,7 @@ __device__ __forceinline__ void degrid_pair(float (&x)[8], const Taps& c, const 
       if (g.mask & (1u << (2 * k + j))) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          const float2 cc = c.t[k][j][e];
+          const float2 cc = c.tap(k, j, e);
 #pragma unroll
           for (int q = 0; q < NP; ++q) {
             x[2 * q] = fmaf(cc.x, gt[k][j][e][q].x, x[2 * q]);
@@ -953,7 +961,7 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     const int n = it.z - it.y;
     stage_item<false>(J.descs, Samples{}, it.y, n, s_desc, nullptr, tid, blockDim.x);
     for_each_pair<false, PIPE, CFP>(s_desc, nullptr, cfr, n, w, lu, lv, 0,
-                           [&](const Taps& c, const PairGeom& g, const PairU& p, int i) {
+                           [&](const auto& c, const PairGeom& g, const PairU& p, int i) {
                              float x[8];
                              degrid_pair<NP>(x, c, gt, g);
                              const float r = wave_reduce_scatter8(x, lane);
