@@ -471,6 +471,35 @@ def wproject_cf(prm: WProjectParams, nx_grid: int, ny_grid: int, uvcell: tuple, 
 # --------------------------------------------------------------------------------------
 # AWProject
 # --------------------------------------------------------------------------------------
+def wproject_pcf_kernel_pix(prm: WProjectParams, uvcell: tuple, iw: int) -> np.float32:
+    """Fitted kernel width in pixels of w plane iw for the preconditioner kernel
+    (WProjectVisGridder.cc:405-418), in the reference's float arithmetic."""
+    wtp = np.float32(abs(prm.sampling().wterm(iw)) / (uvcell[0] * uvcell[0]))
+    if wtp < 1:
+        return np.float32(3.0)
+    if prm.cutoff < 0.01:
+        return np.float32(6.0 + 1.14 * float(wtp))
+    return np.float32(np.sqrt(np.float32(49.0) + wtp * wtp))
+
+
+def wproject_pcf_cf(prm: WProjectParams, uvcell: tuple) -> CFCache:
+    """WProjectVisGridder::doPCFGridder (WProjectVisGridder.cc:385-434): support 1; per w plane and
+    oversampling offset a 3x3 plane with (1, wKernelPix) at index int(-frac/os) from the centre
+    (always the centre: the conversion truncates)."""
+    os_ = prm.oversample
+    planes = [None] * (os_ * os_ * prm.nwplanes)
+    for iw in range(prm.nwplanes):
+        wk = float(wproject_pcf_kernel_pix(prm, uvcell, iw))
+        for fracu in range(os_):
+            for fracv in range(os_):
+                c = np.zeros((3, 3), dtype=np.complex128)
+                ix = int(-float(fracu) / float(os_))
+                iy = int(-float(fracv) / float(os_))
+                c[iy + 1, ix + 1] = complex(1.0, wk)
+                planes[fracu + os_ * (fracv + os_ * iw)] = c
+    return CFCache(planes, os_)
+
+
 @dataclass
 class AWProjectParams(WProjectParams):
     """AProjectGridderBase::createAProjectGridder defaults (AProjectGridderBase.tcc:53-115)."""
@@ -519,10 +548,11 @@ def disk_pattern(nu, nv, ucell, vcell, os_, freq, diameter, blockage, l=0.0, m=0
 
 
 def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_of_feed, field=0, psf=False,
-                     state=None):
+                     state=None, pcf=False):
     """AWProjectVisGridder::initConvolutionFunction for the feeds in ``lm_of_feed`` (dict feed ->
     (l, m) beam offset) of one field.  ``state`` carries the CF cache and the common support
-    between calls.  Returns the state: {'planes': list, 'support': int}."""
+    between calls.  Returns the state: {'planes': list, 'support': int}.  pcf: the preconditioner
+    form, every plane 3x3 with (1, support) at the centre (AWProjectVisGridder.cc:511-516)."""
     os_ = prm.oversample
     nx = ny = prm.maxsupport
     qnx, qny = nx // os_, ny // os_
@@ -581,6 +611,11 @@ def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_o
                         ky = t * os_ + fracv + ny // 2
                         if kx.min() < 0 or ky.min() < 0 or kx.max() >= nx or ky.max() >= ny:
                             raise ValueError("Convolution function overflowing")
+                        if pcf:
+                            c = np.zeros((3, 3), dtype=np.complex128)
+                            c[1, 1] = complex(1.0, S)
+                            state["planes"][fracu + os_ * (fracv + os_ * z)] = c
+                            continue
                         state["planes"][fracu + os_ * (fracv + os_ * z)] = float(os_ * os_) * buf[np.ix_(ky, kx)]
     return state
 
@@ -698,12 +733,14 @@ class OracleGridder:
         elif self.kind == "SphFunc":
             a, s, o = self.sph
             self.cf = sphfunc_cf(a, s, o, pcf=pcf)
+        elif self.kind == "WProject" and pcf:
+            self.cf, self.support = wproject_pcf_cf(self.wp, self.uvcell), 1
         elif self.kind == "WProject":
             self.cf, self.support = wproject_cf(self.wp, self.nx, self.ny, self.uvcell, only=self.only_planes)
         else:
             raise ValueError(self.kind)
 
-    def _init_aw(self, chunk: Chunk, psf: bool):
+    def _init_aw(self, chunk: Chunk, psf: bool, pcf: bool = False):
         """AW: CFs for the feeds of this chunk that are not in the cache yet (one field)."""
         feeds = chunk.feed1 if chunk.feed1 is not None else np.zeros(chunk.nrow, dtype=np.int32)
         new = {}
@@ -713,7 +750,7 @@ class OracleGridder:
                 new[f] = self.aw_lm.get(f, (0.0, 0.0))
         if new or self.aw_state is None:
             self.aw_state = awproject_planes(self.wp, self.nx, self.ny, self.uvcell, chunk.freq, new, 0, psf,
-                                             self.aw_state)
+                                             self.aw_state, pcf=pcf)
             self.aw_done.update(new.keys())
         self.support = self.aw_state["support"]
         self.cf = CFCache(self.aw_state["planes"], self.wp.oversample)
@@ -790,7 +827,7 @@ class OracleGridder:
         L = lib()
         nrow, nchan, npol = chunk.vis.shape
         if self.kind == "AWProject":
-            self._init_aw(chunk, psf=(mode in (1, 2)))
+            self._init_aw(chunk, psf=(mode in (1, 2)), pcf=(mode == 2))
         else:
             self._init_cf(pcf=(mode == 2))
         os_ = self.cf.oversample

@@ -236,6 +236,60 @@ def test_wproject_cf_grid_degrid_match_oracle(gpu_ctx, extra):
     assert rel(gv, ov) < TOL
 
 
+@pytest.mark.parametrize("cutoff", [1e-2, 1e-3])
+def test_wproject_pcf_matches_oracle(gpu_ctx, cutoff):
+    """Preconditioner form (WProjectVisGridder::doPCFGridder): 3x3 kernels carrying the fitted
+    W-kernel width, gridded with conj(C) in half the plane (TableVisGridder.cc:837-844)."""
+    announce(f"wproject pcf, cutoff {cutoff}")
+    N, cell = 256, 30 * ARC
+    d = make_data(1500, 2, 2, N, cell, f0=1.0e9, frac=0.35, wmax_m=2500.0 * O.C_LIGHT / 1.005e9, noise_var=True)
+    params = dict(WP, cutoff=cutoff)
+    prm = O.WProjectParams(**params)
+    g, o, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUWProject", params, dict(kind="WProject", wproject=prm), d,
+                                            N, cell, 2, dopcf=True)
+    assert g.stats().support == 1
+    for k in (0, 5, 16 * 4 + 3, 16 * 8 + 15):
+        assert np.array_equal(g.cf_plane(k), o.cf.planes[k].astype(np.complex64)), k
+    # the centre plane (w = 0) carries the nominal 3 pixels, the outer ones more
+    assert g.cf_plane(16 * 4)[1, 1].imag == 3.0 and g.cf_plane(0)[1, 1].imag > 3.0
+    assert rel(gg, og) < TOL
+    assert rel(gw, ow) < 1e-6
+
+
+def test_awproject_pcf_matches_oracle(gpu_ctx):
+    """Preconditioner form of AWProject (AWProjectVisGridder.cc:436-466,511-516): the support found
+    on the first (beam-centred) plane, carried by 3x3 kernels."""
+    announce("awproject pcf, 2 feeds, 2 chans")
+    import yandasoft_amd as ya
+    N, cell = 128, 60 * ARC
+    centre = (0.5, -0.7)
+    d = make_data(600, 2, 2, N, cell, f0=1.4e9, df=1e7, frac=0.3, wmax_m=400.0 * O.C_LIGHT / 1.41e9, seed=9)
+    feed = (np.arange(600) % 2).astype(np.int32)
+    point = np.zeros((600, 2))
+    point[:, 0] = centre[0] + np.where(feed == 1, math.radians(0.5), 0.0)
+    point[:, 1] = centre[1]
+    params = {"wmax": 500.0, "nwplanes": 3, "oversample": 4, "maxsupport": 128, "maxfeeds": 2, "cutoff": 0.01,
+              "diameter": "12m", "blockage": "1m"}
+    prm = O.AWProjectParams(wmax=500.0, nwplanes=3, oversample=4, maxsupport=128, maxfeeds=2, cutoff=0.01,
+                            diameter=12.0, blockage=1.0)
+    g = ya.Gridder(gpu_ctx, "GPUAWProject", params)
+    g.initialise_grid(ya.ImageDesc(N, N, 2, 1, cell, cell, centre=centre), dopcf=True)
+    g.grid(ya.Chunk(uvw=d["uvw"], freq=d["freq"], vis=d["vis"], noise=d["noise"], feed1=feed,
+                    pointing1=np.ascontiguousarray(point)))
+    o = O.OracleGridder("AWProject", "f32", wproject=prm)
+    o.aw_lm = {0: O.direction_lm(tuple(point[0]), centre), 1: O.direction_lm(tuple(point[1]), centre)}
+    o.aw_nchan = 2
+    o.initialise_grid((N, N, 2, 1), (cell, cell), dopcf=True)
+    och = ochunk(d)
+    och.feed1 = feed
+    o.grid(och)
+    assert o.support > 1
+    for k in (0, 7, 16 * 5 + 9, 16 * 11 + 15):
+        assert np.array_equal(g.cf_plane(k), o.cf.planes[k].astype(np.complex64)), k
+    assert g.cf_plane(0)[1, 1] == complex(1.0, o.support)
+    assert rel(g.grid_array().reshape(1, 2, N, N), o.grid_) < TOL
+
+
 @pytest.mark.parametrize("taper", [False, True])
 def test_awproject_disk_matches_oracle(gpu_ctx, taper):
     announce(f"awproject disk 12m, 2 feeds, 2 chans, spheroidaltaper={taper}")

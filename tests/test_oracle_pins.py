@@ -222,6 +222,32 @@ def test_wproject_cf_w0_plane_is_taper_transform_and_normalised():
         assert np.max(np.abs(cf.planes[fu + os_ * (fv + os_ * z)] - ref)) < 1e-10
 
 
+@pytest.mark.parametrize("cutoff,expect", [
+    # w planes -10, -5, 0, 5, 10 wavelengths, du = 0.5: wThetaPix = |w| / du^2 = 40, 20, 0, 20, 40;
+    # kernel widths worked by hand from the reference's fits (WProjectVisGridder.cc:405-418)
+    (1e-2, [40.6078810, 21.1896201, 3.0, 21.1896201, 40.6078810]),   # sqrt(49 + wThetaPix^2)
+    (1e-3, [51.6, 28.8, 3.0, 28.8, 51.6]),                            # 6 + 1.14 * wThetaPix
+])
+def test_wproject_pcf_kernel_closed_form(cutoff, expect):
+    prm = O.WProjectParams(wmax=10.0, nwplanes=5, cutoff=cutoff, oversample=2)
+    cf = O.wproject_pcf_cf(prm, (0.5, 0.5))
+    assert len(cf.planes) == 5 * 4
+    for z in range(5):
+        for frac in range(4):
+            p = cf.planes[frac + 4 * z]
+            assert p.shape == (3, 3)
+            assert p[1, 1].real == 1.0
+            assert abs(p[1, 1].imag - expect[z]) < 1e-5 * expect[z]
+            q = p.copy()
+            q[1, 1] = 0
+            assert not np.any(q)
+    # below one pixel of w.theta^2 the nominal 3 pixels (|w| = 0.2: 0.8), above it the fit (|w| = 0.3: 1.2)
+    small = O.WProjectParams(wmax=0.2, nwplanes=3, cutoff=1e-3, oversample=1)
+    assert O.wproject_pcf_kernel_pix(small, (0.5, 0.5), 0) == np.float32(3.0)
+    above = O.WProjectParams(wmax=0.3, nwplanes=3, cutoff=1e-3, oversample=1)
+    assert abs(O.wproject_pcf_kernel_pix(above, (0.5, 0.5), 2) - 7.368) < 1e-5
+
+
 # ---------------------------------------------------------------------------------------
 # the gridding loop: closed forms, brute force, adjoint identity
 # ---------------------------------------------------------------------------------------

@@ -129,6 +129,8 @@ class AWProjectGridder : public Gridder {
   void validateCache(const ChunkDev& acc);
   void generateFeed(const ChunkDev& acc, int feed, double lon, double lat);
   void ensureTable(int S);
+  void writePCFPlanes(const Ctx& c, DevBuf& buf, int nx, int ny, int cnt, int z0, int chan, int nchan, int feed);
+  int pcf_support_ = 0;  // preconditioner form: the support found on the first plane
 
   double pointing_tol_ = 1e-4, pa_tol_ = 0.1, freq_tol_ = 1e-6;
   double wmax_ = 10000, cutoff_ = 1e-3, alpha_ = 1.0, diameter_ = 12, blockage_ = 0, sph_wcut_ = 0;
@@ -277,6 +279,10 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
                        dtpw.as<double>() + z0, cnt, buf.as<float2>());
       fft_plain_c2c(c, buf.as<float2>(), nx, ny, cnt, true, planb);
       launch_scale_sign(c, buf.as<float2>(), nx, ny, cnt, (float)(1.0 / (double(nx) * ny)));
+      if (dopcf_) {
+        writePCFPlanes(c, buf, nx, ny, cnt, z0, chan, nchan, feed);
+        continue;
+      }
       if (!cf_ || cf_->support == 0) {
         // support from the first plane generated (AWProjectVisGridder.cc:436-466)
         const SupportResult r = search_support(c, buf.as<float2>(), nx, ny, cutoff_, abs_cutoff_, false, os_);
@@ -307,6 +313,36 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
       YG_HIP(hipStreamSynchronize(c.stream));
     }
   }
+}
+
+// Preconditioner form (AWProjectVisGridder.cc:436-466,511-516): the support is found on the first
+// plane generated as for the imaging kernels, and every plane becomes 3x3 with the centre tap
+// (1, support); the table itself then has support 1.
+void AWProjectGridder::writePCFPlanes(const Ctx& c, DevBuf& buf, int nx, int ny, int cnt, int z0, int chan, int nchan,
+                                      int feed) {
+  if (pcf_support_ == 0) {
+    const SupportResult r = search_support(c, buf.as<float2>(), nx, ny, cutoff_, abs_cutoff_, false, os_);
+    YG_CHECK((r.size + 1) * os_ < nx / 2, 4,
+             "Overflowing convolution function - increase maxSupport or cutoff or decrease overSample");
+    int S = r.size;
+    if (limitsupport_ > 0 && S > limitsupport_) S = limitsupport_;
+    pcf_support_ = S;
+  }
+  const int S = pcf_support_;
+  const int kmin = -S * os_ + nx / 2, kmax = S * os_ + os_ - 1 + nx / 2;
+  YG_CHECK(kmin >= 0 && kmax < nx && kmax < ny, 4,
+           "Convolution function overflowing - increase maxsupport or cutoff or decrease oversample");
+  ensureTable(1);
+  const size_t blk = (size_t)cf_pitch(1) * cf_rows(1);
+  const int zfirst = z0 + nw_ * (chan + nchan * (feed + maxfeeds_ * field_));
+  const size_t k0 = (size_t)zfirst * os_ * os_, nk = (size_t)cnt * os_ * os_;
+  std::vector<float2> host(nk * blk, make_float2(0.f, 0.f));
+  for (size_t k = 0; k < nk; ++k)
+    host[k * blk + (size_t)(1 + kGuardV) * cf_pitch(1) + 1 + kGuardU] = make_float2(1.f, (float)S);
+  // on the context's stream, after the table's clearing
+  YG_HIP(hipMemcpyAsync(cf_->table.as<float2>() + cf_->off[k0], host.data(), host.size() * sizeof(float2),
+                        hipMemcpyHostToDevice, c.stream));
+  YG_HIP(hipStreamSynchronize(c.stream));
 }
 
 void AWProjectGridder::initConvolutionFunction(const ChunkDev& acc) {

@@ -831,11 +831,12 @@ class WProjectGridder : public Gridder {
     return d;
   }
   void initConvolutionFunction(const ChunkDev&) override;
+  void initPCFKernel();
 
   // Learned CFs: every sample its own padded plane of the network's support, at the gridder's
   // CF storage precision, filled by the network after the prep kernel.
   void customizePass(PassSetup& ps) override {
-    if (!learnedActive()) return;
+    if (!learnedActive() || dopcf_) return;  // the preconditioner pass keeps its 3x3 kernels
     const int S = lcf_->support;
     const int P = 2 * S + 1 + 2 * kGuardU, R = 2 * S + 1 + 2 * kGuardV;
     const long long plane = (long long)R * P;
@@ -927,11 +928,48 @@ SupportResult search_support(const Ctx& c, const float2* plane, int nx, int ny, 
   return r;
 }
 
+// Nearest-neighbour preconditioner kernel (WProjectVisGridder::doPCFGridder,
+// WProjectVisGridder.cc:385-434): support 1, per w plane and oversampling offset a 3x3 plane whose
+// centre is (1, wKernelPix), the fitted width of the plane's W-projection kernel in pixels:
+// wThetaPix = |w| / du^2; 3 below one pixel, else 6 + 1.14 wThetaPix (cutoff < 0.01) or
+// sqrt(49 + wThetaPix^2).  Any offset support is left at zero (WProjectVisGridder.cc:164-171).
+void WProjectGridder::initPCFKernel() {
+  auto t = std::make_shared<CFTable>();
+  const int nw = ws_.nw;
+  layout_table(*t, std::vector<int>(nw, 1), os_);
+  std::vector<float2> host(t->n_elems, make_float2(0.f, 0.f));
+  for (int iw = 0; iw < nw; ++iw) {
+    const float wThetaPix = (float)(std::fabs(ws_.wterm(iw)) / (uvcell_[0] * uvcell_[0]));
+    float wKernelPix;
+    if (wThetaPix < 1) {
+      wKernelPix = 3;
+    } else if (cutoff_ < 0.01) {
+      wKernelPix = (float)(6 + 1.14 * wThetaPix);
+    } else {
+      wKernelPix = std::sqrt(49.f + wThetaPix * wThetaPix);
+    }
+    for (int fu = 0; fu < os_; ++fu)
+      for (int fv = 0; fv < os_; ++fv) {
+        const int k = fu + os_ * (fv + os_ * iw);
+        // the reference's tap index int(-frac/os) truncates to 0: the centre of every plane
+        const int ix = (int)(-float(fu) / float(os_)), iy = (int)(-float(fv) / float(os_));
+        host[t->off[k] + (size_t)(iy + 1 + kGuardV) * cf_pitch(1) + ix + 1 + kGuardU] = make_float2(1.f, wKernelPix);
+      }
+  }
+  t->table.resize(host.size() * sizeof(float2));
+  YG_HIP(hipMemcpy(t->table.get(), host.data(), host.size() * sizeof(float2), hipMemcpyHostToDevice));
+  t->upload(*ctx_);
+  cf_ = t;
+}
+
 // WProjectVisGridder::generate (WProjectVisGridder.cc:435-522) with setup (:661-703)
 void WProjectGridder::initConvolutionFunction(const ChunkDev&) {
   if (cf_) return;
+  if (dopcf_) {
+    initPCFKernel();
+    return;
+  }
   const Ctx& c = *ctx_;
-  YG_CHECK(!dopcf_, 1, "the PCF form of GPUWProject is not supported");
   const int nx = maxsupport_, ny = maxsupport_;
   YG_CHECK(nx % 2 == 0, 1, "maxsupport must be even");
   const int qnx = nx / os_, qny = ny / os_;
