@@ -2,9 +2,10 @@
 // that maps (w, fractional oversampling offset u, v, tap dx, dy) to a CF tap, trained on the CF
 // tables the W-projection generator produces and evaluated per sample at its exact w.
 //
-//   k_gemm        C = op(A) op(B) on 64x64 tiles (FP32 FMA, 4x4 outputs per thread) with the
-//                 epilogues a dense layer needs: + bias, SiLU (keeping the pre-activation),
-//                 times SiLU'(Z) for the backward pass; B may be FP16 (inference weights).
+//   k_gemm        C = op(A) op(B) on 64x64 tiles on the matrix cores (FP32 MFMA 16x16x4, each
+//                 wave a 32x32 quarter in 2x2 blocks) with the epilogues a dense layer needs:
+//                 + bias, SiLU (keeping the pre-activation), times SiLU'(Z) for the backward
+//                 pass; B may be FP16 (inference weights, widened to FP32 as they are staged).
 //   k_colsum      bias gradients;  k_mse  loss and its gradient;  k_adam  the optimiser step.
 //   k_lcf_inputs  per-sample network inputs;  k_lcf_scatter  network outputs into per-sample
 //                 padded CF planes (the layout the tile kernels read).
@@ -19,6 +20,7 @@ namespace {
 
 constexpr int kT = 64;   // output tile
 constexpr int kK = 16;   // k step
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float sigm(float z) { return 1.f / (1.f + __expf(-z)); }
 __device__ __forceinline__ float silu(float z) { return z * sigm(z); }
@@ -43,8 +45,17 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, const
   __shared__ float sB[kK][kT + 4];
   const int tid = threadIdx.x;
   const int m0 = blockIdx.y * kT, n0 = blockIdx.x * kT;
-  const int tm = (tid / 16) * 4, tn = (tid % 16) * 4;
-  float acc[4][4] = {};
+  // wave (wm, wn) computes rows 32wm .. +31 and columns 32wn .. +31 of the tile in 2x2 blocks of
+  // 16x16.  v_mfma_f32_16x16x4_f32: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] of
+  // a block's 16x4 and 4x16 operands, and holds D[4 (l / 16) + r][l % 16] in register r.
+  const int lane = tid & 63, wv = tid >> 6;
+  const int wm = wv >> 1, wn = wv & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < K; k0 += kK) {
     // 64 x 16 of A and 16 x 64 of B: 1024 values each, 4 per thread
 #pragma unroll
@@ -65,41 +76,42 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, const
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < kK; ++kk) {
-      float a[4], b[4];
+    for (int kk = 0; kk < kK; kk += 4) {
+      float a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = sA[kk][tm + i];
+      for (int bi = 0; bi < 2; ++bi) a[bi] = sA[kk + lk][32 * wm + 16 * bi + li];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = sB[kk][tn + j];
+      for (int bj = 0; bj < 2; ++bj) b[bj] = sB[kk + lk][32 * wn + 16 * bj + li];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        for (int bj = 0; bj < 2; ++bj)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int m = m0 + tm + i;
-    if (m >= M) continue;
+  for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = n0 + tn + j;
-      if (n >= N) continue;
-      const size_t o = (size_t)m * ldc + n;
-      float v = acc[i][j];
-      if (EPI == kEpiBiasSilu) {
-        v += bias[n];
-        if (Zout) Zout[o] = v;
-        v = silu(v);
-      } else if (EPI == kEpiBias) {
-        v += bias[n];
-      } else if (EPI == kEpiSiluGrad) {
-        v *= silu_grad(Zin[o]);
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 32 * wm + 16 * bi + 4 * lk + r;
+        const int n = n0 + 32 * wn + 16 * bj + li;
+        if (m >= M || n >= N) continue;
+        const size_t o = (size_t)m * ldc + n;
+        float v = acc[bi][bj][r];
+        if (EPI == kEpiBiasSilu) {
+          v += bias[n];
+          if (Zout) Zout[o] = v;
+          v = silu(v);
+        } else if (EPI == kEpiBias) {
+          v += bias[n];
+        } else if (EPI == kEpiSiluGrad) {
+          v *= silu_grad(Zin[o]);
+        }
+        C[o] = v;
       }
-      C[o] = v;
-    }
-  }
 }
 
 // db[n] = sum over m of D[m * ldd + n]
