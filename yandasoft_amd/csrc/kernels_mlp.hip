@@ -40,7 +40,7 @@ template <bool TA, bool TB, typename TBt, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, const TBt* __restrict__ B,
                                               float* __restrict__ C, int M, int N, int K, int lda, int ldb, int ldc,
                                               const float* __restrict__ bias, float* __restrict__ Zout,
-                                              const float* __restrict__ Zin) {
+                                              const float* __restrict__ Zin, int kslice) {
   __shared__ float sA[kK][kT + 4];
   __shared__ float sB[kK][kT + 4];
   const int tid = threadIdx.x;
@@ -56,7 +56,10 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, const
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += kK) {
+  // split K (gridDim.z > 1, no epilogue): this workgroup sums k in [kb, ke) and adds its partial
+  // tile into C, which the caller has zeroed
+  const int kb = blockIdx.z * kslice, ke = min(K, kb + kslice);
+  for (int k0 = kb; k0 < ke; k0 += kK) {
     // 64 x 16 of A and 16 x 64 of B: 1024 values each, 4 per thread
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -65,13 +68,13 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, const
         int mm, kk;
         if (TA) { mm = e % kT; kk = e / kT; } else { kk = e % kK; mm = e / kK; }
         const int m = m0 + mm, k = k0 + kk;
-        sA[kk][mm] = (m < M && k < K) ? (TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k]) : 0.f;
+        sA[kk][mm] = (m < M && k < ke) ? (TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k]) : 0.f;
       }
       {
         int nn, kk;
         if (TB) { kk = e % kK; nn = e / kK; } else { nn = e % kT; kk = e / kT; }
         const int n = n0 + nn, k = k0 + kk;
-        sB[kk][nn] = (n < N && k < K) ? (TB ? ld(B, (size_t)n * ldb + k) : ld(B, (size_t)k * ldb + n)) : 0.f;
+        sB[kk][nn] = (n < N && k < ke) ? (TB ? ld(B, (size_t)n * ldb + k) : ld(B, (size_t)k * ldb + n)) : 0.f;
       }
     }
     __syncthreads();
@@ -110,7 +113,8 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, const
         } else if (EPI == kEpiSiluGrad) {
           v *= silu_grad(Zin[o]);
         }
-        C[o] = v;
+        if (gridDim.z > 1) atomicAdd(&C[o], v);
+        else C[o] = v;
       }
 }
 
@@ -233,10 +237,22 @@ __global__ void k_lcf_scatter(const float4* __restrict__ sin, const float* __res
 void mlp_gemm(const Ctx& c, bool ta, bool tb, const float* A, const void* B, bool b_half, float* C, int M, int N, int K,
               int lda, int ldb, int ldc, int epi, const float* bias, float* Zout, const float* Zin) {
   if (M <= 0 || N <= 0) return;
-  const dim3 g((N + kT - 1) / kT, (M + kT - 1) / kT), b(256);
+  // Few output tiles over a long K (the weight gradients: 128 x 128 over a batch of rows):
+  // split K over workgroups (slices of kSplitK), partial tiles added into a zeroed C.
+  constexpr int kSplitK = 64;
+  const int tiles = ((N + kT - 1) / kT) * ((M + kT - 1) / kT);
+  int splits = 1;
+  if (epi == kEpiNone && tiles < c.n_cu && K > 2 * kSplitK) splits = std::min((K + kSplitK - 1) / kSplitK, 64);
+  const int kslice = splits > 1 ? kSplitK * ((K + kSplitK * splits - 1) / (kSplitK * splits)) : K;
+  if (splits > 1) {
+    splits = (K + kslice - 1) / kslice;
+    YG_CHECK(ldc == N, 1, "split-K products need a dense C");
+    YG_HIP(hipMemsetAsync(C, 0, (size_t)M * N * sizeof(float), c.stream));
+  }
+  const dim3 g((N + kT - 1) / kT, (M + kT - 1) / kT, splits), b(256);
 #define YG_GEMM(TA, TB, TT, E)                                                                                    \
   hipLaunchKernelGGL((k_gemm<TA, TB, TT, E>), g, b, 0, c.stream, A, static_cast<const TT*>(B), C, M, N, K, lda, \
-                     ldb, ldc, bias, Zout, Zin)
+                     ldb, ldc, bias, Zout, Zin, kslice)
   if (b_half) {
     YG_CHECK(!ta && !tb, 1, "FP16 weights only in the forward product");
     if (epi == kEpiBiasSilu) YG_GEMM(false, false, _Float16, kEpiBiasSilu);
