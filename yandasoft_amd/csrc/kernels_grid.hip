@@ -962,9 +962,14 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     stage_item<false>(J.descs, Samples{}, it.y, n, s_desc, nullptr, tid, blockDim.x);
     for_each_pair<false, PIPE, CFP>(s_desc, nullptr, cfr, n, w, lu, lv, 0,
                            [&](const auto& c, const PairGeom& g, const PairU& p, int i) {
-                             float x[8];
-                             degrid_pair<NP>(x, c, gt, g);
-                             const float r = wave_reduce_scatter8(x, lane);
+                             // a wave whose rows the footprint misses (mask 0) contributes zeros
+                             // without the dot product and the reduction
+                             float r = 0.f;
+                             if (g.mask) {
+                               float x[8];
+                               degrid_pair<NP>(x, c, gt, g);
+                               r = wave_reduce_scatter8(x, lane);
+                             }
                              const int b = i % kDegridBatch;
                              if (lane < 8) s_part[b][w][slot] = r;
                              if (tid == 0) s_sid[b] = p.d.x;
