@@ -53,7 +53,7 @@ void Batch::prepare() {
   drop_graph();
   groups_.clear();
   std::map<std::tuple<int, int, int>, int> gidx;
-  const int chunk = ctx_->item_pairs > 0 ? ctx_->item_pairs : 512;
+  const int chunk = ctx_->item_pairs > 0 ? ctx_->item_pairs : kDefaultItemPairs;
   for (Pass& p : passes_) {
     p.active = p.g->setup_pass(p.a, p.forward, p.ps);
     if (!p.active) continue;
@@ -80,7 +80,7 @@ void Batch::prepare() {
     YG_CHECK(gr.items_cap < (size_t)INT_MAX, 1, "too many work items in one batched launch");
     gr.items.resize(gr.items_cap * sizeof(int4));
     gr.base.resize((gr.passes.size() + 1) * sizeof(int));
-    gr.counter.resize(2 * sizeof(int));
+    gr.counter.resize(kItemQueues * sizeof(int));
     std::vector<TileJob> jobs;
     for (int pi : gr.passes) {
       Pass& p = passes_[pi];
@@ -184,7 +184,7 @@ BatchTuning Batch::autotune(int trials) {
   for (const Group& gr : groups_) f32 = f32 && gr.prec == kPrecF32;
   std::vector<TileVariant> vars = {{2, 1}, {2, 0}, {3, 1}, {3, 0}};
   if (!f32) vars = {{2, 1}};  // 16-bit CF tables have the one variant
-  const int chunks[] = {128, 256, 512};
+  const int chunks[] = {256, 512, 2048};
   BatchTuning best{};
   best.ms = -1.0;
   const TileVariant g0 = ctx_->grid_variant, d0 = ctx_->degrid_variant;

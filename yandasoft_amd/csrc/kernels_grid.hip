@@ -6,7 +6,7 @@
 //   k_bin_*       counting sort of (sample, grid tile) pairs by (image chan, CF plane group, tile);
 //                 the key order is plane-major so that concurrently running workgroups read the
 //                 same few CF planes, which then stay resident in each XCD's L2.
-//   k_grid        persistent workgroups take work items (<= kChunk pairs of one key) in key order.
+//   k_grid        persistent workgroups take work items (<= kDefaultItemPairs pairs of one key) in key order.
 //                 A workgroup owns a 64x64 tile of all polarisations in registers (16 pixels per
 //                 lane), spreads every pair into it, and flushes once with float atomics: no per-tap
 //                 HBM atomics (GridKernel::grid, GridKernel.cc:61-83).
@@ -27,7 +27,7 @@ namespace ygrid {
 
 namespace {
 
-constexpr int kChunk = 512;  // max pairs per work item
+constexpr int kChunk = 512;  // pairs per LDS staging round of a work item
 // floats of one wave's flush area in k_grid: 4 rows x 64 columns x 4 outputs, one pad per 32
 constexpr int kFlushWave = 4 * 64 * 4 + 4 * 64 * 4 / 32;
 
@@ -693,6 +693,26 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
                                            (int)(t.bytes + t.pad), kBufferWord3);
 }
 
+// Work queues of the persistent tile kernels: the key-ordered items (w-plane-major) split into
+// kItemQueues contiguous ranges, one per XCD, so that the workgroups of one XCD work through
+// neighbouring keys and the CF planes they read stay in that XCD's L2.  A workgroup's XCD is
+// taken as blockIdx.x % kItemQueues (the dispatcher places consecutive workgroups on the XCDs in
+// turn; were it otherwise, only locality would suffer).  A workgroup takes items from its own
+// range and, once that is exhausted, from the following ones; returns -1 when all are.
+__device__ __forceinline__ int next_item(int* __restrict__ counter, int nitems, int& q, int& left, int* s_item) {
+  for (;;) {
+    const int lo = (int)((long long)nitems * q / kItemQueues);
+    const int hi = (int)((long long)nitems * (q + 1) / kItemQueues);
+    if (threadIdx.x == 0) *s_item = lo + atomicAdd(&counter[q], 1);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(*s_item);  // uniform: item data via scalar loads
+    __syncthreads();
+    if (item < hi) return item;
+    if (--left == 0) return -1;
+    q = q + 1 == kItemQueues ? 0 : q + 1;
+  }
+}
+
 // Runs consume(c, geom, pair, i) for the n staged pairs of a work item in wave w, with the CF
 // taps (and, COEF, the MFMA B operands) of pair i in c.  The descriptor of the next pair is read
 // before the current pair is worked.  PIPE: two register sets; the loads of pair i+1 are issued
@@ -769,12 +789,10 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
   const int ob = (lane >> 2) & 3;
   constexpr int NS = (NP + 1) / 2;  // output sets of two polarisations
   const int nitems = *nitems_p;
+  int queue = blockIdx.x % kItemQueues, queues_left = kItemQueues;
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(counter, 1);
-    __syncthreads();
-    const int item = __builtin_amdgcn_readfirstlane(s_item);  // uniform: item data via scalar loads
-    __syncthreads();
-    if (item >= nitems) break;
+    const int item = next_item(counter, nitems, queue, queues_left, &s_item);
+    if (item < 0) break;
     const int4 it = items[item];
     const TileJob& J = jobs[it.w];
     const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
@@ -795,10 +813,15 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
 #pragma unroll
           for (int m = 0; m < NS; ++m) acc[k][j][e][m] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    // the item's pairs in staging rounds of <= kChunk (the tile stays in registers throughout)
     const int n = it.z - it.y;
-    stage_item<true>(J.descs, J.smp, it.y, n, s_desc, s_coef, tid, blockDim.x);
-    for_each_pair<true, PIPE, CFP>(s_desc, s_coef, cfr, n, w, lu, lv, o,
-                              [&](const auto& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
+    for (int cb = 0; cb < n; cb += kChunk) {
+      const int m = min(kChunk, n - cb);
+      if (cb > 0) __syncthreads();  // every wave is done with the previous round's staging
+      stage_item<true>(J.descs, J.smp, it.y + cb, m, s_desc, s_coef, tid, blockDim.x);
+      for_each_pair<true, PIPE, CFP>(s_desc, s_coef, cfr, m, w, lu, lv, o,
+                                     [&](const auto& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
+    }
     // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of row 4k+lv, column
     // 32j+8(b&3)+2i+e.  Per (m, k) the wave writes its 4 rows x 64 columns x 4 outputs to its own
     // LDS area (the coefficient staging space, free until the next item is staged) and reads them
@@ -922,12 +945,10 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
   const int lu = lane & 15, lv = lane >> 4;
   const int slot = ((lane & 1) << 2) | (lane & 2) | ((lane >> 2) & 1);  // value held after the reduce-scatter
   const int nitems = *nitems_p;
+  int queue = blockIdx.x % kItemQueues, queues_left = kItemQueues;
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(counter, 1);
-    __syncthreads();
-    const int item = __builtin_amdgcn_readfirstlane(s_item);  // uniform: item data via scalar loads
-    __syncthreads();
-    if (item >= nitems) break;
+    const int item = next_item(counter, nitems, queue, queues_left, &s_item);
+    if (item < 0) break;
     const int4 it = items[item];
     const TileJob& J = jobs[it.w];
     const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
@@ -957,9 +978,13 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     }
 
     // per pair: wave dot product, reduce-scatter, one LDS slot per (pair, wave, value); every
-    // kDegridBatch pairs the four waves' partial sums meet and go to the per-sample accumulators
-    const int n = it.z - it.y;
-    stage_item<false>(J.descs, Samples{}, it.y, n, s_desc, nullptr, tid, blockDim.x);
+    // kDegridBatch pairs (and at the end of each staging round of <= kChunk pairs) the four
+    // waves' partial sums meet and go to the per-sample accumulators
+    const int nall = it.z - it.y;
+    for (int cb = 0; cb < nall; cb += kChunk) {
+    const int n = min(kChunk, nall - cb);
+    if (cb > 0) __syncthreads();  // every wave is done with the previous round's staging
+    stage_item<false>(J.descs, Samples{}, it.y + cb, n, s_desc, nullptr, tid, blockDim.x);
     for_each_pair<false, PIPE, CFP>(s_desc, nullptr, cfr, n, w, lu, lv, 0,
                            [&](const auto& c, const PairGeom& g, const PairU& p, int i) {
                              // a wave whose rows the footprint misses (mask 0) contributes zeros
@@ -985,6 +1010,7 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
                                __syncthreads();
                              }
                            });
+    }
   }
 }
 
@@ -1047,7 +1073,7 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
   b.offsets.resize((nkeys + 1) * sizeof(int));
   b.cursor.resize((nkeys + 1) * sizeof(int));
   b.nitems.resize(sizeof(int) * 2);
-  b.counter.resize(sizeof(int) * 2);
+  b.counter.resize(sizeof(int) * kItemQueues);
   // nitems[1] = 0: the item base of this single-job binning
   YG_HIP(hipMemsetAsync(b.nitems.get(), 0, 2 * sizeof(int), c.stream));
   int* counts = b.counts.as<int>();
@@ -1072,8 +1098,7 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
                        b.pairs.as<int4>());
   // work items: reuse cursor as the per-key item count and its scan
   int* icount = b.cursor.as<int>();
-  // the tile kernels stage a whole item in LDS: at most kChunk pairs
-  const int chunk = c.item_pairs > 0 ? std::min(c.item_pairs, kChunk) : kChunk;
+  const int chunk = c.item_pairs > 0 ? c.item_pairs : kDefaultItemPairs;
   hipLaunchKernelGGL(k_item_count, dim3((nkeys + 1 + 255) / 256), dim3(256), 0, c.stream, counts, (int)nkeys, chunk,
                      icount);
   DevBuf& ioffsets = b.ioffsets;
@@ -1104,7 +1129,7 @@ void size_static_bins(const Ctx& c, StaticBins& sb, int nrow, int nchan, int nx,
   YG_CHECK(cap < (size_t)INT_MAX, 1, "too many (sample, tile) pairs for one pass");
   sb.nkeys = nkeys;
   sb.npairs_cap = cap;
-  sb.chunk = std::min(std::max(chunk, 16), kChunk);
+  sb.chunk = std::max(chunk, 16);
   sb.nitems_cap = cap / sb.chunk + std::min(nkeys, cap) + 1;
   sb.nrow = nrow;
   sb.nchan = nchan;
@@ -1293,7 +1318,7 @@ void launch_fill_u32(const Ctx& c, void* p, size_t n, uint32_t value) {
 void launch_tile_jobs(const Ctx& c, const TileVariant& v0, bool forward, int npol, int prec, const TileJob* jobs,
                       const int4* items, const int* nitems, int* counter) {
   const TileVariant v = prec == kPrecF32 ? v0 : TileVariant{2, 1};
-  launch_fill_u32(c, counter, 1, 0u);
+  launch_fill_u32(c, counter, kItemQueues, 0u);
   const dim3 g(persistent_blocks(c, v));
   with_tile_kernels(v, npol, prec, [&](auto k) { k.run(c.stream, g, forward, items, nitems, counter, jobs); });
   YG_HIP(hipGetLastError());

@@ -16,6 +16,11 @@ constexpr double kPi = 3.14159265358979323846;
 
 // Grid tile owned by one workgroup in the grid/degrid kernels.
 constexpr int kTile = 64;
+// default pairs per tile-kernel work item (pairs of one grid tile and key; the tile is loaded or
+// flushed once per item)
+constexpr int kDefaultItemPairs = 2048;
+// work-item queues of a tile-kernel launch (one per XCD; their counters live in one int array)
+constexpr int kItemQueues = 8;
 // Guard band of zeros around every CF plane in device memory so that the 16x4-lane
 // blocks of the grid/degrid kernels may read past the (2S+1)^2 footprint without a
 // per-lane bounds test: columns -kGuardU..2S+kGuardU-1, rows -kGuardV..2S+kGuardV-1.
