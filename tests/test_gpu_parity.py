@@ -478,3 +478,32 @@ def test_binning_cache_reuse_and_invalidation(gpu_ctx):
         ref.initialise_grid(img)
         ref.grid(pchunk(d2))
         assert rel(g.grid_array(), ref.grid_array()) < 1e-6
+
+
+def test_dense_tiles_span_staging_rounds_and_items(gpu_ctx):
+    """Every sample within ~3 pixels of one tile corner: ~4000 pairs per (tile, key), so that a
+    work item is staged in several rounds of 512 pairs and a key is split over several items
+    (taken from several XCD queues)."""
+    announce("dense tile corner, SphFunc os 2, 4 pols, grid + degrid vs oracle")
+    import yandasoft_amd as ya
+    N, cell, npol = 256, 30 * ARC, 4
+    d = make_data(2500, 2, npol, N, cell, frac=0.02, seed=32)
+    params = {"oversample": 2}
+    okw = dict(kind="SphFunc", sph_oversample=2)
+    img = ya.ImageDesc(N, N, npol, 1, cell, cell)
+    g = ya.Gridder(gpu_ctx, "GPUSphFunc", params)
+    g.initialise_grid(img)
+    g.grid(pchunk(d))
+    o = O.OracleGridder(precision="f32", **okw)
+    o.initialise_grid((N, N, npol, 1), (cell, cell))
+    o.grid(ochunk(d))
+    assert g.stats().samples_gridded == o.counters.samples_gridded
+    assert rel(g.grid_array().reshape(1, npol, N, N), o.grid_) < TOL
+    model = random_model(N, npol)
+    dg = ya.Gridder(gpu_ctx, "GPUSphFunc", params)
+    dg.initialise_degrid(img, model.astype(np.float32))
+    out = np.zeros_like(d["vis"])
+    dg.degrid(pchunk(d, vis_out=out))
+    od = O.OracleGridder(precision="f32", **okw)
+    od.initialise_degrid((N, N, npol, 1), (cell, cell), model)
+    assert rel(np.transpose(out, (2, 1, 0)), od.degrid(ochunk(d))) < TOL
