@@ -371,7 +371,8 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
   const int nx = shape_[0], ny = shape_[1], npol = shape_[2], nchan = shape_[3], ncp = npol * nchan;
   const int cnx = std::min(maxsupport_, nx), cny = std::min(maxsupport_, ny);
   std::vector<double> sw((size_t)nsw_rows_ * ncp);
-  YG_HIP(hipMemcpy(sw.data(), sumw_.get(), sw.size() * sizeof(double), hipMemcpyDeviceToHost));
+  YG_HIP(hipMemcpyAsync(sw.data(), sumw_.get(), sw.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  YG_HIP(hipStreamSynchronize(c.stream));
   DevBuf cout((size_t)cnx * cny * ncp * sizeof(double));
   YG_HIP(hipMemsetAsync(cout.get(), 0, (size_t)cnx * cny * ncp * sizeof(double), c.stream));
   DevBuf dtx, dty;
@@ -417,6 +418,8 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
         sup[b] = cf_->supp[k];
         for (int cp = 0; cp < ncp; ++cp) w[(size_t)b * ncp + cp] = sw[z + (size_t)nsw_rows_ * cp];
       }
+      // the previous batch's kernels read doff/dsup/dw: wait for them before overwriting
+      if (r0 > 0) YG_HIP(hipStreamSynchronize(c.stream));
       YG_HIP(hipMemcpy(doff.get(), off.data(), cnt * 4, hipMemcpyHostToDevice));
       YG_HIP(hipMemcpy(dsup.get(), sup.data(), cnt * 4, hipMemcpyHostToDevice));
       YG_HIP(hipMemcpy(dzero.get(), zero.data(), cnt * 4, hipMemcpyHostToDevice));

@@ -612,6 +612,7 @@ void Gridder::getCF(int k, float* out, int* support) const {
   if (support) *support = S;
   if (!out) return;
   std::vector<float2> host((size_t)P * cf_rows(S));
+  YG_HIP(hipStreamSynchronize(ctx_->stream));  // the table may still be filling on the context stream
   YG_HIP(hipMemcpy(host.data(), cf_->table.as<float2>() + cf_->off[k], host.size() * sizeof(float2), hipMemcpyDeviceToHost));
   for (int iy = 0; iy < W; ++iy)
     for (int ix = 0; ix < W; ++ix) {
@@ -626,6 +627,7 @@ void Gridder::getGrid(float* out, size_t* n) const {
   if (n) *n = 2 * ng;
   if (!out) return;
   YG_CHECK(grid_.get(), 2, "no grid");
+  YG_HIP(hipStreamSynchronize(ctx_->stream));  // passes still in flight on the context stream
   YG_HIP(hipMemcpy(out, grid_.get(), ng * sizeof(float2), hipMemcpyDeviceToHost));
 }
 

@@ -149,6 +149,7 @@ double LearnedCF::loss_and_grads_host(const float* x, const float* t, int n, flo
   YG_CHECK(n > 0, 1, "no examples");
   ensure_train_buffers(n);
   DevBuf T((size_t)n * 2 * sizeof(float));
+  YG_HIP(hipStreamSynchronize(ctx_->stream));  // xs_ may still be read by earlier work on the stream
   YG_HIP(hipMemcpy(xs_.get(), x, (size_t)n * kLcfIn * sizeof(float), hipMemcpyHostToDevice));
   YG_HIP(hipMemcpy(T.get(), t, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice));
   forward_dev(xs_.as<float>(), n, false, true, y_.as<float>());
