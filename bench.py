@@ -227,10 +227,14 @@ def main():
     dk_ms = s1d.ms_degrid_kernel / max(1.0, s1d.n_degrid_launches)
     per_launch_g = taps_g / args.steps
     per_launch_d = taps_d / args.steps
-    # kernel names as the profiler prints them: k_grid<pols, waves per SIMD, pipelined, CF precision>
+    # kernel names as the profiler prints them: k_grid<pols, waves per SIMD, pipelined, CF precision>,
+    # k_degrid_mm<pols, CF precision> (matrix-core degridding)
     tf = lambda b: "true" if b else "false"
     gname = f"k_grid<{wl.npol}, {s1g.grid_waves}, {tf(s1g.grid_pipeline)}, {s1g.cf_precision}>"
-    dname = f"k_degrid<{wl.npol}, {s1d.degrid_waves}, {tf(s1d.degrid_pipeline)}, {s1d.cf_precision}>"
+    if s1d.degrid_matrix:
+        dname = f"k_degrid_mm<{wl.npol}, {s1d.cf_precision}>"
+    else:
+        dname = f"k_degrid<{wl.npol}, {s1d.degrid_waves}, {tf(s1d.degrid_pipeline)}, {s1d.cf_precision}>"
     if gk_ms >= dk_ms:
         kname, kms, kt = gname, gk_ms, per_launch_g
     else:

@@ -94,6 +94,9 @@ typedef struct {
   int cf_precision, vis_precision;
   /* 1 when the gridder's CFs come from an attached learned network (ygrid_gridder_set_learned_cf) */
   int learned_cf_active;
+  /* 1 when degridding runs on the matrix cores, four samples at once (2 or more polarisations;
+   * environment YGRID_DEGRID_VARIANT="w,p,0" selects the VALU form) */
+  int degrid_matrix;
 } ygrid_stats;
 
 /* ---- library / context ---- */

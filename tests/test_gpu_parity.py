@@ -507,3 +507,25 @@ def test_dense_tiles_span_staging_rounds_and_items(gpu_ctx):
     od = O.OracleGridder(precision="f32", **okw)
     od.initialise_degrid((N, N, npol, 1), (cell, cell), model)
     assert rel(np.transpose(out, (2, 1, 0)), od.degrid(ochunk(d))) < TOL
+
+
+@pytest.mark.parametrize("npol", [2, 3, 4])
+@pytest.mark.parametrize("matrix", [1, 0])
+def test_degrid_forms_match_oracle(monkeypatch, matrix, npol):
+    """Both degridding forms against the oracle on W-projection data (supports up to ~20, so
+    that footprints cover tiles partially and several tiles): the matrix-core form (four
+    samples per MFMA group, the default for 2 or more polarisations) and the VALU form
+    (one sample per wave), selected through the context's environment."""
+    announce(f"degrid form matrix={matrix} npol={npol}")
+    import yandasoft_amd as ya
+    monkeypatch.setenv("YGRID_DEGRID_VARIANT", f"2,1,{matrix}")
+    ctx = ya.Context(0)
+    try:
+        N, cell = 256, 30 * ARC
+        d = make_data(900, 3, npol, N, cell, f0=1.0e9, frac=0.35, wmax_m=2500.0 * O.C_LIGHT / 1.005e9, seed=40 + npol)
+        gv, ov, g, _ = degrid_both(ctx, "GPUWProject", dict(WP), dict(kind="WProject", wproject=_wp_oracle_params()),
+                                   d, N, cell, npol, random_model(N, npol, seed=npol))
+        assert g.stats().degrid_matrix == matrix
+        assert rel(gv, ov) < TOL
+    finally:
+        ctx.close()

@@ -55,7 +55,8 @@ class Stats(ctypes.Structure):
                 ("n_degrid_launches", ctypes.c_double), ("n_bin_reused", ctypes.c_double),
                 ("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int),
                 ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int),
-                ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int), ("learned_cf_active", ctypes.c_int)]
+                ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int), ("learned_cf_active", ctypes.c_int),
+                ("degrid_matrix", ctypes.c_int)]
 
 
 class Tuning(ctypes.Structure):

@@ -196,7 +196,7 @@ BatchTuning Batch::autotune(int trials) {
       invalidate();
       for (const TileVariant& v : vars) {
         ctx_->grid_variant = v;
-        ctx_->degrid_variant = v;
+        ctx_->degrid_variant = TileVariant{v.waves, v.pipeline, d0.matrix};
         double t = 1e300;
         for (int r = 0; r < trials; ++r) {
           run();
@@ -219,8 +219,8 @@ BatchTuning Batch::autotune(int trials) {
     invalidate();
     throw;
   }
-  ctx_->grid_variant = TileVariant{best.grid_waves, best.grid_pipeline};
-  ctx_->degrid_variant = TileVariant{best.degrid_waves, best.degrid_pipeline};
+  ctx_->grid_variant = TileVariant{best.grid_waves, best.grid_pipeline, 0};
+  ctx_->degrid_variant = TileVariant{best.degrid_waves, best.degrid_pipeline, d0.matrix};
   ctx_->item_pairs = best.item_pairs;
   best.trials = n_tuned_;
   graph_on_ = graph;

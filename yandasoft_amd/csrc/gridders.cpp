@@ -603,6 +603,7 @@ void Gridder::stats(ygrid_stats* o) const {
   o->cf_precision = cf_prec_;
   o->vis_precision = vis_prec_;
   o->learned_cf_active = learnedActive() ? 1 : 0;
+  o->degrid_matrix = ctx_->degrid_variant.matrix && shape_[2] >= 2 ? 1 : 0;
 }
 
 void Gridder::getCF(int k, float* out, int* support) const {

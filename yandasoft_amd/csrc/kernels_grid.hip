@@ -1014,6 +1014,239 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
   }
 }
 
+// ---- Degridding on the matrix cores: four pairs at once ------------------------------------
+//
+// Lane l = 4b + j of a wave: MFMA block b covers the tile columns 32h + 2b + e (e = 0, 1) of one
+// row of the wave's 16, j selects one pair of the group of four (the A operand: that pair's CF
+// tap at the pixel) and one output (the B operand: the model-grid value at the pixel; j = 0, 1:
+// re, im of polarisation 2m; j = 2, 3: of 2m + 1).  One v_mfma_f32_4x4x1_16b_f32 then forms
+// for 16 pixels every product of the four pairs' taps with the four outputs, all of them
+// needed: per block, D1[m][i][j] accumulates C.re(pair i) * B[j] and D2[m][i][j] C.im(pair i)
+// * B[j] over the wave's pixels, and re/im of C * conj(G) (GridKernel.cc:121-122) follow as
+// D1[j] + D2[j^1] (j even) and D2[j^1] - D1[j] (j odd).  A lane loads its pair's two taps of
+// one row and column pair (16 bytes for FP32 tables) where that pair's footprint covers the
+// block, zeros elsewhere; a (row block, half) that none of the four pairs covers is skipped.
+// (Loading coalesced instead, 16 lanes per pair, and moving the taps to the MFMA lanes with
+// ds_bpermute was measured slower: the permutes cost more than the coalescing saves.)  The
+// four pairs of a group are neighbours in binning order, which follows the sample order within
+// a bin (consecutive rows of one channel), so their footprints mostly cover the same blocks;
+// sorting each staging round by footprint in LDS was measured slower than this order.
+
+// Geometry of group g0 (pairs g0 .. g0 + 3) for the loads of lane (c, s) = (lane >> 2,
+// lane & 3): the lane's pair is g0 + s, its column pair 2c, 2c + 1 of each half.
+struct QuadGeom {
+  uint32_t base;  // byte offset of the lane's pair's tap under the wave's first pixel
+  uint32_t rowb;  // bytes per CF row of that pair's plane
+  uint32_t mask;  // that pair's blocks (k, h) of this wave, bit 2k + h
+  uint32_t um;    // union of the four pairs' masks (wave-uniform)
+  int sid;        // that pair's sample
+  bool c0, c1;    // the lane's column pair (2c, 2c + 1 of half 0, 1) touches that footprint
+};
+
+template <int SH>
+__device__ __forceinline__ QuadGeom quad_geom(const int4* s_desc, int g0, int n, int w, int s, int c) {
+  QuadGeom q;
+  const bool valid = g0 + s < n;
+  const int4 d = s_desc[valid ? g0 + s : g0];
+  const uint32_t pw = (uint32_t)d.w;
+  const int P = (int)(pw & 0xffffu), A = (int)(pw << 8) >> 24, B = (int)pw >> 24;
+  q.mask = valid ? ((uint32_t)d.z >> (8 * w)) & 0xffu : 0u;
+  q.base = (uint32_t)d.y + (uint32_t)w * ((uint32_t)P << (4 + SH));  // 16 rows per wave
+  q.rowb = (uint32_t)P << SH;
+  q.sid = d.x;
+  q.c0 = (2 * c >= A) && (2 * c <= B);
+  q.c1 = (32 + 2 * c >= A) && (32 + 2 * c <= B);
+  q.um = (uint32_t)__builtin_amdgcn_readlane((int)q.mask, 0) | (uint32_t)__builtin_amdgcn_readlane((int)q.mask, 1) |
+         (uint32_t)__builtin_amdgcn_readlane((int)q.mask, 2) | (uint32_t)__builtin_amdgcn_readlane((int)q.mask, 3);
+  return q;
+}
+
+template <int CFP>
+constexpr int quad_dwords() {
+  return CFP == kPrecF32 ? 4 : 2;
+}
+
+// The taps of row block k (rows 4k .. 4k + 3 of the wave, both halves) of the lane's pair and
+// column pair c.
+template <int CFP>
+__device__ __forceinline__ void load_quad(uint32_t (&t)[4][2][quad_dwords<CFP>()], __amdgpu_buffer_rsrc_t cfr,
+                                          const QuadGeom& q, int k, int c) {
+  constexpr int SH = cf_shift<CFP>();
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+    if (q.um & (1u << (2 * k + h))) {
+      const bool on = ((q.mask >> (2 * k + h)) & 1u) && (h ? q.c1 : q.c0);
+      const uint32_t col = (uint32_t)(32 * h + 2 * c) << SH;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t vo = on ? q.base + (uint32_t)(4 * k + r) * q.rowb + col : kOutOfRange;
+#pragma unroll
+        for (int i = 0; i < quad_dwords<CFP>(); ++i)
+          t[r][h][i] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo + 4u * i, 0, 0);
+      }
+    }
+}
+
+template <int NP, int CFP>
+__device__ __forceinline__ void mfma_quad(f32x4 (&D1)[(NP + 1) / 2], f32x4 (&D2)[(NP + 1) / 2],
+                                          const uint32_t (&t)[4][2][quad_dwords<CFP>()],
+                                          const float (&gt)[16][2][2][(NP + 1) / 2], uint32_t um, int k) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+    if (um & (1u << (2 * k + h))) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          float cr, ci;
+          if constexpr (CFP == kPrecF32) {
+            cr = __builtin_bit_cast(float, t[r][h][2 * e]);
+            ci = __builtin_bit_cast(float, t[r][h][2 * e + 1]);
+          } else {
+            cr = widen16(t[r][h][e] & 0xffffu, CFP);
+            ci = widen16(t[r][h][e] >> 16, CFP);
+          }
+#pragma unroll
+          for (int m = 0; m < (NP + 1) / 2; ++m) {
+            D1[m] = __builtin_amdgcn_mfma_f32_4x4x1f32(cr, gt[4 * k + r][h][e][m], D1[m], 0, 0, 0);
+            D2[m] = __builtin_amdgcn_mfma_f32_4x4x1f32(ci, gt[4 * k + r][h][e][m], D2[m], 0, 0, 0);
+          }
+        }
+    }
+}
+
+template <int NP, int CFP>
+__global__ __launch_bounds__(256, 2) void k_degrid_mm(const int4* __restrict__ items, const int* __restrict__ nitems_p,
+                                                    int* __restrict__ counter, const TileJob* __restrict__ jobs) {
+  constexpr int SH = cf_shift<CFP>();
+  constexpr int M = (NP + 1) / 2;
+  constexpr int ND = quad_dwords<CFP>();
+  __shared__ int s_item;
+  __shared__ int4 s_desc[kChunk];
+  __shared__ float s_part[kDegridBatch][4][8];
+  __shared__ int s_sid[kDegridBatch];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int b = lane >> 2, j = lane & 3;  // MFMA block (column pair of the loads), pair / output
+  const float sgn = (j & 1) ? -1.f : 1.f;
+  const int nitems = *nitems_p;
+  int queue = blockIdx.x % kItemQueues, queues_left = kItemQueues;
+  for (;;) {
+    const int item = next_item(counter, nitems, queue, queues_left, &s_item);
+    if (item < 0) break;
+    const int4 it = items[item];
+    const TileJob& J = jobs[it.w];
+    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
+    const int nx = J.nx, ny = J.ny;
+    const int tile = it.x % J.ntiles;
+    const int kz = it.x / J.ntiles;
+    const int ichan = kz / J.nzg;
+    const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
+    const size_t plane = (size_t)nx * ny;
+    const float* __restrict__ gridf = J.grid;
+    float* __restrict__ acc_out = J.acc;
+
+    // model-grid tile: gt[r][h][e][m] = output j of polarisation pair m at tile pixel
+    // (16w + r, 32h + 2b + e)
+    float gt[16][2][2][M];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int v = v0 + 16 * w + r;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int u = u0 + 32 * h + 2 * b + e;
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            const int q = 2 * m + (j >> 1);
+            gt[r][h][e][m] = (u < nx && v < ny && q < NP)
+                                 ? gridf[2 * ((size_t)(ichan * NP + q) * plane + (size_t)v * nx + u) + (j & 1)]
+                                 : 0.f;
+          }
+        }
+    }
+
+    const int nall = it.z - it.y;
+    for (int cb = 0; cb < nall; cb += kChunk) {
+      const int n = min(kChunk, nall - cb);
+      if (cb > 0) __syncthreads();  // every wave is done with the previous round's staging
+      stage_item<false>(J.descs, Samples{}, it.y + cb, n, s_desc, nullptr, tid, blockDim.x);
+      uint32_t tA[4][2][ND], tB[4][2][ND];
+      QuadGeom lq = quad_geom<SH>(s_desc, 0, n, w, j, b);
+      load_quad<CFP>(tA, cfr, lq, 0, b);
+      for (int g0 = 0; g0 < n; g0 += 4) {
+        const QuadGeom cq = lq;  // this group; the taps of its row block 0 are in flight into tA
+        f32x4 D1[M], D2[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          D1[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+          D2[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        load_quad<CFP>(tB, cfr, cq, 1, b);
+        mfma_quad<NP, CFP>(D1, D2, tA, gt, cq.um, 0);
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        load_quad<CFP>(tA, cfr, cq, 2, b);
+        mfma_quad<NP, CFP>(D1, D2, tB, gt, cq.um, 1);
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        load_quad<CFP>(tB, cfr, cq, 3, b);
+        mfma_quad<NP, CFP>(D1, D2, tA, gt, cq.um, 2);
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        if (g0 + 4 < n) {
+          lq = quad_geom<SH>(s_desc, g0 + 4, n, w, j, b);
+          load_quad<CFP>(tA, cfr, lq, 0, b);
+        }
+        mfma_quad<NP, CFP>(D1, D2, tB, gt, cq.um, 3);
+
+        // lane (b, j), value 4m + i: re/im (j even/odd) of pair g0 + i, polarisation
+        // 2m + (j >> 1), summed over block b; then summed over the 16 blocks
+        float val[4 * M];
+#pragma unroll
+        for (int t = 0; t < 4 * M; ++t) val[t] = 0.f;
+        if (cq.um) {
+#pragma unroll
+          for (int m = 0; m < M; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) val[4 * m + i] = fmaf(sgn, D1[m][i], dpp<0xB1>(D2[m][i]));  // lane j^1
+#pragma unroll
+          for (int t = 0; t < 4 * M; ++t) {
+            float r = val[t];
+            r += dpp<0x124>(r);  // row_ror:4
+            r += dpp<0x128>(r);  // row_ror:8: the four blocks of the row
+            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(r), __float_as_uint(r), false, false);
+            r = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+            const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(r), __float_as_uint(r), false, false);
+            val[t] = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+          }
+        }
+        const int gn = min(4, n - g0);
+        if (lane < 4) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (i < gn) {
+#pragma unroll
+              for (int m = 0; m < M; ++m) s_part[(g0 + i) % kDegridBatch][w][4 * m + j] = val[4 * m + i];
+            }
+        }
+        if (w == 0 && b == 0 && j < gn) s_sid[(g0 + j) % kDegridBatch] = cq.sid;  // lane j: pair j
+        const int gend = g0 + gn;
+        if (gend % kDegridBatch == 0 || gend == n) {
+          __syncthreads();
+          const int nb = gend - (g0 / kDegridBatch) * kDegridBatch;
+          for (int t = tid; t < nb * 2 * NP; t += blockDim.x) {
+            const int bb = t / (2 * NP), q = t % (2 * NP);
+            const float sum = s_part[bb][0][q] + s_part[bb][1][q] + s_part[bb][2][q] + s_part[bb][3][q];
+            if (sum != 0.f) atomicAdd(&acc_out[(size_t)s_sid[bb] * 8 + q], sum);
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+}
+
 __global__ void k_degrid_finish(Samples smp, const float* __restrict__ acc, int nrow, int nchan, int npol,
                                 float2* __restrict__ vis_out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1315,10 +1548,40 @@ void launch_fill_u32(const Ctx& c, void* p, size_t n, uint32_t value) {
   YG_HIP(hipGetLastError());
 }
 
+template <int NP>
+static void launch_degrid_mm(hipStream_t s, dim3 g, int prec, const int4* items, const int* nitems, int* counter,
+                             const TileJob* jobs) {
+  switch (prec) {
+    case kPrecF32:
+      hipLaunchKernelGGL((k_degrid_mm<NP, kPrecF32>), g, dim3(256), 0, s, items, nitems, counter, jobs);
+      break;
+    case kPrecF16:
+      hipLaunchKernelGGL((k_degrid_mm<NP, kPrecF16>), g, dim3(256), 0, s, items, nitems, counter, jobs);
+      break;
+    case kPrecBF16:
+      hipLaunchKernelGGL((k_degrid_mm<NP, kPrecBF16>), g, dim3(256), 0, s, items, nitems, counter, jobs);
+      break;
+    default: throw Error(1, "unknown CF precision");
+  }
+}
+
 void launch_tile_jobs(const Ctx& c, const TileVariant& v0, bool forward, int npol, int prec, const TileJob* jobs,
                       const int4* items, const int* nitems, int* counter) {
-  const TileVariant v = prec == kPrecF32 ? v0 : TileVariant{2, 1};
   launch_fill_u32(c, counter, kItemQueues, 0u);
+  // one polarisation: half of the matrix form's outputs would be zeros; the VALU form is used
+  if (forward && v0.matrix && npol >= 2) {
+    const dim3 g(c.n_cu * 2);  // two workgroups (8 waves) per CU: <= 256 VGPRs per wave
+    switch (npol) {
+      case 1: launch_degrid_mm<1>(c.stream, g, prec, items, nitems, counter, jobs); break;
+      case 2: launch_degrid_mm<2>(c.stream, g, prec, items, nitems, counter, jobs); break;
+      case 3: launch_degrid_mm<3>(c.stream, g, prec, items, nitems, counter, jobs); break;
+      case 4: launch_degrid_mm<4>(c.stream, g, prec, items, nitems, counter, jobs); break;
+      default: throw Error(1, "unsupported number of polarisations (1..4)");
+    }
+    YG_HIP(hipGetLastError());
+    return;
+  }
+  const TileVariant v = prec == kPrecF32 ? v0 : TileVariant{2, 1, 0};
   const dim3 g(persistent_blocks(c, v));
   with_tile_kernels(v, npol, prec, [&](auto k) { k.run(c.stream, g, forward, items, nitems, counter, jobs); });
   YG_HIP(hipGetLastError());

@@ -65,11 +65,13 @@ int ygrid_ctx_create(int device, ygrid_ctx** out) {
     hipDeviceProp_t prop;
     YG_HIP(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    // tile-kernel variants "waves,pipeline" (e.g. "3,0"): performance tuning only
+    // tile-kernel variants "waves,pipeline[,matrix]" (e.g. "3,0" or "2,1,0"): performance
+    // tuning only; matrix selects the matrix-core degridding (on unless given as 0)
     const auto variant = [](const char* env, ygrid::TileVariant& v) {
       if (const char* s = std::getenv(env)) {
-        int w = 0, p = 0;
-        if (std::sscanf(s, "%d,%d", &w, &p) == 2 && (w == 2 || w == 3)) v = ygrid::TileVariant{w, p != 0};
+        int w = 0, p = 0, m = v.matrix;
+        if (std::sscanf(s, "%d,%d,%d", &w, &p, &m) >= 2 && (w == 2 || w == 3))
+          v = ygrid::TileVariant{w, p != 0, m != 0};
       }
     };
     variant("YGRID_GRID_VARIANT", c->grid_variant);

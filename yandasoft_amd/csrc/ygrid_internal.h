@@ -208,10 +208,12 @@ struct Binning {
 // hipStream + device id of a context.
 // Register budget and pipelining of a tile kernel: waves = waves per SIMD the kernel is built
 // for (2: <= 256 VGPRs, 3: <= 168), pipeline = CF loads of pair i+1 issued before pair i's
-// arithmetic (a second register set).
+// arithmetic (a second register set).  matrix (degridding only): the matrix-core form that
+// works four pairs at once (k_degrid_mm; waves and pipeline do not apply to it).
 struct TileVariant {
   int waves;
   int pipeline;
+  int matrix;
 };
 
 // The binning of the last pass run on a context, with the sample headers and keys it was built
@@ -230,7 +232,7 @@ struct Ctx {
   bool own_stream = false;
   int n_cu = 256;
   TileVariant grid_variant{2, 1};
-  TileVariant degrid_variant{2, 1};
+  TileVariant degrid_variant{2, 1, 1};
   int item_pairs = 0;  // pairs per work item of the tile kernels (0: the built-in default)
   BinCache bins;  // shared by the gridders of the context (their passes are ordered on stream)
 };
