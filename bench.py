@@ -133,7 +133,8 @@ def main():
     ap.add_argument("--cf-precision", default="fp32", choices=["fp32", "fp16", "bf16"],
                     help="wproject_array: CF tap and coefficient storage (the metric is quoted for fp32)")
     ap.add_argument("--workload", default="wproject_array",
-                    choices=["wproject_array", "sphfunc_disc", "awproject_feeds", "batched_beams", "learned_wdisc"])
+                    choices=["wproject_array", "sphfunc_disc", "awproject_feeds", "batched_beams", "learned_wdisc",
+                             "imaging_loop"])
     ap.add_argument("--train-epochs", type=int, default=200, help="learned_wdisc: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -160,6 +161,9 @@ def main():
         return
     if args.workload == "sphfunc_disc":
         bench_sphfunc(args)
+        return
+    if args.workload == "imaging_loop":
+        bench_imaging(args)
         return
     import yandasoft_amd as ya
     from yandasoft_amd import workloads as W
@@ -410,6 +414,86 @@ def bench_awproject(args):
         log("cpu baseline ...")
         result["cpu_baseline"] = aw_cpu_baseline(wl)
     print(json.dumps(result), flush=True)
+
+
+def bench_imaging(args):
+    """The imaging equation's data loop (ygrid_imaging_chunk, ImageFFTEquation.cc:437-480) on the
+    W-projection array workload: per step one model degridder predicts, the residual is formed on
+    the device, and a residual and a PSF gridder grid it (one call per step)."""
+    import torch
+    import yandasoft_amd as ya
+    from yandasoft_amd import workloads as W
+
+    wl = W.wproject_array(nint=args.nint)
+    dev = torch.device("cuda:0")
+    t_uvw = torch.from_numpy(np.ascontiguousarray(wl.uvw)).to(dev)
+    t_freq = torch.from_numpy(np.ascontiguousarray(wl.freq)).to(dev)
+    t_vis = torch.from_numpy(np.ascontiguousarray(wl.vis).view(np.float32)).to(dev)
+    t_noise = torch.from_numpy(np.ascontiguousarray(wl.noise).view(np.float32)).to(dev)
+    torch.cuda.synchronize()
+    chunk = ya.Chunk(uvw=t_uvw, freq=t_freq, vis=t_vis, noise=t_noise, device=True)
+    chunk.npol = wl.npol
+    ctx = ya.Context(0)
+    img = ya.ImageDesc(wl.nx, wl.ny, wl.npol, 1, wl.cell, wl.cell)
+    prm = dict(wl.params)
+    dg, res, psf = (ya.Gridder(ctx, "GPUWProject", prm) for _ in range(3))
+    rng = np.random.default_rng(3)
+    model = np.zeros((1, wl.npol, wl.ny, wl.nx), np.float32)
+    for _ in range(50):
+        model[0, :, rng.integers(500, 3500), rng.integers(500, 3500)] = rng.normal(size=wl.npol)
+    t_model = torch.from_numpy(model).to(dev)
+    torch.cuda.synchronize()
+    dg.initialise_degrid(img, t_model)
+
+    def step():
+        ctx.clear_cache()  # nothing carries over between steps: each step bins once
+        res.initialise_grid(img)
+        psf.initialise_grid(img, dopsf=True)
+        ya.imaging_chunk([dg], [res, psf], chunk)
+
+    t0 = time.perf_counter()
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i} done ({time.perf_counter() - t0:.1f} s)")
+    ctx.synchronize()
+    s0 = [g.stats() for g in (dg, res, psf)]
+    for g in (dg, res, psf):
+        g.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"step {i} done")
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s1 = [g.stats() for g in (dg, res, psf)]
+    taps_d = s1[0].taps_degridded - s0[0].taps_degridded
+    taps_g = sum(s1[k].taps_gridded - s0[k].taps_gridded for k in (1, 2))
+    dk_ms = s1[0].ms_degrid_kernel / max(1.0, s1[0].n_degrid_launches)
+    gk_ms = sum(s1[k].ms_grid_kernel for k in (1, 2)) / max(1.0, sum(s1[k].n_grid_launches for k in (1, 2)))
+    tf = lambda b: "true" if b else "false"
+    if dk_ms >= gk_ms:
+        kname, kms, kt = f"k_degrid_mm<{wl.npol}, 0>" if s1[0].degrid_matrix else \
+            f"k_degrid<{wl.npol}, {s1[0].degrid_waves}, {tf(s1[0].degrid_pipeline)}, 0>", dk_ms, taps_d / args.steps
+    else:
+        kname, kms, kt = f"k_grid<{wl.npol}, {s1[1].grid_waves}, {tf(s1[1].grid_pipeline)}, 0>", gk_ms, \
+            taps_g / (2 * args.steps)
+    achieved = 8.0 * kt / (kms * 1e-3) / 1e12
+    other_ms = lambda f: sum(getattr(s1[k], f) - getattr(s0[k], f) for k in range(3)) / args.steps
+    print(json.dumps({
+        "metric": "CF-tap updates/s (imaging loop: degrid model, residual, grid residual + PSF)",
+        "value": (taps_d + taps_g) / elapsed, "unit": "CF-tap updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "imaging_loop: wproject_array data (189000 rows x 64 ch x 4 pol, 4096^2, 129 "
+                               "w-planes, S = 64), 1 model degridder + residual and PSF gridders, one "
+                               "ygrid_imaging_chunk call per step", "parallelism": "1 GPU"},
+        "phases_ms_per_step": {"degrid_kernel": dk_ms, "grid_kernel_each": gk_ms,
+                               "prep": other_ms("ms_prep"), "binning": other_ms("ms_bin")},
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None},
+    }), flush=True)
 
 
 def bench_sphfunc(args):

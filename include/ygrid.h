@@ -156,6 +156,18 @@ int ygrid_get_cf(const ygrid_gridder *g, int k, float *out, int *support);
 /* Copy the padded grid (complex, casacore layout) to host; *n receives the float count. */
 int ygrid_get_grid(const ygrid_gridder *g, float *out, size_t *n);
 
+/* ---- imaging-equation data loop ----
+ * One chunk of ImageFFTEquation::calcImagingEquations' loop over the data
+ * (ImageFFTEquation.cc:437-480), device-resident: the model visibilities are the sum of the
+ * predictions of the `degridders` (initialised with initialise_degrid; those whose model is
+ * empty add nothing, `:445-455`), the residual is data - model (`:464-465`), and every one of
+ * the `gridders` (initialised with initialise_grid: residual, PSF and preconditioner gridders,
+ * `:467-476`) grids that residual.  chunk->vis holds the data; chunk->vis_out, when not NULL,
+ * receives the residual cube (the model is not added to it); chunk->mem applies to every array.
+ * All gridders must belong to one context; either list may be empty. */
+int ygrid_imaging_chunk(ygrid_gridder *const *degridders, int ndegrid, ygrid_gridder *const *gridders, int ngrid,
+                        const ygrid_chunk *chunk);
+
 /* ---- batched passes: no counterpart in the reference, whose imaging equation runs one gridder
  * after the other (ImageFFTEquation.cc:362-387).  For callers that grid or degrid many images per
  * major cycle (beams x channels x polarisations of uneven sizes).

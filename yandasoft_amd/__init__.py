@@ -16,6 +16,7 @@ from .abi import (  # noqa: F401
     ImageDesc,
     LearnedCF,
     YgridError,
+    imaging_chunk,
     lib,
     library_path,
 )

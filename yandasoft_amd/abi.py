@@ -21,6 +21,7 @@ EXPORTED = [
     "ygrid_batch_autotune", "ygrid_batch_info",
     "ygrid_lcf_create", "ygrid_lcf_destroy", "ygrid_lcf_num_params", "ygrid_lcf_get_params", "ygrid_lcf_set_params",
     "ygrid_lcf_loss_grad", "ygrid_lcf_forward", "ygrid_lcf_train_wproject", "ygrid_gridder_set_learned_cf",
+    "ygrid_imaging_chunk",
 ]
 
 
@@ -305,6 +306,16 @@ class Gridder:
             self.close()
         except Exception:
             pass
+
+
+def imaging_chunk(degridders, gridders, chunk: Chunk):
+    """One chunk of the imaging equation's data loop (ImageFFTEquation.cc:437-480) on the
+    device: residual = chunk.vis - sum of the degridders' predictions, gridded by every gridder;
+    chunk.vis_out (optional) receives the residual."""
+    D = (ctypes.c_void_p * max(1, len(degridders)))(*[g.h.value for g in degridders])
+    G = (ctypes.c_void_p * max(1, len(gridders)))(*[g.h.value for g in gridders])
+    c = chunk.c()
+    _check(lib().ygrid_imaging_chunk(D, len(degridders), G, len(gridders), ctypes.byref(c)))
 
 
 class Batch:

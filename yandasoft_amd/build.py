@@ -31,6 +31,7 @@ SOURCES = [
     "awproject.cpp",
     "batch.cpp",
     "learned_cf.cpp",
+    "imaging.cpp",
     "ygrid_api.cpp",
 ]
 

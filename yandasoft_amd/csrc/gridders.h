@@ -64,6 +64,11 @@ struct SupportResult {
   int size, offu, offv;
 };
 // SupportSearcher + WProjectVisGridder::extractSupport on one device plane (nx x ny).
+class Gridder;
+// ImageFFTEquation::calcImagingEquations' data loop body for one chunk (ImageFFTEquation.cc:437-480).
+void imaging_chunk(const std::vector<Gridder*>& degridders, const std::vector<Gridder*>& gridders,
+                   const ygrid_chunk& a);
+
 SupportResult search_support(const Ctx& c, const float2* plane, int nx, int ny, double cutoff, bool absolute,
                              bool offset, int os);
 
@@ -83,12 +88,14 @@ class Gridder {
   void degrid(const ygrid_chunk& acc);
   void finaliseDegrid();
   bool isModelEmpty() const { return model_empty_; }
+  bool initialisedFor(bool degrid) const { return initialised_ && degrid_mode_ == degrid; }
   void setMFSWeight(double ref, int order) { vw_ref_ = ref; vw_order_ = order; }
   void setProfiling(bool on);
   void stats(ygrid_stats* out) const;
   void getCF(int k, float* out, int* support) const;
   void getGrid(float* out, size_t* n) const;
   const CFTable* cfTable() const { return cf_.get(); }
+  const std::shared_ptr<Ctx>& context() const { return ctx_; }
   // W-projection: the w (wavelengths) of every CF plane and wmax; false for other gridders
   virtual bool wTerms(std::vector<double>& w, double& wmax) const {
     (void)w;

@@ -1541,6 +1541,22 @@ TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, f
   return j;
 }
 
+namespace {
+__global__ void k_residual(const float2* __restrict__ data, float2* __restrict__ r, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float2 d = data[i], m = r[i];
+    r[i] = make_float2(d.x - m.x, d.y - m.y);
+  }
+}
+}  // namespace
+
+void launch_residual(const Ctx& c, const float2* data, float2* r, size_t n) {
+  if (n == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)8 * c.n_cu);
+  hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, c.stream, data, r, n);
+  YG_HIP(hipGetLastError());
+}
+
 void launch_fill_u32(const Ctx& c, void* p, size_t n, uint32_t value) {
   if (n == 0) return;
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)4 * c.n_cu);

@@ -246,6 +246,26 @@ int ygrid_get_grid(const ygrid_gridder* g, float* out, size_t* n) {
   });
 }
 
+int ygrid_imaging_chunk(ygrid_gridder* const* degridders, int ndegrid, ygrid_gridder* const* gridders, int ngrid,
+                        const ygrid_chunk* chunk) {
+  return guard([&] {
+    NEED(chunk);
+    YG_CHECK(ndegrid >= 0 && ngrid >= 0, YGRID_ERR_ARG, "negative gridder count");
+    YG_CHECK(ndegrid == 0 || degridders, YGRID_ERR_ARG, "degridders is null");
+    YG_CHECK(ngrid == 0 || gridders, YGRID_ERR_ARG, "gridders is null");
+    std::vector<ygrid::Gridder*> d, g;
+    for (int i = 0; i < ndegrid; ++i) {
+      NEED(degridders[i]);
+      d.push_back(degridders[i]->g.get());
+    }
+    for (int i = 0; i < ngrid; ++i) {
+      NEED(gridders[i]);
+      g.push_back(gridders[i]->g.get());
+    }
+    ygrid::imaging_chunk(d, g, *chunk);
+  });
+}
+
 int ygrid_batch_create(ygrid_ctx* ctx, ygrid_batch** out) {
   return guard([&] {
     NEED(ctx);

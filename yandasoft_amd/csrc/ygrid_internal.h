@@ -284,6 +284,8 @@ void enqueue_static_binning(const Ctx&, StaticBins& sb, const Samples& smp, cons
                             const int* base_in, int* base_out, int job);
 // fills n 32-bit words with value (a kernel: usable inside captured graphs)
 void launch_fill_u32(const Ctx&, void* p, size_t n, uint32_t value);
+// r[i] = data[i] - r[i] over n complex values (the residual of a model cube r).
+void launch_residual(const Ctx&, const float2* data, float2* r, size_t n);
 // Rounds the FP32 CF table (n complex taps) to prec (FP16 or BF16), packed (re, im) in 32 bits.
 void launch_cf_round(const Ctx&, const float2* src, uint32_t* dst, size_t n, int prec);
 void launch_degrid_finish(const Ctx&, const Samples& smp, const float* acc, int nrow, int nchan, int npol,
