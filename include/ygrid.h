@@ -56,6 +56,10 @@ typedef struct {
   /* image centre direction (TableVisGridder::getImageCentre, TableVisGridder.cc:919-934),
    * longitude/latitude in radians; used by the A-projection beam offsets */
   double centre_lon, centre_lat;
+  /* casacore Stokes codes of the image polarisations (npol; I=1 Q=2 U=3 V=4 RR=5 RL=6 LR=7 LL=8
+   * XX=9 XY=10 YX=11 YY=12; host memory), the grid's stokes axis (TableVisGridder::initStokes,
+   * TableVisGridder.cc:982-994), or NULL: the visibilities' own frame (no conversion) */
+  const int32_t *stokes;
 } ygrid_image_desc;
 
 /* One accessor chunk (IConstDataAccessor / IDataAccessor getters used by
@@ -72,6 +76,12 @@ typedef struct {
   const uint8_t *flag;       /* flag cube or NULL */
   const int32_t *feed1;      /* feed1: nrow or NULL (= 0) */
   const double *pointing1;   /* pointingDir1: nrow x 2 (longitude, latitude; rad) or NULL (= image centre) */
+  /* stokes(): casacore Stokes codes of the visibility polarisations (npol, host memory), or NULL:
+   * the image's frame.  When they differ from the image's, visibilities, noise and flags are
+   * converted to the image frame before gridding and the model back to this frame after
+   * degridding (PolConverter as used at TableVisGridder.cc:517-524,682-693,860; no parallactic
+   * angle rotation) */
+  const int32_t *stokes;
 } ygrid_chunk;
 
 /* Counters reported by TableVisGridder's destructor (TableVisGridder.cc:182-317). */

@@ -30,6 +30,12 @@ Restated reference functions (file:line in alxndrwllmsn/yandasoft):
 * ``disk_pattern`` / ``awproject_cf`` -- DiskIllumination::getPattern (DiskIllumination.cc:85-177)
                                  and AWProjectVisGridder::initConvolutionFunction (AWProjectVisGridder.cc:259-611).
 * ``frequency_map``          -- FrequencyMapper::setupMapping (FrequencyMapper.cc:63-97).
+* ``pol_matrix`` / ``pol_to_image`` / ``pol_to_vis`` -- scimath::PolConverter::convert / noise as
+                                 used at TableVisGridder.cc:517-524,682-693,860.  PolConverter is in
+                                 base-scimath (absent): restated from the casacore / IAU definitions
+                                 of the polarisation products, pinned by closed forms and round
+                                 trips; base-scimath's own conventions beyond them are parity
+                                 unpinned.
 * ``OracleGridder``          -- TableVisGridder::initialiseGrid/grid/finaliseGrid/finaliseWeights/
                                  initialiseDegrid/degrid (TableVisGridder.cc:451-884,997-1319).
 """
@@ -636,6 +642,59 @@ def fft_pad(a: np.ndarray, ny_out: int, nx_out: int) -> np.ndarray:
 # --------------------------------------------------------------------------------------
 # Frequency mapping
 # --------------------------------------------------------------------------------------
+# ---- polarisation conversion (PolConverter as used at TableVisGridder.cc:517-524,682-693,860) ----
+# PolConverter belongs to base-scimath (absent).  Restated from the casacore / IAU definitions of
+# the polarisation products (casacore Stokes codes I=1 Q=2 U=3 V=4 RR=5 RL=6 LR=7 LL=8 XX=9 XY=10
+# YX=11 YY=12): XX = I+Q, XY = U+iV, YX = U-iV, YY = I-Q, RR = I+V, RL = Q+iU, LR = Q-iU,
+# LL = I-V.  Pinned by closed forms (tests/test_oracle_pins.py); base-scimath's own sign and
+# normalisation choices beyond these definitions are parity unpinned.
+_STOKES_ROWS = {
+    1: (1, 0, 0, 0), 2: (0, 1, 0, 0), 3: (0, 0, 1, 0), 4: (0, 0, 0, 1),
+    5: (1, 0, 0, 1), 6: (0, 1, 1j, 0), 7: (0, 1, -1j, 0), 8: (1, 0, 0, -1),
+    9: (1, 1, 0, 0), 10: (0, 0, 1, 1j), 11: (0, 0, 1, -1j), 12: (1, -1, 0, 0),
+}
+
+
+def stokes_rows(codes) -> np.ndarray:
+    """Products x (I, Q, U, V) for a list of Stokes codes."""
+    return np.array([_STOKES_ROWS[int(c)] for c in codes], dtype=np.complex128)
+
+
+def pol_matrix(frm, to) -> np.ndarray:
+    """out = M @ in between two polarisation frames: S_to pinv(S_from); raises when an output
+    is not determined by the inputs."""
+    si, so = stokes_rows(frm), stokes_rows(to)
+    m = so @ np.linalg.pinv(si)
+    if not np.allclose(m @ si, so, atol=1e-9):
+        raise ValueError(f"polarisations {list(to)} cannot be formed from {list(frm)}")
+    return m
+
+
+def pol_to_image(vis, noise, flag, frm, to, shape=None):
+    """Cubes (npol, nchan, nrow) to the image frame: visibilities M @ vis; noise per output
+    (re, im) = sqrt(sum Re(M)^2 var(re|im) + Im(M)^2 var(im|re)) over the inputs (noise None:
+    1); flags: any input polarisation flagged.  shape = (nchan, nrow) when no cube is given."""
+    m = pol_matrix(frm, to)
+    ref = next((a for a in (vis, noise, flag) if a is not None), None)
+    shape = tuple(ref.shape[1:]) if ref is not None else tuple(shape)
+    v = None if vis is None else np.einsum("pq,qcr->pcr", m, vis.astype(np.complex128))
+    n = np.ones((len(frm),) + shape, np.complex128) if noise is None else noise.astype(np.complex128)
+    r2, i2 = np.real(m) ** 2, np.imag(m) ** 2
+    nr, ni = np.real(n) ** 2, np.imag(n) ** 2
+    var_r = np.einsum("pq,qcr->pcr", r2, nr) + np.einsum("pq,qcr->pcr", i2, ni)
+    var_i = np.einsum("pq,qcr->pcr", r2, ni) + np.einsum("pq,qcr->pcr", i2, nr)
+    nout = np.sqrt(var_r) + 1j * np.sqrt(var_i)
+    f = None if flag is None else np.repeat(np.any(flag, axis=0, keepdims=True), len(to), axis=0)
+    return v, nout, f
+
+
+def pol_to_vis(model, frm, to):
+    """Image-frame model cube (npol_img, nchan, nrow) to the visibility frame: M @ model with
+    M = S_vis pinv(S_img) (Stokes parameters the image does not hold are zero)."""
+    m = stokes_rows(to) @ np.linalg.pinv(stokes_rows(frm))
+    return np.einsum("pq,qcr->pcr", m, model.astype(np.complex128))
+
+
 def frequency_map(freqs, nchan_img: int, fstart: float | None = None, fend: float | None = None):
     """FrequencyMapper: per accessor channel, the image channel or -1."""
     freqs = np.asarray(freqs, dtype=np.float64)

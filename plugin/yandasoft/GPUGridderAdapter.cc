@@ -57,7 +57,7 @@ GPUGridderAdapter::~GPUGridderAdapter() { ygrid_gridder_destroy(itsGridder); }
 
 IVisGridder::ShPtr GPUGridderAdapter::clone() { return IVisGridder::ShPtr(new GPUGridderAdapter(*this)); }
 
-ygrid_image_desc GPUGridderAdapter::describe(const scimath::Axes& axes, const casacore::IPosition& shape) const {
+ygrid_image_desc GPUGridderAdapter::describe(const scimath::Axes& axes, const casacore::IPosition& shape) {
   ygrid_image_desc d{};
   d.nx = shape(0);
   d.ny = shape(1);
@@ -75,6 +75,15 @@ ygrid_image_desc GPUGridderAdapter::describe(const scimath::Axes& axes, const ca
     d.freq_start = axes.start("FREQUENCY");
     d.freq_end = axes.end("FREQUENCY");
   }
+  // TableVisGridder::initStokes (TableVisGridder.cc:982-994): the stokes axis, or I
+  itsImageStokes.clear();
+  if (axes.has("STOKES")) {
+    const casacore::Vector<casacore::Stokes::StokesTypes> st = axes.stokesAxis();
+    for (casacore::uInt p = 0; p < st.nelements(); ++p) itsImageStokes.push_back(int32_t(st[p]));
+  } else {
+    itsImageStokes.push_back(int32_t(casacore::Stokes::I));
+  }
+  d.stokes = itsImageStokes.data();
   return d;
 }
 
@@ -109,6 +118,10 @@ ygrid_chunk GPUGridderAdapter::stage(const accessors::IConstDataAccessor& acc, b
   c.feed1 = itsFeed.data();
   c.pointing1 = itsPointing.data();
   c.flag = itsFlag.data();
+  // acc.stokes(): the visibilities' polarisation frame (TableVisGridder.cc:517-524)
+  itsVisStokes.clear();
+  for (casacore::uInt p = 0; p < nPol; ++p) itsVisStokes.push_back(int32_t(acc.stokes()[p]));
+  c.stokes = itsVisStokes.data();
   if (forward) {
     ASKAPCHECK(acc.visibility().contiguousStorage() && acc.noise().contiguousStorage(),
                "GPU gridder needs contiguous visibility and noise cubes");

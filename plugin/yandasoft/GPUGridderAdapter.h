@@ -52,7 +52,7 @@ class GPUGridderAdapter : public IVisGridder {
 
  private:
   GPUGridderAdapter(const GPUGridderAdapter& other);
-  ygrid_image_desc describe(const scimath::Axes& axes, const casacore::IPosition& shape) const;
+  ygrid_image_desc describe(const scimath::Axes& axes, const casacore::IPosition& shape);
   ygrid_chunk stage(const accessors::IConstDataAccessor& acc, bool forward);
   static void check(int status);
 
@@ -63,7 +63,7 @@ class GPUGridderAdapter : public IVisGridder {
   double itsMFSRefFreq = 0.0;  ///< > 0 once MFS visibility weights are set
   // host staging of one accessor (reused across calls)
   std::vector<double> itsUVW, itsDelay, itsPointing;
-  std::vector<int32_t> itsFeed;
+  std::vector<int32_t> itsFeed, itsImageStokes, itsVisStokes;
   std::vector<uint8_t> itsFlag;
 };
 

@@ -409,3 +409,61 @@ def test_mlp_oracle_silu_derivative():
     h = 1e-6
     assert np.max(np.abs((M.silu(z + h) - M.silu(z - h)) / (2 * h) - M.silu_grad(z))) < 1e-8
     assert M.silu(0.0) == 0.0 and M.silu_grad(0.0) == 0.5
+
+
+# ---- polarisation conversion (oracle.pol_matrix / pol_to_image / pol_to_vis) -----------------
+XX, XY, YX, YY, RR, RL, LR, LL, SI, SQ, SU, SV = 9, 10, 11, 12, 5, 6, 7, 8, 1, 2, 3, 4
+
+
+def test_pol_matrix_closed_forms():
+    """Stokes from the linear and circular products by the textbook inversions
+    (I = (XX+YY)/2, Q = (XX-YY)/2, U = (XY+YX)/2, V = (XY-YX)/(2i); I = (RR+LL)/2,
+    V = (RR-LL)/2, Q = (RL+LR)/2, U = (RL-LR)/(2i)), and the forward products of a source
+    with known Stokes parameters."""
+    rng = np.random.default_rng(1)
+    xx, xy, yx, yy = rng.normal(size=4) + 1j * rng.normal(size=4)
+    lin = np.array([xx, xy, yx, yy])
+    got = O.pol_matrix([XX, XY, YX, YY], [SI, SQ, SU, SV]) @ lin
+    want = [(xx + yy) / 2, (xx - yy) / 2, (xy + yx) / 2, (xy - yx) / 2j]
+    assert np.allclose(got, want, atol=1e-14)
+    rr, rl, lr, ll = lin
+    got = O.pol_matrix([RR, RL, LR, LL], [SI, SQ, SU, SV]) @ np.array([rr, rl, lr, ll])
+    assert np.allclose(got, [(rr + ll) / 2, (rl + lr) / 2, (rl - lr) / 2j, (rr - ll) / 2], atol=1e-14)
+    # I only from the parallel hands
+    assert np.allclose(O.pol_matrix([XX, YY], [SI]), [[0.5, 0.5]], atol=1e-15)
+    # a source (I, Q, U, V) = (3, 1, 0.5, 0.25): the products by their definitions
+    s = np.array([3.0, 1.0, 0.5, 0.25])
+    assert np.allclose(O.pol_matrix([SI, SQ, SU, SV], [XX, XY, YX, YY]) @ s,
+                       [4.0, 0.5 + 0.25j, 0.5 - 0.25j, 2.0], atol=1e-14)
+    assert np.allclose(O.pol_matrix([SI, SQ, SU, SV], [RR, RL, LR, LL]) @ s,
+                       [3.25, 1.0 + 0.5j, 1.0 - 0.5j, 2.75], atol=1e-14)
+
+
+def test_pol_matrix_round_trips_and_undetermined_outputs():
+    lin, circ, stk = [XX, XY, YX, YY], [RR, RL, LR, LL], [SI, SQ, SU, SV]
+    for a, b in ((lin, stk), (circ, stk), (lin, circ)):
+        assert np.allclose(O.pol_matrix(b, a) @ O.pol_matrix(a, b), np.eye(4), atol=1e-14)
+    with pytest.raises(ValueError):
+        O.pol_matrix([XX, YY], [SV])  # V needs the cross hands
+    with pytest.raises(ValueError):
+        O.pol_matrix([RR, LL], [SQ])  # Q needs the cross hands
+    # predicting from an I-only image: unpolarised products, zero cross hands
+    m = np.ones((1, 2, 3), np.complex128) * (2 + 1j)
+    v = O.pol_to_vis(m, [SI], lin)
+    assert np.allclose(v[0], 2 + 1j) and np.allclose(v[3], 2 + 1j) and np.allclose(v[1:3], 0)
+
+
+def test_pol_noise_closed_forms():
+    """Independent errors: I = (XX+YY)/2 with sigma (s_re, s_im) on both hands has sigma /
+    sqrt(2) on each part; V = (XY-YX)/(2i) = (-i/2) XY + (i/2) YX moves the imaginary parts'
+    errors into the real part and vice versa."""
+    n = np.empty((4, 1, 1), np.complex128)
+    n[:, 0, 0] = [0.4 + 0.2j, 1.0 + 3.0j, 2.0 + 5.0j, 0.4 + 0.2j]
+    _, nI, _ = O.pol_to_image(None, n, None, [XX, XY, YX, YY], [SI])
+    assert np.allclose(nI[0, 0, 0], (0.4 + 0.2j) / np.sqrt(2))
+    _, nV, _ = O.pol_to_image(None, n, None, [XX, XY, YX, YY], [SV])
+    assert np.allclose(nV[0, 0, 0], 0.5 * np.sqrt(3.0 ** 2 + 5.0 ** 2) + 0.5j * np.sqrt(1.0 + 2.0 ** 2))
+    f = np.zeros((4, 1, 2), bool)
+    f[2, 0, 1] = True
+    _, _, fo = O.pol_to_image(None, None, f, [XX, XY, YX, YY], [SI, SQ])
+    assert fo.shape == (2, 1, 2) and not fo[:, 0, 0].any() and fo[:, 0, 1].all()

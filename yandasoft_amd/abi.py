@@ -35,14 +35,15 @@ class _ImageDesc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("npol", ctypes.c_int), ("nchan", ctypes.c_int),
                 ("cellx", ctypes.c_double), ("celly", ctypes.c_double), ("has_freq_axis", ctypes.c_int),
                 ("freq_start", ctypes.c_double), ("freq_end", ctypes.c_double),
-                ("centre_lon", ctypes.c_double), ("centre_lat", ctypes.c_double)]
+                ("centre_lon", ctypes.c_double), ("centre_lat", ctypes.c_double), ("stokes", ctypes.c_void_p)]
 
 
 class _Chunk(ctypes.Structure):
     _fields_ = [("nrow", ctypes.c_int), ("nchan", ctypes.c_int), ("npol", ctypes.c_int), ("mem", ctypes.c_int),
                 ("uvw", ctypes.c_void_p), ("delay", ctypes.c_void_p), ("freq", ctypes.c_void_p),
                 ("vis", ctypes.c_void_p), ("vis_out", ctypes.c_void_p), ("noise", ctypes.c_void_p),
-                ("flag", ctypes.c_void_p), ("feed1", ctypes.c_void_p), ("pointing1", ctypes.c_void_p)]
+                ("flag", ctypes.c_void_p), ("feed1", ctypes.c_void_p), ("pointing1", ctypes.c_void_p),
+                ("stokes", ctypes.c_void_p)]
 
 
 class Stats(ctypes.Structure):
@@ -115,11 +116,14 @@ class ImageDesc:
     celly: float = 0.0
     freq_axis: tuple | None = None  # (start, end) Hz
     centre: tuple = (0.0, 0.0)      # image centre (lon, lat) rad
+    stokes: tuple | None = None     # casacore Stokes codes of the image pols (None: the data's frame)
 
     def c(self) -> _ImageDesc:
         fa = self.freq_axis
+        self._stokes = None if self.stokes is None else np.asarray(self.stokes, dtype=np.int32)
         return _ImageDesc(self.nx, self.ny, self.npol, self.nchan, self.cellx, self.celly, 1 if fa else 0,
-                          fa[0] if fa else 0.0, fa[1] if fa else 0.0, self.centre[0], self.centre[1])
+                          fa[0] if fa else 0.0, fa[1] if fa else 0.0, self.centre[0], self.centre[1],
+                          None if self._stokes is None else self._stokes.ctypes.data)
 
 
 def _ptr(a):
@@ -144,6 +148,7 @@ class Chunk:
     delay: object = None
     feed1: object = None
     pointing1: object = None
+    stokes: tuple | None = None  # casacore Stokes codes of the visibility pols (None: the image's)
     nrow: int = 0
     nchan: int = 0
     npol: int = 0
@@ -161,9 +166,11 @@ class Chunk:
                     assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], f"{name} must be C-contiguous"
 
     def c(self) -> _Chunk:
+        self._stokes = None if self.stokes is None else np.asarray(self.stokes, dtype=np.int32)
         return _Chunk(self.nrow, self.nchan, self.npol, YGRID_MEM_DEVICE if self.device else YGRID_MEM_HOST,
                       _ptr(self.uvw), _ptr(self.delay), _ptr(self.freq), _ptr(self.vis), _ptr(self.vis_out),
-                      _ptr(self.noise), _ptr(self.flag), _ptr(self.feed1), _ptr(self.pointing1))
+                      _ptr(self.noise), _ptr(self.flag), _ptr(self.feed1), _ptr(self.pointing1),
+                      None if self._stokes is None else self._stokes.ctypes.data)
 
 
 class Context:

@@ -32,6 +32,7 @@ SOURCES = [
     "batch.cpp",
     "learned_cf.cpp",
     "imaging.cpp",
+    "polconv.cpp",
     "ygrid_api.cpp",
 ]
 

@@ -180,6 +180,11 @@ void Gridder::initialiseCommon(const ygrid_image_desc& img) {
   fend_ = img.freq_end;
   centre_[0] = img.centre_lon;
   centre_[1] = img.centre_lat;
+  img_stokes_.clear();
+  if (img.stokes) {
+    img_stokes_.assign(img.stokes, img.stokes + img.npol);
+    pol_matrix(img_stokes_, img_stokes_);  // the codes must be known and independent
+  }
   initialised_ = true;
 }
 
@@ -300,7 +305,7 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
   ChunkDev& d = ps.d;
   stage(a, d, forward);
   YG_CHECK(d.npol == shape_[2], 1,
-           "only the identity polarisation conversion is supported: chunk npol must equal image npol");
+           "chunk npol must equal image npol unless both polarisation frames are given (stokes)");
   if (d.nrow == 0) return false;
   const double r0 = d.freq_host[0] / kC;
   YG_CHECK(r0 > 0.1 && r0 < 30000, 4,
@@ -479,8 +484,14 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   account_pass(forward, cnt, err, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
 }
 
-void Gridder::grid(const ygrid_chunk& a) { generic(a, false); }
-void Gridder::degrid(const ygrid_chunk& a) { generic(a, true); }
+void Gridder::grid(const ygrid_chunk& a) {
+  if (needsPolConversion(a)) convertedPass(a, false);
+  else generic(a, false);
+}
+void Gridder::degrid(const ygrid_chunk& a) {
+  if (needsPolConversion(a)) convertedPass(a, true);
+  else generic(a, true);
+}
 
 // TableVisGridder::finaliseGrid (TableVisGridder.cc:1109-1236)
 void Gridder::finaliseGrid(float* out, int mem) {

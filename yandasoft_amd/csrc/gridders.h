@@ -6,6 +6,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <complex>
 #include <vector>
 
 #include "../../include/ygrid.h"
@@ -69,6 +70,21 @@ class Gridder;
 void imaging_chunk(const std::vector<Gridder*>& degridders, const std::vector<Gridder*>& gridders,
                    const ygrid_chunk& a);
 
+// A device copy of a host chunk's input arrays (vis_out is not copied: NULL in the copy); the
+// arrays of a device chunk are used as they are.
+struct DeviceChunk {
+  ygrid_chunk c{};
+  DevBuf store;
+  DeviceChunk(const Ctx& ctx, const ygrid_chunk& a);
+};
+
+// Polarisation conversion matrix out = M in between two lists of casacore Stokes codes
+// (PolConverter semantics, polconv.cpp); row-major nout x nin.  determined: throw when an output
+// polarisation cannot be formed from the inputs (imaging); else the Stokes parameters the inputs
+// do not hold are taken as zero (predicting).
+std::vector<std::complex<double>> pol_matrix(const std::vector<int>& from, const std::vector<int>& to,
+                                             bool determined = true);
+
 SupportResult search_support(const Ctx& c, const float2* plane, int nx, int ny, double cutoff, bool absolute,
                              bool offset, int os);
 
@@ -129,6 +145,10 @@ class Gridder {
 
   void initialiseCommon(const ygrid_image_desc& img);
   void generic(const ygrid_chunk& acc, bool forward);
+  // polarisation conversion around generic (polconv.cpp): true when the chunk's Stokes codes
+  // differ from the image's
+  bool needsPolConversion(const ygrid_chunk& acc) const;
+  void convertedPass(const ygrid_chunk& acc, bool forward);
   friend class Batch;
   struct PassSetup {
     ChunkDev d;
@@ -165,6 +185,7 @@ class Gridder {
   bool has_freq_axis_ = false;
   double fstart_ = 0, fend_ = 0;
   double centre_[2] = {0, 0};  // image centre (lon, lat)
+  std::vector<int> img_stokes_;  // Stokes codes of the image polarisations (empty: the data's frame)
   double vw_ref_ = 0;
   int vw_order_ = -1;
   int nsw_rows_ = 1;

@@ -14,49 +14,40 @@
 
 namespace ygrid {
 
-namespace {
-
-// A device copy of a host chunk's arrays (every pointer of the returned chunk is a device
-// pointer); a device chunk is returned unchanged.
-struct DeviceChunk {
-  ygrid_chunk c{};
-  DevBuf store;
-
-  DeviceChunk(const Ctx& ctx, const ygrid_chunk& a) : c(a) {
-    if (a.mem == YGRID_MEM_DEVICE) return;
-    YG_CHECK(a.mem == YGRID_MEM_HOST, 1, "unknown memory kind");
-    const size_t nr = a.nrow, nc = a.nchan, ncube = nr * nc * (size_t)a.npol;
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t sizes[] = {a.uvw ? nr * 24 : 0,       a.delay ? nr * 8 : 0,      a.freq ? nc * 8 : 0,
-                            a.vis ? ncube * 8 : 0,     a.noise ? ncube * 8 : 0,   a.flag ? ncube : 0,
-                            a.feed1 ? nr * 4 : 0,      a.pointing1 ? nr * 16 : 0};
-    size_t bytes = 0;
-    for (size_t b : sizes) bytes += al(b);
-    store.resize(std::max<size_t>(bytes, 256));
-    char* p = store.as<char>();
-    auto put = [&](const void* src, size_t b) -> void* {
-      if (!src) return nullptr;
-      void* dst = p;
-      if (b) YG_HIP(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, ctx.stream));
-      p += al(b);
-      return dst;
-    };
-    c.uvw = static_cast<const double*>(put(a.uvw, sizes[0]));
-    c.delay = static_cast<const double*>(put(a.delay, sizes[1]));
-    c.freq = static_cast<const double*>(put(a.freq, sizes[2]));
-    c.vis = static_cast<const float*>(put(a.vis, sizes[3]));
-    c.noise = static_cast<const float*>(put(a.noise, sizes[4]));
-    c.flag = static_cast<const uint8_t*>(put(a.flag, sizes[5]));
-    c.feed1 = static_cast<const int32_t*>(put(a.feed1, sizes[6]));
-    c.pointing1 = static_cast<const double*>(put(a.pointing1, sizes[7]));
-    c.vis_out = nullptr;
-    c.mem = YGRID_MEM_DEVICE;
-    // the host arrays may be reused by the caller as soon as this returns
-    YG_HIP(hipStreamSynchronize(ctx.stream));
-  }
-};
-
-}  // namespace
+// A host chunk's input arrays copied to the device (every pointer of the copy a device pointer,
+// vis_out NULL); a device chunk is used as it is.
+DeviceChunk::DeviceChunk(const Ctx& ctx, const ygrid_chunk& a) : c(a) {
+  if (a.mem == YGRID_MEM_DEVICE) return;
+  YG_CHECK(a.mem == YGRID_MEM_HOST, 1, "unknown memory kind");
+  const size_t nr = a.nrow, nc = a.nchan, ncube = nr * nc * (size_t)a.npol;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t sizes[] = {a.uvw ? nr * 24 : 0,       a.delay ? nr * 8 : 0,      a.freq ? nc * 8 : 0,
+                          a.vis ? ncube * 8 : 0,     a.noise ? ncube * 8 : 0,   a.flag ? ncube : 0,
+                          a.feed1 ? nr * 4 : 0,      a.pointing1 ? nr * 16 : 0};
+  size_t bytes = 0;
+  for (size_t b : sizes) bytes += al(b);
+  store.resize(std::max<size_t>(bytes, 256));
+  char* p = store.as<char>();
+  auto put = [&](const void* src, size_t b) -> void* {
+    if (!src) return nullptr;
+    void* dst = p;
+    if (b) YG_HIP(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, ctx.stream));
+    p += al(b);
+    return dst;
+  };
+  c.uvw = static_cast<const double*>(put(a.uvw, sizes[0]));
+  c.delay = static_cast<const double*>(put(a.delay, sizes[1]));
+  c.freq = static_cast<const double*>(put(a.freq, sizes[2]));
+  c.vis = static_cast<const float*>(put(a.vis, sizes[3]));
+  c.noise = static_cast<const float*>(put(a.noise, sizes[4]));
+  c.flag = static_cast<const uint8_t*>(put(a.flag, sizes[5]));
+  c.feed1 = static_cast<const int32_t*>(put(a.feed1, sizes[6]));
+  c.pointing1 = static_cast<const double*>(put(a.pointing1, sizes[7]));
+  c.vis_out = nullptr;
+  c.mem = YGRID_MEM_DEVICE;
+  // the host arrays may be reused by the caller as soon as this returns
+  YG_HIP(hipStreamSynchronize(ctx.stream));
+}
 
 void imaging_chunk(const std::vector<Gridder*>& degridders, const std::vector<Gridder*>& gridders,
                    const ygrid_chunk& a) {

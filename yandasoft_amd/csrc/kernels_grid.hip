@@ -1550,6 +1550,86 @@ __global__ void k_residual(const float2* __restrict__ data, float2* __restrict__
 }
 }  // namespace
 
+namespace {
+__global__ void k_pol_to_image(PolMatrix M, const float2* __restrict__ vis, const float2* __restrict__ noise,
+                               const uint8_t* __restrict__ flag, size_t nsamp, float2* __restrict__ vis_o,
+                               float2* __restrict__ noise_o, uint8_t* __restrict__ flag_o) {
+  for (size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x; s < nsamp; s += (size_t)gridDim.x * blockDim.x) {
+    float2 v[4], n[4];
+    bool flagged = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < M.nin) {
+        v[q] = vis ? vis[s + nsamp * q] : make_float2(0.f, 0.f);
+        n[q] = noise ? noise[s + nsamp * q] : make_float2(1.f, 1.f);
+        flagged |= flag ? flag[s + nsamp * q] != 0 : false;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      if (p >= M.nout) break;
+      float ar = 0.f, ai = 0.f, vr = 0.f, vi = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q >= M.nin) break;
+        const float2 m = M.m[p][q];
+        ar += m.x * v[q].x - m.y * v[q].y;
+        ai += m.x * v[q].y + m.y * v[q].x;
+        vr += m.x * m.x * n[q].x * n[q].x + m.y * m.y * n[q].y * n[q].y;
+        vi += m.x * m.x * n[q].y * n[q].y + m.y * m.y * n[q].x * n[q].x;
+      }
+      if (vis_o) vis_o[s + nsamp * p] = make_float2(ar, ai);
+      if (noise_o) noise_o[s + nsamp * p] = make_float2(sqrtf(vr), sqrtf(vi));
+      if (flag_o) flag_o[s + nsamp * p] = flagged ? 1 : 0;
+    }
+  }
+}
+
+__global__ void k_pol_to_vis_add(PolMatrix M, const float2* __restrict__ img, size_t nsamp,
+                                 float2* __restrict__ vis_out) {
+  for (size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x; s < nsamp; s += (size_t)gridDim.x * blockDim.x) {
+    float2 x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < M.nin) x[q] = img[s + nsamp * q];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      if (p >= M.nout) break;
+      float ar = 0.f, ai = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q >= M.nin) break;
+        const float2 m = M.m[p][q];
+        ar += m.x * x[q].x - m.y * x[q].y;
+        ai += m.x * x[q].y + m.y * x[q].x;
+      }
+      float2 o = vis_out[s + nsamp * p];
+      o.x += ar;
+      o.y += ai;
+      vis_out[s + nsamp * p] = o;
+    }
+  }
+}
+}  // namespace
+
+void launch_pol_to_image(const Ctx& c, const PolMatrix& M, const float2* vis, const float2* noise,
+                         const uint8_t* flag, size_t nsamp, float2* vis_o, float2* noise_o, uint8_t* flag_o) {
+  YG_CHECK(M.nin >= 1 && M.nin <= 4 && M.nout >= 1 && M.nout <= 4, 1, "bad polarisation matrix");
+  if (nsamp == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((nsamp + 255) / 256, (size_t)8 * c.n_cu);
+  hipLaunchKernelGGL(k_pol_to_image, dim3(blocks), dim3(256), 0, c.stream, M, vis, noise, flag, nsamp, vis_o,
+                     noise_o, flag_o);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_pol_to_vis_add(const Ctx& c, const PolMatrix& M, const float2* img, size_t nsamp, float2* vis_out) {
+  YG_CHECK(M.nin >= 1 && M.nin <= 4 && M.nout >= 1 && M.nout <= 4, 1, "bad polarisation matrix");
+  if (nsamp == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((nsamp + 255) / 256, (size_t)8 * c.n_cu);
+  hipLaunchKernelGGL(k_pol_to_vis_add, dim3(blocks), dim3(256), 0, c.stream, M, img, nsamp, vis_out);
+  YG_HIP(hipGetLastError());
+}
+
 void launch_residual(const Ctx& c, const float2* data, float2* r, size_t n) {
   if (n == 0) return;
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)8 * c.n_cu);

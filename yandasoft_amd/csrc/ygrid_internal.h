@@ -284,6 +284,19 @@ void enqueue_static_binning(const Ctx&, StaticBins& sb, const Samples& smp, cons
                             const int* base_in, int* base_out, int job);
 // fills n 32-bit words with value (a kernel: usable inside captured graphs)
 void launch_fill_u32(const Ctx&, void* p, size_t n, uint32_t value);
+// Polarisation conversion of visibility cubes (casacore Cube(nRow, nChan, nPol) layout: the
+// sample (row, chan) of polarisation p at index s + nsamp * p).  m: row-major nout x nin.
+struct PolMatrix {
+  float2 m[4][4];
+  int nout, nin;
+};
+// To the image frame: vis_o = M vis; noise_o per output (re, im) = sqrt of the sums of
+// Re(M)^2 and Im(M)^2 times the inputs' (re, im) / (im, re) variances (noise NULL: 1);
+// flag_o = any input flagged.  Null outputs are skipped.
+void launch_pol_to_image(const Ctx&, const PolMatrix& M, const float2* vis, const float2* noise,
+                         const uint8_t* flag, size_t nsamp, float2* vis_o, float2* noise_o, uint8_t* flag_o);
+// Back to the visibility frame: vis_out += M img.
+void launch_pol_to_vis_add(const Ctx&, const PolMatrix& M, const float2* img, size_t nsamp, float2* vis_out);
 // r[i] = data[i] - r[i] over n complex values (the residual of a model cube r).
 void launch_residual(const Ctx&, const float2* data, float2* r, size_t n);
 // Rounds the FP32 CF table (n complex taps) to prec (FP16 or BF16), packed (re, im) in 32 bits.
