@@ -134,7 +134,7 @@ def main():
                     help="wproject_array: CF tap and coefficient storage (the metric is quoted for fp32)")
     ap.add_argument("--workload", default="wproject_array",
                     choices=["wproject_array", "sphfunc_disc", "awproject_feeds", "batched_beams", "learned_wdisc",
-                             "imaging_loop"])
+                             "imaging_loop", "wstack_array"])
     ap.add_argument("--train-epochs", type=int, default=200, help="learned_wdisc: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -164,6 +164,9 @@ def main():
         return
     if args.workload == "imaging_loop":
         bench_imaging(args)
+        return
+    if args.workload == "wstack_array":
+        bench_wstack(args)
         return
     import yandasoft_amd as ya
     from yandasoft_amd import workloads as W
@@ -491,6 +494,80 @@ def bench_imaging(args):
                                "ygrid_imaging_chunk call per step", "parallelism": "1 GPU"},
         "phases_ms_per_step": {"degrid_kernel": dk_ms, "grid_kernel_each": gk_ms,
                                "prep": other_ms("ms_prep"), "binning": other_ms("ms_bin")},
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None},
+    }), flush=True)
+
+
+def bench_wstack(args):
+    """W stacking (GPUWStack) on the W-projection array data: 65 w planes of 4096^2 x 4 pol grids
+    (35 GB per gridder), SphFunc support 3, oversample 128; per step one grid and one degrid pass
+    (the degridding reuses the gridding's binning); the per-plane transforms of initialise_degrid
+    and finalise_grid are outside the timed steps."""
+    import torch
+    import yandasoft_amd as ya
+    from yandasoft_amd import workloads as W
+
+    wl = W.wproject_array(nint=args.nint)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    t_vis = t(np.ascontiguousarray(wl.vis).view(np.float32))
+    chunk = ya.Chunk(uvw=t(wl.uvw), freq=t(wl.freq), vis=t_vis, vis_out=torch.zeros_like(t_vis),
+                     noise=t(np.ascontiguousarray(wl.noise).view(np.float32)), device=True)
+    chunk.npol = wl.npol
+    ctx = ya.Context(0)
+    img = ya.ImageDesc(wl.nx, wl.ny, wl.npol, 1, wl.cell, wl.cell)
+    prm = {"wmax": wl.params["wmax"], "nwplanes": 65}
+    gg, dg = ya.Gridder(ctx, "GPUWStack", prm), ya.Gridder(ctx, "GPUWStack", prm)
+    rng = np.random.default_rng(3)
+    model = np.zeros((1, wl.npol, wl.ny, wl.nx), np.float32)
+    for _ in range(50):
+        model[0, :, rng.integers(500, 3500), rng.integers(500, 3500)] = rng.normal(size=wl.npol)
+    dg.initialise_degrid(img, t(model))
+    torch.cuda.synchronize()
+
+    def step():
+        ctx.clear_cache()
+        gg.initialise_grid(img)
+        gg.grid(chunk)
+        dg.degrid(chunk)
+
+    for i in range(args.warmup):
+        step()
+    ctx.synchronize()
+    s0g, s0d = gg.stats(), dg.stats()
+    gg.set_profiling(True)
+    dg.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"step {i} done")
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s1g, s1d = gg.stats(), dg.stats()
+    taps_g, taps_d = s1g.taps_gridded - s0g.taps_gridded, s1d.taps_degridded - s0d.taps_degridded
+    gk = s1g.ms_grid_kernel / max(1.0, s1g.n_grid_launches)
+    dk = s1d.ms_degrid_kernel / max(1.0, s1d.n_degrid_launches)
+    tf = lambda b: "true" if b else "false"
+    if gk >= dk:
+        kname, kms, kt = f"k_grid<{wl.npol}, {s1g.grid_waves}, {tf(s1g.grid_pipeline)}, 0>", gk, taps_g / args.steps
+    else:
+        kname = (f"k_degrid_mm<{wl.npol}, 0>" if s1d.degrid_matrix else
+                 f"k_degrid<{wl.npol}, {s1d.degrid_waves}, {tf(s1d.degrid_pipeline)}, 0>")
+        kms, kt = dk, taps_d / args.steps
+    achieved = 8.0 * kt / (kms * 1e-3) / 1e12
+    print(json.dumps({
+        "metric": "CF-tap updates/s (W-stacking grid+degrid)", "value": (taps_g + taps_d) / elapsed,
+        "unit": "CF-tap updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "wstack_array: wproject_array data (189000 rows x 64 ch x 4 pol, 4096^2), "
+                               "65 w planes, SphFunc support 3, oversample 128", "parallelism": "1 GPU"},
+        "phases_ms_per_step": {"grid_kernel": gk, "degrid_kernel": dk,
+                               "prep": (s1g.ms_prep + s1d.ms_prep) / args.steps,
+                               "binning": (s1g.ms_bin + s1d.ms_bin) / args.steps},
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None},
     }), flush=True)

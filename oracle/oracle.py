@@ -30,6 +30,9 @@ Restated reference functions (file:line in alxndrwllmsn/yandasoft):
 * ``disk_pattern`` / ``awproject_cf`` -- DiskIllumination::getPattern (DiskIllumination.cc:85-177)
                                  and AWProjectVisGridder::initConvolutionFunction (AWProjectVisGridder.cc:259-611).
 * ``frequency_map``          -- FrequencyMapper::setupMapping (FrequencyMapper.cc:63-97).
+* ``wstack_screen`` / ``OracleWStack`` -- WStackVisGridder::multiply / initIndices / gIndex /
+                                 finaliseGrid / initialiseDegrid (WStackVisGridder.cc:71-97,134-171,
+                                 216-344), composed from the SphFunc oracle gridder.
 * ``pol_matrix`` / ``pol_to_image`` / ``pol_to_vis`` -- scimath::PolConverter::convert / noise as
                                  used at TableVisGridder.cc:517-524,682-693,860.  PolConverter is in
                                  base-scimath (absent): restated from the casacore / IAU definitions
@@ -642,6 +645,87 @@ def fft_pad(a: np.ndarray, ny_out: int, nx_out: int) -> np.ndarray:
 # --------------------------------------------------------------------------------------
 # Frequency mapping
 # --------------------------------------------------------------------------------------
+# ---- w stacking (WStackVisGridder.cc) ----------------------------------------------------------
+def wstack_screen(nx: int, ny: int, uvcell, wterm: float) -> np.ndarray:
+    """WStackVisGridder::multiply's screen (WStackVisGridder.cc:134-171) in its float arithmetic:
+    (cos p, -sin p), p = 2 pi w (1 - sqrt(1 - r^2)), r^2 = x^2 + y^2 of the padded pixel offsets
+    times the image cell, where r^2 < 1; 1 elsewhere.  (ny, nx) complex."""
+    f32 = np.float32
+    cellx = f32(1.0 / (float(f32(nx)) * uvcell[0]))
+    celly = f32(1.0 / (float(f32(ny)) * uvcell[1]))
+    w = f32(2.0 * math.pi * wterm)
+    x2 = (np.arange(nx) - nx // 2).astype(f32) * cellx
+    x2 = x2 * x2
+    y2 = (np.arange(ny) - ny // 2).astype(f32) * celly
+    y2 = y2 * y2
+    r2 = (y2[:, None] + x2[None, :]).astype(f32)
+    inside = r2.astype(np.float64) < 1.0
+    phase = (np.float64(w) * (1.0 - np.sqrt(np.where(inside, 1.0 - r2.astype(np.float64), 1.0)))).astype(f32)
+    scr = np.cos(phase).astype(np.float64) - 1j * np.sin(phase).astype(np.float64)
+    return np.where(inside, scr, 1.0 + 0j)
+
+
+class OracleWStack:
+    """WStackVisGridder composed from SphFunc oracle gridders: a sample goes to the grid of its w
+    plane (initIndices / gIndex = getWPlane(w f / c), WStackVisGridder.cc:71-97,339-344; the CF is
+    SphFunc's with its defaults, cIndex 0); finaliseGrid sums Re(screen x inverse transform) over
+    the planes, then corrects and scales (:255-289); initialiseDegrid fills each plane with the
+    forward transform of conj(screen x corrected model) (:314-327)."""
+
+    def __init__(self, wmax: float, nwplanes: int, precision: str = "f32"):
+        self.ws = WSampling(wmax, nwplanes)
+        self.precision = precision
+
+    def _planes(self, chunk: Chunk):
+        w = chunk.uvw[:, 2][:, None] * chunk.freq[None, :] / C_LIGHT  # (nrow, nchan)
+        return np.vectorize(self.ws.plane)(w)
+
+    def _plane_chunk(self, chunk: Chunk, planes, iw):
+        keep = (planes == iw)[:, :, None]
+        base = np.zeros(chunk.vis.shape, bool) if chunk.flag is None else chunk.flag
+        flag = base | ~keep  # samples of the other planes are left out as flagged ones are
+        return Chunk(chunk.uvw, chunk.freq, chunk.vis, noise=chunk.noise, flag=np.ascontiguousarray(flag),
+                     delay=chunk.delay)
+
+    def grid(self, shape, cell, chunk: Chunk):
+        planes = self._planes(chunk)
+        self.grids, self.sumw = [], 0.0
+        for iw in range(self.ws.nw):
+            o = OracleGridder("SphFunc", self.precision)
+            o.initialise_grid(shape, cell)
+            if np.any(planes == iw):
+                o.grid(self._plane_chunk(chunk, planes, iw))
+            self.grids.append(o.grid_.astype(np.complex128))
+            self.sumw = self.sumw + o.finalise_weights()
+        self.o = o
+        return self.grids
+
+    def finalise_grid(self):
+        o = self.o
+        acc = 0.0
+        for iw, g in enumerate(self.grids):
+            acc = acc + (fft_centred(g, False) * wstack_screen(o.nx, o.ny, o.uvcell, self.ws.wterm(iw))).real
+        img = o.correct(acc) * float(o.nx) * float(o.ny)
+        nx0, ny0 = o.shape0[0], o.shape0[1]
+        ox, oy = (o.nx - nx0) // 2, (o.ny - ny0) // 2
+        return img[:, :, oy:oy + ny0, ox:ox + nx0]
+
+    def degrid(self, shape, cell, model, chunk: Chunk):
+        planes = self._planes(chunk)
+        out = 0.0
+        for iw in range(self.ws.nw):
+            o = OracleGridder("SphFunc", self.precision)
+            o.initialise_degrid(shape, cell, model)
+            pad = np.zeros((o.nchan_img, o.npol, o.ny, o.nx))
+            ox, oy = (o.nx - shape[0]) // 2, (o.ny - shape[1]) // 2
+            pad[:, :, oy:oy + shape[1], ox:ox + shape[0]] = model
+            work = np.conj(o.correct(pad) * wstack_screen(o.nx, o.ny, o.uvcell, self.ws.wterm(iw)))
+            o.grid_ = fft_centred(work, True).astype(o.grid_.dtype)
+            if np.any(planes == iw):
+                out = out + o.degrid(self._plane_chunk(chunk, planes, iw))
+        return out
+
+
 # ---- polarisation conversion (PolConverter as used at TableVisGridder.cc:517-524,682-693,860) ----
 # PolConverter belongs to base-scimath (absent).  Restated from the casacore / IAU definitions of
 # the polarisation products (casacore Stokes codes I=1 Q=2 U=3 V=4 RR=5 RL=6 LR=7 LL=8 XX=9 XY=10

@@ -1,0 +1,73 @@
+"""W stacking (GPUWStack; WStackVisGridder.cc) against the oracle's composition (OracleWStack):
+every sample grids into the grid of its w plane with the SphFunc kernel; the image is the sum
+over the planes of the real part of the w-screened inverse transforms, corrected; degridding
+predicts every sample from the grid of its plane, filled from the conjugate-screened corrected
+model."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import announce
+from tests.test_gpu_parity import ARC, TOL, gain_of, image_ok, make_data, ochunk, pchunk, random_model, rel
+
+pytestmark = pytest.mark.gpu
+
+WS = dict(wmax=2000.0, nwplanes=9)
+
+
+def _data(npol, seed):
+    N, cell = 128, 60 * ARC
+    return N, cell, make_data(900, 3, npol, N, cell, f0=1.0e9, frac=0.35, wmax_m=1800.0 * O.C_LIGHT / 1.01e9,
+                              seed=seed, flags=True)
+
+
+@pytest.mark.parametrize("npol", [1, 2])
+def test_wstack_grid_and_image_match_oracle(gpu_ctx, npol):
+    announce(f"wstack grid npol={npol}")
+    import yandasoft_amd as ya
+    N, cell, d = _data(npol, 70 + npol)
+    g = ya.Gridder(gpu_ctx, "GPUWStack", dict(WS))
+    g.initialise_grid(ya.ImageDesc(N, N, npol, 1, cell, cell))
+    g.grid(pchunk(d))
+    o = O.OracleWStack(WS["wmax"], WS["nwplanes"])
+    grids = o.grid((N, N, npol, 1), (cell, cell), ochunk(d))
+    gg = g.grid_array().reshape(WS["nwplanes"], 1, npol, N, N)
+    used = [iw for iw in range(WS["nwplanes"]) if np.abs(grids[iw]).max() > 0]
+    assert len(used) >= 3  # the data spread over several planes
+    peak = max(np.abs(x).max() for x in grids)
+    for iw in range(WS["nwplanes"]):
+        assert np.abs(gg[iw] - grids[iw]).max() <= TOL * peak, iw
+    gi = g.finalise_grid()
+    o64 = O.OracleWStack(WS["wmax"], WS["nwplanes"], precision="f64")
+    o64.grid((N, N, npol, 1), (cell, cell), ochunk(d))
+    # as for the single-grid images: TOL inside gain <= 10, twice the FP32 oracle's own error at
+    # the corrected edges
+    ok, info = image_ok(gi, o.finalise_grid(), o64.finalise_grid(), gain_of(o.o))
+    assert ok, f"image parity (inner, full, f32-oracle) = {info}"
+    assert rel(g.finalise_weights(), o.sumw) < 1e-6
+
+
+@pytest.mark.parametrize("npol", [1, 4])
+def test_wstack_degrid_matches_oracle(gpu_ctx, npol):
+    announce(f"wstack degrid npol={npol}")
+    import yandasoft_amd as ya
+    N, cell, d = _data(npol, 80 + npol)
+    model = random_model(N, npol, seed=9)
+    g = ya.Gridder(gpu_ctx, "GPUWStack", dict(WS))
+    g.initialise_degrid(ya.ImageDesc(N, N, npol, 1, cell, cell), model.astype(np.float32))
+    out = np.zeros_like(d["vis"])
+    g.degrid(pchunk(d, vis_out=out))
+    ov = O.OracleWStack(WS["wmax"], WS["nwplanes"]).degrid((N, N, npol, 1), (cell, cell), model, ochunk(d))
+    assert rel(np.transpose(out, (2, 1, 0)), ov) < TOL
+
+
+def test_wstack_rejects_what_it_does_not_do(gpu_ctx):
+    announce("wstack: PCF mode and batches are errors")
+    import yandasoft_amd as ya
+    N, cell, d = _data(1, 90)
+    g = ya.Gridder(gpu_ctx, "GPUWStack", dict(WS))
+    g.initialise_grid(ya.ImageDesc(N, N, 1, 1, cell, cell), dopcf=True)
+    with pytest.raises(ya.YgridError):
+        g.grid(pchunk(d))
+    with pytest.raises(ya.YgridError):
+        ya.Gridder(gpu_ctx, "GPUWStack", {"nwplanes": 8})  # odd number of planes

@@ -467,3 +467,19 @@ def test_pol_noise_closed_forms():
     f[2, 0, 1] = True
     _, _, fo = O.pol_to_image(None, None, f, [XX, XY, YX, YY], [SI, SQ])
     assert fo.shape == (2, 1, 2) and not fo[:, 0, 0].any() and fo[:, 0, 1].all()
+
+
+# ---- w stacking (oracle.wstack_screen) ----------------------------------------------------------
+def test_wstack_screen_closed_forms():
+    """w = 0: no change; |screen| = 1; the phase 2 pi w (1 - sqrt(1 - r^2)) at a pixel computed
+    by hand; pixels with r^2 >= 1 left unchanged."""
+    nx, ny, uvcell = 16, 8, (1.0 / (16 * 0.1), 1.0 / (8 * 0.15))  # image cells 0.1 and 0.15 (radians)
+    assert np.allclose(O.wstack_screen(nx, ny, uvcell, 0.0), 1.0)
+    s = O.wstack_screen(nx, ny, uvcell, 40.0)
+    assert np.allclose(np.abs(s), 1.0, atol=1e-6)
+    x, y = 0.1 * (11 - 8), 0.15 * (2 - 4)  # pixel (ix 11, iy 2)
+    p = 2 * np.pi * 40.0 * (1 - np.sqrt(1 - x * x - y * y))
+    assert abs(s[2, 11] - np.exp(-1j * p)) < 2e-5
+    # corner (0, 0): r^2 = 0.8^2 + 0.6^2 = 1 -> outside, unchanged
+    assert s[0, 0] == 1.0
+    assert np.angle(s[4, 8]) == 0.0  # the centre

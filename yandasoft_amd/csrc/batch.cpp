@@ -37,6 +37,7 @@ void Batch::invalidate() {
 void Batch::add(Gridder* g, const ygrid_chunk& a) {
   YG_CHECK(g != nullptr, 1, "null gridder");
   YG_CHECK(a.mem == YGRID_MEM_DEVICE, 1, "batched passes need chunks in device memory");
+  YG_CHECK(g->nGridPlanes() == 1, 1, "batched passes take single-grid gridders (not w stacking)");
   YG_CHECK(!g->needsPolConversion(a), 1,
            "batched passes take visibilities in the image's polarisation frame (convert with a single pass)");
   for (const Pass& p : passes_) YG_CHECK(p.g != g, 1, "a gridder can take part in one pass of a batch only");

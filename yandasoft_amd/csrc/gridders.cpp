@@ -215,7 +215,7 @@ void Gridder::initialiseGrid(const ygrid_image_desc& img, bool dopsf, bool dopcf
   dopsf_ = dopsf;
   dopcf_ = dopcf;
   degrid_mode_ = false;
-  const size_t n = (size_t)shape_[0] * shape_[1] * shape_[2] * shape_[3];
+  const size_t n = gridPlaneElems() * nGridPlanes();
   grid_.resize(n * sizeof(float2));
   YG_HIP(hipMemsetAsync(grid_.get(), 0, n * sizeof(float2), ctx_->stream));
   first_psf_ = true;
@@ -389,6 +389,12 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
   if (zdiv < 1) zdiv = 1;
   pp.zdiv = (int)zdiv;
   pp.nzg = (int)((nz + zdiv - 1) / zdiv);
+  if (nGridPlanes() > 1) {
+    // w stacking: the key names the grid, one group per w plane
+    YG_CHECK((long long)nGridPlanes() * ntiles * shape_[3] <= budget, 1, "too many w planes x tiles x channels");
+    pp.zdiv = 1;
+    pp.nzg = nGridPlanes();
+  }
   pp.vis_prec = vis_prec_;
 
   PrepInputs& in = ps.in;
@@ -455,12 +461,14 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   if (forward) {
     YG_HIP(hipMemsetAsync(acc_.get(), 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, acc_.as<float>());
+    launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, acc_.as<float>(),
+                  nGridPlanes() > 1 ? gridPlaneElems() : 0);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
     launch_degrid_finish(*ctx_, smp, acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);
   } else {
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg);
+    launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg,
+                nGridPlanes() > 1 ? gridPlaneElems() : 0);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
   }
   unsigned long long cnt[4] = {0, 0, 0, 0};
@@ -511,12 +519,24 @@ void Gridder::finaliseGrid(float* out, int mem) {
   }
   // FP64 transform: the grid correction amplifies the image edges by 1/grdsf (~1e2), which
   // would lift an FP32 transform's rounding there to ~1e-4 of the image peak
-  grid_to_image_z(*ctx_, grid_.as<float2>(), nxp, nyp, np, cx, cy, img_[0], img_[1], scratch_.as<double2>(), dout,
-                  plan_);
+  gridsToImage(cx, cy, dout);
   if (mem != YGRID_MEM_DEVICE)
     YG_HIP(hipMemcpyAsync(out, dout, nout * sizeof(float), hipMemcpyDeviceToHost, ctx_->stream));
   YG_HIP(hipStreamSynchronize(ctx_->stream));
   if (clear_grid_) grid_.release();
+}
+
+void Gridder::gridsToImage(const double* cx, const double* cy, float* dout) {
+  const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
+  grid_to_image_z(*ctx_, grid_.as<float2>(), nxp, nyp, np, cx, cy, img_[0], img_[1], scratch_.as<double2>(), dout,
+                  plan_);
+}
+
+void Gridder::modelToGrids(const float* dmodel, const double* cx, const double* cy) {
+  const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
+  scratch_.resize(gridPlaneElems() * sizeof(double2));
+  model_to_grid_z(*ctx_, dmodel, img_[0], img_[1], np, cx, cy, nxp, nyp, scratch_.as<double2>(), grid_.as<float2>(),
+                  plan_);
 }
 
 // TableVisGridder::finaliseWeights (TableVisGridder.cc:1255-1284)
@@ -566,7 +586,7 @@ void Gridder::initialiseDegrid(const ygrid_image_desc& img, const float* model, 
     YG_HIP(hipStreamSynchronize(ctx_->stream));
     model_empty_ = h == 0;
   }
-  const size_t ng = (size_t)nxp * nyp * np;
+  const size_t ng = (size_t)nxp * nyp * np * nGridPlanes();
   grid_.resize(ng * sizeof(float2));
   if (model_empty_) {
     YG_HIP(hipMemsetAsync(grid_.get(), 0, ng * sizeof(float2), ctx_->stream));
@@ -576,9 +596,7 @@ void Gridder::initialiseDegrid(const ygrid_image_desc& img, const float* model, 
   if (doesCorrection()) uploadCorrection();
   const double* cx = doesCorrection() ? ccfx_.as<double>() : nullptr;
   const double* cy = doesCorrection() ? ccfy_.as<double>() : nullptr;
-  scratch_.resize(ng * sizeof(double2));
-  model_to_grid_z(*ctx_, dmodel, img_[0], img_[1], np, cx, cy, nxp, nyp, scratch_.as<double2>(), grid_.as<float2>(),
-                  plan_);
+  modelToGrids(dmodel, cx, cy);
   YG_HIP(hipStreamSynchronize(ctx_->stream));
   scratch_.release();
 }
@@ -635,7 +653,7 @@ void Gridder::getCF(int k, float* out, int* support) const {
 }
 
 void Gridder::getGrid(float* out, size_t* n) const {
-  const size_t ng = (size_t)shape_[0] * shape_[1] * shape_[2] * shape_[3];
+  const size_t ng = gridPlaneElems() * nGridPlanes();
   if (n) *n = 2 * ng;
   if (!out) return;
   YG_CHECK(grid_.get(), 2, "no grid");
@@ -772,6 +790,25 @@ struct WSamplingDesc {
   }
 };
 
+// WDependentGridderBase::configureWSampling (WDependentGridderBase.cc:192-245)
+static void configure_wsampling(WSamplingDesc& ws, const Params& p) {
+  const std::string smp = p.str("wsampling", "linear");
+  if (smp == "powerlaw") {
+    ws.kind = 1;
+    ws.exponent = p.f64("wsampling.exponent", 1.0);
+    YG_CHECK(ws.exponent != 0., 1, "Zero exponent in power law doesn't make sense");
+  } else if (smp == "gaussian") {
+    ws.kind = 2;
+    YG_CHECK(p.has("wsampling.nwplanes50"), 1, "wsampling.nwplanes50 is required");
+    const double n50 = p.f64("wsampling.nwplanes50", 1.0);
+    YG_CHECK(n50 > 0. && n50 < double(ws.nw) / std::sqrt(2.), 1, "wsampling.nwplanes50 out of range");
+    ws.gaussian(n50 / ws.nw);
+  } else {
+    YG_CHECK(smp == "linear", 1, "W-sampling " + smp + " is not implemented");
+  }
+  ws.wmaxclip = p.flag("wmaxclip", false);
+}
+
 class WProjectGridder : public Gridder {
  public:
   WProjectGridder(std::shared_ptr<Ctx> c, const Params& p) : Gridder(std::move(c)) {
@@ -792,22 +829,7 @@ class WProjectGridder : public Gridder {
     YG_CHECK(maxsupport_ > 0, 1, "Maximum support must be greater than 0");
     YG_CHECK(!offset_ || variable_, 1, "offsetsupport option of the gridder should only be used together with variablesupport option");
     if (!abs_cutoff_) YG_CHECK(cutoff_ > 0.0 && cutoff_ < 1.0, 1, "Cutoff must be in (0, 1)");
-    // WDependentGridderBase::configureWSampling (WDependentGridderBase.cc:192-245)
-    const std::string smp = p.str("wsampling", "linear");
-    if (smp == "powerlaw") {
-      ws_.kind = 1;
-      ws_.exponent = p.f64("wsampling.exponent", 1.0);
-      YG_CHECK(ws_.exponent != 0., 1, "Zero exponent in power law doesn't make sense");
-    } else if (smp == "gaussian") {
-      ws_.kind = 2;
-      YG_CHECK(p.has("wsampling.nwplanes50"), 1, "wsampling.nwplanes50 is required");
-      const double n50 = p.f64("wsampling.nwplanes50", 1.0);
-      YG_CHECK(n50 > 0. && n50 < double(ws_.nw) / std::sqrt(2.), 1, "wsampling.nwplanes50 out of range");
-      ws_.gaussian(n50 / ws_.nw);
-    } else {
-      YG_CHECK(smp == "linear", 1, "W-sampling " + smp + " is not implemented");
-    }
-    ws_.wmaxclip = p.flag("wmaxclip", false);
+    configure_wsampling(ws_, p);
   }
   std::unique_ptr<Gridder> clone() const override { return std::unique_ptr<Gridder>(new WProjectGridder(*this)); }
   std::string name() const override { return "GPUWProject"; }
@@ -1089,6 +1111,78 @@ void WProjectGridder::initConvolutionFunction(const ChunkDev&) {
 // ---------------------------------------------------------------------------------------
 std::unique_ptr<Gridder> make_awproject(std::shared_ptr<Ctx> ctx, const Params& p);
 
+// ---------------------------------------------------------------------------------------
+// WStack (WStackVisGridder.cc): the SphFunc kernel (WDependentGridderBase derives from
+// SphFuncVisGridder with its defaults, alpha 1, support 3, oversample 128) gridding each sample
+// into the grid of its w plane (gIndex, :339-344); finaliseGrid transforms every plane, applies
+// its w screen and sums the real parts (:255-289), initialiseDegrid fills every plane from the
+// corrected model times the conjugate screen (:314-327).
+// ---------------------------------------------------------------------------------------
+class WStackGridder : public SphFuncGridder {
+ public:
+  WStackGridder(std::shared_ptr<Ctx> c, const Params& p) : SphFuncGridder(std::move(c), 1.0, 3, 128) {
+    // WStackVisGridder::createGridder (WStackVisGridder.cc:355-363)
+    ws_.wmax = p.f64("wmax", 35000.0);
+    ws_.nw = p.i32("nwplanes", 65);
+    YG_CHECK(ws_.wmax > 0.0, 1, "Baseline length must be greater than zero");
+    YG_CHECK(ws_.nw > 0 && ws_.nw % 2 == 1, 1, "Number of w planes must be a positive odd number");
+    configure_wsampling(ws_, p);
+  }
+  std::unique_ptr<Gridder> clone() const override { return std::unique_ptr<Gridder>(new WStackGridder(*this)); }
+  std::string name() const override { return "GPUWStack"; }
+
+ protected:
+  int nGridPlanes() const override { return ws_.nw; }
+  CIndexDesc cindex(const ChunkDev&) const override {
+    YG_CHECK(!dopcf_, 1, "the w-stacking gridder has no preconditioner (PCF) mode here");
+    CIndexDesc d;
+    d.kind = 3;
+    d.nw = ws_.nw;
+    d.wscale = ws_.wscale();
+    d.wmax = ws_.wmax;
+    d.sampling = ws_.kind;
+    d.exponent = ws_.exponent;
+    d.g_ts = ws_.ts;
+    d.g_amp = ws_.amp;
+    d.wmaxclip = ws_.wmaxclip ? 1 : 0;
+    return d;
+  }
+  // the screen's constants as WStackVisGridder::multiply computes them (:137-144)
+  void screen(int iw, float& w2pi, float& cellx, float& celly) const {
+    w2pi = (float)(2.0f * kPi * ws_.wterm(iw));
+    cellx = (float)(1.0 / (float(shape_[0]) * uvcell_[0]));
+    celly = (float)(1.0 / (float(shape_[1]) * uvcell_[1]));
+  }
+  void gridsToImage(const double* cx, const double* cy, float* dout) override {
+    const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
+    const size_t ng = gridPlaneElems();
+    DevBuf acc(ng * sizeof(double));
+    launch_fill_u32(*ctx_, acc.get(), 2 * ng, 0u);
+    for (int iw = 0; iw < ws_.nw; ++iw) {
+      float w2pi, cellx, celly;
+      screen(iw, w2pi, cellx, celly);
+      wstack_accumulate(*ctx_, grid_.as<float2>() + ng * iw, nxp, nyp, np, w2pi, cellx, celly, scratch_.as<double2>(),
+                        acc.as<double>(), plan_);
+    }
+    wstack_finalise(*ctx_, acc.as<double>(), nxp, nyp, np, cx, cy, img_[0], img_[1], dout);
+    YG_HIP(hipStreamSynchronize(ctx_->stream));
+  }
+  void modelToGrids(const float* dmodel, const double* cx, const double* cy) override {
+    const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
+    const size_t ng = gridPlaneElems();
+    scratch_.resize(ng * sizeof(double2));
+    for (int iw = 0; iw < ws_.nw; ++iw) {
+      float w2pi, cellx, celly;
+      screen(iw, w2pi, cellx, celly);
+      wstack_model_to_grid(*ctx_, dmodel, img_[0], img_[1], np, cx, cy, nxp, nyp, w2pi, cellx, celly,
+                           scratch_.as<double2>(), grid_.as<float2>() + ng * iw, plan_);
+    }
+  }
+
+ private:
+  WSamplingDesc ws_;
+};
+
 std::unique_ptr<Gridder> make_gridder(std::shared_ptr<Ctx> ctx, const std::string& name, const ParamMap& parset) {
   const Params p{parset};
   std::unique_ptr<Gridder> g;
@@ -1099,6 +1193,8 @@ std::unique_ptr<Gridder> make_gridder(std::shared_ptr<Ctx> ctx, const std::strin
     g.reset(new SphFuncGridder(ctx, p.f64("alpha", 1.0), p.i32("support", 3), p.i32("oversample", 128)));
   } else if (name == "GPUWProject") {
     g.reset(new WProjectGridder(ctx, p));
+  } else if (name == "GPUWStack") {
+    g.reset(new WStackGridder(ctx, p));
   } else if (name == "GPUAWProject") {
     g = make_awproject(ctx, p);
   } else {

@@ -135,6 +135,12 @@ class Gridder {
   virtual CIndexDesc cindex(const ChunkDev& acc) const { (void)acc; return CIndexDesc{}; }
   // number of CF planes before oversampling (rows of the sum of weights)
   virtual int nZPlanes() const { return 1; }
+  // grids per image plane (w stacking: one per w plane; the tile kernels select it by the
+  // sample's w plane) and the transforms between them and the image
+  virtual int nGridPlanes() const { return 1; }
+  virtual void gridsToImage(const double* ccfx, const double* ccfy, float* out);
+  virtual void modelToGrids(const float* model, const double* ccfx, const double* ccfy);
+  size_t gridPlaneElems() const { return (size_t)shape_[0] * shape_[1] * shape_[2] * shape_[3]; }
   // itsOverSample as configured (the CF cache may not exist yet)
   virtual int oversampling() const { return 1; }
   // 1-D factors of correctConvolution (empty: no correction, e.g. Box)

@@ -244,6 +244,97 @@ void grid_to_image_z(const Ctx& c, const float2* grid, int nxp, int nyp, int npl
   YG_HIP(hipGetLastError());
 }
 
+namespace {
+// WStackVisGridder::multiply's screen at padded pixel (X, Y), in its float arithmetic
+// (WStackVisGridder.cc:134-171); false outside r^2 < 1 (left unchanged there).
+__device__ __forceinline__ bool wstack_screen(int X, int Y, int nxp, int nyp, float w2pi, float cellx, float celly,
+                                              float& cr, float& ci) {
+#pragma clang fp contract(off)
+  float y2 = float(Y - nyp / 2) * celly;
+  y2 *= y2;
+  float x2 = float(X - nxp / 2) * cellx;
+  x2 *= x2;
+  const float r2 = x2 + y2;
+  if (!((double)r2 < 1.0)) return false;
+  const float phase = (float)((double)w2pi * (1.0 - sqrt(1.0 - (double)r2)));
+  cr = cosf(phase);
+  ci = -sinf(phase);
+  return true;
+}
+
+__global__ void k_wstack_acc(const double2* __restrict__ z, int nxp, int nyp, float w2pi, float cellx, float celly,
+                             double gsign, double* __restrict__ acc) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y;
+  if (X >= nxp) return;
+  const size_t i = (size_t)blockIdx.z * nxp * nyp + (size_t)Y * nxp + X;
+  const double s = cb_sign(X, Y) * gsign;
+  const double2 v = make_double2(z[i].x * s, z[i].y * s);
+  float cr, ci;
+  acc[i] += wstack_screen(X, Y, nxp, nyp, w2pi, cellx, celly, cr, ci) ? v.x * (double)cr - v.y * (double)ci : v.x;
+}
+
+__global__ void k_wstack_final(const double* __restrict__ acc, int nxp, int nyp, const double* __restrict__ ccfx,
+                               const double* __restrict__ ccfy, int nx, int ny, int ox, int oy,
+                               float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  const int X = x + ox, Y = y + oy;
+  const double c = (ccfx ? ccfx[X] : 1.0) * (ccfy ? ccfy[Y] : 1.0);
+  out[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] = (float)(acc[(size_t)blockIdx.z * nxp * nyp + (size_t)Y * nxp + X] * c);
+}
+
+// conj(screen * m): m (cos p, sin p) inside r^2 < 1, (m, 0) outside; centring signs applied
+__global__ void k_wstack_model_to_z(const float* __restrict__ model, int nx, int ny, int ox, int oy,
+                                    const double* __restrict__ ccfx, const double* __restrict__ ccfy, int nxp, int nyp,
+                                    float w2pi, float cellx, float celly, double2* __restrict__ z) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y;
+  if (X >= nxp) return;
+  const int x = X - ox, y = Y - oy;
+  double2 val = make_double2(0.0, 0.0);
+  if (x >= 0 && x < nx && y >= 0 && y < ny) {
+    const double c = (ccfx ? ccfx[X] : 1.0) * (ccfy ? ccfy[Y] : 1.0);
+    const double m = (double)model[(size_t)blockIdx.z * nx * ny + (size_t)y * nx + x] * c * cb_sign(X, Y);
+    float cr, ci;
+    if (m != 0.0 && wstack_screen(X, Y, nxp, nyp, w2pi, cellx, celly, cr, ci))
+      val = make_double2(m * (double)cr, -m * (double)ci);
+    else
+      val = make_double2(m, 0.0);
+  }
+  z[(size_t)blockIdx.z * nxp * nyp + (size_t)Y * nxp + X] = val;
+}
+}  // namespace
+
+void wstack_accumulate(const Ctx& c, const float2* grid, int nxp, int nyp, int nplanes, float w2pi, float cellx,
+                       float celly, double2* scratch, double* acc, PlanPtr& cache) {
+  const dim3 gp((nxp + 255) / 256, nyp, nplanes);
+  hipLaunchKernelGGL(k_grid_to_z, gp, dim3(256), 0, c.stream, grid, nxp, nyp, scratch);
+  exec_z2z(c, scratch, nxp, nyp, nplanes, false, cache);
+  hipLaunchKernelGGL(k_wstack_acc, gp, dim3(256), 0, c.stream, scratch, nxp, nyp, w2pi, cellx, celly,
+                     (double)global_sign(nxp, nyp), acc);
+  YG_HIP(hipGetLastError());
+}
+
+void wstack_finalise(const Ctx& c, const double* acc, int nxp, int nyp, int nplanes, const double* ccfx,
+                     const double* ccfy, int nx, int ny, float* out) {
+  hipLaunchKernelGGL(k_wstack_final, dim3((nx + 255) / 256, ny, nplanes), dim3(256), 0, c.stream, acc, nxp, nyp, ccfx,
+                     ccfy, nx, ny, (nxp - nx) / 2, (nyp - ny) / 2, out);
+  YG_HIP(hipGetLastError());
+}
+
+void wstack_model_to_grid(const Ctx& c, const float* model, int nx, int ny, int nplanes, const double* ccfx,
+                          const double* ccfy, int nxp, int nyp, float w2pi, float cellx, float celly, double2* scratch,
+                          float2* grid, PlanPtr& cache) {
+  const dim3 gp((nxp + 255) / 256, nyp, nplanes);
+  hipLaunchKernelGGL(k_wstack_model_to_z, gp, dim3(256), 0, c.stream, model, nx, ny, (nxp - nx) / 2, (nyp - ny) / 2,
+                     ccfx, ccfy, nxp, nyp, w2pi, cellx, celly, scratch);
+  exec_z2z(c, scratch, nxp, nyp, nplanes, true, cache);
+  hipLaunchKernelGGL(k_z_to_grid, gp, dim3(256), 0, c.stream, scratch, nxp, nyp, (double)global_sign(nxp, nyp), grid);
+  YG_HIP(hipGetLastError());
+}
+
 // grid[plane] = centred unnormalised forward transform of ccf * padded(model[plane])
 void model_to_grid_z(const Ctx& c, const float* model, int nx, int ny, int nplanes, const double* ccfx,
                      const double* ccfy, int nxp, int nyp, double2* scratch, float2* grid, PlanPtr& cache) {

@@ -160,13 +160,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
       valid = false;
     } else {
       // cIndex
-      int plane = 0;
+      int plane = 0, gplane = 0;
       if (p.ci.kind != 0) {
         const double w = in.uvw[3 * row + 2] / kC;
         const int iw = w_plane(p.ci, w * f);
         if (iw < 0) {
           if (!p.ci.wmaxclip || iw == -2) err |= 2;
           plane = -1;
+        } else if (p.ci.kind == 3) {
+          gplane = iw;  // WStackVisGridder::gIndex (WStackVisGridder.cc:339-344); cIndex 0
+          plane = 0;
         } else if (p.ci.kind == 2) {
           const int feed = in.feed1 ? in.feed1[row] : 0;
           const int c = p.ci.freqdep ? chan : 0;
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
             rec.cfoff = learned ? (int32_t)((long long)s * p.lcf_plane_elems) : in.cf_off[cInd];
             if (learned) lcf = make_float4((float)(in.uvw[3 * row + 2] / kC * f), (float)fracu, (float)fracv, 1.f);
             rec.spack = S | ((2 * S + 1 + 2 * kGuardU) << 16);
-            key = ichan * p.nzg + plane / p.zdiv;
+            key = ichan * p.nzg + (p.ci.kind == 3 ? gplane : plane / p.zdiv);
             swrow = p.track_per_os ? cInd : plane;
             const float vw = in.visweight ? in.visweight[chan] : 1.0f;
             if (grid_mode) {
@@ -801,7 +804,7 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     const int kz = it.x / J.ntiles;
     const int ichan = kz / J.nzg;
     const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
-    float* __restrict__ grid = J.grid;
+    float* __restrict__ grid = J.grid + 2 * J.wstride * (size_t)(kz % J.nzg);
 
     f32x4 acc[4][2][2][NS];
 #pragma unroll
@@ -958,7 +961,7 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     const int ichan = kz / J.nzg;
     const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
     const size_t plane = (size_t)nx * ny;
-    const float2* __restrict__ grid = reinterpret_cast<const float2*>(J.grid);
+    const float2* __restrict__ grid = reinterpret_cast<const float2*>(J.grid) + J.wstride * (size_t)(kz % J.nzg);
     float* __restrict__ acc_out = J.acc;
 
     float2 gt[4][2][2][NP];
@@ -1144,7 +1147,7 @@ __global__ __launch_bounds__(256, 2) void k_degrid_mm(const int4* __restrict__ i
     const int ichan = kz / J.nzg;
     const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
     const size_t plane = (size_t)nx * ny;
-    const float* __restrict__ gridf = J.grid;
+    const float* __restrict__ gridf = J.grid + 2 * J.wstride * (size_t)(kz % J.nzg);
     float* __restrict__ acc_out = J.acc;
 
     // model-grid tile: gt[r][h][e][m] = output j of polarisation pair m at tile pixel
@@ -1525,7 +1528,7 @@ static void with_tile_kernels(const TileVariant& v, int npol, int prec, F&& f) {
 }
 
 TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, float* grid, float* acc, int nx, int ny,
-                      int npol, int nzg) {
+                      int npol, int nzg, size_t wstride) {
   TileJob j{};
   j.descs = descs;
   j.smp = smp;
@@ -1537,6 +1540,7 @@ TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, f
   j.ntu = (nx + kTile - 1) / kTile;
   j.ntiles = j.ntu * ((ny + kTile - 1) / kTile);
   j.nzg = nzg;
+  j.wstride = wstride;
   j.npol = npol;
   return j;
 }
@@ -1684,8 +1688,9 @@ void launch_tile_jobs(const Ctx& c, const TileVariant& v0, bool forward, int npo
 }
 
 void launch_grid(const Ctx& c, Binning& b, const Samples& smp, const CFView& cf, float2* grid, int nx, int ny,
-                 int npol, int nzg) {
-  b.job_host = make_tile_job(b.pairs.as<int4>(), smp, cf, reinterpret_cast<float*>(grid), nullptr, nx, ny, npol, nzg);
+                 int npol, int nzg, size_t wstride) {
+  b.job_host = make_tile_job(b.pairs.as<int4>(), smp, cf, reinterpret_cast<float*>(grid), nullptr, nx, ny, npol, nzg,
+                             wstride);
   b.job.resize(sizeof(TileJob));
   YG_HIP(hipMemcpyAsync(b.job.get(), &b.job_host, sizeof(TileJob), hipMemcpyHostToDevice, c.stream));
   launch_tile_jobs(c, c.grid_variant, false, npol, cf.prec, b.job.as<TileJob>(), b.items.as<int4>(),
@@ -1693,9 +1698,9 @@ void launch_grid(const Ctx& c, Binning& b, const Samples& smp, const CFView& cf,
 }
 
 void launch_degrid(const Ctx& c, Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol, int nzg,
-                   float* acc) {
+                   float* acc, size_t wstride) {
   b.job_host = make_tile_job(b.pairs.as<int4>(), Samples{}, cf, const_cast<float*>(reinterpret_cast<const float*>(grid)),
-                             acc, nx, ny, npol, nzg);
+                             acc, nx, ny, npol, nzg, wstride);
   b.job.resize(sizeof(TileJob));
   YG_HIP(hipMemcpyAsync(b.job.get(), &b.job_host, sizeof(TileJob), hipMemcpyHostToDevice, c.stream));
   launch_tile_jobs(c, c.degrid_variant, true, npol, cf.prec, b.job.as<TileJob>(), b.items.as<int4>(),

@@ -84,7 +84,8 @@ class DevBuf {
 // overrides (TableVisGridder.cc:1410-1412; WProjectVisGridder.cc:114-147,286-292;
 // AWProjectVisGridder.cc:123-172,855-863).
 struct CIndexDesc {
-  int kind = 0;            // 0: constant 0; 1: w-plane; 2: w-plane + feed/chan (AW)
+  int kind = 0;            // 0: constant 0; 1: w-plane; 2: w-plane + feed/chan (AW); 3: w plane
+                           // selects the grid (w stacking), CF plane 0
   int nw = 1;              // number of w planes
   double wscale = 1.0;     // linear sampling step (WDependentGridderBase.cc:60-65)
   double wmax = 1.0;
@@ -195,6 +196,7 @@ struct TileJob {
   float* grid;
   float* acc;
   int nx, ny, ntu, ntiles, nzg, npol;
+  size_t wstride;  // float2 elements between the grids of consecutive w planes (w stacking), else 0
 };
 
 struct Binning {
@@ -257,12 +259,12 @@ void launch_binning(const Ctx&, const Samples& smp, int nrow, int nchan, int nx,
 // headers and keys with the same parameters; returns true when the cache was reused.
 bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny, int nkeys_z, const CFView& cf);
 void launch_grid(const Ctx&, Binning& b, const Samples& smp, const CFView& cf, float2* grid, int nx, int ny,
-                 int npol, int nzg);
+                 int npol, int nzg, size_t wstride = 0);
 void launch_degrid(const Ctx&, Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol,
-                   int nzg, float* acc);
+                   int nzg, float* acc, size_t wstride = 0);
 // The job of one grid for the tile kernels (descs: its binning's pair descriptors).
 TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, float* grid, float* acc, int nx, int ny,
-                      int npol, int nzg);
+                      int npol, int nzg, size_t wstride = 0);
 // One persistent tile-kernel launch over the work items of several jobs (all with npol
 // polarisations and CF precision prec): items[0 .. *nitems) in device memory, counter zeroed
 // by the launch's own fill kernel.  forward: degridding.
@@ -320,6 +322,19 @@ void grid_to_image_z(const Ctx&, const float2* grid, int nxp, int nyp, int nplan
                      const double* ccfy, int nx, int ny, double2* scratch, float* out, PlanPtr& cache);
 void model_to_grid_z(const Ctx&, const float* model, int nx, int ny, int nplanes, const double* ccfx,
                      const double* ccfy, int nxp, int nyp, double2* scratch, float2* grid, PlanPtr& cache);
+// W stacking (WStackVisGridder.cc:134-171,255-289,314-327).  The w screen of plane w2pi = 2 pi w
+// at padded pixel (X, Y): (cos p, -sin p), p = w2pi (1 - sqrt(1 - r^2)), r^2 = ((X - nxp/2) cellx)^2
+// + ((Y - nyp/2) celly)^2, where r^2 < 1 (1 elsewhere), in the reference's float arithmetic.
+// acc (nplanes x nyp x nxp) += Re(screen * centred unnormalised inverse transform of grid).
+void wstack_accumulate(const Ctx&, const float2* grid, int nxp, int nyp, int nplanes, float w2pi, float cellx,
+                       float celly, double2* scratch, double* acc, PlanPtr& cache);
+// image = ccf * acc, the unpadded centre
+void wstack_finalise(const Ctx&, const double* acc, int nxp, int nyp, int nplanes, const double* ccfx,
+                     const double* ccfy, int nx, int ny, float* out);
+// grid = centred unnormalised forward transform of conj(screen * ccf * padded(model))
+void wstack_model_to_grid(const Ctx&, const float* model, int nx, int ny, int nplanes, const double* ccfx,
+                          const double* ccfy, int nxp, int nyp, float w2pi, float cellx, float celly, double2* scratch,
+                          float2* grid, PlanPtr& cache);
 void launch_finalise_image(const Ctx&, const float2* fftd, int nxp, int nyp, int nplanes, const double* ccfx,
                            const double* ccfy, int nx, int ny, float* out);
 void launch_model_to_grid(const Ctx&, const float* model, int nx, int ny, int nplanes, const double* ccfx,
