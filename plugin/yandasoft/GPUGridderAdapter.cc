@@ -212,12 +212,15 @@ IVisGridder::ShPtr GPUGridderAdapter::createGPUWProject(const LOFAR::ParameterSe
 IVisGridder::ShPtr GPUGridderAdapter::createGPUAWProject(const LOFAR::ParameterSet& p) {
   return IVisGridder::ShPtr(new GPUGridderAdapter("GPUAWProject", p));
 }
+IVisGridder::ShPtr GPUGridderAdapter::createGPUWStack(const LOFAR::ParameterSet& p) {
+  return IVisGridder::ShPtr(new GPUGridderAdapter("GPUWStack", p));
+}
 
 }  // namespace synthesis
 }  // namespace askap
 
 // VisGridderFactory::createGridder (VisGridderFactory.cc:68-98) dlopens libaskap_<name>.so
-// for an unknown gridder name and calls register_<name>(); each of the four names gets a
+// for an unknown gridder name and calls register_<name>(); each of the five names gets a
 // symlink to this one library so that any of them loads it.
 extern "C" {
 static void ygrid_register_all() {
@@ -227,9 +230,11 @@ static void ygrid_register_all() {
   VisGridderFactory::registerGridder("GPUSphFunc", GPUGridderAdapter::createGPUSphFunc);
   VisGridderFactory::registerGridder("GPUWProject", GPUGridderAdapter::createGPUWProject);
   VisGridderFactory::registerGridder("GPUAWProject", GPUGridderAdapter::createGPUAWProject);
+  VisGridderFactory::registerGridder("GPUWStack", GPUGridderAdapter::createGPUWStack);
 }
 void register_gpubox() { ygrid_register_all(); }
 void register_gpusphfunc() { ygrid_register_all(); }
 void register_gpuwproject() { ygrid_register_all(); }
 void register_gpuawproject() { ygrid_register_all(); }
+void register_gpuwstack() { ygrid_register_all(); }
 }

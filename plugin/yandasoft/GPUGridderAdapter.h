@@ -49,6 +49,7 @@ class GPUGridderAdapter : public IVisGridder {
   static IVisGridder::ShPtr createGPUSphFunc(const LOFAR::ParameterSet& parset);
   static IVisGridder::ShPtr createGPUWProject(const LOFAR::ParameterSet& parset);
   static IVisGridder::ShPtr createGPUAWProject(const LOFAR::ParameterSet& parset);
+  static IVisGridder::ShPtr createGPUWStack(const LOFAR::ParameterSet& parset);
 
  private:
   GPUGridderAdapter(const GPUGridderAdapter& other);
