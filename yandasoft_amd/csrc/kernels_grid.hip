@@ -10,10 +10,14 @@
 //                 A workgroup owns a 64x64 tile of all polarisations in registers (16 pixels per
 //                 lane), spreads every pair into it, and flushes once with float atomics: no per-tap
 //                 HBM atomics (GridKernel::grid, GridKernel.cc:61-83).
-//   k_degrid      same work items; the tile of the model grid is loaded into registers, each pair
-//                 is a wave-level dot product over the CF taps (GridKernel::degrid, GridKernel.cc:106-125),
+//   k_degrid_mm   same work items; the tile of the model grid is loaded into registers and four pairs
+//                 at a time are dot products on the matrix cores (GridKernel::degrid, GridKernel.cc:106-125);
 //                 partial sums of the tiles a footprint covers meet in per-sample float accumulators.
+//   k_degrid      the one-polarisation form: each pair a wave-level dot product in FP32 FMAs.
 //   k_degrid_finish  phasor x weight and the add into the visibility cube (TableVisGridder.cc:767-773,861).
+// Around the passes: k_residual (the imaging loop's data - model), k_pol_to_image / k_pol_to_vis_add
+// (polarisation frames, TableVisGridder.cc:682-693,860).  W stacking offsets the tile kernels' grid
+// by the key's w plane (TileJob::wstride).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
