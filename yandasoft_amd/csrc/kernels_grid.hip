@@ -153,8 +153,13 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
     else if (fracv >= p.oversample) { iv -= 1; fracv -= p.oversample; }
     iv += p.ny / 2;
     // delay phasor (TableVisGridder.cc:652-655)
-    const double phase = 2.0f * kPi * f * (in.delay ? in.delay[row] : 0.0) / kC;
-    const float ph_r = (float)cos(phase), ph_i = (float)sin(phase);
+    // (no delays: the phase is 0 and the phasor exactly (1, 0), without the FP64 cos and sin)
+    float ph_r = 1.f, ph_i = 0.f;
+    if (in.delay) {
+      const double phase = 2.0f * kPi * f * in.delay[row] / kC;
+      ph_r = (float)cos(phase);
+      ph_i = (float)sin(phase);
+    }
 
     for (int pol = 0; pol < p.npol; ++pol)
       if (in.flag && in.flag[row + (size_t)p.nrow * (chan + (size_t)p.nchan * pol)]) { flagged = true; break; }
@@ -283,9 +288,22 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
 
   // sum of weights per (row of sumw, pol, image chan): TableVisGridder.cc:776-782
   if (use_lds) {
-    if (valid)
+    // a wave whose valid lanes share one (row of sumw, image chan) -- always so for the gridders
+    // with one CF row (Box, SphFunc, w stacking), whose lanes would otherwise all contend for the
+    // same LDS words -- adds its wave sum once; other waves add per lane
+    const int lkey = valid ? swrow + p.nsw_rows * p.npol * ichan : -1;
+    const unsigned long long vm = __ballot(valid);
+    const int leader = vm ? __ffsll((long long)vm) - 1 : 0;
+    const int k0 = __shfl(lkey, leader, 64);
+    if (vm != 0 && __all(!valid || lkey == k0)) {
+      for (int pol = 0; pol < p.npol; ++pol) {
+        const double t = wave_sum_f64(valid ? (double)wts[pol] : 0.0);
+        if (lane == leader) atomicAdd(&s_sumw[k0 + p.nsw_rows * pol], t);
+      }
+    } else if (valid) {
       for (int pol = 0; pol < p.npol; ++pol)
         atomicAdd(&s_sumw[swrow + p.nsw_rows * (pol + p.npol * ichan)], (double)wts[pol]);
+    }
   } else if (sumw_on) {
     // aggregated per distinct (row of sumw, image chan) within the wave
     long long key = valid ? (long long)swrow + (long long)p.nsw_rows * (long long)p.npol * ichan : -1;
