@@ -1307,7 +1307,10 @@ void launch_prep(const Ctx& c, const PrepParams& p, const PrepInputs& in, const 
     const int g = std::min(blocks, 4 * c.n_cu);
     hipLaunchKernelGGL(k_prep, dim3(g), dim3(256), (size_t)nsw * sizeof(double), c.stream, p, in, out, 1);
   } else {
-    hipLaunchKernelGGL(k_prep, dim3(blocks), dim3(256), 0, c.stream, p, in, out, 0);
+    // also a bounded grid looping over the samples: every wave ends with global atomics on the
+    // pass's few counters, which one wave per 64 samples would serialise on those addresses
+    const int g = std::min(blocks, 8 * c.n_cu);
+    hipLaunchKernelGGL(k_prep, dim3(g), dim3(256), 0, c.stream, p, in, out, 0);
   }
   YG_HIP(hipGetLastError());
 }
