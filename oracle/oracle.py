@@ -15,7 +15,10 @@ Restated reference functions (file:line in alxndrwllmsn/yandasoft):
                                  normalisation (forward unnormalised, inverse 1/N) is pinned by
                                  the point-source closed form of
                                  tests/measurementequation/ImageFFTEquationTest.h:124-175; the
-                                 exponent sign (forward = exp(-i)) is parity unpinned.
+                                 exponent sign (forward = exp(-i)) is fixed relative to the w
+                                 screen (WStackVisGridder.cc:163-164, exp(-i phase)) by imaging a
+                                 w-stacked visibility: only this sign gives the uv and w terms of
+                                 the measurement equation the same sign.
 * ``Spheroidal``             -- scimath::SpheroidalFunction(c, alpha) (SphFuncVisGridder.h:101-106).
 * ``box_cf``                 -- BoxVisGridder::initConvolutionFunction (BoxVisGridder.cc:62-72).
 * ``sphfunc_cf``             -- SphFuncVisGridder::initConvolutionFunction (SphFuncVisGridder.cc:90-169).
@@ -27,12 +30,21 @@ Restated reference functions (file:line in alxndrwllmsn/yandasoft):
 * ``wproject_cf``            -- WProjectVisGridder::generate/setup/populateThisPlane/calcSupport/
                                  extractSupport/populateItsConvFunc/normalise
                                  (WProjectVisGridder.cc:234-284,435-522,562-703).
-* ``disk_pattern`` / ``awproject_cf`` -- DiskIllumination::getPattern (DiskIllumination.cc:85-177)
-                                 and AWProjectVisGridder::initConvolutionFunction (AWProjectVisGridder.cc:259-611).
+* ``disk_pattern`` / ``awproject_planes`` / ``OracleGridder._aw_weights`` / ``fft_pad`` --
+                                 DiskIllumination::getPattern (DiskIllumination.cc:85-177),
+                                 AWProjectVisGridder::initConvolutionFunction / finaliseWeights
+                                 (AWProjectVisGridder.cc:259-611,618-770).  Pinned by the lattice
+                                 count and normalisation of the disk, the beam direction of an
+                                 offset pattern, brute-force centre-relative DFTs of the whole CF
+                                 chain, the w = 0 symmetries, two-tap and taper closed forms of the
+                                 weights and a band-limited closed form of the padding.
 * ``frequency_map``          -- FrequencyMapper::setupMapping (FrequencyMapper.cc:63-97).
 * ``wstack_screen`` / ``OracleWStack`` -- WStackVisGridder::multiply / initIndices / gIndex /
                                  finaliseGrid / initialiseDegrid (WStackVisGridder.cc:71-97,134-171,
-                                 216-344), composed from the SphFunc oracle gridder.
+                                 216-344), composed from the SphFunc oracle gridder.  Pinned by the
+                                 measurement equation with its w term (imaging and prediction of
+                                 on-grid, on-plane samples), which also fixes the sign of the
+                                 transforms relative to the w screen.
 * ``pol_matrix`` / ``pol_to_image`` / ``pol_to_vis`` -- scimath::PolConverter::convert / noise as
                                  used at TableVisGridder.cc:517-524,682-693,860.  PolConverter is in
                                  base-scimath (absent): restated from the casacore / IAU definitions
@@ -631,7 +643,9 @@ def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_o
 
 def fft_pad(a: np.ndarray, ny_out: int, nx_out: int) -> np.ndarray:
     """Fourier interpolation of real [.., y, x] planes onto a larger grid (PaddingUtils::fftPad as
-    called at AWProjectVisGridder.cc:765; base-scimath is absent: parity unpinned)."""
+    called at AWProjectVisGridder.cc:765).  The interpolation is pinned by a band-limited closed
+    form (tests/test_oracle_pins.py); base-scimath (absent) could also rescale the result, and that
+    choice is parity unpinned: values are kept, as a band-limited function's samples are."""
     ny_in, nx_in = a.shape[-2:]
     if (ny_in, nx_in) == (ny_out, nx_out):
         return a.copy()

@@ -483,3 +483,253 @@ def test_wstack_screen_closed_forms():
     # corner (0, 0): r^2 = 0.8^2 + 0.6^2 = 1 -> outside, unchanged
     assert s[0, 0] == 1.0
     assert np.angle(s[4, 8]) == 0.0  # the centre
+
+
+# ---- A-projection: DiskIllumination, AWProjectVisGridder CFs and weights, fftPad ----------------
+def _lattice_points_in_disk(r2max, nu, nv):
+    """Gauss circle count of integer offsets (du, dv), -n/2 <= d < n/2, with du^2 + dv^2 <= r2max,
+    by integer square roots row by row (no distance arrays)."""
+    count = 0
+    for dv in range(-nv // 2, nv - nv // 2):
+        rem = r2max - dv * dv
+        if rem < 0:
+            continue
+        k = math.isqrt(int(math.floor(rem)))
+        lo, hi = max(-k, -nu // 2), min(k, nu - nu // 2 - 1)
+        count += max(0, hi - lo + 1)
+    return count
+
+
+@pytest.mark.parametrize("diameter,blockage", [(12.0, 0.0), (25.0, 6.0)])
+def test_disk_pattern_count_normalisation_and_modulus(diameter, blockage):
+    """DiskIllumination.cc:154-176: the pattern is nU*nV/count on every lattice point of the
+    annulus rmin^2 <= r^2 <= rmax^2 and 0 elsewhere; with a beam offset only its phase changes."""
+    nu = nv = 64
+    ucell, os_, freq = 8.0, 2, 1.0e9
+    cell_m = ucell / os_ * (O.C_LIGHT / freq)
+    r2max = (diameter / (2 * cell_m)) ** 2
+    r2min = (blockage / (2 * cell_m)) ** 2
+    count = _lattice_points_in_disk(r2max, nu, nv) - (
+        _lattice_points_in_disk(r2min - 1e-12, nu, nv) if blockage > 0 else 0)
+    pat = O.disk_pattern(nu, nv, ucell, ucell, os_, freq, diameter, blockage)
+    assert np.count_nonzero(pat) == count
+    assert abs(pat.sum() - nu * nv) < 1e-3 * nu * nv            # nU nV / count on count points
+    assert np.allclose(pat[pat != 0], nu * nv / count, rtol=1e-6)
+    off = O.disk_pattern(nu, nv, ucell, ucell, os_, freq, diameter, blockage, l=0.01, m=-0.02)
+    assert np.array_equal(off != 0, pat != 0)
+    assert np.allclose(np.abs(off[off != 0]), nu * nv / count, rtol=1e-6)
+    with pytest.raises(ValueError):
+        O.disk_pattern(nu, nv, ucell, ucell, os_, freq, 10.0, 12.0)   # hole larger than the dish
+
+
+@pytest.mark.parametrize("kx,ky", [(3, 0), (-2, 5)])
+def test_disk_pattern_beam_points_at_its_offset(kx, ky):
+    """The phase slope of an offset beam (DiskIllumination.cc:105-110,166-167) steers the voltage
+    pattern (its inverse transform, AWProjectVisGridder.cc:383) to the offset: with
+    l = kx / (n cellU) the peak of |A| lies kx pixels from the centre, on the same side."""
+    n, ucell, os_, freq = 64, 8.0, 2, 1.0e9
+    cu = ucell / os_
+    pat = O.disk_pattern(n, n, ucell, ucell, os_, freq, 12.0, 0.0, l=kx / (n * cu), m=ky / (n * cu))
+    A = np.abs(O.fft_centred(pat.astype(np.complex128), False))
+    iy, ix = np.unravel_index(np.argmax(A), A.shape)
+    assert (ix - n // 2, iy - n // 2) == (kx, ky)
+
+
+def test_direction_lm_is_an_orthographic_projection():
+    """(l, m, n) of a direction about the image centre is a rotation of its unit vector:
+    l^2 + m^2 + n^2 = 1 with n = cos(angular distance); the centre maps to (0, 0); a direction
+    on the centre's meridian has l = 0 and m = sin(dlat)."""
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        cl, cb = rng.uniform(0, 2 * np.pi), rng.uniform(-1.2, 1.2)
+        bl, bb = cl + rng.uniform(-0.3, 0.3), np.clip(cb + rng.uniform(-0.3, 0.3), -1.5, 1.5)
+        l, m = O.direction_lm((bl, bb), (cl, cb))
+        nn = math.sin(bb) * math.sin(cb) + math.cos(bb) * math.cos(cb) * math.cos(bl - cl)
+        assert abs(l * l + m * m + nn * nn - 1.0) < 1e-12
+    assert O.direction_lm((1.0, 0.4), (1.0, 0.4)) == (0.0, 0.0)
+    l, m = O.direction_lm((1.0, 0.45), (1.0, 0.4))
+    assert abs(l) < 1e-15 and abs(m - math.sin(0.05)) < 1e-12
+
+
+def _aw_small():
+    prm = O.AWProjectParams(wmax=300.0, nwplanes=3, oversample=2, maxsupport=64, cutoff=1e-2, diameter=12.0,
+                            maxfeeds=2, freqdep=True)
+    freqs = np.array([1.0e9, 1.05e9])
+    lam = O.C_LIGHT / freqs[0]
+    uvcell = (12.0 / lam) / 8.3 * prm.oversample   # dish radius 4.15 pattern cells: no lattice ties
+    cu = uvcell / prm.oversample
+    lm = {0: (0.0, 0.0), 1: (2 / (prm.maxsupport * cu), -1 / (prm.maxsupport * cu))}
+    return prm, freqs, uvcell, lm
+
+
+def test_awproject_planes_match_brute_force_transforms():
+    """AWProjectVisGridder.cc:370-440: per (feed, chan, w) the CF is os^2 x the forward transform,
+    divided by nx ny, of |A|^2 x the w screen inside r^2 < 1, where A is the inverse transform of
+    the disk pattern; planes are cut out at k = t os + frac about the centre.  Restated here with
+    explicit centre-relative DFT sums (no shifts, no FFT), including the beam-offset phase."""
+    prm, freqs, uvcell, lm = _aw_small()
+    st = O.awproject_planes(prm, 64, 64, (uvcell, uvcell), freqs, lm)
+    S, os_, n, nw = st["support"], prm.oversample, prm.maxsupport, prm.nwplanes
+    assert S >= 3
+    ws = prm.sampling()
+    c = np.arange(n) - n / 2.0
+    x = c / ((n // os_) * uvcell)
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    E = np.exp(2j * np.pi * np.outer(c, c) / n)
+    for feed, (l, m) in lm.items():
+        for ch, f in enumerate(freqs):
+            cu = uvcell / os_
+            R2 = (prm.diameter / (2 * cu * O.C_LIGHT / f)) ** 2
+            disk = (c[None, :] ** 2 + c[:, None] ** 2 <= R2)
+            pat = disk * np.exp(-2j * np.pi * cu * (l * c[None, :] + m * c[:, None])) * (n * n / disk.sum())
+            wt = np.abs(E @ pat @ E.T / (n * n)) ** 2
+            for iw in range(nw):
+                ph = 2 * np.pi * ws.wterm(iw) * (1 - np.sqrt(np.clip(1 - r2, 0, None)))
+                scr = np.where(r2 < 1, wt * np.exp(-1j * ph), 0)
+                z = iw + nw * (ch + len(freqs) * feed)
+                for fu in range(os_):
+                    for fv in range(os_):
+                        Eu = np.exp(-2j * np.pi * np.outer(np.arange(-S, S + 1) * os_ + fu, c) / n)
+                        Ev = np.exp(-2j * np.pi * np.outer(np.arange(-S, S + 1) * os_ + fv, c) / n)
+                        ref = Ev @ scr @ Eu.T / (n * n) * os_ * os_
+                        got = st["planes"][fu + os_ * (fv + os_ * z)]
+                        assert np.max(np.abs(got - ref)) < 1e-6 * np.max(np.abs(ref)), (feed, ch, iw, fu, fv)
+
+
+def test_awproject_w0_plane_of_a_centred_disk_is_real_and_symmetric():
+    """With no beam offset |A|^2 is real and symmetric, so at w = 0 the CF is real and has the
+    square's symmetries (x -> -x, y -> -y, x <-> y) on the unshifted oversampling plane."""
+    prm, freqs, uvcell, lm = _aw_small()
+    st = O.awproject_planes(prm, 64, 64, (uvcell, uvcell), freqs, {0: (0.0, 0.0)})
+    nw, os_ = prm.nwplanes, prm.oversample
+    p = st["planes"][os_ * os_ * (nw // 2)]           # fracu = fracv = 0, w plane 0, chan 0, feed 0
+    peak = np.max(np.abs(p))
+    assert np.max(np.abs(p.imag)) < 1e-6 * peak
+    assert np.max(np.abs(p - p[::-1, :])) < 1e-6 * peak
+    assert np.max(np.abs(p - p[:, ::-1])) < 1e-6 * peak
+    assert np.max(np.abs(p - p.T)) < 1e-6 * peak
+    # an offset beam changes the w = 0 CF by a phase slope only (|CF| unchanged)
+    st2 = O.awproject_planes(prm, 64, 64, (uvcell, uvcell), freqs, {0: lm[1]})
+    p2 = st2["planes"][os_ * os_ * (nw // 2)]
+    assert np.max(np.abs(np.abs(p2) - np.abs(p))) < 1e-3 * peak
+    assert np.max(np.abs(p2 - p)) > 1e-2 * peak
+
+
+def _aw_weights_gridder(ngrid, taper=False):
+    prm = O.AWProjectParams(wmax=300.0, nwplanes=1, oversample=2, maxsupport=32, spheroidaltaper=taper,
+                            spheroidalweightscutoff=0.05)
+    g = O.OracleGridder("AWProject", "f64", wproject=prm)
+    g._setup((ngrid, ngrid, 1, 1), (1e-3, 1e-3))
+    g.track_per_os = False
+    return g
+
+
+@pytest.mark.parametrize("ngrid", [32, 96])
+def test_aw_weights_two_tap_closed_form(ngrid):
+    """AWProjectVisGridder.cc:618-770: weights = sum_z wt_z |F_z|^2, F_z = (inverse transform x
+    cnx cny) of CF plane z placed about the centre of a cnx x cny buffer, then Fourier-padded to
+    the grid.  A plane with taps a (centre) and b (offset (dx, dy)) gives
+    |a + b exp(2 pi i (dx X + dy Y) / cnx)|^2, a band-limited function that the padding samples
+    exactly on the finer grid; a second plane with a single tap adds a constant."""
+    g = _aw_weights_gridder(ngrid)
+    S, os_, cn = 3, 2, 32
+    a, b, dx, dy = 0.7 + 0.1j, 0.2 - 0.3j, 2, -1
+    planes = []
+    for k in range(2 * os_ * os_):
+        c = np.zeros((2 * S + 1, 2 * S + 1), complex)
+        if k < os_ * os_:
+            c[S, S], c[S + dy, S + dx] = a, b
+        else:
+            c[S, S] = 0.5
+        planes.append(c)
+    g.cf = O.CFCache(planes, os_)
+    g.sumw = np.array([[[3.0, 2.0]]])
+    w = g._aw_weights()[0, 0]
+    X = (np.arange(ngrid) - ngrid // 2) * (cn / ngrid)
+    ph = 2 * np.pi * (dx * X[None, :] + dy * X[:, None]) / cn
+    exp = 3.0 * np.abs(a + b * np.exp(1j * ph)) ** 2 + 2.0 * 0.25
+    assert np.max(np.abs(w - exp)) < 1e-6 * np.max(exp)
+    wrong = 3.0 * np.abs(a + b * np.exp(-1j * ph)) ** 2 + 2.0 * 0.25
+    assert np.max(np.abs(wrong - exp)) > 1e-1 * np.max(exp)   # the check can see the transform direction
+
+
+def test_aw_weights_spheroidal_taper_is_divided_out():
+    """With the spheroidal taper the weights divide |F|^2 by the taper squared where it exceeds
+    the cutoff and are 0 elsewhere (AWProjectVisGridder.cc:638-667,722-737): a CF whose transform
+    is the taper itself gives flat weights on the taper's support."""
+    g = _aw_weights_gridder(32, taper=True)
+    cn, c0 = 32, 16
+    sph = O.Spheroidal(math.pi * 3, g.wp.alpha)
+    f = sph(np.abs(np.arange(cn) - c0) / c0)
+    t1 = O.interpolate_edge(np.where(f > 0.05, f, 0.0))
+    t = np.outer(t1, t1)
+    plane = np.zeros((cn + 1, cn + 1), complex)
+    plane[:cn, :cn] = O.fft_centred(t.astype(complex), True) / (cn * cn)
+    g.cf = O.CFCache([plane] * 4, 2)
+    g.sumw = np.full((1, 1, 1), 5.0)
+    w = g._aw_weights()[0, 0]
+    assert 0 < np.count_nonzero(t > 0) < t.size
+    assert np.max(np.abs(w - np.where(t > 0, 5.0, 0.0))) < 1e-5 * 5.0
+
+
+def test_fft_pad_reproduces_a_band_limited_function():
+    """Fourier padding (PaddingUtils::fftPad as used at AWProjectVisGridder.cc:765) samples the
+    band-limited function of its input on the finer grid, centre to centre."""
+    ny_in, nx_in, ny_out, nx_out = 16, 24, 40, 60
+
+    def f(x, y):
+        return 1.5 + np.cos(2 * np.pi * (3 * x / nx_in - 2 * y / ny_in) + 0.4)
+
+    a = f((np.arange(nx_in) - nx_in // 2)[None, :], (np.arange(ny_in) - ny_in // 2)[:, None])
+    out = O.fft_pad(a[None], ny_out, nx_out)[0]
+    xo = (np.arange(nx_out) - nx_out // 2) * (nx_in / nx_out)
+    yo = (np.arange(ny_out) - ny_out // 2) * (ny_in / ny_out)
+    assert np.max(np.abs(out - f(xo[None, :], yo[:, None]))) < 1e-12
+    assert np.array_equal(O.fft_pad(a, ny_in, nx_in), a)
+
+
+# ---- w stacking composition (OracleWStack): the measurement equation with its w term ------------
+def _wstack_case():
+    N, cell, f = 128, math.radians(60 / 3600), np.array([1.0e9])
+    lam, uvcell = O.C_LIGHT / f[0], 1 / (cell * N)
+    ws = O.WSampling(2000.0, 9)
+    ks = [(5, -7, 2), (-11, 3, 7), (20, 14, 4)]   # (pixel u, pixel v, w plane): on grid and on planes
+    uvw = np.array([[ku * uvcell * lam, kv * uvcell * lam, ws.wterm(iw) * lam] for ku, kv, iw in ks])
+    V = np.array([1.0 + 0.5j, -0.3 + 0.8j, 0.7 - 0.2j])
+    return N, cell, f, uvcell, ws, ks, uvw, V
+
+
+def test_wstack_image_is_the_direct_transform_with_the_w_term():
+    """Each visibility goes to the grid of its w plane; the image sums the w-screened inverse
+    transforms (WStackVisGridder.cc:134-171,255-289).  For visibilities on grid pixels and on
+    plane w values the dirty image is Re sum V exp(-2 pi i (u l + v m + w (sqrt(1-l^2-m^2) - 1))),
+    up to the spheroidal kernel's correction accuracy: the uv and w terms carry the same sign
+    (a flipped screen fails at O(1))."""
+    N, cell, f, uvcell, ws, ks, uvw, V = _wstack_case()
+    o = O.OracleWStack(2000.0, 9, precision="f64")
+    o.grid((N, N, 1, 1), (cell, cell), O.Chunk(uvw, f, V.reshape(3, 1, 1).astype(np.complex64)))
+    img = o.finalise_grid()[0, 0]
+    X = (np.arange(N) - N // 2) * cell
+    l, m = X[None, :], X[:, None]
+    n = np.sqrt(1 - l * l - m * m)
+    exp = sum((v * np.exp(-2j * np.pi * (ku * uvcell * l + kv * uvcell * m + ws.wterm(iw) * (n - 1)))).real
+              for (ku, kv, iw), v in zip(ks, V))
+    q = slice(N // 4, 3 * N // 4)
+    assert np.max(np.abs(img - exp)[q, q]) < 2e-3 * np.max(np.abs(exp))
+
+
+def test_wstack_degrid_predicts_the_measurement_equation():
+    """Degridding a point source of flux F at (l0, m0) predicts
+    F exp(2 pi i (u l0 + v m0 + w (n0 - 1))) (WStackVisGridder.cc:314-327), the adjoint of the
+    imaging above, to the kernel's correction accuracy."""
+    N, cell, f, uvcell, ws, ks, uvw, V = _wstack_case()
+    model = np.zeros((1, 1, N, N))
+    X0, Y0, F = 17, -9, 2.0
+    model[0, 0, N // 2 + Y0, N // 2 + X0] = F
+    pv = O.OracleWStack(2000.0, 9, precision="f64").degrid(
+        (N, N, 1, 1), (cell, cell), model, O.Chunk(uvw, f, V.reshape(3, 1, 1).astype(np.complex64)))
+    l0, m0 = X0 * cell, Y0 * cell
+    n0 = math.sqrt(1 - l0 * l0 - m0 * m0)
+    for i, (ku, kv, iw) in enumerate(ks):
+        e = F * np.exp(2j * np.pi * (ku * uvcell * l0 + kv * uvcell * m0 + ws.wterm(iw) * (n0 - 1)))
+        assert abs(pv[i, 0, 0] - e) < 2e-3 * F
