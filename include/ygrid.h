@@ -147,6 +147,20 @@ int ygrid_finalise_weights(ygrid_gridder *g, float *out, int mem);
 int ygrid_initialise_degrid(ygrid_gridder *g, const ygrid_image_desc *img, const float *model, int mem);
 /* IVisGridder::degrid (IVisGridder.h:115; TableVisGridder.cc:898-902 -> generic(acc,true)) */
 int ygrid_degrid(ygrid_gridder *g, const ygrid_chunk *acc);
+/* Allocation-free form of ygrid_degrid: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result as ygrid_degrid on the same gridder and chunk, but every device buffer
+ * of the pass (sample records, binning, work items, per-sample sums) comes from one fixed
+ * workspace of YGRID_NOALLOC_WORKSPACE_BYTES that the library holds on each GPU until the process
+ * ends; no hipMalloc/hipFree and no allocating library routine is called.  Limits: the chunk must
+ * be in device memory (mem = YGRID_MEM_DEVICE) and in the image's polarisation frame; the
+ * gridder's CF cache must exist (one ordinary ygrid_degrid pass on it or on the gridder it was
+ * cloned from), and A-projection and learned-CF gridders are refused; a chunk whose pass needs
+ * more workspace fails with YGRID_ERR_ARG and a message giving the size (split the chunk).  All
+ * calls on one GPU share its workspace: the library serialises them with a per-GPU lock, so calls
+ * that would overlap on one GPU wait for each other. */
+#define YGRID_NOALLOC_WORKSPACE_BYTES (64u << 20)
+int ygrid_degrid_noalloc(ygrid_gridder *g, const ygrid_chunk *acc);
 /* IVisGridder::finaliseDegrid (IVisGridder.h:118; TableVisGridder.cc:1364-1372) */
 int ygrid_finalise_degrid(ygrid_gridder *g);
 /* IVisGridder::isModelEmpty (IVisGridder.h:139; TableVisGridder.cc:1467-1470) */

@@ -15,7 +15,7 @@ EXPORTED = [
     "ygrid_last_error", "ygrid_version", "ygrid_ctx_create", "ygrid_ctx_destroy", "ygrid_ctx_set_stream",
     "ygrid_ctx_synchronize", "ygrid_ctx_clear_cache", "ygrid_gridder_create", "ygrid_gridder_clone", "ygrid_gridder_destroy",
     "ygrid_initialise_grid", "ygrid_grid", "ygrid_finalise_grid", "ygrid_finalise_weights",
-    "ygrid_initialise_degrid", "ygrid_degrid", "ygrid_finalise_degrid", "ygrid_is_model_empty",
+    "ygrid_initialise_degrid", "ygrid_degrid", "ygrid_degrid_noalloc", "ygrid_finalise_degrid", "ygrid_is_model_empty",
     "ygrid_set_mfs_weight", "ygrid_get_stats", "ygrid_set_profiling", "ygrid_get_cf", "ygrid_get_grid",
     "ygrid_batch_create", "ygrid_batch_destroy", "ygrid_batch_add", "ygrid_batch_set_graph", "ygrid_batch_run",
     "ygrid_batch_autotune", "ygrid_batch_info",
@@ -262,6 +262,12 @@ class Gridder:
     def degrid(self, chunk: Chunk):
         c = chunk.c()
         _check(lib().ygrid_degrid(self.h, ctypes.byref(c)))
+
+    def degrid_noalloc(self, chunk: Chunk):
+        """ygrid_degrid_noalloc: degrid() with the pass's buffers in the library's fixed per-GPU
+        workspace (device-resident chunk, CF cache already built)."""
+        c = chunk.c()
+        _check(lib().ygrid_degrid_noalloc(self.h, ctypes.byref(c)))
 
     def finalise_degrid(self):
         _check(lib().ygrid_finalise_degrid(self.h))

@@ -33,6 +33,7 @@ SOURCES = [
     "learned_cf.cpp",
     "imaging.cpp",
     "polconv.cpp",
+    "noalloc.cpp",
     "ygrid_api.cpp",
 ]
 

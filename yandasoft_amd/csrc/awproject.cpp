@@ -100,6 +100,7 @@ class AWProjectGridder : public Gridder {
   int oversampling() const override { return os_; }
   int nZPlanes() const override { return nw_ * maxfeeds_ * maxfields_ * std::max(1, nchan_cf_); }
   bool needsRowMeta() const override { return true; }
+  bool cfPerChunk() const override { return true; }
   bool doesCorrection() const override { return sph_taper_; }
   void gridCorrection(std::vector<double>& cx, std::vector<double>& cy) const override {
     if (sph_taper_) sph_grid_correction_xy(shape_[0], shape_[1], alpha_, 3, cx, cy);

@@ -77,7 +77,7 @@ void Batch::prepare() {
     p.group = it->second;
     p.job = (int)gr.passes.size();
     gr.passes.push_back((int)(&p - passes_.data()));
-    gr.items_cap += p.sb.nitems_cap;
+    gr.items_cap += p.sb.v.nitems_cap;
   }
   for (Group& gr : groups_) {
     YG_CHECK(gr.items_cap < (size_t)INT_MAX, 1, "too many work items in one batched launch");
@@ -88,8 +88,8 @@ void Batch::prepare() {
     for (int pi : gr.passes) {
       Pass& p = passes_[pi];
       Gridder& g = *p.g;
-      jobs.push_back(make_tile_job(p.sb.pairs.as<int4>(), p.ps.out.smp, p.ps.cfv, g.grid_.as<float>(),
-                                   p.forward ? g.acc_.as<float>() : nullptr, g.shape_[0], g.shape_[1], g.shape_[2],
+      jobs.push_back(make_tile_job(p.sb.v.pairs, p.ps.out.smp, p.ps.cfv, g.grid_.as<float>(),
+                                   p.forward ? p.ps.acc : nullptr, g.shape_[0], g.shape_[1], g.shape_[2],
                                    p.ps.pp.nzg));
     }
     gr.jobs.resize(jobs.size() * sizeof(TileJob));
@@ -106,13 +106,13 @@ void Batch::enqueue() {
     if (!p.active) continue;
     Gridder& g = *p.g;
     Group& gr = groups_[p.group];
-    launch_fill_u32(c, g.counters_.get(), 8 * 2, 0u);
-    launch_fill_u32(c, g.err_.get(), 1, 0u);
-    if (p.forward) launch_fill_u32(c, g.acc_.get(), (size_t)p.ps.nsamp * 8, 0u);
+    launch_fill_u32(c, p.ps.out.counters, 8 * 2, 0u);
+    launch_fill_u32(c, p.ps.out.error, 1, 0u);
+    if (p.forward) launch_fill_u32(c, p.ps.acc, (size_t)p.ps.nsamp * 8, 0u);
     launch_prep(c, p.ps.pp, p.ps.in, p.ps.out);
     g.afterPrep(p.ps);
     int* base = gr.base.as<int>();
-    enqueue_static_binning(c, p.sb, p.ps.out.smp, p.ps.cfv, gr.items.as<int4>(), base + p.job, base + p.job + 1,
+    enqueue_static_binning(c, p.sb.v, p.ps.out.smp, p.ps.cfv, gr.items.as<int4>(), base + p.job, base + p.job + 1,
                            p.job);
   }
   for (Group& gr : groups_) {
@@ -123,7 +123,7 @@ void Batch::enqueue() {
   for (Pass& p : passes_) {
     if (!p.active || !p.forward) continue;
     const ChunkDev& d = p.ps.d;
-    launch_degrid_finish(c, p.ps.out.smp, p.g->acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);
+    launch_degrid_finish(c, p.ps.out.smp, p.ps.acc, d.nrow, d.nchan, d.npol, d.vis_out);
   }
 }
 
@@ -162,8 +162,8 @@ void Batch::run() {
     if (!p.active) continue;
     unsigned long long cnt[4] = {0, 0, 0, 0};
     int err = 0;
-    YG_HIP(hipMemcpy(cnt, p.g->counters_.get(), sizeof(cnt), hipMemcpyDeviceToHost));
-    YG_HIP(hipMemcpy(&err, p.g->err_.get(), sizeof(int), hipMemcpyDeviceToHost));
+    YG_HIP(hipMemcpy(cnt, p.ps.out.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    YG_HIP(hipMemcpy(&err, p.ps.out.error, sizeof(int), hipMemcpyDeviceToHost));
     p.g->account_pass(p.forward, cnt, err, secs / nact);
   }
 }

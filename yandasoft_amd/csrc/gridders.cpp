@@ -297,7 +297,14 @@ void Gridder::stage(const ygrid_chunk& a, ChunkDev& d, bool forward) {
 // The part of TableVisGridder::generic before the prep kernel: stage the chunk, channel map,
 // MFS weights, CF generation, buffer sizes, and the prep kernel's arguments.  Returns false
 // when the pass has nothing to do (an empty model when degridding, no rows).
-bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
+// A buffer of the pass: from the arena when one is given, else the gridder's own (grown).
+static void* pass_buf(PassArena* arena, DevBuf& own, size_t bytes) {
+  if (arena) return arena->take(bytes);
+  own.resize(bytes);
+  return own.get();
+}
+
+bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps, PassArena* arena) {
   YG_CHECK(initialised_, 2, "gridder not initialised");
   YG_CHECK(forward == degrid_mode_, 2,
            forward ? "degrid called on a gridder initialised for gridding" : "grid called on a degridder");
@@ -322,8 +329,8 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
       cmap[c] = (ic < 0 || ic >= shape_[3]) ? -1 : ic;
     }
   }
-  chanmap_.resize(cmap.size() * sizeof(int32_t));
-  YG_HIP(hipMemcpyAsync(chanmap_.get(), cmap.data(), cmap.size() * 4, hipMemcpyHostToDevice, ctx_->stream));
+  int32_t* dchan = static_cast<int32_t*>(pass_buf(arena, chanmap_, cmap.size() * sizeof(int32_t)));
+  YG_HIP(hipMemcpyAsync(dchan, cmap.data(), cmap.size() * 4, hipMemcpyHostToDevice, ctx_->stream));
   // MFS visibility weights (VisWeightsMultiFrequency::getWeight, VisWeightsMultiFrequency.cc:59-75)
   const float* vw = nullptr;
   if (vw_order_ >= 0 && !dopcf_) {
@@ -333,22 +340,26 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
       const double ratio = (d.freq_host[c] - vw_ref_) / vw_ref_;
       w[c] = (float)(vw_order_ == 1 ? ratio : (vw_order_ == 2 ? ratio * ratio : std::pow(ratio, vw_order_)));
     }
-    visweight_.resize(w.size() * sizeof(float));
-    YG_HIP(hipMemcpyAsync(visweight_.get(), w.data(), w.size() * 4, hipMemcpyHostToDevice, ctx_->stream));
-    vw = visweight_.as<float>();
+    float* dw = static_cast<float*>(pass_buf(arena, visweight_, w.size() * sizeof(float)));
+    YG_HIP(hipMemcpyAsync(dw, w.data(), w.size() * 4, hipMemcpyHostToDevice, ctx_->stream));
+    vw = dw;
   }
+  // the copies read host vectors of this frame: complete them before it returns
+  if (arena) YG_HIP(hipStreamSynchronize(ctx_->stream));
   if (!forward && dopsf_ && first_psf_) {
     psf_feed_ = use_all_psf_ ? -1 : d.feed0;
     first_psf_ = false;
   }
   const auto tcf = std::chrono::steady_clock::now();
-  initConvolutionFunction(d);
+  // with an arena the caller has checked that the CF cache exists and holds every plane
+  if (!arena) initConvolutionFunction(d);
   t_cf_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - tcf).count();
   YG_CHECK(cf_ && cf_->n_elems > 0, 2, "convolution function not initialised");
   if (!forward) {
     // the sum-of-weights cube follows the CF cache (resizeSumOfWeights when the cache is
     // first built, AWProjectVisGridder.cc:274-276)
     const int rows = nZPlanes() * (track_per_os_ ? cf_->oversample * cf_->oversample : 1);
+    YG_CHECK(!arena || rows == nsw_rows_, 2, "the sum-of-weights cube must exist before an allocation-free pass");
     if (rows != nsw_rows_) {
       nsw_rows_ = rows;
       const size_t nsw = (size_t)nsw_rows_ * shape_[2] * shape_[3];
@@ -359,13 +370,17 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
   }
 
   const int nsamp = d.nrow * d.nchan;
-  smp_hdr_.resize((size_t)nsamp * sizeof(SampleHdr));
-  smp_key_.resize((size_t)nsamp * sizeof(int32_t));
-  smp_coef_.resize((size_t)nsamp * kCoef * sizeof(float));
-  if (vis_prec_ != kPrecF32) smp_coef16_.resize((size_t)nsamp * kCoef * sizeof(uint16_t));
-  counters_.resize(8 * sizeof(unsigned long long));
-  err_.resize(sizeof(int) * 2);
-  if (forward) acc_.resize((size_t)nsamp * 8 * sizeof(float));
+  Samples smp{};
+  smp.hdr = static_cast<SampleHdr*>(pass_buf(arena, smp_hdr_, (size_t)nsamp * sizeof(SampleHdr)));
+  smp.key = static_cast<int32_t*>(pass_buf(arena, smp_key_, (size_t)nsamp * sizeof(int32_t)));
+  smp.coef = static_cast<float*>(pass_buf(arena, smp_coef_, (size_t)nsamp * kCoef * sizeof(float)));
+  smp.coef16 = vis_prec_ != kPrecF32
+                   ? static_cast<uint16_t*>(pass_buf(arena, smp_coef16_, (size_t)nsamp * kCoef * sizeof(uint16_t)))
+                   : nullptr;
+  smp.coef_prec = vis_prec_;
+  auto* counters = static_cast<unsigned long long*>(pass_buf(arena, counters_, 8 * sizeof(unsigned long long)));
+  int* errp = static_cast<int*>(pass_buf(arena, err_, sizeof(int) * 2));
+  ps.acc = forward ? static_cast<float*>(pass_buf(arena, acc_, (size_t)nsamp * 8 * sizeof(float))) : nullptr;
 
   PrepParams& pp = ps.pp;
   pp.nrow = d.nrow;
@@ -405,7 +420,7 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
   in.noise = d.noise;
   in.flag = d.flag;
   in.feed1 = d.feed1;
-  in.chanmap = chanmap_.as<int32_t>();
+  in.chanmap = dchan;
   in.visweight = vw;
   in.cf_off = cf_->d_off.as<int32_t>();
   in.cf_supp = cf_->d_supp.as<int32_t>();
@@ -413,14 +428,13 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps) {
   in.cf_ofs_v = cf_->ofs_v.empty() ? nullptr : cf_->d_ofs_v.as<int32_t>();
   in.n_cf = (int)cf_->off.size();
   PrepOutputs& out = ps.out;
-  const Samples smp = samples();
   out.smp = smp;
   out.sumw = forward ? nullptr : sumw_.as<double>();
-  out.counters = counters_.as<unsigned long long>();
-  out.error = err_.as<int>();
+  out.counters = counters;
+  out.error = errp;
   ps.nsamp = nsamp;
   ps.cfv = cf_view();
-  customizePass(ps);
+  if (!arena) customizePass(ps);
   return true;
 }
 
@@ -448,8 +462,8 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   if (!setup_pass(a, forward, ps)) return;
   const ChunkDev& d = ps.d;
   const int nsamp = ps.nsamp;
-  YG_HIP(hipMemsetAsync(counters_.get(), 0, 8 * sizeof(unsigned long long), ctx_->stream));
-  YG_HIP(hipMemsetAsync(err_.get(), 0, sizeof(int), ctx_->stream));
+  YG_HIP(hipMemsetAsync(ps.out.counters, 0, 8 * sizeof(unsigned long long), ctx_->stream));
+  YG_HIP(hipMemsetAsync(ps.out.error, 0, sizeof(int), ctx_->stream));
   if (profiling_) YG_HIP(hipEventRecord(ev_[0], ctx_->stream));
   launch_prep(*ctx_, ps.pp, ps.in, ps.out);
   afterPrep(ps);
@@ -459,12 +473,12 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   if (bin_samples(*ctx_, smp, d.nrow, d.nchan, shape_[0], shape_[1], ps.pp.nzg * shape_[3], cfv)) ++n_bin_reused_;
   Binning& bins = ctx_->bins.bin;
   if (forward) {
-    YG_HIP(hipMemsetAsync(acc_.get(), 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
+    YG_HIP(hipMemsetAsync(ps.acc, 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, acc_.as<float>(),
+    launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, ps.acc,
                   nGridPlanes() > 1 ? gridPlaneElems() : 0);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
-    launch_degrid_finish(*ctx_, smp, acc_.as<float>(), d.nrow, d.nchan, d.npol, d.vis_out);
+    launch_degrid_finish(*ctx_, smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out);
   } else {
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
     launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg,
@@ -473,8 +487,8 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   }
   unsigned long long cnt[4] = {0, 0, 0, 0};
   int err = 0;
-  YG_HIP(hipMemcpyAsync(cnt, counters_.get(), sizeof(cnt), hipMemcpyDeviceToHost, ctx_->stream));
-  YG_HIP(hipMemcpyAsync(&err, err_.get(), sizeof(int), hipMemcpyDeviceToHost, ctx_->stream));
+  YG_HIP(hipMemcpyAsync(cnt, ps.out.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx_->stream));
+  YG_HIP(hipMemcpyAsync(&err, ps.out.error, sizeof(int), hipMemcpyDeviceToHost, ctx_->stream));
   if (forward && a.mem == YGRID_MEM_HOST) {
     YG_HIP(hipMemcpyAsync(a.vis_out, d.vis_out, (size_t)nsamp * d.npol * sizeof(float2), hipMemcpyDeviceToHost,
                           ctx_->stream));

@@ -102,6 +102,9 @@ class Gridder {
   virtual void finaliseWeights(float* out, int mem);
   void initialiseDegrid(const ygrid_image_desc& img, const float* model, int mem);
   void degrid(const ygrid_chunk& acc);
+  // degrid() with every device buffer of the pass carved from `bytes` of caller-held device
+  // memory at `scratch` (no allocation; the CF cache must already exist)
+  void degridNoAlloc(const ygrid_chunk& acc, void* scratch, size_t bytes);
   void finaliseDegrid();
   bool isModelEmpty() const { return model_empty_; }
   bool initialisedFor(bool degrid) const { return initialised_ && degrid_mode_ == degrid; }
@@ -148,6 +151,9 @@ class Gridder {
   virtual bool doesCorrection() const { return true; }
   // whether initConvolutionFunction needs host copies of feed1/pointing1
   virtual bool needsRowMeta() const { return false; }
+  // whether a chunk can need CF planes the cache does not hold yet (A-projection: new feeds or
+  // fields); such gridders build CFs inside their passes
+  virtual bool cfPerChunk() const { return false; }
 
   void initialiseCommon(const ygrid_image_desc& img);
   void generic(const ygrid_chunk& acc, bool forward);
@@ -163,8 +169,11 @@ class Gridder {
     PrepOutputs out{};
     CFView cfv{};
     int nsamp = 0;
+    float* acc = nullptr;  // degridding: per-sample accumulators (nsamp x 8)
   };
-  bool setup_pass(const ygrid_chunk& acc, bool forward, PassSetup& ps);
+  // arena: carve the pass's buffers from it (the allocation-free entry point); null: the
+  // gridder's own buffers, grown as needed
+  bool setup_pass(const ygrid_chunk& acc, bool forward, PassSetup& ps, PassArena* arena = nullptr);
   void account_pass(bool forward, const unsigned long long cnt[4], int err, double seconds);
   // hooks of a pass: after its setup (may replace the CF view and prep arguments), and after the
   // prep kernel (device work only: also runs inside captured graphs)
@@ -200,15 +209,12 @@ class Gridder {
 
   DevBuf grid_, sumw_, scratch_, ccfx_, ccfy_;
   DevBuf smp_hdr_, smp_key_, smp_coef_, acc_, counters_, err_, chanmap_, visweight_, staging_;
-  Samples samples() const {
-    return Samples{smp_hdr_.as<SampleHdr>(), smp_key_.as<int32_t>(), smp_coef_.as<float>(),
-                   vis_prec_ != kPrecF32 ? smp_coef16_.as<uint16_t>() : nullptr, vis_prec_};
-  }
   // the CF table at the configured storage precision (building the 16-bit copy if needed)
   CFView cf_view();
   int cf_prec_ = kPrecF32, vis_prec_ = kPrecF32;  // storage precisions (cfprecision, visprecision)
   DevBuf smp_coef16_;
   double n_bin_reused_ = 0;  // passes that reused the context's cached binning
+  TileJob noalloc_job_{};    // host copy of the allocation-free pass's job (read by its H2D copy)
   PlanPtr plan_;
   double st_samples_g_ = 0, st_taps_g_ = 0, st_samples_d_ = 0, st_taps_d_ = 0, st_flag_ = 0, st_wflag_ = 0;
   double t_grid_ = 0, t_degrid_ = 0, t_cf_ = 0;

@@ -1376,9 +1376,8 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
   YG_HIP(hipGetLastError());
 }
 
-void size_static_bins(const Ctx& c, StaticBins& sb, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
-                      int chunk) {
-  (void)c;
+void static_bins_shape(const Ctx& c, StaticBinsView& v, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
+                       int chunk) {
   const int ntu = (nx + kTile - 1) / kTile, ntv = (ny + kTile - 1) / kTile;
   const size_t nkeys = (size_t)ntu * ntv * (size_t)nkz;
   YG_CHECK(nkeys < (size_t)INT_MAX, 1, "too many binning keys");
@@ -1388,58 +1387,68 @@ void size_static_bins(const Ctx& c, StaticBins& sb, int nrow, int nchan, int nx,
   const size_t nsamp = (size_t)nrow * nchan;
   const size_t cap = nsamp * span * span;
   YG_CHECK(cap < (size_t)INT_MAX, 1, "too many (sample, tile) pairs for one pass");
-  sb.nkeys = nkeys;
-  sb.npairs_cap = cap;
-  sb.chunk = std::max(chunk, 16);
-  sb.nitems_cap = cap / sb.chunk + std::min(nkeys, cap) + 1;
-  sb.nrow = nrow;
-  sb.nchan = nchan;
-  sb.nx = nx;
-  sb.ny = ny;
-  sb.nkz = nkz;
-  sb.counts.resize((nkeys + 1) * sizeof(int));
-  sb.offsets.resize((nkeys + 1) * sizeof(int));
-  sb.cursor.resize((nkeys + 1) * sizeof(int));
-  sb.icount.resize((nkeys + 1) * sizeof(int));
-  sb.ioffsets.resize((nkeys + 1) * sizeof(int));
-  sb.pairs.resize(std::max<size_t>(cap, 1) * sizeof(int4));
-  size_t t1 = 0, t2 = 0;
-  YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, sb.counts.as<int>(), sb.offsets.as<int>(), (int)(nkeys + 1),
-                                          c.stream));
-  YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, sb.icount.as<int>(), sb.ioffsets.as<int>(), (int)(nkeys + 1),
-                                          c.stream));
-  sb.temp_bytes = std::max(t1, t2);
-  sb.temp.resize(std::max<size_t>(sb.temp_bytes, 1));
+  v.nkeys = nkeys;
+  v.npairs_cap = cap;
+  v.chunk = std::max(chunk, 16);
+  v.nitems_cap = cap / v.chunk + std::min(nkeys, cap) + 1;
+  v.nrow = nrow;
+  v.nchan = nchan;
+  v.nx = nx;
+  v.ny = ny;
+  v.nkz = nkz;
+  // the scan's temporary storage: a size query with null buffers
+  size_t t1 = 0;
+  YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, (int*)nullptr, (int*)nullptr, (int)(nkeys + 1), c.stream));
+  v.temp_bytes = std::max<size_t>(t1, 1);
 }
 
-void enqueue_static_binning(const Ctx& c, StaticBins& sb, const Samples& smp, const CFView& cf, int4* items_out,
-                            const int* base_in, int* base_out, int job) {
+void size_static_bins(const Ctx& c, StaticBins& sb, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
+                      int chunk) {
+  StaticBinsView& v = sb.v;
+  static_bins_shape(c, v, nrow, nchan, nx, ny, nkz, cf, chunk);
+  const size_t nk1 = v.nkeys + 1;
+  sb.counts.resize(nk1 * sizeof(int));
+  sb.offsets.resize(nk1 * sizeof(int));
+  sb.cursor.resize(nk1 * sizeof(int));
+  sb.icount.resize(nk1 * sizeof(int));
+  sb.ioffsets.resize(nk1 * sizeof(int));
+  sb.pairs.resize(std::max<size_t>(v.npairs_cap, 1) * sizeof(int4));
+  sb.temp.resize(v.temp_bytes);
+  v.counts = sb.counts.as<int>();
+  v.offsets = sb.offsets.as<int>();
+  v.cursor = sb.cursor.as<int>();
+  v.icount = sb.icount.as<int>();
+  v.ioffsets = sb.ioffsets.as<int>();
+  v.pairs = sb.pairs.as<int4>();
+  v.temp = sb.temp.get();
+}
+
+void enqueue_static_binning(const Ctx& c, const StaticBinsView& sb, const Samples& smp, const CFView& cf,
+                            int4* items_out, const int* base_in, int* base_out, int job) {
   const int n = sb.nrow * sb.nchan;
   const int ntu = (sb.nx + kTile - 1) / kTile, ntv = (sb.ny + kTile - 1) / kTile;
   const int ntiles = ntu * ntv;
   const int nk = (int)sb.nkeys;
-  int* counts = sb.counts.as<int>();
-  int* offsets = sb.offsets.as<int>();
+  int* counts = sb.counts;
+  int* offsets = sb.offsets;
   launch_fill_u32(c, counts, (size_t)nk + 1, 0u);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, smp, sb.nrow, sb.nchan, ntu, ntiles,
                        counts);
   size_t tb = sb.temp_bytes;
-  YG_HIP(hipcub::DeviceScan::ExclusiveSum(sb.temp.get(), tb, counts, offsets, nk + 1, c.stream));
+  YG_HIP(hipcub::DeviceScan::ExclusiveSum(sb.temp, tb, counts, offsets, nk + 1, c.stream));
   hipLaunchKernelGGL(k_copy_i32, dim3(std::min((nk + 256) / 256, 4 * c.n_cu)), dim3(256), 0, c.stream, offsets,
-                     sb.cursor.as<int>(), (size_t)nk + 1);
+                     sb.cursor, (size_t)nk + 1);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, smp, sb.nrow, sb.nchan, ntu, ntiles,
-                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), sb.cursor.as<int>(),
-                       sb.pairs.as<int4>());
+                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), sb.cursor, sb.pairs);
   hipLaunchKernelGGL(k_item_count, dim3((nk + 1 + 255) / 256), dim3(256), 0, c.stream, counts, nk, sb.chunk,
-                     sb.icount.as<int>());
+                     sb.icount);
   tb = sb.temp_bytes;
-  YG_HIP(hipcub::DeviceScan::ExclusiveSum(sb.temp.get(), tb, sb.icount.as<int>(), sb.ioffsets.as<int>(), nk + 1,
-                                          c.stream));
-  hipLaunchKernelGGL(k_item_emit, dim3((nk + 255) / 256), dim3(256), 0, c.stream, counts, offsets,
-                     sb.ioffsets.as<int>(), nk, sb.chunk, base_in, job, items_out);
-  hipLaunchKernelGGL(k_item_base, dim3(1), dim3(64), 0, c.stream, base_in, sb.ioffsets.as<int>() + nk, base_out);
+  YG_HIP(hipcub::DeviceScan::ExclusiveSum(sb.temp, tb, sb.icount, sb.ioffsets, nk + 1, c.stream));
+  hipLaunchKernelGGL(k_item_emit, dim3((nk + 255) / 256), dim3(256), 0, c.stream, counts, offsets, sb.ioffsets, nk,
+                     sb.chunk, base_in, job, items_out);
+  hipLaunchKernelGGL(k_item_base, dim3(1), dim3(64), 0, c.stream, base_in, sb.ioffsets + nk, base_out);
   YG_HIP(hipGetLastError());
 }
 

@@ -77,6 +77,8 @@ int ygrid_ctx_create(int device, ygrid_ctx** out) {
     variant("YGRID_GRID_VARIANT", c->grid_variant);
     variant("YGRID_DEGRID_VARIANT", c->degrid_variant);
     if (const char* s = std::getenv("YGRID_ITEM_PAIRS")) c->item_pairs = std::max(16, std::atoi(s));
+    // the allocation-free entry point's workspace on this device (its device is current here)
+    ygrid::noalloc_setup(device);
     YG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
     *out = new ygrid_ctx{c};
@@ -190,6 +192,16 @@ int ygrid_degrid(ygrid_gridder* g, const ygrid_chunk* acc) {
     NEED(g);
     NEED(acc);
     g->g->degrid(*acc);
+  });
+}
+
+int ygrid_degrid_noalloc(ygrid_gridder* g, const ygrid_chunk* acc) {
+  return guard([&] {
+    NEED(g);
+    NEED(acc);
+    const int dev = g->g->context()->device;
+    YG_HIP(hipSetDevice(dev));
+    g->g->degridNoAlloc(*acc, ygrid::noalloc_scratch(dev), ygrid::kNoAllocScratchBytes);
   });
 }
 

@@ -271,19 +271,54 @@ TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, f
 void launch_tile_jobs(const Ctx&, const TileVariant& v, bool forward, int npol, int prec, const TileJob* jobs,
                       const int4* items, const int* nitems, int* counter);
 
-// Sync-free binning of one pass into buffers sized by upper bounds (for batches and graph
-// capture): no host readback, no allocation, no memset nodes.  The pass's work items go to
-// items_out[*base_in ...) with job index `job`, and *base_out = *base_in + its item count.
-struct StaticBins {
-  DevBuf counts, offsets, cursor, icount, ioffsets, pairs, temp;
+// Sync-free binning of one pass into buffers sized by upper bounds (for batches, graph capture
+// and the allocation-free entry point): no host readback, no allocation, no memset nodes.  The
+// pass's work items go to items_out[*base_in ...) with job index `job`, and
+// *base_out = *base_in + its item count.  The view holds the buffers and their sizes; they are
+// owned by a StaticBins or carved from a caller's workspace.
+struct StaticBinsView {
+  int *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *icount = nullptr, *ioffsets = nullptr;
+  int4* pairs = nullptr;
+  void* temp = nullptr;
   size_t nkeys = 0, npairs_cap = 0, nitems_cap = 0, temp_bytes = 0;
   int nrow = 0, nchan = 0, nx = 0, ny = 0, nkz = 0, chunk = 0;
+};
+// The sizes of the binning of nrow x nchan samples of support <= cf.max_support (no allocation:
+// the scan's temporary size is a query).
+void static_bins_shape(const Ctx&, StaticBinsView& v, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
+                       int chunk);
+struct StaticBins {
+  DevBuf counts, offsets, cursor, icount, ioffsets, pairs, temp;
+  StaticBinsView v;
 };
 // Sizes sb for nrow x nchan samples of support <= cf.max_support (allocates).
 void size_static_bins(const Ctx&, StaticBins& sb, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
                       int chunk);
-void enqueue_static_binning(const Ctx&, StaticBins& sb, const Samples& smp, const CFView& cf, int4* items_out,
+void enqueue_static_binning(const Ctx&, const StaticBinsView& sb, const Samples& smp, const CFView& cf, int4* items_out,
                             const int* base_in, int* base_out, int job);
+
+// A fixed device workspace that a pass's buffers are carved from (256-byte aligned pieces);
+// running out of it is an error, never an allocation.
+struct PassArena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t a = (used + 255) & ~(size_t)255;
+    YG_CHECK(a <= cap && bytes <= cap - a, 1,
+             "the pass needs more than " + std::to_string((a + bytes + (1 << 20) - 1) >> 20) +
+                 " MiB of device workspace; the allocation-free entry point has " + std::to_string(cap >> 20) +
+                 " MiB: split the chunk");
+    used = a + bytes;
+    return base + a;
+  }
+  template <class T> T* take_n(size_t n) { return static_cast<T*>(take(n * sizeof(T))); }
+};
+// The allocation-free entry point's device workspace (noalloc.cpp): the library's file-scope
+// device array `scratch`, resolved once per device when a context is created on it.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+void noalloc_setup(int device);
+// the current device's copy of `scratch` (after noalloc_setup on that device)
+void* noalloc_scratch(int device);
 // fills n 32-bit words with value (a kernel: usable inside captured graphs)
 void launch_fill_u32(const Ctx&, void* p, size_t n, uint32_t value);
 // Polarisation conversion of visibility cubes (casacore Cube(nRow, nChan, nPol) layout: the
