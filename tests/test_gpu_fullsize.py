@@ -14,7 +14,7 @@ import pytest
 
 from oracle import oracle as O
 from tests.conftest import announce
-from tests.test_gpu_parity import gain_of, image_ok
+from tests.test_gpu_parity import finalise_with, gain_of, image_ok
 from yandasoft_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -53,7 +53,7 @@ def test_sphfunc_disc_full_size(gpu_ctx):
     # accumulation order is no closer to it than ours)
     assert rel(ggrid, o64.grid_) < TOL
     assert rel(ggrid, o.grid_) < 2 * TOL
-    ok, info = image_ok(gimg, o.finalise_grid(), o64.finalise_grid(), gain_of(o))
+    ok, info = image_ok(gimg, o64.finalise_grid(), gain_of(o), lambda: finalise_with(o, ggrid))
     assert ok, info
     assert rel(g.finalise_weights(), o.finalise_weights()) < 1e-6
     # degrid a point-source model

@@ -3,11 +3,15 @@
 Bar: max|gpu - oracle| / max|oracle| <= 1e-5 (FP32) for the grid, the sum of weights and the
 degridded visibilities; CF planes to 1e-5 of their peak.
 
-Dirty images: the grid correction multiplies the image edges by 1/grdsf (up to ~1e2), which
-lifts the rounding of any FP32 grid accumulation there to ~1e-4 of the peak (the oracle's own
-FP32 restatement differs from its FP64 form by that much).  Images are therefore compared with
-the FP64 oracle: <= 1e-5 of the peak where the correction gain is <= 10, and over the whole
-image no worse than twice the FP32 restatement's own deviation from FP64.
+Dirty images (image_ok): the grid correction multiplies the image edges by 1/grdsf (up to ~1e4
+at 128^2), which lifts the rounding of any FP32 grid accumulation there -- the reference's
+included: the oracle's own FP32 restatement differs from its FP64 form by ~4e-5 of the peak at
+the edges.  An image is therefore pinned in three parts, none statistical: against the FP64
+oracle image, <= 1e-5 of the peak where the correction gain is <= 10 and, everywhere, <= 1e-5 of
+the peak once divided by the gain (the gridder's error before the correction amplifies it);
+and, everywhere including the edges, equal to the oracle's FP64 finalise (transform,
+correction, crop) of the GPU's own grid to 1e-6 of its peak (the FP32 output rounding).  With
+the grid parity above this fixes the whole image.
 """
 import math
 
@@ -28,14 +32,28 @@ def rel(a, b):
     return float(np.max(np.abs(np.asarray(a) - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-def image_ok(gi, oi32, oi64, gain):
-    """gain: (ny, nx) grid-correction factor of the oracle gridder."""
+def image_ok(gi, oi64, gain, refinalise=None):
+    """gain: (ny, nx) grid-correction factor of the oracle gridder; refinalise: callable returning
+    the oracle's FP64 finalise of the GPU's own grid (None: not checked)."""
     peak = max(float(np.max(np.abs(oi64))), 1e-300)
     err = np.abs(np.asarray(gi, np.float64) - oi64)
     inner = float(np.max(err[..., gain <= 10.0])) / peak
-    full = float(np.max(err)) / peak
-    ref = float(np.max(np.abs(oi32 - oi64))) / peak
-    return inner <= TOL and full <= max(TOL, 2.0 * ref), (inner, full, ref)
+    normed = float(np.max(err / gain)) / peak
+    fin = 0.0
+    if refinalise is not None:
+        f = refinalise()
+        fin = float(np.max(np.abs(np.asarray(gi, np.float64) - f))) / max(float(np.max(np.abs(f))), 1e-300)
+    return inner <= TOL and normed <= TOL and fin <= 1e-6, (inner, normed, fin)
+
+
+def finalise_with(o, grid):
+    """The oracle's finaliseGrid (FP64) applied to another grid of the same shape."""
+    saved = o.grid_
+    o.grid_ = np.asarray(grid, np.complex128).reshape(saved.shape)
+    try:
+        return o.finalise_grid()
+    finally:
+        o.grid_ = saved
 
 
 def gain_of(o):
@@ -101,8 +119,8 @@ def grid_both(ctx, name, params, ogrid_kwargs, d, N, cell, npol, nchan_img=1, do
         o.grid(ochunk(d))
         imgs[prec] = o.finalise_grid()
     ow = o.finalise_weights()
-    ok, info = image_ok(gimg, imgs["f32"], imgs["f64"], gain_of(o))
-    assert ok, f"image parity (inner, full, f32-oracle) = {info}"
+    ok, info = image_ok(gimg, imgs["f64"], gain_of(o), lambda: finalise_with(o, ggrid))
+    assert ok, f"image parity (inner, gain-normalised, own-grid finalise) = {info}"
     return g, o, ggrid, o.grid_, gimg, imgs["f32"], gw, ow
 
 
@@ -334,7 +352,7 @@ def test_awproject_disk_matches_oracle(gpu_ctx, taper):
     o64.aw_lm, o64.aw_nchan = o.aw_lm, 2
     o64.initialise_grid((N, N, 2, 1), (cell, cell))
     o64.grid(och)
-    ok, info = image_ok(g.finalise_grid(), o.finalise_grid(), o64.finalise_grid(), gain_of(o))
+    ok, info = image_ok(g.finalise_grid(), o64.finalise_grid(), gain_of(o), lambda: finalise_with(o, gg))
     assert ok, info
     gw, ow = g.finalise_weights(), o.finalise_weights()
     if taper:

@@ -40,10 +40,18 @@ def test_wstack_grid_and_image_match_oracle(gpu_ctx, npol):
     gi = g.finalise_grid()
     o64 = O.OracleWStack(WS["wmax"], WS["nwplanes"], precision="f64")
     o64.grid((N, N, npol, 1), (cell, cell), ochunk(d))
-    # as for the single-grid images: TOL inside gain <= 10, twice the FP32 oracle's own error at
-    # the corrected edges
-    ok, info = image_ok(gi, o.finalise_grid(), o64.finalise_grid(), gain_of(o.o))
-    assert ok, f"image parity (inner, full, f32-oracle) = {info}"
+
+    def own_grids():
+        saved = o64.grids
+        o64.grids = [np.asarray(gg[iw], np.complex128) for iw in range(WS["nwplanes"])]
+        try:
+            return o64.finalise_grid()
+        finally:
+            o64.grids = saved
+
+    # as for the single-grid images (test_gpu_parity's image_ok)
+    ok, info = image_ok(gi, o64.finalise_grid(), gain_of(o.o), own_grids)
+    assert ok, f"image parity (inner, gain-normalised, own-grid finalise) = {info}"
     assert rel(g.finalise_weights(), o.sumw) < 1e-6
 
 

@@ -603,14 +603,6 @@ struct PairU {
   int4 d;
 };
 
-__device__ __forceinline__ PairU staged_pair(const int4* s_desc, int i) {
-  const int4 d = s_desc[i];
-  PairU p;
-  p.d = make_int4(__builtin_amdgcn_readfirstlane(d.x), __builtin_amdgcn_readfirstlane(d.y),
-                  __builtin_amdgcn_readfirstlane(d.z), __builtin_amdgcn_readfirstlane(d.w));
-  return p;
-}
-
 // Geometry of a pair seen from wave w: bit 2k+j of mask = block (k, j) active, soff0 = byte
 // offset of the CF tap under the wave's first pixel (lane part excluded), rowstep = bytes per 4
 // CF rows, and the lane column-pair range [A, B] (see pair_descriptor).  The rows of an active
@@ -620,6 +612,23 @@ struct PairGeom {
   int P, A, B;
   uint32_t mask, soff0, rowstep;
 };
+
+// The active blocks of a pair in one wave, from its 8-bit mask (bit 2k+j: block (k, j)), tested
+// per block at run time.  (Straight-line code per block rectangle, chosen by one dispatch, was
+// built and compiled to kernels that spill hundreds of registers; the tests stay.)
+struct MaskShape {
+  uint32_t mask;
+};
+template <int V>
+using IC = std::integral_constant<int, V>;
+// f(IC<k>, IC<j>) for the active blocks, rows first
+template <int B = 0, class F>
+__device__ __forceinline__ void for_blocks(MaskShape shp, F&& f) {
+  if constexpr (B < 8) {
+    if (shp.mask & (1u << B)) f(IC<(B >> 1)>{}, IC<(B & 1)>{});
+    for_blocks<B + 1>(shp, f);
+  }
+}
 
 template <int SH>
 __device__ __forceinline__ PairGeom pair_geom(const PairU& p, int w) {
@@ -659,17 +668,14 @@ constexpr int cf_shift() {
   return CFP == kPrecF32 ? 3 : 2;
 }
 
-template <bool COEF, int CFP>
+template <bool COEF, int CFP, class SHP>
 __device__ __forceinline__ void load_cf(Taps<CFP>& c, __amdgpu_buffer_rsrc_t cfr, const PairGeom& g, int lu, int lv,
-                                        const f32x4* s_coef_pair, int o) {
+                                        const f32x4* s_coef_pair, int o, SHP shp) {
   if constexpr (COEF) c.b = s_coef_pair[o];
   uint32_t vo[2];
   lane_offsets<cf_shift<CFP>()>(vo, g, lu, lv);
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-      if (g.mask & (1u << (2 * k + j))) {
+  for_blocks(shp, [&](auto kc, auto jc) {
+        constexpr int k = decltype(kc)::value, j = decltype(jc)::value;
         // dword loads, merged into one dwordx4 (dwordx2) by the compiler (the vector-valued
         // buffer load builtins of this toolchain load only the first dword)
         const uint32_t so = g.soff0 + k * g.rowstep;
@@ -683,7 +689,7 @@ __device__ __forceinline__ void load_cf(Taps<CFP>& c, __amdgpu_buffer_rsrc_t cfr
 #pragma unroll
           for (int e = 0; e < 2; ++e) c.t[k][j][e] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo[j] + 4u * e, so, 0);
         }
-      }
+  });
 }
 
 // acc += v * C (or v * conj(C) in the PCF half plane, TableVisGridder.cc:837-844) for the
@@ -692,13 +698,10 @@ __device__ __forceinline__ void load_cf(Taps<CFP>& c, __amdgpu_buffer_rsrc_t cfr
 // C.im (x) B[2m+1].  MFMA block b = lane/4 holds the taps of its lanes 4b..4b+3 (4 pixels) as A
 // and the sample's coefficients for outputs (pol 2m re, pol 2m im, pol 2m+1 re, pol 2m+1 im) as B;
 // the result for the pixel of lane 4b+i and output o lands in lane 4b+o, register i.
-template <int NP, class T>
-__device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], const T& c, const PairGeom& g) {
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-      if (g.mask & (1u << (2 * k + j))) {
+template <int NP, class T, class SHP>
+__device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], const T& c, SHP shp) {
+  for_blocks(shp, [&](auto kc, auto jc) {
+        constexpr int k = decltype(kc)::value, j = decltype(jc)::value;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const float2 x = c.tap(k, j, e);
@@ -709,7 +712,7 @@ __device__ __forceinline__ void grid_pair(f32x4 (&acc)[4][2][2][(NP + 1) / 2], c
             acc[k][j][e][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(x.y, c.b[3], acc[k][j][e][1], 0, 0, 0);
           }
         }
-      }
+  });
 }
 
 
@@ -738,58 +741,122 @@ __device__ __forceinline__ int next_item(int* __restrict__ counter, int nitems, 
   }
 }
 
-// Runs consume(c, geom, pair, i) for the n staged pairs of a work item in wave w, with the CF
-// taps (and, COEF, the MFMA B operands) of pair i in c.  The descriptor of the next pair is read
-// before the current pair is worked.  PIPE: two register sets; the loads of pair i+1 are issued
-// before pair i is consumed.  consume must not issue vector memory loads whose results it waits
-// for after returning (the wait before the next loads covers them).
-template <bool COEF, bool PIPE, int CFP, class F>
-__device__ __forceinline__ void for_each_pair(const int4* s_desc, const f32x4* s_coef, __amdgpu_buffer_rsrc_t cfr,
-                                              int n, int w, int lu, int lv, int o, F&& consume) {
+// The staged pairs whose footprint touches wave w's rows, as indices into the round's staging
+// arrays: the wave then skips the pairs it has nothing to add for.  Returns their number.
+__device__ __forceinline__ int build_wave_list(const int4* s_desc, int n, int w, int lane, uint16_t* list) {
+  int cnt = 0;
+  for (int b = 0; b < n; b += 64) {
+    const int t = b + lane;
+    const bool act = t < n && ((((uint32_t)s_desc[t].z) >> (8 * w)) & 0xffu) != 0u;
+    const unsigned long long bal = __ballot(act);
+    const int before = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+    if (act) list[cnt + before] = (uint16_t)t;
+    cnt += __popcll(bal);
+  }
+  return __builtin_amdgcn_readfirstlane(cnt);
+}
+
+__device__ __forceinline__ PairU make_uniform(const int4& d) {
+  PairU p;
+  p.d = make_int4(__builtin_amdgcn_readfirstlane(d.x), __builtin_amdgcn_readfirstlane(d.y),
+                  __builtin_amdgcn_readfirstlane(d.z), __builtin_amdgcn_readfirstlane(d.w));
+  return p;
+}
+
+// Runs consume(c, geom, pair, i) for n staged pairs of a work item in wave w, with the CF taps
+// (and, COEF, the MFMA B operands) of pair i in c.
+// COMPACT: the pairs are list[0 .. n) (build_wave_list), else the staged pairs 0 .. n.  A pair's
+// loads are issued for its active blocks.  The descriptor of pair i+1 is read
+// from LDS before pair i is consumed and made wave-uniform after it, and the list entry one pair
+// further ahead, so that no LDS latency stands between the waits and the products.  PIPE: two
+// register sets; the loads of pair i+1 are issued before pair i is consumed.  consume must not
+// issue vector memory loads whose results it waits for after returning (the wait before the
+// next loads covers them).
+template <bool COEF, bool PIPE, int CFP, bool COMPACT, class F>
+__device__ __forceinline__ void for_each_pair(const int4* s_desc, const f32x4* s_coef, const uint16_t* list,
+                                              __amdgpu_buffer_rsrc_t cfr, int n, int w, int lu, int lv, int o,
+                                              F&& consume) {
   constexpr int SH = cf_shift<CFP>();
   if (n <= 0) return;
   constexpr int CW = kCoef / 4;  // f32x4 per staged pair
+  auto entry = [&](int i) -> int {
+    if constexpr (COMPACT) return list[i];
+    else return i;
+  };
+  auto issue = [&](Taps<CFP>& c, const PairGeom& g, int ix) {
+    load_cf<COEF, CFP>(c, cfr, g, lu, lv, COEF ? s_coef + CW * ix : s_coef, o, MaskShape{g.mask});
+  };
   if constexpr (!PIPE) {
-    PairU pn = staged_pair(s_desc, 0);
+    int ixn = entry(0);
+    int4 rn = s_desc[ixn];
+    int ixq = n > 1 ? entry(1) : 0;
     for (int i = 0; i < n; ++i) {
       // Nothing is in flight here (the previous pair consumed its taps); saying so keeps the
-      // wait-count analysis, which cannot correlate the load and use conditions of a block,
-      // from waiting for all loads before each load that reuses a register.
+      // wait-count analysis, which cannot correlate the loads of a shape with their uses, from
+      // waiting for all loads before each load that reuses a register.
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      const PairU p = pn;
-      if (i + 1 < n) pn = staged_pair(s_desc, i + 1);
+      const PairU p = make_uniform(rn);
+      const int ix = ixn;
+      if (i + 1 < n) {
+        ixn = ixq;
+        rn = s_desc[ixn];
+      }
+      if (i + 2 < n) ixq = entry(i + 2);
       const PairGeom g = pair_geom<SH>(p, w);
       Taps<CFP> c;
-      load_cf<COEF, CFP>(c, cfr, g, lu, lv, COEF ? s_coef + CW * i : s_coef, o);
+      issue(c, g, ix);
       consume(c, g, p, i);
     }
   } else {
     int i = 0;
     Taps<CFP> cA, cB;
-    PairU pA = staged_pair(s_desc, 0);
+    const int ix0 = entry(0);
+    PairU pA = make_uniform(s_desc[ix0]);
     PairGeom gA = pair_geom<SH>(pA, w);
-    load_cf<COEF, CFP>(cA, cfr, gA, lu, lv, s_coef, o);
+    issue(cA, gA, ix0);
+    // (ixn, rn): list entry and raw descriptor of the next pair; ixq: entry of the one after
+    int ixn = ix0, ixq = 0;
+    int4 rn = make_int4(0, 0, 0, 0);
+    if (n > 1) {
+      ixn = entry(1);
+      rn = s_desc[ixn];
+    }
+    if (n > 2) ixq = entry(2);
     PairU pB = pA;
-    if (n > 1) pB = staged_pair(s_desc, 1);
+    PairGeom gB = gA;
     for (;;) {
-      // pair i: taps in flight into cA; pB = record of pair i+1
+      // pair i: taps in flight into cA
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      const PairGeom gB = pair_geom<SH>(pB, w);
-      if (i + 1 < n) load_cf<COEF, CFP>(cB, cfr, gB, lu, lv, COEF ? s_coef + CW * (i + 1) : s_coef, o);
-      PairU pN = pB;
-      if (i + 2 < n) pN = staged_pair(s_desc, i + 2);
+      if (i + 1 < n) {
+        pB = make_uniform(rn);
+        gB = pair_geom<SH>(pB, w);
+        issue(cB, gB, ixn);
+      }
+      if (i + 2 < n) {
+        ixn = ixq;
+        rn = s_desc[ixn];
+      }
+      if (i + 3 < n) ixq = entry(i + 3);
+      __builtin_amdgcn_sched_barrier(0);
       consume(cA, gA, pA, i);
+      __builtin_amdgcn_sched_barrier(0);
       if (++i >= n) break;
-      // pair i: taps in flight into cB; pN = record of pair i+1
+      // pair i: taps in flight into cB
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-      gA = pair_geom<SH>(pN, w);
-      if (i + 1 < n) load_cf<COEF, CFP>(cA, cfr, gA, lu, lv, COEF ? s_coef + CW * (i + 1) : s_coef, o);
-      PairU pM = pN;
-      if (i + 2 < n) pM = staged_pair(s_desc, i + 2);
+      if (i + 1 < n) {
+        pA = make_uniform(rn);
+        gA = pair_geom<SH>(pA, w);
+        issue(cA, gA, ixn);
+      }
+      if (i + 2 < n) {
+        ixn = ixq;
+        rn = s_desc[ixn];
+      }
+      if (i + 3 < n) ixq = entry(i + 3);
+      __builtin_amdgcn_sched_barrier(0);
       consume(cB, gB, pB, i);
+      __builtin_amdgcn_sched_barrier(0);
       if (++i >= n) break;
-      pA = pN;
-      pB = pM;
     }
   }
 }
@@ -800,6 +867,7 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
   __shared__ int s_item;
   __shared__ int4 s_desc[kChunk];
   __shared__ f32x4 s_coef[kChunk * (kCoef / 4)];
+  __shared__ uint16_t s_list[4][kChunk];  // per wave: the staged pairs that touch its rows
   static_assert(4 * kFlushWave <= kChunk * kCoef, "flush areas must fit in the staging space");
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -844,8 +912,11 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
       const int m = min(kChunk, n - cb);
       if (cb > 0) __syncthreads();  // every wave is done with the previous round's staging
       stage_item<true>(J.descs, J.smp, it.y + cb, m, s_desc, s_coef, tid, blockDim.x);
-      for_each_pair<true, PIPE, CFP>(s_desc, s_coef, cfr, m, w, lu, lv, o,
-                                     [&](const auto& c, const PairGeom& g, const PairU&, int) { grid_pair<NP>(acc, c, g); });
+      const int mw = build_wave_list(s_desc, m, w, lane, s_list[w]);
+      for_each_pair<true, PIPE, CFP, true>(s_desc, s_coef, s_list[w], cfr, mw, w, lu, lv, o,
+                                           [&](const auto& c, const PairGeom& g, const PairU&, int) {
+                                             grid_pair<NP>(acc, c, MaskShape{g.mask});
+                                           });
     }
     // flush: lane 4b+o holds output o (pol 2m + o/2, re/im o&1) of row 4k+lv, column
     // 32j+8(b&3)+2i+e.  Per (m, k) the wave writes its 4 rows x 64 columns x 4 outputs to its own
@@ -933,16 +1004,12 @@ __device__ __forceinline__ float wave_reduce_scatter8(const float (&x)[8], int l
 
 // sum over the wave's active blocks of C * conj(G) per polarisation (GridKernel.cc:121-122);
 // x[2q] / x[2q+1] = real / imaginary part for polarisation q, x[2NP..7] = 0
-template <int NP, class T>
-__device__ __forceinline__ void degrid_pair(float (&x)[8], const T& c, const float2 (&gt)[4][2][2][NP],
-                                            const PairGeom& g) {
+template <int NP, class T, class SHP>
+__device__ __forceinline__ void degrid_pair(float (&x)[8], const T& c, const float2 (&gt)[4][2][2][NP], SHP shp) {
 #pragma unroll
   for (int i = 0; i < 8; ++i) x[i] = 0.f;
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-      if (g.mask & (1u << (2 * k + j))) {
+  for_blocks(shp, [&](auto kc, auto jc) {
+        constexpr int k = decltype(kc)::value, j = decltype(jc)::value;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const float2 cc = c.tap(k, j, e);
@@ -954,7 +1021,7 @@ __device__ __forceinline__ void degrid_pair(float (&x)[8], const T& c, const flo
             x[2 * q + 1] = fmaf(-cc.x, gt[k][j][e][q].y, x[2 * q + 1]);
           }
         }
-      }
+  });
 }
 
 template <int NP, int WPE, bool PIPE, int CFP>
@@ -1010,14 +1077,14 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     const int n = min(kChunk, nall - cb);
     if (cb > 0) __syncthreads();  // every wave is done with the previous round's staging
     stage_item<false>(J.descs, Samples{}, it.y + cb, n, s_desc, nullptr, tid, blockDim.x);
-    for_each_pair<false, PIPE, CFP>(s_desc, nullptr, cfr, n, w, lu, lv, 0,
+    for_each_pair<false, PIPE, CFP, false>(s_desc, nullptr, nullptr, cfr, n, w, lu, lv, 0,
                            [&](const auto& c, const PairGeom& g, const PairU& p, int i) {
                              // a wave whose rows the footprint misses (mask 0) contributes zeros
                              // without the dot product and the reduction
                              float r = 0.f;
                              if (g.mask) {
                                float x[8];
-                               degrid_pair<NP>(x, c, gt, g);
+                               degrid_pair<NP>(x, c, gt, MaskShape{g.mask});
                                r = wave_reduce_scatter8(x, lane);
                              }
                              const int b = i % kDegridBatch;
