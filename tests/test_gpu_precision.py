@@ -10,9 +10,15 @@ BF16, round to nearest even, when stored; products and sums stay FP32.
   image where the grid correction's gain is <= 10 (max |image - image64| / max |image64|), and
   the degridded visibilities.  The rounding of the CF taps is a white error floor in the image
   plane, which the correction (1/grdsf, up to ~1e3 at the image edge) lifts toward O(1) of the
-  peak at the edge, so the edge is not a measure of the storage precision.  The bounds are what
-  the precision allows (FP16 keeps 11 significant bits, BF16 8) with headroom; the measured
-  values are printed.
+  peak at the edge, so the edge is not a measure of the storage precision.  The bound BOUND is
+  what the precision's rounding allows (FP16 keeps 11 significant bits: a relative step of
+  2^-11 = 4.9e-4; BF16 8 bits: 3.9e-3); the measured values are printed.
+* The target of 1e-4 of the peak for the 16-bit form is asserted for FP16 in
+  test_fp16_reaches_1e4_of_peak, an expected failure: FP16 taps give 2-5e-4 on the grid, the
+  image and the degridded visibilities at every support measured (3 to 14; the errors of one
+  plane's taps are shared by all its samples), FP32 coefficients included, and a power-of-two
+  scale of the table changes nothing (its taps lie far above FP16's subnormal range).  BF16 is
+  reported only.
 """
 import math
 
@@ -26,8 +32,10 @@ from tests.test_gpu_parity import ARC, degrid_both, gain_of, make_data, ochunk, 
 pytestmark = pytest.mark.gpu
 
 PRECS = ("fp16", "bf16")
-# grid, image (gain <= 10) and degridded-visibility error vs FP64, relative to the peak
-BOUND = {"fp16": 2e-3, "bf16": 2e-2}
+# grid, image (gain <= 10) and degridded-visibility error vs FP64, relative to the peak: four
+# relative rounding steps of the precision (2^-11 FP16, 2^-8 BF16)
+BOUND = {"fp16": 4 * 2.0 ** -11, "bf16": 4 * 2.0 ** -8}
+TARGET = 1e-4  # the 16-bit form's image-error target against the FP64 oracle
 
 
 def inner_err(img, img64, gain):
@@ -85,9 +93,9 @@ def test_single_sample_grids_rounded_cf(gpu_ctx, cfp, visp):
 WP = dict(wmax=3000.0, nwplanes=9, oversample=4, maxsupport=256, cutoff=1e-2)
 
 
-@pytest.mark.parametrize("prec", PRECS)
-def test_wproject_mixed_precision_error_vs_fp64(gpu_ctx, prec):
-    announce(f"wproject cf/vis precision {prec} vs fp64 oracle")
+def _wproject_errors(gpu_ctx, prec, visprec=None):
+    """(grid, image (gain <= 10), degrid) errors of W-projection at storage precision prec vs FP64,
+    and the FP32 mode's grid and image errors."""
     import yandasoft_amd as ya
     N, cell = 256, 30 * ARC
     d = make_data(1500, 2, 4, N, cell, f0=1.0e9, frac=0.35, wmax_m=2500.0 * O.C_LIGHT / 1.005e9)
@@ -100,7 +108,7 @@ def test_wproject_mixed_precision_error_vs_fp64(gpu_ctx, prec):
     gain = gain_of(o)
     eg, eh = {}, {}
     for p in ("fp32", prec):
-        params = dict(WP, cfprecision=p, visprecision=p)
+        params = dict(WP, cfprecision=p, visprecision=visprec or p)
         g = ya.Gridder(gpu_ctx, "GPUWProject", params)
         g.initialise_grid(ya.ImageDesc(N, N, 4, 1, cell, cell))
         g.grid(pchunk(d))
@@ -114,10 +122,25 @@ def test_wproject_mixed_precision_error_vs_fp64(gpu_ctx, prec):
     od.initialise_degrid((N, N, 4, 1), (cell, cell), model)
     ov64 = od.degrid(ochunk(d))
     ev = rel(gv, ov64)
-    print(f"[precision] wproject {prec}: grid err {eg[prec]:.3e} (fp32 {eg['fp32']:.3e}), image err (gain<=10) "
-          f"{eh[prec]:.3e} (fp32 {eh['fp32']:.3e}), degrid err {ev:.3e}", flush=True)
+    print(f"[precision] wproject cf {prec} vis {visprec or prec}: grid err {eg[prec]:.3e} (fp32 {eg['fp32']:.3e}), "
+          f"image err (gain<=10) {eh[prec]:.3e} (fp32 {eh['fp32']:.3e}), degrid err {ev:.3e}", flush=True)
+    return eg, eh, ev
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_wproject_mixed_precision_error_vs_fp64(gpu_ctx, prec):
+    announce(f"wproject cf/vis precision {prec} vs fp64 oracle")
+    eg, eh, ev = _wproject_errors(gpu_ctx, prec)
     assert eg[prec] <= BOUND[prec] and eh[prec] <= BOUND[prec] and ev <= BOUND[prec]
     assert eg[prec] > 10 * eg["fp32"]  # the 16-bit mode is in effect
+
+
+@pytest.mark.xfail(strict=True, reason="FP16 CF taps (11-bit mantissa) give 2-5e-4 of the peak against the FP64 "
+                                        "oracle, above the 1e-4 target, with FP32 coefficients too")
+def test_fp16_reaches_1e4_of_peak(gpu_ctx):
+    announce("wproject fp16 taps, fp32 coefficients: the 1e-4 target")
+    eg, eh, ev = _wproject_errors(gpu_ctx, "fp16", visprec="fp32")
+    assert eg["fp16"] <= TARGET and eh["fp16"] <= TARGET and ev <= TARGET
 
 
 @pytest.mark.parametrize("prec", PRECS)
