@@ -82,6 +82,14 @@ typedef struct {
    * degridding (PolConverter as used at TableVisGridder.cc:517-524,682-693,860; no parallactic
    * angle rotation) */
   const int32_t *stokes;
+  /* dishPointing1: nrow x 2 (longitude, latitude; rad) or NULL (= pointing1): the representative
+   * field of a PSF pass is the first used row's feed and dish pointing (TableVisGridder.cc:581-593,
+   * 809-812) */
+  const double *dish_pointing1;
+  /* rows of the cube vis_out points into (its leading dimension), 0: nrow.  Direction-dependent
+   * calibration degrids source s into rows s*nrow .. (s+1)*nrow-1 of a cube of nsource*nrow rows
+   * (TableVisGridder.cc:567,861, itsSourceIndex): pass vis_out at row s*nrow and the cube's rows */
+  int vis_out_rows;
 } ygrid_chunk;
 
 /* Counters reported by TableVisGridder's destructor (TableVisGridder.cc:182-317). */
@@ -107,6 +115,9 @@ typedef struct {
   /* 1 when degridding runs on the matrix cores, four samples at once (2 or more polarisations;
    * environment YGRID_DEGRID_VARIANT="w,p,0" selects the VALU form) */
   int degrid_matrix;
+  /* rows left out because their pointing lies farther than MaxPointingSeparation from the image
+   * centre (itsRowsRejectedDueToMaxPointingSeparation, TableVisGridder.cc:571-578) */
+  double rows_rejected_pointing;
 } ygrid_stats;
 
 /* ---- library / context ---- */
@@ -126,9 +137,16 @@ int ygrid_ctx_clear_cache(ygrid_ctx *ctx);
 
 /* ---- factory: VisGridderFactory::make / createGridder (VisGridderFactory.cc:112-286) ----
  * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUAWProject".
- * parset: the gridder.<name>. subset as "key=value" lines (';' also separates), plus the
- * global keys padding, alldatapsf, oversampleweight, cleargrids. */
+ * parset: the gridder.<name>. subset as "key=value" lines (';' also separates); the global keys
+ * of ygrid_gridder_configure may be given here too. */
 int ygrid_gridder_create(ygrid_ctx *ctx, const char *name, const char *parset, ygrid_gridder **out);
+/* The options VisGridderFactory::make applies to a created gridder (VisGridderFactory.cc:137-232),
+ * as "key=value" text: padding (VisGridderWithPadding::setPaddingFactor), alldatapsf
+ * (TableVisGridder::useAllDataForPSF), oversampleweight (trackWeightPerPlane), cleargrids
+ * (doClearGrid), MaxPointingSeparation (maxPointingSeparation: radians, or a quantity such as
+ * "1.5deg"; negative: off), parotation (doPARotation: only false is supported).  Call before
+ * initialise_grid / initialise_degrid. */
+int ygrid_gridder_configure(ygrid_gridder *g, const char *options);
 /* IVisGridder::clone (IVisGridder.h:75; TableVisGridder.cc:126-168) */
 int ygrid_gridder_clone(const ygrid_gridder *g, ygrid_gridder **out);
 int ygrid_gridder_destroy(ygrid_gridder *g);

@@ -815,7 +815,15 @@ class _Params(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("npol_img", ctypes.c_int), ("nchan_img", ctypes.c_int),
                 ("uvcell_u", ctypes.c_double), ("uvcell_v", ctypes.c_double), ("oversample", ctypes.c_int),
                 ("mode", ctypes.c_int), ("track_per_os", ctypes.c_int), ("nsw_rows", ctypes.c_int),
-                ("use_all_psf", ctypes.c_int), ("stale_offgrid", ctypes.c_int)]
+                ("use_all_psf", ctypes.c_int), ("stale_offgrid", ctypes.c_int), ("max_sep", ctypes.c_double),
+                ("centre_lon", ctypes.c_double), ("centre_lat", ctypes.c_double)]
+
+
+class _Rows(ctypes.Structure):
+    """orc_rows: per-row pointings and the PSF state kept across calls."""
+    _fields_ = [("pointing1", ctypes.c_void_p), ("dish1", ctypes.c_void_p), ("psf_set", ctypes.c_int),
+                ("psf_feed", ctypes.c_int), ("psf_field", ctypes.c_int), ("psf_lon", ctypes.c_double),
+                ("psf_lat", ctypes.c_double), ("rows_rejected", ctypes.c_double)]
 
 
 class _Counters(ctypes.Structure):
@@ -836,6 +844,8 @@ class Chunk:
     flag: np.ndarray | None = None   # (nrow, nchan, npol) bool
     delay: np.ndarray | None = None  # (nrow,)
     feed1: np.ndarray | None = None  # (nrow,) int
+    pointing1: np.ndarray | None = None  # (nrow, 2) pointingDir1 (lon, lat) radians
+    dish1: np.ndarray | None = None      # (nrow, 2) dishPointing1, or None (= pointing1)
 
     @property
     def nrow(self):
@@ -872,6 +882,9 @@ class OracleGridder:
         self.aw_nchan = None     # AWProject: channels of the CF cache (set before initialise_grid)
         self.counters = _Counters()
         self.model_empty = False
+        self.max_sep = -1.0           # MaxPointingSeparation (radians; <= 0: off)
+        self.centre = (0.0, 0.0)      # image centre (lon, lat) for the pointing test
+        self.rows = _Rows(None, None, 0, -1, 0, 0.0, 0.0, 0.0)
 
     # ---- geometry ----
     def _setup(self, shape, cell):
@@ -941,6 +954,7 @@ class OracleGridder:
         self.sumw = np.zeros((self.nchan_img, self.npol, nsw), dtype=np.float64)
         self.freq_axis = freq_axis
         self.counters = _Counters()
+        self.rows = _Rows(None, None, 0, -1, 0, 0.0, 0.0, 0.0)
 
     def initialise_degrid(self, shape, cell, model, freq_axis=None):
         self._setup(shape, cell)
@@ -995,7 +1009,12 @@ class OracleGridder:
             ofs_u = np.array([o[0] for o in self.cf.offsets], dtype=np.int32)
             ofs_v = np.array([o[1] for o in self.cf.offsets], dtype=np.int32)
         p = _Params(self.nx, self.ny, self.npol, self.nchan_img, self.uvcell[0], self.uvcell[1], os_, mode,
-                    int(self.track_per_os), self.sumw.shape[2] if mode != 3 else 1, int(self.use_all_psf), 1)
+                    int(self.track_per_os), self.sumw.shape[2] if mode != 3 else 1, int(self.use_all_psf), 1,
+                    float(self.max_sep), float(self.centre[0]), float(self.centre[1]))
+        point = None if chunk.pointing1 is None else np.ascontiguousarray(chunk.pointing1, dtype=np.float64)
+        dish = None if chunk.dish1 is None else np.ascontiguousarray(chunk.dish1, dtype=np.float64)
+        self.rows.pointing1 = None if point is None else point.ctypes.data
+        self.rows.dish1 = None if dish is None else dish.ctypes.data
         vis = _cube_f(chunk.vis.astype(np.complex64))
         noise = _cube_f(chunk.noise.astype(np.complex64)) if chunk.noise is not None else None
         flag = _cube_f(chunk.flag.astype(np.uint8)) if chunk.flag is not None else None
@@ -1021,7 +1040,9 @@ class OracleGridder:
         cnt = _Counters()
         fn(ctypes.byref(p), nrow, nchan, npol, _p(uvw), _p(delay), _p(freq), _p(vis), _p(noise), _p(flag), _p(feed1),
            _p(cmap), _p(chanmap), _p(vw), _p(cf_data), _p(cf_off), _p(cf_supp), len(self.cf.planes), _p(ofs_u),
-           _p(ofs_v), _p(self.grid_), _p(sumw), _p(vout), ctypes.byref(cnt))
+           _p(ofs_v), _p(self.grid_), _p(sumw), _p(vout), ctypes.byref(cnt), ctypes.byref(self.rows))
+        self.rows.pointing1 = None
+        self.rows.dish1 = None
         if cnt.error:
             raise ValueError(f"oracle generic error {cnt.error}")
         for name, _ in _Counters._fields_[:-1]:

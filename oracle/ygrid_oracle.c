@@ -234,7 +234,30 @@ typedef struct {
   int nsw_rows;        /* first dimension of itsSumWeights */
   int use_all_psf;     /* itsUseAllDataForPSF */
   int stale_offgrid;   /* degrid: add the previous on-grid value for off-grid samples (reference) */
+  double max_sep;      /* itsMaxPointingSeparation (radians; <= 0: off) */
+  double centre_lon, centre_lat; /* image centre (getImageCentre) */
 } orc_params;
+
+/* Per-row pointings and the PSF state that TableVisGridder keeps across calls
+ * (itsFirstGriddedVis, itsFeedUsedForPSF, itsPointingUsedForPSF, TableVisGridder.cc:581-593)
+ * and its count of rows rejected by the pointing test (:571-578). */
+typedef struct {
+  const double *pointing1; /* pointingDir1: nrow x 2 (lon, lat), or NULL (= the image centre) */
+  const double *dish1;     /* dishPointing1: nrow x 2, or NULL (= pointing1) */
+  int psf_set;             /* the representative feed and field are chosen */
+  int psf_feed, psf_field; /* psf_field: psf_lon/psf_lat hold the representative dish pointing */
+  double psf_lon, psf_lat;
+  double rows_rejected;
+} orc_rows;
+
+/* MVDirection::separation: the great-circle angle between two directions */
+static double orc_separation(double l1, double b1, double l2, double b2) {
+  const double dl = l2 - l1;
+  const double a = cos(b2) * sin(dl);
+  const double b = cos(b1) * sin(b2) - sin(b1) * cos(b2) * cos(dl);
+  const double c = sin(b1) * sin(b2) + cos(b1) * cos(b2) * cos(dl);
+  return atan2(sqrt(a * a + b * b), c);
+}
 
 typedef struct {
   double samples_gridded, taps_gridded, samples_degridded, taps_degridded;
@@ -253,11 +276,22 @@ typedef struct {
   const int os = p->oversample;                                                                                    \
   const int isPSF = p->mode == ORC_PSF, isPCF = p->mode == ORC_PCF, forward = p->mode == ORC_DEGRID;               \
   REAL stale_r[4] = {0, 0, 0, 0}, stale_i[4] = {0, 0, 0, 0};                                                       \
-  int first_psf = 1, psf_feed = -1;                                                                                \
+  orc_rows local_rows = {NULL, NULL, 0, -1, 0, 0.0, 0.0, 0.0};                                                      \
+  orc_rows *rw = rows ? rows : &local_rows;                                                                        \
+  const double *dish = rw->dish1 ? rw->dish1 : rw->pointing1;                                                      \
   for (int i = 0; i < nrow; ++i) {                                                                                 \
-    if (first_psf && isPSF) {                                                                                      \
-      if (!p->use_all_psf) psf_feed = feed1 ? feed1[i] : 0;                                                         \
-      first_psf = 0;                                                                                               \
+    if (p->max_sep > 0. && rw->pointing1 &&                                                                        \
+        orc_separation(p->centre_lon, p->centre_lat, rw->pointing1[2 * i], rw->pointing1[2 * i + 1]) > p->max_sep) { \
+      rw->rows_rejected += 1;                                                                                      \
+      continue;                                                                                                    \
+    }                                                                                                              \
+    if (!rw->psf_set && isPSF) {                                                                                   \
+      if (!p->use_all_psf) {                                                                                       \
+        rw->psf_feed = feed1 ? feed1[i] : 0;                                                                       \
+        rw->psf_field = dish != NULL;                                                                              \
+        if (dish) { rw->psf_lon = dish[2 * i]; rw->psf_lat = dish[2 * i + 1]; }                                    \
+      }                                                                                                            \
+      rw->psf_set = 1;                                                                                             \
     }                                                                                                              \
     for (int chan = 0; chan < nchan; ++chan) {                                                                     \
       const double f = freq[chan];                                                                                 \
@@ -339,7 +373,10 @@ typedef struct {
             if (isPSF && visweight) rr *= (REAL)visweight[chan];                                                   \
           }                                                                                                        \
           int doIt = 1;                                                                                            \
-          if (isPSF && !p->use_all_psf) doIt = (feed1 ? feed1[i] : 0) == psf_feed;                                 \
+          if (isPSF && !p->use_all_psf)                                                                            \
+            doIt = (feed1 ? feed1[i] : 0) == rw->psf_feed &&                                                       \
+                   (!rw->psf_field ||                                                                              \
+                    orc_separation(rw->psf_lon, rw->psf_lat, dish[2 * i], dish[2 * i + 1]) < 1e-6);                 \
           if (!doIt) continue;                                                                                     \
           int conjCF = 0;                                                                                          \
           if (isPCF) conjCF = (ivo < p->ny / 2 && iuo >= p->nx / 2) || (ivo <= p->ny / 2 && iuo < p->nx / 2);       \
@@ -379,7 +416,7 @@ void orc_generic_f32(const orc_params *p, int nrow, int nchan, int npol, const d
                      const double *freq, const float *vis, const float *noise, const uint8_t *flag, const int *feed1,
                      const int *cmap, const int *chanmap, const float *visweight, const float *cf_data,
                      const int64_t *cf_off, const int *cf_supp, int ncf, const int *cf_ofs_u, const int *cf_ofs_v,
-                     float *grid, double *sumw, float *vis_out, orc_counters *cnt) {
+                     float *grid, double *sumw, float *vis_out, orc_counters *cnt, orc_rows *rows) {
 #define GRID_T float
 #define CF_T float
 #define VIS_T float
@@ -394,7 +431,7 @@ void orc_generic_f64(const orc_params *p, int nrow, int nchan, int npol, const d
                      const double *freq, const float *vis, const float *noise, const uint8_t *flag, const int *feed1,
                      const int *cmap, const int *chanmap, const float *visweight, const double *cf_data,
                      const int64_t *cf_off, const int *cf_supp, int ncf, const int *cf_ofs_u, const int *cf_ofs_v,
-                     double *grid, double *sumw, double *vis_out, orc_counters *cnt) {
+                     double *grid, double *sumw, double *vis_out, orc_counters *cnt, orc_rows *rows) {
 #define GRID_T double
 #define CF_T double
 #define VIS_T double

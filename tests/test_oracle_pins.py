@@ -733,3 +733,60 @@ def test_wstack_degrid_predicts_the_measurement_equation():
     for i, (ku, kv, iw) in enumerate(ks):
         e = F * np.exp(2j * np.pi * (ku * uvcell * l0 + kv * uvcell * m0 + ws.wterm(iw) * (n0 - 1)))
         assert abs(pv[i, 0, 0] - e) < 2e-3 * F
+
+
+# ---- row selection: MaxPointingSeparation and the representative PSF feed / field -------------
+def _two_field_chunk(seed=3, nrow=200):
+    rng = np.random.default_rng(seed)
+    N, cell, f = 128, math.radians(20 / 3600), np.array([1.4e9, 1.41e9])
+    uvw = _disc(nrow, N, cell, f[0], rng)
+    vis = (rng.normal(size=(nrow, 2, 1)) + 1j * rng.normal(size=(nrow, 2, 1))).astype(np.complex64)
+    centre = (1.0, -0.5)
+    far = rng.random(nrow) < 0.4                       # rows pointing 2 deg away
+    point = np.tile(np.array(centre), (nrow, 1))
+    point[far, 0] += math.radians(2.0) / math.cos(centre[1])
+    feed = (rng.random(nrow) < 0.5).astype(np.int32)
+    return N, cell, f, uvw, vis, centre, far, point, feed
+
+
+def test_max_pointing_separation_rejects_whole_rows():
+    """TableVisGridder.cc:571-578: a row whose pointing lies farther than the threshold from the
+    image centre is skipped before anything else; gridding the chunk equals gridding the kept
+    rows alone, and the rejected rows are counted once each."""
+    N, cell, f, uvw, vis, centre, far, point, feed = _two_field_chunk()
+    g = O.OracleGridder("SphFunc", "f64", sph_oversample=16)
+    g.max_sep, g.centre = math.radians(1.0), centre
+    g.initialise_grid((N, N, 1, 1), (cell, cell))
+    g.grid(O.Chunk(uvw, f, vis, pointing1=point))
+    keep = ~far
+    h = O.OracleGridder("SphFunc", "f64", sph_oversample=16)
+    h.initialise_grid((N, N, 1, 1), (cell, cell))
+    h.grid(O.Chunk(np.ascontiguousarray(uvw[keep]), f, np.ascontiguousarray(vis[keep])))
+    assert np.array_equal(g.grid_, h.grid_)
+    assert g.rows.rows_rejected == np.count_nonzero(far)
+    assert g.counters.samples_gridded == h.counters.samples_gridded
+    # a threshold beyond the offset keeps every row
+    g2 = O.OracleGridder("SphFunc", "f64", sph_oversample=16)
+    g2.max_sep, g2.centre = math.radians(3.0), centre
+    g2.initialise_grid((N, N, 1, 1), (cell, cell))
+    g2.grid(O.Chunk(uvw, f, vis, pointing1=point))
+    assert g2.rows.rows_rejected == 0
+
+
+def test_psf_uses_the_first_rows_feed_and_field_across_calls():
+    """TableVisGridder.cc:581-593,809-812: without alldatapsf the PSF grids only the rows with the
+    feed and the dish pointing of the first row it reaches, and that choice persists over the
+    following calls until the next initialiseGrid."""
+    N, cell, f, uvw, vis, centre, far, point, feed = _two_field_chunk(seed=4)
+    n = uvw.shape[0] // 2
+    g = O.OracleGridder("SphFunc", "f64", sph_oversample=16, use_all_psf=False)
+    g.initialise_grid((N, N, 1, 1), (cell, cell), dopsf=True)
+    for sl in (slice(0, n), slice(n, None)):
+        g.grid(O.Chunk(np.ascontiguousarray(uvw[sl]), f, np.ascontiguousarray(vis[sl]), feed1=feed[sl],
+                       pointing1=np.ascontiguousarray(point[sl])))
+    sel = (feed == feed[0]) & (far == far[0])
+    h = O.OracleGridder("SphFunc", "f64", sph_oversample=16, use_all_psf=True)
+    h.initialise_grid((N, N, 1, 1), (cell, cell), dopsf=True)
+    h.grid(O.Chunk(np.ascontiguousarray(uvw[sel]), f, np.ascontiguousarray(vis[sel])))
+    assert 0 < np.count_nonzero(sel) < sel.size
+    assert np.array_equal(g.grid_, h.grid_)

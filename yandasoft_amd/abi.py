@@ -13,7 +13,8 @@ YGRID_MEM_DEVICE = 1
 
 EXPORTED = [
     "ygrid_last_error", "ygrid_version", "ygrid_ctx_create", "ygrid_ctx_destroy", "ygrid_ctx_set_stream",
-    "ygrid_ctx_synchronize", "ygrid_ctx_clear_cache", "ygrid_gridder_create", "ygrid_gridder_clone", "ygrid_gridder_destroy",
+    "ygrid_ctx_synchronize", "ygrid_ctx_clear_cache", "ygrid_gridder_create", "ygrid_gridder_configure",
+    "ygrid_gridder_clone", "ygrid_gridder_destroy",
     "ygrid_initialise_grid", "ygrid_grid", "ygrid_finalise_grid", "ygrid_finalise_weights",
     "ygrid_initialise_degrid", "ygrid_degrid", "ygrid_degrid_noalloc", "ygrid_finalise_degrid", "ygrid_is_model_empty",
     "ygrid_set_mfs_weight", "ygrid_get_stats", "ygrid_set_profiling", "ygrid_get_cf", "ygrid_get_grid",
@@ -43,7 +44,7 @@ class _Chunk(ctypes.Structure):
                 ("uvw", ctypes.c_void_p), ("delay", ctypes.c_void_p), ("freq", ctypes.c_void_p),
                 ("vis", ctypes.c_void_p), ("vis_out", ctypes.c_void_p), ("noise", ctypes.c_void_p),
                 ("flag", ctypes.c_void_p), ("feed1", ctypes.c_void_p), ("pointing1", ctypes.c_void_p),
-                ("stokes", ctypes.c_void_p)]
+                ("stokes", ctypes.c_void_p), ("dish_pointing1", ctypes.c_void_p), ("vis_out_rows", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -58,7 +59,7 @@ class Stats(ctypes.Structure):
                 ("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int),
                 ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int),
                 ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int), ("learned_cf_active", ctypes.c_int),
-                ("degrid_matrix", ctypes.c_int)]
+                ("degrid_matrix", ctypes.c_int), ("rows_rejected_pointing", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):
@@ -149,6 +150,8 @@ class Chunk:
     feed1: object = None
     pointing1: object = None
     stokes: tuple | None = None  # casacore Stokes codes of the visibility pols (None: the image's)
+    dish_pointing1: object = None  # (nrow, 2) radians or None (= pointing1)
+    vis_out_rows: int = 0          # rows of the cube vis_out points into (0: nrow)
     nrow: int = 0
     nchan: int = 0
     npol: int = 0
@@ -160,7 +163,8 @@ class Chunk:
         cube = self.vis if self.vis is not None else (self.vis_out if self.vis_out is not None else self.noise)
         self.npol = int(cube.shape[0]) if cube is not None else self.npol
         if not self.device:
-            for name in ("uvw", "freq", "vis", "vis_out", "noise", "flag", "delay", "feed1", "pointing1"):
+            for name in ("uvw", "freq", "vis", "vis_out", "noise", "flag", "delay", "feed1", "pointing1",
+                         "dish_pointing1"):
                 a = getattr(self, name)
                 if a is not None:
                     assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], f"{name} must be C-contiguous"
@@ -170,7 +174,8 @@ class Chunk:
         return _Chunk(self.nrow, self.nchan, self.npol, YGRID_MEM_DEVICE if self.device else YGRID_MEM_HOST,
                       _ptr(self.uvw), _ptr(self.delay), _ptr(self.freq), _ptr(self.vis), _ptr(self.vis_out),
                       _ptr(self.noise), _ptr(self.flag), _ptr(self.feed1), _ptr(self.pointing1),
-                      None if self._stokes is None else self._stokes.ctypes.data)
+                      None if self._stokes is None else self._stokes.ctypes.data, _ptr(self.dish_pointing1),
+                      int(self.vis_out_rows))
 
 
 class Context:
@@ -217,6 +222,12 @@ class Gridder:
             self.h = ctypes.c_void_p()
             _check(lib().ygrid_gridder_create(ctx.h, name.encode(), parset_text(params), ctypes.byref(self.h)))
         self.img = None
+
+    def configure(self, options: dict):
+        """ygrid_gridder_configure: the factory's global options (padding, alldatapsf,
+        oversampleweight, cleargrids, MaxPointingSeparation, parotation)."""
+        text = "\n".join(f"{k}={v}" for k, v in options.items()).encode()
+        _check(lib().ygrid_gridder_configure(self.h, text))
 
     def clone(self) -> "Gridder":
         h = ctypes.c_void_p()

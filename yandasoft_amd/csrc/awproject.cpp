@@ -10,40 +10,6 @@
 
 namespace ygrid {
 
-namespace {
-
-// SynthesisParamsHelper::convertQuantity for the units the A-projection keys use.
-double quantity(const std::string& s, const std::string& unit) {
-  size_t i = 0;
-  double v = 0;
-  try { v = std::stod(s, &i); } catch (...) { throw Error(1, "bad quantity " + s); }
-  std::string u = s.substr(i);
-  u.erase(0, u.find_first_not_of(" \t"));
-  if (u.empty() || u == unit) return v;
-  if (unit == "rad") {
-    if (u == "deg") return v * kPi / 180.;
-    if (u == "arcmin") return v * kPi / 180. / 60.;
-    if (u == "arcsec") return v * kPi / 180. / 3600.;
-  }
-  if (unit == "m") {
-    if (u == "cm") return v * 0.01;
-    if (u == "mm") return v * 1e-3;
-    if (u == "km") return v * 1e3;
-  }
-  throw Error(1, "cannot convert " + s + " to " + unit);
-}
-
-// MVDirection::separation (great-circle angle, numerically robust form)
-double separation(double l1, double b1, double l2, double b2) {
-  const double dl = l2 - l1;
-  const double a = std::cos(b2) * std::sin(dl);
-  const double b = std::cos(b1) * std::sin(b2) - std::sin(b1) * std::cos(b2) * std::cos(dl);
-  const double c = std::sin(b1) * std::sin(b2) + std::cos(b1) * std::cos(b2) * std::cos(dl);
-  return std::atan2(std::sqrt(a * a + b * b), c);
-}
-
-}  // namespace
-
 class AWProjectGridder : public Gridder {
  public:
   AWProjectGridder(std::shared_ptr<Ctx> c, const Params& p) : Gridder(std::move(c)) {

@@ -123,7 +123,7 @@ void Batch::enqueue() {
   for (Pass& p : passes_) {
     if (!p.active || !p.forward) continue;
     const ChunkDev& d = p.ps.d;
-    launch_degrid_finish(c, p.ps.out.smp, p.ps.acc, d.nrow, d.nchan, d.npol, d.vis_out);
+    launch_degrid_finish(c, p.ps.out.smp, p.ps.acc, d.nrow, d.nchan, d.npol, d.vis_out, d.vis_out_ld);
   }
 }
 
@@ -160,7 +160,7 @@ void Batch::run() {
   for (const Pass& p : passes_) nact += p.active ? 1 : 0;
   for (Pass& p : passes_) {
     if (!p.active) continue;
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     int err = 0;
     YG_HIP(hipMemcpy(cnt, p.ps.out.counters, sizeof(cnt), hipMemcpyDeviceToHost));
     YG_HIP(hipMemcpy(&err, p.ps.out.error, sizeof(int), hipMemcpyDeviceToHost));

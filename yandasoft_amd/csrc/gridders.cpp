@@ -43,6 +43,34 @@ ParamMap parse_parset(const char* text) {
   return m;
 }
 
+double quantity(const std::string& s, const std::string& unit) {
+  size_t i = 0;
+  double v = 0;
+  try { v = std::stod(s, &i); } catch (...) { throw Error(1, "bad quantity " + s); }
+  std::string u = s.substr(i);
+  u.erase(0, u.find_first_not_of(" \t"));
+  if (u.empty() || u == unit) return v;
+  if (unit == "rad") {
+    if (u == "deg") return v * kPi / 180.;
+    if (u == "arcmin") return v * kPi / 180. / 60.;
+    if (u == "arcsec") return v * kPi / 180. / 3600.;
+  }
+  if (unit == "m") {
+    if (u == "cm") return v * 0.01;
+    if (u == "mm") return v * 1e-3;
+    if (u == "km") return v * 1e3;
+  }
+  throw Error(1, "cannot convert " + s + " to " + unit);
+}
+
+double separation(double l1, double b1, double l2, double b2) {
+  const double dl = l2 - l1;
+  const double a = std::cos(b2) * std::sin(dl);
+  const double b = std::cos(b1) * std::sin(b2) - std::sin(b1) * std::cos(b2) * std::cos(dl);
+  const double c = std::sin(b1) * std::sin(b2) + std::cos(b1) * std::cos(b2) * std::cos(dl);
+  return std::atan2(std::sqrt(a * a + b * b), c);
+}
+
 std::string Params::str(const std::string& k, const std::string& d) const {
   auto it = m.find(k);
   return it == m.end() ? d : it->second;
@@ -108,8 +136,8 @@ static void layout_table(CFTable& t, const std::vector<int>& supp_z, int os) {
 // ---------------------------------------------------------------------------------------
 Gridder::Gridder(const Gridder& o)
     : ctx_(o.ctx_), cf_(o.cf_), padding_(o.padding_), track_per_os_(o.track_per_os_), use_all_psf_(o.use_all_psf_),
-      clear_grid_(o.clear_grid_), vw_ref_(o.vw_ref_), vw_order_(o.vw_order_), alpha_int_(o.alpha_int_),
-      cf_prec_(o.cf_prec_), vis_prec_(o.vis_prec_) {
+      clear_grid_(o.clear_grid_), max_sep_(o.max_sep_), vw_ref_(o.vw_ref_), vw_order_(o.vw_order_),
+      alpha_int_(o.alpha_int_), cf_prec_(o.cf_prec_), vis_prec_(o.vis_prec_) {
   std::memcpy(shape_, o.shape_, sizeof(shape_));
   std::memcpy(img_, o.img_, sizeof(img_));
   uvcell_[0] = o.uvcell_[0];
@@ -128,13 +156,21 @@ void Gridder::setProfiling(bool on) {
       if (!e) YG_HIP(hipEventCreate(&e));
 }
 
-// Global gridder options (VisGridderFactory.cc:137-232).
-void Gridder::configure_common(const Params& p) {
-  padding_ = (float)p.f64("padding", 1.0);
+// Global gridder options (VisGridderFactory.cc:137-232); keys a parset does not give keep their
+// values.
+void Gridder::configure_globals(const Params& p) {
+  padding_ = (float)p.f64("padding", padding_);
   YG_CHECK(padding_ >= 1.f, 1, "padding must be >= 1");
-  use_all_psf_ = p.flag("alldatapsf", false);
-  track_per_os_ = p.flag("oversampleweight", false);
-  clear_grid_ = p.flag("cleargrids", false);
+  use_all_psf_ = p.flag("alldatapsf", use_all_psf_);
+  track_per_os_ = p.flag("oversampleweight", track_per_os_);
+  clear_grid_ = p.flag("cleargrids", clear_grid_);
+  if (p.has("MaxPointingSeparation")) max_sep_ = quantity(p.str("MaxPointingSeparation", "-1rad"), "rad");
+  YG_CHECK(!p.flag("parotation", false), 1,
+           "parallactic angle rotation (gridder.parotation) is not supported by the GPU gridders");
+}
+
+void Gridder::configure_common(const Params& p) {
+  configure_globals(p);
   // Storage precision of the CF taps and of the gridding coefficients (weighted visibilities);
   // no counterpart in the reference, whose tables and accessors are FP32 throughout.
   cf_prec_ = parse_precision(p.str("cfprecision", "fp32"), "cfprecision");
@@ -250,6 +286,8 @@ void Gridder::stage(const ygrid_chunk& a, ChunkDev& d, bool forward) {
       YG_HIP(hipMemcpy(d.point_h.data(), a.pointing1, nr * 2 * sizeof(double), k));
     }
   }
+  d.vis_out_ld = a.vis_out_rows > 0 ? a.vis_out_rows : a.nrow;
+  YG_CHECK(d.vis_out_ld >= a.nrow, 1, "vis_out_rows must be 0 or at least nrow");
   if (a.mem == YGRID_MEM_DEVICE) {
     d.uvw = a.uvw;
     d.delay = a.delay;
@@ -259,6 +297,8 @@ void Gridder::stage(const ygrid_chunk& a, ChunkDev& d, bool forward) {
     d.noise = reinterpret_cast<const float2*>(a.noise);
     d.flag = a.flag;
     d.feed1 = a.feed1;
+    d.pointing1 = a.pointing1;
+    d.dish1 = a.dish_pointing1;
     YG_HIP(hipMemcpyAsync(d.freq_host.data(), a.freq, nc * sizeof(double), hipMemcpyDeviceToHost, ctx_->stream));
     d.feed0 = 0;
     if (a.feed1 && a.nrow > 0)
@@ -272,8 +312,13 @@ void Gridder::stage(const ygrid_chunk& a, ChunkDev& d, bool forward) {
   // one staging allocation, 256-byte aligned sub-buffers
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const bool want_vis = !forward && a.vis && !(dopsf_ || dopcf_);
-  size_t bytes = al(nr * 3 * 8) + al(nr * 8) + al(nc * 8) + (want_vis ? al(ncube * 8) : 0) + (forward ? al(ncube * 8) : 0) +
-                 (a.noise ? al(ncube * 8) : 0) + (a.flag ? al(ncube) : 0) + (a.feed1 ? al(nr * 4) : 0);
+  // the pointings are read by the pointing test and by a PSF pass's field test
+  const bool want_point = max_sep_ > 0 || (dopsf_ && !use_all_psf_);
+  // degridding: the span of the (ld-row) cube this chunk's rows occupy
+  const size_t vspan = ncube ? nr + (size_t)d.vis_out_ld * (nc * np - 1) : 0;
+  size_t bytes = al(nr * 3 * 8) + al(nr * 8) + al(nc * 8) + (want_vis ? al(ncube * 8) : 0) + (forward ? al(vspan * 8) : 0) +
+                 (a.noise ? al(ncube * 8) : 0) + (a.flag ? al(ncube) : 0) + (a.feed1 ? al(nr * 4) : 0) +
+                 (want_point && a.pointing1 ? al(nr * 16) : 0) + (want_point && a.dish_pointing1 ? al(nr * 16) : 0);
   staging_.resize(bytes);
   char* p = staging_.as<char>();
   auto put = [&](const void* src, size_t b) -> void* {
@@ -287,10 +332,41 @@ void Gridder::stage(const ygrid_chunk& a, ChunkDev& d, bool forward) {
   if (!a.delay) p += al(nr * 8);
   d.freq = static_cast<const double*>(put(a.freq, nc * 8));
   d.vis = want_vis ? static_cast<const float2*>(put(a.vis, ncube * 8)) : nullptr;
-  d.vis_out = forward ? static_cast<float2*>(put(a.vis_out, ncube * 8)) : nullptr;
+  d.vis_out = forward ? static_cast<float2*>(put(a.vis_out, vspan * 8)) : nullptr;
   d.noise = a.noise ? static_cast<const float2*>(put(a.noise, ncube * 8)) : nullptr;
   d.flag = a.flag ? static_cast<const uint8_t*>(put(a.flag, ncube)) : nullptr;
   d.feed1 = a.feed1 ? static_cast<const int32_t*>(put(a.feed1, nr * 4)) : nullptr;
+  d.pointing1 = want_point && a.pointing1 ? static_cast<const double*>(put(a.pointing1, nr * 16)) : nullptr;
+  d.dish1 = want_point && a.dish_pointing1 ? static_cast<const double*>(put(a.dish_pointing1, nr * 16)) : nullptr;
+}
+
+bool Gridder::choosePSFField(const ygrid_chunk& a) {
+  const size_t nr = a.nrow;
+  if (nr == 0) return false;
+  const hipMemcpyKind k = a.mem == YGRID_MEM_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+  std::vector<double> point;
+  if (a.pointing1) {
+    point.resize(nr * 2);
+    YG_HIP(hipMemcpy(point.data(), a.pointing1, nr * 2 * sizeof(double), k));
+  }
+  size_t i0 = 0;
+  if (max_sep_ > 0 && !point.empty()) {
+    while (i0 < nr && separation(centre_[0], centre_[1], point[2 * i0], point[2 * i0 + 1]) > max_sep_) ++i0;
+    if (i0 == nr) return false;
+  }
+  int32_t feed = 0;
+  if (a.feed1) YG_HIP(hipMemcpy(&feed, a.feed1 + i0, sizeof(int32_t), k));
+  psf_feed_ = feed;
+  psf_field_ = 0;
+  if (a.dish_pointing1) {
+    YG_HIP(hipMemcpy(psf_dir_, a.dish_pointing1 + 2 * i0, 2 * sizeof(double), k));
+    psf_field_ = 1;
+  } else if (!point.empty()) {
+    psf_dir_[0] = point[2 * i0];
+    psf_dir_[1] = point[2 * i0 + 1];
+    psf_field_ = 1;
+  }
+  return true;
 }
 
 // TableVisGridder::generic (TableVisGridder.cc:451-884)
@@ -347,8 +423,14 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps, Pass
   // the copies read host vectors of this frame: complete them before it returns
   if (arena) YG_HIP(hipStreamSynchronize(ctx_->stream));
   if (!forward && dopsf_ && first_psf_) {
-    psf_feed_ = use_all_psf_ ? -1 : d.feed0;
-    first_psf_ = false;
+    // the first row the pointing test keeps fixes the representative feed and field
+    // (TableVisGridder.cc:581-593), once per initialiseGrid
+    if (use_all_psf_) {
+      psf_feed_ = -1;
+      first_psf_ = false;
+    } else if (choosePSFField(a)) {
+      first_psf_ = false;
+    }
   }
   const auto tcf = std::chrono::steady_clock::now();
   // with an arena the caller has checked that the CF cache exists and holds every plane
@@ -396,6 +478,12 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps, Pass
   pp.track_per_os = track_per_os_ ? 1 : 0;
   pp.nsw_rows = nsw_rows_;
   pp.psf_feed = psf_feed_;
+  pp.psf_field = psf_field_;
+  pp.psf_lon = psf_dir_[0];
+  pp.psf_lat = psf_dir_[1];
+  pp.max_sep = max_sep_;
+  pp.centre_lon = centre_[0];
+  pp.centre_lat = centre_[1];
   pp.ci = cindex(d);
   const int ntiles = ((shape_[0] + kTile - 1) / kTile) * ((shape_[1] + kTile - 1) / kTile);
   const long long budget = 1ll << 24;
@@ -420,6 +508,8 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps, Pass
   in.noise = d.noise;
   in.flag = d.flag;
   in.feed1 = d.feed1;
+  in.pointing1 = d.pointing1;
+  in.dish1 = d.dish1;
   in.chanmap = dchan;
   in.visweight = vw;
   in.cf_off = cf_->d_off.as<int32_t>();
@@ -439,7 +529,7 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps, Pass
 }
 
 // Error flags and counters of a finished pass (TableVisGridder.cc:182-317 statistics).
-void Gridder::account_pass(bool forward, const unsigned long long cnt[4], int err, double seconds) {
+void Gridder::account_pass(bool forward, const unsigned long long cnt[5], int err, double seconds) {
   YG_CHECK(!(err & 1), 4, "Weight is supposed to be a positive number (zero noise)");
   YG_CHECK(!(err & 2), 4, "W scaling error: recommend allowing larger range of w");
   YG_CHECK(!(err & 4), 4, "Index into convolution functions exceeds number of planes");
@@ -454,6 +544,7 @@ void Gridder::account_pass(bool forward, const unsigned long long cnt[4], int er
     t_grid_ += seconds;
   }
   st_wflag_ += (double)cnt[3];
+  st_rejected_ += (double)cnt[4];
 }
 
 void Gridder::generic(const ygrid_chunk& a, bool forward) {
@@ -478,20 +569,20 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
     launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, ps.acc,
                   nGridPlanes() > 1 ? gridPlaneElems() : 0);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
-    launch_degrid_finish(*ctx_, smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out);
+    launch_degrid_finish(*ctx_, smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out, d.vis_out_ld);
   } else {
     if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
     launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg,
                 nGridPlanes() > 1 ? gridPlaneElems() : 0);
     if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
   }
-  unsigned long long cnt[4] = {0, 0, 0, 0};
+  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
   int err = 0;
   YG_HIP(hipMemcpyAsync(cnt, ps.out.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx_->stream));
   YG_HIP(hipMemcpyAsync(&err, ps.out.error, sizeof(int), hipMemcpyDeviceToHost, ctx_->stream));
-  if (forward && a.mem == YGRID_MEM_HOST) {
-    YG_HIP(hipMemcpyAsync(a.vis_out, d.vis_out, (size_t)nsamp * d.npol * sizeof(float2), hipMemcpyDeviceToHost,
-                          ctx_->stream));
+  if (forward && a.mem == YGRID_MEM_HOST && nsamp) {
+    const size_t span = (size_t)d.nrow + (size_t)d.vis_out_ld * ((size_t)d.nchan * d.npol - 1);
+    YG_HIP(hipMemcpyAsync(a.vis_out, d.vis_out, span * sizeof(float2), hipMemcpyDeviceToHost, ctx_->stream));
   }
   YG_HIP(hipStreamSynchronize(ctx_->stream));
   if (profiling_) {
@@ -647,6 +738,7 @@ void Gridder::stats(ygrid_stats* o) const {
   o->vis_precision = vis_prec_;
   o->learned_cf_active = learnedActive() ? 1 : 0;
   o->degrid_matrix = ctx_->degrid_variant.matrix && shape_[2] >= 2 ? 1 : 0;
+  o->rows_rejected_pointing = st_rejected_;
 }
 
 void Gridder::getCF(int k, float* out, int* support) const {

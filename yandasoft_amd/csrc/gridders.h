@@ -17,6 +17,11 @@ namespace ygrid {
 
 using ParamMap = std::map<std::string, std::string>;
 ParamMap parse_parset(const char* text);
+// SynthesisParamsHelper::convertQuantity for angle (rad, deg, arcmin, arcsec) and length (m, cm,
+// mm, km) units; a bare number is taken in `unit`
+double quantity(const std::string& s, const std::string& unit);
+// MVDirection::separation: great-circle angle between two (longitude, latitude) directions
+double separation(double l1, double b1, double l2, double b2);
 
 struct Params {
   const ParamMap& m;
@@ -54,6 +59,9 @@ struct ChunkDev {
   const float2* noise = nullptr;
   const uint8_t* flag = nullptr;
   const int32_t* feed1 = nullptr;
+  const double* pointing1 = nullptr;  // nrow x 2 (lon, lat) or null
+  const double* dish1 = nullptr;      // nrow x 2 or null
+  int vis_out_ld = 0;                 // rows of the cube vis_out points into
   std::vector<double> freq_host;
   int feed0 = 0;
   // host copies of per-row metadata for gridders whose CF cache depends on it (AWProject)
@@ -96,6 +104,8 @@ class Gridder {
   virtual std::string name() const = 0;
 
   void configure_common(const Params& p);
+  // the options VisGridderFactory::make sets on a created gridder (VisGridderFactory.cc:137-232)
+  void configure_globals(const Params& p);
   void initialiseGrid(const ygrid_image_desc& img, bool dopsf, bool dopcf);
   void grid(const ygrid_chunk& acc);
   void finaliseGrid(float* out, int mem);
@@ -174,7 +184,8 @@ class Gridder {
   // arena: carve the pass's buffers from it (the allocation-free entry point); null: the
   // gridder's own buffers, grown as needed
   bool setup_pass(const ygrid_chunk& acc, bool forward, PassSetup& ps, PassArena* arena = nullptr);
-  void account_pass(bool forward, const unsigned long long cnt[4], int err, double seconds);
+  // cnt: samples, taps, flagged, w-flagged, rows rejected by pointing
+  void account_pass(bool forward, const unsigned long long cnt[5], int err, double seconds);
   // hooks of a pass: after its setup (may replace the CF view and prep arguments), and after the
   // prep kernel (device work only: also runs inside captured graphs)
   virtual void customizePass(PassSetup& ps) { (void)ps; }
@@ -184,6 +195,9 @@ class Gridder {
   int npol() const { return shape_[2]; }
  protected:
   void stage(const ygrid_chunk& acc, ChunkDev& d, bool forward);
+  // the representative feed and field of a PSF pass: those of the first row the pointing test
+  // keeps (TableVisGridder.cc:571-593); false when the chunk has no such row
+  bool choosePSFField(const ygrid_chunk& acc);
   void uploadCorrection();
 
   std::shared_ptr<Ctx> ctx_;
@@ -197,6 +211,10 @@ class Gridder {
   bool track_per_os_ = false, use_all_psf_ = false, clear_grid_ = false;
   bool first_psf_ = true;
   int psf_feed_ = -1;
+  int psf_field_ = 0;           // the representative field's pointing is known (psf_dir_)
+  double psf_dir_[2] = {0, 0};  // itsPointingUsedForPSF (dish pointing of the first used row)
+  double max_sep_ = -1.0;       // itsMaxPointingSeparation (radians; <= 0: every row is used)
+  double st_rejected_ = 0;      // itsRowsRejectedDueToMaxPointingSeparation
   bool has_freq_axis_ = false;
   double fstart_ = 0, fend_ = 0;
   double centre_[2] = {0, 0};  // image centre (lon, lat)

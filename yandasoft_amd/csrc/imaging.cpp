@@ -23,7 +23,7 @@ DeviceChunk::DeviceChunk(const Ctx& ctx, const ygrid_chunk& a) : c(a) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t sizes[] = {a.uvw ? nr * 24 : 0,       a.delay ? nr * 8 : 0,      a.freq ? nc * 8 : 0,
                           a.vis ? ncube * 8 : 0,     a.noise ? ncube * 8 : 0,   a.flag ? ncube : 0,
-                          a.feed1 ? nr * 4 : 0,      a.pointing1 ? nr * 16 : 0};
+                          a.feed1 ? nr * 4 : 0,      a.pointing1 ? nr * 16 : 0, a.dish_pointing1 ? nr * 16 : 0};
   size_t bytes = 0;
   for (size_t b : sizes) bytes += al(b);
   store.resize(std::max<size_t>(bytes, 256));
@@ -43,7 +43,9 @@ DeviceChunk::DeviceChunk(const Ctx& ctx, const ygrid_chunk& a) : c(a) {
   c.flag = static_cast<const uint8_t*>(put(a.flag, sizes[5]));
   c.feed1 = static_cast<const int32_t*>(put(a.feed1, sizes[6]));
   c.pointing1 = static_cast<const double*>(put(a.pointing1, sizes[7]));
+  c.dish_pointing1 = static_cast<const double*>(put(a.dish_pointing1, sizes[8]));
   c.vis_out = nullptr;
+  c.vis_out_rows = 0;
   c.mem = YGRID_MEM_DEVICE;
   // the host arrays may be reused by the caller as soon as this returns
   YG_HIP(hipStreamSynchronize(ctx.stream));
@@ -63,6 +65,7 @@ void imaging_chunk(const std::vector<Gridder*>& degridders, const std::vector<Gr
   }
   YG_CHECK(a.nrow >= 0 && a.nchan > 0 && a.npol > 0, 1, "bad chunk shape");
   YG_CHECK(a.vis != nullptr, 1, "the imaging pass needs the data visibilities (chunk vis)");
+  YG_CHECK(a.vis_out_rows == 0 || a.vis_out_rows == a.nrow, 1, "the imaging pass returns a residual cube of nrow rows");
   const Ctx& c = *cp;
   const size_t ncube = (size_t)a.nrow * a.nchan * a.npol;
   DeviceChunk dc(c, a);

@@ -67,6 +67,15 @@ __device__ int w_plane(const CIndexDesc& ci, double w) {
 
 // FP32 -> FP16 or BF16 bits, round to nearest even (finite values; BF16 by the carry into the
 // upper half, FP16 by the hardware conversion), and back.
+// MVDirection::separation: great-circle angle between two (longitude, latitude) directions
+__device__ __forceinline__ double separation_d(double l1, double b1, double l2, double b2) {
+  const double dl = l2 - l1;
+  const double a = cos(b2) * sin(dl);
+  const double b = cos(b1) * sin(b2) - sin(b1) * cos(b2) * cos(dl);
+  const double c = sin(b1) * sin(b2) + cos(b1) * cos(b2) * cos(dl);
+  return atan2(sqrt(a * a + b * b), c);
+}
+
 __device__ __forceinline__ uint16_t round16(float x, int prec) {
   if (prec == kPrecF16) return __builtin_bit_cast(uint16_t, (_Float16)x);
   const uint32_t u = __builtin_bit_cast(uint32_t, x);
@@ -110,7 +119,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
     for (int e = tid; e < nsw; e += blockDim.x) s_sumw[e] = 0.0;
     __syncthreads();
   }
-  unsigned long long c_valid = 0, c_taps = 0, c_flag = 0, c_wflag = 0;
+  unsigned long long c_valid = 0, c_taps = 0, c_flag = 0, c_wflag = 0, c_reject = 0;
   int err = 0;
   for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += (long long)gridDim.x * blockDim.x) {
   const int s = (int)base + tid;  // s = row + nrow*chan
@@ -130,13 +139,21 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
   for (int q = 0; q < kCoef; ++q) coef[q] = 0.f;
 
   bool valid = inb;
+  // rows whose pointing lies farther than MaxPointingSeparation from the image centre are left
+  // out before anything else (TableVisGridder.cc:571-578)
+  if (valid && p.max_sep > 0.0 && in.pointing1 &&
+      separation_d(p.centre_lon, p.centre_lat, in.pointing1[2 * row], in.pointing1[2 * row + 1]) > p.max_sep) {
+    valid = false;
+    if (chan == 0) ++c_reject;
+  }
   bool flagged = false, wflagged = false;
+  const bool used = valid;
   int swrow = 0, ichan = 0;
   float wts[4] = {0.f, 0.f, 0.f, 0.f};
   float4 lcf = make_float4(0.f, 0.f, 0.f, 0.f);
   unsigned long long taps = 0;
 
-  if (valid) {
+  if (used) {
     const double f = in.freq[chan];
     const double os = (double)p.oversample;
     // TableVisGridder.cc:616-649
@@ -212,7 +229,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
             const float vw = in.visweight ? in.visweight[chan] : 1.0f;
             if (grid_mode) {
               if (p.mode == kModePSF && p.psf_feed >= 0) {
+                // the representative feed and field (TableVisGridder.cc:809-812)
                 if ((in.feed1 ? in.feed1[row] : 0) != p.psf_feed) valid = false;
+                const double* dish = in.dish1 ? in.dish1 : in.pointing1;
+                if (valid && p.psf_field && dish &&
+                    !(separation_d(p.psf_lon, p.psf_lat, dish[2 * row], dish[2 * row + 1]) < 1e-6))
+                  valid = false;
               }
               if (p.mode == kModePCF) {
                 conj = (ivo < p.ny / 2 && iuo >= p.nx / 2) || (ivo <= p.ny / 2 && iuo < p.nx / 2);
@@ -328,7 +350,9 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
   const unsigned long long nflag = wave_sum_u64(c_flag);
   const unsigned long long nwflag = wave_sum_u64(c_wflag);
   const unsigned long long tsum = wave_sum_u64(c_taps);
+  const unsigned long long nrej = wave_sum_u64(c_reject);
   if (lane == 0) {
+    if (nrej) atomicAdd(&out.counters[4], nrej);
     if (nvalid) atomicAdd(&out.counters[0], nvalid * (unsigned long long)p.npol);
     if (tsum) atomicAdd(&out.counters[1], tsum);
     if (nflag) atomicAdd(&out.counters[2], nflag);
@@ -1340,7 +1364,7 @@ __global__ __launch_bounds__(256, 2) void k_degrid_mm(const int4* __restrict__ i
 }
 
 __global__ void k_degrid_finish(Samples smp, const float* __restrict__ acc, int nrow, int nchan, int npol,
-                                float2* __restrict__ vis_out) {
+                                float2* __restrict__ vis_out, int ld) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nrow * nchan) return;
   if (smp.hdr[s].iu0 == INT_MIN) return;
@@ -1349,7 +1373,7 @@ __global__ void k_degrid_finish(Samples smp, const float* __restrict__ acc, int 
   const float fr = ph.x, fi = ph.y;
   for (int pol = 0; pol < npol; ++pol) {
     const float cr = acc[(size_t)s * 8 + 2 * pol], ci = acc[(size_t)s * 8 + 2 * pol + 1];
-    float2* o = vis_out + row + (size_t)nrow * (chan + (size_t)nchan * pol);
+    float2* o = vis_out + row + (size_t)ld * (chan + (size_t)nchan * pol);
     float2 v = *o;
     v.x += cr * fr - ci * fi;
     v.y += cr * fi + ci * fr;
@@ -1691,7 +1715,7 @@ __global__ void k_pol_to_image(PolMatrix M, const float2* __restrict__ vis, cons
 }
 
 __global__ void k_pol_to_vis_add(PolMatrix M, const float2* __restrict__ img, size_t nsamp,
-                                 float2* __restrict__ vis_out) {
+                                 float2* __restrict__ vis_out, int nrow, int ld) {
   for (size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x; s < nsamp; s += (size_t)gridDim.x * blockDim.x) {
     float2 x[4];
 #pragma unroll
@@ -1708,10 +1732,12 @@ __global__ void k_pol_to_vis_add(PolMatrix M, const float2* __restrict__ img, si
         ar += m.x * x[q].x - m.y * x[q].y;
         ai += m.x * x[q].y + m.y * x[q].x;
       }
-      float2 o = vis_out[s + nsamp * p];
+      // sample s = row + nrow * chan of polarisation p, in a cube of ld rows
+      const size_t vo = (s % nrow) + (size_t)ld * (s / nrow + (nsamp / nrow) * p);
+      float2 o = vis_out[vo];
       o.x += ar;
       o.y += ai;
-      vis_out[s + nsamp * p] = o;
+      vis_out[vo] = o;
     }
   }
 }
@@ -1727,11 +1753,13 @@ void launch_pol_to_image(const Ctx& c, const PolMatrix& M, const float2* vis, co
   YG_HIP(hipGetLastError());
 }
 
-void launch_pol_to_vis_add(const Ctx& c, const PolMatrix& M, const float2* img, size_t nsamp, float2* vis_out) {
+void launch_pol_to_vis_add(const Ctx& c, const PolMatrix& M, const float2* img, size_t nsamp, float2* vis_out,
+                           int nrow, int ld) {
   YG_CHECK(M.nin >= 1 && M.nin <= 4 && M.nout >= 1 && M.nout <= 4, 1, "bad polarisation matrix");
   if (nsamp == 0) return;
+  YG_CHECK(nrow > 0 && ld >= nrow && nsamp % nrow == 0, 1, "bad visibility cube shape");
   const unsigned blocks = (unsigned)std::min<size_t>((nsamp + 255) / 256, (size_t)8 * c.n_cu);
-  hipLaunchKernelGGL(k_pol_to_vis_add, dim3(blocks), dim3(256), 0, c.stream, M, img, nsamp, vis_out);
+  hipLaunchKernelGGL(k_pol_to_vis_add, dim3(blocks), dim3(256), 0, c.stream, M, img, nsamp, vis_out, nrow, ld);
   YG_HIP(hipGetLastError());
 }
 
@@ -1809,11 +1837,12 @@ void launch_degrid(const Ctx& c, Binning& b, const CFView& cf, const float2* gri
 }
 
 void launch_degrid_finish(const Ctx& c, const Samples& recs, const float* acc, int nrow, int nchan, int npol,
-                          float2* vis_out) {
+                          float2* vis_out, int ld) {
   const int n = nrow * nchan;
   if (n == 0) return;
+  YG_CHECK(ld >= nrow, 1, "vis_out_rows must be 0 or at least nrow");
   hipLaunchKernelGGL(k_degrid_finish, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, acc, nrow, nchan, npol,
-                     vis_out);
+                     vis_out, ld);
   YG_HIP(hipGetLastError());
 }
 

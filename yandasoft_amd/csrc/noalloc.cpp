@@ -101,8 +101,8 @@ void Gridder::degridNoAlloc(const ygrid_chunk& a, void* ws, size_t bytes) {
   if (profiling_) YG_HIP(hipEventRecord(ev_[2], c.stream));
   launch_tile_jobs(c, c.degrid_variant, true, shape_[2], ps.cfv.prec, job, items, base + 1, counter);
   if (profiling_) YG_HIP(hipEventRecord(ev_[3], c.stream));
-  launch_degrid_finish(c, ps.out.smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out);
-  unsigned long long cnt[4] = {0, 0, 0, 0};
+  launch_degrid_finish(c, ps.out.smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out, d.vis_out_ld);
+  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
   int err = 0;
   YG_HIP(hipMemcpyAsync(cnt, ps.out.counters, sizeof(cnt), hipMemcpyDeviceToHost, c.stream));
   YG_HIP(hipMemcpyAsync(&err, ps.out.error, sizeof(int), hipMemcpyDeviceToHost, c.stream));

@@ -196,14 +196,19 @@ void Gridder::convertedPass(const ygrid_chunk& a, bool forward) {
   ic.noise = nullptr;
   ic.flag = dc.c.flag ? f.as<uint8_t>() : nullptr;
   ic.vis_out = model.as<float>();
+  ic.vis_out_rows = 0;
   generic(ic, true);
+  const int ld = a.vis_out_rows > 0 ? a.vis_out_rows : a.nrow;
+  YG_CHECK(ld >= a.nrow, 1, "vis_out_rows must be 0 or at least nrow");
   if (a.mem == YGRID_MEM_DEVICE) {
-    launch_pol_to_vis_add(c, Mb, model.as<float2>(), nsamp, reinterpret_cast<float2*>(a.vis_out));
-  } else {
-    DevBuf out(std::max<size_t>(vcube, 1) * sizeof(float2));
-    YG_HIP(hipMemcpyAsync(out.get(), a.vis_out, vcube * sizeof(float2), hipMemcpyHostToDevice, c.stream));
-    launch_pol_to_vis_add(c, Mb, model.as<float2>(), nsamp, out.as<float2>());
-    YG_HIP(hipMemcpyAsync(a.vis_out, out.get(), vcube * sizeof(float2), hipMemcpyDeviceToHost, c.stream));
+    launch_pol_to_vis_add(c, Mb, model.as<float2>(), nsamp, reinterpret_cast<float2*>(a.vis_out), a.nrow, ld);
+  } else if (vcube) {
+    // the span of the (ld-row) cube this chunk's rows occupy
+    const size_t span = (size_t)a.nrow + (size_t)ld * ((size_t)a.nchan * a.npol - 1);
+    DevBuf out(span * sizeof(float2));
+    YG_HIP(hipMemcpyAsync(out.get(), a.vis_out, span * sizeof(float2), hipMemcpyHostToDevice, c.stream));
+    launch_pol_to_vis_add(c, Mb, model.as<float2>(), nsamp, out.as<float2>(), a.nrow, ld);
+    YG_HIP(hipMemcpyAsync(a.vis_out, out.get(), span * sizeof(float2), hipMemcpyDeviceToHost, c.stream));
     YG_HIP(hipStreamSynchronize(c.stream));
   }
   YG_HIP(hipStreamSynchronize(c.stream));

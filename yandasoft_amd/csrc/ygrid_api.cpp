@@ -134,6 +134,23 @@ int ygrid_gridder_create(ygrid_ctx* ctx, const char* name, const char* parset, y
   });
 }
 
+int ygrid_gridder_configure(ygrid_gridder* g, const char* options) {
+  return guard([&] {
+    NEED(g);
+    const ygrid::ParamMap m = ygrid::parse_parset(options);
+    for (const auto& kv : m) {
+      static const char* known[] = {"padding", "alldatapsf", "oversampleweight", "cleargrids", "MaxPointingSeparation",
+                                    "parotation", "parotation.angle", "swappols"};
+      bool ok = false;
+      for (const char* k : known) ok = ok || kv.first == k;
+      YG_CHECK(ok, YGRID_ERR_ARG, "ygrid_gridder_configure: unknown option " + kv.first);
+    }
+    YG_CHECK(!ygrid::Params{m}.flag("swappols", false), YGRID_ERR_ARG,
+             "polarisation swapping (gridder.swappols) needs parallactic angle rotation, which the GPU gridders do not do");
+    g->g->configure_globals(ygrid::Params{m});
+  });
+}
+
 int ygrid_gridder_clone(const ygrid_gridder* g, ygrid_gridder** out) {
   return guard([&] {
     NEED(g);

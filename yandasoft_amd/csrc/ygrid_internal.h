@@ -151,12 +151,18 @@ struct PrepParams {
   // (0: the CF table's planes)
   int lcf_S;
   long long lcf_plane_elems;
+  // MaxPointingSeparation (radians; <= 0: off) about the image centre (TableVisGridder.cc:571-578)
+  double max_sep;
+  double centre_lon, centre_lat;
+  // PSF passes with a representative field (psf_feed >= 0): also its dish pointing (psf_field)
+  int psf_field;
+  double psf_lon, psf_lat;
 };
 
 struct PrepOutputs {
   Samples smp;                  // nrow*nchan
   double* sumw;                 // nsw_rows * npol * nchan_img
-  unsigned long long* counters; // [0] samples, [1] taps, [2] flagged, [3] wflagged
+  unsigned long long* counters; // [0] samples, [1] taps, [2] flagged, [3] wflagged, [4] rows rejected by pointing
   int* error;                   // bit 1: zero noise weight; bit 2: w out of range
   float4* lcf_in;               // learned CFs: (w in wavelengths, fracu, fracv, gridded) per sample
 };
@@ -169,6 +175,8 @@ struct PrepInputs {
   const float2* noise;
   const uint8_t* flag;
   const int32_t* feed1;
+  const double* pointing1;     // nrow x 2 (lon, lat) or null (= the image centre)
+  const double* dish1;         // nrow x 2 or null (= pointing1)
   const int32_t* chanmap;      // nchan: image channel or -1
   const float* visweight;      // nchan or null
   const int32_t* cf_off;       // per CF plane (incl. oversampling)
@@ -332,14 +340,17 @@ struct PolMatrix {
 // flag_o = any input flagged.  Null outputs are skipped.
 void launch_pol_to_image(const Ctx&, const PolMatrix& M, const float2* vis, const float2* noise,
                          const uint8_t* flag, size_t nsamp, float2* vis_o, float2* noise_o, uint8_t* flag_o);
-// Back to the visibility frame: vis_out += M img.
-void launch_pol_to_vis_add(const Ctx&, const PolMatrix& M, const float2* img, size_t nsamp, float2* vis_out);
+// Back to the visibility frame: vis_out += M img, img the nsamp = nrow x nchan samples per
+// polarisation, vis_out a cube of ld rows.
+void launch_pol_to_vis_add(const Ctx&, const PolMatrix& M, const float2* img, size_t nsamp, float2* vis_out, int nrow,
+                           int ld);
 // r[i] = data[i] - r[i] over n complex values (the residual of a model cube r).
 void launch_residual(const Ctx&, const float2* data, float2* r, size_t n);
 // Rounds the FP32 CF table (n complex taps) to prec (FP16 or BF16), packed (re, im) in 32 bits.
 void launch_cf_round(const Ctx&, const float2* src, uint32_t* dst, size_t n, int prec);
+// vis_out: a cube of ld >= nrow rows
 void launch_degrid_finish(const Ctx&, const Samples& smp, const float* acc, int nrow, int nchan, int npol,
-                          float2* vis_out);
+                          float2* vis_out, int ld);
 
 // FFT-domain helpers (kernels_fft.hip)
 struct FFTPlan;
