@@ -11,14 +11,41 @@
 #include <askap/gridding/VisGridderFactory.h>
 #include <askap/measurementequation/ImageParamsHelper.h>
 #include <askap/measurementequation/SynthesisParamsHelper.h>
+#include <askap/scimath/utils/PaddingUtils.h>
 #include <casacore/coordinates/Coordinates/DirectionCoordinate.h>
 
+#include <iomanip>
 #include <sstream>
 
 namespace askap {
 namespace synthesis {
 
 namespace {
+// VisGridderFactory::make applies its global options after creating a gridder, through
+// TableVisGridder's setters (VisGridderFactory.cc:137-232); the fields they set are private and
+// have no getters, and setSourceIndex (ImageFFTEquation.cc:283-287) is the same.  The usual
+// access checks do not apply to the names in an explicit template instantiation, so a pointer to
+// each field can be formed there, which lets the adapter read the values without a change to the
+// reference's headers.  (With a getter added upstream, read that instead.)
+template <class Tag, typename Tag::type M>
+struct MemberOf {
+  friend typename Tag::type field(Tag) { return M; }
+};
+#define YGRID_TVG_FIELD(TAG, TYPE, NAME)                  \
+  struct TAG {                                            \
+    typedef TYPE TableVisGridder::*type;                  \
+    friend type field(TAG);                               \
+  };                                                      \
+  template struct MemberOf<TAG, &TableVisGridder::NAME>;
+YGRID_TVG_FIELD(UseAllPSF, bool, itsUseAllDataForPSF)
+YGRID_TVG_FIELD(MaxPointingSep, double, itsMaxPointingSeparation)
+YGRID_TVG_FIELD(TrackPerPlane, bool, itsTrackWeightPerOversamplePlane)
+YGRID_TVG_FIELD(PARotation, bool, itsPARotation)
+YGRID_TVG_FIELD(SwapPols, bool, itsSwapPols)
+YGRID_TVG_FIELD(ClearGrid, bool, itsClearGrid)
+YGRID_TVG_FIELD(SourceIndex, int, itsSourceIndex)
+#undef YGRID_TVG_FIELD
+
 std::shared_ptr<ygrid_ctx> shared_context() {
   static std::weak_ptr<ygrid_ctx> cache;
   auto ctx = cache.lock();
@@ -44,13 +71,44 @@ void GPUGridderAdapter::check(int status) {
 }
 
 GPUGridderAdapter::GPUGridderAdapter(const std::string& kind, const LOFAR::ParameterSet& parset)
-    : itsKind(kind), itsCtx(shared_context()) {
+    : TableVisGridder(1, 1), itsKind(kind), itsCtx(shared_context()) {
   check(ygrid_gridder_create(itsCtx.get(), kind.c_str(), to_text(parset).c_str(), &itsGridder));
 }
 
+// TableVisGridder's copy constructor copies the factory's options (TableVisGridder.cc:126-168);
+// the library clone shares the CF cache (WProjectVisGridder.cc:51-52)
 GPUGridderAdapter::GPUGridderAdapter(const GPUGridderAdapter& other)
-    : IVisGridder(other), itsKind(other.itsKind), itsCtx(other.itsCtx), itsImage(other.itsImage) {
+    : TableVisGridder(other), itsKind(other.itsKind), itsCtx(other.itsCtx), itsImage(other.itsImage),
+      itsMFSRefFreq(other.itsMFSRefFreq) {
   check(ygrid_gridder_clone(other.itsGridder, &itsGridder));
+}
+
+void GPUGridderAdapter::applyFactoryOptions() {
+  ASKAPCHECK(!(this->*field(PARotation{})) && !(this->*field(SwapPols{})),
+             "gridder.parotation / gridder.swappols: parallactic angle rotation is not done by the GPU gridders");
+  std::ostringstream os;
+  os << std::setprecision(17) << "padding = " << paddingFactor() << "\n"
+     << "alldatapsf = " << ((this->*field(UseAllPSF{})) ? "true" : "false") << "\n"
+     << "oversampleweight = " << ((this->*field(TrackPerPlane{})) ? "true" : "false") << "\n"
+     << "cleargrids = " << ((this->*field(ClearGrid{})) ? "true" : "false") << "\n"
+     << "MaxPointingSeparation = " << (this->*field(MaxPointingSep{})) << "\n";
+  check(ygrid_gridder_configure(itsGridder, os.str().c_str()));
+}
+
+void GPUGridderAdapter::setGeometry(const scimath::Axes& axes, const casacore::IPosition& shape) {
+  // TableVisGridder::initialiseGrid (TableVisGridder.cc:1002-1003)
+  itsAxes = axes;
+  itsShape = scimath::PaddingUtils::paddedShape(shape, paddingFactor());
+}
+
+void GPUGridderAdapter::initConvolutionFunction(const accessors::IConstDataAccessor&) {
+  ASKAPTHROW(AskapError, "GPUGridderAdapter: the convolution functions live in libygrid");
+}
+void GPUGridderAdapter::initIndices(const accessors::IConstDataAccessor&) {
+  ASKAPTHROW(AskapError, "GPUGridderAdapter: the CF indices are computed in libygrid");
+}
+void GPUGridderAdapter::correctConvolution(casacore::Array<imtype>&) {
+  ASKAPTHROW(AskapError, "GPUGridderAdapter: the grid correction is applied in libygrid");
 }
 
 GPUGridderAdapter::~GPUGridderAdapter() { ygrid_gridder_destroy(itsGridder); }
@@ -75,7 +133,8 @@ ygrid_image_desc GPUGridderAdapter::describe(const scimath::Axes& axes, const ca
     d.freq_start = axes.start("FREQUENCY");
     d.freq_end = axes.end("FREQUENCY");
   }
-  // TableVisGridder::initStokes (TableVisGridder.cc:982-994): the stokes axis, or I
+  // TableVisGridder::initStokes (TableVisGridder.cc:982-994): the stokes axis, or I, one code per
+  // polarisation plane of the grid
   itsImageStokes.clear();
   if (axes.has("STOKES")) {
     const casacore::Vector<casacore::Stokes::StokesTypes> st = axes.stokesAxis();
@@ -83,26 +142,34 @@ ygrid_image_desc GPUGridderAdapter::describe(const scimath::Axes& axes, const ca
   } else {
     itsImageStokes.push_back(int32_t(casacore::Stokes::I));
   }
+  ASKAPCHECK(int(itsImageStokes.size()) == d.npol, "Stokes axis is not consistent with the shape of the grid. There are "
+             << d.npol << " planes in the grid and " << itsImageStokes.size()
+             << " polarisation descriptors defined by the STOKES axis");
   d.stokes = itsImageStokes.data();
   return d;
 }
 
 ygrid_chunk GPUGridderAdapter::stage(const accessors::IConstDataAccessor& acc, bool forward) {
   const casacore::uInt nRow = acc.nRow(), nChan = acc.nChannel(), nPol = acc.nPol();
-  const casacore::MVDirection centre(itsImage.centre_lon, itsImage.centre_lat);
-  // TableVisGridder.cc:487-491: rotated uvw and the rotation delay for the tangent point
-  const auto& uvw = acc.rotatedUVW(centre);
-  const auto& delay = acc.uvwRotationDelay(centre, centre);
+  // TableVisGridder.cc:481-495: uvw rotated to the tangent point and the delay that moves the
+  // phase centre from the tangent point to the image centre
+  const casacore::MVDirection imageCentre = getImageCentre();
+  const casacore::MVDirection tangentPoint = getTangentPoint();
+  const auto& uvw = acc.rotatedUVW(tangentPoint);
+  const auto& delay = acc.uvwRotationDelay(tangentPoint, imageCentre);
   itsUVW.resize(3 * nRow);
   itsDelay.resize(nRow);
   itsFeed.resize(nRow);
   itsPointing.resize(2 * nRow);
+  itsDish.resize(2 * nRow);
   for (casacore::uInt i = 0; i < nRow; ++i) {
     for (int k = 0; k < 3; ++k) itsUVW[3 * i + k] = uvw(i)(k);
     itsDelay[i] = delay(i);
     itsFeed[i] = acc.feed1()(i);
     itsPointing[2 * i] = acc.pointingDir1()(i).getLong();
     itsPointing[2 * i + 1] = acc.pointingDir1()(i).getLat();
+    itsDish[2 * i] = acc.dishPointing1()(i).getLong();
+    itsDish[2 * i + 1] = acc.dishPointing1()(i).getLat();
   }
   const casacore::Cube<casacore::Bool>& flags = acc.flag();
   itsFlag.assign(flags.begin(), flags.end());
@@ -117,6 +184,7 @@ ygrid_chunk GPUGridderAdapter::stage(const accessors::IConstDataAccessor& acc, b
   c.freq = acc.frequency().data();
   c.feed1 = itsFeed.data();
   c.pointing1 = itsPointing.data();
+  c.dish_pointing1 = itsDish.data();
   c.flag = itsFlag.data();
   // acc.stokes(): the visibilities' polarisation frame (TableVisGridder.cc:517-524)
   itsVisStokes.clear();
@@ -133,6 +201,10 @@ ygrid_chunk GPUGridderAdapter::stage(const accessors::IConstDataAccessor& acc, b
 
 void GPUGridderAdapter::initialiseGrid(const scimath::Axes& axes, const casacore::IPosition& shape,
                                        const bool dopsf, const bool dopcf) {
+  applyFactoryOptions();
+  setGeometry(axes, shape);
+  configureForPSF(dopsf);
+  configureForPCF(dopcf);
   itsImage = describe(axes, shape);
   check(ygrid_initialise_grid(itsGridder, &itsImage, dopsf ? 1 : 0, dopcf ? 1 : 0));
 }
@@ -155,6 +227,10 @@ void GPUGridderAdapter::finaliseWeights(casacore::Array<imtype>& out) {
 }
 
 void GPUGridderAdapter::initialiseDegrid(const scimath::Axes& axes, const casacore::Array<imtype>& image) {
+  applyFactoryOptions();
+  setGeometry(axes, image.shape());
+  configureForPSF(false);
+  configureForPCF(false);
   itsImage = describe(axes, image.shape());
   casacore::Array<float> buf(image.shape());
   casacore::convertArray(buf, image);
@@ -165,7 +241,13 @@ void GPUGridderAdapter::degrid(accessors::IDataAccessor& acc) {
   ygrid_chunk c = stage(acc, false);
   casacore::Cube<casacore::Complex>& vis = acc.rwVisibility();
   ASKAPCHECK(vis.contiguousStorage(), "GPU degridder needs a contiguous visibility cube");
-  c.vis_out = reinterpret_cast<float*>(vis.data());
+  // direction-dependent calibration: source s predicts into rows s*nRow .. of the cube
+  // (TableVisGridder.cc:567,861)
+  const int source = this->*field(SourceIndex{});
+  const size_t offset = size_t(source) * c.nrow;
+  ASKAPCHECK(offset + c.nrow <= vis.nrow(), "source index " << source << " beyond the visibility cube's rows");
+  c.vis_out = reinterpret_cast<float*>(vis.data() + offset);
+  c.vis_out_rows = int(vis.nrow());
   check(ygrid_degrid(itsGridder, &c));
 }
 
