@@ -883,6 +883,10 @@ class OracleGridder:
         self.counters = _Counters()
         self.model_empty = False
         self.max_sep = -1.0           # MaxPointingSeparation (radians; <= 0: off)
+        # degrid of a sample whose footprint is off the grid: True adds the previous on-grid
+        # sample's value, as TableVisGridder.cc:763-861 does; False leaves the visibility
+        # unchanged (zero in the returned cube)
+        self.stale_offgrid = True
         self.centre = (0.0, 0.0)      # image centre (lon, lat) for the pointing test
         self.rows = _Rows(None, None, 0, -1, 0, 0.0, 0.0, 0.0)
 
@@ -1009,7 +1013,8 @@ class OracleGridder:
             ofs_u = np.array([o[0] for o in self.cf.offsets], dtype=np.int32)
             ofs_v = np.array([o[1] for o in self.cf.offsets], dtype=np.int32)
         p = _Params(self.nx, self.ny, self.npol, self.nchan_img, self.uvcell[0], self.uvcell[1], os_, mode,
-                    int(self.track_per_os), self.sumw.shape[2] if mode != 3 else 1, int(self.use_all_psf), 1,
+                    int(self.track_per_os), self.sumw.shape[2] if mode != 3 else 1, int(self.use_all_psf),
+                    int(self.stale_offgrid),
                     float(self.max_sep), float(self.centre[0]), float(self.centre[1]))
         point = None if chunk.pointing1 is None else np.ascontiguousarray(chunk.pointing1, dtype=np.float64)
         dish = None if chunk.dish1 is None else np.ascontiguousarray(chunk.dish1, dtype=np.float64)

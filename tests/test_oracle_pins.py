@@ -379,6 +379,10 @@ def test_degrid_offgrid_sample_reference_adds_stale_value():
     d.initialise_degrid((N, N, 1, 1), (cell, cell), model)
     v = d.degrid(O.Chunk(uvw, f, np.zeros((2, 1, 1), np.complex64)))
     assert abs(v[0, 0, 0] - 1.0) < 1e-9 and abs(v[1, 0, 0] - v[0, 0, 0]) < 1e-12
+    # without the stale add the off-grid sample stays zero and the on-grid one is unchanged
+    d.stale_offgrid = False
+    v2 = d.degrid(O.Chunk(uvw, f, np.zeros((2, 1, 1), np.complex64)))
+    assert v2[1, 0, 0] == 0 and v2[0, 0, 0] == v[0, 0, 0]
 
 
 # ---------------------------------------------------------------------------------------
