@@ -9,6 +9,12 @@ oversample 4, support 64), with every input already resident in HBM.
 Metric: CF-tap updates/s (one complex multiply-accumulate of a (visibility, pol) with a CF tap =
 8 FP32 FLOP, TableVisGridder.cc:264-267,804), whole job.  One JSON line is printed.
 
+After the timed region, the oracle leg (skipped with --no-cpu-baseline) times the oracle's FP32
+restatement of the reference loop on the host on bounded samples of the same workload, on one
+thread (``cpu_baseline``) and on all cores as independent gridders (``cpu_baseline_all_cores``,
+with the host's core count and CPU model), and measures the GPU path's image and degridded-
+visibility error against the FP64 oracle on a row subset (``error_vs_fp64``).
+
 ``--workload sphfunc_disc`` measures the plain SphFunc case: 160,000 single-pol visibilities
 (uv uniform in a 2 km disc at 1.4 GHz, w = 0), 2048^2 grid, support 3, oversample 128, grid then
 degrid; HBM- and launch-bound by construction.
@@ -68,57 +74,180 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu() -> dict:
+    """The host's core count (all, and those this process may run on) and CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def host_threads() -> int:
+    """Worker threads for the all-cores CPU baseline: the cores this process may use, at most
+    the thread count the environment allows (OMP_NUM_THREADS) and 16."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, min(16, n))
+
+
+class _CpuPass:
+    """One oracle gridder and one degridder (the FP32 restatement of TableVisGridder::generic)
+    over a set of rows, set up outside the timing: run() grids the rows, then degrids the same
+    rows from the grid it made, and returns the taps of both passes."""
+
+    def __init__(self, wl, cf, prm, ws, rows):
+        from oracle import oracle as O
+        uvw = wl.uvw[rows]
+        cm = np.zeros((wl.nchan, wl.npol, len(rows)), dtype=np.int32)
+        for c, f in enumerate(wl.freq):
+            cm[c, :, :] = np.array([ws.plane(w / O.C_LIGHT * f) for w in uvw[:, 2]], dtype=np.int32)[None, :]
+        self.ch = O.Chunk(uvw, wl.freq, np.transpose(wl.vis[:, :, rows], (2, 1, 0)),
+                          noise=np.transpose(wl.noise[:, :, rows], (2, 1, 0)))
+        self.og = O.OracleGridder("WProject", "f32", wproject=prm)
+        self.og.cf = cf
+        self.og.cmap_override = cm
+        self.og.initialise_grid((wl.nx, wl.ny, wl.npol, 1), (wl.cell, wl.cell))
+        self.od = O.OracleGridder("WProject", "f32", wproject=prm)
+        self.od.cf = cf
+        self.od.cmap_override = cm
+        self.od._setup((wl.nx, wl.ny, wl.npol, 1), (wl.cell, wl.cell))
+        self.od.mode = 3
+        self.od.freq_axis = None
+        self.od.model_empty = False
+
+    def run(self) -> int:
+        self.og.grid(self.ch)
+        self.od.grid_ = self.og.grid_
+        self.od.degrid(self.ch)
+        return self.og.counters.taps_gridded + self.od.counters.taps_degridded
+
+
 def cpu_baseline(wl, g_grid, target_s: float = 12.0):
-    """Oracle's single-thread FP32 restatement of TableVisGridder::generic on a bounded sample
-    of the same workload (same CF table, downloaded from the device)."""
+    """The oracle's FP32 restatement of TableVisGridder::generic on bounded samples of the same
+    workload (same CF table, downloaded from the device), grid + degrid:
+
+    * single thread, as one reference gridder runs (the ``cpu_baseline`` object);
+    * all host cores: independent gridders, one per thread, each gridding and degridding its own
+      slice of rows into its own grid (as the reference's parallel imager gives each worker its
+      own gridder and grid; the merge of the workers' grids is not timed).  The C calls release
+      the interpreter lock, so the threads run in parallel.
+    Returns (single-thread object, all-cores object)."""
+    import threading
     from oracle import oracle as O
     os_ = wl.params["oversample"]
     nz = wl.params["nwplanes"]
-    planes = [g_grid.cf_plane(k) for k in range(nz * os_ * os_)]
+    cf = O.CFCache([g_grid.cf_plane(k) for k in range(nz * os_ * os_)], os_)
     prm = O.WProjectParams(wmax=wl.params["wmax"], nwplanes=nz, oversample=os_,
                            maxsupport=wl.params["maxsupport"], limitsupport=wl.params["limitsupport"])
     ws = prm.sampling()
 
-    def run(nrow_s):
-        rows = np.linspace(0, wl.nrow - 1, nrow_s).astype(int)
-        uvw = wl.uvw[rows]
-        vis = np.transpose(wl.vis[:, :, rows], (2, 1, 0))
-        noise = np.transpose(wl.noise[:, :, rows], (2, 1, 0))
-        cm = np.zeros((wl.nchan, wl.npol, nrow_s), dtype=np.int32)
-        for c, f in enumerate(wl.freq):
-            cm[c, :, :] = np.array([ws.plane(w / O.C_LIGHT * f) for w in uvw[:, 2]], dtype=np.int32)[None, :]
-        ch = O.Chunk(uvw, wl.freq, vis, noise=noise)
-        og = O.OracleGridder("WProject", "f32", wproject=prm)
-        og.cf = O.CFCache(planes, os_)
-        og.cmap_override = cm
-        og.initialise_grid((wl.nx, wl.ny, wl.npol, 1), (wl.cell, wl.cell))
+    def timed(passes):
+        taps = [0] * len(passes)
+        start = threading.Barrier(len(passes) + 1)
+
+        def work(i):
+            start.wait()
+            taps[i] = passes[i].run()
+
+        th = [threading.Thread(target=work, args=(i,)) for i in range(len(passes))]
+        for t in th:
+            t.start()
+        start.wait()
         t0 = time.perf_counter()
-        og.grid(ch)
-        t1 = time.perf_counter()
-        od = O.OracleGridder("WProject", "f32", wproject=prm)
-        od.cf = og.cf
-        od.cmap_override = cm
-        od._setup((wl.nx, wl.ny, wl.npol, 1), (wl.cell, wl.cell))
-        od.mode = 3
-        od.freq_axis = None
-        od.model_empty = False
-        od.grid_ = og.grid_
-        t2 = time.perf_counter()
-        od.degrid(ch)
-        t3 = time.perf_counter()
-        taps = og.counters.taps_gridded + od.counters.taps_degridded
-        return taps, (t1 - t0) + (t3 - t2)
+        for t in th:
+            t.join()
+        return sum(taps), time.perf_counter() - t0
+
+    def rows_of(n):
+        return np.linspace(0, wl.nrow - 1, n).astype(int)
 
     n = 4
-    taps, dt = run(n)
+    taps, dt = timed([_CpuPass(wl, cf, prm, ws, rows_of(n))])
     for _ in range(3):  # the first sizes are dominated by fixed costs: rescale until near the target
         if dt >= 0.5 * target_s or n >= wl.nrow:
             break
         n = min(wl.nrow, max(n + 1, int(n * target_s / max(dt, 1e-3))))
-        taps, dt = run(n)
-    return {"value": taps / dt, "unit": "CF-tap updates/s", "cores": 1, "kind": "port",
-            "sample": f"{n} rows x {wl.nchan} chans x {wl.npol} pols of the same workload, grid + degrid, "
-                      f"oracle FP32 restatement (gcc -O2, 1 thread), {dt:.1f} s"}
+        taps, dt = timed([_CpuPass(wl, cf, prm, ws, rows_of(n))])
+    hc = host_cpu()
+    one = {"value": taps / dt, "unit": "CF-tap updates/s", "cores": 1, "kind": "port",
+           "sample": f"{n} rows x {wl.nchan} chans x {wl.npol} pols of the same workload, grid + degrid, "
+                     f"oracle FP32 restatement (gcc -O2, 1 thread), {dt:.1f} s",
+           "host": hc}
+    T = host_threads()
+    rows = rows_of(min(wl.nrow, n * T))
+    passes = [_CpuPass(wl, cf, prm, ws, rows[t::T]) for t in range(T)]
+    taps_t, dt_t = timed(passes)
+    del passes
+    allc = {"value": taps_t / dt_t, "unit": "CF-tap updates/s", "cores": T, "kind": "port",
+            "sample": f"{len(rows)} rows x {wl.nchan} chans x {wl.npol} pols of the same workload, grid + degrid, "
+                      f"{T} independent oracle FP32 gridders (one per thread, rows interleaved, own grids; "
+                      f"grid merge not timed), {dt_t:.1f} s wall",
+            "host": hc, "speedup_vs_1_thread": (taps_t / dt_t) / (taps / dt)}
+    return one, allc
+
+
+def fp64_errors(ctx, wl, params, model):
+    """Error of the GPU path against the FP64 oracle (the checker) on a row subset of the same
+    workload: every 97th row at the band's two edge channels, kept where both channels' w planes
+    are among the 8 most used (the oracle transforms only those planes).  Image: max |x - x64| /
+    max |x64| over the whole image, where the grid-correction gain is <= 10, and divided by the
+    gain; degridded visibilities: max |v - v64| / max |v64| (off-grid samples left unchanged on
+    both sides)."""
+    import yandasoft_amd as ya
+    from oracle import oracle as O
+    rows = np.arange(0, wl.nrow, 97)
+    chans = np.array([0, wl.nchan - 1])
+    p = wl.params
+    prm = O.WProjectParams(wmax=p["wmax"], nwplanes=p["nwplanes"], cutoff=p["cutoff"], oversample=p["oversample"],
+                           maxsupport=p["maxsupport"], limitsupport=p["limitsupport"])
+    ws = prm.sampling()
+    freq = wl.freq[chans]
+    planes = np.array([[ws.plane(wl.uvw[r, 2] / O.C_LIGHT * f) for f in freq] for r in rows])
+    vals, cnt = np.unique(planes, return_counts=True)
+    keep = set(vals[np.argsort(-cnt)][:8].tolist())
+    sel = rows[[i for i in range(len(rows)) if set(planes[i].tolist()) <= keep]]
+    uvw = np.ascontiguousarray(wl.uvw[sel])
+    vis = np.ascontiguousarray(wl.vis[:, chans][:, :, sel])
+    noise = np.ascontiguousarray(wl.noise[:, chans][:, :, sel])
+    img = ya.ImageDesc(wl.nx, wl.ny, wl.npol, 1, wl.cell, wl.cell)
+    g = ya.Gridder(ctx, "GPUWProject", params)
+    g.initialise_grid(img)
+    g.grid(ya.Chunk(uvw=uvw, freq=freq, vis=vis, noise=noise))
+    gimg = np.asarray(g.finalise_grid(), np.float64)
+    och = O.Chunk(uvw, freq, np.transpose(vis, (2, 1, 0)), noise=np.transpose(noise, (2, 1, 0)))
+    o = O.OracleGridder("WProject", "f64", wproject=prm)
+    o.only_planes = set(keep)
+    o.initialise_grid((wl.nx, wl.ny, wl.npol, 1), (wl.cell, wl.cell))
+    o.grid(och)
+    oimg = o.finalise_grid()
+    gain = o.correct(np.ones((1, 1, o.ny, o.nx)))[0, 0]
+    peak = float(np.max(np.abs(oimg)))
+    err = np.abs(gimg.reshape(oimg.shape) - oimg)
+    d = ya.Gridder(ctx, "GPUWProject", params)
+    d.initialise_degrid(img, model)
+    out = np.zeros_like(vis)
+    d.degrid(ya.Chunk(uvw=uvw, freq=freq, vis=vis, noise=noise, vis_out=out))
+    od = O.OracleGridder("WProject", "f64", wproject=prm)
+    od.only_planes = set(keep)
+    od.stale_offgrid = False
+    od.initialise_degrid((wl.nx, wl.ny, wl.npol, 1), (wl.cell, wl.cell), np.asarray(model, np.float64))
+    ov = od.degrid(och)
+    verr = float(np.max(np.abs(np.transpose(out, (2, 1, 0)) - ov)) / np.max(np.abs(ov)))
+    return {"image_max_err": float(np.max(err)) / peak,
+            "image_max_err_gain_le_10": float(np.max(err[..., gain <= 10.0])) / peak,
+            "image_max_err_over_gain": float(np.max(err / gain)) / peak,
+            "degrid_max_err": verr,
+            "sample": f"{len(sel)} rows x 2 chans (band edges) x {wl.npol} pols, {len(keep)} w planes, "
+                      f"FP64 oracle (relative to the FP64 peak)"}
 
 
 def main():
@@ -127,7 +256,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nint", type=int, default=300, help="integrations (workload rows / 630)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the oracle leg: CPU baseline timings and (wproject_array) the FP64 error check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (3x), the reference for rocprof byte counters")
     ap.add_argument("--cf-precision", default="fp32", choices=["fp32", "fp16", "bf16"],
@@ -275,7 +405,13 @@ def main():
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline(wl, gg)
+        result["cpu_baseline"], result["cpu_baseline_all_cores"] = cpu_baseline(wl, gg)
+        log("error vs the FP64 oracle (row subset) ...")
+        result["error_vs_fp64"] = fp64_errors(ctx, wl, prm, model)
+        log(f"error vs FP64: {result['error_vs_fp64']}")
+        log(f"cpu baseline: 1 thread {result['cpu_baseline']['value'] / 1e6:.1f} Mtaps/s, "
+            f"{result['cpu_baseline_all_cores']['cores']} threads "
+            f"{result['cpu_baseline_all_cores']['value'] / 1e6:.1f} Mtaps/s")
     print(json.dumps(result), flush=True)
 
 
