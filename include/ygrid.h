@@ -118,6 +118,9 @@ typedef struct {
   /* rows left out because their pointing lies farther than MaxPointingSeparation from the image
    * centre (itsRowsRejectedDueToMaxPointingSeparation, TableVisGridder.cc:571-578) */
   double rows_rejected_pointing;
+  /* passes that ran the small-support kernels (CF support <= the context's small-support limit,
+   * ygrid_ctx_set_small_support) instead of the 64x64 tile kernels */
+  double small_support_passes;
 } ygrid_stats;
 
 /* ---- library / context ---- */
@@ -134,6 +137,13 @@ int ygrid_ctx_synchronize(ygrid_ctx *ctx);
  * pixels, CF planes and supports, as the degridding and gridding of one chunk do; no
  * counterpart in the reference (TableVisGridder recomputes the mapping on every pass). */
 int ygrid_ctx_clear_cache(ygrid_ctx *ctx);
+/* Passes whose CF support is at most max_support (0..8; default 8, environment
+ * YGRID_SMALL_SUPPORT at context creation) run the small-support kernels: gridding accumulates
+ * each 64x64 tile in LDS with one work item per tile, degridding reads every sample's
+ * footprint directly without binning.  0 keeps every pass on the 64x64 tile kernels.  Same
+ * results either way (within FP32 accumulation order); performance tuning with no counterpart
+ * in the reference. */
+int ygrid_ctx_set_small_support(ygrid_ctx *ctx, int max_support);
 
 /* ---- factory: VisGridderFactory::make / createGridder (VisGridderFactory.cc:112-286) ----
  * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUAWProject".

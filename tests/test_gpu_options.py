@@ -154,7 +154,10 @@ def test_configure_options(gpu_ctx):
     for g in (a, b):
         g.initialise_grid(img)
         g.grid(pchunk(d))
-    assert np.array_equal(a.finalise_grid(), b.finalise_grid())
+    # the same up to FP32 accumulation order (the order in which a tile's pairs are added is
+    # fixed by atomics at run time, so two passes agree to rounding, not bit for bit)
+    ia, ib = a.finalise_grid(), b.finalise_grid()
+    assert ia.shape == ib.shape and rel(ia, ib) < 1e-6
     with pytest.raises(ya.YgridError):
         b.configure({"parotation": "true"})
     with pytest.raises(ya.YgridError):

@@ -470,6 +470,16 @@ def test_binning_cache_reuse_and_invalidation(gpu_ctx):
     model = np.zeros((1, npol, N, N), np.float32)
     model[0, :, rng.integers(20, 108, 30), rng.integers(20, 108, 30)] = rng.normal(size=(30, npol))
 
+    # the tile kernels' binning (the small-support kernels do not bin when degridding)
+    gpu_ctx.set_small_support(0)
+    try:
+        _binning_cache_checks(gpu_ctx, img, d, model, N, cell, npol)
+    finally:
+        gpu_ctx.set_small_support(8)
+
+
+def _binning_cache_checks(gpu_ctx, img, d, model, N, cell, npol):
+    import yandasoft_amd as ya
     gpu_ctx.clear_cache()
     g = ya.Gridder(gpu_ctx, "GPUSphFunc", {"oversample": 8})
     g.initialise_grid(img)
@@ -498,11 +508,21 @@ def test_binning_cache_reuse_and_invalidation(gpu_ctx):
         assert rel(g.grid_array(), ref.grid_array()) < 1e-6
 
 
-def test_dense_tiles_span_staging_rounds_and_items(gpu_ctx):
+@pytest.mark.parametrize("small", [0, 8])
+def test_dense_tiles_span_staging_rounds_and_items(gpu_ctx, small):
     """Every sample within ~3 pixels of one tile corner: ~4000 pairs per (tile, key), so that a
-    work item is staged in several rounds of 512 pairs and a key is split over several items
-    (taken from several XCD queues)."""
-    announce("dense tile corner, SphFunc os 2, 4 pols, grid + degrid vs oracle")
+    work item of the tile kernels is staged in several rounds of 512 pairs and a key is split
+    over several items (taken from several XCD queues); the small-support kernels (small = 8)
+    take the same corner as four heavily loaded tiles of one item each."""
+    announce(f"dense tile corner, SphFunc os 2, 4 pols, grid + degrid vs oracle, small-support limit {small}")
+    gpu_ctx.set_small_support(small)
+    try:
+        _dense_corner(gpu_ctx)
+    finally:
+        gpu_ctx.set_small_support(8)
+
+
+def _dense_corner(gpu_ctx):
     import yandasoft_amd as ya
     N, cell, npol = 256, 30 * ARC, 4
     d = make_data(2500, 2, npol, N, cell, frac=0.02, seed=32)

@@ -13,7 +13,7 @@ YGRID_MEM_DEVICE = 1
 
 EXPORTED = [
     "ygrid_last_error", "ygrid_version", "ygrid_ctx_create", "ygrid_ctx_destroy", "ygrid_ctx_set_stream",
-    "ygrid_ctx_synchronize", "ygrid_ctx_clear_cache", "ygrid_gridder_create", "ygrid_gridder_configure",
+    "ygrid_ctx_synchronize", "ygrid_ctx_clear_cache", "ygrid_ctx_set_small_support", "ygrid_gridder_create", "ygrid_gridder_configure",
     "ygrid_gridder_clone", "ygrid_gridder_destroy",
     "ygrid_initialise_grid", "ygrid_grid", "ygrid_finalise_grid", "ygrid_finalise_weights",
     "ygrid_initialise_degrid", "ygrid_degrid", "ygrid_degrid_noalloc", "ygrid_finalise_degrid", "ygrid_is_model_empty",
@@ -59,7 +59,8 @@ class Stats(ctypes.Structure):
                 ("grid_waves", ctypes.c_int), ("grid_pipeline", ctypes.c_int),
                 ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int),
                 ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int), ("learned_cf_active", ctypes.c_int),
-                ("degrid_matrix", ctypes.c_int), ("rows_rejected_pointing", ctypes.c_double)]
+                ("degrid_matrix", ctypes.c_int), ("rows_rejected_pointing", ctypes.c_double),
+                ("small_support_passes", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):
@@ -192,6 +193,10 @@ class Context:
     def clear_cache(self):
         """Drop the cached sample binning; the next pass bins again."""
         _check(lib().ygrid_ctx_clear_cache(self.h))
+
+    def set_small_support(self, max_support: int):
+        """Passes with CF support <= max_support (0..8) run the small-support kernels; 0: none."""
+        _check(lib().ygrid_ctx_set_small_support(self.h, int(max_support)))
 
     def close(self):
         if self.h:

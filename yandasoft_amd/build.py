@@ -22,6 +22,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 SOURCES = [
     "kernels_grid.hip",
+    "kernels_small.hip",
     "kernels_fft.hip",
     "kernels_cf.hip",
     "kernels_awp.hip",

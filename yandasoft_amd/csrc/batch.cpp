@@ -83,7 +83,7 @@ void Batch::prepare() {
     YG_CHECK(gr.items_cap < (size_t)INT_MAX, 1, "too many work items in one batched launch");
     gr.items.resize(gr.items_cap * sizeof(int4));
     gr.base.resize((gr.passes.size() + 1) * sizeof(int));
-    gr.counter.resize(kItemQueues * sizeof(int));
+    gr.counter.resize(kQueueInts * sizeof(int));
     std::vector<TileJob> jobs;
     for (int pi : gr.passes) {
       Pass& p = passes_[pi];

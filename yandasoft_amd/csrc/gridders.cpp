@@ -525,6 +525,14 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps, Pass
   ps.nsamp = nsamp;
   ps.cfv = cf_view();
   if (!arena) customizePass(ps);
+  // small supports: the small-support kernels, and one CF-plane group, so that a binning key
+  // names its own region of the grid (the small-support gridding kernel owns a key's region;
+  // with w stacking the key already names a grid)
+  ps.small = ps.cfv.max_support <= std::min(ctx_->small_support_max, kSmallSupportMax);
+  if (ps.small && nGridPlanes() == 1) {
+    pp.zdiv = nz;
+    pp.nzg = 1;
+  }
   return true;
 }
 
@@ -561,20 +569,58 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
   if (profiling_) YG_HIP(hipEventRecord(ev_[1], ctx_->stream));
   const Samples smp = ps.out.smp;
   const CFView& cfv = ps.cfv;
-  if (bin_samples(*ctx_, smp, d.nrow, d.nchan, shape_[0], shape_[1], ps.pp.nzg * shape_[3], cfv)) ++n_bin_reused_;
-  Binning& bins = ctx_->bins.bin;
-  if (forward) {
-    YG_HIP(hipMemsetAsync(ps.acc, 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
-    if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, ps.acc,
-                  nGridPlanes() > 1 ? gridPlaneElems() : 0);
-    if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
-    launch_degrid_finish(*ctx_, smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out, d.vis_out_ld);
+  const size_t wstride = nGridPlanes() > 1 ? gridPlaneElems() : 0;
+  // w stacking grids on the tile kernels: its keys (w grid x tile) hold few samples each, which
+  // the small-support kernel's per-band work units serve worse
+  if (ps.small && (forward || nGridPlanes() == 1)) {
+    // small supports: degridding reads each sample's footprint directly (no binning);
+    // gridding bins with one work item per key, without a host round trip
+    ++n_small_;
+    if (forward) {
+      if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
+      launch_degrid_small(*ctx_,
+                          SmallDegridArgs{smp, cfv.ptr, grid_.as<float>(), ps.acc, d.nrow, d.nchan, shape_[0],
+                                          shape_[1], ps.pp.nzg, wstride},
+                          shape_[2], cfv.prec, cfv.max_support);
+      if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
+      launch_degrid_finish(*ctx_, smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out, d.vis_out_ld);
+    } else {
+      SmallGridBufs& sb = ctx_->small;
+      constexpr int kOneItemPerKey = (1 << 30) - 1;
+      size_static_bins(*ctx_, sb.bins, d.nrow, d.nchan, shape_[0], shape_[1], ps.pp.nzg * shape_[3], cfv,
+                       kOneItemPerKey);
+      const StaticBinsView& v = sb.bins.v;
+      YG_CHECK(v.npairs_cap < (size_t)kOneItemPerKey && v.nitems_cap < (size_t)INT_MAX, 1,
+               "too many (sample, tile) pairs for one pass: split the chunk");
+      sb.items.resize(v.nitems_cap * sizeof(int4));
+      sb.base.resize(2 * sizeof(int));
+      sb.counter.resize(kQueueInts * sizeof(int));
+      int* base = sb.base.as<int>();
+      launch_fill_u32(*ctx_, base, 2, 0u);
+      launch_fill_u32(*ctx_, sb.counter.get(), kQueueInts, 0u);
+      enqueue_static_binning(*ctx_, v, smp, cfv, sb.items.as<int4>(), base, base + 1, 0);
+      if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
+      const int ntu = (shape_[0] + kTile - 1) / kTile, ntv = (shape_[1] + kTile - 1) / kTile;
+      launch_grid_small(*ctx_, sb.items.as<int4>(), base + 1, sb.counter.as<int>(), (int)v.nitems_cap,
+                        SmallGridArgs{v.pairs, smp, cfv.ptr, grid_.as<float>(), shape_[0], shape_[1], ntu, ntu * ntv,
+                                      ps.pp.nzg, wstride},
+                        shape_[2], cfv.prec, cfv.max_support);
+      if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
+    }
   } else {
-    if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-    launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg,
-                nGridPlanes() > 1 ? gridPlaneElems() : 0);
-    if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
+    if (bin_samples(*ctx_, smp, d.nrow, d.nchan, shape_[0], shape_[1], ps.pp.nzg * shape_[3], cfv)) ++n_bin_reused_;
+    Binning& bins = ctx_->bins.bin;
+    if (forward) {
+      YG_HIP(hipMemsetAsync(ps.acc, 0, (size_t)nsamp * 8 * sizeof(float), ctx_->stream));
+      if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
+      launch_degrid(*ctx_, bins, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, ps.acc, wstride);
+      if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
+      launch_degrid_finish(*ctx_, smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out, d.vis_out_ld);
+    } else {
+      if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
+      launch_grid(*ctx_, bins, smp, cfv, grid_.as<float2>(), shape_[0], shape_[1], shape_[2], ps.pp.nzg, wstride);
+      if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
+    }
   }
   unsigned long long cnt[5] = {0, 0, 0, 0, 0};
   int err = 0;
@@ -730,6 +776,7 @@ void Gridder::stats(ygrid_stats* o) const {
   o->n_grid_launches = n_gk_;
   o->n_degrid_launches = n_dk_;
   o->n_bin_reused = n_bin_reused_;
+  o->small_support_passes = n_small_;
   o->grid_waves = ctx_->grid_variant.waves;
   o->grid_pipeline = ctx_->grid_variant.pipeline;
   o->degrid_waves = ctx_->degrid_variant.waves;

@@ -180,6 +180,7 @@ class Gridder {
     CFView cfv{};
     int nsamp = 0;
     float* acc = nullptr;  // degridding: per-sample accumulators (nsamp x 8)
+    bool small = false;    // the CF support allows the small-support kernels
   };
   // arena: carve the pass's buffers from it (the allocation-free entry point); null: the
   // gridder's own buffers, grown as needed
@@ -232,6 +233,7 @@ class Gridder {
   int cf_prec_ = kPrecF32, vis_prec_ = kPrecF32;  // storage precisions (cfprecision, visprecision)
   DevBuf smp_coef16_;
   double n_bin_reused_ = 0;  // passes that reused the context's cached binning
+  double n_small_ = 0;       // passes that ran the small-support kernels
   TileJob noalloc_job_{};    // host copy of the allocation-free pass's job (read by its H2D copy)
   PlanPtr plan_;
   double st_samples_g_ = 0, st_taps_g_ = 0, st_samples_d_ = 0, st_taps_d_ = 0, st_flag_ = 0, st_wflag_ = 0;

@@ -25,6 +25,7 @@
 #include <climits>
 #include <type_traits>
 
+#include "device_util.h"
 #include "ygrid_internal.h"
 
 namespace ygrid {
@@ -65,8 +66,6 @@ __device__ int w_plane(const CIndexDesc& ci, double w) {
   return (r < ci.nw && r > -1) ? r : -1;
 }
 
-// FP32 -> FP16 or BF16 bits, round to nearest even (finite values; BF16 by the carry into the
-// upper half, FP16 by the hardware conversion), and back.
 // MVDirection::separation: great-circle angle between two (longitude, latitude) directions
 __device__ __forceinline__ double separation_d(double l1, double b1, double l2, double b2) {
   const double dl = l2 - l1;
@@ -76,15 +75,6 @@ __device__ __forceinline__ double separation_d(double l1, double b1, double l2, 
   return atan2(sqrt(a * a + b * b), c);
 }
 
-__device__ __forceinline__ uint16_t round16(float x, int prec) {
-  if (prec == kPrecF16) return __builtin_bit_cast(uint16_t, (_Float16)x);
-  const uint32_t u = __builtin_bit_cast(uint32_t, x);
-  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ __forceinline__ float widen16(uint32_t h, int prec) {
-  if (prec == kPrecF16) return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
-  return __builtin_bit_cast(float, h << 16);
-}
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -108,6 +98,10 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
 __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepOutputs out, int lds_sumw) {
 #pragma clang fp contract(off)
   extern __shared__ double s_sumw[];
+  // the block's counters, added to the pass's counters once per block (the counters share one
+  // cache line: per-wave atomics from thousands of waves serialise there)
+  __shared__ unsigned long long s_cnt[5];
+  __shared__ int s_err;
   const int n = p.nrow * p.nchan;
   const bool grid_mode = p.mode != kModeDegrid;
   const bool sumw_on = grid_mode && p.mode != kModePCF;
@@ -115,10 +109,11 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
   const int nsw = p.nsw_rows * p.npol * p.nchan_img;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  if (use_lds) {
+  if (tid < 5) s_cnt[tid] = 0;
+  if (tid == 0) s_err = 0;
+  if (use_lds)
     for (int e = tid; e < nsw; e += blockDim.x) s_sumw[e] = 0.0;
-    __syncthreads();
-  }
+  __syncthreads();
   unsigned long long c_valid = 0, c_taps = 0, c_flag = 0, c_wflag = 0, c_reject = 0;
   int err = 0;
   for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += (long long)gridDim.x * blockDim.x) {
@@ -352,18 +347,19 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
   const unsigned long long tsum = wave_sum_u64(c_taps);
   const unsigned long long nrej = wave_sum_u64(c_reject);
   if (lane == 0) {
-    if (nrej) atomicAdd(&out.counters[4], nrej);
-    if (nvalid) atomicAdd(&out.counters[0], nvalid * (unsigned long long)p.npol);
-    if (tsum) atomicAdd(&out.counters[1], tsum);
-    if (nflag) atomicAdd(&out.counters[2], nflag);
-    if (nwflag) atomicAdd(&out.counters[3], nwflag * (grid_mode ? 2ull : 1ull));
+    if (nrej) atomicAdd(&s_cnt[4], nrej);
+    if (nvalid) atomicAdd(&s_cnt[0], nvalid * (unsigned long long)p.npol);
+    if (tsum) atomicAdd(&s_cnt[1], tsum);
+    if (nflag) atomicAdd(&s_cnt[2], nflag);
+    if (nwflag) atomicAdd(&s_cnt[3], nwflag * (grid_mode ? 2ull : 1ull));
   }
-  if (err) atomicOr(out.error, err);
-  if (use_lds) {
-    __syncthreads();
+  if (err) atomicOr(&s_err, err);
+  __syncthreads();
+  if (tid < 5 && s_cnt[tid]) atomicAdd(&out.counters[tid], s_cnt[tid]);
+  if (tid == 0 && s_err) atomicOr(out.error, s_err);
+  if (use_lds)
     for (int e = tid; e < nsw; e += blockDim.x)
       if (s_sumw[e] != 0.0) atomicAdd(&out.sumw[e], s_sumw[e]);
-  }
 }
 
 // --------------------------------------------------------------------------------------
@@ -437,7 +433,10 @@ __device__ __forceinline__ int wave_max_int(int v) {
   return v;
 }
 
-__global__ void k_bin_count(Samples smp, int nrow, int nchan, int ntu, int ntiles, int* counts) {
+// counts (and cursor in k_bin_fill) of key k live at [k * kstride]: with few keys, kstride 32
+// gives every key its own 128-byte line, so that atomics on different keys do not serialise on
+// a shared line
+__global__ void k_bin_count(Samples smp, int nrow, int nchan, int ntu, int ntiles, int* counts, int kstride) {
   const long long tglob = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const BinLane b = bin_lane(smp, tglob, nrow, nchan, ntiles);
@@ -448,7 +447,7 @@ __global__ void k_bin_count(Samples smp, int nrow, int nchan, int ntu, int ntile
       const bool active = b.s >= 0 && tv <= b.t.tv1 && tu <= b.t.tu1;
       const long long idx = active ? b.base + tv * ntu + tu : -1;
       const Run run = wave_run(idx, active, lane);
-      if (run.head) atomicAdd(&counts[idx], run.len);
+      if (run.head) atomicAdd(&counts[idx * kstride], run.len);
     }
 }
 
@@ -492,7 +491,7 @@ __device__ __forceinline__ int4 pair_descriptor(const SampleHdr& r, int s, int u
 }
 
 __global__ void k_bin_fill(Samples smp, int nrow, int nchan, int ntu, int ntiles, uint32_t pad, uint32_t esz,
-                           int* cursor, int4* descs) {
+                           int* cursor, int kstride, int4* descs) {
   const long long tglob = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const BinLane b = bin_lane(smp, tglob, nrow, nchan, ntiles);
@@ -503,7 +502,7 @@ __global__ void k_bin_fill(Samples smp, int nrow, int nchan, int ntu, int ntiles
       const bool active = b.s >= 0 && tv <= b.t.tv1 && tu <= b.t.tu1;
       const long long idx = active ? b.base + tv * ntu + tu : -1;
       const Run run = wave_run(idx, active, lane);
-      const int hpos = run.head ? atomicAdd(&cursor[idx], run.len) : 0;
+      const int hpos = run.head ? atomicAdd(&cursor[idx * kstride], run.len) : 0;
       const int pos = __shfl(hpos, run.hlane, 64) + run.rank;
       if (active) descs[pos] = pair_descriptor(b.r, b.s, tu * kTile, tv * kTile, pad, esz);
     }
@@ -542,9 +541,11 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-__global__ void k_copy_i32(const int* __restrict__ src, int* __restrict__ dst, size_t n) {
+// dst[i * dstride] = src[i * sstride], i < n
+__global__ void k_copy_strided_i32(const int* __restrict__ src, int sstride, int* __restrict__ dst, int dstride,
+                                   size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+    dst[i * dstride] = src[i * sstride];
 }
 
 // --------------------------------------------------------------------------------------
@@ -751,19 +752,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
 // taken as blockIdx.x % kItemQueues (the dispatcher places consecutive workgroups on the XCDs in
 // turn; were it otherwise, only locality would suffer).  A workgroup takes items from its own
 // range and, once that is exhausted, from the following ones; returns -1 when all are.
-__device__ __forceinline__ int next_item(int* __restrict__ counter, int nitems, int& q, int& left, int* s_item) {
-  for (;;) {
-    const int lo = (int)((long long)nitems * q / kItemQueues);
-    const int hi = (int)((long long)nitems * (q + 1) / kItemQueues);
-    if (threadIdx.x == 0) *s_item = lo + atomicAdd(&counter[q], 1);
-    __syncthreads();
-    const int item = __builtin_amdgcn_readfirstlane(*s_item);  // uniform: item data via scalar loads
-    __syncthreads();
-    if (item < hi) return item;
-    if (--left == 0) return -1;
-    q = q + 1 == kItemQueues ? 0 : q + 1;
-  }
-}
 
 // The staged pairs whose footprint touches wave w's rows, as indices into the round's staging
 // arrays: the wave then skips the pairs it has nothing to add for.  Returns their number.
@@ -1393,14 +1381,13 @@ void launch_prep(const Ctx& c, const PrepParams& p, const PrepInputs& in, const 
   // sums of weights in shared memory when they fit in 64 KB; then a few blocks per CU loop over
   // the samples, so that each block adds its sums to the global ones once
   const long long nsw = (long long)p.nsw_rows * p.npol * p.nchan_img;
+  // Both forms run a bounded grid looping over the samples: every block ends with global atomics
+  // on the pass's few counters (one cache line), which serialise there.
   const bool lds = p.mode != kModeDegrid && p.mode != kModePCF && nsw > 0 && nsw <= 8192;
+  const int g = std::min(blocks, 2 * c.n_cu);
   if (lds) {
-    const int g = std::min(blocks, 4 * c.n_cu);
     hipLaunchKernelGGL(k_prep, dim3(g), dim3(256), (size_t)nsw * sizeof(double), c.stream, p, in, out, 1);
   } else {
-    // also a bounded grid looping over the samples: every wave ends with global atomics on the
-    // pass's few counters, which one wave per 64 samples would serialise on those addresses
-    const int g = std::min(blocks, 8 * c.n_cu);
     hipLaunchKernelGGL(k_prep, dim3(g), dim3(256), 0, c.stream, p, in, out, 0);
   }
   YG_HIP(hipGetLastError());
@@ -1425,14 +1412,20 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
   b.offsets.resize((nkeys + 1) * sizeof(int));
   b.cursor.resize((nkeys + 1) * sizeof(int));
   b.nitems.resize(sizeof(int) * 2);
-  b.counter.resize(sizeof(int) * kItemQueues);
+  b.counter.resize(sizeof(int) * kQueueInts);
   // nitems[1] = 0: the item base of this single-job binning
   YG_HIP(hipMemsetAsync(b.nitems.get(), 0, 2 * sizeof(int), c.stream));
   int* counts = b.counts.as<int>();
   int* offsets = b.offsets.as<int>();
-  YG_HIP(hipMemsetAsync(counts, 0, (nkeys + 1) * sizeof(int), c.stream));
+  const int ks = bin_key_stride(nkeys);
+  if (ks > 1) b.wide.resize((nkeys + 1) * ks * sizeof(int));
+  int* cw = ks > 1 ? b.wide.as<int>() : counts;
+  const int cgrid = (int)std::min<size_t>((nkeys + 256) / 256, 4 * (size_t)c.n_cu);
+  YG_HIP(hipMemsetAsync(cw, 0, (nkeys + 1) * ks * sizeof(int), c.stream));
   if (n > 0) hipLaunchKernelGGL(k_bin_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, nrow, nchan, ntu,
-                                (int)ntiles, counts);
+                                (int)ntiles, cw, ks);
+  if (ks > 1)
+    hipLaunchKernelGGL(k_copy_strided_i32, dim3(cgrid), dim3(256), 0, c.stream, cw, ks, counts, 1, nkeys + 1);
   size_t tbytes = 0;
   YG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, counts, offsets, (int)(nkeys + 1), c.stream));
   b.temp.resize(tbytes);
@@ -1443,10 +1436,14 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
   YG_HIP(hipStreamSynchronize(c.stream));
   YG_CHECK(npairs >= 0, 3, "pair count overflow");
   b.pairs.resize((size_t)(npairs > 0 ? npairs : 1) * sizeof(int4));
-  YG_HIP(hipMemcpyAsync(b.cursor.get(), offsets, (nkeys + 1) * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
+  int* cur = ks > 1 ? cw : b.cursor.as<int>();
+  if (ks > 1)
+    hipLaunchKernelGGL(k_copy_strided_i32, dim3(cgrid), dim3(256), 0, c.stream, offsets, 1, cw, ks, nkeys + 1);
+  else
+    YG_HIP(hipMemcpyAsync(cur, offsets, (nkeys + 1) * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, nrow, nchan, ntu, (int)ntiles,
-                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), b.cursor.as<int>(),
+                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), cur, ks,
                        b.pairs.as<int4>());
   // work items: reuse cursor as the per-key item count and its scan
   int* icount = b.cursor.as<int>();
@@ -1482,6 +1479,7 @@ void static_bins_shape(const Ctx& c, StaticBinsView& v, int nrow, int nchan, int
   v.npairs_cap = cap;
   v.chunk = std::max(chunk, 16);
   v.nitems_cap = cap / v.chunk + std::min(nkeys, cap) + 1;
+  v.kstride = bin_key_stride(nkeys);
   v.nrow = nrow;
   v.nchan = nchan;
   v.nx = nx;
@@ -1505,6 +1503,8 @@ void size_static_bins(const Ctx& c, StaticBins& sb, int nrow, int nchan, int nx,
   sb.ioffsets.resize(nk1 * sizeof(int));
   sb.pairs.resize(std::max<size_t>(v.npairs_cap, 1) * sizeof(int4));
   sb.temp.resize(v.temp_bytes);
+  if (v.kstride > 1) sb.wide.resize(nk1 * v.kstride * sizeof(int));
+  v.wide = v.kstride > 1 ? sb.wide.as<int>() : nullptr;
   v.counts = sb.counts.as<int>();
   v.offsets = sb.offsets.as<int>();
   v.cursor = sb.cursor.as<int>();
@@ -1522,17 +1522,23 @@ void enqueue_static_binning(const Ctx& c, const StaticBinsView& sb, const Sample
   const int nk = (int)sb.nkeys;
   int* counts = sb.counts;
   int* offsets = sb.offsets;
-  launch_fill_u32(c, counts, (size_t)nk + 1, 0u);
+  const int ks = sb.kstride;
+  YG_CHECK(ks == 1 || sb.wide, 2, "padded binning counters not allocated");
+  int* cw = ks > 1 ? sb.wide : counts;
+  const int cgrid = std::min((nk + 256) / 256, 4 * c.n_cu);
+  launch_fill_u32(c, reinterpret_cast<uint32_t*>(cw), ((size_t)nk + 1) * ks, 0u);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_count, dim3((n + 255) / 256), dim3(256), 0, c.stream, smp, sb.nrow, sb.nchan, ntu, ntiles,
-                       counts);
+                       cw, ks);
+  if (ks > 1)
+    hipLaunchKernelGGL(k_copy_strided_i32, dim3(cgrid), dim3(256), 0, c.stream, cw, ks, counts, 1, (size_t)nk + 1);
   size_t tb = sb.temp_bytes;
   YG_HIP(hipcub::DeviceScan::ExclusiveSum(sb.temp, tb, counts, offsets, nk + 1, c.stream));
-  hipLaunchKernelGGL(k_copy_i32, dim3(std::min((nk + 256) / 256, 4 * c.n_cu)), dim3(256), 0, c.stream, offsets,
-                     sb.cursor, (size_t)nk + 1);
+  int* cur = ks > 1 ? cw : sb.cursor;
+  hipLaunchKernelGGL(k_copy_strided_i32, dim3(cgrid), dim3(256), 0, c.stream, offsets, 1, cur, ks, (size_t)nk + 1);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, smp, sb.nrow, sb.nchan, ntu, ntiles,
-                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), sb.cursor, sb.pairs);
+                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), cur, ks, sb.pairs);
   hipLaunchKernelGGL(k_item_count, dim3((nk + 1 + 255) / 256), dim3(256), 0, c.stream, counts, nk, sb.chunk,
                      sb.icount);
   tb = sb.temp_bytes;
@@ -1796,7 +1802,7 @@ static void launch_degrid_mm(hipStream_t s, dim3 g, int prec, const int4* items,
 
 void launch_tile_jobs(const Ctx& c, const TileVariant& v0, bool forward, int npol, int prec, const TileJob* jobs,
                       const int4* items, const int* nitems, int* counter) {
-  launch_fill_u32(c, counter, kItemQueues, 0u);
+  launch_fill_u32(c, counter, kQueueInts, 0u);
   // one polarisation: half of the matrix form's outputs would be zeros; the VALU form is used
   if (forward && v0.matrix && npol >= 2) {
     const dim3 g(c.n_cu * 2);  // two workgroups (8 waves) per CU: <= 256 VGPRs per wave

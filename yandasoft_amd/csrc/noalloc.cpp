@@ -81,9 +81,10 @@ void Gridder::degridNoAlloc(const ygrid_chunk& a, void* ws, size_t bytes) {
   v.ioffsets = arena.take_n<int>(nk1);
   v.pairs = arena.take_n<int4>(std::max<size_t>(v.npairs_cap, 1));
   v.temp = arena.take(v.temp_bytes);
+  v.wide = v.kstride > 1 ? arena.take_n<int>(nk1 * v.kstride) : nullptr;
   int4* items = arena.take_n<int4>(v.nitems_cap);
   int* base = arena.take_n<int>(2);  // [0] = 0: the item base; [1]: the item count
-  int* counter = arena.take_n<int>(kItemQueues);
+  int* counter = arena.take_n<int>(kQueueInts);
   TileJob* job = arena.take_n<TileJob>(1);
   // the workspace keeps the previous call's contents: clear what must start at zero (fill
   // kernels, no memset nodes)

@@ -77,6 +77,8 @@ int ygrid_ctx_create(int device, ygrid_ctx** out) {
     variant("YGRID_GRID_VARIANT", c->grid_variant);
     variant("YGRID_DEGRID_VARIANT", c->degrid_variant);
     if (const char* s = std::getenv("YGRID_ITEM_PAIRS")) c->item_pairs = std::max(16, std::atoi(s));
+    if (const char* s = std::getenv("YGRID_SMALL_SUPPORT"))
+      c->small_support_max = std::min(std::max(0, std::atoi(s)), ygrid::kSmallSupportMax);
     // the allocation-free entry point's workspace on this device (its device is current here)
     ygrid::noalloc_setup(device);
     YG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -120,6 +122,15 @@ int ygrid_ctx_clear_cache(ygrid_ctx* ctx) {
   return guard([&] {
     NEED(ctx);
     ctx->c->bins.valid = false;
+  });
+}
+
+int ygrid_ctx_set_small_support(ygrid_ctx* ctx, int max_support) {
+  return guard([&] {
+    NEED(ctx);
+    YG_CHECK(max_support >= 0 && max_support <= ygrid::kSmallSupportMax, YGRID_ERR_ARG,
+             "small-support limit must lie in 0..8");
+    ctx->c->small_support_max = max_support;
   });
 }
 

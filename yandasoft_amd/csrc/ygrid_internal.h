@@ -21,6 +21,10 @@ constexpr int kTile = 64;
 constexpr int kDefaultItemPairs = 2048;
 // work-item queues of a tile-kernel launch (one per XCD; their counters live in one int array)
 constexpr int kItemQueues = 8;
+// ints between two queue counters (each its own 128-byte line: the workgroups of a launch hit
+// them all at once) and the size of a launch's counter array
+constexpr int kQueueStride = 32;
+constexpr int kQueueInts = kItemQueues * kQueueStride;
 // Guard band of zeros around every CF plane in device memory so that the 16x4-lane
 // blocks of the grid/degrid kernels may read past the (2S+1)^2 footprint without a
 // per-lane bounds test: columns -kGuardU..2S+kGuardU-1, rows -kGuardV..2S+kGuardV-1.
@@ -208,7 +212,7 @@ struct TileJob {
 };
 
 struct Binning {
-  DevBuf counts, offsets, cursor, ioffsets, pairs, items, nitems, counter, temp;
+  DevBuf counts, offsets, cursor, ioffsets, pairs, items, nitems, counter, temp, wide;
   size_t nkeys = 0, npairs_cap = 0;
   // the one-job table of a single-grid launch, and its host copy (kept until the pass has synced)
   DevBuf job;
@@ -236,6 +240,35 @@ struct BinCache {
   Binning bin;
 };
 
+// Spacing (in ints) of the per-key counters the binning kernels update atomically: with few
+// keys each key gets its own 128-byte line (a padded array of at most 1 MB), else 1.
+inline int bin_key_stride(size_t nkeys) { return nkeys + 1 <= 8192 ? 32 : 1; }
+
+struct StaticBinsView {
+  int *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *icount = nullptr, *ioffsets = nullptr;
+  // the padded counters when kstride > 1 (counts, then cursor), else null
+  int* wide = nullptr;
+  int kstride = 1;
+  int4* pairs = nullptr;
+  void* temp = nullptr;
+  size_t nkeys = 0, npairs_cap = 0, nitems_cap = 0, temp_bytes = 0;
+  int nrow = 0, nchan = 0, nx = 0, ny = 0, nkz = 0, chunk = 0;
+};
+struct StaticBins {
+  DevBuf counts, offsets, cursor, icount, ioffsets, pairs, temp, wide;
+  StaticBinsView v;
+};
+
+// Largest CF support the small-support kernels take (kernels_small.hip).
+constexpr int kSmallSupportMax = 8;
+
+// Buffers of the small-support gridding pass: its static binning (one work item per key) and
+// the persistent launch's item list, item count and queue counters.
+struct SmallGridBufs {
+  StaticBins bins;
+  DevBuf items, base, counter;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -244,8 +277,37 @@ struct Ctx {
   TileVariant grid_variant{2, 1};
   TileVariant degrid_variant{2, 1, 1};
   int item_pairs = 0;  // pairs per work item of the tile kernels (0: the built-in default)
+  // passes whose CF support is at most this use the small-support kernels (0: never;
+  // at most kSmallSupportMax)
+  int small_support_max = kSmallSupportMax;
   BinCache bins;  // shared by the gridders of the context (their passes are ordered on stream)
+  SmallGridBufs small;  // likewise
 };
+
+// Arguments of the small-support kernels (kernels_small.hip).  The grid holds nPol planes per
+// image channel (float2 x nx x ny each); with w stacking, grid w of a key's (key % nzg) starts
+// wstride float2 elements after grid w - 1, else wstride is 0.
+struct SmallGridArgs {
+  const int4* pairs;  // the binning's pair descriptors (x: the sample)
+  Samples smp;
+  const void* cf;     // the CF table (taps of cf_elem_bytes(prec) bytes; SampleHdr::cfoff indexes it)
+  float* grid;
+  int nx, ny, ntu, ntiles, nzg;
+  size_t wstride;
+};
+struct SmallDegridArgs {
+  Samples smp;
+  const void* cf;
+  const float* grid;
+  float* acc;  // per sample 8 floats: (re, im) of C * conj(G) summed, per polarisation
+  int nrow, nchan, nx, ny, nzg;
+  size_t wstride;
+};
+// Persistent small-support gridding over the items of a one-item-per-key static binning
+// (nitems: device item count, nitems_cap its bound; counter: kItemQueues zeros).
+void launch_grid_small(const Ctx&, const int4* items, const int* nitems, int* counter, int nitems_cap,
+                       const SmallGridArgs& a, int npol, int prec, int max_support);
+void launch_degrid_small(const Ctx&, const SmallDegridArgs& a, int npol, int prec, int max_support);
 
 // Kernel launchers (kernels_grid.hip)
 void launch_prep(const Ctx&, const PrepParams&, const PrepInputs&, const PrepOutputs&);
@@ -284,21 +346,10 @@ void launch_tile_jobs(const Ctx&, const TileVariant& v, bool forward, int npol, 
 // pass's work items go to items_out[*base_in ...) with job index `job`, and
 // *base_out = *base_in + its item count.  The view holds the buffers and their sizes; they are
 // owned by a StaticBins or carved from a caller's workspace.
-struct StaticBinsView {
-  int *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *icount = nullptr, *ioffsets = nullptr;
-  int4* pairs = nullptr;
-  void* temp = nullptr;
-  size_t nkeys = 0, npairs_cap = 0, nitems_cap = 0, temp_bytes = 0;
-  int nrow = 0, nchan = 0, nx = 0, ny = 0, nkz = 0, chunk = 0;
-};
 // The sizes of the binning of nrow x nchan samples of support <= cf.max_support (no allocation:
 // the scan's temporary size is a query).
 void static_bins_shape(const Ctx&, StaticBinsView& v, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
                        int chunk);
-struct StaticBins {
-  DevBuf counts, offsets, cursor, icount, ioffsets, pairs, temp;
-  StaticBinsView v;
-};
 // Sizes sb for nrow x nchan samples of support <= cf.max_support (allocates).
 void size_static_bins(const Ctx&, StaticBins& sb, int nrow, int nchan, int nx, int ny, int nkz, const CFView& cf,
                       int chunk);
