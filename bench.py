@@ -54,13 +54,14 @@ HBM_PEAK_GBS = 8000.0
 
 
 PROFILE = "profiles/k_grid_k_degrid_wproject_array.md"
+PROFILE_SPH = "profiles/k_grid_small_k_degrid_small_sphfunc_disc.md"
 
 
-def profiled_traffic(kernel):
+def profiled_traffic(kernel, profile=PROFILE):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated) recorded by
-    the rocprofv3 --pmc runs of this benchmark summarised in PROFILE; None if not recorded."""
+    the rocprofv3 --pmc runs of this benchmark summarised in `profile`; None if not recorded."""
     try:
-        with open(os.path.join(ROOT, PROFILE)) as f:
+        with open(os.path.join(ROOT, profile)) as f:
             for line in f:
                 parts = line.split()
                 if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel.replace(" ", ""):
@@ -750,20 +751,30 @@ def bench_sphfunc(args):
     s1g, s1d = gg.stats(), dg.stats()
     taps = (s1g.taps_gridded - s0g.taps_gridded) + (s1d.taps_degridded - s0d.taps_degridded)
     gk = s1g.ms_grid_kernel / max(1.0, s1g.n_grid_launches)
+    dk = s1d.ms_degrid_kernel / max(1.0, s1d.n_degrid_launches)
     # compulsory bytes of one gridding pass: uvw 24 B + vis 8 B + noise 8 B per sample, the CF
     # table once (128^2 planes x 7 x 7 taps x 8 B), the grid written once (2048^2 x 8 B)
     nbytes = wl.nrow * (24 + 8 + 8) + 128 * 128 * 49 * 8 + wl.nx * wl.ny * 8
     achieved = nbytes / (gk * 1e-3) / 1e9
+    small = s1g.small_support_passes > 0
+    lanes = 8 if s1g.support <= 3 else (16 if s1g.support <= 7 else 32)
+    gname = f"k_grid_small<1, 0, {lanes}>" if small else "k_grid<1, 2, true, 0>"
+    traffic = profiled_traffic(gname, PROFILE_SPH)
     result = {
         "metric": "CF-tap updates/s (SphFunc grid+degrid)", "value": taps / el, "unit": "CF-tap updates/s",
         "n_gpus": 1, "steps": nsteps, "warmup": args.warmup, "ms_per_step": 1e3 * el / nsteps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "sphfunc_disc: 160000 vis, uv uniform in a 2 km disc, 1.4 GHz, w = 0, 2048^2, "
                                "support 3, oversample 128", "rows": wl.nrow, "grid": wl.nx, "parallelism": "1 GPU"},
-        "phases_ms_per_pass": {"grid_kernel": gk, "prep": s1g.ms_prep / max(1.0, s1g.n_grid_launches),
+        "phases_ms_per_pass": {"grid_kernel": gk, "degrid_kernel": dk,
+                               "prep": s1g.ms_prep / max(1.0, s1g.n_grid_launches),
                                "binning": s1g.ms_bin / max(1.0, s1g.n_grid_launches)},
-        "roofline": {"bound": "hbm", "kernel": "k_grid<1, 2, true, 0>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "small_support_kernels": small,
+        "roofline": {"bound": "hbm", "kernel": gname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": PROFILE_SPH,
+                     "measured_traffic_gbs": None if traffic is None else traffic / (gk * 1e-3) / 1e9,
+                     "measured_traffic_frac": None if traffic is None else traffic / (gk * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     print(json.dumps(result), flush=True)
 

@@ -291,10 +291,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
 #pragma unroll
       for (int q = 0; q < kCoef / 8; ++q) c4[q] = make_uint4(wds[4 * q], wds[4 * q + 1], wds[4 * q + 2], wds[4 * q + 3]);
     } else {
+      // (degridding reads only [0..1], the phasor times the weight)
       float4* c4 = reinterpret_cast<float4*>(out.smp.coef + (size_t)s * kCoef);
+      const int nq = p.mode == kModeDegrid ? 1 : kCoef / 4;
 #pragma unroll
       for (int q = 0; q < kCoef / 4; ++q)
-        c4[q] = make_float4(coef[4 * q], coef[4 * q + 1], coef[4 * q + 2], coef[4 * q + 3]);
+        if (q < nq) c4[q] = make_float4(coef[4 * q], coef[4 * q + 1], coef[4 * q + 2], coef[4 * q + 3]);
     }
   }
 

@@ -45,16 +45,19 @@ __device__ __forceinline__ float2 load_tap(const void* __restrict__ cf, long lon
 
 // the four MAC coefficients (a, b, c, d) of polarisation q of sample s: re += a*C.re + b*C.im,
 // im += c*C.re + d*C.im (the layout the prep kernel writes)
+// (i0 and i1 are even: two 8-byte, or two 4-byte, loads)
 __device__ __forceinline__ float4 pol_coef(const Samples& smp, int s, int q) {
   const int ore = 2 * (q & 1), mre = 2 * (q >> 1);
   const int i0 = 4 * ore + mre, i1 = 4 * (ore + 1) + mre;
   if (smp.coef_prec == kPrecF32) {
     const float* c = smp.coef + (size_t)s * kCoef;
-    return make_float4(c[i0], c[i0 + 1], c[i1], c[i1 + 1]);
+    const float2 a = *reinterpret_cast<const float2*>(c + i0), b = *reinterpret_cast<const float2*>(c + i1);
+    return make_float4(a.x, a.y, b.x, b.y);
   }
   const uint16_t* c = smp.coef16 + (size_t)s * kCoef;
-  return make_float4(widen16(c[i0], smp.coef_prec), widen16(c[i0 + 1], smp.coef_prec), widen16(c[i1], smp.coef_prec),
-                     widen16(c[i1 + 1], smp.coef_prec));
+  const uint32_t a = *reinterpret_cast<const uint32_t*>(c + i0), b = *reinterpret_cast<const uint32_t*>(c + i1);
+  return make_float4(widen16(a & 0xffffu, smp.coef_prec), widen16(a >> 16, smp.coef_prec),
+                     widen16(b & 0xffffu, smp.coef_prec), widen16(b >> 16, smp.coef_prec));
 }
 
 // Rows of the band a workgroup of k_grid_small owns: work unit u is band u % kBands of item
@@ -68,7 +71,8 @@ constexpr int kGridSmallBS = 256;
 // descriptors a workgroup of k_grid_small stages per round (two per thread)
 constexpr int kGridSmallRound = 2 * kGridSmallBS;
 constexpr int kGridSmallWaves = kGridSmallBS / 64;
-// pairs a wave of k_grid_small loads before it adds them (taps and coefficients in flight)
+// pairs a wave of k_grid_small loads before it adds them (taps and coefficients in flight;
+// deeper batches were measured slower: the registers cost occupancy)
 constexpr int kGridSmallBatch = 4;
 
 // L: columns per lane row (the power of two >= 2S+1); a wave instruction covers 64/L band rows
@@ -261,12 +265,12 @@ __global__ __launch_bounds__(256) void k_degrid_small(SmallDegridArgs a) {
       sr[q] += __shfl_xor(sr[q], o, 64);
       si[q] += __shfl_xor(si[q], o, 64);
     }
+  // (the visibility cube is written by k_degrid_finish, one thread per sample in row order:
+  // coalesced, where this kernel's channel-fastest order would scatter the writes)
   if (s >= 0 && c == 0) {
 #pragma unroll
-    for (int q = 0; q < NP; ++q) {
-      a.acc[(size_t)s * 8 + 2 * q] = sr[q];
-      a.acc[(size_t)s * 8 + 2 * q + 1] = si[q];
-    }
+    for (int q = 0; q < NP; ++q)
+      *reinterpret_cast<float2*>(a.acc + (size_t)s * 8 + 2 * q) = make_float2(sr[q], si[q]);
   }
 }
 
