@@ -686,8 +686,10 @@ class OracleWStack:
     the planes, then corrects and scales (:255-289); initialiseDegrid fills each plane with the
     forward transform of conj(screen x corrected model) (:314-327)."""
 
-    def __init__(self, wmax: float, nwplanes: int, precision: str = "f32"):
-        self.ws = WSampling(wmax, nwplanes)
+    def __init__(self, wmax: float, nwplanes: int, precision: str = "f32", **sampling):
+        """sampling: WSampling's kind, exponent, nwplanes50, wmaxclip (the gridder's wsampling,
+        wsampling.exponent, wsampling.nwplanes50, wmaxclip keys; WDependentGridderBase.cc:190-215)"""
+        self.ws = WSampling(wmax, nwplanes, **sampling)
         self.precision = precision
 
     def _planes(self, chunk: Chunk):
@@ -701,12 +703,15 @@ class OracleWStack:
         return Chunk(chunk.uvw, chunk.freq, chunk.vis, noise=chunk.noise, flag=np.ascontiguousarray(flag),
                      delay=chunk.delay)
 
-    def grid(self, shape, cell, chunk: Chunk):
+    def grid(self, shape, cell, chunk: Chunk, dopcf: bool = False):
+        """dopcf: the preconditioner pass (SphFunc's 1 + 3i nearest-neighbour kernel per plane,
+        SphFuncVisGridder.cc:99-123)"""
         planes = self._planes(chunk)
         self.grids, self.sumw = [], 0.0
+        self.dopcf = dopcf
         for iw in range(self.ws.nw):
             o = OracleGridder("SphFunc", self.precision)
-            o.initialise_grid(shape, cell)
+            o.initialise_grid(shape, cell, dopcf=dopcf)
             if np.any(planes == iw):
                 o.grid(self._plane_chunk(chunk, planes, iw))
             self.grids.append(o.grid_.astype(np.complex128))
@@ -714,8 +719,29 @@ class OracleWStack:
         self.o = o
         return self.grids
 
+    def pcf_scale(self, iw: int) -> float:
+        """updatew's factor on plane iw's imaginary part: the would-be W-projection kernel size
+        (3 pixels below one pixel of w spread, else 6 + 1.14 wThetaPix) over the 3 of the
+        preconditioner kernel, in the reference's float arithmetic (WStackVisGridder.cc:180-190)"""
+        o = self.o
+        wtp = np.float32(abs(self.ws.wterm(iw)) / (o.uvcell[0] * o.uvcell[0]))
+        wkp = np.float32(3.0) if wtp < 1 else np.float32(6 + 1.14 * float(wtp))
+        return float(wkp / np.float32(3.0))
+
     def finalise_grid(self):
         o = self.o
+        if getattr(self, "dopcf", False):
+            # WStackVisGridder::finaliseGrid's preconditioner form (:222-262): the planes that hold
+            # something, imaginary parts scaled (updatew), stacked, one normalised inverse
+            # transform, real part, no correction, no scaling
+            stack = 0.0
+            for iw, g in enumerate(self.grids):
+                if np.abs(g).max() > 0:
+                    stack = stack + (g.real + 1j * g.imag * self.pcf_scale(iw))
+            img = fft_centred(np.asarray(stack, np.complex128), False).real
+            nx0, ny0 = o.shape0[0], o.shape0[1]
+            ox, oy = (o.nx - nx0) // 2, (o.ny - ny0) // 2
+            return img[:, :, oy:oy + ny0, ox:ox + nx0]
         acc = 0.0
         for iw, g in enumerate(self.grids):
             acc = acc + (fft_centred(g, False) * wstack_screen(o.nx, o.ny, o.uvcell, self.ws.wterm(iw))).real

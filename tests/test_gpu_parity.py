@@ -229,7 +229,8 @@ def _wp_oracle_params(**kw):
 
 
 @pytest.mark.parametrize("extra", [{}, {"variablesupport": True}, {"variablesupport": True, "offsetsupport": True},
-                                   {"wsampling": "powerlaw", "wsampling.exponent": 0.5}])
+                                   {"wsampling": "powerlaw", "wsampling.exponent": 0.5},
+                                   {"wsampling": "gaussian", "wsampling.nwplanes50": 4.0}])
 def test_wproject_cf_grid_degrid_match_oracle(gpu_ctx, extra):
     announce(f"wproject {extra}")
     N, cell = 256, 30 * ARC
@@ -237,7 +238,8 @@ def test_wproject_cf_grid_degrid_match_oracle(gpu_ctx, extra):
     params = dict(WP)
     params.update(extra)
     okw = dict(WP)
-    okw.update({k.replace("wsampling.exponent", "exponent"): v for k, v in extra.items()})
+    # parset "wsampling.<param>" keys are the oracle's <param> fields
+    okw.update({k.split(".", 1)[1] if k.startswith("wsampling.") else k: v for k, v in extra.items()})
     prm = O.WProjectParams(**okw)
     g, o, gg, og, gi, oi, gw, ow = grid_both(gpu_ctx, "GPUWProject", params, dict(kind="WProject", wproject=prm), d,
                                             N, cell, 4)

@@ -794,3 +794,29 @@ def test_psf_uses_the_first_rows_feed_and_field_across_calls():
     h.grid(O.Chunk(np.ascontiguousarray(uvw[sel]), f, np.ascontiguousarray(vis[sel])))
     assert 0 < np.count_nonzero(sel) < sel.size
     assert np.array_equal(g.grid_, h.grid_)
+
+
+def test_wstack_pcf_on_one_w0_plane_is_the_single_grid_preconditioner_over_n():
+    # WStackVisGridder.cc:222-262 against TableVisGridder.cc:1109-1236 for the preconditioner:
+    # with every sample at w = 0 (one plane, updatew's factor 3 / 3 = 1) the w-stacking image
+    # is the real part of the normalised inverse transform of the one plane, where the single
+    # grid's is that times nx * ny (neither corrects the preconditioner)
+    rng = np.random.default_rng(8)
+    N, cell, f = 64, math.radians(40 / 3600), np.array([1.0e9, 1.05e9])
+    lam = O.C_LIGHT / f[-1]
+    uvmax = 0.4 * (N / 2) / (cell * N) * lam
+    uvw = np.zeros((40, 3))
+    uvw[:, 0] = rng.uniform(-uvmax, uvmax, 40)
+    uvw[:, 1] = rng.uniform(-uvmax, uvmax, 40)
+    vis = np.zeros((40, 2, 1), np.complex64)
+    noise = (0.5 + rng.random((40, 2, 1))).astype(np.float32).astype(np.complex64)
+    ch = O.Chunk(uvw, f, vis, noise=noise)
+    ws = O.OracleWStack(2000.0, 9, precision="f64")
+    grids = ws.grid((N, N, 1, 1), (cell, cell), ch, dopcf=True)
+    assert [iw for iw in range(9) if np.abs(grids[iw]).max() > 0] == [4] and ws.pcf_scale(4) == 1.0
+    one = O.OracleGridder("SphFunc", "f64")
+    one.initialise_grid((N, N, 1, 1), (cell, cell), dopcf=True)
+    one.grid(ch)
+    assert np.abs(one.grid_).max() > 0
+    a, b = ws.finalise_grid(), one.finalise_grid() / (N * N)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))

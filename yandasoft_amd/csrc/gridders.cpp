@@ -1287,7 +1287,6 @@ class WStackGridder : public SphFuncGridder {
  protected:
   int nGridPlanes() const override { return ws_.nw; }
   CIndexDesc cindex(const ChunkDev&) const override {
-    YG_CHECK(!dopcf_, 1, "the w-stacking gridder has no preconditioner (PCF) mode here");
     CIndexDesc d;
     d.kind = 3;
     d.nw = ws_.nw;
@@ -1306,12 +1305,49 @@ class WStackGridder : public SphFuncGridder {
     cellx = (float)(1.0 / (float(shape_[0]) * uvcell_[0]));
     celly = (float)(1.0 / (float(shape_[1]) * uvcell_[1]));
   }
+  // the planes that received samples: only those are transformed (WStackVisGridder.cc:236,272
+  // skip planes whose largest amplitude is 0)
+  std::vector<int> usedPlanes() {
+    const size_t ng = gridPlaneElems();
+    DevBuf flags((size_t)ws_.nw * sizeof(int));
+    launch_fill_u32(*ctx_, flags.get(), ws_.nw, 0u);
+    for (int iw = 0; iw < ws_.nw; ++iw)
+      launch_any_nonzero(*ctx_, reinterpret_cast<const float*>(grid_.as<float2>() + ng * iw), 2 * ng,
+                         flags.as<int>() + iw);
+    std::vector<int> f(ws_.nw, 0), used;
+    YG_HIP(hipMemcpyAsync(f.data(), flags.get(), f.size() * sizeof(int), hipMemcpyDeviceToHost, ctx_->stream));
+    YG_HIP(hipStreamSynchronize(ctx_->stream));
+    for (int iw = 0; iw < ws_.nw; ++iw)
+      if (f[iw]) used.push_back(iw);
+    return used;
+  }
   void gridsToImage(const double* cx, const double* cy, float* dout) override {
     const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
     const size_t ng = gridPlaneElems();
+    const std::vector<int> used = usedPlanes();
+    if (dopcf_) {
+      // WStackVisGridder::finaliseGrid's preconditioner form (:222-262): the planes' uv grids,
+      // their imaginary parts (the w support of the preconditioner kernel, gridded as 1 +- 3i)
+      // scaled by the would-be W-projection kernel size over 3 (updatew, :173-209), are summed and
+      // transformed once; the image is the real part of the normalised inverse transform, with
+      // no correction and no scaling
+      scratch_.resize(ng * sizeof(double2));
+      launch_fill_u32(*ctx_, scratch_.get(), 4 * ng, 0u);
+      const double norm = 1.0 / ((double)nxp * (double)nyp);
+      for (int iw : used) {
+        const float wThetaPix = (float)(std::fabs(ws_.wterm(iw)) / (uvcell_[0] * uvcell_[0]));
+        float wKernelPix = wThetaPix < 1 ? 3.f : (float)(6 + 1.14 * wThetaPix);
+        wKernelPix /= 3.f;
+        wstack_pcf_stack(*ctx_, grid_.as<float2>() + ng * iw, nxp, nyp, np, (double)wKernelPix, norm,
+                         scratch_.as<double2>());
+      }
+      stack_to_image_z(*ctx_, scratch_.as<double2>(), nxp, nyp, np, img_[0], img_[1], dout, plan_);
+      YG_HIP(hipStreamSynchronize(ctx_->stream));
+      return;
+    }
     DevBuf acc(ng * sizeof(double));
     launch_fill_u32(*ctx_, acc.get(), 2 * ng, 0u);
-    for (int iw = 0; iw < ws_.nw; ++iw) {
+    for (int iw : used) {
       float w2pi, cellx, celly;
       screen(iw, w2pi, cellx, celly);
       wstack_accumulate(*ctx_, grid_.as<float2>() + ng * iw, nxp, nyp, np, w2pi, cellx, celly, scratch_.as<double2>(),

@@ -317,6 +317,43 @@ void wstack_accumulate(const Ctx& c, const float2* grid, int nxp, int nyp, int n
   YG_HIP(hipGetLastError());
 }
 
+namespace {
+// stack += norm * cb(X, Y) * (Re g, im_scale * Im g): one plane of the w-stacking
+// preconditioner's uv stack (WStackVisGridder::updatew, WStackVisGridder.cc:173-209, scales the
+// imaginary part where |g| > 0; zero pixels are unchanged either way), with the centring signs
+// and the inverse transform's normalisation applied here (linear)
+__global__ void k_wstack_pcf_stack(const float2* __restrict__ g, int nxp, int nyp, double im_scale, double norm,
+                                   double2* __restrict__ stack) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y;
+  if (X >= nxp) return;
+  const size_t i = (size_t)blockIdx.z * nxp * nyp + (size_t)Y * nxp + X;
+  const float2 v = g[i];
+  const double s = cb_sign(X, Y) * norm;
+  double2 a = stack[i];
+  a.x += s * (double)v.x;
+  a.y += s * (double)v.y * im_scale;
+  stack[i] = a;
+}
+}  // namespace
+
+void wstack_pcf_stack(const Ctx& c, const float2* grid, int nxp, int nyp, int nplanes, double im_scale, double norm,
+                      double2* stack) {
+  hipLaunchKernelGGL(k_wstack_pcf_stack, dim3((nxp + 255) / 256, nyp, nplanes), dim3(256), 0, c.stream, grid, nxp, nyp,
+                     im_scale, norm, stack);
+  YG_HIP(hipGetLastError());
+}
+
+// image[plane] = Re(backward transform of a stack that carries its centring signs), unpadded
+// centre, no correction
+void stack_to_image_z(const Ctx& c, double2* stack, int nxp, int nyp, int nplanes, int nx, int ny, float* out,
+                      PlanPtr& cache) {
+  exec_z2z(c, stack, nxp, nyp, nplanes, false, cache);
+  hipLaunchKernelGGL(k_finalise_image_z, dim3((nx + 255) / 256, ny, nplanes), dim3(256), 0, c.stream, stack, nxp, nyp,
+                     nullptr, nullptr, nx, ny, (nxp - nx) / 2, (nyp - ny) / 2, (double)global_sign(nxp, nyp), out);
+  YG_HIP(hipGetLastError());
+}
+
 void wstack_finalise(const Ctx& c, const double* acc, int nxp, int nyp, int nplanes, const double* ccfx,
                      const double* ccfy, int nx, int ny, float* out) {
   hipLaunchKernelGGL(k_wstack_final, dim3((nx + 255) / 256, ny, nplanes), dim3(256), 0, c.stream, acc, nxp, nyp, ccfx,

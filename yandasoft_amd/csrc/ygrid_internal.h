@@ -428,6 +428,14 @@ void wstack_accumulate(const Ctx&, const float2* grid, int nxp, int nyp, int npl
 // image = ccf * acc, the unpadded centre
 void wstack_finalise(const Ctx&, const double* acc, int nxp, int nyp, int nplanes, const double* ccfx,
                      const double* ccfy, int nx, int ny, float* out);
+// preconditioner stack (WStackVisGridder.cc:173-209,222-262): stack += norm * centring signs *
+// (Re g, im_scale * Im g) for one w plane's grids
+void wstack_pcf_stack(const Ctx&, const float2* grid, int nxp, int nyp, int nplanes, double im_scale, double norm,
+                      double2* stack);
+// image = Re(unnormalised backward transform of a stack carrying its centring signs), the
+// unpadded centre, no correction
+void stack_to_image_z(const Ctx&, double2* stack, int nxp, int nyp, int nplanes, int nx, int ny, float* out,
+                      PlanPtr& cache);
 // grid = centred unnormalised forward transform of conj(screen * ccf * padded(model))
 void wstack_model_to_grid(const Ctx&, const float* model, int nx, int ny, int nplanes, const double* ccfx,
                           const double* ccfy, int nxp, int nyp, float w2pi, float cellx, float celly, double2* scratch,
