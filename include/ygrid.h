@@ -90,6 +90,11 @@ typedef struct {
    * calibration degrids source s into rows s*nrow .. (s+1)*nrow-1 of a cube of nsource*nrow rows
    * (TableVisGridder.cc:567,861, itsSourceIndex): pass vis_out at row s*nrow and the cube's rows */
   int vis_out_rows;
+  /* feed1PA: nrow float32 parallactic angle of each row's first feed (rad) or NULL (= 0).  Read
+   * by A-projection with an illumination that rotates with it (ATCA with feed legs): the cache of
+   * convolution functions is rebuilt when a row's angle differs from the cached one by more than
+   * the gridder's patolerance (AProjectGridderBase.cc:202-253, AWProjectVisGridder.cc:369) */
+  const float *feed1_pa;
 } ygrid_chunk;
 
 /* Counters reported by TableVisGridder's destructor (TableVisGridder.cc:182-317). */
@@ -121,6 +126,10 @@ typedef struct {
   /* passes that ran the small-support kernels (CF support <= the context's small-support limit,
    * ygrid_ctx_set_small_support) instead of the 64x64 tile kernels */
   double small_support_passes;
+  /* A-projection CF cache (AProjectGridderBase::updateStats, AProjectGridderBase.cc:117-140,
+   * 272-282): calls of initConvolutionFunction, feed/field CF sets generated, and those of them
+   * generated after the parallactic angle or the frequencies invalidated the cache */
+  double cf_iterations, cf_generations, cf_generations_pa, cf_generations_freq;
 } ygrid_stats;
 
 /* ---- library / context ---- */
@@ -146,7 +155,9 @@ int ygrid_ctx_clear_cache(ygrid_ctx *ctx);
 int ygrid_ctx_set_small_support(ygrid_ctx *ctx, int max_support);
 
 /* ---- factory: VisGridderFactory::make / createGridder (VisGridderFactory.cc:112-286) ----
- * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUAWProject".
+ * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUWStack", "GPUAWProject" (illumination "disk" or
+ * "ATCA" with the illumination.* keys of AProjectGridderBase::makeIllumination,
+ * AProjectGridderBase.cc:290-348).
  * parset: the gridder.<name>. subset as "key=value" lines (';' also separates); the global keys
  * of ygrid_gridder_configure may be given here too. */
 int ygrid_gridder_create(ygrid_ctx *ctx, const char *name, const char *parset, ygrid_gridder **out);

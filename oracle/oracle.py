@@ -38,6 +38,15 @@ Restated reference functions (file:line in alxndrwllmsn/yandasoft):
                                  offset pattern, brute-force centre-relative DFTs of the whole CF
                                  chain, the w = 0 symmetries, two-tap and taper closed forms of the
                                  weights and a band-limited closed form of the padding.
+* ``atca_jamesian`` / ``atca_pattern`` / ``OracleGridder._validate_aw`` --
+                                 ATCAIllumination::jamesian / getPattern (ATCAIllumination.cc:
+                                 119-331) and AProjectGridderBase::validateCFCache / updateStats
+                                 (AProjectGridderBase.cc:202-282).  Pinned by the disk lattice with
+                                 every effect off, unit power normalisation, the quadratic
+                                 defocusing phase, the documented leg / wedge geometry in polar
+                                 form, 90-degree rotation with the parallactic angle and a chunk
+                                 sequence for the cache policy.  The Jamesian's constants are
+                                 transcribed and parity unpinned beyond the taper's continuity.
 * ``frequency_map``          -- FrequencyMapper::setupMapping (FrequencyMapper.cc:63-97).
 * ``wstack_screen`` / ``OracleWStack`` -- WStackVisGridder::multiply / initIndices / gIndex /
                                  finaliseGrid / initialiseDegrid (WStackVisGridder.cc:71-97,134-171,
@@ -535,6 +544,27 @@ class AWProjectParams(WProjectParams):
     spheroidalweightscutoff: float = 0.0
     diameter: float = 12.0
     blockage: float = 0.0
+    # makeIllumination (AProjectGridderBase.cc:290-348): "disk" or "ATCA" with its parset defaults
+    illumination: str = "disk"
+    tapering: bool = True
+    defocusing: float = 0.0                # rad, maximum defocusing phase
+    feedlegs: bool = True
+    feedlegs_width: float = 1.8            # m
+    feedlegs_rotation: float = math.pi / 4  # rad
+    feedlegs_shadowing: float = 0.75
+    wedges: bool = True
+    wedges_shadowing: tuple = (0.6, 0.5)
+    wedges_angle: float = 15.0 * math.pi / 180.0
+    wedges_startradius: float = 3.5        # m
+    # validateCFCache tolerances (AProjectGridderBase.tcc:53-115): patolerance (rad) and
+    # freqtolerance (None: "infinite", the frequency check off)
+    patolerance: float = 0.1
+    freqtolerance: float | None = 1e-6
+
+    def symmetric(self) -> bool:
+        """IBasicIllumination::isSymmetric: the disk, and ATCA without feed legs
+        (ATCAIllumination.cc:333-340)."""
+        return self.illumination != "ATCA" or not self.feedlegs
 
 
 def direction_lm(beam, centre):
@@ -568,10 +598,96 @@ def disk_pattern(nu, nv, ucell, vcell, os_, freq, diameter, blockage, l=0.0, m=0
     return pat * np.float32(np.float32(nu) * np.float32(nv) / np.float32(s))
 
 
+def atca_jamesian(fr):
+    """ATCAIllumination::jamesian with innerJamesian / outerJamesian (ATCAIllumination.cc:119-157):
+    the reference's empirical taper of the ATCA dish, as a function of the fractional radius."""
+    fr = np.asarray(fr, dtype=np.float64)
+    dsub = 1.8 / 11.0
+
+    def inner(x):
+        return 0.58 + 0.27 * np.arctan(130 * (x - 1.1 * dsub))
+
+    def outer(x):
+        return 1.0 - 0.95 * np.exp(8.0 * (x * x - 1.0))
+
+    t = 0.4
+    x = np.minimum(fr, 1.0)   # beyond the rim the value is 0; clipping only avoids exp overflow
+    v = np.where(x < t, inner(x) / inner(t), outer(x) / outer(t)) * (1.0 - 0.3 * (1.0 - x))
+    return np.where(fr > 1.0, 0.0, v)
+
+
+def atca_pattern(nu, nv, ucell, vcell, os_, freq, prm: AWProjectParams, l=0.0, m=0.0, pa=0.0):
+    """ATCAIllumination::getPattern (ATCAIllumination.cc:183-331): returns [iv, iu] complex64.
+    The value is built in double (taper x defocusing phase, then each feed-leg and wedge shadow
+    covering the cell in turn), rounded to complex float and multiplied by the pointing phase
+    slope in complex float; the whole pattern is scaled by nu nv / sqrt(sum |value|^2)."""
+    cellU, cellV = ucell / os_, vcell / os_
+    if abs(abs(cellU / cellV) - 1.0) >= 1e-7:
+        raise ValueError("Rectangular cells are not supported")
+    lS, mS = 2 * math.pi * cellU * l, 2 * math.pi * cellV * m
+    cell = abs(cellU * (C_LIGHT / freq))
+    rmax2 = (prm.diameter / (2.0 * cell)) ** 2
+    rmin2 = (prm.blockage / (2.0 * cell)) ** 2
+    if not (nu * nu > rmax2 or nv * nv > rmax2):
+        raise ValueError("pattern buffer too small for the dish")
+    ou = (np.arange(nu) - nu / 2.0)[None, :]
+    ov = (np.arange(nv) - nv / 2.0)[:, None]
+    r2 = ov ** 2 + ou ** 2
+    inside = (r2 >= rmin2) & (r2 <= rmax2)
+    val = np.ones(r2.shape, dtype=np.complex128)
+    if prm.tapering:
+        frs = r2 / rmax2
+        extra = prm.defocusing * frs
+        val = atca_jamesian(np.sqrt(frs)) * (np.cos(extra) + 1j * np.sin(extra))
+    if prm.feedlegs:
+        ang = prm.feedlegs_rotation + pa
+        ca, sa = math.cos(ang), math.sin(ang)
+        dU = ca * ou + sa * ov
+        dV = -sa * ou + ca * ov
+        half = (prm.feedlegs_width / 2.0) / cell
+        s = prm.feedlegs_shadowing
+        val = np.where((np.abs(dU) < half) | (np.abs(dV) < half), val * s, val)
+        if prm.wedges:
+            ws = list(prm.wedges_shadowing)
+            if len(ws) == 1:
+                ws = ws * 2
+            t = math.tan(prm.wedges_angle)
+            r0 = prm.wedges_startradius / cell
+            val = np.where((dU > r0) & (np.abs(dV) < t * np.abs(dU - r0)), val * (s * ws[0]), val)
+            val = np.where((dU < -r0) & (np.abs(dV) < t * np.abs(dU + r0)), val * (s * ws[0]), val)
+            val = np.where((dV > r0) & (np.abs(dU) < t * np.abs(dV - r0)), val * (s * ws[1]), val)
+            val = np.where((dV < -r0) & (np.abs(dU) < t * np.abs(dV + r0)), val * (s * ws[1]), val)
+    val = np.where(inside, val, 0)
+    total = float(np.sum(val.real ** 2 + val.imag ** 2))
+    if not total > 0.0:
+        raise ValueError("Integral of the pattern should be non-zero")
+    f32 = np.float32
+    ph = lS * ou + mS * ov
+    a, b = val.real.astype(f32), val.imag.astype(f32)
+    c, d = np.cos(ph).astype(f32), (-np.sin(ph)).astype(f32)
+    # complex float product, each product and sum rounded on its own
+    re = (a * c).astype(f32) - (b * d).astype(f32)
+    im = (a * d).astype(f32) + (b * c).astype(f32)
+    scale = f32(float(nu) * float(nv) / math.sqrt(total))
+    pat = np.where(inside, (re * scale) + 1j * (im * scale), 0)
+    return pat.astype(np.complex64)
+
+
+def illumination_pattern(nu, nv, ucell, vcell, os_, freq, prm: AWProjectParams, l=0.0, m=0.0, pa=0.0):
+    """IBasicIllumination::getPattern of the configured model (makeIllumination,
+    AProjectGridderBase.cc:290-348)."""
+    if prm.illumination == "disk":
+        return disk_pattern(nu, nv, ucell, vcell, os_, freq, prm.diameter, prm.blockage, l, m)
+    if prm.illumination == "ATCA":
+        return atca_pattern(nu, nv, ucell, vcell, os_, freq, prm, l, m, pa)
+    raise ValueError("Unknown illumination type " + prm.illumination)
+
+
 def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_of_feed, field=0, psf=False,
                      state=None, pcf=False):
     """AWProjectVisGridder::initConvolutionFunction for the feeds in ``lm_of_feed`` (dict feed ->
-    (l, m) beam offset) of one field.  ``state`` carries the CF cache and the common support
+    (l, m) beam offset, or (l, m, pa) with the parallactic angle of the feed's first row) of one
+    field.  ``state`` carries the CF cache and the common support
     between calls.  Returns the state: {'planes': list, 'support': int}.  pcf: the preconditioner
     form, every plane 3x3 with (1, support) at the centre (AWProjectVisGridder.cc:511-516)."""
     os_ = prm.oversample
@@ -602,10 +718,12 @@ def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_o
         region &= inqy[:, None] & inq[None, :]
         taper = np.zeros((ny, nx))
         taper[np.ix_(inqy, inq)] = np.outer(ty, tx)
-    for feed, (l, m) in lm_of_feed.items():
+    for feed, geo in lm_of_feed.items():
+        l, m = geo[0], geo[1]
+        pa = 0.0 if prm.symmetric() or len(geo) < 3 else geo[2]
         for chan in range(nchan):
-            pat = disk_pattern(nx, ny, uvcell[0], uvcell[1], os_, freqs[chan], prm.diameter, prm.blockage,
-                               0.0 if psf else l, 0.0 if psf else m)
+            pat = illumination_pattern(nx, ny, uvcell[0], uvcell[1], os_, freqs[chan], prm,
+                                       0.0 if psf else l, 0.0 if psf else m, pa)
             A = fft_centred(pat.astype(np.complex128), False)
             A = A.astype(np.complex64).astype(np.complex128)
             wt = (A * np.conj(A)) * taper
@@ -872,6 +990,7 @@ class Chunk:
     feed1: np.ndarray | None = None  # (nrow,) int
     pointing1: np.ndarray | None = None  # (nrow, 2) pointingDir1 (lon, lat) radians
     dish1: np.ndarray | None = None      # (nrow, 2) dishPointing1, or None (= pointing1)
+    feed1_pa: np.ndarray | None = None   # (nrow,) float32 feed1PA (rad), or None (= 0)
 
     @property
     def nrow(self):
@@ -906,6 +1025,10 @@ class OracleGridder:
         self.aw_done = set()
         self.aw_state = None
         self.aw_nchan = None     # AWProject: channels of the CF cache (set before initialise_grid)
+        self.aw_cf_pa = np.float32(0.0)  # itsCFParallacticAngle
+        self.aw_cached_freq = None       # itsCachedFrequencies
+        self.aw_invalid = (False, False)  # itsCFInvalidDueToPA / DueToFreq of the last chunk
+        self.aw_stats = [0, 0, 0, 0]      # iterations, generations, due to PA, due to frequency
         self.counters = _Counters()
         self.model_empty = False
         self.max_sep = -1.0           # MaxPointingSeparation (radians; <= 0: off)
@@ -940,18 +1063,53 @@ class OracleGridder:
         else:
             raise ValueError(self.kind)
 
+    def _validate_aw(self, chunk: Chunk):
+        """AProjectGridderBase::validateCFCache (AProjectGridderBase.cc:202-253): a pattern that
+        rotates with the parallactic angle drops the whole cache at the first row whose angle
+        differs from the cached one (float difference) by more than patolerance and caches that
+        row's angle; otherwise frequencies that moved by more than freqtolerance (relative) drop
+        it.  The frequencies are cached whenever the cache is dropped and the check is on."""
+        prm = self.wp
+        inv_pa = inv_f = False
+        if not prm.symmetric() and chunk.feed1_pa is not None:
+            pas = np.asarray(chunk.feed1_pa, dtype=np.float32)
+            for row in range(chunk.nrow):
+                if float(abs(np.float32(pas[row] - self.aw_cf_pa))) > prm.patolerance:
+                    inv_pa = True
+                    self.aw_cf_pa = np.float32(pas[row])
+                    self.aw_done.clear()
+                    break
+        freq = np.asarray(chunk.freq, dtype=np.float64)
+        if not inv_pa and prm.freqtolerance is not None:
+            c = self.aw_cached_freq
+            inv_f = c is None or len(c) != len(freq) or bool(np.any(np.abs(c - freq) / freq > prm.freqtolerance))
+            if inv_f:
+                self.aw_done.clear()
+        if (inv_pa or inv_f) and prm.freqtolerance is not None:
+            self.aw_cached_freq = freq.copy()
+        self.aw_invalid = (inv_pa, inv_f)
+
     def _init_aw(self, chunk: Chunk, psf: bool, pcf: bool = False):
-        """AW: CFs for the feeds of this chunk that are not in the cache yet (one field)."""
+        """AW: CFs for the feeds of this chunk that are not in the cache yet (one field), each
+        with its first row's parallactic angle (AWProjectVisGridder.cc:366-380); statistics as
+        AProjectGridderBase::updateStats (AProjectGridderBase.cc:272-282)."""
+        self._validate_aw(chunk)
         feeds = chunk.feed1 if chunk.feed1 is not None else np.zeros(chunk.nrow, dtype=np.int32)
+        pas = chunk.feed1_pa if chunk.feed1_pa is not None else np.zeros(chunk.nrow, dtype=np.float32)
         new = {}
-        for f in feeds:
+        for f, pa in zip(feeds, pas):
             f = int(f)
             if f not in self.aw_done and f not in new:
-                new[f] = self.aw_lm.get(f, (0.0, 0.0))
+                new[f] = tuple(self.aw_lm.get(f, (0.0, 0.0))[:2]) + (float(np.float32(pa)),)
         if new or self.aw_state is None:
             self.aw_state = awproject_planes(self.wp, self.nx, self.ny, self.uvcell, chunk.freq, new, 0, psf,
                                              self.aw_state, pcf=pcf)
             self.aw_done.update(new.keys())
+        st = self.aw_stats
+        st[0] += 1
+        st[1] += len(new)
+        st[2] += len(new) if self.aw_invalid[0] else 0
+        st[3] += len(new) if self.aw_invalid[1] else 0
         self.support = self.aw_state["support"]
         self.cf = CFCache(self.aw_state["planes"], self.wp.oversample)
 

@@ -162,10 +162,12 @@ ygrid_chunk GPUGridderAdapter::stage(const accessors::IConstDataAccessor& acc, b
   itsFeed.resize(nRow);
   itsPointing.resize(2 * nRow);
   itsDish.resize(2 * nRow);
+  itsPA.resize(nRow);
   for (casacore::uInt i = 0; i < nRow; ++i) {
     for (int k = 0; k < 3; ++k) itsUVW[3 * i + k] = uvw(i)(k);
     itsDelay[i] = delay(i);
     itsFeed[i] = acc.feed1()(i);
+    itsPA[i] = acc.feed1PA()(i);
     itsPointing[2 * i] = acc.pointingDir1()(i).getLong();
     itsPointing[2 * i + 1] = acc.pointingDir1()(i).getLat();
     itsDish[2 * i] = acc.dishPointing1()(i).getLong();
@@ -185,6 +187,8 @@ ygrid_chunk GPUGridderAdapter::stage(const accessors::IConstDataAccessor& acc, b
   c.feed1 = itsFeed.data();
   c.pointing1 = itsPointing.data();
   c.dish_pointing1 = itsDish.data();
+  // AProjectGridderBase.cc:212-219: the A-projection CF cache follows the feeds' parallactic angle
+  c.feed1_pa = itsPA.data();
   c.flag = itsFlag.data();
   // acc.stokes(): the visibilities' polarisation frame (TableVisGridder.cc:517-524)
   itsVisStokes.clear();

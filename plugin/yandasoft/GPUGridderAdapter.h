@@ -81,6 +81,7 @@ class GPUGridderAdapter : public TableVisGridder {
   // host staging of one accessor (reused across calls)
   std::vector<double> itsUVW, itsDelay, itsPointing, itsDish;
   std::vector<int32_t> itsFeed, itsImageStokes, itsVisStokes;
+  std::vector<float> itsPA;
   std::vector<uint8_t> itsFlag;
 };
 

@@ -820,3 +820,190 @@ def test_wstack_pcf_on_one_w0_plane_is_the_single_grid_preconditioner_over_n():
     assert np.abs(one.grid_).max() > 0
     a, b = ws.finalise_grid(), one.finalise_grid() / (N * N)
     assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+# ---- ATCAIllumination and the parallactic-angle / frequency cache policy ------------------------
+def _atca(**kw):
+    base = dict(illumination="ATCA", diameter=22.0, blockage=2.0)
+    base.update(kw)
+    return O.AWProjectParams(**base)
+
+
+_ATCA_GEOM = dict(nu=256, nv=256, ucell=8.0, vcell=8.0, os_=4, freq=1.4e9)  # 0.428 m cells, R = 25.7
+
+
+def _atca_pattern(prm, **kw):
+    g = dict(_ATCA_GEOM)
+    g.update(kw)
+    return O.atca_pattern(g["nu"], g["nv"], g["ucell"], g["vcell"], g["os_"], g["freq"], prm,
+                          kw.get("l", 0.0), kw.get("m", 0.0), kw.get("pa", 0.0))
+
+
+def _atca_cell_m():
+    g = _ATCA_GEOM
+    return g["ucell"] / g["os_"] * (O.C_LIGHT / g["freq"])
+
+
+def test_atca_without_taper_and_legs_is_the_disk_with_root_normalisation():
+    """ATCAIllumination.cc:253-330 with every effect off: 1 on the annulus lattice (the disk's),
+    scaled by nU nV / sqrt(count) (the disk divides by the count itself); a beam offset changes
+    only the phase."""
+    prm = _atca(tapering=False, feedlegs=False)
+    g = _ATCA_GEOM
+    disk = O.disk_pattern(g["nu"], g["nv"], g["ucell"], g["vcell"], g["os_"], g["freq"], 22.0, 2.0)
+    pat = _atca_pattern(prm)
+    count = np.count_nonzero(disk)
+    assert np.array_equal(pat != 0, disk != 0)
+    assert np.allclose(pat[pat != 0], g["nu"] * g["nv"] / math.sqrt(count), rtol=1e-6)
+    off = _atca_pattern(prm, l=0.003, m=-0.001)
+    assert np.allclose(np.abs(off[pat != 0]), g["nu"] * g["nv"] / math.sqrt(count), rtol=1e-6)
+    assert not np.allclose(off, pat)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(defocusing=1.3), dict(feedlegs=False, defocusing=-0.4),
+                                dict(wedges_shadowing=(0.2,), pa=0.7, l=0.002, m=0.001)])
+def test_atca_pattern_is_normalised_to_unit_power(kw):
+    """The normalisation divides by sqrt(sum |value|^2) (ATCAIllumination.cc:320-330): whatever the
+    taper, defocusing, shadows and beam offset, sum |pattern|^2 = (nU nV)^2."""
+    geo = {k: kw.pop(k) for k in ("pa", "l", "m") if k in kw}
+    pat = _atca_pattern(_atca(**kw), **geo).astype(np.complex128)
+    n2 = float(_ATCA_GEOM["nu"] * _ATCA_GEOM["nv"]) ** 2
+    assert abs(float(np.sum(np.abs(pat) ** 2)) / n2 - 1.0) < 1e-5
+
+
+def test_atca_taper_is_continuous_and_the_defocusing_phase_is_quadratic():
+    """simulateTapering (ATCAIllumination.h): the defocusing phase is the given value at the dish
+    edge and grows quadratically with the radius; the Jamesian taper joins its inner and outer
+    forms continuously at 0.4 of the radius (both normalised to 1 there, times 1 - 0.3 x 0.6) and
+    is zero beyond the rim."""
+    j = O.atca_jamesian(np.array([0.4 - 1e-12, 0.4, 0.4 + 1e-12, 1.0 + 1e-9, 3.0]))
+    assert abs(j[0] - 0.82) < 1e-9 and abs(j[1] - 0.82) < 1e-12 and abs(j[2] - 0.82) < 1e-9
+    assert j[3] == 0.0 and j[4] == 0.0
+    phi = 1.3
+    pat = _atca_pattern(_atca(feedlegs=False, defocusing=phi))
+    n = _ATCA_GEOM["nu"]
+    R = 11.0 / _atca_cell_m()
+    for r in (3, 10, 18, 25):   # along the u axis; no beam offset, so the phase is the defocusing's
+        v = pat[n // 2, n // 2 + r]
+        expect = phi * r * r / (R * R)
+        assert abs(np.angle(v) - expect) < 2e-6, r
+    # the rim cell of the u axis carries (nearly) the full defocusing phase
+    rim = int(math.floor(R))
+    assert abs(np.angle(pat[n // 2, n // 2 + rim]) - phi) < 0.1
+
+
+def _leg_geometry(ou, ov, t, half, r0, tan_a):
+    """The shadow model in polar form, from the documented geometry: two legs through the centre at
+    angles t (from u towards v) and t + 90 deg, each shadow a band of half width ``half``; a wedge
+    on each leg arm from radius r0 outwards, the arm plus or minus the opening angle about its
+    apex.  Returns (in a band, in an arm-t wedge, in an arm-(t+90) wedge)."""
+    r = math.hypot(ou, ov)
+    psi = math.atan2(ov, ou)
+    along1, across1 = r * math.cos(psi - t), r * math.sin(psi - t)   # along / across the leg at t
+    along2, across2 = r * math.cos(psi - t - math.pi / 2), r * math.sin(psi - t - math.pi / 2)
+    band = abs(across1) < half or abs(across2) < half
+    w1 = abs(along1) > r0 and abs(across1) < tan_a * (abs(along1) - r0)
+    w2 = abs(along2) > r0 and abs(across2) < tan_a * (abs(along2) - r0)
+    return band, w1, w2
+
+
+@pytest.mark.parametrize("pa", [0.0, 0.37, -1.1])
+def test_atca_feed_leg_shadows_follow_the_documented_geometry(pa):
+    """simulateFeedLegShadows / simulateFeedLegWedges (ATCAIllumination.h): shadows of the given
+    full width along the legs at rotation + pa, attenuated by the shadowing factor; wedge 1 (on
+    the legs at that angle) and wedge 2 (on the orthogonal pair) multiply by shadowing x their
+    own factor.  Without the taper every cell of the dish carries the product of the factors that
+    cover it."""
+    s, w1, w2 = 0.75, 0.6, 0.5
+    rot = 0.3
+    prm = _atca(tapering=False, feedlegs_rotation=rot, feedlegs_shadowing=s, wedges_shadowing=(w1, w2),
+                feedlegs_width=1.8, wedges_startradius=3.5, wedges_angle=math.radians(15.0))
+    pat = _atca_pattern(prm, pa=pa).astype(np.complex128)
+    cell = _atca_cell_m()
+    half, r0, tan_a = 0.9 / cell, 3.5 / cell, math.tan(math.radians(15.0))
+    n = _ATCA_GEOM["nu"]
+    unit = None
+    counts = [0, 0, 0]
+    for iv in range(n):
+        for iu in range(n):
+            v = pat[iv, iu]
+            if v == 0:
+                continue
+            band, a1, a2 = _leg_geometry(iu - n / 2, iv - n / 2, rot + pa, half, r0, tan_a)
+            f = (s if band else 1.0) * ((s * w1) if a1 else 1.0) * ((s * w2) if a2 else 1.0)
+            counts[0] += band
+            counts[1] += a1
+            counts[2] += a2
+            if unit is None and f == 1.0:
+                unit = v
+            assert abs(v.imag) < 1e-9 * abs(v.real)
+            if unit is not None:
+                assert abs(v.real / unit.real - f) < 1e-6, (iu, iv)
+    # each kind of region is populated: bands of ~2 x (2 half) x 2 R cells, four wedges
+    R = 11.0 / cell
+    assert abs(counts[0] / (2 * 2 * half * 2 * R) - 1.0) < 0.2
+    assert counts[1] > 50 and counts[2] > 50
+
+
+def test_atca_pattern_rotates_with_the_parallactic_angle():
+    """The pattern is defined in a frame turned by the parallactic angle: increasing pa by 90 deg
+    turns the whole (asymmetric, s1 != s2) shadow pattern by 90 deg from u towards v, so that
+    P(pa + 90)[(-dv, du)] = P(pa)[(du, dv)] on the lattice."""
+    prm = _atca(defocusing=0.8, wedges_shadowing=(0.9, 0.2))
+    n = _ATCA_GEOM["nu"]
+    a = _atca_pattern(prm, pa=0.4)
+    b = _atca_pattern(prm, pa=0.4 + math.pi / 2)
+    iv, iu = np.nonzero(a)
+    assert iv.min() > 0 and iu.min() > 0
+    # cell (du, dv) = (iu - n/2, iv - n/2) -> (-dv, du): column n - iv, row iu
+    rot = b[iu, n - iv]
+    assert np.allclose(rot, a[iv, iu], rtol=1e-5, atol=0)
+    assert not np.allclose(b[iv, iu], a[iv, iu])          # the pattern itself is not 4-fold symmetric
+    sym = _atca_pattern(_atca(feedlegs=False), pa=0.4)
+    assert np.array_equal(sym, _atca_pattern(_atca(feedlegs=False), pa=1.9))
+
+
+def _aw_oracle(prm, pas=None, freq=(1.4e9,)):
+    o = O.OracleGridder("AWProject", "f32", wproject=prm)
+    return o, O.Chunk(np.zeros((len(pas), 3)), np.asarray(freq, dtype=np.float64),
+                      np.zeros((len(pas), len(freq), 1), np.complex64), feed1_pa=np.asarray(pas, np.float32))
+
+
+def test_cf_cache_validation_by_parallactic_angle_and_frequency():
+    """AProjectGridderBase::validateCFCache (AProjectGridderBase.cc:202-253) as restated: the cache
+    is dropped at the first row whose angle is off the cached one by more than patolerance, and
+    that row's angle is cached; a symmetric illumination never checks angles; the frequency check
+    runs only when the angle did not drop the cache, and frequencies are cached on every drop."""
+    prm = _atca(patolerance=0.1, freqtolerance=1e-6)
+    o, ch = _aw_oracle(prm, pas=[0.05, -0.08])
+    o._validate_aw(ch)
+    assert o.aw_invalid == (False, True) and o.aw_cf_pa == 0.0          # first chunk: frequencies
+    ch.feed1_pa = np.array([0.05, 0.3, 0.9], np.float32)
+    o.aw_done = {0}
+    o._validate_aw(ch)
+    assert o.aw_invalid == (True, False) and o.aw_cf_pa == np.float32(0.3) and not o.aw_done
+    ch.feed1_pa = np.array([0.35, 0.25], np.float32)
+    o.aw_done = {0}
+    o._validate_aw(ch)                                                     # within tolerance of 0.3
+    assert o.aw_invalid == (False, False) and o.aw_done == {0}
+    ch.freq = np.array([1.4e9 * (1 + 5e-7)])
+    o._validate_aw(ch)
+    assert o.aw_invalid == (False, False)
+    ch.freq = np.array([1.4e9 * (1 + 5e-6)])
+    o._validate_aw(ch)
+    assert o.aw_invalid == (False, True) and not o.aw_done
+    ch.freq = np.array([1.5e9])
+    ch.feed1_pa = np.array([1.0], np.float32)
+    o._validate_aw(ch)                                                     # angle first; frequency cached
+    assert o.aw_invalid == (True, False) and o.aw_cached_freq[0] == 1.5e9
+    disk = O.AWProjectParams(diameter=12.0, blockage=1.0)
+    od, chd = _aw_oracle(disk, pas=[0.0, 2.0])
+    od._validate_aw(chd)
+    assert od.aw_invalid == (False, True)
+    chd.feed1_pa = np.array([3.0], np.float32)
+    od._validate_aw(chd)
+    assert od.aw_invalid == (False, False)
+    nofreq = _atca(freqtolerance=None)
+    on, chn = _aw_oracle(nofreq, pas=[0.0])
+    on._validate_aw(chn)
+    assert on.aw_invalid == (False, False) and on.aw_cached_freq is None

@@ -44,7 +44,8 @@ class _Chunk(ctypes.Structure):
                 ("uvw", ctypes.c_void_p), ("delay", ctypes.c_void_p), ("freq", ctypes.c_void_p),
                 ("vis", ctypes.c_void_p), ("vis_out", ctypes.c_void_p), ("noise", ctypes.c_void_p),
                 ("flag", ctypes.c_void_p), ("feed1", ctypes.c_void_p), ("pointing1", ctypes.c_void_p),
-                ("stokes", ctypes.c_void_p), ("dish_pointing1", ctypes.c_void_p), ("vis_out_rows", ctypes.c_int)]
+                ("stokes", ctypes.c_void_p), ("dish_pointing1", ctypes.c_void_p), ("vis_out_rows", ctypes.c_int),
+                ("feed1_pa", ctypes.c_void_p)]
 
 
 class Stats(ctypes.Structure):
@@ -60,7 +61,9 @@ class Stats(ctypes.Structure):
                 ("degrid_waves", ctypes.c_int), ("degrid_pipeline", ctypes.c_int),
                 ("cf_precision", ctypes.c_int), ("vis_precision", ctypes.c_int), ("learned_cf_active", ctypes.c_int),
                 ("degrid_matrix", ctypes.c_int), ("rows_rejected_pointing", ctypes.c_double),
-                ("small_support_passes", ctypes.c_double)]
+                ("small_support_passes", ctypes.c_double),
+                ("cf_iterations", ctypes.c_double), ("cf_generations", ctypes.c_double),
+                ("cf_generations_pa", ctypes.c_double), ("cf_generations_freq", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):
@@ -153,6 +156,7 @@ class Chunk:
     stokes: tuple | None = None  # casacore Stokes codes of the visibility pols (None: the image's)
     dish_pointing1: object = None  # (nrow, 2) radians or None (= pointing1)
     vis_out_rows: int = 0          # rows of the cube vis_out points into (0: nrow)
+    feed1_pa: object = None        # (nrow,) float32 parallactic angle of each row's first feed, rad
     nrow: int = 0
     nchan: int = 0
     npol: int = 0
@@ -165,10 +169,13 @@ class Chunk:
         self.npol = int(cube.shape[0]) if cube is not None else self.npol
         if not self.device:
             for name in ("uvw", "freq", "vis", "vis_out", "noise", "flag", "delay", "feed1", "pointing1",
-                         "dish_pointing1"):
+                         "dish_pointing1", "feed1_pa"):
                 a = getattr(self, name)
                 if a is not None:
                     assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], f"{name} must be C-contiguous"
+            if self.feed1_pa is not None:
+                assert self.feed1_pa.dtype == np.float32 and self.feed1_pa.shape == (self.nrow,), \
+                    "feed1_pa must be (nrow,) float32"
 
     def c(self) -> _Chunk:
         self._stokes = None if self.stokes is None else np.asarray(self.stokes, dtype=np.int32)
@@ -176,7 +183,7 @@ class Chunk:
                       _ptr(self.uvw), _ptr(self.delay), _ptr(self.freq), _ptr(self.vis), _ptr(self.vis_out),
                       _ptr(self.noise), _ptr(self.flag), _ptr(self.feed1), _ptr(self.pointing1),
                       None if self._stokes is None else self._stokes.ctypes.data, _ptr(self.dish_pointing1),
-                      int(self.vis_out_rows))
+                      int(self.vis_out_rows), _ptr(self.feed1_pa))
 
 
 class Context:

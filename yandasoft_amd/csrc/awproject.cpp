@@ -1,12 +1,15 @@
-// AWProject gridder: A-projection (disk illumination) x W-projection convolution functions
-// (AWProjectVisGridder.cc, AProjectGridderBase.cc/.tcc, DiskIllumination.cc), built on the device.
+// AWProject gridder: A-projection (disk or ATCA illumination) x W-projection convolution functions
+// (AWProjectVisGridder.cc, AProjectGridderBase.cc/.tcc, DiskIllumination.cc, ATCAIllumination.cc),
+// built on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <sstream>
 
 #include "gridders.h"
+#include "illumination.h"
 
 namespace ygrid {
 
@@ -43,13 +46,46 @@ class AWProjectGridder : public Gridder {
     YG_CHECK(os_ > 0 && maxsupport_ > 0 && maxsupport_ % 2 == 0, 1, "bad oversample / maxsupport");
     YG_CHECK(maxfeeds_ > 0 && maxfields_ > 0, 1, "maxfeeds and maxfields must be positive");
     if (!abs_cutoff_) YG_CHECK(cutoff_ > 0.0 && cutoff_ < 1.0, 1, "Cutoff must be in (0, 1)");
-    // makeIllumination (AProjectGridderBase.cc:290-303): the disk model
+    // makeIllumination (AProjectGridderBase.cc:290-348): the disk and the ATCA models
     const std::string ill = p.str("illumination", "disk");
-    YG_CHECK(ill == "disk", 1, "only the disk illumination is available in GPUAWProject, not " + ill);
-    YG_CHECK(p.has("diameter") && p.has("blockage"), 1, "disk illumination needs diameter and blockage");
+    YG_CHECK(ill == "disk" || ill == "ATCA", 1,
+             "GPUAWProject has the disk and ATCA illuminations, not " + ill + " (Unknown illumination type)");
+    YG_CHECK(p.has("diameter") && p.has("blockage"), 1, ill + " illumination needs diameter and blockage");
     diameter_ = quantity(p.str("diameter", "12m"), "m");
     blockage_ = quantity(p.str("blockage", "0m"), "m");
     YG_CHECK(diameter_ > 0 && blockage_ >= 0 && diameter_ > blockage_, 1, "bad disk diameter/blockage");
+    if (ill == "ATCA") {
+      atca_ = true;
+      // ATCAIllumination::simulateTapering / simulateFeedLegShadows / simulateFeedLegWedges and
+      // their parset defaults (AProjectGridderBase.cc:305-345, ATCAIllumination.cc:37-74)
+      a_taper_ = p.flag("illumination.tapering", true);
+      if (a_taper_) a_defocus_ = quantity(p.str("illumination.tapering.defocusing", "0rad"), "rad");
+      a_legs_ = p.flag("illumination.feedlegs", true);
+      if (a_legs_) {
+        a_leg_half_ = quantity(p.str("illumination.feedlegs.width", "1.8m"), "m") / 2.;
+        a_leg_rot_ = quantity(p.str("illumination.feedlegs.rotation", "45deg"), "rad");
+        a_leg_shadow_ = p.f64("illumination.feedlegs.shadowing", 0.75);
+        YG_CHECK(a_leg_shadow_ <= 1. && a_leg_shadow_ >= 0., 1,
+                 "shadowingFactor is supposed to be from [0,1] interval");
+        a_wedges_ = p.flag("illumination.feedlegs.wedges", true);
+        if (a_wedges_) {
+          std::vector<double> ws = doubles(p.str("illumination.feedlegs.wedges.shadowing", "[0.6,0.5]"));
+          YG_CHECK(!ws.empty() && ws.size() < 3, 1,
+                   "illumination.feedlegs.wedges.shadowing can have either 1 or 2 elements only");
+          if (ws.size() == 1) ws.push_back(ws[0]);
+          a_wedge_s1_ = ws[0];
+          a_wedge_s2_ = ws[1];
+          a_wedge_angle_ = quantity(p.str("illumination.feedlegs.wedges.angle", "15deg"), "rad");
+          a_wedge_r0_ = quantity(p.str("illumination.feedlegs.wedges.startradius", "3.5m"), "m");
+          YG_CHECK(a_wedge_s1_ <= 1. && a_wedge_s1_ >= 0. && a_wedge_s2_ <= 1. && a_wedge_s2_ >= 0., 1,
+                   "wedge shadowing factors are supposed to be from [0,1] interval");
+          YG_CHECK(a_wedge_angle_ < kPi / 2. && a_wedge_angle_ > 0, 1,
+                   "Opening angle is supposed to be from the (0, pi/2) interval");
+          YG_CHECK(a_wedge_r0_ < diameter_ / 2., 1, "the wedge starting radius must lie inside the dish");
+        }
+        YG_CHECK(a_leg_half_ < diameter_ / 2., 1, "the feed-leg width must be less than the dish diameter");
+      }
+    }
     const std::string smp = p.str("wsampling", "linear");
     YG_CHECK(smp == "linear", 1, "GPUAWProject supports linear w sampling");
     wmaxclip_ = p.flag("wmaxclip", false);
@@ -92,9 +128,23 @@ class AWProjectGridder : public Gridder {
     const int half = (nw_ - 1) / 2;
     return double(plane - half) * (nw_ > 1 ? wmax_ / double(half) : 1.0);
   }
+  // ATCAIllumination::isSymmetric: without feed legs the pattern does not rotate with the
+  // parallactic angle; the disk never does
+  bool symmetric() const { return !atca_ || !a_legs_; }
+  // a parset vector "[a, b, ...]" (or "a, b") of numbers
+  static std::vector<double> doubles(std::string s) {
+    for (char& ch : s)
+      if (ch == '[' || ch == ']' || ch == ',') ch = ' ';
+    std::istringstream is(s);
+    std::vector<double> v;
+    double x;
+    while (is >> x) v.push_back(x);
+    YG_CHECK(is.eof(), 1, "bad number list: " + s);
+    return v;
+  }
   void indexField(const ChunkDev& acc);
   void validateCache(const ChunkDev& acc);
-  void generateFeed(const ChunkDev& acc, int feed, double lon, double lat);
+  void generateFeed(const ChunkDev& acc, int feed, double lon, double lat, double pa);
   void ensureTable(int S);
   void writePCFPlanes(const Ctx& c, DevBuf& buf, int nx, int ny, int cnt, int z0, int chan, int nchan, int feed);
   int pcf_support_ = 0;  // preconditioner form: the support found on the first plane
@@ -104,6 +154,12 @@ class AWProjectGridder : public Gridder {
   int nw_ = 65, os_ = 8, maxsupport_ = 128, limitsupport_ = 0, maxfeeds_ = 1, maxfields_ = 1;
   bool freqdep_ = true, sph_taper_ = false, abs_cutoff_ = false, wmaxclip_ = false;
   int nchan_cf_ = 0, field_ = 0, last_field_ = -1;
+  // ATCA illumination (atca_ false: the disk)
+  bool atca_ = false, a_taper_ = false, a_legs_ = false, a_wedges_ = false;
+  double a_defocus_ = 0, a_leg_half_ = 0, a_leg_rot_ = 0, a_leg_shadow_ = 1, a_wedge_s1_ = 1, a_wedge_s2_ = 1,
+         a_wedge_angle_ = 0, a_wedge_r0_ = 0;
+  float cf_pa_ = 0.f;  // itsCFParallacticAngle (float): the angle the cache was last built for
+  bool invalid_pa_ = false, invalid_freq_ = false;  // itsCFInvalidDueToPA / DueToFreq of this chunk
   std::vector<char> done_;
   std::vector<double> pointings_;
   std::vector<double> cached_freq_;
@@ -130,17 +186,30 @@ void AWProjectGridder::indexField(const ChunkDev& acc) {
   pt[1] = lat;
 }
 
-// AProjectGridderBase::validateCFCache (AProjectGridderBase.cc:202-253); the disk is symmetric so
-// the parallactic angle never invalidates the cache.
+// AProjectGridderBase::validateCFCache (AProjectGridderBase.cc:202-253): a pattern that rotates
+// with the parallactic angle invalidates the whole cache at the first row whose angle differs from
+// the cached one by more than the tolerance (and caches that row's angle); otherwise a change of
+// the frequencies beyond the relative tolerance does.
 void AWProjectGridder::validateCache(const ChunkDev& acc) {
-  if (freq_tol_ < 0.) return;
-  bool invalid = acc.freq_host.size() != cached_freq_.size();
-  for (size_t c = 0; !invalid && c < acc.freq_host.size(); ++c)
-    invalid = std::fabs(cached_freq_[c] - acc.freq_host[c]) / acc.freq_host[c] > freq_tol_;
-  if (invalid) {
-    std::fill(done_.begin(), done_.end(), 0);
-    cached_freq_ = acc.freq_host;
+  invalid_pa_ = invalid_freq_ = false;
+  bool& invalid_pa = invalid_pa_;
+  if (!symmetric()) {
+    for (size_t row = 0; row < acc.pa_h.size(); ++row)
+      if ((double)std::fabs(acc.pa_h[row] - cf_pa_) > pa_tol_) {  // float difference, as the reference
+        invalid_pa = true;
+        cf_pa_ = acc.pa_h[row];
+        std::fill(done_.begin(), done_.end(), 0);
+        break;
+      }
   }
+  bool& invalid_freq = invalid_freq_;
+  if (!invalid_pa && freq_tol_ >= 0.) {
+    invalid_freq = acc.freq_host.size() != cached_freq_.size();
+    for (size_t c = 0; !invalid_freq && c < acc.freq_host.size(); ++c)
+      invalid_freq = std::fabs(cached_freq_[c] - acc.freq_host[c]) / acc.freq_host[c] > freq_tol_;
+    if (invalid_freq) std::fill(done_.begin(), done_.end(), 0);
+  }
+  if ((invalid_pa || invalid_freq) && freq_tol_ >= 0.) cached_freq_ = acc.freq_host;
 }
 
 void AWProjectGridder::ensureTable(int S) {
@@ -173,7 +242,7 @@ void AWProjectGridder::ensureTable(int S) {
 // (AWProjectVisGridder.cc:366-573): per channel, the disk pattern is transformed to the image
 // plane; per w plane, |A|^2 x taper x phase screen is transformed back, the support is found on
 // the first plane ever generated, and the os^2 cut-outs are scaled by os^2.
-void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, double lat) {
+void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, double lat, double pa) {
   const Ctx& c = *ctx_;
   const int nx = maxsupport_, ny = maxsupport_;
   const int qnx = nx / os_, qny = ny / os_;
@@ -208,35 +277,68 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
   const size_t plane_bytes = (size_t)nx * ny * sizeof(float2);
   int nb = (int)std::max<size_t>(1, ((size_t)1 << 29) / plane_bytes);
   nb = std::min(nb, nw_);
-  DevBuf pat(plane_bytes), buf(plane_bytes * nb), dprm(sizeof(DiskParams));
+  DevBuf pat(plane_bytes), buf(plane_bytes * nb), dprm(sizeof(IllumParams));
   PlanPtr plan1, planb;
   const int nchan = nchan_cf_;
   for (int chan = 0; chan < nchan; ++chan) {
-    // DiskIllumination::getPattern (DiskIllumination.cc:85-177)
+    // DiskIllumination::getPattern (DiskIllumination.cc:85-177), ATCAIllumination::getPattern
+    // (ATCAIllumination.cc:159-331)
     const double freq = acc.freq_host[chan];
     const double cellU = uvcell_[0] / os_, cellV = uvcell_[1] / os_;
     YG_CHECK(std::fabs(std::fabs(cellU / cellV) - 1.) < 1e-7, 1, "Rectangular cells are not supported");
     const double cell = std::fabs(cellU * (kC / freq));
     const double rmax2 = (diameter_ / (2.0 * cell)) * (diameter_ / (2.0 * cell));
     double rmin2 = (blockage_ / (2.0 * cell)) * (blockage_ / (2.0 * cell));
-    YG_CHECK(rmax2 > rmin2, 1, "Disk hole is supposed to be less than the disk size");
-    YG_CHECK(double(nx) * nx > rmax2 && double(ny) * ny > rmax2, 1,
-             "The pattern buffer is too small for the dish: increase maxsupport");
-    if (rmin2 + 1 >= rmax2) --rmin2;
-    double count = 0;
+    IllumParams ip{};
+    ip.lscaled = 2. * kPi * cellU * l;
+    ip.mscaled = 2. * kPi * cellV * m;
+    if (!atca_) {
+      YG_CHECK(rmax2 > rmin2, 1, "Disk hole is supposed to be less than the disk size");
+      YG_CHECK(double(nx) * nx > rmax2 && double(ny) * ny > rmax2, 1,
+               "The pattern buffer is too small for the dish: increase maxsupport");
+      if (rmin2 + 1 >= rmax2) --rmin2;
+      ip.kind = kIllumDisk;
+    } else {
+      YG_CHECK(double(nx) * nx > rmax2 || double(ny) * ny > rmax2, 1,
+               "The pattern buffer is too small for the dish: increase maxsupport");
+      ip.kind = kIllumATCA;
+      ip.taper = a_taper_ ? 1 : 0;
+      ip.legs = a_legs_ ? 1 : 0;
+      ip.wedges = a_legs_ && a_wedges_ ? 1 : 0;
+      ip.defocus = a_defocus_;
+      ip.leg_half = a_leg_half_ / cell;
+      ip.leg_angle = a_leg_rot_ + pa;
+      ip.leg_shadow = a_leg_shadow_;
+      ip.wedge_tan = std::tan(a_wedge_angle_);
+      ip.wedge_r0 = a_wedge_r0_ / cell;
+      ip.wedge_s1 = a_wedge_s1_;
+      ip.wedge_s2 = a_wedge_s2_;
+    }
+    ip.rmin2 = rmin2;
+    ip.rmax2 = rmax2;
+    // the normalisation: the disk counts its cells (in float), ATCA sums |value|^2 (in double)
+    double sum = 0;
     for (int iu = 0; iu < nx; ++iu) {
       const double ou = double(iu) - double(nx) / 2.;
       for (int iv = 0; iv < ny; ++iv) {
         const double ov = double(iv) - double(ny) / 2.;
         const double r2 = ou * ou + ov * ov;
-        if (r2 >= rmin2 && r2 <= rmax2) count += 1.;
+        if (r2 >= rmin2 && r2 <= rmax2) {
+          double vr, vi;
+          illum_value(ip, ou, ov, r2, vr, vi);
+          sum += atca_ ? vr * vr + vi * vi : 1.;
+        }
       }
     }
-    YG_CHECK(count > 0., 4, "Integral of the disk should be non-zero");
-    DiskParams dp{2. * kPi * cellU * l, 2. * kPi * cellV * m, rmin2, rmax2,
-                  (float)(float(nx) * float(ny) / float(count)), 0.f};
-    YG_HIP(hipMemcpyAsync(dprm.get(), &dp, sizeof(dp), hipMemcpyHostToDevice, c.stream));
-    launch_disk_pattern(c, pat.as<float2>(), nx, ny, dprm.as<DiskParams>(), 1);
+    if (!atca_) {
+      YG_CHECK(sum > 0., 4, "Integral of the disk should be non-zero");
+      ip.scale = (float)(float(nx) * float(ny) / float(sum));
+    } else {
+      YG_CHECK(sum > 0., 4, "Integral of the pattern should be non-zero");
+      ip.scale = (float)(double(nx) * double(ny) / std::sqrt(sum));
+    }
+    YG_HIP(hipMemcpyAsync(dprm.get(), &ip, sizeof(ip), hipMemcpyHostToDevice, c.stream));
+    launch_illum_pattern(c, pat.as<float2>(), nx, ny, dprm.as<IllumParams>(), 1);
     // fft2d(pattern, false): backward, normalised by 1/(nx ny) -> folded into the screen kernel
     fft_plain_c2c(c, pat.as<float2>(), nx, ny, 1, false, plan1);
     for (int z0 = 0; z0 < nw_; z0 += nb) {
@@ -318,16 +420,25 @@ void AWProjectGridder::initConvolutionFunction(const ChunkDev& acc) {
   YG_CHECK(!freqdep_ || acc.nchan == nchan_cf_, 1, "the number of channels changed between chunks");
   indexField(acc);
   validateCache(acc);
+  int ndone = 0;
   for (int row = 0; row < acc.nrow; ++row) {
     const int feed = acc.feed_h.empty() ? 0 : acc.feed_h[row];
     YG_CHECK(feed >= 0 && feed < maxfeeds_, 1, "Exceeded specified maximum number of feeds");
     char& d = done_[(size_t)feed * maxfields_ + field_];
     if (d) continue;
     d = 1;
+    ++ndone;
     const double lon = acc.point_h.empty() ? centre_[0] : acc.point_h[2 * (size_t)row];
     const double lat = acc.point_h.empty() ? centre_[1] : acc.point_h[2 * (size_t)row + 1];
-    generateFeed(acc, feed, lon, lat);
+    // the row's parallactic angle where the pattern rotates with it (AWProjectVisGridder.cc:369)
+    const double pa = symmetric() || acc.pa_h.empty() ? 0.0 : (double)acc.pa_h[row];
+    generateFeed(acc, feed, lon, lat, pa);
   }
+  // AProjectGridderBase::updateStats (AProjectGridderBase.cc:272-282)
+  ++cf_iter_;
+  cf_gen_ += ndone;
+  if (invalid_pa_) cf_gen_pa_ += ndone;
+  if (invalid_freq_) cf_gen_freq_ += ndone;
 }
 
 // AWProjectVisGridder::finaliseWeights (AWProjectVisGridder.cc:618-770): the weights image is

@@ -285,6 +285,11 @@ void Gridder::stage(const ygrid_chunk& a, ChunkDev& d, bool forward) {
       d.point_h.resize(nr * 2);
       YG_HIP(hipMemcpy(d.point_h.data(), a.pointing1, nr * 2 * sizeof(double), k));
     }
+    d.pa_h.clear();
+    if (a.feed1_pa && nr) {
+      d.pa_h.resize(nr);
+      YG_HIP(hipMemcpy(d.pa_h.data(), a.feed1_pa, nr * sizeof(float), k));
+    }
   }
   d.vis_out_ld = a.vis_out_rows > 0 ? a.vis_out_rows : a.nrow;
   YG_CHECK(d.vis_out_ld >= a.nrow, 1, "vis_out_rows must be 0 or at least nrow");
@@ -786,6 +791,10 @@ void Gridder::stats(ygrid_stats* o) const {
   o->learned_cf_active = learnedActive() ? 1 : 0;
   o->degrid_matrix = ctx_->degrid_variant.matrix && shape_[2] >= 2 ? 1 : 0;
   o->rows_rejected_pointing = st_rejected_;
+  o->cf_iterations = cf_iter_;
+  o->cf_generations = cf_gen_;
+  o->cf_generations_pa = cf_gen_pa_;
+  o->cf_generations_freq = cf_gen_freq_;
 }
 
 void Gridder::getCF(int k, float* out, int* support) const {

@@ -67,6 +67,7 @@ struct ChunkDev {
   // host copies of per-row metadata for gridders whose CF cache depends on it (AWProject)
   std::vector<int32_t> feed_h;
   std::vector<double> point_h;  // nrow x 2, empty: every row points at the image centre
+  std::vector<float> pa_h;      // nrow parallactic angles (rad), empty: all 0
 };
 
 struct SupportResult {
@@ -234,6 +235,8 @@ class Gridder {
   DevBuf smp_coef16_;
   double n_bin_reused_ = 0;  // passes that reused the context's cached binning
   double n_small_ = 0;       // passes that ran the small-support kernels
+  // A-projection CF cache statistics (itsNumberOfIterations, itsNumberOfCFGenerations[DueToPA/Freq])
+  double cf_iter_ = 0, cf_gen_ = 0, cf_gen_pa_ = 0, cf_gen_freq_ = 0;
   TileJob noalloc_job_{};    // host copy of the allocation-free pass's job (read by its H2D copy)
   PlanPtr plan_;
   double st_samples_g_ = 0, st_taps_g_ = 0, st_samples_d_ = 0, st_taps_d_ = 0, st_flag_ = 0, st_wflag_ = 0;

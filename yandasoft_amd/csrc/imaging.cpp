@@ -23,7 +23,8 @@ DeviceChunk::DeviceChunk(const Ctx& ctx, const ygrid_chunk& a) : c(a) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t sizes[] = {a.uvw ? nr * 24 : 0,       a.delay ? nr * 8 : 0,      a.freq ? nc * 8 : 0,
                           a.vis ? ncube * 8 : 0,     a.noise ? ncube * 8 : 0,   a.flag ? ncube : 0,
-                          a.feed1 ? nr * 4 : 0,      a.pointing1 ? nr * 16 : 0, a.dish_pointing1 ? nr * 16 : 0};
+                          a.feed1 ? nr * 4 : 0,      a.pointing1 ? nr * 16 : 0, a.dish_pointing1 ? nr * 16 : 0,
+                          a.feed1_pa ? nr * 4 : 0};
   size_t bytes = 0;
   for (size_t b : sizes) bytes += al(b);
   store.resize(std::max<size_t>(bytes, 256));
@@ -44,6 +45,7 @@ DeviceChunk::DeviceChunk(const Ctx& ctx, const ygrid_chunk& a) : c(a) {
   c.feed1 = static_cast<const int32_t*>(put(a.feed1, sizes[6]));
   c.pointing1 = static_cast<const double*>(put(a.pointing1, sizes[7]));
   c.dish_pointing1 = static_cast<const double*>(put(a.dish_pointing1, sizes[8]));
+  c.feed1_pa = static_cast<const float*>(put(a.feed1_pa, sizes[9]));
   c.vis_out = nullptr;
   c.vis_out_rows = 0;
   c.mem = YGRID_MEM_DEVICE;

@@ -1,7 +1,8 @@
 // AWProject convolution functions and weights image on the device
-// (AWProjectVisGridder.cc:259-611,618-770; DiskIllumination.cc:85-177).
+// (AWProjectVisGridder.cc:259-611,618-770; DiskIllumination.cc:85-177, ATCAIllumination.cc:159-331).
 #include <hip/hip_runtime.h>
 
+#include "illumination.h"
 #include "ygrid_internal.h"
 
 namespace ygrid {
@@ -10,20 +11,26 @@ namespace {
 
 __device__ __forceinline__ float cb_sign(int x, int y) { return ((x + y) & 1) ? -1.f : 1.f; }
 
-// Disk aperture in the uv plane (iU fastest), pre-multiplied by the input checkerboard of a
-// centred transform.  One plane per channel of the batch.
-__global__ void k_disk_pattern(float2* __restrict__ buf, int nu, int nv, const DiskParams* __restrict__ prm) {
+// Aperture illumination in the uv plane (iU fastest), pre-multiplied by the input checkerboard of
+// a centred transform.  One plane per channel of the batch.  As getPattern: the value (double)
+// is rounded to complex float and multiplied by the pointing phase slope in complex float, then
+// scaled by the normalisation.
+__global__ void k_illum_pattern(float2* __restrict__ buf, int nu, int nv, const IllumParams* __restrict__ prm) {
+#pragma clang fp contract(off)
   const int iU = blockIdx.x * blockDim.x + threadIdx.x;
   const int iV = blockIdx.y;
   if (iU >= nu) return;
-  const DiskParams p = prm[blockIdx.z];
+  const IllumParams p = prm[blockIdx.z];
   const double ou = double(iU) - double(nu) / 2., ov = double(iV) - double(nv) / 2.;
   const double r2 = ou * ou + ov * ov;
   float2 v = make_float2(0.f, 0.f);
   if (r2 >= p.rmin2 && r2 <= p.rmax2) {
+    double vr, vi;
+    illum_value(p, ou, ov, r2, vr, vi);
     const double ph = p.lscaled * ou + p.mscaled * ov;
+    const float a = (float)vr, b = (float)vi, c = (float)cos(ph), d = (float)(-sin(ph));
     const float s = p.scale * cb_sign(iU, iV);
-    v = make_float2((float)cos(ph) * s, (float)(-sin(ph)) * s);
+    v = make_float2((a * c - b * d) * s, (a * d + b * c) * s);
   }
   buf[(size_t)blockIdx.z * nu * nv + (size_t)iV * nu + iU] = v;
 }
@@ -142,8 +149,8 @@ __global__ void k_extract_real(const double2* __restrict__ in, int nxp, int nyp,
 
 static float gsign(int nx, int ny) { return (((nx / 2) + (ny / 2)) & 1) ? -1.f : 1.f; }
 
-void launch_disk_pattern(const Ctx& c, float2* buf, int nu, int nv, const DiskParams* prm, int nb) {
-  hipLaunchKernelGGL(k_disk_pattern, dim3((nu + 255) / 256, nv, nb), dim3(256), 0, c.stream, buf, nu, nv, prm);
+void launch_illum_pattern(const Ctx& c, float2* buf, int nu, int nv, const IllumParams* prm, int nb) {
+  hipLaunchKernelGGL(k_illum_pattern, dim3((nu + 255) / 256, nv, nb), dim3(256), 0, c.stream, buf, nu, nv, prm);
   YG_HIP(hipGetLastError());
 }
 

@@ -463,13 +463,8 @@ void launch_sph_cf(const Ctx&, const double* bx, const double* by, int os, int s
                    float2* table);
 
 // AWProject kernels (kernels_awp.hip)
-struct DiskParams {
-  double lscaled, mscaled;  // 2 pi cell l, 2 pi cell m
-  double rmin2, rmax2;      // annulus in pattern cells^2
-  float scale;              // nU nV / number of cells inside
-  float pad;
-};
-void launch_disk_pattern(const Ctx&, float2* buf, int nu, int nv, const DiskParams* prm, int nb);
+struct IllumParams;  // illumination.h
+void launch_illum_pattern(const Ctx&, float2* buf, int nu, int nv, const IllumParams* prm, int nb);
 void launch_aw_screen(const Ctx&, const float2* A, int nx, int ny, int qnx, int qny, double ccellx, double ccelly,
                       float ascale, const double* tx, const double* ty, const double* two_pi_w, int nb, float2* out);
 // multiply by the output checkerboard of a centred transform (incl. its global sign) and a scale
