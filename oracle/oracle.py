@@ -698,8 +698,8 @@ def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_o
     nchan = len(freqs) if prm.freqdep else 1
     nw = prm.nwplanes
     if state is None:
-        state = {"planes": [None] * (os_ * os_ * nw * prm.maxfeeds * prm.maxfields * nchan), "support": 0,
-                 "nchan": nchan}
+        nz = nw * prm.maxfeeds * prm.maxfields * nchan
+        state = {"planes": [None] * (os_ * os_ * nz), "support": 0, "nchan": nchan, "offsets": [(0, 0)] * nz}
     ws = prm.sampling()
     if prm.spheroidaltaper:
         sph = Spheroidal(math.pi * 3, prm.alpha)
@@ -735,19 +735,26 @@ def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_o
                 buf = fft_centred(buf, True) / (nx * ny)
                 buf = buf.astype(np.complex64).astype(np.complex128)
                 z = iw + nw * (chan + nchan * (feed + prm.maxfeeds * field))
-                if state["support"] == 0:
-                    size, _, _ = _extract_support(buf.T, prm, os_)
+                # AWProjectVisGridder.cc:444-496: the support (and, with offsetsupport, the
+                # offset) of every plane when it is plane dependent, else of the first plane
+                offu = offv = 0
+                size = state["support"]
+                if prm.variablesupport or state["support"] == 0:
+                    size, offu, offv = _extract_support(buf.T, prm, os_)
                     if not ((size + 1) * os_ < nx // 2):
                         raise ValueError("Overflowing convolution function")
                     if prm.limitsupport > 0 and size > prm.limitsupport:
                         size = prm.limitsupport
-                    state["support"] = size
-                S = state["support"]
+                    if state["support"] == 0:
+                        state["support"] = size
+                    if prm.offsetsupport:
+                        state["offsets"][z] = (offu, offv)
+                S = size if prm.variablesupport else state["support"]
                 t = np.arange(-S, S + 1)
                 for fracu in range(os_):
-                    kx = t * os_ + fracu + nx // 2
+                    kx = (t + offu) * os_ + fracu + nx // 2
                     for fracv in range(os_):
-                        ky = t * os_ + fracv + ny // 2
+                        ky = (t + offv) * os_ + fracv + ny // 2
                         if kx.min() < 0 or ky.min() < 0 or kx.max() >= nx or ky.max() >= ny:
                             raise ValueError("Convolution function overflowing")
                         if pcf:
@@ -1111,7 +1118,8 @@ class OracleGridder:
         st[2] += len(new) if self.aw_invalid[0] else 0
         st[3] += len(new) if self.aw_invalid[1] else 0
         self.support = self.aw_state["support"]
-        self.cf = CFCache(self.aw_state["planes"], self.wp.oversample)
+        self.cf = CFCache(self.aw_state["planes"], self.wp.oversample,
+                          self.aw_state["offsets"] if self.wp.offsetsupport else None)
 
     def n_cf_planes(self):
         if self.kind == "WProject":
@@ -1285,10 +1293,14 @@ class OracleGridder:
             k = iz if self.track_per_os else os_ * os_ * iz
             cf = self.cf.planes[k].astype(np.complex64).astype(np.complex128)  # the CF cache is FP32
             S = (cf.shape[0] - 1) // 2
+            # getConvFuncOffset(iz): indexed by the sum-of-weights row (AWProjectVisGridder.cc:706),
+            # (0, 0) beyond the offsets held (TableVisGridder.cc:1428-1435)
+            offs = self.cf.offsets
+            ou, ov = offs[iz] if offs is not None and iz < len(offs) else (0, 0)
             pl = np.zeros((cny, cnx), dtype=np.complex128)
             for iy in range(-S, S + 1):
                 for ix in range(-S, S + 1):
-                    xp, yp = ix + cx0, iy + cy0
+                    xp, yp = ix + cx0 + ou, iy + cy0 + ov
                     if 0 <= xp < cnx and 0 <= yp < cny:
                         pl[yp, xp] = cf[iy + S, ix + S]
             pl = fft_centred(pl, False) * (cnx * cny)

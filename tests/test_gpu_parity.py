@@ -379,6 +379,90 @@ def test_awproject_disk_matches_oracle(gpu_ctx, taper):
     assert rel(np.transpose(out, (2, 1, 0)), ov) < TOL
 
 
+@pytest.mark.parametrize("sampling", ["linear", "powerlaw"])
+def test_awproject_variable_and_offset_support_match_oracle(gpu_ctx, sampling):
+    """variablesupport + offsetsupport (AWProjectVisGridder.cc:444-503) with linear or power-law w
+    planes: per-plane supports and offsets (a 1.5 deg beam offset moves the w != 0 kernels' peaks by
+    more than a grid pixel), CF planes, grid, image, weights (offsets applied,
+    AWProjectVisGridder.cc:706-712) and degridding against the oracle; the preconditioner form
+    carries every plane's own support at its centre."""
+    announce(f"awproject variable + offset support, {sampling} w sampling")
+    import yandasoft_amd as ya
+    N, cell = 128, 120 * ARC
+    centre = (0.5, -0.7)
+    d = make_data(600, 2, 2, N, cell, f0=1.4e9, df=1e7, frac=0.3, wmax_m=2700.0 * O.C_LIGHT / 1.41e9, seed=11)
+    feed = (np.arange(600) % 2).astype(np.int32)
+    point = np.zeros((600, 2))
+    point[:, 0] = centre[0] + np.where(feed == 1, math.radians(1.5), 0.0)
+    point[:, 1] = centre[1]
+    params = {"wmax": 3000.0, "nwplanes": 5, "oversample": 4, "maxsupport": 128, "maxfeeds": 2, "cutoff": 0.01,
+              "diameter": "12m", "blockage": "1m", "variablesupport": True, "offsetsupport": True,
+              "wsampling": sampling, "wsampling.exponent": 2.0}
+    prm = O.AWProjectParams(wmax=3000.0, nwplanes=5, oversample=4, maxsupport=128, maxfeeds=2, cutoff=0.01,
+                            diameter=12.0, blockage=1.0, variablesupport=True, offsetsupport=True,
+                            wsampling=sampling, exponent=2.0)
+    lm = {0: O.direction_lm(tuple(point[0]), centre), 1: O.direction_lm(tuple(point[1]), centre)}
+    img = ya.ImageDesc(N, N, 2, 1, cell, cell, centre=centre)
+    ch = ya.Chunk(uvw=d["uvw"], freq=d["freq"], vis=d["vis"], noise=d["noise"], feed1=feed,
+                  pointing1=np.ascontiguousarray(point))
+    och = ochunk(d)
+    och.feed1 = feed
+
+    def oracle(prec, **kw):
+        o = O.OracleGridder("AWProject", prec, wproject=prm)
+        o.aw_lm, o.aw_nchan = lm, 2
+        o.initialise_grid((N, N, 2, 1), (cell, cell), **kw)
+        o.grid(och)
+        return o
+
+    g = ya.Gridder(gpu_ctx, "GPUAWProject", params)
+    g.initialise_grid(img)
+    g.grid(ch)
+    o = oracle("f32")
+    offs = o.cf.offsets
+    assert any(v != (0, 0) for v in offs), offs
+    os_ = 4
+    sizes = set()
+    for z in range(5 * 2 * 2):
+        for frac in (0, 6, 15):
+            k = frac + os_ * os_ * z
+            gp, op = g.cf_plane(k), o.cf.planes[k]
+            assert gp.shape == op.shape, (z, gp.shape, op.shape)
+            assert rel(gp, op) < TOL, (z, frac)
+        sizes.add(o.cf.planes[os_ * os_ * z].shape[0])
+    assert len(sizes) > 1
+    assert g.stats().support == max(sizes) // 2
+    gg = g.grid_array().reshape(1, 2, N, N)
+    assert rel(gg, o.grid_) < TOL
+    o64 = oracle("f64")
+    ok, info = image_ok(g.finalise_grid(), o64.finalise_grid(), gain_of(o), lambda: finalise_with(o, gg))
+    assert ok, info
+    assert rel(g.finalise_weights(), o.finalise_weights()) < TOL
+    # degrid
+    model = random_model(N, 2)
+    dg = ya.Gridder(gpu_ctx, "GPUAWProject", params)
+    dg.initialise_degrid(img, model.astype(np.float32))
+    out = np.zeros_like(d["vis"])
+    dg.degrid(ya.Chunk(uvw=d["uvw"], freq=d["freq"], vis=d["vis"], vis_out=out, noise=d["noise"], feed1=feed,
+                       pointing1=np.ascontiguousarray(point)))
+    od = O.OracleGridder("AWProject", "f32", wproject=prm)
+    od.aw_lm, od.aw_nchan = lm, 2
+    od.initialise_degrid((N, N, 2, 1), (cell, cell), model)
+    assert rel(np.transpose(out, (2, 1, 0)), od.degrid(och)) < TOL
+    # preconditioner form: 3x3 planes whose centre holds each plane's own support, offsets applied
+    gp = ya.Gridder(gpu_ctx, "GPUAWProject", params)
+    gp.initialise_grid(img, dopcf=True)
+    gp.grid(ch)
+    op = oracle("f32", dopcf=True)
+    centres = set()
+    for z in range(5 * 2 * 2):
+        k = os_ * os_ * z + 5
+        assert np.array_equal(gp.cf_plane(k), op.cf.planes[k].astype(np.complex64)), z
+        centres.add(gp.cf_plane(k)[1, 1].imag)
+    assert len(centres) > 1, centres
+    assert rel(gp.grid_array().reshape(1, 2, N, N), op.grid_) < TOL
+
+
 def test_wproject_wmax_handling(gpu_ctx):
     announce("wproject wmaxclip / w out of range")
     import yandasoft_amd as ya

@@ -600,6 +600,55 @@ def test_awproject_planes_match_brute_force_transforms():
                         assert np.max(np.abs(got - ref)) < 1e-6 * np.max(np.abs(ref)), (feed, ch, iw, fu, fv)
 
 
+def test_awproject_variable_and_offset_support_planes_match_brute_force():
+    """variablesupport + offsetsupport with power-law w planes (AWProjectVisGridder.cc:444-503,
+    WProjectVisGridder.cc:233-263): every plane is cut out at its own support about its own offset,
+    the offset being the plane's peak (in grid pixels, truncated).  For a beam offset (l0, m0) the
+    w screen exp(-i pi w l^2) of the beam-centred aperture power, |A(l - l0)|^2, carries the slope
+    exp(-2 pi i w l0 l), so the forward transform puts the kernel's peak at -w (l0, m0)
+    wavelengths and the offsets follow that closed form; the planes themselves are checked against the same centre-relative DFT sums as above."""
+    prm, freqs, uvcell, _ = _aw_small()
+    prm.wmax, prm.variablesupport, prm.offsetsupport = 2000.0, True, True
+    prm.wsampling, prm.exponent = "powerlaw", 2.0
+    os_, n, nw = prm.oversample, prm.maxsupport, prm.nwplanes
+    cu = uvcell / os_
+    lm = {0: (0.0, 0.0), 1: (4 / (n * cu), -2 / (n * cu))}
+    st = O.awproject_planes(prm, 64, 64, (uvcell, uvcell), freqs, lm)
+    ws = prm.sampling()
+    c = np.arange(n) - n / 2.0
+    x = c / ((n // os_) * uvcell)
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    E = np.exp(2j * np.pi * np.outer(c, c) / n)
+    sizes = set()
+    for feed, (l, m) in lm.items():
+        for ch, f in enumerate(freqs):
+            R2 = (prm.diameter / (2 * cu * O.C_LIGHT / f)) ** 2
+            disk = (c[None, :] ** 2 + c[:, None] ** 2 <= R2)
+            pat = disk * np.exp(-2j * np.pi * cu * (l * c[None, :] + m * c[:, None])) * (n * n / disk.sum())
+            wt = np.abs(E @ pat @ E.T / (n * n)) ** 2
+            for iw in range(nw):
+                w = ws.wterm(iw)
+                ph = 2 * np.pi * w * (1 - np.sqrt(np.clip(1 - r2, 0, None)))
+                scr = np.where(r2 < 1, wt * np.exp(-1j * ph), 0)
+                z = iw + nw * (ch + len(freqs) * feed)
+                ou, ov = st["offsets"][z]
+                expect = (O.cdiv(round(-w * l / cu), os_), O.cdiv(round(-w * m / cu), os_))
+                assert (ou, ov) == expect, (feed, ch, iw, (ou, ov), expect)
+                S = (st["planes"][os_ * os_ * z].shape[0] - 1) // 2
+                sizes.add(S)
+                for fu in range(os_):
+                    for fv in range(os_):
+                        Eu = np.exp(-2j * np.pi * np.outer((np.arange(-S, S + 1) + ou) * os_ + fu, c) / n)
+                        Ev = np.exp(-2j * np.pi * np.outer((np.arange(-S, S + 1) + ov) * os_ + fv, c) / n)
+                        ref = Ev @ scr @ Eu.T / (n * n) * os_ * os_
+                        got = st["planes"][fu + os_ * (fv + os_ * z)]
+                        assert got.shape == (2 * S + 1, 2 * S + 1)
+                        assert np.max(np.abs(got - ref)) < 1e-6 * np.max(np.abs(ref)), (feed, ch, iw, fu, fv)
+    assert len(sizes) > 1 and any(o != (0, 0) for o in st["offsets"])
+    # the w = 0 plane (the smallest kernel) keeps the minimum support of 3 pixels or more
+    assert st["support"] == (st["planes"][0].shape[0] - 1) // 2
+
+
 def test_awproject_w0_plane_of_a_centred_disk_is_real_and_symmetric():
     """With no beam offset |A|^2 is real and symmetric, so at w = 0 the CF is real and has the
     square's symmetries (x -> -x, y -> -y, x <-> y) on the unshifted oversampling plane."""

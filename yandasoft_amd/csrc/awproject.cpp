@@ -26,8 +26,8 @@ class AWProjectGridder : public Gridder {
       freq_tol_ = p.f64("freqtolerance", 1e-6);
       YG_CHECK(freq_tol_ >= 0., 1, "Frequency tolerance parameter is supposed to be either a non-negative number or infinite");
     }
-    wmax_ = p.f64("wmax", 10000.0);
-    nw_ = p.i32("nwplanes", 65);
+    ws_.wmax = p.f64("wmax", 10000.0);
+    ws_.nw = p.i32("nwplanes", 65);
     cutoff_ = p.f64("cutoff", 1e-3);
     os_ = p.i32("oversample", 8);
     maxsupport_ = p.i32("maxsupport", 128);
@@ -39,10 +39,14 @@ class AWProjectGridder : public Gridder {
     sph_wcut_ = p.f64("spheroidalweightscutoff", 0.0);
     alpha_ = p.f64("alpha", 1.0);
     abs_cutoff_ = p.flag("cutoff.absolute", false);
-    YG_CHECK(!p.flag("variablesupport", false), 1, "variablesupport is not supported by GPUAWProject");
-    YG_CHECK(!p.flag("offsetsupport", false), 1, "offsetsupport is not supported by GPUAWProject");
-    YG_CHECK(wmax_ > 0.0, 1, "Baseline length must be greater than zero");
-    YG_CHECK(nw_ > 0 && nw_ % 2 == 1, 1, "Number of w planes must be a positive odd number");
+    // WProjectVisGridder::configureGridder (WProjectVisGridder.cc:331-363), which
+    // AWProjectVisGridder::createGridder calls (AWProjectVisGridder.cc:883-888)
+    variable_ = p.flag("variablesupport", false);
+    offset_ = p.flag("offsetsupport", false);
+    YG_CHECK(!offset_ || variable_, 1,
+             "offsetsupport option of the gridder should only be used together with variablesupport option");
+    YG_CHECK(ws_.wmax > 0.0, 1, "Baseline length must be greater than zero");
+    YG_CHECK(ws_.nw > 0 && ws_.nw % 2 == 1, 1, "Number of w planes must be a positive odd number");
     YG_CHECK(os_ > 0 && maxsupport_ > 0 && maxsupport_ % 2 == 0, 1, "bad oversample / maxsupport");
     YG_CHECK(maxfeeds_ > 0 && maxfields_ > 0, 1, "maxfeeds and maxfields must be positive");
     if (!abs_cutoff_) YG_CHECK(cutoff_ > 0.0 && cutoff_ < 1.0, 1, "Cutoff must be in (0, 1)");
@@ -86,9 +90,8 @@ class AWProjectGridder : public Gridder {
         YG_CHECK(a_leg_half_ < diameter_ / 2., 1, "the feed-leg width must be less than the dish diameter");
       }
     }
-    const std::string smp = p.str("wsampling", "linear");
-    YG_CHECK(smp == "linear", 1, "GPUAWProject supports linear w sampling");
-    wmaxclip_ = p.flag("wmaxclip", false);
+    // configureWSampling (AProjectGridderBase.tcc:112): linear, power-law or gaussian planes
+    configure_wsampling(ws_, p);
     done_.assign((size_t)maxfeeds_ * maxfields_, 0);
     pointings_.assign((size_t)maxfeeds_ * maxfields_ * 2, 0.0);
   }
@@ -100,7 +103,7 @@ class AWProjectGridder : public Gridder {
 
  protected:
   int oversampling() const override { return os_; }
-  int nZPlanes() const override { return nw_ * maxfeeds_ * maxfields_ * std::max(1, nchan_cf_); }
+  int nZPlanes() const override { return ws_.nw * maxfeeds_ * maxfields_ * std::max(1, nchan_cf_); }
   bool needsRowMeta() const override { return true; }
   bool cfPerChunk() const override { return true; }
   bool doesCorrection() const override { return sph_taper_; }
@@ -110,10 +113,14 @@ class AWProjectGridder : public Gridder {
   CIndexDesc cindex(const ChunkDev&) const override {
     CIndexDesc d;
     d.kind = 2;
-    d.nw = nw_;
-    d.wscale = nw_ > 1 ? wmax_ / double((nw_ - 1) / 2) : 1.0;
-    d.wmax = wmax_;
-    d.wmaxclip = wmaxclip_ ? 1 : 0;
+    d.nw = ws_.nw;
+    d.wscale = ws_.wscale();
+    d.wmax = ws_.wmax;
+    d.sampling = ws_.kind;
+    d.exponent = ws_.exponent;
+    d.g_ts = ws_.ts;
+    d.g_amp = ws_.amp;
+    d.wmaxclip = ws_.wmaxclip ? 1 : 0;
     d.freqdep = freqdep_ ? 1 : 0;
     d.nchan_cf = std::max(1, nchan_cf_);
     d.maxfeeds = maxfeeds_;
@@ -124,10 +131,8 @@ class AWProjectGridder : public Gridder {
   void finaliseWeights(float* out, int mem) override;
 
  private:
-  double wterm(int plane) const {
-    const int half = (nw_ - 1) / 2;
-    return double(plane - half) * (nw_ > 1 ? wmax_ / double(half) : 1.0);
-  }
+  // WDependentGridderBase::getWTerm (WDependentGridderBase.cc:157-167)
+  double wterm(int plane) const { return ws_.wterm(plane); }
   // ATCAIllumination::isSymmetric: without feed legs the pattern does not rotate with the
   // parallactic angle; the disk never does
   bool symmetric() const { return !atca_ || !a_legs_; }
@@ -148,11 +153,29 @@ class AWProjectGridder : public Gridder {
   void ensureTable(int S);
   void writePCFPlanes(const Ctx& c, DevBuf& buf, int nx, int ny, int cnt, int z0, int chan, int nchan, int feed);
   int pcf_support_ = 0;  // preconditioner form: the support found on the first plane
+  // Plane-dependent support (variablesupport, offsetsupport).  Each plane's support (and offset)
+  // is found on the plane itself; a (feed, field)'s planes are cut out into pieces of their own
+  // sizes and the table is rebuilt around them (VarPiece: the first z, the z count, the piece).
+  struct VarPiece {
+    int z0, cnt;
+    size_t elems;
+    DevBuf data;
+  };
+  // search the support (and offset) of the cnt planes in buf, as extractSupport +
+  // limitSupportIfNecessary + calcConvFuncOverflow (AWProjectVisGridder.cc:444-496)
+  void planeSupports(const Ctx& c, const DevBuf& buf, int nx, int ny, int cnt, int zfirst, std::vector<int>& sz,
+                     std::vector<int>& ou, std::vector<int>& ov);
+  void cutVariable(const Ctx& c, const DevBuf& buf, int nx, int ny, int cnt, int zfirst, std::vector<VarPiece>& out);
+  void commitVariable(int feed, std::vector<VarPiece>& pieces);
+  bool variable_ = false, offset_ = false;
+  int first_support_ = 0;                  // itsSupport: the first plane's support
+  std::vector<int> zsupp_, zou_, zov_;     // per z plane before oversampling (0: not generated)
 
   double pointing_tol_ = 1e-4, pa_tol_ = 0.1, freq_tol_ = 1e-6;
-  double wmax_ = 10000, cutoff_ = 1e-3, alpha_ = 1.0, diameter_ = 12, blockage_ = 0, sph_wcut_ = 0;
-  int nw_ = 65, os_ = 8, maxsupport_ = 128, limitsupport_ = 0, maxfeeds_ = 1, maxfields_ = 1;
-  bool freqdep_ = true, sph_taper_ = false, abs_cutoff_ = false, wmaxclip_ = false;
+  WSamplingDesc ws_{10000.0, 65};
+  double cutoff_ = 1e-3, alpha_ = 1.0, diameter_ = 12, blockage_ = 0, sph_wcut_ = 0;
+  int os_ = 8, maxsupport_ = 128, limitsupport_ = 0, maxfeeds_ = 1, maxfields_ = 1;
+  bool freqdep_ = true, sph_taper_ = false, abs_cutoff_ = false;
   int nchan_cf_ = 0, field_ = 0, last_field_ = -1;
   // ATCA illumination (atca_ false: the disk)
   bool atca_ = false, a_taper_ = false, a_legs_ = false, a_wedges_ = false;
@@ -215,7 +238,7 @@ void AWProjectGridder::validateCache(const ChunkDev& acc) {
 void AWProjectGridder::ensureTable(int S) {
   if (cf_ && cf_.use_count() == 1 && cf_->support == S) return;
   auto t = std::make_shared<CFTable>();
-  const int nz = nw_ * maxfeeds_ * maxfields_ * nchan_cf_;
+  const int nz = ws_.nw * maxfeeds_ * maxfields_ * nchan_cf_;
   t->nz = nz;
   t->oversample = os_;
   t->support = S;
@@ -270,16 +293,17 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
     YG_HIP(hipMemcpy(dtx.get(), hx.data(), hx.size() * 8, hipMemcpyHostToDevice));
     YG_HIP(hipMemcpy(dty.get(), hy.data(), hy.size() * 8, hipMemcpyHostToDevice));
   }
-  std::vector<double> tpw(nw_);
-  for (int iw = 0; iw < nw_; ++iw) tpw[iw] = 2.0f * kPi * wterm(iw);
-  DevBuf dtpw(nw_ * sizeof(double));
-  YG_HIP(hipMemcpy(dtpw.get(), tpw.data(), nw_ * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> tpw(ws_.nw);
+  for (int iw = 0; iw < ws_.nw; ++iw) tpw[iw] = 2.0f * kPi * wterm(iw);
+  DevBuf dtpw(ws_.nw * sizeof(double));
+  YG_HIP(hipMemcpy(dtpw.get(), tpw.data(), ws_.nw * sizeof(double), hipMemcpyHostToDevice));
   const size_t plane_bytes = (size_t)nx * ny * sizeof(float2);
   int nb = (int)std::max<size_t>(1, ((size_t)1 << 29) / plane_bytes);
-  nb = std::min(nb, nw_);
+  nb = std::min(nb, ws_.nw);
   DevBuf pat(plane_bytes), buf(plane_bytes * nb), dprm(sizeof(IllumParams));
   PlanPtr plan1, planb;
   const int nchan = nchan_cf_;
+  std::vector<VarPiece> pieces;  // variablesupport: this (feed, field)'s planes
   for (int chan = 0; chan < nchan; ++chan) {
     // DiskIllumination::getPattern (DiskIllumination.cc:85-177), ATCAIllumination::getPattern
     // (ATCAIllumination.cc:159-331)
@@ -341,8 +365,8 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
     launch_illum_pattern(c, pat.as<float2>(), nx, ny, dprm.as<IllumParams>(), 1);
     // fft2d(pattern, false): backward, normalised by 1/(nx ny) -> folded into the screen kernel
     fft_plain_c2c(c, pat.as<float2>(), nx, ny, 1, false, plan1);
-    for (int z0 = 0; z0 < nw_; z0 += nb) {
-      const int cnt = std::min(nb, nw_ - z0);
+    for (int z0 = 0; z0 < ws_.nw; z0 += nb) {
+      const int cnt = std::min(nb, ws_.nw - z0);
       launch_aw_screen(c, pat.as<float2>(), nx, ny, qnx, qny, ccellx, ccelly, (float)(1.0 / (double(nx) * ny)),
                        sph_taper_ ? dtx.as<double>() : nullptr, sph_taper_ ? dty.as<double>() : nullptr,
                        dtpw.as<double>() + z0, cnt, buf.as<float2>());
@@ -350,6 +374,10 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
       launch_scale_sign(c, buf.as<float2>(), nx, ny, cnt, (float)(1.0 / (double(nx) * ny)));
       if (dopcf_) {
         writePCFPlanes(c, buf, nx, ny, cnt, z0, chan, nchan, feed);
+        continue;
+      }
+      if (variable_) {
+        cutVariable(c, buf, nx, ny, cnt, z0 + ws_.nw * (chan + nchan * (feed + maxfeeds_ * field_)), pieces);
         continue;
       }
       if (!cf_ || cf_->support == 0) {
@@ -370,7 +398,7 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
       std::vector<int32_t> first(cnt);
       std::vector<int> sup(cnt, S), zero(cnt, 0);
       for (int b = 0; b < cnt; ++b) {
-        const int z = (z0 + b) + nw_ * (chan + nchan * (feed + maxfeeds_ * field_));
+        const int z = (z0 + b) + ws_.nw * (chan + nchan * (feed + maxfeeds_ * field_));
         first[b] = cf_->off[(size_t)z * os_ * os_];
       }
       DevBuf dfirst(cnt * 4), dsup(cnt * 4), dzero(cnt * 4);
@@ -382,6 +410,118 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
       YG_HIP(hipStreamSynchronize(c.stream));
     }
   }
+  if (variable_ && !dopcf_) commitVariable(feed, pieces);
+}
+
+void AWProjectGridder::planeSupports(const Ctx& c, const DevBuf& buf, int nx, int ny, int cnt, int zfirst,
+                                     std::vector<int>& sz, std::vector<int>& ou, std::vector<int>& ov) {
+  sz.assign(cnt, 0);
+  ou.assign(cnt, 0);
+  ov.assign(cnt, 0);
+  for (int b = 0; b < cnt; ++b) {
+    const SupportResult r =
+        search_support(c, buf.as<float2>() + (size_t)b * nx * ny, nx, ny, cutoff_, abs_cutoff_, offset_, os_);
+    YG_CHECK((r.size + 1) * os_ < nx / 2, 4,
+             "Overflowing convolution function - increase maxSupport or cutoff or decrease overSample");
+    int S = r.size;
+    if (limitsupport_ > 0 && S > limitsupport_) S = limitsupport_;
+    if (first_support_ == 0) first_support_ = S;
+    if (offset_) {
+      ou[b] = r.offu;
+      ov[b] = r.offv;
+    }
+    sz[b] = S;
+    const int kxmin = (-S + ou[b]) * os_ + nx / 2, kxmax = (S + ou[b]) * os_ + os_ - 1 + nx / 2;
+    const int kymin = (-S + ov[b]) * os_ + ny / 2, kymax = (S + ov[b]) * os_ + os_ - 1 + ny / 2;
+    YG_CHECK(kxmin >= 0 && kymin >= 0 && kxmax < nx && kymax < ny, 4,
+             "Convolution function overflowing - increase maxsupport or cutoff or decrease oversample");
+    const int z = zfirst + b;
+    zsupp_[z] = S;
+    zou_[z] = ou[b];
+    zov_[z] = ov[b];
+  }
+}
+
+// The cnt planes of z = zfirst .. zfirst + cnt - 1 cut out at their own supports and offsets (each
+// z's os^2 planes consecutive, as the table holds them) into one piece.
+void AWProjectGridder::cutVariable(const Ctx& c, const DevBuf& buf, int nx, int ny, int cnt, int zfirst,
+                                   std::vector<VarPiece>& out) {
+  std::vector<int> sz, ou, ov;
+  planeSupports(c, buf, nx, ny, cnt, zfirst, sz, ou, ov);
+  std::vector<int32_t> first(cnt);
+  size_t total = 0;
+  for (int b = 0; b < cnt; ++b) {
+    first[b] = (int32_t)total;
+    total += (size_t)os_ * os_ * cf_pitch(sz[b]) * cf_rows(sz[b]);
+    YG_CHECK(total < (size_t)INT_MAX, 1, "convolution function table exceeds 2^31 elements");
+  }
+  DevBuf dfirst(cnt * 4), dsup(cnt * 4), dou(cnt * 4), dov(cnt * 4);
+  YG_HIP(hipMemcpy(dfirst.get(), first.data(), cnt * 4, hipMemcpyHostToDevice));
+  YG_HIP(hipMemcpy(dsup.get(), sz.data(), cnt * 4, hipMemcpyHostToDevice));
+  YG_HIP(hipMemcpy(dou.get(), ou.data(), cnt * 4, hipMemcpyHostToDevice));
+  YG_HIP(hipMemcpy(dov.get(), ov.data(), cnt * 4, hipMemcpyHostToDevice));
+  VarPiece pc{zfirst, cnt, total, DevBuf(total * sizeof(float2))};
+  launch_cf_cutout(c, buf.as<float2>(), nx, ny, cnt, os_, dsup.as<int>(), dou.as<int>(), dov.as<int>(),
+                   dfirst.as<int32_t>(), 1, double(os_) * os_, pc.data.as<float2>());
+  YG_HIP(hipStreamSynchronize(c.stream));
+  out.push_back(std::move(pc));
+}
+
+// A new table holding every plane generated so far at its own support: the planes of the other
+// (feed, field)s copied from the current table, those of `feed` (this field) from the pieces.
+// Planes not generated yet take no space (support 0, never selected by a sample).  The current
+// table is left untouched for the clones that share it.
+void AWProjectGridder::commitVariable(int feed, std::vector<VarPiece>& pieces) {
+  const Ctx& c = *ctx_;
+  const int nz = ws_.nw * maxfeeds_ * maxfields_ * nchan_cf_;
+  const size_t os2 = (size_t)os_ * os_;
+  const int zb = ws_.nw * nchan_cf_ * (feed + maxfeeds_ * field_), ze = zb + ws_.nw * nchan_cf_;
+  auto t = std::make_shared<CFTable>();
+  t->nz = nz;
+  t->oversample = os_;
+  t->off.assign((size_t)nz * os2, 0);
+  t->supp.assign((size_t)nz * os2, 0);
+  size_t total = 0, new_zb = 0, new_ze = 0;
+  t->support = 0;
+  for (int z = 0; z < nz; ++z) {
+    if (z == zb) new_zb = total;
+    if (z == ze) new_ze = total;
+    const int S = zsupp_[z];
+    const size_t blk = S > 0 ? (size_t)cf_pitch(S) * cf_rows(S) : 0;
+    for (size_t f = 0; f < os2; ++f) {
+      YG_CHECK(total < (size_t)INT_MAX, 1, "convolution function table exceeds 2^31 elements");
+      t->off[(size_t)z * os2 + f] = (int32_t)total;
+      t->supp[(size_t)z * os2 + f] = S;
+      total += blk;
+    }
+    t->support = std::max(t->support, S);
+  }
+  if (ze == nz) new_ze = total;
+  t->n_elems = total;
+  if (offset_) {
+    t->ofs_u.assign(zou_.begin(), zou_.end());
+    t->ofs_v.assign(zov_.begin(), zov_.end());
+  }
+  t->table.resize(std::max<size_t>(total, 1) * sizeof(float2));
+  float2* dst = t->table.as<float2>();
+  if (cf_ && cf_->n_elems > 0) {
+    // the planes before and after the block keep their sizes: two contiguous runs
+    const size_t old_zb = cf_->off[(size_t)zb * os2];
+    const size_t old_ze = ze < nz ? (size_t)cf_->off[(size_t)ze * os2] : cf_->n_elems;
+    const float2* src = cf_->table.as<float2>();
+    if (old_zb > 0)
+      YG_HIP(hipMemcpyAsync(dst, src, old_zb * sizeof(float2), hipMemcpyDeviceToDevice, c.stream));
+    if (cf_->n_elems > old_ze)
+      YG_HIP(hipMemcpyAsync(dst + new_ze, src + old_ze, (cf_->n_elems - old_ze) * sizeof(float2),
+                            hipMemcpyDeviceToDevice, c.stream));
+  }
+  for (const VarPiece& pc : pieces)
+    YG_HIP(hipMemcpyAsync(dst + t->off[(size_t)pc.z0 * os2], pc.data.get(), pc.elems * sizeof(float2),
+                          hipMemcpyDeviceToDevice, c.stream));
+  (void)new_zb;
+  t->upload(c);
+  cf_ = t;
+  pieces.clear();
 }
 
 // Preconditioner form (AWProjectVisGridder.cc:436-466,511-516): the support is found on the first
@@ -389,6 +529,30 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
 // (1, support); the table itself then has support 1.
 void AWProjectGridder::writePCFPlanes(const Ctx& c, DevBuf& buf, int nx, int ny, int cnt, int z0, int chan, int nchan,
                                       int feed) {
+  const int zfirst = z0 + ws_.nw * (chan + nchan * (feed + maxfeeds_ * field_));
+  if (variable_) {
+    // each plane's centre carries its own support, and its offset applies when it is gridded
+    // (AWProjectVisGridder.cc:444-466,499,511-516)
+    std::vector<int> sz, ou, ov;
+    planeSupports(c, buf, nx, ny, cnt, zfirst, sz, ou, ov);
+    if (pcf_support_ == 0) pcf_support_ = sz[0];
+    ensureTable(1);
+    const size_t blk = (size_t)cf_pitch(1) * cf_rows(1);
+    const size_t k0 = (size_t)zfirst * os_ * os_, nk = (size_t)cnt * os_ * os_;
+    std::vector<float2> host(nk * blk, make_float2(0.f, 0.f));
+    for (size_t k = 0; k < nk; ++k)
+      host[k * blk + (size_t)(1 + kGuardV) * cf_pitch(1) + 1 + kGuardU] =
+          make_float2(1.f, (float)sz[k / ((size_t)os_ * os_)]);
+    YG_HIP(hipMemcpyAsync(cf_->table.as<float2>() + cf_->off[k0], host.data(), host.size() * sizeof(float2),
+                          hipMemcpyHostToDevice, c.stream));
+    if (offset_) {
+      cf_->ofs_u.assign(zou_.begin(), zou_.end());
+      cf_->ofs_v.assign(zov_.begin(), zov_.end());
+      cf_->upload(c);
+    }
+    YG_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
   if (pcf_support_ == 0) {
     const SupportResult r = search_support(c, buf.as<float2>(), nx, ny, cutoff_, abs_cutoff_, false, os_);
     YG_CHECK((r.size + 1) * os_ < nx / 2, 4,
@@ -403,7 +567,6 @@ void AWProjectGridder::writePCFPlanes(const Ctx& c, DevBuf& buf, int nx, int ny,
            "Convolution function overflowing - increase maxsupport or cutoff or decrease oversample");
   ensureTable(1);
   const size_t blk = (size_t)cf_pitch(1) * cf_rows(1);
-  const int zfirst = z0 + nw_ * (chan + nchan * (feed + maxfeeds_ * field_));
   const size_t k0 = (size_t)zfirst * os_ * os_, nk = (size_t)cnt * os_ * os_;
   std::vector<float2> host(nk * blk, make_float2(0.f, 0.f));
   for (size_t k = 0; k < nk; ++k)
@@ -418,6 +581,12 @@ void AWProjectGridder::initConvolutionFunction(const ChunkDev& acc) {
   YG_CHECK(!acc.freq_host.empty(), 1, "no channels");
   if (nchan_cf_ == 0) nchan_cf_ = freqdep_ ? acc.nchan : 1;
   YG_CHECK(!freqdep_ || acc.nchan == nchan_cf_, 1, "the number of channels changed between chunks");
+  if (zsupp_.empty()) {
+    const size_t nz = (size_t)ws_.nw * maxfeeds_ * maxfields_ * nchan_cf_;
+    zsupp_.assign(nz, 0);
+    zou_.assign(nz, 0);
+    zov_.assign(nz, 0);
+  }
   indexField(acc);
   validateCache(acc);
   int ndone = 0;
@@ -482,27 +651,35 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
   if (cf_ && !rows.empty()) {
     const int nb = std::max(1, std::min((int)rows.size(), (int)(((size_t)1 << 28) / ((size_t)cnx * cny * 8))));
     DevBuf planes((size_t)nb * cnx * cny * sizeof(double2)), dw((size_t)nb * ncp * sizeof(double));
-    DevBuf doff(nb * 4), dsup(nb * 4), dzero(nb * 4);
+    DevBuf doff(nb * 4), dsup(nb * 4), dou(nb * 4), dov(nb * 4);
     PlanPtr plan;
     for (size_t r0 = 0; r0 < rows.size(); r0 += nb) {
       const int cnt = (int)std::min<size_t>(nb, rows.size() - r0);
       std::vector<int32_t> off(cnt);
-      std::vector<int> sup(cnt), zero(cnt, 0);
+      std::vector<int> sup(cnt), ou(cnt, 0), ov(cnt, 0);
       std::vector<double> w((size_t)cnt * ncp);
       for (int b = 0; b < cnt; ++b) {
         const int z = rows[r0 + b];
         const int k = track_per_os_ ? z : os_ * os_ * z;  // cfIndexFromSumOfWeightsRow
         off[b] = cf_->off[k];
         sup[b] = cf_->supp[k];
+        // getConvFuncOffset(iz) takes the sum-of-weights row (AWProjectVisGridder.cc:706), which
+        // is the plane before oversampling unless weights are tracked per oversampling plane;
+        // rows beyond the offsets give (0, 0) (TableVisGridder.cc:1428-1435)
+        if (!cf_->ofs_u.empty() && z < (int)cf_->ofs_u.size()) {
+          ou[b] = cf_->ofs_u[z];
+          ov[b] = cf_->ofs_v[z];
+        }
         for (int cp = 0; cp < ncp; ++cp) w[(size_t)b * ncp + cp] = sw[z + (size_t)nsw_rows_ * cp];
       }
       // the previous batch's kernels read doff/dsup/dw: wait for them before overwriting
       if (r0 > 0) YG_HIP(hipStreamSynchronize(c.stream));
       YG_HIP(hipMemcpy(doff.get(), off.data(), cnt * 4, hipMemcpyHostToDevice));
       YG_HIP(hipMemcpy(dsup.get(), sup.data(), cnt * 4, hipMemcpyHostToDevice));
-      YG_HIP(hipMemcpy(dzero.get(), zero.data(), cnt * 4, hipMemcpyHostToDevice));
+      YG_HIP(hipMemcpy(dou.get(), ou.data(), cnt * 4, hipMemcpyHostToDevice));
+      YG_HIP(hipMemcpy(dov.get(), ov.data(), cnt * 4, hipMemcpyHostToDevice));
       YG_HIP(hipMemcpy(dw.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice));
-      launch_place_cf(c, cf_->table.as<float2>(), doff.as<int32_t>(), dsup.as<int>(), dzero.as<int>(), dzero.as<int>(),
+      launch_place_cf(c, cf_->table.as<float2>(), doff.as<int32_t>(), dsup.as<int>(), dou.as<int>(), dov.as<int>(),
                       cnx, cny, cnt, planes.as<double2>());
       // fft2d(thisPlane, false) * (cnx*cny): the unnormalised backward transform
       fft_plain_z2z(c, planes.as<double2>(), cnx, cny, cnt, false, plan);

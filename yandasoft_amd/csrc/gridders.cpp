@@ -914,46 +914,8 @@ class SphFuncGridder : public Gridder {
 // ---------------------------------------------------------------------------------------
 // WProject (WProjectVisGridder.cc, WDependentGridderBase.cc)
 // ---------------------------------------------------------------------------------------
-struct WSamplingDesc {
-  double wmax = 35000;
-  int nw = 65;
-  int kind = 0;  // 0 linear, 1 powerlaw, 2 gaussian
-  double exponent = 1, ts = 1, amp = 1;
-  bool wmaxclip = false;
-  double wscale() const { return nw > 1 ? wmax / double((nw - 1) / 2) : 1.0; }
-  double map(double plane) const {
-    if (kind == 1) return plane > 0 ? std::pow(plane, exponent) : (plane < 0 ? -std::pow(-plane, exponent) : 0.);
-    if (kind == 2) {
-      const double a = amp * (1 - std::exp(-plane * plane / ts));
-      return plane < 0 ? -a : (plane > 0 ? a : 0.);
-    }
-    return plane;
-  }
-  // WDependentGridderBase::getWTerm (WDependentGridderBase.cc:157-167)
-  double wterm(int plane) const {
-    const int half = (nw - 1) / 2;
-    if (kind != 0 && nw > 1) return map(double(plane - half) / double(half)) * wmax;
-    return double(plane - half) * wscale();
-  }
-  // GaussianWSampling::calculateDistributionParameters (GaussianWSampling.cc:127-166)
-  void gaussian(double w50) {
-    auto target = [](double t) { return std::sqrt(-t * std::log(0.5 + 0.5 * std::exp(-1. / t))); };
-    YG_CHECK(w50 > 0 && w50 * std::sqrt(2.) < 1., 1, "wsampling.nwplanes50 out of range");
-    double lo = w50 * w50 / std::log(2.) / 2., hi = 0.2 / (1 / std::sqrt(2.) - w50);
-    if (hi < lo) { lo = hi; hi = 5.; }
-    double t = lo;
-    for (int it = 0; it < 100; ++it) {
-      t = (lo + hi) / 2.;
-      if (target(t) < w50) lo = t; else hi = t;
-      if (hi - lo < 1e-5) break;
-    }
-    ts = t;
-    amp = 1. / (1. - std::exp(-1. / ts));
-  }
-};
-
 // WDependentGridderBase::configureWSampling (WDependentGridderBase.cc:192-245)
-static void configure_wsampling(WSamplingDesc& ws, const Params& p) {
+void configure_wsampling(WSamplingDesc& ws, const Params& p) {
   const std::string smp = p.str("wsampling", "linear");
   if (smp == "powerlaw") {
     ws.kind = 1;
