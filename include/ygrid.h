@@ -291,10 +291,16 @@ int ygrid_lcf_num_params(const ygrid_lcf *l, int *n);
 /* parameters (host, FP32): per layer the weights (inputs x outputs, row-major) then the biases */
 int ygrid_lcf_get_params(const ygrid_lcf *l, float *out);
 int ygrid_lcf_set_params(ygrid_lcf *l, const float *in);
+/* forget the network's last evaluation: the next pass evaluates its samples' CFs even when they
+ * equal the last ones (a pass over the samples of the previous pass otherwise reuses its planes).
+ * No counterpart in the reference: its CFs are FFT tables. */
+int ygrid_lcf_clear_cache(ygrid_lcf *l);
 /* one forward and backward pass over n examples x (n x 5) with targets t (n x 2), host arrays:
  * loss = sum (y - t)^2 / (2n) and its gradient (num_params floats) */
 int ygrid_lcf_loss_grad(ygrid_lcf *l, const float *x, const float *t, int n, double *loss, float *grad);
-/* inference of n rows (host arrays); half: FP16 weights */
+/* inference of n rows (host arrays); half: 0 FP32, 1 FP16 weights with FP32 activations, 2 the
+ * fused FP16 matrix-core form (FP16 weights and activations, FP32 sums: the form gridding uses when
+ * the network has hidden width 128 and at most 5 layers) */
 int ygrid_lcf_forward(ygrid_lcf *l, const float *x, float *y, int n, int half);
 /* trains on the CF table of a W-projection gridder (which must have run a pass) */
 int ygrid_lcf_train_wproject(ygrid_lcf *l, const ygrid_gridder *wproject, const ygrid_lcf_train_opts *opts,

@@ -49,6 +49,20 @@ def forward(p, x, hidden, layers):
     return a @ W + b
 
 
+def forward_f16(p, x, hidden, layers, rnd=None):
+    """The rounding of the fused FP16 inference: weights and inputs rounded to FP16, every hidden
+    activation rounded to FP16 after its SiLU, biases as stored (FP32); products and sums here in
+    FP64 (the GPU sums in FP32).  rnd: the rounding (default: to numpy float16 and back)."""
+    if rnd is None:
+        rnd = lambda a: np.asarray(a).astype(np.float16).astype(np.float64)  # noqa: E731
+    layers_ = unpack(np.asarray(p, np.float64), hidden, layers)
+    a = rnd(np.asarray(x, np.float64))
+    for W, b in layers_[:-1]:
+        a = rnd(silu(a @ rnd(W) + np.float32(0) + b))
+    W, b = layers_[-1]
+    return a @ rnd(W) + b
+
+
 def loss_grad(p, x, t, hidden, layers):
     """(loss, gradient) in FP64."""
     p = np.asarray(p, np.float64)

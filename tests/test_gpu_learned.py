@@ -39,6 +39,13 @@ def test_forward_and_gradients_match_oracle(gpu_ctx):
         o += a * b + b
     ref16 = M.forward(p16, x, 128, 4)
     assert rel(y16, ref16) < 1e-5
+    # the fused FP16 matrix-core form (the gridders' inference): FP16 weights, inputs and hidden
+    # activations, FP32 sums -- against the oracle with the same roundings (FP64 sums; an
+    # activation whose FP32 and FP64 pre-activations round to neighbouring FP16 values differs by
+    # one step of 2^-11), and within the propagated rounding bound of the exact network
+    y2 = lcf.forward(x, half=2)
+    assert rel(y2, M.forward_f16(p, x, 128, 4)) < 1e-3
+    assert rel(y2, ref) < 4 * 5 * 2.0 ** -11
     loss, g = lcf.loss_grad(x, t)
     oloss, og = M.loss_grad(p, x, t, 128, 4)
     assert loss == pytest.approx(oloss, rel=1e-5)
@@ -87,6 +94,52 @@ def test_learned_cf_trains_and_beats_nearest_plane(gpu_ctx):
     print(f"[learned] grid error vs 1025-plane FP64 oracle: 9-plane table {e_coarse:.3e}, "
           f"129-plane table {e_fine:.3e}, network at each sample's w {e_learned:.3e}", flush=True)
     assert e_learned < e_coarse
+
+
+def test_learned_planes_are_evaluated_once_per_chunk(gpu_ctx):
+    """A pass over the samples of the network's last evaluation copies its planes instead of
+    evaluating again (a degridding after the gridding of the same chunk); other samples are
+    evaluated.  Results equal those of a fresh evaluation (to the float-atomic summation order)."""
+    announce("learned CF: one network evaluation per chunk shared by the grid and degrid passes")
+    import yandasoft_amd as ya
+    N, cell = 128, 60 * ARC
+    dA = make_data(400, 2, 1, N, cell, f0=1.0e9, frac=0.35, wmax_m=1000.0 * O.C_LIGHT / 1.005e9, seed=5)
+    dB = make_data(400, 2, 1, N, cell, f0=1.0e9, frac=0.35, wmax_m=1000.0 * O.C_LIGHT / 1.005e9, seed=6)
+    params = dict(WP, nwplanes=33)
+    tg = _grid(gpu_ctx, params, dA, N, cell)
+    lcf = ya.LearnedCF(gpu_ctx, hidden=128, layers=2, seed=2)
+    lcf.train_wproject(tg, epochs=3, batch=1024, lr=3e-3, seed=3, holdout_every=8)
+    rng = np.random.default_rng(4)
+    model = np.zeros((1, 1, N, N), np.float32)
+    model[0, 0, rng.integers(32, 96, 15), rng.integers(32, 96, 15)] = 1.0
+    img = ya.ImageDesc(N, N, 1, 1, cell, cell)
+
+    def degrid(d, clear):
+        if clear:
+            lcf.clear_cache()
+        g = ya.Gridder(gpu_ctx, "GPUWProject", params)
+        g.set_learned_cf(lcf, 1.0)
+        g.initialise_degrid(img, model)
+        out = np.zeros_like(d["vis"])
+        g.degrid(pchunk(d, vis_out=out))
+        assert g.stats().learned_cf_active == 1
+        return out
+
+    gA = _grid(gpu_ctx, params, dA, N, cell, lcf=lcf)   # evaluates A
+    reuse_A = degrid(dA, clear=False)                    # copies A's planes from the gridder's buffer
+    fresh_A = degrid(dA, clear=True)
+    assert rel(reuse_A, fresh_A) < 1e-6
+    after_A = degrid(dB, clear=False)                    # other samples: evaluated
+    fresh_B = degrid(dB, clear=True)
+    assert rel(after_A, fresh_B) < 1e-6
+    assert rel(fresh_A, fresh_B) > 1e-2                  # the two chunks' planes do differ
+    # the gridder's own buffer: a second pass over A reuses what it holds
+    gA.initialise_grid(img)
+    gA.grid(pchunk(dA))
+    g1 = gA.grid_array().copy()
+    gA.initialise_grid(img)
+    gA.grid(pchunk(dA))
+    assert rel(gA.grid_array(), g1) < 1e-6
 
 
 def test_learned_cf_falls_back_to_the_table(gpu_ctx):

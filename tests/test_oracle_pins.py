@@ -407,6 +407,21 @@ def test_mlp_oracle_gradient_matches_finite_differences():
     assert np.max(np.abs(fd - g)) <= 1e-7 * max(1.0, np.max(np.abs(g)))
 
 
+def test_mlp_oracle_fp16_rounding_form():
+    """forward_f16 is forward with FP16 rounding of weights, inputs and hidden activations: with the
+    rounding replaced by the identity it is forward exactly, and with FP16 rounding it stays within
+    the bound that rounding steps of 2^-11 propagated through the layers allow."""
+    from oracle import mlp as M
+    rng = np.random.default_rng(3)
+    d = M.dims(32, 3)
+    p = rng.normal(0, 0.15, sum(a * b + b for a, b in zip(d[:-1], d[1:])))
+    x = rng.uniform(-1, 1, (64, 5))
+    exact = M.forward(p, x, 32, 3)
+    assert np.array_equal(M.forward_f16(p, x, 32, 3, rnd=lambda a: np.asarray(a, np.float64)), exact)
+    d = np.max(np.abs(M.forward_f16(p, x, 32, 3) - exact)) / np.max(np.abs(exact))
+    assert 0 < d < 3 * 2.0 ** -11 * 4
+
+
 def test_mlp_oracle_silu_derivative():
     from oracle import mlp as M
     z = np.linspace(-8, 8, 101)

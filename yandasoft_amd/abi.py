@@ -21,6 +21,7 @@ EXPORTED = [
     "ygrid_batch_create", "ygrid_batch_destroy", "ygrid_batch_add", "ygrid_batch_set_graph", "ygrid_batch_run",
     "ygrid_batch_autotune", "ygrid_batch_info",
     "ygrid_lcf_create", "ygrid_lcf_destroy", "ygrid_lcf_num_params", "ygrid_lcf_get_params", "ygrid_lcf_set_params",
+    "ygrid_lcf_clear_cache",
     "ygrid_lcf_loss_grad", "ygrid_lcf_forward", "ygrid_lcf_train_wproject", "ygrid_gridder_set_learned_cf",
     "ygrid_imaging_chunk",
 ]
@@ -423,6 +424,10 @@ class LearnedCF:
         assert p.size == self.nparams
         _check(lib().ygrid_lcf_set_params(self.h, ctypes.c_void_p(p.ctypes.data)))
 
+    def clear_cache(self):
+        """Forget the last evaluation (the next pass evaluates the network)."""
+        _check(lib().ygrid_lcf_clear_cache(self.h))
+
     def loss_grad(self, x, t):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)
@@ -432,7 +437,8 @@ class LearnedCF:
                                          x.shape[0], ctypes.byref(loss), ctypes.c_void_p(g.ctypes.data)))
         return loss.value, g
 
-    def forward(self, x, half: bool = False) -> np.ndarray:
+    def forward(self, x, half=False) -> np.ndarray:
+        """half: False/0 FP32, True/1 FP16 weights, 2 the fused FP16 matrix-core inference."""
         x = np.ascontiguousarray(x, np.float32)
         y = np.zeros((x.shape[0], 2), np.float32)
         _check(lib().ygrid_lcf_forward(self.h, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(y.ctypes.data),

@@ -1003,10 +1003,10 @@ class WProjectGridder : public Gridder {
     const size_t bytes = (size_t)ps.nsamp * plane * cf_elem_bytes(cf_prec_);
     YG_CHECK(bytes < ((size_t)1 << 31) - ((size_t)1 << 24), 1,
              "learned CF planes of this chunk exceed 2 GiB: use fewer samples per chunk or cfprecision fp16");
-    if (lcf_buf_.table.size() < bytes || lcf_buf_.S != S || lcf_buf_.prec != cf_prec_) {
-      if (lcf_buf_.table.size() < bytes) lcf_buf_.table.release();
-      lcf_buf_.table.resize(bytes);
-      YG_HIP(hipMemsetAsync(lcf_buf_.table.get(), 0, lcf_buf_.table.size(), ctx_->stream));  // guard bands
+    if (!lcf_buf_.table || lcf_buf_.table->size() < bytes || lcf_buf_.S != S || lcf_buf_.prec != cf_prec_) {
+      // a new buffer (the network may still hold the old one as its last evaluation)
+      lcf_buf_.table = std::make_shared<DevBuf>(bytes);
+      YG_HIP(hipMemsetAsync(lcf_buf_.table->get(), 0, lcf_buf_.table->size(), ctx_->stream));  // guard bands
       lcf_buf_.S = S;
       lcf_buf_.prec = cf_prec_;
     }
@@ -1015,19 +1015,20 @@ class WProjectGridder : public Gridder {
     ps.pp.lcf_S = S;
     ps.pp.lcf_plane_elems = plane;
     ps.out.lcf_in = lcf_buf_.in.as<float4>();
-    ps.cfv = CFView{lcf_buf_.table.get(), lcf_buf_.table.size(), S, cf_prec_};
+    ps.cfv = CFView{lcf_buf_.table->get(), lcf_buf_.table->size(), S, cf_prec_};
   }
   void afterPrep(PassSetup& ps) override {
     if (!ps.out.lcf_in) return;
     const int S = ps.pp.lcf_S;
-    lcf_->eval_samples(ps.out.lcf_in, ps.nsamp, lcf_buf_.table.get(), 2 * S + 1 + 2 * kGuardU, ps.pp.lcf_plane_elems,
+    lcf_->eval_samples(ps.out.lcf_in, ps.nsamp, lcf_buf_.table, 2 * S + 1 + 2 * kGuardU, ps.pp.lcf_plane_elems,
                        ps.cfv.prec);
   }
 
  private:
   // per-gridder buffers of the learned planes (clones start without them)
   struct LcfBuffers {
-    DevBuf table, in;
+    std::shared_ptr<DevBuf> table;
+    DevBuf in;
     int S = -1, prec = -1;
     LcfBuffers() = default;
     LcfBuffers(const LcfBuffers&) {}

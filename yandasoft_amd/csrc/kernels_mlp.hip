@@ -9,6 +9,11 @@
 //   k_colsum      bias gradients;  k_mse  loss and its gradient;  k_adam  the optimiser step.
 //   k_lcf_inputs  per-sample network inputs;  k_lcf_scatter  network outputs into per-sample
 //                 padded CF planes (the layout the tile kernels read).
+//   k_lcf_fused   inference of a width-128 network in one kernel on the FP16 matrix cores
+//                 (v_mfma_f32_32x32x16_f16, FP32 accumulation): the weights of every layer sit
+//                 in LDS (packed by k_lcf_pack), a wave carries 64 rows through all layers in
+//                 registers, and with per-sample inputs it forms the inputs and writes the
+//                 outputs into the CF planes itself.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -232,6 +237,260 @@ __global__ void k_lcf_scatter(const float4* __restrict__ sin, const float* __res
   }
 }
 
+// ---- Fused FP16 inference (hidden width 128) ------------------------------------------------
+//
+// Layer products run transposed, out^T = W^T x^T: the MFMA's A operand is a 32 x 16 slice of W^T
+// (32 output neurons, 16 inputs), the B operand 16 inputs of 32 rows, and D a 32 neuron x 32 row
+// block: lane l holds row l % 32 and, in register r, neuron 8 (r / 4) + 4 (l / 32) + r % 4 of its
+// block of 32.  A lane's D registers are therefore the next layer's B operand for the same row
+// without any data movement, provided that layer's inputs are taken in that order: K step c,
+// half j of lane half h is input neuron nrn(h, 8c + j) (below), and the packed weights follow it.
+// The A and B operands of one instruction agree on which k each (lane half, half) holds, so the
+// instruction's own k order does not matter.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int kFH = 128;     // hidden width of the fused form
+constexpr int kFRows = 64;   // rows per wave and round (two blocks of 32)
+
+// input neuron held by lane half h at position rho = 8c + j of the K order
+__host__ __device__ __forceinline__ int fused_nrn(int h, int rho) {
+  return 32 * (rho >> 4) + 8 * ((rho & 15) >> 2) + 4 * h + (rho & 3);
+}
+
+struct FusedLayout {
+  int L;                       // hidden layers (dims 5, 128 x L, 2)
+  long long woff[17], boff[17];  // parameter offsets of each dense layer (0 .. L)
+};
+
+// LDS image, in uint4 (8 halfs): A0 [mb 4][lane 64] (K = the 5 inputs, input 8h + j), then per
+// hidden layer l = 1 .. L-1 [mb 4][c 8][lane 64], then the output layer [c 8][lane 64] (outputs 0
+// and 1 in rows 0, 1 of its 32-neuron block, zero rows otherwise); then FP32 biases [L][128] and
+// the output layer's 2 (padded to 4).
+__host__ __device__ inline long long fused_a0() { return 0; }
+__host__ __device__ inline long long fused_ah(int l) { return 256 + (long long)(l - 1) * 2048; }
+__host__ __device__ inline long long fused_al(int L) { return 256 + (long long)(L - 1) * 2048; }
+__host__ __device__ inline long long fused_u4(int L) { return fused_al(L) + 512 + (L * kFH + 4) / 4; }
+
+__global__ void k_lcf_pack(FusedLayout fl, const _Float16* __restrict__ p16, const float* __restrict__ p32,
+                           _Float16* __restrict__ img) {
+  const int L = fl.L;
+  const long long nh = (fused_al(L) + 512) * 8;  // halfs of the weight part
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nh) {
+    const int j = (int)(i & 7);
+    const long long u = i >> 3;  // uint4 index
+    const int lane = (int)(u & 63), h = lane >> 5, m = lane & 31;
+    _Float16 v = (_Float16)0.f;
+    if (u < 256) {
+      const int mb = (int)(u >> 6), f = 8 * h + j;
+      if (f < 5) v = p16[fl.woff[0] + (long long)f * kFH + 32 * mb + m];
+    } else if (u < fused_al(L)) {
+      const long long q = u - 256;
+      const int l = 1 + (int)(q / 2048), r = (int)(q % 2048);
+      const int mb = r / 512, c = (r / 64) & 7;
+      v = p16[fl.woff[l] + (long long)fused_nrn(h, 8 * c + j) * kFH + 32 * mb + m];
+    } else {
+      const int c = (int)((u - fused_al(L)) >> 6);
+      if (m < 2) v = p16[fl.woff[L] + (long long)fused_nrn(h, 8 * c + j) * 2 + m];
+    }
+    img[i] = v;
+  } else {
+    const long long b = i - nh;  // bias float index
+    float* bias = reinterpret_cast<float*>(img + nh);
+    if (b < (long long)L * kFH) bias[b] = p32[fl.boff[b / kFH] + b % kFH];
+    else if (b < (long long)L * kFH + 4) bias[b] = b - (long long)L * kFH < 2 ? p32[fl.boff[L] + (b - (long long)L * kFH)] : 0.f;
+  }
+}
+
+// *flag = 1 when the samples' network inputs differ from the cached ones (or n differs)
+__global__ void k_lcf_differ(const float4* __restrict__ a, const float4* __restrict__ b, int n, int* flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 x = a[i], y = b[i];
+  if (__float_as_uint(x.x) != __float_as_uint(y.x) || __float_as_uint(x.y) != __float_as_uint(y.y) ||
+      __float_as_uint(x.z) != __float_as_uint(y.z) || __float_as_uint(x.w) != __float_as_uint(y.w))
+    *flag = 1;
+}
+
+__global__ void k_lcf_keep(const float4* __restrict__ a, float4* __restrict__ b, int n, int* flag, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && flag) *flag = v;
+  if (i < n) b[i] = a[i];
+}
+
+struct FusedArgs {
+  const uint4* pack;
+  int pack_u4, L;
+  long long nrows;
+  const float* x;  // rows mode: nrows x 5 inputs, nrows x 2 outputs
+  float* y;
+  const float4* sin;  // samples mode (k_lcf_inputs / k_lcf_scatter conventions)
+  int S, P, prec;
+  float inv_os, inv_wnorm, inv_tscale;
+  long long plane_elems;
+  void* table;
+  // reuse: when *same is 0 the samples equal those of the last evaluation, whose planes are at
+  // src (null: already in table); the kernel then copies them instead of evaluating
+  const int* same;
+  const void* src;
+};
+
+__device__ __forceinline__ half8 as_half8(uint4 u) { return __builtin_bit_cast(half8, u); }
+
+// bias + SiLU of one 32-neuron block mb of a layer's D registers (both row blocks), packed as
+// K steps 2mb, 2mb + 1 of the next layer's B operands
+__device__ __forceinline__ void fused_epilogue(const f32x16& d0, const f32x16& d1, const float* __restrict__ bias,
+                                               int mb, int h, half8 (&nxt)[2][8]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 b4 = *reinterpret_cast<const float4*>(bias + 32 * mb + 8 * q + 4 * h);
+    const float bq[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int r = 4 * q + e;
+      const float z0 = d0[r] + bq[e], z1 = d1[r] + bq[e];
+      nxt[0][2 * mb + (r >> 3)][r & 7] = (_Float16)(z0 * (1.f / (1.f + __expf(-z0))));
+      nxt[1][2 * mb + (r >> 3)][r & 7] = (_Float16)(z1 * (1.f / (1.f + __expf(-z1))));
+    }
+  }
+}
+
+template <bool SAMPLES>
+__global__ __launch_bounds__(512, 1) void k_lcf_fused(FusedArgs a) {
+  extern __shared__ uint4 s_img[];
+  if (SAMPLES && a.same && *a.same == 0) {
+    if (a.src) {
+      const size_t n16 = (size_t)(a.nrows / ((2 * a.S + 1) * (2 * a.S + 1))) * (size_t)a.plane_elems *
+                         (a.prec == kPrecF32 ? 8 : 4) / 16;
+      const uint4* src = static_cast<const uint4*>(a.src);
+      uint4* dst = static_cast<uint4*>(a.table);
+      for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+    }
+    return;
+  }
+  for (int i = threadIdx.x; i < a.pack_u4; i += blockDim.x) s_img[i] = a.pack[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int h = lane >> 5, n = lane & 31;
+  const int L = a.L;
+  const uint4* A0 = s_img + fused_a0();
+  const uint4* AL = s_img + fused_al(L);
+  const float* bias = reinterpret_cast<const float*>(AL + 512);
+  const int W = 2 * a.S + 1, W2 = W * W;
+  const float invS = a.S > 0 ? 1.f / (float)a.S : 0.f;
+  const f32x16 zero = {};
+  for (long long base = ((long long)blockIdx.x * nwv + wv) * kFRows; base < a.nrows;
+       base += (long long)gridDim.x * nwv * kFRows) {
+    half8 bop[2][8];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      const long long r = base + 32 * rb + n;
+      float x[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      if (r < a.nrows) {
+        if constexpr (SAMPLES) {
+          const long long s = r / W2;
+          const int t = (int)(r - s * W2);
+          const float4 q = a.sin[s];
+          x[0] = q.x * a.inv_wnorm;
+          x[1] = q.y * a.inv_os;
+          x[2] = q.z * a.inv_os;
+          x[3] = (float)(t % W - a.S) * invS;
+          x[4] = (float)(t / W - a.S) * invS;
+        } else {
+#pragma unroll
+          for (int f = 0; f < 5; ++f) x[f] = a.x[r * 5 + f];
+        }
+      }
+      half8 v = {};
+      if (h == 0) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) v[f] = (_Float16)x[f];
+      }
+      bop[rb][0] = v;
+    }
+    // layer 0 (K = the 5 inputs), then the hidden layers, one 32-neuron block at a time so that
+    // only that block's sums are live beside the layer's inputs and outputs
+    {
+      half8 nxt[2][8];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const half8 w = as_half8(A0[mb * 64 + lane]);
+        const f32x16 d0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(w, bop[0][0], zero, 0, 0, 0);
+        const f32x16 d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(w, bop[1][0], zero, 0, 0, 0);
+        fused_epilogue(d0, d1, bias, mb, h, nxt);
+      }
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) bop[rb][c] = nxt[rb][c];
+    }
+    for (int l = 1; l < L; ++l) {
+      const uint4* Al = s_img + fused_ah(l);
+      half8 nxt[2][8];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        f32x16 d0 = zero, d1 = zero;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const half8 w = as_half8(Al[(mb * 8 + c) * 64 + lane]);
+          d0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(w, bop[0][c], d0, 0, 0, 0);
+          d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(w, bop[1][c], d1, 0, 0, 0);
+        }
+        fused_epilogue(d0, d1, bias + l * kFH, mb, h, nxt);
+        // keep the scheduler from hoisting the next block's weight reads over this one (they
+        // would all be live at once)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) bop[rb][c] = nxt[rb][c];
+    }
+    f32x16 out[2] = {zero, zero};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const half8 w = as_half8(AL[c * 64 + lane]);
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) out[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(w, bop[rb][c], out[rb], 0, 0, 0);
+    }
+    // outputs 0 and 1 of row n sit in registers 0 and 1 of the lanes of half 0
+    if (h == 0) {
+      const float b0 = bias[L * kFH], b1 = bias[L * kFH + 1];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const long long r = base + 32 * rb + n;
+        if (r >= a.nrows) continue;
+        const float y0 = out[rb][0] + b0, y1 = out[rb][1] + b1;
+        if constexpr (SAMPLES) {
+          const long long s = r / W2;
+          if (a.sin[s].w == 0.f) continue;  // not gridded: its plane is never read
+          const int t = (int)(r - s * W2);
+          const size_t e = (size_t)s * a.plane_elems + (size_t)(t / W + kGuardV) * a.P + (t % W + kGuardU);
+          const float re = y0 * a.inv_tscale, im = y1 * a.inv_tscale;
+          if (a.prec == kPrecF32) {
+            reinterpret_cast<float2*>(a.table)[e] = make_float2(re, im);
+          } else {
+            uint16_t hr, hi;
+            if (a.prec == kPrecF16) {
+              hr = __builtin_bit_cast(uint16_t, (_Float16)re);
+              hi = __builtin_bit_cast(uint16_t, (_Float16)im);
+            } else {
+              const uint32_t ur = __builtin_bit_cast(uint32_t, re), ui = __builtin_bit_cast(uint32_t, im);
+              hr = (uint16_t)((ur + 0x7fffu + ((ur >> 16) & 1u)) >> 16);
+              hi = (uint16_t)((ui + 0x7fffu + ((ui >> 16) & 1u)) >> 16);
+            }
+            reinterpret_cast<uint32_t*>(a.table)[e] = (uint32_t)hr | ((uint32_t)hi << 16);
+          }
+        } else {
+          a.y[2 * r] = y0;
+          a.y[2 * r + 1] = y1;
+        }
+      }
+    }
+  }
+}
+
 }  // namespace
 
 void mlp_gemm(const Ctx& c, bool ta, bool tb, const float* A, const void* B, bool b_half, float* C, int M, int N, int K,
@@ -317,6 +576,89 @@ void lcf_scatter(const Ctx& c, const float4* sin, const float* Y, int s0, int ns
   hipLaunchKernelGGL(k_lcf_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, sin, Y, s0, ns, S, P,
                      plane_elems, inv_tscale, table, prec);
   YG_HIP(hipGetLastError());
+}
+
+bool lcf_fused_supported(int hidden, int layers) { return hidden == kFH && layers >= 1 && fused_u4(layers) * 16 <= 160 * 1024; }
+
+size_t lcf_fused_pack_bytes(int layers) { return (size_t)fused_u4(layers) * 16; }
+
+void lcf_fused_pack(const Ctx& c, int layers, const long long* woff, const long long* boff, const void* p16,
+                    const float* p32, void* img) {
+  FusedLayout fl{};
+  fl.L = layers;
+  for (int l = 0; l <= layers; ++l) {
+    fl.woff[l] = woff[l];
+    fl.boff[l] = boff[l];
+  }
+  const long long n = (fused_al(layers) + 512) * 8 + (long long)layers * kFH + 4;
+  hipLaunchKernelGGL(k_lcf_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, fl,
+                     static_cast<const _Float16*>(p16), p32, static_cast<_Float16*>(img));
+  YG_HIP(hipGetLastError());
+}
+
+static void launch_fused(const Ctx& c, const FusedArgs& a, bool samples) {
+  if (a.nrows <= 0) return;
+  const size_t lds = (size_t)a.pack_u4 * 16;
+  static bool attr[2] = {false, false};
+  const void* fn = samples ? reinterpret_cast<const void*>(&k_lcf_fused<true>)
+                           : reinterpret_cast<const void*>(&k_lcf_fused<false>);
+  if (!attr[samples]) {
+    YG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr[samples] = true;
+  }
+  const long long waves = (a.nrows + kFRows - 1) / kFRows;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(c.n_cu, (waves + 7) / 8));
+  if (samples)
+    hipLaunchKernelGGL(k_lcf_fused<true>, dim3(blocks), dim3(512), lds, c.stream, a);
+  else
+    hipLaunchKernelGGL(k_lcf_fused<false>, dim3(blocks), dim3(512), lds, c.stream, a);
+  YG_HIP(hipGetLastError());
+}
+
+void lcf_fused_rows(const Ctx& c, const void* img, int layers, const float* x, float* y, long long n) {
+  FusedArgs a{};
+  a.pack = static_cast<const uint4*>(img);
+  a.pack_u4 = (int)fused_u4(layers);
+  a.L = layers;
+  a.nrows = n;
+  a.x = x;
+  a.y = y;
+  launch_fused(c, a, false);
+}
+
+void lcf_differ(const Ctx& c, const float4* a, const float4* b, int n, int* flag, bool compare) {
+  // flag = !compare, then (compare) set where the inputs differ
+  hipLaunchKernelGGL(k_lcf_keep, dim3(1), dim3(64), 0, c.stream, a, (float4*)nullptr, 0, flag, compare ? 0 : 1);
+  if (compare && n > 0)
+    hipLaunchKernelGGL(k_lcf_differ, dim3((n + 255) / 256), dim3(256), 0, c.stream, a, b, n, flag);
+  YG_HIP(hipGetLastError());
+}
+
+void lcf_keep(const Ctx& c, const float4* a, float4* b, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_lcf_keep, dim3((n + 255) / 256), dim3(256), 0, c.stream, a, b, n, (int*)nullptr, 0);
+  YG_HIP(hipGetLastError());
+}
+
+void lcf_fused_samples(const Ctx& c, const void* img, int layers, const float4* sin, int ns, int S, float inv_os,
+                       float inv_wnorm, int P, long long plane_elems, float inv_tscale, void* table, int prec,
+                       const int* same, const void* src) {
+  FusedArgs a{};
+  a.same = same;
+  a.src = src;
+  a.pack = static_cast<const uint4*>(img);
+  a.pack_u4 = (int)fused_u4(layers);
+  a.L = layers;
+  a.nrows = (long long)ns * (2 * S + 1) * (2 * S + 1);
+  a.sin = sin;
+  a.S = S;
+  a.P = P;
+  a.prec = prec;
+  a.inv_os = inv_os;
+  a.inv_wnorm = inv_wnorm;
+  a.inv_tscale = inv_tscale;
+  a.plane_elems = plane_elems;
+  a.table = table;
+  launch_fused(c, a, true);
 }
 
 }  // namespace ygrid

@@ -61,7 +61,21 @@ void LearnedCF::set_params(const float* host) {
   YG_HIP(hipMemsetAsync(m_.get(), 0, nparams_ * sizeof(float), ctx_->stream));
   YG_HIP(hipMemsetAsync(v_.get(), 0, nparams_ * sizeof(float), ctx_->stream));
   adam_step_ = 0;
+  ++version_;
   YG_HIP(hipStreamSynchronize(ctx_->stream));
+}
+
+bool LearnedCF::fused() const { return lcf_fused_supported(H_, L_); }
+
+// The weights in the layout of the fused kernel's LDS image, rebuilt after the weights change.
+const void* LearnedCF::packed() {
+  if (packed_version_ != version_) {
+    fused_img_.resize(lcf_fused_pack_bytes(L_));
+    std::vector<long long> wo(woff_.begin(), woff_.end()), bo(boff_.begin(), boff_.end());
+    lcf_fused_pack(*ctx_, L_, wo.data(), bo.data(), p16_.get(), p_.as<float>(), fused_img_.get());
+    packed_version_ = version_;
+  }
+  return fused_img_.get();
 }
 
 void LearnedCF::ensure_train_buffers(int M) {
@@ -158,13 +172,19 @@ double LearnedCF::loss_and_grads_host(const float* x, const float* t, int n, flo
   return loss;
 }
 
-void LearnedCF::forward_host(const float* x, float* y, int n, bool half) {
+void LearnedCF::forward_host(const float* x, float* y, int n, int mode) {
   YG_CHECK(n > 0, 1, "no rows");
+  YG_CHECK(mode >= 0 && mode <= 2, 1, "inference mode is 0 (FP32), 1 (FP16 weights) or 2 (FP16 matrix cores)");
+  YG_CHECK(mode != 2 || fused(), 1, "the FP16 matrix-core inference needs hidden width 128 and at most 5 layers");
   DevBuf X((size_t)n * kLcfIn * sizeof(float)), Y((size_t)n * 2 * sizeof(float));
-  ea_.resize((size_t)n * H_ * sizeof(float));
-  eb_.resize((size_t)n * H_ * sizeof(float));
   YG_HIP(hipMemcpy(X.get(), x, (size_t)n * kLcfIn * sizeof(float), hipMemcpyHostToDevice));
-  forward_dev(X.as<float>(), n, half, false, Y.as<float>());
+  if (mode == 2) {
+    lcf_fused_rows(*ctx_, packed(), L_, X.as<float>(), Y.as<float>(), n);
+  } else {
+    ea_.resize((size_t)n * H_ * sizeof(float));
+    eb_.resize((size_t)n * H_ * sizeof(float));
+    forward_dev(X.as<float>(), n, mode == 1, false, Y.as<float>());
+  }
   YG_HIP(hipStreamSynchronize(ctx_->stream));
   YG_HIP(hipMemcpy(y, Y.get(), (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost));
 }
@@ -293,7 +313,8 @@ LcfReport LearnedCF::train_wproject(const Gridder& wp, const LcfTrainOptions& o)
   const int NH = (int)(HX.size() / kLcfIn);
   if (NH > 0) {
     std::vector<float> y((size_t)NH * 2);
-    forward_host(HX.data(), y.data(), NH, true);
+    ++version_;  // new weights: the packed copy is stale
+    forward_host(HX.data(), y.data(), NH, fused() ? 2 : 1);
     double mx = 0, ss = 0;
     for (int i = 0; i < NH; ++i) {
       // targets are the taps divided by the table's peak: errors are in units of the peak
@@ -327,8 +348,31 @@ void LearnedCF::reserve_eval(int S) {
 
 // Kernels only (capturable): chunks of samples through the network (FP16 weights), outputs
 // into the samples' padded planes.
-void LearnedCF::eval_samples(const float4* sin, int nsamp, void* table, int P, long long plane_elems, int prec) {
+void LearnedCF::eval_samples(const float4* sin, int nsamp, const std::shared_ptr<DevBuf>& tbuf, int P,
+                             long long plane_elems, int prec) {
   const int S = support;
+  void* table = tbuf->get();
+  if (fused()) {
+    const Ctx& c = *ctx_;
+    EvalKey key;
+    key.nsamp = nsamp;
+    key.S = S;
+    key.P = P;
+    key.prec = prec;
+    key.plane_elems = plane_elems;
+    key.version = version_;
+    const bool comparable = last_planes_ && key == last_key_;
+    same_.resize(sizeof(int));
+    last_in_.resize((size_t)std::max(nsamp, 1) * sizeof(float4));
+    lcf_differ(c, sin, last_in_.as<float4>(), nsamp, same_.as<int>(), comparable);
+    const void* src = comparable && last_planes_ != tbuf ? last_planes_->get() : nullptr;
+    lcf_fused_samples(c, packed(), L_, sin, nsamp, S, 1.f / oversample, (float)(1.0 / wnorm), P, plane_elems,
+                      1.f / tscale, table, prec, same_.as<int>(), src);
+    lcf_keep(c, sin, last_in_.as<float4>(), nsamp);
+    last_planes_ = tbuf;
+    last_key_ = key;
+    return;
+  }
   const int W2 = (2 * S + 1) * (2 * S + 1);
   reserve_eval(S);
   const int ns = std::max(1, eval_rows_ / W2);

@@ -395,6 +395,13 @@ int ygrid_lcf_set_params(ygrid_lcf* l, const float* in) {
   });
 }
 
+int ygrid_lcf_clear_cache(ygrid_lcf* l) {
+  return guard([&] {
+    NEED(l);
+    l->l->clear_cache();
+  });
+}
+
 int ygrid_lcf_loss_grad(ygrid_lcf* l, const float* x, const float* t, int n, double* loss, float* grad) {
   return guard([&] {
     NEED(l);
@@ -410,7 +417,7 @@ int ygrid_lcf_forward(ygrid_lcf* l, const float* x, float* y, int n, int half) {
     NEED(l);
     NEED(x);
     NEED(y);
-    l->l->forward_host(x, y, n, half != 0);
+    l->l->forward_host(x, y, n, half);
   });
 }
 
