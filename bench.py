@@ -374,6 +374,12 @@ def main():
         dname = f"k_degrid_mm<{wl.npol}, {s1d.cf_precision}>"
     else:
         dname = f"k_degrid<{wl.npol}, {s1d.degrid_waves}, {tf(s1d.degrid_pipeline)}, {s1d.cf_precision}>"
+    peak = FP32_PEAK_TFLOPS
+    if args.cf_precision != "fp32":
+        # 16-bit taps and coefficients (and model grid): the 16-bit matrix-core kernels
+        gname = f"k_grid16<{wl.npol}, {s1g.cf_precision}>"
+        dname = f"k_degrid16<{wl.npol}, {s1d.cf_precision}>"
+        peak = FP16_PEAK_TFLOPS
     if gk_ms >= dk_ms:
         kname, kms, kt = gname, gk_ms, per_launch_g
     else:
@@ -401,8 +407,8 @@ def main():
         "phases_ms_per_step": {"grid_kernel": gk_ms, "degrid_kernel": dk_ms,
                                "prep": (s1g.ms_prep + s1d.ms_prep) / args.steps,
                                "binning": (s1g.ms_bin + s1d.ms_bin) / args.steps},
-        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": profiled_traffic(kname),
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": profiled_traffic(kname),
                      "traffic_source": PROFILE},
     }
     if not args.no_cpu_baseline:

@@ -60,11 +60,18 @@ def round_c(z, prec):
     return (round_np(z.real, prec) + 1j * round_np(z.imag, prec)).astype(np.complex64)
 
 
+@pytest.mark.parametrize("tiles", [False, True])
 @pytest.mark.parametrize("cfp,visp", [("fp32", "fp32"), ("fp16", "fp32"), ("bf16", "fp32"), ("fp16", "fp16"),
                                       ("bf16", "bf16")])
-def test_single_sample_grids_rounded_cf(gpu_ctx, cfp, visp):
-    announce(f"single sample, cfprecision={cfp} visprecision={visp}")
+def test_single_sample_grids_rounded_cf(gpu_ctx, cfp, visp, tiles):
+    """tiles: the 64x64 tile kernels instead of the small-support ones (support 3); with both
+    storage precisions 16-bit they multiply on the 16-bit matrix cores, whose FP32 sums of
+    16-bit products are exact here too."""
+    announce(f"single sample, cfprecision={cfp} visprecision={visp}, tile kernels {tiles}")
     import yandasoft_amd as ya
+    if tiles:
+        gpu_ctx = ya.Context(0)
+        gpu_ctx.set_small_support(0)
     N, cell, f = 256, 30 * ARC, 1.4e9
     du = 1.0 / (cell * N)
     lam = O.C_LIGHT / f
@@ -116,8 +123,8 @@ def _wproject_errors(gpu_ctx, prec, visprec=None):
         eh[p] = inner_err(g.finalise_grid(), img64, gain)
     # degrid with the same storage precision against the FP64 oracle
     model = random_model(N, 4)
-    gv, _, _, _ = degrid_both(gpu_ctx, "GPUWProject", dict(WP, cfprecision=prec), dict(kind="WProject", wproject=prm),
-                              d, N, cell, 4, model)
+    gv, _, _, _ = degrid_both(gpu_ctx, "GPUWProject", dict(WP, cfprecision=prec, visprecision=visprec or prec),
+                              dict(kind="WProject", wproject=prm), d, N, cell, 4, model)
     od = O.OracleGridder("WProject", "f64", wproject=prm)
     od.initialise_degrid((N, N, 4, 1), (cell, cell), model)
     ov64 = od.degrid(ochunk(d))
@@ -133,6 +140,19 @@ def test_wproject_mixed_precision_error_vs_fp64(gpu_ctx, prec):
     eg, eh, ev = _wproject_errors(gpu_ctx, prec)
     assert eg[prec] <= BOUND[prec] and eh[prec] <= BOUND[prec] and ev <= BOUND[prec]
     assert eg[prec] > 10 * eg["fp32"]  # the 16-bit mode is in effect
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_wproject_16bit_matrix_cores_and_widened_form_agree(gpu_ctx, prec):
+    """visprecision equal to cfprecision runs the 16-bit matrix-core tile kernels (coefficients,
+    and when degridding the model grid, rounded to the same format); visprecision fp32 runs the
+    FP32 ones on widened taps.  Both stay within the precision's bound against FP64."""
+    announce(f"wproject {prec}: 16-bit matrix-core form vs widened form")
+    eg, eh, ev = _wproject_errors(gpu_ctx, prec, visprec="fp32")
+    assert eg[prec] <= BOUND[prec] and eh[prec] <= BOUND[prec] and ev <= BOUND[prec]
+    eg2, eh2, ev2 = _wproject_errors(gpu_ctx, prec, visprec=prec)
+    assert eg2[prec] <= BOUND[prec] and eh2[prec] <= BOUND[prec] and ev2 <= BOUND[prec]
+    assert eg2[prec] != eg[prec]  # the coefficients' rounding is in effect
 
 
 @pytest.mark.xfail(strict=True, reason="FP16 CF taps (11-bit mantissa) give 2-5e-4 of the peak against the FP64 "

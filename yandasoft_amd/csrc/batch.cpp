@@ -63,14 +63,14 @@ void Batch::prepare() {
     const Gridder::PassSetup& ps = p.ps;
     const int nkz = ps.pp.nzg * p.g->shape_[3];
     size_static_bins(*ctx_, p.sb, ps.d.nrow, ps.d.nchan, p.g->shape_[0], p.g->shape_[1], nkz, ps.cfv, chunk);
-    const auto key = std::make_tuple((int)p.forward, p.g->shape_[2], ps.cfv.prec);
+    const auto key = std::make_tuple((int)p.forward, p.g->shape_[2], tile_prec(ps.cfv));
     auto it = gidx.find(key);
     if (it == gidx.end()) {
       it = gidx.emplace(key, (int)groups_.size()).first;
       Group gr;
       gr.forward = p.forward;
       gr.npol = p.g->shape_[2];
-      gr.prec = ps.cfv.prec;
+      gr.prec = tile_prec(ps.cfv);
       groups_.push_back(std::move(gr));
     }
     Group& gr = groups_[it->second];

@@ -194,7 +194,9 @@ CFView Gridder::cf_view() {
     launch_cf_round(*ctx_, cf_->table.as<float2>(), cf_->table16.as<uint32_t>(), cf_->n_elems, cf_prec_);
     cf_->table16_prec = cf_prec_;
   }
-  return CFView{cf_->table16.get(), cf_->table16.size(), smax, cf_prec_};
+  CFView v{cf_->table16.get(), cf_->table16.size(), smax, cf_prec_, vis_prec_ == cf_prec_ ? 1 : 0};
+  if (degrid_mode_ && cf_prec_ == kPrecF16 && gscale_.get()) v.gscale = gscale_.as<float>() + 1;
+  return v;
 }
 
 void Gridder::initialiseCommon(const ygrid_image_desc& img) {
@@ -753,6 +755,12 @@ void Gridder::initialiseDegrid(const ygrid_image_desc& img, const float* model, 
   const double* cx = doesCorrection() ? ccfx_.as<double>() : nullptr;
   const double* cy = doesCorrection() ? ccfy_.as<double>() : nullptr;
   modelToGrids(dmodel, cx, cy);
+  if (cf_prec_ == kPrecF16) {
+    // the power-of-two scale that keeps the model grid inside FP16's range when the 16-bit
+    // matrix-core degridding rounds it
+    gscale_.resize(2 * sizeof(uint32_t));
+    launch_grid_scale16(*ctx_, grid_.as<float>(), 2 * ng, gscale_.as<uint32_t>(), gscale_.as<float>() + 1);
+  }
   YG_HIP(hipStreamSynchronize(ctx_->stream));
   scratch_.release();
 }
@@ -1015,7 +1023,9 @@ class WProjectGridder : public Gridder {
     ps.pp.lcf_S = S;
     ps.pp.lcf_plane_elems = plane;
     ps.out.lcf_in = lcf_buf_.in.as<float4>();
-    ps.cfv = CFView{lcf_buf_.table->get(), lcf_buf_.table->size(), S, cf_prec_};
+    ps.cfv = CFView{lcf_buf_.table->get(), lcf_buf_.table->size(), S, cf_prec_,
+                    cf_prec_ != kPrecF32 && vis_prec_ == cf_prec_ ? 1 : 0};
+    if (degrid_mode_ && cf_prec_ == kPrecF16 && gscale_.get()) ps.cfv.gscale = gscale_.as<float>() + 1;
   }
   void afterPrep(PassSetup& ps) override {
     if (!ps.out.lcf_in) return;

@@ -277,6 +277,7 @@ class Gridder {
   CFView cf_view();
   int cf_prec_ = kPrecF32, vis_prec_ = kPrecF32;  // storage precisions (cfprecision, visprecision)
   DevBuf smp_coef16_;
+  DevBuf gscale_;  // degridding, FP16 tables: max |model grid| bits and its power-of-two scale
   double n_bin_reused_ = 0;  // passes that reused the context's cached binning
   double n_small_ = 0;       // passes that ran the small-support kernels
   // A-projection CF cache statistics (itsNumberOfIterations, itsNumberOfCFGenerations[DueToPA/Freq])

@@ -1353,6 +1353,425 @@ __global__ __launch_bounds__(256, 2) void k_degrid_mm(const int4* __restrict__ i
   }
 }
 
+// ---- 16-bit tables on the 16-bit matrix cores --------------------------------------------
+//
+// v_mfma_f32_4x4x4_16b_f16 (bf16: _4x4x4bf16_1k) takes four 16-bit values per lane as A and as B
+// and sums their products in FP32: C_b[i][o] += sum_k A_b[i][k] B_b[k][o] for the 16 blocks b of
+// four lanes.  A 16-bit CF tap is one dword (re, im), so the two taps a lane loads (8 bytes) are
+// an A operand as they arrive.
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef short short4v __attribute__((ext_vector_type(4)));
+
+template <int CFP>
+__device__ __forceinline__ f32x4 mma16(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1, f32x4 c) {
+  const uint2 a = make_uint2(a0, a1), b = make_uint2(b0, b1);
+  if constexpr (CFP == kPrecF16)
+    return __builtin_amdgcn_mfma_f32_4x4x4f16(__builtin_bit_cast(half4v, a), __builtin_bit_cast(half4v, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(__builtin_bit_cast(short4v, a), __builtin_bit_cast(short4v, b), c, 0,
+                                                 0, 0);
+}
+
+// (lo, hi) rounded to nearest even in the 16-bit format prec, as one dword
+__device__ __forceinline__ uint32_t pack16(float lo, float hi, int prec) {
+  if (prec == kPrecBF16)
+    return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)lo) | ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)hi) << 16);
+  return (uint32_t)round16(lo, prec) | ((uint32_t)round16(hi, prec) << 16);
+}
+
+// Degridding with 16-bit taps (the visibilities' side also 16-bit): groups of four pairs as in
+// k_degrid_mm, with lane l = 4b + j holding pair j's four taps at the tile columns 4b .. 4b + 3 of
+// a row (one 16-byte load; a load instruction covers a whole row of the tile for four pairs).  K =
+// (re, im) of two of those taps, so one MFMA forms the complete complex products of four pairs,
+// two pixels and four outputs: the B operand of output j at pixels x, x + 1 is
+// (G0.re, G0.im, G1.re, G1.im) for the real part of C * conj(G) and (-G0.im, G0.re, -G1.im, G1.re)
+// for the imaginary part (GridKernel.cc:121-122).  The model-grid tile is rounded to the table's
+// 16-bit format once per item; FP16 scales it first by the power of two the degridder computed
+// for its grid (largest magnitude to [2^13, 2^14), exact), which the sums undo.
+struct Quad16 {
+  uint32_t base;  // byte offset of the lane's pair's tap under the wave's first pixel (column 4b)
+  uint32_t rowb;  // bytes per CF row of that pair's plane
+  uint32_t mask;  // that pair's row blocks of this wave (bit k), 0 where the lane's taps miss it
+  uint32_t um;    // union of the four pairs' row blocks (wave-uniform)
+  int sid;
+};
+
+__device__ __forceinline__ Quad16 quad16(const int4* s_desc, int g0, int n, int w, int j, int b) {
+  Quad16 q;
+  const bool valid = g0 + j < n;
+  const int4 d = s_desc[valid ? g0 + j : g0];
+  const uint32_t pw = (uint32_t)d.w;
+  const int P = (int)(pw & 0xffffu), A = (int)(pw << 8) >> 24, B = (int)pw >> 24;
+  const uint32_t m8 = valid ? ((uint32_t)d.z >> (8 * w)) & 0xffu : 0u;
+  uint32_t m4 = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) m4 |= ((m8 >> (2 * k)) & 3u) ? (1u << k) : 0u;
+  q.um = (uint32_t)__builtin_amdgcn_readlane((int)m4, 0) | (uint32_t)__builtin_amdgcn_readlane((int)m4, 1) |
+         (uint32_t)__builtin_amdgcn_readlane((int)m4, 2) | (uint32_t)__builtin_amdgcn_readlane((int)m4, 3);
+  const int x = 4 * b;
+  q.mask = (x >= A - 2 && x <= B) ? m4 : 0u;
+  q.base = (uint32_t)d.y + (uint32_t)w * ((uint32_t)P << 6) + ((uint32_t)x << 2);  // 16 rows of 4-byte taps
+  q.rowb = (uint32_t)P << 2;
+  q.sid = d.x;
+  return q;
+}
+
+// the four taps of rows 4k .. 4k + 3 of the lane's pair and column quad (zeros where it misses)
+__device__ __forceinline__ void load_q16(uint32_t (&t)[4][4], __amdgpu_buffer_rsrc_t cfr, const Quad16& q, int k) {
+  if (q.um & (1u << k)) {
+    const bool on = (q.mask >> k) & 1u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t vo = on ? q.base + (uint32_t)(4 * k + r) * q.rowb : kOutOfRange;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) t[r][e] = __builtin_amdgcn_raw_buffer_load_b32(cfr, vo + 4u * e, 0, 0);
+    }
+  }
+}
+
+template <int NP, int CFP>
+__global__ __launch_bounds__(256, 2) void k_degrid16(const int4* __restrict__ items, const int* __restrict__ nitems_p,
+                                                   int* __restrict__ counter, const TileJob* __restrict__ jobs) {
+  constexpr int M = (NP + 1) / 2;
+  __shared__ int s_item;
+  __shared__ int4 s_desc[kChunk];
+  __shared__ float s_part[kDegridBatch][4][8];
+  __shared__ int s_sid[kDegridBatch];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int b = lane >> 2, j = lane & 3;
+  const int nitems = *nitems_p;
+  int queue = blockIdx.x % kItemQueues, queues_left = kItemQueues;
+  for (;;) {
+    const int item = next_item(counter, nitems, queue, queues_left, &s_item);
+    if (item < 0) break;
+    const int4 it = items[item];
+    const TileJob& J = jobs[it.w];
+    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
+    const int nx = J.nx, ny = J.ny;
+    const int tile = it.x % J.ntiles;
+    const int kz = it.x / J.ntiles;
+    const int ichan = kz / J.nzg;
+    const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
+    const size_t plane = (size_t)nx * ny;
+    const float* __restrict__ gridf = J.grid + 2 * J.wstride * (size_t)(kz % J.nzg);
+    float* __restrict__ acc_out = J.acc;
+
+    // lane (b, j), B operands gt[r][e][m]: model grid of polarisation 2m + (j >> 1) at the tile
+    // pixels (16w + r, 4b + 2e) and (16w + r, 4b + 2e + 1)
+    const float scale = (CFP == kPrecF16 && J.gscale) ? *J.gscale : 1.f;
+    const float inv_scale = 1.f / scale;  // a power of two: exact
+    uint2 gt[16][2][M];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int v = v0 + 16 * w + r;
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int q = 2 * m + (j >> 1);
+          float2 g2[2];
+#pragma unroll
+          for (int f = 0; f < 2; ++f) {
+            const int u = u0 + 4 * b + 2 * e + f;
+            g2[f] = (u < nx && v < ny && q < NP)
+                        ? *reinterpret_cast<const float2*>(gridf +
+                                                           2 * ((size_t)(ichan * NP + q) * plane + (size_t)v * nx + u))
+                        : make_float2(0.f, 0.f);
+            g2[f].x *= scale;
+            g2[f].y *= scale;
+          }
+          if (j & 1)
+            gt[r][e][m] = make_uint2(pack16(-g2[0].y, g2[0].x, CFP), pack16(-g2[1].y, g2[1].x, CFP));
+          else
+            gt[r][e][m] = make_uint2(pack16(g2[0].x, g2[0].y, CFP), pack16(g2[1].x, g2[1].y, CFP));
+        }
+    }
+
+    const int nall = it.z - it.y;
+    for (int cb = 0; cb < nall; cb += kChunk) {
+      const int n = min(kChunk, nall - cb);
+      if (cb > 0) __syncthreads();  // every wave is done with the previous round's staging
+      stage_item<false>(J.descs, Samples{}, it.y + cb, n, s_desc, nullptr, tid, blockDim.x);
+      uint32_t tA[4][4], tB[4][4];
+      Quad16 lq = quad16(s_desc, 0, n, w, j, b);
+      load_q16(tA, cfr, lq, 0);
+      for (int g0 = 0; g0 < n; g0 += 4) {
+        const Quad16 cq = lq;
+        f32x4 D[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) D[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        auto mfma_rows = [&](const uint32_t (&t)[4][4], int k) {
+          if (cq.um & (1u << k)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+              for (int e = 0; e < 2; ++e)
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                  D[m] = mma16<CFP>(t[r][2 * e], t[r][2 * e + 1], gt[4 * k + r][e][m].x, gt[4 * k + r][e][m].y, D[m]);
+          }
+        };
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        load_q16(tB, cfr, cq, 1);
+        mfma_rows(tA, 0);
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        load_q16(tA, cfr, cq, 2);
+        mfma_rows(tB, 1);
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        load_q16(tB, cfr, cq, 3);
+        mfma_rows(tA, 2);
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        if (g0 + 4 < n) {
+          lq = quad16(s_desc, g0 + 4, n, w, j, b);
+          load_q16(tA, cfr, lq, 0);
+        }
+        mfma_rows(tB, 3);
+
+        // lane (b, j), value 4m + i: output j (re / im of polarisation 2m + (j >> 1)) of pair
+        // g0 + i over block b; then summed over the 16 blocks
+        float val[4 * M];
+#pragma unroll
+        for (int t = 0; t < 4 * M; ++t) val[t] = 0.f;
+        if (cq.um) {
+#pragma unroll
+          for (int m = 0; m < M; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) val[4 * m + i] = D[m][i] * inv_scale;
+#pragma unroll
+          for (int t = 0; t < 4 * M; ++t) {
+            float r = val[t];
+            r += dpp<0x124>(r);  // row_ror:4
+            r += dpp<0x128>(r);  // row_ror:8: the four blocks of the row
+            const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(r), __float_as_uint(r), false, false);
+            r = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+            const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(r), __float_as_uint(r), false, false);
+            val[t] = __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+          }
+        }
+        const int gn = min(4, n - g0);
+        if (lane < 4) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (i < gn) {
+#pragma unroll
+              for (int m = 0; m < M; ++m) s_part[(g0 + i) % kDegridBatch][w][4 * m + j] = val[4 * m + i];
+            }
+        }
+        if (w == 0 && b == 0 && j < gn) s_sid[(g0 + j) % kDegridBatch] = cq.sid;
+        const int gend = g0 + gn;
+        if (gend % kDegridBatch == 0 || gend == n) {
+          __syncthreads();
+          const int nb = gend - (g0 / kDegridBatch) * kDegridBatch;
+          for (int t = tid; t < nb * 2 * NP; t += blockDim.x) {
+            const int bb = t / (2 * NP), q = t % (2 * NP);
+            const float sum = s_part[bb][0][q] + s_part[bb][1][q] + s_part[bb][2][q] + s_part[bb][3][q];
+            if (sum != 0.f) atomicAdd(&acc_out[(size_t)s_sid[bb] * 8 + q], sum);
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+}
+
+// Gridding with 16-bit taps and 16-bit gridding coefficients, two pairs per MFMA.  Lane layout:
+// wave w owns rows 16w .. 16w + 15 of the 64x64 tile in 4 row blocks k of 4 rows; lane (lu, lv) =
+// (lane & 15, lane >> 4) holds the four pixels (row 4k + lv, columns 4lu .. 4lu + 3) of every
+// block, whose four taps are one 16-byte load (one load instruction covers a whole 4 x 64 block,
+// half the load instructions of two-tap lanes).  For tap e of the lane: K = (re, im of pair p's
+// tap, re, im of pair q's tap) and B = (the coefficients that multiply them for output o: two of
+// pair p's, two of pair q's), which are a dword of each pair's 16-bit coefficients as staged, so
+// one instruction adds both pairs' contributions to 4 pixels x 4 outputs and neither operand
+// needs repacking.  Pairs are taken two at a time from the wave's list (the second of an odd
+// count is empty); a block only one of the two covers gets zero taps for the other.
+struct Geom16 {
+  uint32_t mask;  // bit k: row block k of the wave overlaps the footprint
+  uint32_t soff0, rowstep;
+  uint32_t vo;    // lane offset of its four taps (out of range: none of them in the footprint)
+};
+
+__device__ __forceinline__ Geom16 geom16(const int4& raw, int w, int lu, int lv, bool valid) {
+  Geom16 g;
+  const PairU p = make_uniform(raw);
+  const PairGeom pg = pair_geom<2>(p, w);
+  uint32_t m4 = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) m4 |= ((pg.mask >> (2 * k)) & 3u) ? (1u << k) : 0u;
+  g.mask = valid ? m4 : 0u;
+  g.soff0 = pg.soff0;
+  g.rowstep = pg.rowstep;
+  // taps x .. x + 3 (x = 4 lu) meet the footprint when A - 2 <= x <= B (A, B as for column pairs);
+  // the guard bands hold the up to 3 taps beyond either end
+  const int x = 4 * lu;
+  g.vo = (x >= pg.A - 2 && x <= pg.B) ? (uint32_t)(lv * pg.P + x) << 2 : kOutOfRange;
+  return g;
+}
+
+// the four taps (dwords) of the lane in the blocks of um, zero where g does not cover the block
+__device__ __forceinline__ void load16(uint32_t (&t)[4][4], __amdgpu_buffer_rsrc_t cfr, const Geom16& g, uint32_t um) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (um & (1u << k)) {
+      if (g.mask & (1u << k)) {
+        const uint32_t so = g.soff0 + k * g.rowstep;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) t[k][e] = __builtin_amdgcn_raw_buffer_load_b32(cfr, g.vo + 4u * e, so, 0);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) t[k][e] = 0u;
+      }
+    }
+}
+
+template <int NP, int CFP>
+__global__ __launch_bounds__(256, 2) void k_grid16(const int4* __restrict__ items, const int* __restrict__ nitems_p,
+                                                 int* __restrict__ counter, const TileJob* __restrict__ jobs) {
+  __shared__ int s_item;
+  __shared__ int4 s_desc[kChunk];
+  __shared__ uint2 s_cf16[kChunk * 4];    // per staged pair and output o: its 16-bit coefficients
+  __shared__ float s_flush[4 * kFlushWave];
+  __shared__ uint16_t s_list[4][kChunk];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lu = lane & 15, lv = lane >> 4;
+  const int o = lane & 3;
+  const int ob = (lane >> 2) & 3;
+  constexpr int NS = (NP + 1) / 2;
+  const int nitems = *nitems_p;
+  int queue = blockIdx.x % kItemQueues, queues_left = kItemQueues;
+  for (;;) {
+    const int item = next_item(counter, nitems, queue, queues_left, &s_item);
+    if (item < 0) break;
+    const int4 it = items[item];
+    const TileJob& J = jobs[it.w];
+    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
+    const int nx = J.nx, ny = J.ny;
+    const int tile = it.x % J.ntiles;
+    const int kz = it.x / J.ntiles;
+    const int ichan = kz / J.nzg;
+    const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
+    float* __restrict__ grid = J.grid + 2 * J.wstride * (size_t)(kz % J.nzg);
+
+    // acc[k][e][m]: lane 4b + o, register i: output o (polarisation 2m + o / 2, re / im o & 1) of
+    // the pixel (row 4k + lv, column 4 (4 ob + i) + e)
+    f32x4 acc[4][4][NS];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int m = 0; m < NS; ++m) acc[k][e][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int nall = it.z - it.y;
+    for (int cb = 0; cb < nall; cb += kChunk) {
+      const int n = min(kChunk, nall - cb);
+      if (cb > 0) __syncthreads();
+      // staging: descriptors and the samples' 16-bit coefficients (output o: dwords 2o, 2o + 1)
+      for (int t = tid; t < n; t += blockDim.x) {
+        const int4 d = J.descs[it.y + cb + t];
+        s_desc[t] = d;
+        const uint4* src = reinterpret_cast<const uint4*>(J.smp.coef16 + (size_t)d.x * kCoef);
+        const uint4 c0 = src[0], c1 = src[1];
+        s_cf16[4 * t + 0] = make_uint2(c0.x, c0.y);
+        s_cf16[4 * t + 1] = make_uint2(c0.z, c0.w);
+        s_cf16[4 * t + 2] = make_uint2(c1.x, c1.y);
+        s_cf16[4 * t + 3] = make_uint2(c1.z, c1.w);
+      }
+      __syncthreads();
+      const int mw = build_wave_list(s_desc, n, w, lane, s_list[w]);
+      auto setup = [&](int i, Geom16& gp, Geom16& gq, uint2& bp, uint2& bq) {
+        const int ip = s_list[w][i];
+        const bool hq = i + 1 < mw;
+        const int iq = hq ? s_list[w][i + 1] : ip;
+        gp = geom16(s_desc[ip], w, lu, lv, true);
+        gq = geom16(s_desc[iq], w, lu, lv, hq);
+        bp = s_cf16[4 * ip + o];
+        bq = hq ? s_cf16[4 * iq + o] : make_uint2(0u, 0u);
+      };
+      auto products = [&](const uint32_t (&ta)[4][4], const uint32_t (&tb)[4][4], uint32_t um, uint2 bp, uint2 bq) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (um & (1u << k)) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              acc[k][e][0] = mma16<CFP>(ta[k][e], tb[k][e], bp.x, bq.x, acc[k][e][0]);
+              if constexpr (NP > 2) acc[k][e][1] = mma16<CFP>(ta[k][e], tb[k][e], bp.y, bq.y, acc[k][e][1]);
+            }
+          }
+      };
+      // two register sets: the taps of the next two pairs are loaded before these two's products
+      if (mw > 0) {
+        uint32_t a0[4][4], b0[4][4], a1[4][4], b1[4][4];
+        uint32_t um0 = 0, um1 = 0;
+        uint2 bp0, bq0, bp1, bq1;
+        Geom16 gp, gq;
+        setup(0, gp, gq, bp0, bq0);
+        um0 = gp.mask | gq.mask;
+        load16(a0, cfr, gp, um0);
+        load16(b0, cfr, gq, um0);
+        for (int i = 0; i < mw; i += 4) {
+          __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+          if (i + 2 < mw) {
+            setup(i + 2, gp, gq, bp1, bq1);
+            um1 = gp.mask | gq.mask;
+            load16(a1, cfr, gp, um1);
+            load16(b1, cfr, gq, um1);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          products(a0, b0, um0, bp0, bq0);
+          __builtin_amdgcn_sched_barrier(0);
+          if (i + 2 >= mw) break;
+          __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+          if (i + 4 < mw) {
+            setup(i + 4, gp, gq, bp0, bq0);
+            um0 = gp.mask | gq.mask;
+            load16(a0, cfr, gp, um0);
+            load16(b0, cfr, gq, um0);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          products(a1, b1, um1, bp1, bq1);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    // flush as k_grid: per (m, k) the wave writes its 4 rows x 64 columns x 4 outputs to its LDS
+    // area and reads them back so that each float-atomic instruction covers one row's 16
+    // consecutive pixels of two polarisations
+    float* s_fl = s_flush + w * kFlushWave;
+    const size_t plane = (size_t)nx * ny;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int idx = ((lv * kTile) + 16 * ob + 4 * i + e) * 4 + o;
+            s_fl[idx + (idx >> 5)] = acc[k][e][m][i];
+          }
+        __builtin_amdgcn_wave_barrier();
+        const int q = 2 * m + (o >> 1);
+        if (q < NP) {
+          float* gq = grid + 2 * (size_t)(ichan * NP + q) * plane + (o & 1);
+#pragma unroll
+          for (int t = 0; t < 16; ++t) {
+            const int idx = t * 64 + lane;
+            const float val = s_fl[idx + (idx >> 5)];
+            const int v = v0 + 16 * w + 4 * k + (t >> 2);
+            const int u = u0 + 16 * (t & 3) + (lane >> 2);
+            if (val != 0.f && u < nx && v < ny) atomicAdd(gq + 2 * ((size_t)v * nx + u), val);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+}
+
 __global__ void k_degrid_finish(Samples smp, const float* __restrict__ acc, int nrow, int nchan, int npol,
                                 float2* __restrict__ vis_out, int ld) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1675,6 +2094,7 @@ TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, f
   j.nzg = nzg;
   j.wstride = wstride;
   j.npol = npol;
+  j.gscale = cf.gscale;
   return j;
 }
 
@@ -1802,9 +2222,37 @@ static void launch_degrid_mm(hipStream_t s, dim3 g, int prec, const int4* items,
   }
 }
 
-void launch_tile_jobs(const Ctx& c, const TileVariant& v0, bool forward, int npol, int prec, const TileJob* jobs,
+template <int NP>
+static void launch_mma16(hipStream_t s, dim3 g, bool forward, int prec, const int4* items, const int* nitems,
+                         int* counter, const TileJob* jobs) {
+  if (prec == kPrecF16) {
+    if (forward) hipLaunchKernelGGL((k_degrid16<NP, kPrecF16>), g, dim3(256), 0, s, items, nitems, counter, jobs);
+    else hipLaunchKernelGGL((k_grid16<NP, kPrecF16>), g, dim3(256), 0, s, items, nitems, counter, jobs);
+  } else if (prec == kPrecBF16) {
+    if (forward) hipLaunchKernelGGL((k_degrid16<NP, kPrecBF16>), g, dim3(256), 0, s, items, nitems, counter, jobs);
+    else hipLaunchKernelGGL((k_grid16<NP, kPrecBF16>), g, dim3(256), 0, s, items, nitems, counter, jobs);
+  } else {
+    throw Error(1, "the 16-bit matrix-core kernels need a 16-bit table");
+  }
+}
+
+void launch_tile_jobs(const Ctx& c, const TileVariant& v0, bool forward, int npol, int prec_arg, const TileJob* jobs,
                       const int4* items, const int* nitems, int* counter) {
   launch_fill_u32(c, counter, kQueueInts, 0u);
+  const int prec = prec_arg & ~kPrecMma16;
+  if ((prec_arg & kPrecMma16) && prec != kPrecF32) {
+    // 16-bit table and operands: the 16-bit matrix-core kernels (two workgroups per CU)
+    const dim3 g(c.n_cu * 2);
+    switch (npol) {
+      case 1: launch_mma16<1>(c.stream, g, forward, prec, items, nitems, counter, jobs); break;
+      case 2: launch_mma16<2>(c.stream, g, forward, prec, items, nitems, counter, jobs); break;
+      case 3: launch_mma16<3>(c.stream, g, forward, prec, items, nitems, counter, jobs); break;
+      case 4: launch_mma16<4>(c.stream, g, forward, prec, items, nitems, counter, jobs); break;
+      default: throw Error(1, "unsupported number of polarisations (1..4)");
+    }
+    YG_HIP(hipGetLastError());
+    return;
+  }
   // one polarisation: half of the matrix form's outputs would be zeros; the VALU form is used
   if (forward && v0.matrix && npol >= 2) {
     const dim3 g(c.n_cu * 2);  // two workgroups (8 waves) per CU: <= 256 VGPRs per wave
@@ -1830,7 +2278,7 @@ void launch_grid(const Ctx& c, Binning& b, const Samples& smp, const CFView& cf,
                              wstride);
   b.job.resize(sizeof(TileJob));
   YG_HIP(hipMemcpyAsync(b.job.get(), &b.job_host, sizeof(TileJob), hipMemcpyHostToDevice, c.stream));
-  launch_tile_jobs(c, c.grid_variant, false, npol, cf.prec, b.job.as<TileJob>(), b.items.as<int4>(),
+  launch_tile_jobs(c, c.grid_variant, false, npol, tile_prec(cf), b.job.as<TileJob>(), b.items.as<int4>(),
                    b.nitems.as<int>(), b.counter.as<int>());
 }
 
@@ -1840,7 +2288,7 @@ void launch_degrid(const Ctx& c, Binning& b, const CFView& cf, const float2* gri
                              acc, nx, ny, npol, nzg, wstride);
   b.job.resize(sizeof(TileJob));
   YG_HIP(hipMemcpyAsync(b.job.get(), &b.job_host, sizeof(TileJob), hipMemcpyHostToDevice, c.stream));
-  launch_tile_jobs(c, c.degrid_variant, true, npol, cf.prec, b.job.as<TileJob>(), b.items.as<int4>(),
+  launch_tile_jobs(c, c.degrid_variant, true, npol, tile_prec(cf), b.job.as<TileJob>(), b.items.as<int4>(),
                    b.nitems.as<int>(), b.counter.as<int>());
 }
 
@@ -1851,6 +2299,29 @@ void launch_degrid_finish(const Ctx& c, const Samples& recs, const float* acc, i
   YG_CHECK(ld >= nrow, 1, "vis_out_rows must be 0 or at least nrow");
   hipLaunchKernelGGL(k_degrid_finish, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, acc, nrow, nchan, npol,
                      vis_out, ld);
+  YG_HIP(hipGetLastError());
+}
+
+namespace {
+__global__ void k_absmax_bits(const float* __restrict__ a, size_t n, uint32_t* out) {
+  float m = 0.f;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(a[i]));
+  for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d));
+  // non-negative floats order as their bit patterns
+  if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(out, __float_as_uint(m));
+}
+__global__ void k_pow2_scale(const uint32_t* mx, float* scale) {
+  const float m = __uint_as_float(*mx);
+  *scale = m > 0.f ? ldexpf(1.f, 13 - ilogbf(m)) : 1.f;
+}
+}  // namespace
+
+void launch_grid_scale16(const Ctx& c, const float* grid, size_t n, uint32_t* tmp, float* scale) {
+  launch_fill_u32(c, tmp, 1, 0u);
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, (size_t)c.n_cu * 8);
+  if (n > 0) hipLaunchKernelGGL(k_absmax_bits, dim3(std::max(blocks, 1)), dim3(256), 0, c.stream, grid, n, tmp);
+  hipLaunchKernelGGL(k_pow2_scale, dim3(1), dim3(1), 0, c.stream, tmp, scale);
   YG_HIP(hipGetLastError());
 }
 

@@ -100,7 +100,7 @@ void Gridder::degridNoAlloc(const ygrid_chunk& a, void* ws, size_t bytes) {
                                ps.pp.nzg, nGridPlanes() > 1 ? gridPlaneElems() : 0);
   YG_HIP(hipMemcpyAsync(job, &noalloc_job_, sizeof(TileJob), hipMemcpyHostToDevice, c.stream));
   if (profiling_) YG_HIP(hipEventRecord(ev_[2], c.stream));
-  launch_tile_jobs(c, c.degrid_variant, true, shape_[2], ps.cfv.prec, job, items, base + 1, counter);
+  launch_tile_jobs(c, c.degrid_variant, true, shape_[2], tile_prec(ps.cfv), job, items, base + 1, counter);
   if (profiling_) YG_HIP(hipEventRecord(ev_[3], c.stream));
   launch_degrid_finish(c, ps.out.smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out, d.vis_out_ld);
   unsigned long long cnt[5] = {0, 0, 0, 0, 0};

@@ -209,6 +209,7 @@ struct TileJob {
   float* acc;
   int nx, ny, ntu, ntiles, nzg, npol;
   size_t wstride;  // float2 elements between the grids of consecutive w planes (w stacking), else 0
+  const float* gscale;  // see CFView::gscale (null: 1)
 };
 
 struct Binning {
@@ -319,7 +320,17 @@ struct CFView {
   size_t bytes;
   int max_support;
   int prec;
+  // 16-bit table whose passes' other operand (the gridding coefficients; the model grid when
+  // degridding) is stored at the same precision: the tile kernels multiply on the 16-bit
+  // matrix cores
+  int mma16 = 0;
+  // degridding with FP16 operands: a power-of-two scale of the model grid (device float) that
+  // brings its largest magnitude to [2^13, 2^14)
+  const float* gscale = nullptr;
 };
+// the precision argument of launch_tile_jobs: the table's, with kPrecMma16 for mma16 views
+constexpr int kPrecMma16 = 0x10;
+inline int tile_prec(const CFView& v) { return v.prec | (v.mma16 ? kPrecMma16 : 0); }
 // Bins valid records by (key, tile); returns nothing, work lands in b.  The samples are the
 // nrow x nchan (row, channel) samples of one chunk, s = row + nrow*chan.  The pair descriptors
 // address the CF table cf describes.
@@ -333,6 +344,9 @@ void launch_grid(const Ctx&, Binning& b, const Samples& smp, const CFView& cf, f
 void launch_degrid(const Ctx&, Binning& b, const CFView& cf, const float2* grid, int nx, int ny, int npol,
                    int nzg, float* acc, size_t wstride = 0);
 // The job of one grid for the tile kernels (descs: its binning's pair descriptors).
+// *scale = 2^(13 - e) with 2^e <= max |grid[i]| < 2^(e+1) over n floats (1 when all are zero);
+// tmp: one device uint
+void launch_grid_scale16(const Ctx&, const float* grid, size_t n, uint32_t* tmp, float* scale);
 TileJob make_tile_job(const int4* descs, const Samples& smp, const CFView& cf, float* grid, float* acc, int nx, int ny,
                       int npol, int nzg, size_t wstride = 0);
 // One persistent tile-kernel launch over the work items of several jobs (all with npol
