@@ -56,6 +56,7 @@ HBM_PEAK_GBS = 8000.0
 
 PROFILE = "profiles/k_grid_k_degrid_wproject_array.md"
 PROFILE_SPH = "profiles/k_grid_small_k_degrid_small_sphfunc_disc.md"
+PROFILE_F16 = "profiles/k_grid16_k_degrid16_wproject_array_fp16.md"
 
 
 def profiled_traffic(kernel, profile=PROFILE):
@@ -374,8 +375,9 @@ def main():
         dname = f"k_degrid_mm<{wl.npol}, {s1d.cf_precision}>"
     else:
         dname = f"k_degrid<{wl.npol}, {s1d.degrid_waves}, {tf(s1d.degrid_pipeline)}, {s1d.cf_precision}>"
-    peak = FP32_PEAK_TFLOPS
+    peak, prof = FP32_PEAK_TFLOPS, PROFILE
     if args.cf_precision != "fp32":
+        prof = PROFILE_F16
         # 16-bit taps and coefficients (and model grid): the 16-bit matrix-core kernels
         gname = f"k_grid16<{wl.npol}, {s1g.cf_precision}>"
         dname = f"k_degrid16<{wl.npol}, {s1d.cf_precision}>"
@@ -408,8 +410,8 @@ def main():
                                "prep": (s1g.ms_prep + s1d.ms_prep) / args.steps,
                                "binning": (s1g.ms_bin + s1d.ms_bin) / args.steps},
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": profiled_traffic(kname),
-                     "traffic_source": PROFILE},
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": profiled_traffic(kname, prof),
+                     "traffic_source": prof},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

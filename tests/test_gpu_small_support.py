@@ -93,7 +93,9 @@ def test_small_support_matches_tile_kernels_and_oracle(gpu_ctx, case):
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 def test_small_support_16bit_storage_matches_tile_kernels(gpu_ctx, prec):
-    """Both paths read the same rounded taps and coefficients and widen them to FP32."""
+    """Both paths read the same rounded taps and coefficients and, degridding, round the model grid to
+    the same 16-bit format (the tile kernels multiply on the 16-bit matrix cores, the small ones
+    widen to FP32: the products are equal, their sums differ in order only)."""
     announce(f"small-support kernels: {prec} CF taps and coefficients vs the tile kernels")
     N, cell, npol = 256, 10 * ARC, 4
     d = make_data(2500, 2, npol, N, cell, seed=71)

@@ -585,10 +585,11 @@ void Gridder::generic(const ygrid_chunk& a, bool forward) {
     ++n_small_;
     if (forward) {
       if (profiling_) YG_HIP(hipEventRecord(ev_[2], ctx_->stream));
-      launch_degrid_small(*ctx_,
-                          SmallDegridArgs{smp, cfv.ptr, grid_.as<float>(), ps.acc, d.nrow, d.nchan, shape_[0],
-                                          shape_[1], ps.pp.nzg, wstride},
-                          shape_[2], cfv.prec, cfv.max_support);
+      SmallDegridArgs da{smp, cfv.ptr, grid_.as<float>(), ps.acc, d.nrow, d.nchan, shape_[0],
+                         shape_[1], ps.pp.nzg, wstride};
+      da.round16 = cfv.mma16;
+      da.gscale = cfv.gscale;
+      launch_degrid_small(*ctx_, da, shape_[2], cfv.prec, cfv.max_support);
       if (profiling_) YG_HIP(hipEventRecord(ev_[3], ctx_->stream));
       launch_degrid_finish(*ctx_, smp, ps.acc, d.nrow, d.nchan, d.npol, d.vis_out, d.vis_out_ld);
     } else {

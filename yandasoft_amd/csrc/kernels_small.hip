@@ -231,6 +231,11 @@ __global__ __launch_bounds__(256) void k_degrid_small(SmallDegridArgs a) {
 #pragma unroll
   for (int q = 0; q < NP; ++q) sr[q] = si[q] = 0.f;
   int s = -1;
+  float gs = 1.f, igs = 1.f;
+  if (CFP != kPrecF32 && a.round16 && a.gscale) {
+    gs = *a.gscale;
+    igs = 1.f / gs;  // a power of two: exact
+  }
   if (gi < n) {
     const int row = (int)(gi / a.nchan), chan = (int)(gi % a.nchan);
     s = row + a.nrow * chan;
@@ -248,7 +253,9 @@ __global__ __launch_bounds__(256) void k_degrid_small(SmallDegridArgs a) {
           const float2 C = load_tap<CFP>(a.cf, e0 + (long long)r * P);
 #pragma unroll
           for (int q = 0; q < NP; ++q) {
-            const float2 G = gp[(size_t)q * plane + (size_t)r * a.nx];
+            float2 G = gp[(size_t)q * plane + (size_t)r * a.nx];
+            if (CFP != kPrecF32 && a.round16)
+              G = make_float2(widen16(round16(G.x * gs, CFP), CFP) * igs, widen16(round16(G.y * gs, CFP), CFP) * igs);
             // C * conj(G) (GridKernel.cc:121-122)
             sr[q] += C.x * G.x + C.y * G.y;
             si[q] += C.y * G.x - C.x * G.y;

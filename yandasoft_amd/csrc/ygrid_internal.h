@@ -303,6 +303,11 @@ struct SmallDegridArgs {
   float* acc;  // per sample 8 floats: (re, im) of C * conj(G) summed, per polarisation
   int nrow, nchan, nx, ny, nzg;
   size_t wstride;
+  // 16-bit table with a 16-bit visibility side (CFView::mma16): the model grid's values are
+  // rounded to the table's format, after the scale gscale (FP16; null: 1), as the tile kernels
+  // round their operand
+  int round16 = 0;
+  const float* gscale = nullptr;
 };
 // Persistent small-support gridding over the items of a one-item-per-key static binning
 // (nitems: device item count, nitems_cap its bound; counter: kItemQueues zeros).
