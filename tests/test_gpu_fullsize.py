@@ -164,3 +164,56 @@ def test_wproject_array_all_rows_adjoint_and_linearity(gpu_ctx, wproj):
     g2.initialise_grid(img)
     g2.grid(_pchunk(wl.uvw, freq, np.ascontiguousarray(2 * V), noise))
     assert rel(g2.grid_array(), 2 * G) < TOL
+
+
+def test_cf_table_over_2gib_matches_oracle(gpu_ctx):
+    """A W-projection table of 2.78 GiB (129 w planes x 64 oversampling planes, support 96: the
+    reference's table has no size limit, TableVisGridder.h:356).  The tile kernels address it
+    through one 32-bit buffer resource per w-plane group (pair descriptors relative to the group's
+    first plane); samples on planes at the start, middle and end of the table (offsets beyond
+    2 GiB) grid and degrid as the oracle does."""
+    announce("CF table of 2.78 GiB (129 w planes, oversample 8, support 96): grid and degrid vs oracle")
+    import yandasoft_amd as ya
+    from tests.test_gpu_parity import random_model
+    N, cell, f = 512, 30 * math.pi / 180 / 3600, 1.0e9
+    params = dict(wmax=32000.0, nwplanes=129, oversample=8, maxsupport=2048, cutoff=1e-3, limitsupport=96)
+    prm = O.WProjectParams(**params)
+    rng = np.random.default_rng(31)
+    planes = [0, 1, 64, 127, 128]
+    nrow = 250
+    lam = O.C_LIGHT / f
+    pl = rng.choice(planes, nrow)
+    w = ((pl - 64) * 500.0 + rng.uniform(-200.0, 200.0, nrow)) * lam      # metres
+    uvmax = 0.8 * (N / 2 - 97) / (N * cell) * lam
+    r = uvmax * np.sqrt(rng.random(nrow))
+    th = 2 * np.pi * rng.random(nrow)
+    uvw = np.ascontiguousarray(np.stack([r * np.cos(th), r * np.sin(th), w], 1))
+    vis = (rng.normal(size=(1, 1, nrow)) + 1j * rng.normal(size=(1, 1, nrow))).astype(np.complex64)
+    noise = np.ones_like(vis)
+    freq = np.array([f])
+    img = ya.ImageDesc(N, N, 1, 1, cell, cell)
+    g = ya.Gridder(gpu_ctx, "GPUWProject", params)
+    g.initialise_grid(img)
+    g.grid(ya.Chunk(uvw=uvw, freq=freq, vis=vis, noise=noise))
+    st = g.stats()
+    assert st.support == 96
+    table_bytes = 129 * 64 * (2 * 96 + 9) * (2 * 96 + 33) * 8
+    assert table_bytes >= 2 ** 31
+    print(f"[big cf] table {table_bytes / 2 ** 30:.2f} GiB, samples gridded {st.samples_gridded:.0f}", flush=True)
+    o = O.OracleGridder("WProject", "f32", wproject=prm)
+    o.only_planes = set(planes)
+    o.initialise_grid((N, N, 1, 1), (cell, cell))
+    ch = O.Chunk(uvw, freq, np.transpose(vis, (2, 1, 0)), noise=np.transpose(noise, (2, 1, 0)))
+    o.grid(ch)
+    assert rel(g.grid_array().reshape(1, 1, N, N), o.grid_) < 1e-5
+    for k in (0, 64 * 64 + 9, 64 * 128 + 63):  # first plane, a middle one, the table's last
+        assert rel(g.cf_plane(k), o.cf.planes[k]) < 1e-5, k
+    model = random_model(N, 1)
+    dg = ya.Gridder(gpu_ctx, "GPUWProject", params)
+    dg.initialise_degrid(img, model.astype(np.float32))
+    out = np.zeros_like(vis)
+    dg.degrid(ya.Chunk(uvw=uvw, freq=freq, vis=vis, vis_out=out, noise=noise))
+    od = O.OracleGridder("WProject", "f32", wproject=prm)
+    od.only_planes = set(planes)
+    od.initialise_degrid((N, N, 1, 1), (cell, cell), model)
+    assert rel(np.transpose(out, (2, 1, 0)), od.degrid(ch)) < 1e-5

@@ -540,6 +540,33 @@ bool Gridder::setup_pass(const ygrid_chunk& a, bool forward, PassSetup& ps, Pass
     pp.zdiv = nz;
     pp.nzg = 1;
   }
+  // the tile kernels address a table through 32-bit buffer offsets: one of 2 GiB or more is
+  // addressed per key group (the small-support kernels use 64-bit element offsets)
+  const bool own_table = ps.cfv.ptr == cf_->table.get() || ps.cfv.ptr == cf_->table16.get();
+  if (!ps.small && own_table && nGridPlanes() == 1 &&
+      ps.cfv.bytes + cf_resource_pad(ps.cfv.max_support, ps.cfv.prec) > kCFResourceMax) {
+    const size_t os2 = (size_t)cf_->oversample * cf_->oversample;
+    const size_t esz = cf_elem_bytes(ps.cfv.prec);
+    if (cf_->zg_zdiv != pp.zdiv || cf_->zg_n != pp.nzg) {
+      YG_CHECK(!arena, 1, "a CF table of 2 GiB or more needs a pass that may allocate before the allocation-free one");
+      std::vector<long long> base(pp.nzg);
+      for (int g = 0; g < pp.nzg; ++g) {
+        const size_t z0 = (size_t)g * pp.zdiv, z1 = std::min((size_t)nz, (size_t)(g + 1) * pp.zdiv);
+        const size_t b = (size_t)cf_->off[z0 * os2];
+        const size_t e = z1 < (size_t)nz ? (size_t)cf_->off[z1 * os2] : cf_->n_elems;
+        YG_CHECK((e - b) * esz + cf_resource_pad(ps.cfv.max_support, ps.cfv.prec) <= kCFResourceMax, 1,
+                 "the CF planes of one w-plane group span 2 GiB or more");
+        base[g] = (long long)b;
+      }
+      cf_->d_zg.release();
+      cf_->d_zg.resize(base.size() * sizeof(long long));
+      YG_HIP(hipMemcpy(cf_->d_zg.get(), base.data(), base.size() * sizeof(long long), hipMemcpyHostToDevice));
+      cf_->zg_zdiv = pp.zdiv;
+      cf_->zg_n = pp.nzg;
+    }
+    ps.cfv.zg_base = cf_->d_zg.as<long long>();
+    ps.cfv.nzg = pp.nzg;
+  }
   return true;
 }
 

@@ -89,6 +89,10 @@ struct CFTable {
   // the table rounded to 16-bit precision table16_prec (built on first use at that precision)
   DevBuf table16;
   int table16_prec = 0;
+  // tables of 2 GiB or more: per key group of zg_zdiv planes, the element offset of its first
+  // plane (device; built on first use for that grouping)
+  DevBuf d_zg;
+  int zg_zdiv = 0, zg_n = 0;
   void upload(const Ctx& c);
 };
 

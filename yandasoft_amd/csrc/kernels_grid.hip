@@ -492,11 +492,14 @@ __device__ __forceinline__ int4 pair_descriptor(const SampleHdr& r, int s, int u
   return make_int4(s, (int)soff, (int)masks, (int)pw);
 }
 
+// zg_base (tables of 2 GiB or more): descriptor offsets relative to the sample's key group's
+// first plane (the group: key % nzg)
 __global__ void k_bin_fill(Samples smp, int nrow, int nchan, int ntu, int ntiles, uint32_t pad, uint32_t esz,
-                           int* cursor, int kstride, int4* descs) {
+                           int* cursor, int kstride, int4* descs, const long long* __restrict__ zg_base, int nzg) {
   const long long tglob = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const BinLane b = bin_lane(smp, tglob, nrow, nchan, ntiles);
+  BinLane b = bin_lane(smp, tglob, nrow, nchan, ntiles);
+  if (zg_base && b.s >= 0) b.r.cfoff = (int32_t)((long long)b.r.cfoff - zg_base[smp.key[b.s] % nzg]);
   const int nv = wave_max_int(b.t.tv1 - b.t.tv0 + 1), nu = wave_max_int(b.t.tu1 - b.t.tu0 + 1);
   for (int dv = 0; dv < nv; ++dv)
     for (int du = 0; du < nu; ++du) {
@@ -748,6 +751,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
                                            (int)(t.bytes + t.pad), kBufferWord3);
 }
 
+// The CF resource of a work item of key group kz: the whole table, or (tables of 2 GiB or more)
+// a resource based at the group's first plane that the descriptors' offsets are relative to.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_item_rsrc(const TileJob& J, int kz) {
+  const CFRef& t = J.cf;
+  if (!t.zg_base) return make_cf_rsrc(t);
+  const unsigned long long b = (unsigned long long)t.zg_base[kz % J.nzg] * t.esz;
+  const unsigned long long rec = min(t.total - b + t.pad, kCFResourceMax);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(reinterpret_cast<const char*>(t.ptr)) + b - t.pad,
+                                           (short)0, (int)rec, kBufferWord3);
+}
+
 // Work queues of the persistent tile kernels: the key-ordered items (w-plane-major) split into
 // kItemQueues contiguous ranges, one per XCD, so that the workgroups of one XCD work through
 // neighbouring keys and the CF planes they read stay in that XCD's L2.  A workgroup's XCD is
@@ -902,10 +916,10 @@ __global__ __launch_bounds__(256, WPE) void k_grid(const int4* __restrict__ item
     if (item < 0) break;
     const int4 it = items[item];
     const TileJob& J = jobs[it.w];
-    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
     const int nx = J.nx, ny = J.ny;
     const int tile = it.x % J.ntiles;
     const int kz = it.x / J.ntiles;
+    const __amdgpu_buffer_rsrc_t cfr = make_item_rsrc(J, kz);
     const int ichan = kz / J.nzg;
     const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
     float* __restrict__ grid = J.grid + 2 * J.wstride * (size_t)(kz % J.nzg);
@@ -1057,10 +1071,10 @@ __global__ __launch_bounds__(256, WPE) void k_degrid(const int4* __restrict__ it
     if (item < 0) break;
     const int4 it = items[item];
     const TileJob& J = jobs[it.w];
-    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
     const int nx = J.nx, ny = J.ny;
     const int tile = it.x % J.ntiles;
     const int kz = it.x / J.ntiles;
+    const __amdgpu_buffer_rsrc_t cfr = make_item_rsrc(J, kz);
     const int ichan = kz / J.nzg;
     const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
     const size_t plane = (size_t)nx * ny;
@@ -1243,10 +1257,10 @@ __global__ __launch_bounds__(256, 2) void k_degrid_mm(const int4* __restrict__ i
     if (item < 0) break;
     const int4 it = items[item];
     const TileJob& J = jobs[it.w];
-    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
     const int nx = J.nx, ny = J.ny;
     const int tile = it.x % J.ntiles;
     const int kz = it.x / J.ntiles;
+    const __amdgpu_buffer_rsrc_t cfr = make_item_rsrc(J, kz);
     const int ichan = kz / J.nzg;
     const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
     const size_t plane = (size_t)nx * ny;
@@ -1448,10 +1462,10 @@ __global__ __launch_bounds__(256, 2) void k_degrid16(const int4* __restrict__ it
     if (item < 0) break;
     const int4 it = items[item];
     const TileJob& J = jobs[it.w];
-    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
     const int nx = J.nx, ny = J.ny;
     const int tile = it.x % J.ntiles;
     const int kz = it.x / J.ntiles;
+    const __amdgpu_buffer_rsrc_t cfr = make_item_rsrc(J, kz);
     const int ichan = kz / J.nzg;
     const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
     const size_t plane = (size_t)nx * ny;
@@ -1647,10 +1661,10 @@ __global__ __launch_bounds__(256, 2) void k_grid16(const int4* __restrict__ item
     if (item < 0) break;
     const int4 it = items[item];
     const TileJob& J = jobs[it.w];
-    const __amdgpu_buffer_rsrc_t cfr = make_cf_rsrc(J.cf);
     const int nx = J.nx, ny = J.ny;
     const int tile = it.x % J.ntiles;
     const int kz = it.x / J.ntiles;
+    const __amdgpu_buffer_rsrc_t cfr = make_item_rsrc(J, kz);
     const int ichan = kz / J.nzg;
     const int u0 = (tile % J.ntu) * kTile, v0 = (tile / J.ntu) * kTile;
     float* __restrict__ grid = J.grid + 2 * J.wstride * (size_t)(kz % J.nzg);
@@ -1865,7 +1879,7 @@ void launch_binning(const Ctx& c, const Samples& recs, int nrow, int nchan, int 
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, recs, nrow, nchan, ntu, (int)ntiles,
                        cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), cur, ks,
-                       b.pairs.as<int4>());
+                       b.pairs.as<int4>(), cf.zg_base, cf.nzg);
   // work items: reuse cursor as the per-key item count and its scan
   int* icount = b.cursor.as<int>();
   const int chunk = c.item_pairs > 0 ? c.item_pairs : kDefaultItemPairs;
@@ -1959,7 +1973,8 @@ void enqueue_static_binning(const Ctx& c, const StaticBinsView& sb, const Sample
   hipLaunchKernelGGL(k_copy_strided_i32, dim3(cgrid), dim3(256), 0, c.stream, offsets, 1, cur, ks, (size_t)nk + 1);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3((n + 255) / 256), dim3(256), 0, c.stream, smp, sb.nrow, sb.nchan, ntu, ntiles,
-                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), cur, ks, sb.pairs);
+                       cf_pad_bytes(cf.max_support, cf.prec), (uint32_t)cf_elem_bytes(cf.prec), cur, ks, sb.pairs,
+                       cf.zg_base, cf.nzg);
   hipLaunchKernelGGL(k_item_count, dim3((nk + 1 + 255) / 256), dim3(256), 0, c.stream, counts, nk, sb.chunk,
                      sb.icount);
   tb = sb.temp_bytes;
@@ -1988,7 +2003,7 @@ bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny
   BinCache& bc = c.bins;
   const int n = nrow * nchan;
   if (bc.valid && bc.nrow == nrow && bc.nchan == nchan && bc.nx == nx && bc.ny == ny && bc.nkz == nkz &&
-      bc.smax == cf.max_support && bc.cf_prec == cf.prec) {
+      bc.smax == cf.max_support && bc.cf_prec == cf.prec && bc.zg_base == cf.zg_base) {
     bc.flag.resize(sizeof(int));
     YG_HIP(hipMemsetAsync(bc.flag.get(), 0, sizeof(int), c.stream));
     if (n > 0)
@@ -2014,6 +2029,7 @@ bool bin_samples(Ctx& c, const Samples& smp, int nrow, int nchan, int nx, int ny
   bc.nkz = nkz;
   bc.smax = cf.max_support;
   bc.cf_prec = cf.prec;
+  bc.zg_base = cf.zg_base;
   bc.valid = true;
   return false;
 }
@@ -2037,11 +2053,19 @@ struct TileKernels {
 // that starts cf_pad_bytes before the table.
 static CFRef cf_ref(const CFView& cf) {
   const size_t pad = cf_pad_bytes(cf.max_support, cf.prec);
+  YG_CHECK(cf.bytes > 0, 1, "empty CF table");
+  if (cf.zg_base) {
+    // per-group resources (make_item_rsrc); the caller checked each group's span
+    return CFRef{cf.ptr, 0u, (uint32_t)pad, cf.zg_base, (unsigned long long)cf.bytes, (uint32_t)cf_elem_bytes(cf.prec)};
+  }
   // below 2 GiB, so that an out-of-range lane offset (2^31) plus any in-range scalar offset
   // neither wraps nor lands inside the buffer
-  YG_CHECK(cf.bytes > 0 && cf.bytes + pad < kOutOfRange, 1, "CF table must be smaller than 2 GiB");
-  return CFRef{cf.ptr, (uint32_t)cf.bytes, (uint32_t)pad};
+  YG_CHECK(cf.bytes + pad <= kCFResourceMax, 1, "CF table of 2 GiB or more without key-group bases");
+  return CFRef{cf.ptr, (uint32_t)cf.bytes, (uint32_t)pad, nullptr, (unsigned long long)cf.bytes,
+               (uint32_t)cf_elem_bytes(cf.prec)};
 }
+
+size_t cf_resource_pad(int max_support, int prec) { return cf_pad_bytes(max_support, prec); }
 
 // FP32 tables: every variant; 16-bit tables: the two-wave pipelined form only
 template <int NP, int CFP, class F>

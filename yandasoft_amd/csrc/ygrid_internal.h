@@ -196,6 +196,12 @@ struct PrepInputs {
 struct CFRef {
   const void* ptr;
   uint32_t bytes, pad;
+  // tables of 2 GiB or more: the element offset of each key group's first plane (the pair
+  // descriptors' offsets are relative to it, and each work item addresses its group through a
+  // buffer resource based there); total bytes of the table.  Null: one resource for the table.
+  const long long* zg_base;
+  unsigned long long total;
+  uint32_t esz;
 };
 
 // One grid of a tile-kernel launch: its pair descriptors, samples, CF buffer, grid (gridding:
@@ -237,6 +243,7 @@ struct TileVariant {
 struct BinCache {
   DevBuf hdr, key, flag;
   int nrow = -1, nchan = -1, nx = 0, ny = 0, nkz = 0, smax = 0, cf_prec = 0;
+  const long long* zg_base = nullptr;
   bool valid = false;
   Binning bin;
 };
@@ -332,7 +339,16 @@ struct CFView {
   // degridding with FP16 operands: a power-of-two scale of the model grid (device float) that
   // brings its largest magnitude to [2^13, 2^14)
   const float* gscale = nullptr;
+  // tables of 2 GiB or more (see CFRef): per key group (nzg of them per image channel) the
+  // element offset of the group's first plane, on the device; null when the table fits one
+  // buffer resource
+  const long long* zg_base = nullptr;
+  int nzg = 0;
 };
+// bytes a CF table may span (with the resource's leading pad) for one buffer resource of the
+// tile kernels; the largest pad the tile kernels put before a table of this support and precision
+constexpr unsigned long long kCFResourceMax = 0x7F000000ull;
+size_t cf_resource_pad(int max_support, int prec);
 // the precision argument of launch_tile_jobs: the table's, with kPrecMma16 for mma16 views
 constexpr int kPrecMma16 = 0x10;
 inline int tile_prec(const CFView& v) { return v.prec | (v.mma16 ? kPrecMma16 : 0); }
