@@ -155,9 +155,9 @@ int ygrid_ctx_clear_cache(ygrid_ctx *ctx);
 int ygrid_ctx_set_small_support(ygrid_ctx *ctx, int max_support);
 
 /* ---- factory: VisGridderFactory::make / createGridder (VisGridderFactory.cc:112-286) ----
- * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUWStack", "GPUAWProject" (illumination "disk" or
- * "ATCA" with the illumination.* keys of AProjectGridderBase::makeIllumination,
- * AProjectGridderBase.cc:290-348).
+ * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUWStack", "GPUAWProject" or "GPUAProjectWStack"
+ * (AProjectWStackVisGridder); the last two with illumination "disk" or "ATCA" and the
+ * illumination.* keys of AProjectGridderBase::makeIllumination (AProjectGridderBase.cc:290-348).
  * parset: the gridder.<name>. subset as "key=value" lines (';' also separates); the global keys
  * of ygrid_gridder_configure may be given here too. */
 int ygrid_gridder_create(ygrid_ctx *ctx, const char *name, const char *parset, ygrid_gridder **out);

@@ -68,7 +68,7 @@ from __future__ import annotations
 import ctypes
 import math
 import os
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace as _dc_replace
 
 import numpy as np
 
@@ -766,6 +766,57 @@ def awproject_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_o
     return state
 
 
+def aprojectwstack_planes(prm: AWProjectParams, nx_grid, ny_grid, uvcell, freqs, lm_of_feed, field=0, psf=False,
+                          state=None):
+    """AProjectWStackVisGridder::initConvolutionFunction (AProjectWStackVisGridder.cc:222-370) for
+    the feeds in ``lm_of_feed`` (feed -> (l, m) or (l, m, pa)) of one field.  Per channel the
+    pattern on min(maxsupport, grid) cells of uvcell / os is transformed to the image plane
+    (fft2d(pattern, false)), its power |A|^2 in complex float is divided by its peak, transformed
+    back (fft2d(pattern, true)) and divided by nx ny; the CF of (chan, feed, field) is cut out at
+    k = t os + frac about the centre for t in [-S, S-1] (the plane's last row and column stay
+    zero) times os^2.  S is set once, from the first channel: 1 + 2 int(R) / os with R the dish
+    radius in pattern cells (DiskIllumination.cc:139, ATCAIllumination.cc:245), limited by
+    limitsupport.  No w term and no per-plane normalisation."""
+    os_ = prm.oversample
+    nx, ny = min(prm.maxsupport, nx_grid), min(prm.maxsupport, ny_grid)
+    nchan = len(freqs) if prm.freqdep else 1
+    if state is None:
+        state = {"planes": [None] * (os_ * os_ * prm.maxfeeds * prm.maxfields * nchan), "support": 0,
+                 "nchan": nchan, "offsets": [(0, 0)] * (prm.maxfeeds * prm.maxfields * nchan)}
+    for feed, geo in lm_of_feed.items():
+        l, m = geo[0], geo[1]
+        pa = 0.0 if prm.symmetric() or len(geo) < 3 else geo[2]
+        for chan in range(nchan):
+            pat = illumination_pattern(nx, ny, uvcell[0], uvcell[1], os_, freqs[chan], prm,
+                                       0.0 if psf else l, 0.0 if psf else m, pa)
+            A = fft_centred(pat.astype(np.complex128), False).astype(np.complex64)
+            p = (A * np.conj(A)).astype(np.complex64)
+            peak = np.max(np.abs(p))
+            if peak > 0:
+                p = (p * np.complex64(1.0 / peak)).astype(np.complex64)
+            buf = fft_centred(p.astype(np.complex128), True) / (nx * ny)
+            buf = buf.astype(np.complex64).astype(np.complex128)
+            if state["support"] == 0:
+                cell = abs(uvcell[0] / os_ * (C_LIGHT / freqs[chan]))
+                S = 1 + (2 * int(prm.diameter / (2.0 * cell))) // os_
+                if not (S * os_ < nx // 2):
+                    raise ValueError("Overflowing convolution function")
+                if prm.limitsupport > 0 and S > prm.limitsupport:
+                    S = prm.limitsupport
+                state["support"] = S
+            S = state["support"]
+            z = chan + nchan * (feed + prm.maxfeeds * field)
+            t = np.arange(-S, S)
+            for fracu in range(os_):
+                kx = t * os_ + fracu + nx // 2
+                for fracv in range(os_):
+                    ky = t * os_ + fracv + ny // 2
+                    c = np.zeros((2 * S + 1, 2 * S + 1), dtype=np.complex128)
+                    c[:2 * S, :2 * S] = float(os_ * os_) * buf[np.ix_(ky, kx)]
+                    state["planes"][fracu + os_ * (fracv + os_ * z)] = c
+    return state
+
+
 def fft_pad(a: np.ndarray, ny_out: int, nx_out: int) -> np.ndarray:
     """Fourier interpolation of real [.., y, x] planes onto a larger grid (PaddingUtils::fftPad as
     called at AWProjectVisGridder.cc:765).  The interpolation is pinned by a band-limited closed
@@ -805,17 +856,23 @@ def wstack_screen(nx: int, ny: int, uvcell, wterm: float) -> np.ndarray:
 
 
 class OracleWStack:
-    """WStackVisGridder composed from SphFunc oracle gridders: a sample goes to the grid of its w
+    """WStackVisGridder composed from per-plane oracle gridders: a sample goes to the grid of its w
     plane (initIndices / gIndex = getWPlane(w f / c), WStackVisGridder.cc:71-97,339-344; the CF is
     SphFunc's with its defaults, cIndex 0); finaliseGrid sums Re(screen x inverse transform) over
     the planes, then corrects and scales (:255-289); initialiseDegrid fills each plane with the
-    forward transform of conj(screen x corrected model) (:314-327)."""
+    forward transform of conj(screen x corrected model) (:314-327).
 
-    def __init__(self, wmax: float, nwplanes: int, precision: str = "f32", **sampling):
+    ``make`` builds the per-plane gridder (default SphFunc).  AProjectWStackVisGridder is this
+    composition with an 'AProjectWStack' OracleGridder per plane (its CFs indexed by channel and
+    feed, no correction, AProjectWStackVisGridder.cc:122-170,480-481); its weights are formed once
+    from the planes' summed sums of weights (:377-470)."""
+
+    def __init__(self, wmax: float, nwplanes: int, precision: str = "f32", make=None, **sampling):
         """sampling: WSampling's kind, exponent, nwplanes50, wmaxclip (the gridder's wsampling,
         wsampling.exponent, wsampling.nwplanes50, wmaxclip keys; WDependentGridderBase.cc:190-215)"""
         self.ws = WSampling(wmax, nwplanes, **sampling)
         self.precision = precision
+        self.make = make if make is not None else (lambda: OracleGridder("SphFunc", precision))
 
     def _planes(self, chunk: Chunk):
         w = chunk.uvw[:, 2][:, None] * chunk.freq[None, :] / C_LIGHT  # (nrow, nchan)
@@ -825,22 +882,33 @@ class OracleWStack:
         keep = (planes == iw)[:, :, None]
         base = np.zeros(chunk.vis.shape, bool) if chunk.flag is None else chunk.flag
         flag = base | ~keep  # samples of the other planes are left out as flagged ones are
-        return Chunk(chunk.uvw, chunk.freq, chunk.vis, noise=chunk.noise, flag=np.ascontiguousarray(flag),
-                     delay=chunk.delay)
+        return _dc_replace(chunk, flag=np.ascontiguousarray(flag))  # feeds, pointings, angles kept
 
-    def grid(self, shape, cell, chunk: Chunk, dopcf: bool = False):
+    def grid(self, shape, cell, chunk: Chunk, dopcf: bool = False, dopsf: bool = False, freq_axis=None):
         """dopcf: the preconditioner pass (SphFunc's 1 + 3i nearest-neighbour kernel per plane,
         SphFuncVisGridder.cc:99-123)"""
         planes = self._planes(chunk)
         self.grids, self.sumw = [], 0.0
         self.dopcf = dopcf
+        ran, rows = None, 0.0
         for iw in range(self.ws.nw):
-            o = OracleGridder("SphFunc", self.precision)
-            o.initialise_grid(shape, cell, dopcf=dopcf)
+            o = self.make()
+            o.initialise_grid(shape, cell, dopsf=dopsf, dopcf=dopcf, freq_axis=freq_axis)
             if np.any(planes == iw):
                 o.grid(self._plane_chunk(chunk, planes, iw))
+                ran = o
             self.grids.append(o.grid_.astype(np.complex128))
-            self.sumw = self.sumw + o.finalise_weights()
+            if o.kind == "AProjectWStack":
+                rows = rows + o.sumw
+            else:
+                self.sumw = self.sumw + o.finalise_weights()
+        if o.kind == "AProjectWStack":
+            ref = ran if ran is not None else o
+            if ran is not None:
+                ref.sumw = rows
+                self.sumw = ref.finalise_weights()
+            else:
+                self.sumw = np.zeros((o.nchan_img, o.npol, shape[1], shape[0]))
         self.o = o
         return self.grids
 
@@ -875,12 +943,12 @@ class OracleWStack:
         ox, oy = (o.nx - nx0) // 2, (o.ny - ny0) // 2
         return img[:, :, oy:oy + ny0, ox:ox + nx0]
 
-    def degrid(self, shape, cell, model, chunk: Chunk):
+    def degrid(self, shape, cell, model, chunk: Chunk, freq_axis=None):
         planes = self._planes(chunk)
         out = 0.0
         for iw in range(self.ws.nw):
-            o = OracleGridder("SphFunc", self.precision)
-            o.initialise_degrid(shape, cell, model)
+            o = self.make()
+            o.initialise_degrid(shape, cell, model, freq_axis=freq_axis)
             pad = np.zeros((o.nchan_img, o.npol, o.ny, o.nx))
             ox, oy = (o.nx - shape[0]) // 2, (o.ny - shape[1]) // 2
             pad[:, :, oy:oy + shape[1], ox:ox + shape[0]] = model
@@ -1010,7 +1078,8 @@ def _cube_f(a):
 
 
 class OracleGridder:
-    """TableVisGridder restated; kind in {'Box', 'SphFunc', 'WProject'}.
+    """TableVisGridder restated; kind in {'Box', 'SphFunc', 'WProject', 'AWProject',
+    'AProjectWStack'} (the last one grid of an OracleWStack: its CFs without the w term).
 
     Images/grids are numpy arrays indexed [chan, pol, y, x]."""
 
@@ -1108,7 +1177,13 @@ class OracleGridder:
             f = int(f)
             if f not in self.aw_done and f not in new:
                 new[f] = tuple(self.aw_lm.get(f, (0.0, 0.0))[:2]) + (float(np.float32(pa)),)
-        if new or self.aw_state is None:
+        if (new or self.aw_state is None) and self.kind == "AProjectWStack":
+            if pcf:
+                raise ValueError("AProjectWStack has no preconditioner (PCF) pass")
+            self.aw_state = aprojectwstack_planes(self.wp, self.nx, self.ny, self.uvcell, chunk.freq, new, 0,
+                                                  psf, self.aw_state)
+            self.aw_done.update(new.keys())
+        elif new or self.aw_state is None:
             self.aw_state = awproject_planes(self.wp, self.nx, self.ny, self.uvcell, chunk.freq, new, 0, psf,
                                              self.aw_state, pcf=pcf)
             self.aw_done.update(new.keys())
@@ -1118,21 +1193,23 @@ class OracleGridder:
         st[2] += len(new) if self.aw_invalid[0] else 0
         st[3] += len(new) if self.aw_invalid[1] else 0
         self.support = self.aw_state["support"]
-        self.cf = CFCache(self.aw_state["planes"], self.wp.oversample,
-                          self.aw_state["offsets"] if self.wp.offsetsupport else None)
+        offsets = self.aw_state["offsets"] if self.wp.offsetsupport and self.kind == "AWProject" else None
+        self.cf = CFCache(self.aw_state["planes"], self.wp.oversample, offsets)
 
     def n_cf_planes(self):
         if self.kind == "WProject":
             return self.wp.nwplanes
         if self.kind == "AWProject":
             return self.wp.nwplanes * self.wp.maxfeeds * self.wp.maxfields * (self.aw_nchan or 1)
+        if self.kind == "AProjectWStack":
+            return self.wp.maxfeeds * self.wp.maxfields * (self.aw_nchan or 1)
         return 1
 
     def correct(self, img):
         """correctConvolution on a padded [.., y, x] image (none for the PCF gridder,
         SphFuncVisGridder.cc:185)."""
-        if self.kind == "Box" or (self.kind == "AWProject" and not self.wp.spheroidaltaper):
-            return img
+        if self.kind in ("Box", "AProjectWStack") or (self.kind == "AWProject" and not self.wp.spheroidaltaper):
+            return img  # AProjectWStackVisGridder::correctConvolution is empty (:480-481)
         if getattr(self, "mode", 0) == 2:
             return img
         alpha = self.sph[0] if self.kind == "SphFunc" else self.wp.alpha
@@ -1173,6 +1250,15 @@ class OracleGridder:
         nrow, nchan, npol = chunk.vis.shape
         if getattr(self, "cmap_override", None) is not None:
             return self.cmap_override
+        if self.kind == "AProjectWStack":
+            # AProjectWStackVisGridder::initIndices (:122-170): (chan, feed, field), no w term
+            nch_cf = self.aw_state["nchan"]
+            cm = np.zeros((nchan, npol, nrow), dtype=np.int32)
+            for i in range(nrow):
+                feed = int(chunk.feed1[i]) if chunk.feed1 is not None else 0
+                for c in range(nchan):
+                    cm[c, :, i] = (c if self.wp.freqdep else 0) + nch_cf * (feed + self.wp.maxfeeds * 0)
+            return cm
         if self.kind not in ("WProject", "AWProject"):
             return None
         ws = self.wp.sampling()
@@ -1193,7 +1279,7 @@ class OracleGridder:
     def _run(self, chunk: Chunk, mode: int):
         L = lib()
         nrow, nchan, npol = chunk.vis.shape
-        if self.kind == "AWProject":
+        if self.kind in ("AWProject", "AProjectWStack"):
             self._init_aw(chunk, psf=(mode in (1, 2)), pcf=(mode == 2))
         else:
             self._init_cf(pcf=(mode == 2))
@@ -1271,6 +1357,8 @@ class OracleGridder:
         nx0, ny0 = self.shape0[0], self.shape0[1]
         if self.kind == "AWProject":
             return self._aw_weights()
+        if self.kind == "AProjectWStack":
+            return self._aws_weights()
         s = self.sumw.sum(axis=2)  # (chan, pol)
         return np.broadcast_to(s[:, :, None, None], (self.nchan_img, self.npol, ny0, nx0)).copy()
 
@@ -1307,6 +1395,38 @@ class OracleGridder:
             if self.wp.spheroidaltaper:
                 t = np.outer(fy, fx)
                 pl = np.where(t > 0, pl / np.where(t > 0, t, 1), 0)
+            cout += wts[:, :, None, None] * np.abs(pl)[None, None] ** 2
+        full = fft_pad(cout, self.ny, self.nx)
+        nx0, ny0 = self.shape0[0], self.shape0[1]
+        ox, oy = (self.nx - nx0) // 2, (self.ny - ny0) // 2
+        return full[:, :, oy:oy + ny0, ox:ox + nx0]
+
+    def _aws_weights(self):
+        """AProjectWStackVisGridder::finaliseWeights (AProjectWStackVisGridder.cc:377-470): each CF
+        with weight is placed over taps -S..S-1 about the centre of a min(maxsupport, grid) plane,
+        transformed to the image plane, divided by its peak modulus, and its |.|^2 summed with the
+        weights; the sum is Fourier-padded to the grid."""
+        os_ = self.wp.oversample
+        cnx, cny = min(self.wp.maxsupport, self.nx), min(self.wp.maxsupport, self.ny)
+        cx0, cy0 = cnx // 2, cny // 2
+        cout = np.zeros((self.nchan_img, self.npol, cny, cnx))
+        for iz in range(self.sumw.shape[2]):
+            wts = self.sumw[:, :, iz]
+            if not np.any(wts > 0):
+                continue
+            k = iz if self.track_per_os else os_ * os_ * iz
+            cf = self.cf.planes[k].astype(np.complex64).astype(np.complex128)
+            S = (cf.shape[0] - 1) // 2
+            pl = np.zeros((cny, cnx), dtype=np.complex128)
+            for iy in range(-S, S):
+                for ix in range(-S, S):
+                    xp, yp = ix + cx0, iy + cy0
+                    if 0 <= xp < cnx and 0 <= yp < cny:
+                        pl[yp, xp] = cf[iy + S, ix + S]
+            pl = fft_centred(pl, False)
+            peak = np.max(np.abs(pl))
+            if peak > 0:
+                pl = pl / peak
             cout += wts[:, :, None, None] * np.abs(pl)[None, None] ** 2
         full = fft_pad(cout, self.ny, self.nx)
         nx0, ny0 = self.shape0[0], self.shape0[1]

@@ -301,6 +301,9 @@ IVisGridder::ShPtr GPUGridderAdapter::createGPUAWProject(const LOFAR::ParameterS
 IVisGridder::ShPtr GPUGridderAdapter::createGPUWStack(const LOFAR::ParameterSet& p) {
   return IVisGridder::ShPtr(new GPUGridderAdapter("GPUWStack", p));
 }
+IVisGridder::ShPtr GPUGridderAdapter::createGPUAProjectWStack(const LOFAR::ParameterSet& p) {
+  return IVisGridder::ShPtr(new GPUGridderAdapter("GPUAProjectWStack", p));
+}
 
 }  // namespace synthesis
 }  // namespace askap
@@ -317,6 +320,7 @@ static void ygrid_register_all() {
   VisGridderFactory::registerGridder("GPUWProject", GPUGridderAdapter::createGPUWProject);
   VisGridderFactory::registerGridder("GPUAWProject", GPUGridderAdapter::createGPUAWProject);
   VisGridderFactory::registerGridder("GPUWStack", GPUGridderAdapter::createGPUWStack);
+  VisGridderFactory::registerGridder("GPUAProjectWStack", GPUGridderAdapter::createGPUAProjectWStack);
 }
 void register_gpubox() { ygrid_register_all(); }
 void register_gpusphfunc() { ygrid_register_all(); }

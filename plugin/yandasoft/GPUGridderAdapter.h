@@ -9,8 +9,8 @@
 /// ImageFFTEquation's source index for direction-dependent calibration
 /// (ImageFFTEquation.cc:283-287).  It overrides every IVisGridder call (IVisGridder.h:64-140) and
 /// forwards it to the C ABI of include/ygrid.h one to one; TableVisGridder's own loop is never
-/// run.  Five gridder names are registered: GPUBox, GPUSphFunc, GPUWProject, GPUAWProject and
-/// GPUWStack; a parset selects one with `gridder = GPUWProject` and keeps every
+/// run.  Six gridder names are registered: GPUBox, GPUSphFunc, GPUWProject, GPUAWProject,
+/// GPUWStack and GPUAProjectWStack; a parset selects one with `gridder = GPUWProject` and keeps every
 /// `gridder.WProject.*` key under `gridder.GPUWProject.*`.
 #ifndef YGRID_GPU_GRIDDER_ADAPTER_H
 #define YGRID_GPU_GRIDDER_ADAPTER_H
@@ -30,7 +30,8 @@ namespace synthesis {
 
 class GPUGridderAdapter : public TableVisGridder {
  public:
-  /// @param kind one of "GPUBox", "GPUSphFunc", "GPUWProject", "GPUAWProject", "GPUWStack"
+  /// @param kind one of "GPUBox", "GPUSphFunc", "GPUWProject", "GPUAWProject", "GPUWStack",
+  ///             "GPUAProjectWStack"
   /// @param parset gridder subset (gridder name already stripped), as the factory passes it
   GPUGridderAdapter(const std::string& kind, const LOFAR::ParameterSet& parset);
   ~GPUGridderAdapter() override;
@@ -54,6 +55,7 @@ class GPUGridderAdapter : public TableVisGridder {
   static IVisGridder::ShPtr createGPUWProject(const LOFAR::ParameterSet& parset);
   static IVisGridder::ShPtr createGPUAWProject(const LOFAR::ParameterSet& parset);
   static IVisGridder::ShPtr createGPUWStack(const LOFAR::ParameterSet& parset);
+  static IVisGridder::ShPtr createGPUAProjectWStack(const LOFAR::ParameterSet& parset);
 
  protected:
   /// TableVisGridder's per-gridder hooks: only its own loop calls them, and every entry point

@@ -683,6 +683,75 @@ def test_awproject_w0_plane_of_a_centred_disk_is_real_and_symmetric():
     assert np.max(np.abs(p2 - p)) > 1e-2 * peak
 
 
+def test_aprojectwstack_planes_match_brute_force_transforms():
+    """AProjectWStackVisGridder.cc:222-370: per (chan, feed) the pattern's image-plane power |A|^2,
+    divided by its peak, transformed forward and divided by nx ny, cut out at k = t os + frac for
+    t in [-S, S-1] times os^2 (the last row and column zero); no w term.  Restated with explicit
+    centre-relative DFT sums, the beam-offset phase included.  The support is 1 + 2 int(R) / os
+    with R the dish radius in pattern cells (DiskIllumination.cc:139): R = 4.15 here, S = 5, set
+    from the first channel and kept for the second (R = 4.36 there)."""
+    prm, freqs, uvcell, lm = _aw_small()
+    st = O.aprojectwstack_planes(prm, 64, 64, (uvcell, uvcell), freqs, lm)
+    S, os_, n = st["support"], prm.oversample, prm.maxsupport
+    assert S == 1 + (2 * 4) // os_ == 5
+    c = np.arange(n) - n / 2.0
+    E = np.exp(2j * np.pi * np.outer(c, c) / n)
+    cu = uvcell / os_
+    for feed, (l, m) in lm.items():
+        for ch, f in enumerate(freqs):
+            R2 = (prm.diameter / (2 * cu * O.C_LIGHT / f)) ** 2
+            disk = (c[None, :] ** 2 + c[:, None] ** 2 <= R2)
+            pat = disk * np.exp(-2j * np.pi * cu * (l * c[None, :] + m * c[:, None]))
+            p = np.abs(E @ pat @ E.T) ** 2
+            p /= p.max()
+            z = ch + len(freqs) * feed
+            t = np.arange(-S, S)
+            for fu in range(os_):
+                for fv in range(os_):
+                    Eu = np.exp(-2j * np.pi * np.outer(t * os_ + fu, c) / n)
+                    Ev = np.exp(-2j * np.pi * np.outer(t * os_ + fv, c) / n)
+                    ref = Ev @ p @ Eu.T / (n * n) * os_ * os_
+                    got = st["planes"][fu + os_ * (fv + os_ * z)]
+                    assert got.shape == (2 * S + 1, 2 * S + 1)
+                    assert not np.any(got[2 * S, :]) and not np.any(got[:, 2 * S])
+                    assert np.max(np.abs(got[:2 * S, :2 * S] - ref)) < 1e-6 * np.max(np.abs(ref)), (feed, ch, fu, fv)
+    # the peak-normalised power transforms to a kernel whose taps, summed over the oversampling
+    # planes, give os^2 / (nx ny) x the power at the origin (1 for the centred feed, where it peaks)
+    z0 = 0
+    tot = sum(st["planes"][k + os_ * os_ * z0][:2 * S, :2 * S].sum() for k in range(os_ * os_))
+    assert abs(tot - os_ * os_) < 1e-3 * os_ * os_
+
+
+def test_aprojectwstack_weights_closed_form():
+    """AProjectWStackVisGridder.cc:377-470: one CF with one nonzero tap at (dx, dy) from the centre
+    (the cut-out's taps -S..S-1) transforms to a pure phase ramp of modulus 1 after the peak
+    normalisation, so the weights image is the sum of weights everywhere; a CF with taps a at
+    the centre and b at (dx, 0) gives |a + b e^{2 pi i dx X / cnx}|^2 over its largest value on the
+    cnx pixels (cnx = grid here, so the padding leaves it as is)."""
+    prm = O.AWProjectParams(wmax=300.0, nwplanes=1, oversample=2, maxsupport=32)
+    g = O.OracleGridder("AProjectWStack", "f64", wproject=prm)
+    ngrid = 32
+    g._setup((ngrid, ngrid, 1, 1), (1e-3, 1e-3))
+    S = 3
+    a, b, dx = 0.8 - 0.1j, 0.3 + 0.2j, 2
+    cf = np.zeros((2 * S + 1, 2 * S + 1), np.complex128)
+    cf[S, S], cf[S, S + dx] = a, b
+    planes = [cf if k == 0 else np.zeros_like(cf) for k in range(4)]
+    g.cf = O.CFCache(planes, 2, None)
+    g.sumw = np.zeros((1, 1, 1))
+    g.sumw[0, 0, 0] = 2.5
+    w = g._aws_weights()[0, 0]
+    X = np.arange(ngrid) - ngrid // 2
+    e = np.abs(a + b * np.exp(2j * np.pi * dx * X / ngrid)) ** 2
+    e /= e.max()                                   # the peak over the sampled pixels
+    assert np.max(np.abs(w - 2.5 * e[None, :])) < 1e-6      # the CF cache holds complex float
+    # the last row and column of the cut-out are left out of the placement
+    cf2 = np.zeros_like(cf)
+    cf2[S, S], cf2[2 * S, S] = 1.0, 5.0
+    g.cf = O.CFCache([cf2] + [np.zeros_like(cf)] * 3, 2, None)
+    assert np.max(np.abs(g._aws_weights()[0, 0] - 2.5)) < 1e-6
+
+
 def _aw_weights_gridder(ngrid, taper=False):
     prm = O.AWProjectParams(wmax=300.0, nwplanes=1, oversample=2, maxsupport=32, spheroidaltaper=taper,
                             spheroidalweightscutoff=0.05)

@@ -1,6 +1,7 @@
 // AWProject gridder: A-projection (disk or ATCA illumination) x W-projection convolution functions
 // (AWProjectVisGridder.cc, AProjectGridderBase.cc/.tcc, DiskIllumination.cc, ATCAIllumination.cc),
-// built on the device.
+// built on the device; and its w-stacking form, AProjectWStack (AProjectWStackVisGridder.cc):
+// A-projection CFs without the w term, one grid per w plane.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,7 +16,9 @@ namespace ygrid {
 
 class AWProjectGridder : public Gridder {
  public:
-  AWProjectGridder(std::shared_ptr<Ctx> c, const Params& p) : Gridder(std::move(c)) {
+  // wstack: AProjectWStackVisGridder (its createGridder is createAProjectGridder alone:
+  // AProjectWStackVisGridder.cc:490-493, AProjectGridderBase.tcc:53-115)
+  AWProjectGridder(std::shared_ptr<Ctx> c, const Params& p, bool wstack = false) : Gridder(std::move(c)), wstack_(wstack) {
     // AProjectGridderBase::createAProjectGridder (AProjectGridderBase.tcc:53-115)
     pointing_tol_ = quantity(p.str("pointingtolerance", "0.0001rad"), "rad");
     pa_tol_ = quantity(p.str("patolerance", "0.1rad"), "rad");
@@ -41,15 +44,16 @@ class AWProjectGridder : public Gridder {
     abs_cutoff_ = p.flag("cutoff.absolute", false);
     // WProjectVisGridder::configureGridder (WProjectVisGridder.cc:331-363), which
     // AWProjectVisGridder::createGridder calls (AWProjectVisGridder.cc:883-888)
-    variable_ = p.flag("variablesupport", false);
-    offset_ = p.flag("offsetsupport", false);
+    variable_ = !wstack && p.flag("variablesupport", false);
+    offset_ = !wstack && p.flag("offsetsupport", false);
     YG_CHECK(!offset_ || variable_, 1,
              "offsetsupport option of the gridder should only be used together with variablesupport option");
     YG_CHECK(ws_.wmax > 0.0, 1, "Baseline length must be greater than zero");
     YG_CHECK(ws_.nw > 0 && ws_.nw % 2 == 1, 1, "Number of w planes must be a positive odd number");
     YG_CHECK(os_ > 0 && maxsupport_ > 0 && maxsupport_ % 2 == 0, 1, "bad oversample / maxsupport");
     YG_CHECK(maxfeeds_ > 0 && maxfields_ > 0, 1, "maxfeeds and maxfields must be positive");
-    if (!abs_cutoff_) YG_CHECK(cutoff_ > 0.0 && cutoff_ < 1.0, 1, "Cutoff must be in (0, 1)");
+    if (!abs_cutoff_ && !wstack) YG_CHECK(cutoff_ > 0.0 && cutoff_ < 1.0, 1, "Cutoff must be in (0, 1)");
+    YG_CHECK(!(wstack && sph_taper_), 1, "Spheroidal taper not currently availble for AProjectWStack");
     // makeIllumination (AProjectGridderBase.cc:290-348): the disk and the ATCA models
     const std::string ill = p.str("illumination", "disk");
     YG_CHECK(ill == "disk" || ill == "ATCA", 1,
@@ -99,20 +103,26 @@ class AWProjectGridder : public Gridder {
     auto g = std::unique_ptr<AWProjectGridder>(new AWProjectGridder(*this));
     return g;
   }
-  std::string name() const override { return "GPUAWProject"; }
+  std::string name() const override { return wstack_ ? "GPUAProjectWStack" : "GPUAWProject"; }
 
  protected:
   int oversampling() const override { return os_; }
-  int nZPlanes() const override { return ws_.nw * maxfeeds_ * maxfields_ * std::max(1, nchan_cf_); }
+  // CF planes before oversampling: AWProject w x feeds x fields x channels; AProjectWStack
+  // feeds x fields x channels (its w term is in the grids)
+  int nZPlanes() const override { return (wstack_ ? 1 : ws_.nw) * maxfeeds_ * maxfields_ * std::max(1, nchan_cf_); }
+  int nGridPlanes() const override { return wstack_ ? ws_.nw : 1; }
   bool needsRowMeta() const override { return true; }
   bool cfPerChunk() const override { return true; }
-  bool doesCorrection() const override { return sph_taper_; }
+  // AProjectWStackVisGridder::correctConvolution is empty (AProjectWStackVisGridder.cc:479-480)
+  bool doesCorrection() const override { return !wstack_ && sph_taper_; }
+  void gridsToImage(const double* cx, const double* cy, float* out) override;
+  void modelToGrids(const float* model, const double* cx, const double* cy) override;
   void gridCorrection(std::vector<double>& cx, std::vector<double>& cy) const override {
     if (sph_taper_) sph_grid_correction_xy(shape_[0], shape_[1], alpha_, 3, cx, cy);
   }
   CIndexDesc cindex(const ChunkDev&) const override {
     CIndexDesc d;
-    d.kind = 2;
+    d.kind = wstack_ ? 4 : 2;
     d.nw = ws_.nw;
     d.wscale = ws_.wscale();
     d.wmax = ws_.wmax;
@@ -150,6 +160,14 @@ class AWProjectGridder : public Gridder {
   void indexField(const ChunkDev& acc);
   void validateCache(const ChunkDev& acc);
   void generateFeed(const ChunkDev& acc, int feed, double lon, double lat, double pa);
+  // the aperture pattern of one channel (DiskIllumination::getPattern / ATCAIllumination::getPattern
+  // parameters) on an nx x ny buffer
+  IllumParams illumParams(double freq, double l, double m, double pa, int nx, int ny) const;
+  // AProjectWStackVisGridder::initConvolutionFunction for one (feed, field) (:258-370)
+  void generateWStackFeed(const ChunkDev& acc, int feed, double lon, double lat, double pa);
+  std::vector<int> usedPlanes();
+  void wscreen(int iw, float& w2pi, float& cellx, float& celly) const;
+  bool wstack_ = false;
   void ensureTable(int S);
   void writePCFPlanes(const Ctx& c, DevBuf& buf, int nx, int ny, int cnt, int z0, int chan, int nchan, int feed);
   int pcf_support_ = 0;  // preconditioner form: the support found on the first plane
@@ -238,7 +256,7 @@ void AWProjectGridder::validateCache(const ChunkDev& acc) {
 void AWProjectGridder::ensureTable(int S) {
   if (cf_ && cf_.use_count() == 1 && cf_->support == S) return;
   auto t = std::make_shared<CFTable>();
-  const int nz = ws_.nw * maxfeeds_ * maxfields_ * nchan_cf_;
+  const int nz = nZPlanes();
   t->nz = nz;
   t->oversample = os_;
   t->support = S;
@@ -265,7 +283,71 @@ void AWProjectGridder::ensureTable(int S) {
 // (AWProjectVisGridder.cc:366-573): per channel, the disk pattern is transformed to the image
 // plane; per w plane, |A|^2 x taper x phase screen is transformed back, the support is found on
 // the first plane ever generated, and the os^2 cut-outs are scaled by os^2.
+// DiskIllumination::getPattern (DiskIllumination.cc:85-177), ATCAIllumination::getPattern
+// (ATCAIllumination.cc:159-331): the parameters of one channel's pattern on an nx x ny buffer of
+// cells uvcell / os, with the normalisation of the pattern
+IllumParams AWProjectGridder::illumParams(double freq, double l, double m, double pa, int nx, int ny) const {
+  const double cellU = uvcell_[0] / os_, cellV = uvcell_[1] / os_;
+  YG_CHECK(std::fabs(std::fabs(cellU / cellV) - 1.) < 1e-7, 1, "Rectangular cells are not supported");
+  const double cell = std::fabs(cellU * (kC / freq));
+  const double rmax2 = (diameter_ / (2.0 * cell)) * (diameter_ / (2.0 * cell));
+  double rmin2 = (blockage_ / (2.0 * cell)) * (blockage_ / (2.0 * cell));
+  IllumParams ip{};
+  ip.lscaled = 2. * kPi * cellU * l;
+  ip.mscaled = 2. * kPi * cellV * m;
+  if (!atca_) {
+    YG_CHECK(rmax2 > rmin2, 1, "Disk hole is supposed to be less than the disk size");
+    YG_CHECK(double(nx) * nx > rmax2 && double(ny) * ny > rmax2, 1,
+             "The pattern buffer is too small for the dish: increase maxsupport");
+    if (rmin2 + 1 >= rmax2) --rmin2;
+    ip.kind = kIllumDisk;
+  } else {
+    YG_CHECK(double(nx) * nx > rmax2 || double(ny) * ny > rmax2, 1,
+             "The pattern buffer is too small for the dish: increase maxsupport");
+    ip.kind = kIllumATCA;
+    ip.taper = a_taper_ ? 1 : 0;
+    ip.legs = a_legs_ ? 1 : 0;
+    ip.wedges = a_legs_ && a_wedges_ ? 1 : 0;
+    ip.defocus = a_defocus_;
+    ip.leg_half = a_leg_half_ / cell;
+    ip.leg_angle = a_leg_rot_ + pa;
+    ip.leg_shadow = a_leg_shadow_;
+    ip.wedge_tan = std::tan(a_wedge_angle_);
+    ip.wedge_r0 = a_wedge_r0_ / cell;
+    ip.wedge_s1 = a_wedge_s1_;
+    ip.wedge_s2 = a_wedge_s2_;
+  }
+  ip.rmin2 = rmin2;
+  ip.rmax2 = rmax2;
+  // the normalisation: the disk counts its cells (in float), ATCA sums |value|^2 (in double)
+  double sum = 0;
+  for (int iu = 0; iu < nx; ++iu) {
+    const double ou = double(iu) - double(nx) / 2.;
+    for (int iv = 0; iv < ny; ++iv) {
+      const double ov = double(iv) - double(ny) / 2.;
+      const double r2 = ou * ou + ov * ov;
+      if (r2 >= rmin2 && r2 <= rmax2) {
+        double vr, vi;
+        illum_value(ip, ou, ov, r2, vr, vi);
+        sum += atca_ ? vr * vr + vi * vi : 1.;
+      }
+    }
+  }
+  if (!atca_) {
+    YG_CHECK(sum > 0., 4, "Integral of the disk should be non-zero");
+    ip.scale = (float)(float(nx) * float(ny) / float(sum));
+  } else {
+    YG_CHECK(sum > 0., 4, "Integral of the pattern should be non-zero");
+    ip.scale = (float)(double(nx) * double(ny) / std::sqrt(sum));
+  }
+  return ip;
+}
+
 void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, double lat, double pa) {
+  if (wstack_) {
+    generateWStackFeed(acc, feed, lon, lat, pa);
+    return;
+  }
   const Ctx& c = *ctx_;
   const int nx = maxsupport_, ny = maxsupport_;
   const int qnx = nx / os_, qny = ny / os_;
@@ -305,62 +387,7 @@ void AWProjectGridder::generateFeed(const ChunkDev& acc, int feed, double lon, d
   const int nchan = nchan_cf_;
   std::vector<VarPiece> pieces;  // variablesupport: this (feed, field)'s planes
   for (int chan = 0; chan < nchan; ++chan) {
-    // DiskIllumination::getPattern (DiskIllumination.cc:85-177), ATCAIllumination::getPattern
-    // (ATCAIllumination.cc:159-331)
-    const double freq = acc.freq_host[chan];
-    const double cellU = uvcell_[0] / os_, cellV = uvcell_[1] / os_;
-    YG_CHECK(std::fabs(std::fabs(cellU / cellV) - 1.) < 1e-7, 1, "Rectangular cells are not supported");
-    const double cell = std::fabs(cellU * (kC / freq));
-    const double rmax2 = (diameter_ / (2.0 * cell)) * (diameter_ / (2.0 * cell));
-    double rmin2 = (blockage_ / (2.0 * cell)) * (blockage_ / (2.0 * cell));
-    IllumParams ip{};
-    ip.lscaled = 2. * kPi * cellU * l;
-    ip.mscaled = 2. * kPi * cellV * m;
-    if (!atca_) {
-      YG_CHECK(rmax2 > rmin2, 1, "Disk hole is supposed to be less than the disk size");
-      YG_CHECK(double(nx) * nx > rmax2 && double(ny) * ny > rmax2, 1,
-               "The pattern buffer is too small for the dish: increase maxsupport");
-      if (rmin2 + 1 >= rmax2) --rmin2;
-      ip.kind = kIllumDisk;
-    } else {
-      YG_CHECK(double(nx) * nx > rmax2 || double(ny) * ny > rmax2, 1,
-               "The pattern buffer is too small for the dish: increase maxsupport");
-      ip.kind = kIllumATCA;
-      ip.taper = a_taper_ ? 1 : 0;
-      ip.legs = a_legs_ ? 1 : 0;
-      ip.wedges = a_legs_ && a_wedges_ ? 1 : 0;
-      ip.defocus = a_defocus_;
-      ip.leg_half = a_leg_half_ / cell;
-      ip.leg_angle = a_leg_rot_ + pa;
-      ip.leg_shadow = a_leg_shadow_;
-      ip.wedge_tan = std::tan(a_wedge_angle_);
-      ip.wedge_r0 = a_wedge_r0_ / cell;
-      ip.wedge_s1 = a_wedge_s1_;
-      ip.wedge_s2 = a_wedge_s2_;
-    }
-    ip.rmin2 = rmin2;
-    ip.rmax2 = rmax2;
-    // the normalisation: the disk counts its cells (in float), ATCA sums |value|^2 (in double)
-    double sum = 0;
-    for (int iu = 0; iu < nx; ++iu) {
-      const double ou = double(iu) - double(nx) / 2.;
-      for (int iv = 0; iv < ny; ++iv) {
-        const double ov = double(iv) - double(ny) / 2.;
-        const double r2 = ou * ou + ov * ov;
-        if (r2 >= rmin2 && r2 <= rmax2) {
-          double vr, vi;
-          illum_value(ip, ou, ov, r2, vr, vi);
-          sum += atca_ ? vr * vr + vi * vi : 1.;
-        }
-      }
-    }
-    if (!atca_) {
-      YG_CHECK(sum > 0., 4, "Integral of the disk should be non-zero");
-      ip.scale = (float)(float(nx) * float(ny) / float(sum));
-    } else {
-      YG_CHECK(sum > 0., 4, "Integral of the pattern should be non-zero");
-      ip.scale = (float)(double(nx) * double(ny) / std::sqrt(sum));
-    }
+    const IllumParams ip = illumParams(acc.freq_host[chan], l, m, pa, nx, ny);
     YG_HIP(hipMemcpyAsync(dprm.get(), &ip, sizeof(ip), hipMemcpyHostToDevice, c.stream));
     launch_illum_pattern(c, pat.as<float2>(), nx, ny, dprm.as<IllumParams>(), 1);
     // fft2d(pattern, false): backward, normalised by 1/(nx ny) -> folded into the screen kernel
@@ -524,6 +551,121 @@ void AWProjectGridder::commitVariable(int feed, std::vector<VarPiece>& pieces) {
   pieces.clear();
 }
 
+// AProjectWStackVisGridder::initConvolutionFunction for one (feed, field) (:258-370): per channel
+// the pattern (on min(maxsupport, grid) cells of uvcell / os) is transformed to the image plane,
+// its power |A|^2 normalised to a peak of 1 is transformed back (/ nx ny), and cut out at
+// k = t os + frac about the centre for t in [-S, S-1] (the last row and column of each plane stay
+// zero), times os^2.  S is set once, from the first channel's pattern: the dish diameter in
+// pattern cells, 1 + 2 int(R) / os (DiskIllumination.cc:139, ATCAIllumination.cc:245).
+void AWProjectGridder::generateWStackFeed(const ChunkDev& acc, int feed, double lon, double lat, double pa) {
+  const Ctx& c = *ctx_;
+  const int nx = std::min(maxsupport_, shape_[0]), ny = std::min(maxsupport_, shape_[1]);
+  const bool psf = dopsf_ || dopcf_;
+  double l = 0., m = 0.;
+  if (!psf) {
+    l = std::sin(lon - centre_[0]) * std::cos(lat);
+    m = std::sin(lat) * std::cos(centre_[1]) - std::cos(lat) * std::sin(centre_[1]) * std::cos(lon - centre_[0]);
+  }
+  const size_t plane_bytes = (size_t)nx * ny * sizeof(float2);
+  DevBuf pat(plane_bytes), buf(plane_bytes), dprm(sizeof(IllumParams)), tmp(sizeof(uint32_t));
+  DevBuf dfirst(4), dsup(4);
+  PlanPtr plan1, plan2;
+  for (int chan = 0; chan < nchan_cf_; ++chan) {
+    const double freq = acc.freq_host[chan];
+    const IllumParams ip = illumParams(freq, l, m, pa, nx, ny);
+    YG_HIP(hipMemcpyAsync(dprm.get(), &ip, sizeof(ip), hipMemcpyHostToDevice, c.stream));
+    launch_illum_pattern(c, pat.as<float2>(), nx, ny, dprm.as<IllumParams>(), 1);
+    fft_plain_c2c(c, pat.as<float2>(), nx, ny, 1, false, plan1);  // fft2d(pattern, false)
+    launch_aws_cf_power(c, pat.as<float2>(), nx, ny, (float)(1.0 / (double(nx) * ny)), buf.as<float2>(),
+                        tmp.as<uint32_t>());
+    fft_plain_c2c(c, buf.as<float2>(), nx, ny, 1, true, plan2);  // fft2d(pattern, true) / (nx ny)
+    launch_scale_sign(c, buf.as<float2>(), nx, ny, 1, (float)(1.0 / (double(nx) * ny)));
+    if (!cf_ || cf_->support == 0) {
+      const double cell = std::fabs(uvcell_[0] / os_ * (kC / freq));
+      int S = 1 + (int)(2u * (unsigned)(diameter_ / (2.0 * cell)) / (unsigned)os_);
+      YG_CHECK(S > 0, 4, "Unable to determine support of convolution function");
+      YG_CHECK(S * os_ < nx / 2, 4, "Overflowing convolution function - increase maxSupport or decrease overSample");
+      if (limitsupport_ > 0 && S > limitsupport_) S = limitsupport_;
+      ensureTable(S);
+    } else {
+      ensureTable(cf_->support);
+    }
+    const int S = cf_->support;
+    const int z = chan + nchan_cf_ * (feed + maxfeeds_ * field_);
+    const int32_t first = cf_->off[(size_t)z * os_ * os_];
+    YG_HIP(hipMemcpy(dfirst.get(), &first, 4, hipMemcpyHostToDevice));
+    YG_HIP(hipMemcpy(dsup.get(), &S, 4, hipMemcpyHostToDevice));
+    launch_cf_cutout(c, buf.as<float2>(), nx, ny, 1, os_, dsup.as<int>(), nullptr, nullptr, dfirst.as<int32_t>(), 2,
+                     double(os_) * os_, cf_->table.as<float2>());
+    YG_HIP(hipStreamSynchronize(c.stream));
+  }
+}
+
+// the w planes whose grids received samples (WStackVisGridder.cc:236,272 skip planes whose
+// largest amplitude is 0)
+std::vector<int> AWProjectGridder::usedPlanes() {
+  const size_t ng = gridPlaneElems();
+  DevBuf flags((size_t)ws_.nw * sizeof(int));
+  launch_fill_u32(*ctx_, flags.get(), ws_.nw, 0u);
+  for (int iw = 0; iw < ws_.nw; ++iw)
+    launch_any_nonzero(*ctx_, reinterpret_cast<const float*>(grid_.as<float2>() + ng * iw), 2 * ng,
+                       flags.as<int>() + iw);
+  std::vector<int> f(ws_.nw, 0), used;
+  YG_HIP(hipMemcpyAsync(f.data(), flags.get(), f.size() * sizeof(int), hipMemcpyDeviceToHost, ctx_->stream));
+  YG_HIP(hipStreamSynchronize(ctx_->stream));
+  for (int iw = 0; iw < ws_.nw; ++iw)
+    if (f[iw]) used.push_back(iw);
+  return used;
+}
+
+// WStackVisGridder::multiply's screen constants (WStackVisGridder.cc:137-144)
+void AWProjectGridder::wscreen(int iw, float& w2pi, float& cellx, float& celly) const {
+  w2pi = (float)(2.0f * kPi * ws_.wterm(iw));
+  cellx = (float)(1.0 / (float(shape_[0]) * uvcell_[0]));
+  celly = (float)(1.0 / (float(shape_[1]) * uvcell_[1]));
+}
+
+// AProjectWStack: WStackVisGridder::finaliseGrid (WStackVisGridder.cc:255-289) with the empty
+// correction: per used w plane the inverse transform times its screen, real parts summed.
+void AWProjectGridder::gridsToImage(const double* cx, const double* cy, float* dout) {
+  if (!wstack_) {
+    Gridder::gridsToImage(cx, cy, dout);
+    return;
+  }
+  const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
+  const size_t ng = gridPlaneElems();
+  const std::vector<int> used = usedPlanes();
+  DevBuf acc(ng * sizeof(double));
+  launch_fill_u32(*ctx_, acc.get(), 2 * ng, 0u);
+  scratch_.resize(ng * sizeof(double2));
+  for (int iw : used) {
+    float w2pi, cellx, celly;
+    wscreen(iw, w2pi, cellx, celly);
+    wstack_accumulate(*ctx_, grid_.as<float2>() + ng * iw, nxp, nyp, np, w2pi, cellx, celly, scratch_.as<double2>(),
+                      acc.as<double>(), plan_);
+  }
+  wstack_finalise(*ctx_, acc.as<double>(), nxp, nyp, np, nullptr, nullptr, img_[0], img_[1], dout);
+  YG_HIP(hipStreamSynchronize(ctx_->stream));
+}
+
+// AProjectWStack: WStackVisGridder::initialiseDegrid (WStackVisGridder.cc:314-327) without the
+// correction: each w plane's grid is the forward transform of conj(screen x model).
+void AWProjectGridder::modelToGrids(const float* dmodel, const double* cx, const double* cy) {
+  if (!wstack_) {
+    Gridder::modelToGrids(dmodel, cx, cy);
+    return;
+  }
+  const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
+  const size_t ng = gridPlaneElems();
+  scratch_.resize(ng * sizeof(double2));
+  for (int iw = 0; iw < ws_.nw; ++iw) {
+    float w2pi, cellx, celly;
+    wscreen(iw, w2pi, cellx, celly);
+    wstack_model_to_grid(*ctx_, dmodel, img_[0], img_[1], np, nullptr, nullptr, nxp, nyp, w2pi, cellx, celly,
+                         scratch_.as<double2>(), grid_.as<float2>() + ng * iw, plan_);
+  }
+}
+
 // Preconditioner form (AWProjectVisGridder.cc:436-466,511-516): the support is found on the first
 // plane generated as for the imaging kernels, and every plane becomes 3x3 with the centre tap
 // (1, support); the table itself then has support 1.
@@ -581,6 +723,7 @@ void AWProjectGridder::initConvolutionFunction(const ChunkDev& acc) {
   YG_CHECK(!acc.freq_host.empty(), 1, "no channels");
   if (nchan_cf_ == 0) nchan_cf_ = freqdep_ ? acc.nchan : 1;
   YG_CHECK(!freqdep_ || acc.nchan == nchan_cf_, 1, "the number of channels changed between chunks");
+  YG_CHECK(!(wstack_ && dopcf_), 1, "GPUAProjectWStack has no preconditioner (PCF) pass");
   if (zsupp_.empty()) {
     const size_t nz = (size_t)ws_.nw * maxfeeds_ * maxfields_ * nchan_cf_;
     zsupp_.assign(nz, 0);
@@ -617,6 +760,8 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
   const Ctx& c = *ctx_;
   const int nx = shape_[0], ny = shape_[1], npol = shape_[2], nchan = shape_[3], ncp = npol * nchan;
   const int cnx = std::min(maxsupport_, nx), cny = std::min(maxsupport_, ny);
+  // AProjectWStackVisGridder::finaliseWeights (:373-473): taps -S .. S-1 placed, each transform
+  // normalised to its peak before squaring
   std::vector<double> sw((size_t)nsw_rows_ * ncp);
   YG_HIP(hipMemcpyAsync(sw.data(), sumw_.get(), sw.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   YG_HIP(hipStreamSynchronize(c.stream));
@@ -652,6 +797,7 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
     const int nb = std::max(1, std::min((int)rows.size(), (int)(((size_t)1 << 28) / ((size_t)cnx * cny * 8))));
     DevBuf planes((size_t)nb * cnx * cny * sizeof(double2)), dw((size_t)nb * ncp * sizeof(double));
     DevBuf doff(nb * 4), dsup(nb * 4), dou(nb * 4), dov(nb * 4);
+    DevBuf max2(wstack_ ? (size_t)nb * sizeof(unsigned long long) : 0);
     PlanPtr plan;
     for (size_t r0 = 0; r0 < rows.size(); r0 += nb) {
       const int cnt = (int)std::min<size_t>(nb, rows.size() - r0);
@@ -680,12 +826,12 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
       YG_HIP(hipMemcpy(dov.get(), ov.data(), cnt * 4, hipMemcpyHostToDevice));
       YG_HIP(hipMemcpy(dw.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice));
       launch_place_cf(c, cf_->table.as<float2>(), doff.as<int32_t>(), dsup.as<int>(), dou.as<int>(), dov.as<int>(),
-                      cnx, cny, cnt, planes.as<double2>());
+                      cnx, cny, cnt, planes.as<double2>(), wstack_ ? 1 : 0);
       // fft2d(thisPlane, false) * (cnx*cny): the unnormalised backward transform
       fft_plain_z2z(c, planes.as<double2>(), cnx, cny, cnt, false, plan);
       launch_aw_weights_acc(c, planes.as<double2>(), cnt, cnx, cny, dw.as<double>(), ncp,
                             sph_taper_ ? dtx.as<double>() : nullptr, sph_taper_ ? dty.as<double>() : nullptr,
-                            cout.as<double>());
+                            cout.as<double>(), wstack_ ? max2.as<unsigned long long>() : nullptr);
     }
   }
   // PaddingUtils::fftPad onto the padded image, then the unpadded centre
@@ -710,6 +856,10 @@ void AWProjectGridder::finaliseWeights(float* out, int mem) {
 
 std::unique_ptr<Gridder> make_awproject(std::shared_ptr<Ctx> ctx, const Params& p) {
   return std::unique_ptr<Gridder>(new AWProjectGridder(std::move(ctx), p));
+}
+
+std::unique_ptr<Gridder> make_aprojectwstack(std::shared_ptr<Ctx> ctx, const Params& p) {
+  return std::unique_ptr<Gridder>(new AWProjectGridder(std::move(ctx), p, true));
 }
 
 }  // namespace ygrid

@@ -16,7 +16,7 @@ namespace ygrid {
 
 // helpers declared in the kernel units
 void launch_gather_cross(const Ctx& c, const float2* plane, int nx, int ny, int px, int py, float2* out);
-void launch_any_nonzero(const Ctx& c, const float* data, size_t n, int* flag);
+
 
 // ---------------------------------------------------------------------------------------
 // parset
@@ -1273,6 +1273,7 @@ void WProjectGridder::initConvolutionFunction(const ChunkDev&) {
 // factory (VisGridderFactory::make, VisGridderFactory.cc:112-286)
 // ---------------------------------------------------------------------------------------
 std::unique_ptr<Gridder> make_awproject(std::shared_ptr<Ctx> ctx, const Params& p);
+std::unique_ptr<Gridder> make_aprojectwstack(std::shared_ptr<Ctx> ctx, const Params& p);
 
 // ---------------------------------------------------------------------------------------
 // WStack (WStackVisGridder.cc): the SphFunc kernel (WDependentGridderBase derives from
@@ -1396,6 +1397,8 @@ std::unique_ptr<Gridder> make_gridder(std::shared_ptr<Ctx> ctx, const std::strin
     g.reset(new WStackGridder(ctx, p));
   } else if (name == "GPUAWProject") {
     g = make_awproject(ctx, p);
+  } else if (name == "GPUAProjectWStack") {
+    g = make_aprojectwstack(ctx, p);
   } else {
     throw Error(5, "Unknown gridder " + name);
   }

@@ -2,6 +2,8 @@
 // (AWProjectVisGridder.cc:259-611,618-770; DiskIllumination.cc:85-177, ATCAIllumination.cc:159-331).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "illumination.h"
 #include "ygrid_internal.h"
 
@@ -76,16 +78,18 @@ __global__ void k_scale_sign(float2* data, int nx, int ny, float gsign, float sc
 
 // Weights image: place the frac-0 CF plane of row z at the centre of a cnx x cny plane
 // (AWProjectVisGridder.cc:683-697), pre-multiplied by the centred backward transform's checkerboard.
+// asym: taps -S .. S-1 only (AProjectWStackVisGridder.cc:432-441)
 __global__ void k_place_cf(const float2* __restrict__ table, const int32_t* __restrict__ off, const int* __restrict__ supp,
-                           const int* __restrict__ ou, const int* __restrict__ ov, int cnx, int cny,
+                           const int* __restrict__ ou, const int* __restrict__ ov, int cnx, int cny, int asym,
                            double2* __restrict__ out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, b = blockIdx.z;
   if (x >= cnx) return;
   const int S = supp[b];
+  const int hi = asym ? S - 1 : S;
   const int ix = x - cnx / 2 - ou[b], iy = y - cny / 2 - ov[b];
   double2 v = make_double2(0.0, 0.0);
-  if (ix >= -S && ix <= S && iy >= -S && iy <= S) {
+  if (ix >= -S && ix <= hi && iy >= -S && iy <= hi) {
     const int P = 2 * S + 1 + 2 * kGuardU;
     const float2 t = table[off[b] + (size_t)(iy + S + kGuardV) * P + ix + S + kGuardU];
     const double s = cb_sign(x, y);
@@ -98,9 +102,11 @@ __global__ void k_place_cf(const float2* __restrict__ table, const int32_t* __re
 
 // cOut[cp] += sum_b wt[b][cp] |val_b / taper|^2 where val_b is the raw transform of plane b
 // (centring sign drops out of |.|^2).
+// inv_max2 (or null): per plane b a factor on |val_b|^2 (1 / max |val_b|^2: AProjectWStack's
+// transforms normalised to their peak)
 __global__ void k_aw_weights_acc(const double2* __restrict__ planes, int nb, int cnx, int cny, const double* __restrict__ wt,
                                  int ncp, const double* __restrict__ tx, const double* __restrict__ ty,
-                                 double* __restrict__ cout) {
+                                 const unsigned long long* __restrict__ max2_bits, double* __restrict__ cout) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, cp = blockIdx.z;
   if (x >= cnx) return;
@@ -115,9 +121,55 @@ __global__ void k_aw_weights_acc(const double2* __restrict__ planes, int nb, int
     if (tx) {
       if (t > 0) { vr /= t; vi /= t; } else { vr = vi = 0.0; }
     }
-    acc += w * (vr * vr + vi * vi);
+    double p = vr * vr + vi * vi;
+    if (max2_bits) {
+      const double mx = __longlong_as_double((long long)max2_bits[b]);
+      p = mx > 0.0 ? p / mx : 0.0;
+    }
+    acc += w * p;
   }
   cout[(size_t)cp * cnx * cny + (size_t)y * cnx + x] += acc;
+}
+
+// per plane b: the largest |v|^2, as the bits of a non-negative double (they order as the values)
+__global__ void k_plane_max2(const double2* __restrict__ planes, int n, unsigned long long* __restrict__ out) {
+  const int b = blockIdx.y;
+  double m = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double2 v = planes[(size_t)b * n + i];
+    m = fmax(m, v.x * v.x + v.y * v.y);
+  }
+  for (int d = 32; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d));
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(&out[b], (unsigned long long)__double_as_longlong(m));
+}
+
+// AProjectWStack (AProjectWStackVisGridder.cc:296-317): the aperture power |A|^2 of the
+// backward-transformed pattern (raw transform times ascale), in complex float, and its peak
+// (float bits; AProjectWStack divides by it before the forward transform)
+__global__ void k_aws_power(const float2* __restrict__ A, int n, float ascale, float2* __restrict__ out,
+                            uint32_t* __restrict__ peak_bits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float p = 0.f;
+  if (i < n) {
+    const float ar = A[i].x * ascale, ai = A[i].y * ascale;
+    p = ar * ar + ai * ai;
+    out[i] = make_float2(p, 0.f);
+  }
+  float m = p;
+  for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d));
+  if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(peak_bits, __float_as_uint(m));
+}
+
+// power / peak, pre-multiplied by the forward transform's checkerboard
+__global__ void k_aws_norm(float2* __restrict__ buf, int nx, int ny, const uint32_t* __restrict__ peak_bits) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= nx) return;
+  const float pk = __uint_as_float(*peak_bits);
+  const float inv = pk > 0.f ? 1.f / pk : 0.f;
+  float2& v = buf[(size_t)y * nx + x];
+  v.x = v.x * inv * cb_sign(x, y);
+  v.y = v.y * inv * cb_sign(x, y);
 }
 
 // Zero-padding of centred spectra: out (nx x ny) <- in (cnx x cny) at the centre.
@@ -168,16 +220,30 @@ void launch_scale_sign(const Ctx& c, float2* data, int nx, int ny, int nb, float
 }
 
 void launch_place_cf(const Ctx& c, const float2* table, const int32_t* off, const int* supp, const int* ou, const int* ov,
-                     int cnx, int cny, int nb, double2* out) {
+                     int cnx, int cny, int nb, double2* out, int asym) {
   hipLaunchKernelGGL(k_place_cf, dim3((cnx + 255) / 256, cny, nb), dim3(256), 0, c.stream, table, off, supp, ou, ov, cnx,
-                     cny, out);
+                     cny, asym, out);
   YG_HIP(hipGetLastError());
 }
 
 void launch_aw_weights_acc(const Ctx& c, const double2* planes, int nb, int cnx, int cny, const double* wt, int ncp,
-                           const double* tx, const double* ty, double* cout) {
+                           const double* tx, const double* ty, double* cout, unsigned long long* max2_tmp) {
+  if (max2_tmp) {
+    launch_fill_u32(c, max2_tmp, 2 * (size_t)nb, 0u);
+    const int n = cnx * cny;
+    hipLaunchKernelGGL(k_plane_max2, dim3(std::max(1, std::min(64, (n + 255) / 256)), nb), dim3(256), 0, c.stream,
+                       planes, n, max2_tmp);
+  }
   hipLaunchKernelGGL(k_aw_weights_acc, dim3((cnx + 255) / 256, cny, ncp), dim3(256), 0, c.stream, planes, nb, cnx, cny,
-                     wt, ncp, tx, ty, cout);
+                     wt, ncp, tx, ty, max2_tmp, cout);
+  YG_HIP(hipGetLastError());
+}
+
+void launch_aws_cf_power(const Ctx& c, const float2* A, int nx, int ny, float ascale, float2* buf, uint32_t* tmp) {
+  launch_fill_u32(c, tmp, 1, 0u);
+  const int n = nx * ny;
+  hipLaunchKernelGGL(k_aws_power, dim3((n + 255) / 256), dim3(256), 0, c.stream, A, n, ascale, buf, tmp);
+  hipLaunchKernelGGL(k_aws_norm, dim3((nx + 255) / 256, ny), dim3(256), 0, c.stream, buf, nx, ny, tmp);
   YG_HIP(hipGetLastError());
 }
 

@@ -104,10 +104,13 @@ __global__ __launch_bounds__(256) void k_cf_cutout(const float2* __restrict__ bu
   const int P = W + 2 * kGuardU, R = W + 2 * kGuardV;
   // the os^2 planes of one z share S, so they are consecutive blocks of P*R elements
   float2* dst = table + cf_off_first[b] + (size_t)frac * (size_t)P * R;
+  // mode 2: taps -S .. S-1 only, the last row and column of the plane zero (the cut-out loops of
+  // AProjectWStackVisGridder.cc:342-350)
+  const int Wc = mode == 2 ? W - 1 : W;
   for (int t = threadIdx.x; t < P * R; t += blockDim.x) {
     const int du = t % P - kGuardU, dv = t / P - kGuardV;
     float2 v = make_float2(0.f, 0.f);
-    if (du >= 0 && du < W && dv >= 0 && dv < W) {
+    if (du >= 0 && du < Wc && dv >= 0 && dv < Wc) {
       const int kx = (du - S + ou) * os + fracu + nx / 2, ky = (dv - S + ov) * os + fracv + ny / 2;
       const float2 c = src[(size_t)ky * nx + kx];
       v = make_float2(c.x * fs, c.y * fs);

@@ -191,6 +191,13 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
         } else if (p.ci.kind == 3) {
           gplane = iw;  // WStackVisGridder::gIndex (WStackVisGridder.cc:339-344); cIndex 0
           plane = 0;
+        } else if (p.ci.kind == 4) {
+          // AProjectWStackVisGridder::initIndices (AProjectWStackVisGridder.cc:125-172): the grid of
+          // the w plane, the CF of (channel, feed, field)
+          gplane = iw;
+          const int feed = in.feed1 ? in.feed1[row] : 0;
+          const int c = p.ci.freqdep ? chan : 0;
+          plane = c + p.ci.nchan_cf * (feed + p.ci.maxfeeds * p.ci.field);
         } else if (p.ci.kind == 2) {
           const int feed = in.feed1 ? in.feed1[row] : 0;
           const int c = p.ci.freqdep ? chan : 0;
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepParams p, PrepInputs in, PrepO
             rec.cfoff = learned ? (int32_t)((long long)s * p.lcf_plane_elems) : in.cf_off[cInd];
             if (learned) lcf = make_float4((float)(in.uvw[3 * row + 2] / kC * f), (float)fracu, (float)fracv, 1.f);
             rec.spack = S | ((2 * S + 1 + 2 * kGuardU) << 16);
-            key = ichan * p.nzg + (p.ci.kind == 3 ? gplane : plane / p.zdiv);
+            key = ichan * p.nzg + (p.ci.kind >= 3 ? gplane : plane / p.zdiv);
             swrow = p.track_per_os ? cInd : plane;
             const float vw = in.visweight ? in.visweight[chan] : 1.0f;
             if (grid_mode) {

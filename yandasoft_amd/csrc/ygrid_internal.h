@@ -89,7 +89,8 @@ class DevBuf {
 // AWProjectVisGridder.cc:123-172,855-863).
 struct CIndexDesc {
   int kind = 0;            // 0: constant 0; 1: w-plane; 2: w-plane + feed/chan (AW); 3: w plane
-                           // selects the grid (w stacking), CF plane 0
+                           // selects the grid (w stacking), CF plane 0; 4: w plane selects the
+                           // grid, CF plane of (chan, feed, field) (A-projection w stacking)
   int nw = 1;              // number of w planes
   double wscale = 1.0;     // linear sampling step (WDependentGridderBase.cc:60-65)
   double wmax = 1.0;
@@ -415,6 +416,8 @@ void noalloc_setup(int device);
 void* noalloc_scratch(int device);
 // fills n 32-bit words with value (a kernel: usable inside captured graphs)
 void launch_fill_u32(const Ctx&, void* p, size_t n, uint32_t value);
+// *flag |= (any of data[0 .. n) is non-zero)
+void launch_any_nonzero(const Ctx& c, const float* data, size_t n, int* flag);
 // Polarisation conversion of visibility cubes (casacore Cube(nRow, nChan, nPol) layout: the
 // sample (row, chan) of polarisation p at index s + nsamp * p).  m: row-major nout x nin.
 struct PolMatrix {
@@ -504,10 +507,15 @@ void launch_aw_screen(const Ctx&, const float2* A, int nx, int ny, int qnx, int 
                       float ascale, const double* tx, const double* ty, const double* two_pi_w, int nb, float2* out);
 // multiply by the output checkerboard of a centred transform (incl. its global sign) and a scale
 void launch_scale_sign(const Ctx&, float2* data, int nx, int ny, int nb, float scale);
+// asym: taps -S .. S-1 only (AProjectWStack)
 void launch_place_cf(const Ctx&, const float2* table, const int32_t* off, const int* supp, const int* ou,
-                     const int* ov, int cnx, int cny, int nb, double2* out);
+                     const int* ov, int cnx, int cny, int nb, double2* out, int asym = 0);
+// max2_tmp (2 * nb uints, or null): each plane's |.|^2 divided by its largest value (AProjectWStack)
 void launch_aw_weights_acc(const Ctx&, const double2* planes, int nb, int cnx, int cny, const double* wt, int ncp,
-                           const double* tx, const double* ty, double* cout);
+                           const double* tx, const double* ty, double* cout, unsigned long long* max2_tmp = nullptr);
+// AProjectWStack's CF before its forward transform: |A ascale|^2 / max, with the forward
+// transform's checkerboard (tmp: one device uint)
+void launch_aws_cf_power(const Ctx&, const float2* A, int nx, int ny, float ascale, float2* buf, uint32_t* tmp);
 void launch_pad_centre(const Ctx&, const double2* in, int cnx, int cny, int nx, int ny, int nb, double2* out);
 void launch_d2c(const Ctx&, const double* in, size_t n, double scale, double2* out);
 void launch_extract_real(const Ctx&, const double2* in, int nxp, int nyp, int nx, int ny, int nb, float* out);
