@@ -231,6 +231,29 @@ int ygrid_get_grid(const ygrid_gridder *g, float *out, size_t *n);
 int ygrid_imaging_chunk(ygrid_gridder *const *degridders, int ndegrid, ygrid_gridder *const *gridders, int ngrid,
                         const ygrid_chunk *chunk);
 
+/* ---- the accessor's coordinate step on the device ----
+ * Replaces IConstDataAccessor::rotatedUVW(tangentPoint) and
+ * ::uvwRotationDelay(tangentPoint, imageCentre) as TableVisGridder::generic calls them
+ * (TableVisGridder.cc:481-495; the accessor is base-accessors', outside the reference).  Each
+ * row's uvw (metres, frame of its phase centre: u east, v north, w toward the centre) is
+ * rotated to the tangent point's frame (uvw_out, nrow x 3) and delay_out (nrow, or NULL) receives
+ * the path difference b.s_centre - w that rephases the row from its phase centre to the image
+ * centre, in the sense ygrid_chunk.delay takes it.  The outputs are ygrid_chunk's uvw and delay.
+ * FP64; mem applies to every pointer; with YGRID_MEM_DEVICE the call is asynchronous on the
+ * context's stream. */
+typedef struct {
+  int nrow;
+  int mem;                     /* enum ygrid_mem */
+  const double *uvw;           /* nrow x 3, row-major (u, v, w) */
+  const double *phase_centre;  /* (lon, lat) radians: 2 values, or nrow x 2 when per_row */
+  int phase_centre_per_row;
+  double tangent_lon, tangent_lat;  /* the image's tangent point (reference direction) */
+  double centre_lon, centre_lat;    /* the image centre */
+  double *uvw_out;             /* nrow x 3 */
+  double *delay_out;           /* nrow, or NULL */
+} ygrid_uvw_rotation;
+int ygrid_uvw_rotate(ygrid_ctx *ctx, const ygrid_uvw_rotation *r);
+
 /* ---- batched passes: no counterpart in the reference, whose imaging equation runs one gridder
  * after the other (ImageFFTEquation.cc:362-387).  For callers that grid or degrid many images per
  * major cycle (beams x channels x polarisations of uneven sizes).

@@ -575,6 +575,33 @@ def direction_lm(beam, centre):
     return l, m
 
 
+def _frame(lon, lat):
+    """Unit vectors (e_u east, e_v north, e_w toward the direction) in equatorial xyz."""
+    sl, cl, sb, cb = math.sin(lon), math.cos(lon), math.sin(lat), math.cos(lat)
+    return (np.array([-sl, cl, 0.0]), np.array([-sb * cl, -sb * sl, cb]), np.array([cb * cl, cb * sl, sb]))
+
+
+def uvw_rotate(uvw, phase_centre, tangent, centre):
+    """rotatedUVW(tangent) and uvwRotationDelay(tangent, centre) of an accessor chunk, as
+    TableVisGridder::generic takes them (TableVisGridder.cc:481-495; the accessor belongs to
+    base-accessors, absent: restated as the change of frame).  uvw (nrow, 3) metres in the frame
+    of each row's phase centre ((2,) or (nrow, 2) lon, lat); returns the uvw in the tangent
+    point's frame and the delay b.s_centre - w (metres) that rephases each row from its phase
+    centre to the image centre."""
+    uvw = np.asarray(uvw, dtype=np.float64)
+    pc = np.asarray(phase_centre, dtype=np.float64).reshape(-1, 2)
+    out = np.empty_like(uvw)
+    delay = np.empty(uvw.shape[0])
+    tu, tv, tw = _frame(*tangent)
+    cw = _frame(*centre)[2]
+    for i in range(uvw.shape[0]):
+        pu, pv, pw = _frame(*pc[i if pc.shape[0] > 1 else 0])
+        b = uvw[i, 0] * pu + uvw[i, 1] * pv + uvw[i, 2] * pw
+        out[i] = (b @ tu, b @ tv, b @ tw)
+        delay[i] = b @ cw - uvw[i, 2]
+    return out, delay
+
+
 def disk_pattern(nu, nv, ucell, vcell, os_, freq, diameter, blockage, l=0.0, m=0.0):
     """DiskIllumination::getPattern (DiskIllumination.cc:85-177): returns [iv, iu] complex."""
     cellU, cellV = ucell / os_, vcell / os_

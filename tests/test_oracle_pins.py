@@ -1140,3 +1140,66 @@ def test_cf_cache_validation_by_parallactic_angle_and_frequency():
     on, chn = _aw_oracle(nofreq, pas=[0.0])
     on._validate_aw(chn)
     assert on.aw_invalid == (False, False) and on.aw_cached_freq is None
+
+
+# ---- rotatedUVW / uvwRotationDelay (oracle.uvw_rotate) ------------------------------------------
+def _uvw_case(seed=12, nrow=300):
+    rng = np.random.default_rng(seed)
+    uvw = rng.normal(size=(nrow, 3)) * np.array([3000.0, 3000.0, 800.0])
+    p = (1.1, -0.6)
+    t = (p[0] + math.radians(2.0), p[1] - math.radians(1.5))     # tangent point 2.5 deg away
+    c = (t[0] + math.radians(0.4), t[1] + math.radians(0.3))     # image centre off the tangent point
+    return uvw, p, t, c
+
+
+def _lmn(d, ref):
+    l, m = O.direction_lm(d, ref)
+    return l, m, math.sqrt(1.0 - l * l - m * m)
+
+
+def test_uvw_rotation_is_a_change_of_frame():
+    """TableVisGridder.cc:481-495: the rotation keeps each baseline's length, is the identity when
+    the tangent point is the phase centre, and composes (p -> t -> q equals p -> q)."""
+    uvw, p, t, c = _uvw_case()
+    rt, _ = O.uvw_rotate(uvw, p, t, c)
+    assert np.allclose(np.linalg.norm(rt, axis=1), np.linalg.norm(uvw, axis=1), rtol=0, atol=1e-9)
+    same, d0 = O.uvw_rotate(uvw, p, p, p)
+    assert np.max(np.abs(same - uvw)) < 1e-9 and np.max(np.abs(d0)) < 1e-9
+    q = (p[0] - 0.3, p[1] + 0.2)
+    a, _ = O.uvw_rotate(rt, t, q, q)
+    b, _ = O.uvw_rotate(uvw, p, q, q)
+    assert np.max(np.abs(a - b)) < 1e-9
+    # per-row phase centres: each row as its own one-row chunk
+    pcs = np.stack([np.full(len(uvw), p[0]), np.full(len(uvw), p[1])], 1)
+    pcs[::2, 0] += 0.01
+    rows, drows = O.uvw_rotate(uvw, pcs, t, c)
+    for i in (0, 1, 7, 8):
+        one, d1 = O.uvw_rotate(uvw[i:i + 1], tuple(pcs[i]), t, c)
+        assert np.allclose(rows[i], one[0], rtol=0, atol=1e-12) and abs(drows[i] - d1[0]) < 1e-12
+
+
+def test_uvw_rotation_delay_rephases_to_the_image_centre():
+    """The gridder multiplies conj(V) by exp(2 pi i f delay / c) (TableVisGridder.cc:652-655).  With
+    visibilities phased to p, V = exp(2 pi i (u l + v m + w (n - 1)) / lambda) for a source at
+    direction cosines (l, m, n) about p (the sign the degridding pins fix), a source at the image
+    centre becomes exactly 1 and a source at d becomes
+    exp(-2 pi i (u' (l_d - l_c) + v' (m_d - m_c) + w' (n_d - n_c)) / lambda) in the rotated uvw
+    (u', v', w') with direction cosines about the tangent point: the delay rephases to the image
+    centre and the rotation carries the tangent point's projection.  Direction cosines here come
+    from the spherical-trigonometry formula, not from the frame vectors."""
+    uvw, p, t, c = _uvw_case()
+    lam = O.C_LIGHT / 1.2e9
+    rt, delay = O.uvw_rotate(uvw, p, t, c)
+    phasor = np.exp(2j * np.pi * delay / lam)
+
+    def vis(d):
+        l, m, n = _lmn(d, p)
+        return np.exp(2j * np.pi * (uvw[:, 0] * l + uvw[:, 1] * m + uvw[:, 2] * (n - 1)) / lam)
+    assert np.max(np.abs(phasor * np.conj(vis(c)) - 1.0)) < 1e-8
+    d = (c[0] - math.radians(0.7), c[1] + math.radians(0.5))
+    lc, mc, nc = _lmn(c, t)
+    ld, md, nd = _lmn(d, t)
+    expect = np.exp(-2j * np.pi * (rt[:, 0] * (ld - lc) + rt[:, 1] * (md - mc) + rt[:, 2] * (nd - nc)) / lam)
+    assert np.max(np.abs(phasor * np.conj(vis(d)) - expect)) < 1e-8
+    # a flipped delay fails at O(1)
+    assert np.max(np.abs(np.exp(-2j * np.pi * delay / lam) * np.conj(vis(c)) - 1.0)) > 0.5

@@ -19,4 +19,5 @@ from .abi import (  # noqa: F401
     imaging_chunk,
     lib,
     library_path,
+    uvw_rotate,
 )

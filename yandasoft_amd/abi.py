@@ -23,7 +23,7 @@ EXPORTED = [
     "ygrid_lcf_create", "ygrid_lcf_destroy", "ygrid_lcf_num_params", "ygrid_lcf_get_params", "ygrid_lcf_set_params",
     "ygrid_lcf_clear_cache",
     "ygrid_lcf_loss_grad", "ygrid_lcf_forward", "ygrid_lcf_train_wproject", "ygrid_gridder_set_learned_cf",
-    "ygrid_imaging_chunk",
+    "ygrid_imaging_chunk", "ygrid_uvw_rotate",
 ]
 
 
@@ -47,6 +47,14 @@ class _Chunk(ctypes.Structure):
                 ("flag", ctypes.c_void_p), ("feed1", ctypes.c_void_p), ("pointing1", ctypes.c_void_p),
                 ("stokes", ctypes.c_void_p), ("dish_pointing1", ctypes.c_void_p), ("vis_out_rows", ctypes.c_int),
                 ("feed1_pa", ctypes.c_void_p)]
+
+
+class _UvwRotation(ctypes.Structure):
+    _fields_ = [("nrow", ctypes.c_int), ("mem", ctypes.c_int), ("uvw", ctypes.c_void_p),
+                ("phase_centre", ctypes.c_void_p), ("phase_centre_per_row", ctypes.c_int),
+                ("tangent_lon", ctypes.c_double), ("tangent_lat", ctypes.c_double),
+                ("centre_lon", ctypes.c_double), ("centre_lat", ctypes.c_double),
+                ("uvw_out", ctypes.c_void_p), ("delay_out", ctypes.c_void_p)]
 
 
 class Stats(ctypes.Structure):
@@ -353,6 +361,27 @@ def imaging_chunk(degridders, gridders, chunk: Chunk):
     G = (ctypes.c_void_p * max(1, len(gridders)))(*[g.h.value for g in gridders])
     c = chunk.c()
     _check(lib().ygrid_imaging_chunk(D, len(degridders), G, len(gridders), ctypes.byref(c)))
+
+
+def uvw_rotate(ctx: Context, uvw, phase_centre, tangent, centre, uvw_out=None, delay_out=None):
+    """rotatedUVW(tangent) and uvwRotationDelay(tangent, centre) of an accessor chunk on the device
+    (ygrid_uvw_rotate).  uvw: (nrow, 3) metres in each row's phase-centre frame; phase_centre:
+    (2,) or (nrow, 2) (lon, lat) radians; tangent, centre: (lon, lat).  numpy arrays (host: the
+    results are returned as new arrays unless given) or device arrays with data_ptr() (device:
+    uvw_out and delay_out must be given; asynchronous on the context's stream)."""
+    device = not isinstance(uvw, np.ndarray)
+    n = int(uvw.shape[0])
+    if not device:
+        uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+        phase_centre = np.ascontiguousarray(phase_centre, dtype=np.float64)
+        uvw_out = np.empty((n, 3)) if uvw_out is None else uvw_out
+        delay_out = np.empty(n) if delay_out is None else delay_out
+    per_row = 1 if (len(phase_centre.shape) == 2 and phase_centre.shape[0] == n and n != 1) else 0
+    r = _UvwRotation(n, YGRID_MEM_DEVICE if device else YGRID_MEM_HOST, _ptr(uvw), _ptr(phase_centre), per_row,
+                     float(tangent[0]), float(tangent[1]), float(centre[0]), float(centre[1]), _ptr(uvw_out),
+                     _ptr(delay_out))
+    _check(lib().ygrid_uvw_rotate(ctx.h, ctypes.byref(r)))
+    return uvw_out, delay_out
 
 
 class Batch:

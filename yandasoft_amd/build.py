@@ -27,6 +27,7 @@ SOURCES = [
     "kernels_cf.hip",
     "kernels_awp.hip",
     "kernels_mlp.hip",
+    "kernels_uvw.hip",
     "spheroidal.cpp",
     "gridders.cpp",
     "awproject.cpp",

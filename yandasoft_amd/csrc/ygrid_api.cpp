@@ -306,6 +306,39 @@ int ygrid_imaging_chunk(ygrid_gridder* const* degridders, int ndegrid, ygrid_gri
   });
 }
 
+int ygrid_uvw_rotate(ygrid_ctx* ctx, const ygrid_uvw_rotation* r) {
+  return guard([&] {
+    NEED(ctx);
+    NEED(r);
+    YG_CHECK(r->nrow >= 0, YGRID_ERR_ARG, "negative row count");
+    if (r->nrow == 0) return;
+    NEED(r->uvw);
+    NEED(r->phase_centre);
+    NEED(r->uvw_out);
+    YG_HIP(hipSetDevice(ctx->c->device));
+    const ygrid::Ctx& c = *ctx->c;
+    const ygrid::UvwTargets T = ygrid::uvw_targets(r->tangent_lon, r->tangent_lat, r->centre_lon, r->centre_lat);
+    const size_t n = (size_t)r->nrow;
+    const size_t npc = r->phase_centre_per_row ? 2 * n : 2;
+    if (r->mem == YGRID_MEM_DEVICE) {
+      ygrid::launch_uvw_rotate(c, r->uvw, r->phase_centre, r->phase_centre_per_row ? 1 : 0, T, r->nrow, r->uvw_out,
+                               r->delay_out);
+      return;
+    }
+    YG_CHECK(r->mem == YGRID_MEM_HOST, YGRID_ERR_ARG, "mem must be YGRID_MEM_HOST or YGRID_MEM_DEVICE");
+    ygrid::DevBuf in(n * 3 * sizeof(double)), pc(npc * sizeof(double)), out(n * 3 * sizeof(double)),
+        dl(r->delay_out ? n * sizeof(double) : 0);
+    YG_HIP(hipMemcpyAsync(in.get(), r->uvw, n * 3 * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    YG_HIP(hipMemcpyAsync(pc.get(), r->phase_centre, npc * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    ygrid::launch_uvw_rotate(c, in.as<double>(), pc.as<double>(), r->phase_centre_per_row ? 1 : 0, T, r->nrow,
+                             out.as<double>(), r->delay_out ? dl.as<double>() : nullptr);
+    YG_HIP(hipMemcpyAsync(r->uvw_out, out.get(), n * 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (r->delay_out)
+      YG_HIP(hipMemcpyAsync(r->delay_out, dl.get(), n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    YG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
 int ygrid_batch_create(ygrid_ctx* ctx, ygrid_batch** out) {
   return guard([&] {
     NEED(ctx);

@@ -435,6 +435,17 @@ void launch_pol_to_vis_add(const Ctx&, const PolMatrix& M, const float2* img, si
                            int ld);
 // r[i] = data[i] - r[i] over n complex values (the residual of a model cube r).
 void launch_residual(const Ctx&, const float2* data, float2* r, size_t n);
+// rotatedUVW / uvwRotationDelay on the device (kernels_uvw.hip): the tangent point's frame
+// vectors (equatorial xyz) and the image centre's direction.
+struct UvwTargets {
+  double tu[3], tv[3], tw[3], cw[3];
+};
+UvwTargets uvw_targets(double tangent_lon, double tangent_lat, double centre_lon, double centre_lat);
+// out (nrow x 3) = uvw rotated from each row's phase centre (pc: 2 doubles, or 2 per row) to the
+// tangent point; delay (nrow, or null) = the path difference rephasing from the phase centre to
+// the image centre.
+void launch_uvw_rotate(const Ctx&, const double* uvw, const double* pc, int pc_per_row, const UvwTargets& T,
+                       int nrow, double* out, double* delay);
 // Rounds the FP32 CF table (n complex taps) to prec (FP16 or BF16), packed (re, im) in 32 bits.
 void launch_cf_round(const Ctx&, const float2* src, uint32_t* dst, size_t n, int prec);
 // vis_out: a cube of ld >= nrow rows
