@@ -130,6 +130,9 @@ typedef struct {
    * 272-282): calls of initConvolutionFunction, feed/field CF sets generated, and those of them
    * generated after the parallactic angle or the frequencies invalidated the cache */
   double cf_iterations, cf_generations, cf_generations_pa, cf_generations_freq;
+  /* snapshot imaging (gridder.snapshotimaging): w-plane fits made, image-plane regrids done, and
+   * the current plane w = snapshot_a u + snapshot_b v; zero for other gridders */
+  double snapshot_fits, snapshot_regrids, snapshot_a, snapshot_b;
 } ygrid_stats;
 
 /* ---- library / context ---- */
@@ -166,7 +169,13 @@ int ygrid_gridder_create(ygrid_ctx *ctx, const char *name, const char *parset, y
  * (TableVisGridder::useAllDataForPSF), oversampleweight (trackWeightPerPlane), cleargrids
  * (doClearGrid), MaxPointingSeparation (maxPointingSeparation: radians, or a quantity such as
  * "1.5deg"; negative: off), parotation (doPARotation: only false is supported).  Call before
- * initialise_grid / initialise_degrid. */
+ * initialise_grid / initialise_degrid.
+ * snapshotimaging=true wraps the gridder in SnapShotImagingGridderAdapter's behaviour
+ * (VisGridderFactory.cc:248-270, SnapShotImagingGridderAdapter.cc): snapshotimaging.wtolerance
+ * (metres, required), .clipping, .weightsclipping, .reprojectpsf, .interpmethod (cubic | linear),
+ * .coorddecimation (accepted; coordinates are computed for every pixel); .longtrack is refused.
+ * Every call on the handle then goes through the wrapper; ygrid_stats reports its fits and
+ * regrids.  Not for ygrid_imaging_chunk, batches or ygrid_degrid_noalloc. */
 int ygrid_gridder_configure(ygrid_gridder *g, const char *options);
 /* IVisGridder::clone (IVisGridder.h:75; TableVisGridder.cc:126-168) */
 int ygrid_gridder_clone(const ygrid_gridder *g, ygrid_gridder **out);

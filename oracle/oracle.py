@@ -986,6 +986,208 @@ class OracleWStack:
         return out
 
 
+# ---- snapshot imaging (SnapShotImagingGridderAdapter.cc) ---------------------------------------
+# The w-plane fit belongs to base-accessors' BestWPlaneDataAccessor and the image-plane regrid to
+# casacore's ImageRegrid / DirectionCoordinate, all absent.  Restated: the plane w = A u + B v by
+# least squares over the chunk's rows, refitted when a row departs from the current plane by more
+# than the tolerance (the snapshot w-projection method, Cornwell, Voronkov & Humphreys 2012); the
+# snapshot's frame is the SIN projection with the slant parameters (xi, eta) = (-A, -B)
+# (SnapShotImagingGridderAdapter.cc:450-470; Calabretta & Greisen 2002 eq. 55-60); images move
+# between frames by bilinear or Keys (a = -1/2) cubic interpolation with every pixel's coordinates
+# computed exactly (the reference interpolates coordinates computed on a grid decimated by
+# coorddecimation).  Pinned by closed forms (tests/test_oracle_pins.py); casacore's cubic variant
+# and its edge handling are parity unpinned.
+def wplane_fit(uvw):
+    """Least-squares plane w = A u + B v through the rows' uvw; (0, 0) when u, v are degenerate."""
+    u, v, w = (np.asarray(uvw, np.float64)[:, k] for k in range(3))
+    suu, suv, svv, suw, svw = u @ u, u @ v, v @ v, u @ w, v @ w
+    det = suu * svv - suv * suv
+    if not det > 0:
+        return 0.0, 0.0
+    return (suw * svv - svw * suv) / det, (svw * suu - suw * suv) / det
+
+
+def slant_forward(l, m, A, B):
+    """Direction cosines (l, m) to the snapshot frame's projection-plane coordinates: with
+    w' = w - A u - B v a source at (l, m) images at (l - A (1 - n), m - B (1 - n))."""
+    n = np.sqrt(np.maximum(0.0, 1.0 - l * l - m * m))
+    return l - A * (1.0 - n), m - B * (1.0 - n)
+
+
+def slant_inverse(x, y, A, B):
+    """Inverse of slant_forward: n solves (x - xi + xi n)^2 + (y - eta + eta n)^2 + n^2 = 1 with
+    (xi, eta) = (-A, -B), the root near 1 (Calabretta & Greisen 2002 eq. 59)."""
+    xi, eta = -A, -B
+    X, Y = x - xi, y - eta
+    a = xi * xi + eta * eta + 1.0
+    b = X * xi + Y * eta
+    c = X * X + Y * Y - 1.0
+    n = (-b + np.sqrt(np.maximum(0.0, b * b - a * c))) / a
+    return x - xi * (1.0 - n), y - eta * (1.0 - n)
+
+
+def _keys(t):
+    t = np.abs(t)
+    return np.where(t <= 1, (1.5 * t - 2.5) * t * t + 1, np.where(t < 2, ((-0.5 * t + 2.5) * t - 4) * t + 2, 0.0))
+
+
+def interp2(img, px, py, method="cubic"):
+    """img [.., ny, nx] sampled at pixel coordinates (px, py) (arrays of one shape); 0 where the
+    interpolation stencil leaves the image."""
+    ny, nx = img.shape[-2:]
+    x0, y0 = np.floor(px).astype(np.int64), np.floor(py).astype(np.int64)
+    fx, fy = px - x0, py - y0
+    if method == "linear":
+        offs, wx, wy = (0, 1), [1 - fx, fx], [1 - fy, fy]
+    elif method == "cubic":
+        offs = (-1, 0, 1, 2)
+        wx = [_keys(fx - o) for o in offs]
+        wy = [_keys(fy - o) for o in offs]
+    else:
+        raise ValueError("interpolation method " + method)
+    ok = (x0 + offs[0] >= 0) & (x0 + offs[-1] <= nx - 1) & (y0 + offs[0] >= 0) & (y0 + offs[-1] <= ny - 1)
+    xs = np.clip(x0, -offs[0], nx - 1 - offs[-1])
+    ys = np.clip(y0, -offs[0], ny - 1 - offs[-1])
+    out = 0.0
+    for j, oy in enumerate(offs):
+        for i, ox in enumerate(offs):
+            out = out + (wy[j] * wx[i]) * img[..., ys + oy, xs + ox]
+    return np.where(ok, out, 0.0)
+
+
+def snapshot_regrid(src, A, B, cellx, celly, to_target, method="cubic"):
+    """SnapShotImagingGridderAdapter::imageRegrid (:488-562): src [.., ny, nx] from the snapshot
+    frame to the target frame (to_target) or back; pixel (x, y) lies at
+    ((x - nx/2) cellx, (y - ny/2) celly) in its frame's projection plane."""
+    ny, nx = src.shape[-2:]
+    X = (np.arange(nx) - nx // 2) * cellx
+    Y = (np.arange(ny) - ny // 2) * celly
+    l, m = np.meshgrid(X, Y)
+    sl, sm = slant_forward(l, m, A, B) if to_target else slant_inverse(l, m, A, B)
+    return interp2(np.asarray(src, np.float64), sl / cellx + nx // 2, sm / celly + ny // 2, method)
+
+
+def snapshot_clip(img, factor):
+    """imageClip (:640-652): zero all but the centred int(n (1 - factor)) pixels of each axis."""
+    if factor == 0:
+        return img
+    ny, nx = img.shape[-2:]
+    kx, ky = int(nx * (1.0 - factor)), int(ny * (1.0 - factor))
+    out = np.zeros_like(img)
+    ox, oy = (nx - kx) // 2, (ny - ky) // 2
+    out[..., oy:oy + ky, ox:ox + kx] = img[..., oy:oy + ky, ox:ox + kx]
+    return out
+
+
+class OracleSnapshot:
+    """SnapShotImagingGridderAdapter around an oracle gridder (``make()`` builds it): every chunk's
+    w is taken relative to the current fitted plane; when a chunk leaves the plane (a row deviates
+    by more than wtolerance) the snapshot gridded so far is finalised, regridded to the target
+    frame and added to the image and weight buffers (:222-258,416-443), and gridding restarts
+    in the new plane's frame; degridding regrids the model into each plane's frame (:347-395).
+    The PSF and preconditioner passes grid the plane-relative w without regridding unless
+    reprojectpsf (:195-211)."""
+
+    def __init__(self, make, wtolerance, interp="cubic", clipping=0.0, weightsclipping=0.0, reprojectpsf=False):
+        self.inner = make()
+        self.tol, self.method = wtolerance, interp
+        self.clip, self.wclip, self.reproject = clipping, weightsclipping, reprojectpsf
+        self.A = self.B = 0.0
+        self.fits = self.regrids = 0
+
+    def _update_plane(self, chunk):
+        uvw = np.asarray(chunk.uvw, np.float64)
+        dev = np.max(np.abs(uvw[:, 2] - self.A * uvw[:, 0] - self.B * uvw[:, 1])) if len(uvw) else 0.0
+        if dev > self.tol:
+            self.fits += 1
+            return True, wplane_fit(uvw)
+        return False, (self.A, self.B)
+
+    def _relative(self, chunk):
+        uvw = np.array(chunk.uvw, np.float64)
+        uvw[:, 2] -= self.A * uvw[:, 0] + self.B * uvw[:, 1]
+        return _dc_replace(chunk, uvw=uvw)
+
+    def _passthrough(self):
+        return (self.dopsf or self.dopcf) and not self.reproject
+
+    def initialise_grid(self, shape, cell, dopsf=False, dopcf=False):
+        self.shape, self.cell, self.dopsf, self.dopcf = shape, cell, dopsf, dopcf
+        if self._passthrough():
+            self.inner.initialise_grid(shape, cell, dopsf=dopsf, dopcf=dopcf)
+            return
+        nx, ny, npol, nchan = shape
+        self.image = np.zeros((nchan, npol, ny, nx))
+        self.weights = np.zeros((nchan, npol, ny, nx))
+        self.first, self.done = True, True
+
+    def _finalise_current(self):
+        img = self.inner.finalise_grid()
+        self.image = snapshot_clip(self.image + snapshot_regrid(img, self.A, self.B, self.cell[0], self.cell[1], True,
+                                                                self.method), self.clip)
+        self.regrids += 1
+        if not self.dopcf:
+            w = self.inner.finalise_weights()
+            self.weights = snapshot_clip(self.weights + snapshot_regrid(w, self.A, self.B, self.cell[0], self.cell[1],
+                                                                        True, self.method),
+                                         self.wclip if self.wclip != 0 else self.clip)
+            self.regrids += 1
+        self.done = True
+
+    def grid(self, chunk):
+        changed, ab = self._update_plane(chunk)
+        if self._passthrough():
+            self.A, self.B = ab
+            self.inner.grid(self._relative(chunk))
+            return
+        if changed or self.first:
+            if not self.first:
+                self._finalise_current()
+                self.first = True
+            self.A, self.B = ab
+        if self.first:
+            self.inner.initialise_grid(self.shape, self.cell, dopsf=self.dopsf, dopcf=self.dopcf)
+            self.first = False
+        self.inner.grid(self._relative(chunk))
+        self.done = False
+
+    def finalise_grid(self):
+        if self._passthrough():
+            return self.inner.finalise_grid()
+        if not self.done:
+            self._finalise_current()
+        return self.image.copy()
+
+    def finalise_weights(self):
+        if self._passthrough():
+            return self.inner.finalise_weights()
+        if not self.done:
+            self._finalise_current()
+        return self.weights.copy()
+
+    def initialise_degrid(self, shape, cell, model):
+        self.shape, self.cell = shape, cell
+        self.model_empty = not np.max(np.abs(model)) > 0
+        self.model = np.asarray(model, np.float64)
+        self.first = True
+
+    def degrid(self, chunk):
+        nrow, nchan, npol = chunk.vis.shape
+        if self.model_empty:
+            return np.zeros((nrow, nchan, npol), np.complex64)
+        changed, ab = self._update_plane(chunk)
+        if changed or self.first:
+            self.first = True
+            self.A, self.B = ab
+        if self.first:
+            m = snapshot_clip(snapshot_regrid(self.model, self.A, self.B, self.cell[0], self.cell[1], False,
+                                              self.method), self.clip)
+            self.regrids += 1
+            self.inner.initialise_degrid(self.shape, self.cell, m)
+            self.first = False
+        return self.inner.degrid(self._relative(chunk))
+
+
 # ---- polarisation conversion (PolConverter as used at TableVisGridder.cc:517-524,682-693,860) ----
 # PolConverter belongs to base-scimath (absent).  Restated from the casacore / IAU definitions of
 # the polarisation products (casacore Stokes codes I=1 Q=2 U=3 V=4 RR=5 RL=6 LR=7 LL=8 XX=9 XY=10

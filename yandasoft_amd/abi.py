@@ -72,7 +72,9 @@ class Stats(ctypes.Structure):
                 ("degrid_matrix", ctypes.c_int), ("rows_rejected_pointing", ctypes.c_double),
                 ("small_support_passes", ctypes.c_double),
                 ("cf_iterations", ctypes.c_double), ("cf_generations", ctypes.c_double),
-                ("cf_generations_pa", ctypes.c_double), ("cf_generations_freq", ctypes.c_double)]
+                ("cf_generations_pa", ctypes.c_double), ("cf_generations_freq", ctypes.c_double),
+                ("snapshot_fits", ctypes.c_double), ("snapshot_regrids", ctypes.c_double),
+                ("snapshot_a", ctypes.c_double), ("snapshot_b", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):

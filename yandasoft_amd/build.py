@@ -28,6 +28,7 @@ SOURCES = [
     "kernels_awp.hip",
     "kernels_mlp.hip",
     "kernels_uvw.hip",
+    "kernels_snapshot.hip",
     "spheroidal.cpp",
     "gridders.cpp",
     "awproject.cpp",
@@ -35,6 +36,7 @@ SOURCES = [
     "learned_cf.cpp",
     "imaging.cpp",
     "polconv.cpp",
+    "snapshot.cpp",
     "noalloc.cpp",
     "ygrid_api.cpp",
 ]

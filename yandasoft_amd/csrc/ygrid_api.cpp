@@ -11,12 +11,14 @@
 
 #include "batch.h"
 #include "gridders.h"
+#include "snapshot.h"
 
 struct ygrid_ctx {
   std::shared_ptr<ygrid::Ctx> c;
 };
 struct ygrid_gridder {
   std::unique_ptr<ygrid::Gridder> g;
+  std::unique_ptr<ygrid::Snapshot> s;  // snapshot imaging around g (gridder.snapshotimaging)
 };
 struct ygrid_batch {
   std::unique_ptr<ygrid::Batch> b;
@@ -140,8 +142,11 @@ int ygrid_gridder_create(ygrid_ctx* ctx, const char* name, const char* parset, y
     NEED(name);
     NEED(out);
     YG_HIP(hipSetDevice(ctx->c->device));
-    auto g = ygrid::make_gridder(ctx->c, name, ygrid::parse_parset(parset));
-    *out = new ygrid_gridder{std::move(g)};
+    const ygrid::ParamMap m = ygrid::parse_parset(parset);
+    auto g = ygrid::make_gridder(ctx->c, name, m);
+    auto h = std::unique_ptr<ygrid_gridder>(new ygrid_gridder{std::move(g)});
+    if (ygrid::Params{m}.flag("snapshotimaging", false)) h->s.reset(new ygrid::Snapshot(h->g.get(), ygrid::Params{m}));
+    *out = h.release();
   });
 }
 
@@ -151,7 +156,11 @@ int ygrid_gridder_configure(ygrid_gridder* g, const char* options) {
     const ygrid::ParamMap m = ygrid::parse_parset(options);
     for (const auto& kv : m) {
       static const char* known[] = {"padding", "alldatapsf", "oversampleweight", "cleargrids", "MaxPointingSeparation",
-                                    "parotation", "parotation.angle", "swappols"};
+                                    "parotation", "parotation.angle", "swappols", "snapshotimaging",
+                                    "snapshotimaging.wtolerance", "snapshotimaging.clipping",
+                                    "snapshotimaging.weightsclipping", "snapshotimaging.reprojectpsf",
+                                    "snapshotimaging.interpmethod", "snapshotimaging.coorddecimation",
+                                    "snapshotimaging.longtrack"};
       bool ok = false;
       for (const char* k : known) ok = ok || kv.first == k;
       YG_CHECK(ok, YGRID_ERR_ARG, "ygrid_gridder_configure: unknown option " + kv.first);
@@ -159,6 +168,15 @@ int ygrid_gridder_configure(ygrid_gridder* g, const char* options) {
     YG_CHECK(!ygrid::Params{m}.flag("swappols", false), YGRID_ERR_ARG,
              "polarisation swapping (gridder.swappols) needs parallactic angle rotation, which the GPU gridders do not do");
     g->g->configure_globals(ygrid::Params{m});
+    // VisGridderFactory.cc:248-270: gridder.snapshotimaging wraps the gridder in the snapshot
+    // imaging adapter (once; later configurations leave an existing wrapper as it is)
+    const ygrid::Params p{m};
+    if (p.has("snapshotimaging")) {
+      if (!p.flag("snapshotimaging", false))
+        g->s.reset();
+      else if (!g->s)
+        g->s.reset(new ygrid::Snapshot(g->g.get(), p));
+    }
   });
 }
 
@@ -166,7 +184,9 @@ int ygrid_gridder_clone(const ygrid_gridder* g, ygrid_gridder** out) {
   return guard([&] {
     NEED(g);
     NEED(out);
-    *out = new ygrid_gridder{g->g->clone()};
+    auto c = new ygrid_gridder{g->g->clone()};
+    if (g->s) c->s.reset(new ygrid::Snapshot(*g->s, c->g.get()));
+    *out = c;
   });
 }
 
@@ -178,7 +198,8 @@ int ygrid_initialise_grid(ygrid_gridder* g, const ygrid_image_desc* img, int dop
   return guard([&] {
     NEED(g);
     NEED(img);
-    g->g->initialiseGrid(*img, dopsf != 0, dopcf != 0);
+    if (g->s) g->s->initialiseGrid(*img, dopsf != 0, dopcf != 0);
+    else g->g->initialiseGrid(*img, dopsf != 0, dopcf != 0);
   });
 }
 
@@ -186,7 +207,8 @@ int ygrid_grid(ygrid_gridder* g, const ygrid_chunk* acc) {
   return guard([&] {
     NEED(g);
     NEED(acc);
-    g->g->grid(*acc);
+    if (g->s) g->s->grid(*acc);
+    else g->g->grid(*acc);
   });
 }
 
@@ -194,7 +216,8 @@ int ygrid_finalise_grid(ygrid_gridder* g, float* out, int mem) {
   return guard([&] {
     NEED(g);
     NEED(out);
-    g->g->finaliseGrid(out, mem);
+    if (g->s) g->s->finaliseGrid(out, mem);
+    else g->g->finaliseGrid(out, mem);
   });
 }
 
@@ -202,7 +225,8 @@ int ygrid_finalise_weights(ygrid_gridder* g, float* out, int mem) {
   return guard([&] {
     NEED(g);
     NEED(out);
-    g->g->finaliseWeights(out, mem);
+    if (g->s) g->s->finaliseWeights(out, mem);
+    else g->g->finaliseWeights(out, mem);
   });
 }
 
@@ -211,7 +235,8 @@ int ygrid_initialise_degrid(ygrid_gridder* g, const ygrid_image_desc* img, const
     NEED(g);
     NEED(img);
     NEED(model);
-    g->g->initialiseDegrid(*img, model, mem);
+    if (g->s) g->s->initialiseDegrid(*img, model, mem);
+    else g->g->initialiseDegrid(*img, model, mem);
   });
 }
 
@@ -219,7 +244,8 @@ int ygrid_degrid(ygrid_gridder* g, const ygrid_chunk* acc) {
   return guard([&] {
     NEED(g);
     NEED(acc);
-    g->g->degrid(*acc);
+    if (g->s) g->s->degrid(*acc);
+    else g->g->degrid(*acc);
   });
 }
 
@@ -227,6 +253,7 @@ int ygrid_degrid_noalloc(ygrid_gridder* g, const ygrid_chunk* acc) {
   return guard([&] {
     NEED(g);
     NEED(acc);
+    YG_CHECK(!g->s, YGRID_ERR_ARG, "ygrid_degrid_noalloc: not for snapshot imaging gridders");
     const int dev = g->g->context()->device;
     YG_HIP(hipSetDevice(dev));
     g->g->degridNoAlloc(*acc, ygrid::noalloc_scratch(dev), ygrid::kNoAllocScratchBytes);
@@ -236,7 +263,8 @@ int ygrid_degrid_noalloc(ygrid_gridder* g, const ygrid_chunk* acc) {
 int ygrid_finalise_degrid(ygrid_gridder* g) {
   return guard([&] {
     NEED(g);
-    g->g->finaliseDegrid();
+    if (g->s) g->s->finaliseDegrid();
+    else g->g->finaliseDegrid();
   });
 }
 
@@ -244,7 +272,7 @@ int ygrid_is_model_empty(const ygrid_gridder* g, int* out) {
   return guard([&] {
     NEED(g);
     NEED(out);
-    *out = g->g->isModelEmpty() ? 1 : 0;
+    *out = (g->s ? g->s->isModelEmpty() : g->g->isModelEmpty()) ? 1 : 0;
   });
 }
 
@@ -262,6 +290,7 @@ int ygrid_get_stats(const ygrid_gridder* g, ygrid_stats* out) {
     NEED(out);
     std::memset(out, 0, sizeof(*out));
     g->g->stats(out);
+    if (g->s) g->s->stats(out);
   });
 }
 
@@ -296,10 +325,12 @@ int ygrid_imaging_chunk(ygrid_gridder* const* degridders, int ndegrid, ygrid_gri
     std::vector<ygrid::Gridder*> d, g;
     for (int i = 0; i < ndegrid; ++i) {
       NEED(degridders[i]);
+      YG_CHECK(!degridders[i]->s, YGRID_ERR_ARG, "ygrid_imaging_chunk: not for snapshot imaging gridders");
       d.push_back(degridders[i]->g.get());
     }
     for (int i = 0; i < ngrid; ++i) {
       NEED(gridders[i]);
+      YG_CHECK(!gridders[i]->s, YGRID_ERR_ARG, "ygrid_imaging_chunk: not for snapshot imaging gridders");
       g.push_back(gridders[i]->g.get());
     }
     ygrid::imaging_chunk(d, g, *chunk);
@@ -357,6 +388,7 @@ int ygrid_batch_add(ygrid_batch* b, ygrid_gridder* g, const ygrid_chunk* acc) {
     NEED(b);
     NEED(g);
     NEED(acc);
+    YG_CHECK(!g->s, YGRID_ERR_ARG, "ygrid_batch_add: not for snapshot imaging gridders");
     b->b->add(g->g.get(), *acc);
   });
 }

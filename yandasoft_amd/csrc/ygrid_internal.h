@@ -446,6 +446,21 @@ UvwTargets uvw_targets(double tangent_lon, double tangent_lat, double centre_lon
 // the image centre.
 void launch_uvw_rotate(const Ctx&, const double* uvw, const double* pc, int pc_per_row, const UvwTargets& T,
                        int nrow, double* out, double* delay);
+// Snapshot imaging (kernels_snapshot.hip).  sums6: u.u, u.v, v.v, u.w, v.w and (as the bits of a
+// non-negative double) the largest |w - A u - B v| over the rows.
+void launch_wplane_sums(const Ctx&, const double* uvw, int nrow, double A, double B, double* sums6);
+// out = uvw with w - A u - B v (nrow x 3)
+void launch_wplane_apply(const Ctx&, const double* uvw, int nrow, double A, double B, double* out);
+struct SnapshotRegridArgs {
+  double A = 0, B = 0;        // the snapshot's plane w = A u + B v
+  double cellx = 0, celly = 0;
+  int to_target = 1;          // 1: snapshot frame -> target frame, 0: target -> snapshot
+  int cubic = 1;              // 1: Keys cubic convolution, 0: bilinear
+  int add = 0;                // 1: dst += regridded, 0: dst = regridded
+  int clip_x0 = 0, clip_y0 = 0, clip_kx = 0, clip_ky = 0;  // pixels kept (the rest zeroed)
+};
+void launch_snapshot_regrid(const Ctx&, const float* src, float* dst, int nx, int ny, int nplanes,
+                            const SnapshotRegridArgs& a);
 // Rounds the FP32 CF table (n complex taps) to prec (FP16 or BF16), packed (re, im) in 32 bits.
 void launch_cf_round(const Ctx&, const float2* src, uint32_t* dst, size_t n, int prec);
 // vis_out: a cube of ld >= nrow rows
