@@ -760,13 +760,21 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_cf_rsrc(const CFRef& t) {
 
 // The CF resource of a work item of key group kz: the whole table, or (tables of 2 GiB or more)
 // a resource based at the group's first plane that the descriptors' offsets are relative to.
+// The group base is read through a generic pointer, a load the compiler takes as per-lane: the
+// resource is rebuilt from readfirstlane'd words so that it stays in scalar registers (a
+// per-lane resource makes every buffer load a waterfall loop and blocks their merging into
+// 16-byte loads).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_item_rsrc(const TileJob& J, int kz) {
   const CFRef& t = J.cf;
   if (!t.zg_base) return make_cf_rsrc(t);
   const unsigned long long b = (unsigned long long)t.zg_base[kz % J.nzg] * t.esz;
   const unsigned long long rec = min(t.total - b + t.pad, kCFResourceMax);
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(reinterpret_cast<const char*>(t.ptr)) + b - t.pad,
-                                           (short)0, (int)rec, kBufferWord3);
+  const unsigned long long base = reinterpret_cast<unsigned long long>(t.ptr) + b - t.pad;
+  const unsigned long long ub =
+      ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(base >> 32)) << 32) |
+      (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
+  const int ur = __builtin_amdgcn_readfirstlane((int)rec);
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<char*>(ub), (short)0, ur, kBufferWord3);
 }
 
 // Work queues of the persistent tile kernels: the key-ordered items (w-plane-major) split into
