@@ -738,6 +738,8 @@ def bench_sphfunc(args):
     gg = ya.Gridder(ctx, "GPUSphFunc", wl.params)
     dg = ya.Gridder(ctx, "GPUSphFunc", wl.params)
     dg.initialise_degrid(img, t(model))
+    # the small-support gridding kernel's LDS band height, tuned on this chunk (kept by the context)
+    band_rows, band_ms = gg.autotune_small(img, chunk, trials=3)
 
     def step():
         ctx.clear_cache()
@@ -779,6 +781,7 @@ def bench_sphfunc(args):
                                "prep": s1g.ms_prep / max(1.0, s1g.n_grid_launches),
                                "binning": s1g.ms_bin / max(1.0, s1g.n_grid_launches)},
         "small_support_kernels": small,
+        "small_band_rows": {"chosen": band_rows, "tuning_pass_ms": band_ms},
         "roofline": {"bound": "hbm", "kernel": gname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": PROFILE_SPH,

@@ -156,6 +156,18 @@ int ygrid_ctx_clear_cache(ygrid_ctx *ctx);
  * results either way (within FP32 accumulation order); performance tuning with no counterpart
  * in the reference. */
 int ygrid_ctx_set_small_support(ygrid_ctx *ctx, int max_support);
+/* Rows (4, 8 or 16; default 8) of the LDS band a workgroup of the small-support gridding kernel
+ * owns in its 64-pixel-wide tile; raised to the rows one instruction covers (8 for supports up to
+ * 3).  Same results to FP32 accumulation order.  No counterpart in the reference. */
+int ygrid_ctx_set_small_band_rows(ygrid_ctx *ctx, int rows);
+/* Autotuning of that LDS tile shape for one gridder and chunk: a clone of the gridder, set up for
+ * img, grids the chunk `trials` times at each band height; the fastest (host clock around the
+ * synchronised pass) becomes the context's setting and is returned with its time.  The gridder
+ * itself is not changed.  A pass whose support is above the small-support limit runs the tile
+ * kernels, on which the setting has no effect (*best_rows is then the unchanged setting).  No
+ * counterpart in the reference. */
+int ygrid_gridder_autotune_small(ygrid_gridder *g, const ygrid_image_desc *img, const ygrid_chunk *acc, int trials,
+                                 int *best_rows, double *best_ms);
 
 /* ---- factory: VisGridderFactory::make / createGridder (VisGridderFactory.cc:112-286) ----
  * name: "GPUBox", "GPUSphFunc", "GPUWProject", "GPUWStack", "GPUAWProject" or "GPUAProjectWStack"

@@ -143,3 +143,55 @@ def test_small_support_limit_is_checked(gpu_ctx):
         gpu_ctx.set_small_support(9)
     with pytest.raises(ya.YgridError):
         gpu_ctx.set_small_support(-1)
+
+
+@pytest.mark.parametrize("rows", [4, 8, 16])
+@pytest.mark.parametrize("case", [CASES[2], CASES[4], CASES[5]], ids=["sphfunc S=3 4 pols", "wproject S<=5", "wproject S<=8"])
+def test_small_support_band_rows_match_oracle(gpu_ctx, case, rows):
+    """Every LDS band height of the small-support gridding kernel (ygrid_ctx_set_small_band_rows;
+    4 is raised to 8 for supports up to 3) grids as the oracle does (1e-5) and as the default
+    height (1e-6: FP32 accumulation order)."""
+    label, name, params, okw, N, cell_as, npol, nchan, opt = case
+    announce(f"small-support band rows {rows}: {label}")
+    cell = cell_as * ARC
+    kw = dict(opt)
+    if kw.pop("w", False):
+        kw.update(f0=1.0e9, frac=0.35, wmax_m=2500.0 * O.C_LIGHT / 1.005e9)
+    d = make_data(2500, nchan, npol, N, cell, seed=90 + npol, **kw)
+    model = random_model(N, npol, seed=3)
+    try:
+        gpu_ctx.set_small_band_rows(rows)
+        g, _, st, _ = _run(gpu_ctx, 8, name, params, d, N, cell, npol, model)
+    finally:
+        gpu_ctx.set_small_band_rows(8)
+    g8, _, _, _ = _run(gpu_ctx, 8, name, params, d, N, cell, npol, model)
+    assert st.small_support_passes == 1
+    og, _, _ = _oracle(okw, d, N, cell, npol, model)
+    assert rel(g.reshape(og.shape), og) < TOL
+    assert rel(g, g8) < 1e-6
+
+
+def test_small_support_band_rows_autotune(gpu_ctx):
+    """ygrid_gridder_autotune_small times the three band heights on a clone and keeps the fastest
+    in the context; the gridder's own grid is untouched and a later pass still matches the oracle."""
+    import yandasoft_amd as ya
+    label, name, params, okw, N, cell_as, npol, nchan, opt = CASES[2]
+    announce("small-support band rows autotune")
+    cell = cell_as * ARC
+    d = make_data(2500, nchan, npol, N, cell, seed=97, **{k: v for k, v in opt.items() if k != "w"})
+    img = ya.ImageDesc(N, N, npol, 1, cell, cell)
+    g = ya.Gridder(gpu_ctx, name, params)
+    g.initialise_grid(img)
+    try:
+        rows, ms = g.autotune_small(img, pchunk(d), trials=2)
+        assert rows in (4, 8, 16) and ms > 0
+        assert not np.any(g.grid_array())
+        g.grid(pchunk(d))
+        o = O.OracleGridder(precision="f32", **okw)
+        o.initialise_grid((N, N, npol, 1), (cell, cell))
+        o.grid(ochunk(d))
+        assert rel(g.grid_array().reshape(o.grid_.shape), o.grid_) < TOL
+        with pytest.raises(ya.YgridError):
+            gpu_ctx.set_small_band_rows(12)
+    finally:
+        gpu_ctx.set_small_band_rows(8)

@@ -23,7 +23,7 @@ EXPORTED = [
     "ygrid_lcf_create", "ygrid_lcf_destroy", "ygrid_lcf_num_params", "ygrid_lcf_get_params", "ygrid_lcf_set_params",
     "ygrid_lcf_clear_cache",
     "ygrid_lcf_loss_grad", "ygrid_lcf_forward", "ygrid_lcf_train_wproject", "ygrid_gridder_set_learned_cf",
-    "ygrid_imaging_chunk", "ygrid_uvw_rotate",
+    "ygrid_imaging_chunk", "ygrid_uvw_rotate", "ygrid_ctx_set_small_band_rows", "ygrid_gridder_autotune_small",
 ]
 
 
@@ -216,6 +216,10 @@ class Context:
         """Passes with CF support <= max_support (0..8) run the small-support kernels; 0: none."""
         _check(lib().ygrid_ctx_set_small_support(self.h, int(max_support)))
 
+    def set_small_band_rows(self, rows: int):
+        """Rows (4, 8, 16) of the small-support gridding kernel's LDS band."""
+        _check(lib().ygrid_ctx_set_small_band_rows(self.h, int(rows)))
+
     def close(self):
         if self.h:
             _check(lib().ygrid_ctx_destroy(self.h))
@@ -296,6 +300,14 @@ class Gridder:
     def degrid(self, chunk: Chunk):
         c = chunk.c()
         _check(lib().ygrid_degrid(self.h, ctypes.byref(c)))
+
+    def autotune_small(self, img: ImageDesc, chunk: Chunk, trials: int = 2):
+        """ygrid_gridder_autotune_small: (best band rows, its ms); the context keeps the choice."""
+        d, c = img.c(), chunk.c()
+        rows, ms = ctypes.c_int(), ctypes.c_double()
+        _check(lib().ygrid_gridder_autotune_small(self.h, ctypes.byref(d), ctypes.byref(c), int(trials),
+                                                  ctypes.byref(rows), ctypes.byref(ms)))
+        return rows.value, ms.value
 
     def degrid_noalloc(self, chunk: Chunk):
         """ygrid_degrid_noalloc: degrid() with the pass's buffers in the library's fixed per-GPU

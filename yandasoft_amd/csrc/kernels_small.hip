@@ -60,12 +60,12 @@ __device__ __forceinline__ float4 pol_coef(const Samples& smp, int s, int q) {
                      widen16(b & 0xffffu, smp.coef_prec), widen16(b >> 16, smp.coef_prec));
 }
 
-// Rows of the band a workgroup of k_grid_small owns: work unit u is band u % kBands of item
-// u / kBands, so that the eight bands of a tile are worked at the same time by eight
-// workgroups (a tile's pairs are the samples of ~64x64 pixels: one workgroup per tile would
-// leave most of the chip idle where the samples fill few tiles).
-constexpr int kBandRows = 8;
-constexpr int kBands = kTile / kBandRows;
+// Rows of the band a workgroup of k_grid_small owns (BR: 4, 8 or 16, the context's
+// small_band_rows, ygrid_ctx_set_small_band_rows; at least the rows one instruction covers):
+// work unit u is band u % (64 / BR) of item u / (64 / BR), so that the bands of a tile are worked
+// at the same time by several workgroups (a tile's pairs are the samples of ~64x64 pixels: one
+// workgroup per tile would leave most of the chip idle where the samples fill few tiles).  A
+// taller band stages fewer pairs twice but holds 4 x NP x BR x 512 bytes of LDS per workgroup.
 constexpr int kGridSmallBS = 256;
 
 // descriptors a workgroup of k_grid_small stages per round (two per thread)
@@ -77,13 +77,15 @@ constexpr int kGridSmallBatch = 4;
 
 // L: columns per lane row (the power of two >= 2S+1); a wave instruction covers 64/L band rows
 // of one pair's footprint, lane = (row, column)
-template <int NP, int CFP, int L>
+template <int NP, int CFP, int L, int BR>
 __global__ __launch_bounds__(kGridSmallBS) void k_grid_small(const int4* __restrict__ items,
                                                             const int* __restrict__ nitems_p, int* __restrict__ counter,
                                                             SmallGridArgs a) {
-  constexpr int BAND = kBandRows * kTile;  // pixels of a band
-  constexpr int RP = 64 / L;                // band rows per instruction
-  constexpr int NSTEP = kBandRows / RP;     // instructions per pair and band
+  constexpr int kBands = kTile / BR;
+  constexpr int BAND = BR * kTile;       // pixels of a band
+  constexpr int RP = 64 / L;             // band rows per instruction
+  constexpr int NSTEP = BR / RP;         // instructions per pair and band
+  static_assert(NSTEP >= 1, "a band holds at least the rows of one instruction");
   constexpr int U = kGridSmallBatch;
   // each wave's own copy of the unit's band, every polarisation: [wave][pol][row][column]
   // (re, im).  A wave adds one pair at a time, its lanes on distinct pixels, so plain LDS reads
@@ -112,10 +114,11 @@ __global__ __launch_bounds__(kGridSmallBS) void k_grid_small(const int4* __restr
     const int kz = it.x / a.ntiles;
     const int ichan = kz / a.nzg;
     const int u0 = (tile % a.ntu) * kTile;
-    const int vb = (tile / a.ntu) * kTile + band * kBandRows;  // first row of the band
+    const int vb = (tile / a.ntu) * kTile + band * BR;  // first row of the band
     // the pair descriptor's block bits 8w + 2k + j (blocks of 4 rows x 32 columns, four per
-    // 16-row wave band w) that cover this band's rows
-    const uint32_t mshift = 8 * (band >> 1) + 4 * (band & 1);
+    // 16-row wave band w: bits 2 (row / 4) and 2 (row / 4) + 1) that cover this band's rows
+    const uint32_t mshift = (uint32_t)(band * BR / 2);
+    constexpr uint32_t kMask = BR >= 16 ? 0xffu : ((1u << (BR / 2)) - 1u);
     for (int i = tid; i < kGridSmallWaves * NP * BAND; i += kGridSmallBS) s_acc[i] = make_float2(0.f, 0.f);
     for (int base = it.y; base < it.z; base += kGridSmallRound) {
       // stage: the round's descriptors, kept where the footprint reaches the band (compacted
@@ -127,8 +130,8 @@ __global__ __launch_bounds__(kGridSmallBS) void k_grid_small(const int4* __restr
       const int pa = base + tid, pb = base + kGridSmallBS + tid;
       const int4 da = pa < it.z ? a.pairs[pa] : make_int4(0, 0, 0, 0);
       const int4 db = pb < it.z ? a.pairs[pb] : make_int4(0, 0, 0, 0);
-      const bool ka = pa < it.z && (((uint32_t)da.z >> mshift) & 0xfu) != 0u;
-      const bool kb = pb < it.z && (((uint32_t)db.z >> mshift) & 0xfu) != 0u;
+      const bool ka = pa < it.z && (((uint32_t)da.z >> mshift) & kMask) != 0u;
+      const bool kb = pb < it.z && (((uint32_t)db.z >> mshift) & kMask) != 0u;
       const int4* hdr4 = reinterpret_cast<const int4*>(a.smp.hdr);
       const int4 ha = ka ? hdr4[da.x] : make_int4(0, 0, 0, 0);
       const int4 hb = kb ? hdr4[db.x] : make_int4(0, 0, 0, 0);
@@ -288,6 +291,15 @@ void with_lanes(int max_support, F&& f) {
   else f(std::integral_constant<int, 32>{});
 }
 
+// band rows: the requested 4, 8 or 16, raised to the rows one instruction of L lanes covers
+template <int L, class F>
+void with_band_rows(int rows, F&& f) {
+  const int r = std::max(rows, 64 / L);
+  if (r <= 4) f(std::integral_constant<int, (64 / L <= 4 ? 4 : 64 / L)>{});
+  else if (r <= 8) f(std::integral_constant<int, 8>{});
+  else f(std::integral_constant<int, 16>{});
+}
+
 template <int NP, class F>
 void with_cf_prec(int prec, F&& f) {
   switch (prec) {
@@ -321,11 +333,14 @@ void launch_grid_small(const Ctx& c, const int4* items, const int* nitems, int* 
       constexpr int CFP = decltype(cp)::value;
       with_lanes<NP, CFP>(max_support, [&](auto l) {
         constexpr int L = decltype(l)::value;
-        // eight workgroups of 256 threads per CU (the band takes NP x 4 KB of LDS)
-        const long long units = (long long)nitems_cap * kBands;
-        const int g = (int)std::max(1ll, std::min(units, 8ll * c.n_cu));
-        hipLaunchKernelGGL((k_grid_small<NP, CFP, L>), dim3(g), dim3(kGridSmallBS), 0, c.stream, items, nitems, counter,
-                           a);
+        with_band_rows<L>(c.small_band_rows, [&](auto br) {
+          constexpr int BR = decltype(br)::value;
+          // up to eight workgroups of 256 threads per CU (as the band's LDS allows)
+          const long long units = (long long)nitems_cap * (kTile / BR);
+          const int g = (int)std::max(1ll, std::min(units, 8ll * c.n_cu));
+          hipLaunchKernelGGL((k_grid_small<NP, CFP, L, BR>), dim3(g), dim3(kGridSmallBS), 0, c.stream, items, nitems,
+                             counter, a);
+        });
       });
     });
   });

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -133,6 +134,56 @@ int ygrid_ctx_set_small_support(ygrid_ctx* ctx, int max_support) {
     YG_CHECK(max_support >= 0 && max_support <= ygrid::kSmallSupportMax, YGRID_ERR_ARG,
              "small-support limit must lie in 0..8");
     ctx->c->small_support_max = max_support;
+  });
+}
+
+int ygrid_ctx_set_small_band_rows(ygrid_ctx* ctx, int rows) {
+  return guard([&] {
+    NEED(ctx);
+    YG_CHECK(rows == 4 || rows == 8 || rows == 16, YGRID_ERR_ARG, "small-support band rows must be 4, 8 or 16");
+    ctx->c->small_band_rows = rows;
+  });
+}
+
+int ygrid_gridder_autotune_small(ygrid_gridder* g, const ygrid_image_desc* img, const ygrid_chunk* acc, int trials,
+                                 int* best_rows, double* best_ms) {
+  return guard([&] {
+    NEED(g);
+    NEED(img);
+    NEED(acc);
+    NEED(best_rows);
+    NEED(best_ms);
+    YG_HIP(hipSetDevice(g->g->context()->device));
+    ygrid::Ctx& c = *g->g->context();
+    const int r0 = c.small_band_rows;
+    const int cand[] = {4, 8, 16};
+    int best = r0;
+    double bms = -1.0;
+    try {
+      for (int rows : cand) {
+        c.small_band_rows = rows;
+        double t = 1e300;
+        for (int k = 0; k < std::max(1, trials); ++k) {
+          auto t1 = g->g->clone();
+          t1->initialiseGrid(*img, false, false);
+          YG_HIP(hipStreamSynchronize(c.stream));
+          const auto a = std::chrono::steady_clock::now();
+          t1->grid(*acc);
+          YG_HIP(hipStreamSynchronize(c.stream));
+          t = std::min(t, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count());
+        }
+        if (bms < 0 || t < bms) {
+          bms = t;
+          best = rows;
+        }
+      }
+    } catch (...) {
+      c.small_band_rows = r0;
+      throw;
+    }
+    c.small_band_rows = best;
+    *best_rows = best;
+    *best_ms = bms;
   });
 }
 
