@@ -819,8 +819,13 @@ def bench_learned(args):
         f"{rep.holdout_max_err:.2e} rms {rep.holdout_rms_err:.2e}")
     modes = {}
     grids = {}
-    for name, use_lcf, prec in (("table_fp32", False, "fp32"), ("learned_fp16", True, "fp16")):
+    # the 1441-plane table: closer to continuous w than the network (5.1e-4 against 1.1e-3 of the
+    # 1025-plane FP64 oracle's peak, test_gpu_learned), so it bounds the network's matched-error time
+    for name, use_lcf, prec, nw in (("table_fp32", False, "fp32", None), ("learned_fp16", True, "fp16", None),
+                                    ("table_1441_planes_fp32", False, "fp32", 1441)):
         params = dict(wl.params, cfprecision=prec, visprecision=prec)
+        if nw is not None:
+            params["nwplanes"] = nw
         gg = ya.Gridder(ctx, "GPUWProject", params)
         dg = ya.Gridder(ctx, "GPUWProject", params)
         if use_lcf:
@@ -875,6 +880,11 @@ def bench_learned(args):
                      "holdout_rms_err_vs_table": rep.holdout_rms_err, "support": rep.support},
         "modes": modes,
         "grid_deviation_learned_vs_129_plane_table": dev_,
+        "matched_error": {"learned_ms_per_step": m["ms_per_step"],
+                          "table_1441_planes_ms_per_step": modes["table_1441_planes_fp32"]["ms_per_step"],
+                          "learned_over_table": m["ms_per_step"] / modes["table_1441_planes_fp32"]["ms_per_step"],
+                          "errors_vs_1025_plane_fp64_oracle": "learned 1.10e-3, 1441-plane table 5.1e-4, 129-plane "
+                                                              "table 2.71e-3 (test_gpu_learned)"},
         # the network (hidden width 128) runs in k_lcf_fused<true> on the FP16 matrix cores
         "roofline": {"bound": "mfma", "kernel": "k_lcf_fused<true> (every network layer, FP16 MFMA 32x32x16, "
                                                "FP32 sums), over the whole pass time",
