@@ -267,7 +267,7 @@ def main():
                     help="wproject_array: CF tap and coefficient storage (the metric is quoted for fp32)")
     ap.add_argument("--workload", default="wproject_array",
                     choices=["wproject_array", "sphfunc_disc", "awproject_feeds", "batched_beams", "learned_wdisc",
-                             "imaging_loop", "wstack_array"])
+                             "imaging_loop", "wstack_array", "snapshot_array"])
     ap.add_argument("--train-epochs", type=int, default=200, help="learned_wdisc: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -300,6 +300,9 @@ def main():
         return
     if args.workload == "wstack_array":
         bench_wstack(args)
+        return
+    if args.workload == "snapshot_array":
+        bench_snapshot(args)
         return
     import yandasoft_amd as ya
     from yandasoft_amd import workloads as W
@@ -787,6 +790,59 @@ def bench_sphfunc(args):
                      "traffic_source": PROFILE_SPH,
                      "measured_traffic_gbs": None if traffic is None else traffic / (gk * 1e-3) / 1e9,
                      "measured_traffic_frac": None if traffic is None else traffic / (gk * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    print(json.dumps(result), flush=True)
+
+
+def bench_snapshot(args):
+    """Snapshot imaging (gridder.snapshotimaging) around the SphFunc gridder on a 4096^2 x 4 pol
+    image: per step three coplanar snapshots of 20,000 rows (w = A u + B v with three planes), so
+    three w-plane fits and, at the plane changes and the end, the wrapped gridder's finalisation
+    and the regrid of its image and weights into the target frame.  The dominant new kernel is
+    k_snapshot_regrid: its compulsory bytes per launch are the source image, the output read and
+    the output written (3 x 4 pol x 4096^2 x 4 B)."""
+    import yandasoft_amd as ya
+
+    N, cell = 4096, math.radians(5 / 3600)
+    rng = np.random.default_rng(20261204)
+    F = np.array([1.4e9])
+    chunks = []
+    for A, B in ((1.5, 0.5), (-1.0, 1.2), (0.3, -2.0)):
+        u, v = rng.normal(size=(2, 20000)) * 800
+        uvw = np.ascontiguousarray(np.column_stack([u, v, A * u + B * v]))
+        vis = np.ascontiguousarray((rng.normal(size=(4, 1, 20000)) + 1j * rng.normal(size=(4, 1, 20000))).astype(np.complex64))
+        chunks.append(ya.Chunk(uvw=uvw, freq=F, vis=vis, noise=np.ones_like(vis)))
+    ctx = ya.Context(0)
+    img = ya.ImageDesc(N, N, 4, 1, cell, cell)
+    g = ya.Gridder(ctx, "GPUSphFunc", {})
+    g.configure({"snapshotimaging": True, "snapshotimaging.wtolerance": 1.0})
+
+    def step():
+        g.initialise_grid(img)
+        for ch in chunks:
+            g.grid(ch)
+        g.finalise_grid()
+        g.finalise_weights()
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    s0 = g.stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    s1 = g.stats()
+    result = {
+        "metric": "snapshot imaging steps/s (3 snapshots, fits, finalisations and regrids)",
+        "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 images, f64 coordinates", "data": "synthetic",
+        "config": {"workload": "snapshot_array: SphFunc, 4096^2 x 4 pol, 3 coplanar snapshots of 20000 rows",
+                   "parallelism": "1 GPU"},
+        "fits_per_step": (s1.snapshot_fits - s0.snapshot_fits) / args.steps,
+        "regrids_per_step": (s1.snapshot_regrids - s0.snapshot_regrids) / args.steps,
+        "regrid_compulsory_bytes": 3 * 4 * N * N * 4,
     }
     print(json.dumps(result), flush=True)
 

@@ -105,33 +105,37 @@ __global__ void k_snapshot_regrid(const float* __restrict__ src, float* __restri
   const double fx = px - fx0, fy = py - fy0;
   const int lo = a.cubic ? -1 : 0, hi = a.cubic ? 2 : 1;
   const bool ok = fx0 + lo >= 0. && fx0 + hi <= double(nx - 1) && fy0 + lo >= 0. && fy0 + hi <= double(ny - 1);
-  double wx[4] = {0., 0., 0., 0.}, wy[4] = {0., 0., 0., 0.};
+  // interpolation weights in FP32 (the coordinates above are FP64; the images are FP32)
+  float wx[4] = {0.f, 0.f, 0.f, 0.f}, wy[4] = {0.f, 0.f, 0.f, 0.f};
   if (a.cubic) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      wx[i] = keys(fx - double(i - 1));
-      wy[i] = keys(fy - double(i - 1));
+      wx[i] = (float)keys(fx - double(i - 1));
+      wy[i] = (float)keys(fy - double(i - 1));
     }
   } else {
-    wx[0] = 1. - fx;
-    wx[1] = fx;
-    wy[0] = 1. - fy;
-    wy[1] = fy;
+    wx[0] = (float)(1. - fx);
+    wx[1] = (float)fx;
+    wy[0] = (float)(1. - fy);
+    wy[1] = (float)fy;
   }
-  const int nt = hi - lo + 1;
+  const size_t s0 = ok ? (size_t)(y0 + lo) * nx + (size_t)(x0 + lo) : 0;
   for (int p = 0; p < nplanes; ++p) {
-    double v = 0.;
+    float v = 0.f;
     if (ok) {
-      const float* s = src + p * plane;
-      for (int j = 0; j < nt; ++j) {
-        const float* row = s + (size_t)(y0 + lo + j) * nx + (x0 + lo);
-        double r = 0.;
-        for (int i = 0; i < nt; ++i) r += wx[i] * (double)row[i];
-        v += wy[j] * r;
+      const float* s = src + p * plane + s0;
+      if (a.cubic) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float* row = s + (size_t)j * nx;
+          v += wy[j] * (wx[0] * row[0] + wx[1] * row[1] + wx[2] * row[2] + wx[3] * row[3]);
+        }
+      } else {
+        v = wy[0] * (wx[0] * s[0] + wx[1] * s[1]) + wy[1] * (wx[0] * s[nx] + wx[1] * s[nx + 1]);
       }
     }
     float* d = dst + p * plane + o;
-    *d = a.add ? (float)((double)*d + v) : (float)v;
+    *d = a.add ? *d + v : v;
   }
 }
 
