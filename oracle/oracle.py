@@ -60,6 +60,30 @@ Restated reference functions (file:line in alxndrwllmsn/yandasoft):
                                  of the polarisation products, pinned by closed forms and round
                                  trips; base-scimath's own conventions beyond them are parity
                                  unpinned.
+* ``aprojectwstack_planes`` / ``OracleGridder._aws_weights`` / ``OracleWStack(make=...)`` --
+                                 AProjectWStackVisGridder::initIndices / initConvolutionFunction /
+                                 finaliseWeights (AProjectWStackVisGridder.cc:122-170,222-470).
+                                 Pinned by brute-force centre-relative DFTs of the CF chain (peak-
+                                 normalised |A|^2, the -S..S-1 cut-out and its zero row and column),
+                                 the support formula (DiskIllumination.cc:139) and a two-tap closed
+                                 form of the peak-normalised weights; the w stacking as above.
+* ``uvw_rotate``             -- rotatedUVW / uvwRotationDelay as TableVisGridder::generic takes
+                                 them (TableVisGridder.cc:481-495; base-accessors, absent): the
+                                 change of frame and the rephasing to the image centre.  Pinned by
+                                 length preservation, the identity, composition and the imaging
+                                 identity (a source at the image centre becomes 1, one at d the
+                                 tangent-frame phase of d - c); casacore's UVWMachine projection
+                                 mode, if base-accessors uses it, is parity unpinned.
+* ``wplane_fit`` / ``slant_*`` / ``interp2`` / ``snapshot_regrid`` / ``OracleSnapshot`` --
+                                 SnapShotImagingGridderAdapter (SnapShotImagingGridderAdapter.cc:
+                                 195-652) with base-accessors' BestWPlaneDataAccessor and casacore's
+                                 ImageRegrid (absent) restated: least-squares plane, slant-SIN
+                                 frame, bilinear / Keys cubic interpolation.  Pinned by exact plane
+                                 recovery and orthogonal residuals, the slant closed form and its
+                                 inverse, polynomial reproduction of the interpolations, and the
+                                 position of a wide-field source through two snapshots (the
+                                 reference's wtermtest.py bar); casacore's cubic variant, edge
+                                 masking and clipping rounding are parity unpinned.
 * ``OracleGridder``          -- TableVisGridder::initialiseGrid/grid/finaliseGrid/finaliseWeights/
                                  initialiseDegrid/degrid (TableVisGridder.cc:451-884,997-1319).
 """
