@@ -1120,9 +1120,12 @@ class OracleSnapshot:
         self.fits = self.regrids = 0
 
     def _update_plane(self, chunk):
+        """wtolerance is in wavelengths (SnapShotImagingGridderAdapter.h:75): the largest
+        deviation in metres at the chunk's highest frequency (the conversion the reference's
+        statistics apply to the adapter's residual w, VisMetaDataStats.cc:408,451-452)."""
         uvw = np.asarray(chunk.uvw, np.float64)
         dev = np.max(np.abs(uvw[:, 2] - self.A * uvw[:, 0] - self.B * uvw[:, 1])) if len(uvw) else 0.0
-        if dev > self.tol:
+        if dev * float(np.max(chunk.freq)) / C_LIGHT > self.tol:
             self.fits += 1
             return True, wplane_fit(uvw)
         return False, (self.A, self.B)

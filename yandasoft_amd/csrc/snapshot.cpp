@@ -87,8 +87,17 @@ bool Snapshot::updatePlane(const ygrid_chunk& acc, double& A, double& B) {
   launch_wplane_sums(*ctx_, duvw, n, A_, B_, sums_.as<double>());
   double s[6];
   YG_HIP(hipMemcpyAsync(s, sums_.get(), sizeof(s), hipMemcpyDeviceToHost, ctx_->stream));
+  // the tolerance is in wavelengths: the largest deviation (metres) at the chunk's highest
+  // frequency, as the reference's statistics convert the adapter's residual w
+  // (VisMetaDataStats.cc:408,451-452)
+  std::vector<double> f(acc.nchan > 0 ? acc.nchan : 0);
+  if (acc.nchan > 0)
+    YG_HIP(hipMemcpyAsync(f.data(), acc.freq, f.size() * sizeof(double),
+                          acc.mem == YGRID_MEM_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, ctx_->stream));
   YG_HIP(hipStreamSynchronize(ctx_->stream));
-  if (!(s[5] > tol_)) return false;
+  double fmax = 0.;
+  for (double x : f) fmax = std::max(fmax, x);
+  if (!(s[5] * fmax / kC > tol_)) return false;
   // least-squares plane: [uu uv; uv vv] [A B]' = [uw vw]'
   const double det = s[0] * s[2] - s[1] * s[1];
   if (det > 0.) {

@@ -12,7 +12,8 @@ namespace ygrid {
 
 class Snapshot {
  public:
-  // VisGridderFactory.cc:248-270: gridder.snapshotimaging.{wtolerance, clipping, weightsclipping,
+  // VisGridderFactory.cc:248-270: gridder.snapshotimaging.{wtolerance (wavelengths at a chunk's
+  // highest frequency), clipping, weightsclipping,
   // reprojectpsf, interpmethod, coorddecimation, longtrack}, keys here without the prefix
   // "gridder." ("snapshotimaging.wtolerance", ...)
   Snapshot(Gridder* inner, const Params& p);
