@@ -196,3 +196,23 @@ def test_snapshot_around_wprojection_matches_oracle(gpu_ctx):
         d.degrid(ya.Chunk(uvw=uvw, freq=F, vis=vis, noise=np.ones_like(vis), vis_out=out))
         ov = od.degrid(O.Chunk(uvw, F, np.transpose(vis, (2, 1, 0))))
         assert rel(np.transpose(out, (2, 1, 0)), ov) < TOL
+
+
+def test_snapshot_psf_with_reprojection_matches_oracle(gpu_ctx):
+    """reprojectpsf (SnapShotImagingGridderAdapter.cc:195-211, setPSFReprojection): the PSF pass
+    is finalised and regridded per snapshot like the dirty image."""
+    announce("snapshot imaging PSF with reprojectpsf")
+    import yandasoft_amd as ya
+    g = ya.Gridder(gpu_ctx, "GPUSphFunc", {})
+    g.configure(dict(OPTS, **{"snapshotimaging.reprojectpsf": True}))
+    g.initialise_grid(ya.ImageDesc(N, N, 1, 1, CELL, CELL), dopsf=True)
+    o = _oracle(reprojectpsf=True)
+    o.initialise_grid((N, N, 1, 1), (CELL, CELL), dopsf=True)
+    for uvw, vis in _chunks(seed=8):
+        g.grid(_ya_chunk(uvw, vis))
+        o.grid(_och(uvw, vis))
+    gi, oi = g.finalise_grid(), o.finalise_grid()
+    q = slice(N // 4, 3 * N // 4)
+    assert g.stats().snapshot_regrids == 6 and o.regrids == 6
+    assert rel(gi[..., q, q], oi[..., q, q]) < TOL
+    assert rel(g.finalise_weights(), o.finalise_weights()) < 1e-6
