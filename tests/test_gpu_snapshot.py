@@ -138,6 +138,16 @@ def test_snapshot_options_are_checked(gpu_ctx):
         g.configure(dict(OPTS, **{"snapshotimaging.interpmethod": "lanczos"}))
     with pytest.raises(Exception, match="Clipping"):
         g.configure(dict(OPTS, **{"snapshotimaging.clipping": 1.0}))
+    # a snapshot gridder is refused where its wrapper would be bypassed
+    s = ya.Gridder(gpu_ctx, "GPUSphFunc", {})
+    s.configure(OPTS)
+    s.initialise_grid(ya.ImageDesc(N, N, 1, 1, CELL, CELL))
+    uvw, vis = _chunks(nrow=50)[0]
+    with pytest.raises(ya.YgridError, match="snapshot"):
+        ya.imaging_chunk([], [s], _ya_chunk(uvw, vis))
+    b = ya.Batch(gpu_ctx)
+    with pytest.raises(ya.YgridError, match="snapshot"):
+        b.add(s, _ya_chunk(uvw, vis, device=True))
 
 
 def test_snapshot_around_wprojection_matches_oracle(gpu_ctx):
