@@ -1434,10 +1434,10 @@ class OracleGridder:
             if f not in self.aw_done and f not in new:
                 new[f] = tuple(self.aw_lm.get(f, (0.0, 0.0))[:2]) + (float(np.float32(pa)),)
         if (new or self.aw_state is None) and self.kind == "AProjectWStack":
-            if pcf:
-                raise ValueError("AProjectWStack has no preconditioner (PCF) pass")
+            # no preconditioner form of its own: the PCF pass grids these CFs (pattern unshifted,
+            # as for the PSF: AProjectWStackVisGridder.cc:262-268)
             self.aw_state = aprojectwstack_planes(self.wp, self.nx, self.ny, self.uvcell, chunk.freq, new, 0,
-                                                  psf, self.aw_state)
+                                                  psf or pcf, self.aw_state)
             self.aw_done.update(new.keys())
         elif new or self.aw_state is None:
             self.aw_state = awproject_planes(self.wp, self.nx, self.ny, self.uvcell, chunk.freq, new, 0, psf,

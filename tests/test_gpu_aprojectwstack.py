@@ -103,12 +103,33 @@ def test_aprojectwstack_degrid_matches_oracle(gpu_ctx):
     assert rel(np.transpose(out, (2, 1, 0)), ov) < TOL
 
 
-def test_aprojectwstack_rejects_the_preconditioner_and_the_taper(gpu_ctx):
+def test_aprojectwstack_preconditioner_pass_matches_oracle(gpu_ctx):
+    """The preconditioner pass: AProjectWStackVisGridder has no form of its own, so its A-term
+    CFs (unshifted, as for the PSF) go through the generic preconditioner branch (1 for every
+    sample, conjugate CF in half the uv plane) into the grid of each sample's w plane, and
+    WStackVisGridder::finaliseGrid's preconditioner form (:222-262) stacks the used planes with
+    their imaginary parts scaled (updatew) into one normalised inverse transform."""
+    announce("aprojectwstack preconditioner pass")
     import yandasoft_amd as ya
     d, feed, point, lm, och = _case(33)
     g = ya.Gridder(gpu_ctx, "GPUAProjectWStack", dict(PARAMS))
     g.initialise_grid(ya.ImageDesc(N, N, 2, 1, CELL, CELL, centre=CENTRE), dopcf=True)
-    with pytest.raises(Exception, match="PCF"):
-        g.grid(ya.Chunk(uvw=d["uvw"], freq=d["freq"], vis=d["vis"], noise=d["noise"], feed1=feed, pointing1=point))
+    g.grid(ya.Chunk(uvw=d["uvw"], freq=d["freq"], vis=d["vis"], noise=d["noise"], feed1=feed, pointing1=point))
+    o = _oracle("f32", lm)
+    grids = o.grid((N, N, 2, 1), (CELL, CELL), och, dopcf=True)
+    gg = g.grid_array().reshape(7, 1, 2, N, N)
+    peak = max(np.abs(x).max() for x in grids)
+    for iw in range(7):
+        assert np.abs(gg[iw] - grids[iw]).max() <= TOL * peak, iw
+    gi = g.finalise_grid()
+    o.grids = [np.asarray(gg[iw], np.complex128) for iw in range(7)]
+    assert rel(gi, o.finalise_grid()) < 1e-5
+    o64 = _oracle("f64", lm)
+    o64.grid((N, N, 2, 1), (CELL, CELL), och, dopcf=True)
+    assert rel(gi, o64.finalise_grid()) < TOL
+
+
+def test_aprojectwstack_rejects_the_taper(gpu_ctx):
+    import yandasoft_amd as ya
     with pytest.raises(Exception, match="[Tt]aper"):
         ya.Gridder(gpu_ctx, "GPUAProjectWStack", dict(PARAMS, spheroidaltaper=True))

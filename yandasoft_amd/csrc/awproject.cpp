@@ -635,6 +635,25 @@ void AWProjectGridder::gridsToImage(const double* cx, const double* cy, float* d
   const int nxp = shape_[0], nyp = shape_[1], np = shape_[2] * shape_[3];
   const size_t ng = gridPlaneElems();
   const std::vector<int> used = usedPlanes();
+  if (dopcf_) {
+    // the preconditioner pass: AProjectWStackVisGridder has no form of its own, so its A-term CFs
+    // are gridded by the generic preconditioner branch and WStackVisGridder::finaliseGrid's
+    // preconditioner form (:222-262) stacks the used planes, imaginary parts scaled by the would-be
+    // W-projection kernel size over 3 (updatew, :173-209), into one normalised inverse transform
+    scratch_.resize(ng * sizeof(double2));
+    launch_fill_u32(*ctx_, scratch_.get(), 4 * ng, 0u);
+    const double norm = 1.0 / ((double)nxp * (double)nyp);
+    for (int iw : used) {
+      const float wThetaPix = (float)(std::fabs(ws_.wterm(iw)) / (uvcell_[0] * uvcell_[0]));
+      float wKernelPix = wThetaPix < 1 ? 3.f : (float)(6 + 1.14 * wThetaPix);
+      wKernelPix /= 3.f;
+      wstack_pcf_stack(*ctx_, grid_.as<float2>() + ng * iw, nxp, nyp, np, (double)wKernelPix, norm,
+                       scratch_.as<double2>());
+    }
+    stack_to_image_z(*ctx_, scratch_.as<double2>(), nxp, nyp, np, img_[0], img_[1], dout, plan_);
+    YG_HIP(hipStreamSynchronize(ctx_->stream));
+    return;
+  }
   DevBuf acc(ng * sizeof(double));
   launch_fill_u32(*ctx_, acc.get(), 2 * ng, 0u);
   scratch_.resize(ng * sizeof(double2));
@@ -723,7 +742,6 @@ void AWProjectGridder::initConvolutionFunction(const ChunkDev& acc) {
   YG_CHECK(!acc.freq_host.empty(), 1, "no channels");
   if (nchan_cf_ == 0) nchan_cf_ = freqdep_ ? acc.nchan : 1;
   YG_CHECK(!freqdep_ || acc.nchan == nchan_cf_, 1, "the number of channels changed between chunks");
-  YG_CHECK(!(wstack_ && dopcf_), 1, "GPUAProjectWStack has no preconditioner (PCF) pass");
   if (zsupp_.empty()) {
     const size_t nz = (size_t)ws_.nw * maxfeeds_ * maxfields_ * nchan_cf_;
     zsupp_.assign(nz, 0);
