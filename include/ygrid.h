@@ -156,9 +156,10 @@ int ygrid_ctx_clear_cache(ygrid_ctx *ctx);
  * results either way (within FP32 accumulation order); performance tuning with no counterpart
  * in the reference. */
 int ygrid_ctx_set_small_support(ygrid_ctx *ctx, int max_support);
-/* Rows (4, 8 or 16; default 8) of the LDS band a workgroup of the small-support gridding kernel
- * owns in its 64-pixel-wide tile; raised to the rows one instruction covers (8 for supports up to
- * 3).  Same results to FP32 accumulation order.  No counterpart in the reference. */
+/* Rows (4, 8 or 16; 0, the default: 4 for supports 4 to 8, 8 below) of the LDS band a workgroup
+ * of the small-support gridding kernel owns in its 64-pixel-wide tile; raised to the rows one
+ * instruction covers (8 for supports up to 3).  Same results to FP32 accumulation order.  No
+ * counterpart in the reference. */
 int ygrid_ctx_set_small_band_rows(ygrid_ctx *ctx, int rows);
 /* Autotuning of that LDS tile shape for one gridder and chunk: a clone of the gridder, set up for
  * img, grids the chunk `trials` times at each band height; the fastest (host clock around the

@@ -150,7 +150,7 @@ def test_small_support_limit_is_checked(gpu_ctx):
 def test_small_support_band_rows_match_oracle(gpu_ctx, case, rows):
     """Every LDS band height of the small-support gridding kernel (ygrid_ctx_set_small_band_rows;
     4 is raised to 8 for supports up to 3) grids as the oracle does (1e-5) and as the default
-    height (1e-6: FP32 accumulation order)."""
+    choice by support (1e-6: FP32 accumulation order)."""
     label, name, params, okw, N, cell_as, npol, nchan, opt = case
     announce(f"small-support band rows {rows}: {label}")
     cell = cell_as * ARC
@@ -163,7 +163,7 @@ def test_small_support_band_rows_match_oracle(gpu_ctx, case, rows):
         gpu_ctx.set_small_band_rows(rows)
         g, _, st, _ = _run(gpu_ctx, 8, name, params, d, N, cell, npol, model)
     finally:
-        gpu_ctx.set_small_band_rows(8)
+        gpu_ctx.set_small_band_rows(0)
     g8, _, _, _ = _run(gpu_ctx, 8, name, params, d, N, cell, npol, model)
     assert st.small_support_passes == 1
     og, _, _ = _oracle(okw, d, N, cell, npol, model)
@@ -194,4 +194,4 @@ def test_small_support_band_rows_autotune(gpu_ctx):
         with pytest.raises(ya.YgridError):
             gpu_ctx.set_small_band_rows(12)
     finally:
-        gpu_ctx.set_small_band_rows(8)
+        gpu_ctx.set_small_band_rows(0)

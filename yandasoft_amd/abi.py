@@ -217,7 +217,7 @@ class Context:
         _check(lib().ygrid_ctx_set_small_support(self.h, int(max_support)))
 
     def set_small_band_rows(self, rows: int):
-        """Rows (4, 8, 16) of the small-support gridding kernel's LDS band."""
+        """Rows (4, 8, 16; 0: by support) of the small-support gridding kernel's LDS band."""
         _check(lib().ygrid_ctx_set_small_band_rows(self.h, int(rows)))
 
     def close(self):

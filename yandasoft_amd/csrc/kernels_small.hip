@@ -291,10 +291,12 @@ void with_lanes(int max_support, F&& f) {
   else f(std::integral_constant<int, 32>{});
 }
 
-// band rows: the requested 4, 8 or 16, raised to the rows one instruction of L lanes covers
+// band rows: the requested 4, 8 or 16 (0: 4 where one instruction covers 4 rows or fewer, as
+// measured faster for supports 4 to 8, else 8), raised to the rows one instruction of L lanes
+// covers
 template <int L, class F>
 void with_band_rows(int rows, F&& f) {
-  const int r = std::max(rows, 64 / L);
+  const int r = std::max(rows > 0 ? rows : (64 / L <= 4 ? 4 : 8), 64 / L);
   if (r <= 4) f(std::integral_constant<int, (64 / L <= 4 ? 4 : 64 / L)>{});
   else if (r <= 8) f(std::integral_constant<int, 8>{});
   else f(std::integral_constant<int, 16>{});

@@ -140,7 +140,8 @@ int ygrid_ctx_set_small_support(ygrid_ctx* ctx, int max_support) {
 int ygrid_ctx_set_small_band_rows(ygrid_ctx* ctx, int rows) {
   return guard([&] {
     NEED(ctx);
-    YG_CHECK(rows == 4 || rows == 8 || rows == 16, YGRID_ERR_ARG, "small-support band rows must be 4, 8 or 16");
+    YG_CHECK(rows == 0 || rows == 4 || rows == 8 || rows == 16, YGRID_ERR_ARG,
+             "small-support band rows must be 0 (by support), 4, 8 or 16");
     ctx->c->small_band_rows = rows;
   });
 }

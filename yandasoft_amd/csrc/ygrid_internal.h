@@ -289,8 +289,9 @@ struct Ctx {
   // passes whose CF support is at most this use the small-support kernels (0: never;
   // at most kSmallSupportMax)
   int small_support_max = kSmallSupportMax;
-  // rows of the LDS band a workgroup of the small-support gridding kernel owns (4, 8 or 16)
-  int small_band_rows = 8;
+  // rows of the LDS band a workgroup of the small-support gridding kernel owns (4, 8 or 16; 0:
+  // chosen by the support)
+  int small_band_rows = 0;
   BinCache bins;  // shared by the gridders of the context (their passes are ordered on stream)
   SmallGridBufs small;  // likewise
 };
